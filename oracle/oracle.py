@@ -1,0 +1,685 @@
+"""CPU oracle for the waam_twin coupled step — TEST INFRASTRUCTURE ONLY.
+
+Loaded by tests/, ``__graft_entry__.smoke()`` and the CPU-baseline leg of ``bench.py``; the
+product package never imports it. It wraps ``liboracle.so`` (oracle/waam_oracle.c, one C
+function per reference Taichi kernel) with numpy arrays and restates, independently of the
+product, the host logic around those kernels:
+
+* lattice constants — reference grid.py:89-105 and twin.py:244-274;
+* the per-step kernel order — reference solvers/coupled_step.py:73-464 (scalar-material path);
+* the droplet schedule — reference physics/deposition_balance.py:12-86,
+  physics/deposition.py:27-31, physics/weld_forces.py:61-119,
+  physics/electrical_stickout.py:23-36;
+* reset — reference twin.py:486-531; telemetry — twin.py:706-744.
+
+``mode="jacobi"`` selects the GPU semantics for the two order-dependent in-place kernels
+(thermal advection, contact-angle BC); ``mode="serial"`` selects in-place lexicographic order.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+from dataclasses import dataclass, field, replace
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+BUILD = HERE / "_build"
+
+FLUID, SOLID, GAS, IFACE = 0, 1, 2, 4
+Q = 19
+EX = (0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0)
+EY = (0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1)
+EZ = (0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1)
+W = (1 / 3,) + (1 / 18,) * 6 + (1 / 36,) * 12
+OPP = (0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17)
+
+_F = ctypes.c_float
+_I = ctypes.c_int
+_P = ctypes.c_void_p
+
+
+def build(verbose: bool = False) -> None:
+    """Compile liboracle*.so with the recipe in oracle/Makefile."""
+    import subprocess
+
+    cmd = ["make", "-C", str(HERE), "-B"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"oracle build failed:\n{res.stdout}\n{res.stderr}")
+    if verbose:
+        print(res.stdout)
+
+
+def _load(name: str) -> ctypes.CDLL:
+    path = BUILD / name
+    if not path.exists():
+        build()
+    return ctypes.CDLL(str(path))
+
+
+_SIGS = {
+    "or_init_grid": [_P] * 10 + [_F, _F, _F, _I, _I, _I, _I, _I, _I, _I, _I],
+    "or_arc_weight_sum": [_P, _P, _F, _F, _F, _F, _F, _I, _I, _I, _I],
+    "or_inject_arc_heat": [_P, _P, _P, _P] + [_F] * 9 + [_I, _I, _I, _I],
+    "or_inject_goldak_heat": [_P, _P, _P, _P] + [_F] * 15 + [_I, _I, _I, _I],
+    "or_surface_height_at": [_P, _P, _I, _I, _I, _I, _I, _I],
+    "or_feed_wire_surface": [_P] * 7 + [_F] * 10 + [_P, _P, _F, _I, _I, _I],
+    "or_feed_wire_momentum_impact": [_P] * 7 + [_F] * 7 + [_I, _I, _I],
+    "or_apply_droplet_impact_pressure": [_P] * 4 + [_F] * 8 + [_I, _I, _I],
+    "or_compute_csf_tension": [_P] * 5 + [_F, _I, _F, _F, _I, _I, _I],
+    "or_compute_curvature_field": [_P, _P, _P, _I, _I, _I],
+    "or_apply_contact_angle_phi_bc_serial": [_P, _P, _F, _F, _I, _I, _I],
+    "or_apply_contact_angle_phi_bc_jacobi": [_P, _P, _P, _F, _F, _I, _I, _I],
+    "or_update_phase": [_P, _P, _P, _F, _F, _F, _F, _I, _I, _I],
+    "or_advect_diffuse_temperature_serial": [_P] * 6 + [_F, _F, _F, _I, _I, _I],
+    "or_advect_diffuse_temperature_jacobi": [_P] * 7 + [_F, _F, _F, _I, _I, _I],
+    "or_apply_thermal_boundary_losses": [_P, _P, _P, _F, _F, _F, _I, _I, _F, _F, _F, _F, _I, _I, _I],
+    "or_clamp_enthalpy_floor_scalar": [_P, _F, _P, _F, _I, _I, _I],
+    "or_clamp_enthalpy_ceiling_scalar": [_P, _P, _F, _F, _F, _F, _F, _I, _I, _I],
+    "or_update_cooling_rate": [_P, _P, _P, _P, _F, _I, _I, _I],
+    "or_update_T_max": [_P, _P, _P, _I, _I, _I],
+    "or_update_time_above_T": [_P] * 5 + [_F, _F, _F, _F, _I, _I, _I],
+    "or_compute_marangoni_force": [_P] * 7 + [_F, _I, _I, _I],
+    "or_add_buoyancy": [_P] * 4 + [_F, _F, _F, _F, _I, _I, _I],
+    "or_add_hydrostatic_gravity": [_P] * 3 + [_F, _F, _I, _I, _I],
+    "or_apply_arc_pressure": [_P] * 3 + [_F] * 8 + [_I, _I, _I],
+    "or_clamp_body_force_magnitude": [_P] * 4 + [_F, _I, _I, _I],
+    "or_advect_phi": [_P] * 6 + [_I, _I, _I],
+    "or_reinitialize_phi": [_P, _P, _I, _I, _I],
+    "or_update_flags_from_phi": [_P, _P, _P, _I, _I, _I, _I],
+    "or_sync_phi_liquid_fraction": [_P, _P, _P, _I, _I, _I],
+    "or_solidify_cooled_metal": [_P] * 7 + [_F, _I, _I, _I, _I],
+    "or_remelt_hot_solid_scalar": [_P] * 5 + [_F, _F, _F, _F, _I, _I, _I],
+    "or_solidify_trailing_pool_scalar": [_P] * 8 + [_F] * 11 + [_I, _I, _I],
+    "or_collide_srt": [_P] * 11 + [_F, _F, _F, _F, _I, _I, _I],
+    "or_collide_srt_variable_tau": [_P] * 12 + [_F, _I, _I, _I],
+    "or_collide_mrt": [_P] * 11 + [_F, _F, _F, _I, _I, _I],
+    "or_stream": [_P, _P, _P, _I, _I, _I],
+    "or_clamp_velocity_mach": [_P] * 4 + [_F, _I, _I, _I],
+    "or_telemetry_pool_reduce": [_P] * 8 + [_I, _I, _I],
+    "or_inject_tracers": [_P, _P, _P, _I, _F, _F, _F, _F, _I, ctypes.c_uint32],
+    "or_advect_tracers": [_P] * 7 + [_F, _F, _I, _I, _I, _I],
+}
+
+
+class _Lib:
+    def __init__(self, name: str):
+        self.dll = _load(name)
+        for fn, sig in _SIGS.items():
+            f = getattr(self.dll, fn)
+            f.argtypes = sig
+            f.restype = None
+        self.dll.or_arc_weight_sum.restype = _F
+        self.dll.or_surface_height_at.restype = _F
+
+    def __getattr__(self, item):
+        return getattr(self.dll, item)
+
+
+_LIBS: dict[str, _Lib] = {}
+
+
+def lib(parallel: bool = False) -> _Lib:
+    name = "liboracle_omp.so" if parallel else "liboracle.so"
+    if name not in _LIBS:
+        _LIBS[name] = _Lib(name)
+    return _LIBS[name]
+
+
+def ptr(a: np.ndarray) -> ctypes.c_void_p:
+    assert a.flags["C_CONTIGUOUS"], "oracle arrays must be C-contiguous"
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+# --------------------------------------------------------------------------------------------
+# Materials: the constants block of a material file plus the Heiple dγ/dT scaling
+# (reference materials.py:61-82, physics/surfactant.py:28-50, :177-194).
+# --------------------------------------------------------------------------------------------
+@dataclass
+class Material:
+    name: str
+    k: float
+    rho: float
+    cp: float
+    T_solidus: float
+    T_liquidus: float
+    L_fusion: float
+    gamma_0: float
+    dgamma_dT: float
+    mu: float
+    beta_T: float = 1.2e-4
+    alpha: float | None = None
+    contact_angle_deg: float = 80.0
+    sulphur_ppm: float = 30.0
+    rho_e_ohm_m: float = 1.5e-7
+    eta_stick: float = 0.85
+
+    def __post_init__(self):
+        if self.alpha is None:
+            self.alpha = self.k / (self.rho * self.cp)
+        s = max(0.0, self.sulphur_ppm)
+        if s <= 30.0:
+            scale = 1.0
+        elif s >= 60.0:
+            scale = -0.35
+        else:
+            scale = 1.0 - 1.35 * (s - 30.0) / 30.0
+        self.dgamma_dT = -abs(self.dgamma_dT) * scale if scale < 0 else self.dgamma_dT * scale
+
+
+# --------------------------------------------------------------------------------------------
+# Grid: numpy SoA arrays (reference grid.py:132-206). f_b starts at zero like a fresh Taichi
+# field; f_a is written by init_grid.
+# --------------------------------------------------------------------------------------------
+class OracleGrid:
+    def __init__(self, nx: int, ny: int, nz: int, dx: float, mat: Material, max_tracers: int):
+        self.nx, self.ny, self.nz, self.dx = nx, ny, nz, dx
+        self.u_ref_phys, self.u_ref_lu = 0.5, 0.05
+        self.dt = dx * self.u_ref_lu / self.u_ref_phys
+        nu_phys = mat.mu / mat.rho
+        nu_lu = nu_phys * self.dt / (dx ** 2)
+        self.tau = 3.0 * nu_lu + 0.5
+        self.alpha_lu = mat.alpha * self.dt / (dx ** 2)
+        if self.tau <= 0.5:
+            raise ValueError(f"LBM unstable: tau={self.tau:.4f} <= 0.5")
+        if self.alpha_lu > 1.0 / 6.0:
+            raise ValueError(f"Thermal step unstable: alpha_lu={self.alpha_lu:.4f} > 1/6")
+        s3 = (nx, ny, nz)
+        z = lambda: np.zeros(s3, np.float32)  # noqa: E731
+        self.f_a = np.zeros((Q,) + s3, np.float32)
+        self.f_b = np.zeros((Q,) + s3, np.float32)
+        for name in ("rho", "T", "T_max", "H", "f_l", "phi", "dT_dt", "T_prev",
+                     "time_above_800_s", "time_above_1100_s", "time_above_solidus_s",
+                     "Fx_snap", "Fy_snap", "Fz_snap", "ux", "uy", "uz", "Fx", "Fy", "Fz",
+                     "phi_tmp"):
+            setattr(self, name, z())
+        self.flags = np.zeros(s3, np.uint8)
+        self.max_tracers = max_tracers
+        self.porosity_pos = np.zeros((max_tracers, 3), np.float32)
+        self.porosity_active = np.zeros(max_tracers, np.int32)
+        self.tracer_head = np.zeros(1, np.int32)
+        self.deposit_vol = np.zeros(1, np.float32)
+        self.deposit_real = np.zeros(1, np.float32)
+        self.read_a = True
+
+    @property
+    def f_src(self) -> np.ndarray:
+        return self.f_a if self.read_a else self.f_b
+
+    @property
+    def f_dst(self) -> np.ndarray:
+        return self.f_b if self.read_a else self.f_a
+
+    def swap_buffers(self) -> None:
+        self.read_a = not self.read_a
+
+    @property
+    def dims(self) -> tuple[int, int, int]:
+        return self.nx, self.ny, self.nz
+
+
+class OracleTwin:
+    """Independent restatement of WAAMTwin's constructor, reset and coupled step (scalar path)."""
+
+    def __init__(self, material: Material, nx=256, ny=128, nz=64, dx=2e-4, arc_power_W=2800.0,
+                 arc_efficiency=0.8, T_ambient=300.0, use_srt=True, C_darcy=1.6e5,
+                 arc_pressure_pa=500.0, droplet_freq_hz=50.0, max_tracers=None,
+                 enable_heat_loss=False, h_conv=25.0, eps_rad=0.3, enable_convection=True,
+                 enable_radiation=False, heat_source="gaussian2d", enable_vof=False,
+                 enable_csf_tension=False, enable_deposition_momentum=True,
+                 enable_droplet_impact_pressure=True, enable_substrate_growth=False,
+                 enable_wetting=False, enable_hydrostatic_gravity=False,
+                 enable_bead_freeze=False, contact_angle_deg=None,
+                 deposition_superheat_K=500.0, deposition_footprint_sigma_scale=1.0,
+                 welding_current_A=140.0, travel_speed_m_s=0.0, enable_enthalpy_cap=True,
+                 T_vapor_cap_K=3200.0, arc_surface_weighting=True, arc_penetration_mm=2.0,
+                 wire_diameter_mm=1.2, arc_sigma_mm=2.0, bulk_tau=None, mode="jacobi",
+                 parallel=False, tracer_seed=0):
+        self.mat = material
+        self.lib = lib(parallel)
+        self.mode = mode
+        self.Q_w, self.eta, self.T_amb = arc_power_W, arc_efficiency, T_ambient
+        self.use_srt, self.C_darcy, self.arc_pressure = use_srt, C_darcy, arc_pressure_pa
+        self.droplet_freq = droplet_freq_hz
+        self.enable_heat_loss, self.h_conv, self.eps_rad = enable_heat_loss, h_conv, eps_rad
+        self.enable_convection, self.enable_radiation = enable_convection, enable_radiation
+        self.sigma_sb = 5.670374419e-8
+        self.heat_source_name = heat_source
+        self.enable_vof, self.enable_csf_tension = enable_vof, enable_csf_tension
+        self.enable_deposition_momentum = enable_deposition_momentum
+        self.enable_droplet_impact_pressure = enable_droplet_impact_pressure
+        self.enable_substrate_growth, self.enable_wetting = enable_substrate_growth, enable_wetting
+        self.enable_hydrostatic_gravity = enable_hydrostatic_gravity
+        self.enable_bead_freeze = enable_bead_freeze
+        self.contact_angle_deg = (contact_angle_deg if contact_angle_deg is not None
+                                  else material.contact_angle_deg)
+        self.theta_rad = math.radians(self.contact_angle_deg)
+        self.deposition_superheat_K = deposition_superheat_K
+        self.deposition_footprint_sigma_scale = deposition_footprint_sigma_scale
+        self.welding_current_A, self.travel_speed_m_s = welding_current_A, travel_speed_m_s
+        self.enable_enthalpy_cap, self.T_vapor_cap_K = enable_enthalpy_cap, T_vapor_cap_K
+        self.arc_surface_weighting = arc_surface_weighting
+        self.arc_penetration_m = arc_penetration_mm * 1e-3
+        self.wire_diameter_m = wire_diameter_mm * 1e-3
+        self.wire_feed_m_s = 0.0
+        self.droplet_transfer_mode = "auto"
+        self.pulse_frequency_hz = 0.0
+        self.droplet_size_jitter = 0.12
+        self.impact_lead_angle_deg = 0.0
+        self.trailing_solidify_lookback_mm = 2.5
+        self.trailing_solidify_temp_margin_K = 35.0
+        self.u_mach_limit_lu, self.force_limit_lu = 0.08, 0.05
+        self.enable_ctwd = False
+        self.plate_thickness_mm = None
+        self.plate_size_mm = None
+        self.plate_origin_mm = None
+        self.nz_solid = max(1, nz // 5)
+        self.tracer_seed = tracer_seed
+        self.g = OracleGrid(nx, ny, nz, dx, material, 50_000 if max_tracers is None else max_tracers)
+        g = self.g
+        self.cp_rho = material.rho * material.cp
+        self.L_rho = material.rho * material.L_fusion
+        self.alpha_lu = material.alpha * g.dt / (g.dx ** 2)
+        self.omega = 1.0 / g.tau
+        self.omega_bulk = 1.0 / bulk_tau if bulk_tau else self.omega
+        self.arc_sigma_m = arc_sigma_mm * 1e-3
+        self.sigma_cells = self.arc_sigma_m / g.dx
+        self.g_lu = 9.81 * (g.dt ** 2) / g.dx
+        self.beta_T = material.beta_T
+        scale_F = (g.dt ** 2) / (material.rho * g.dx ** 2)
+        self.force_scale = scale_F
+        self.dgamma_dT_lu = material.dgamma_dT * scale_F / g.dx
+        self.gamma_lu = material.gamma_0 * scale_F / g.dx
+        self.marangoni_scale = 1.0
+        self.goldak = None  # (ff, fr, a_front, a_rear, b, c) in cells when heat_source is Goldak
+        self._reset_counters()
+
+    # ---------------------------------------------------------------- geometry (twin.py:289-342)
+    def resolve_nz_solid(self, test_fluid_domain=False):
+        if test_fluid_domain:
+            return 0
+        g = self.g
+        headroom = max(6, g.nz // 4)
+        max_solid = max(1, g.nz - headroom)
+        if self.plate_thickness_mm is not None and float(self.plate_thickness_mm) > 0.0:
+            n = int(round(float(self.plate_thickness_mm) / (g.dx * 1000.0)))
+            return max(1, min(n, max_solid))
+        return max(1, min(g.nz // 5, max_solid))
+
+    def resolve_plate_ij(self):
+        g = self.g
+        dx_mm = g.dx * 1000.0
+        if not self.plate_size_mm:
+            return 0, g.nx, 0, g.ny
+        lx, ly = float(self.plate_size_mm[0]), float(self.plate_size_mm[1])
+        ni = min(max(1, int(round(lx / dx_mm))), g.nx)
+        nj = min(max(1, int(round(ly / dx_mm))), g.ny)
+        if self.plate_origin_mm is not None:
+            i0 = int(round(float(self.plate_origin_mm[0]) / dx_mm))
+            j0 = int(round(float(self.plate_origin_mm[1]) / dx_mm))
+        else:
+            i0, j0 = (g.nx - ni) // 2, (g.ny - nj) // 2
+        i0 = max(0, min(i0, g.nx - ni))
+        j0 = max(0, min(j0, g.ny - nj))
+        return i0, i0 + ni, j0, j0 + nj
+
+    def _reset_counters(self):
+        self._step_n = 0
+        self._last_droplet_time = 0.0
+        self._deposited_volume_m3 = 0.0
+        self._n_droplets_fired = 0
+        self._deposition_overflow = 0
+        self._last_torch_pos_m = None
+        self._torch_dir_xyz = (1.0, 0.0, 0.0)
+
+    # ---------------------------------------------------------------- reset (twin.py:486-531)
+    def reset(self, T_ambient=None, test_fluid_domain=False):
+        T0 = T_ambient or self.T_amb
+        g, L = self.g, self.lib
+        if test_fluid_domain:
+            nzs = -1
+            self.nz_solid = 0
+            i0, i1, j0, j1 = 0, g.nx, 0, g.ny
+        else:
+            self.nz_solid = self.resolve_nz_solid(False)
+            nzs = self.nz_solid
+            i0, i1, j0, j1 = self.resolve_plate_ij()
+        L.or_init_grid(ptr(g.f_a), ptr(g.rho), ptr(g.ux), ptr(g.uy), ptr(g.uz), ptr(g.T),
+                       ptr(g.H), ptr(g.f_l), ptr(g.phi), ptr(g.flags), T0, 1.0, self.cp_rho,
+                       nzs, i0, i1, j0, j1, *g.dims)
+        for a in (g.T_max, g.T_prev):
+            a.fill(np.float32(T0))
+        for a in (g.dT_dt, g.time_above_800_s, g.time_above_1100_s, g.time_above_solidus_s,
+                  g.Fx, g.Fy, g.Fz, g.Fx_snap, g.Fy_snap, g.Fz_snap):
+            a.fill(0)
+        g.porosity_active.fill(0)
+        g.tracer_head[0] = 0
+        L.or_stream(ptr(g.f_a), ptr(g.f_b), ptr(g.flags), *g.dims)
+        g.read_a = True
+        self._reset_counters()
+        g.deposit_vol[0] = 0
+        g.deposit_real[0] = 0
+
+    # ---------------------------------------------------------------- host helpers
+    def wire_mass_flux_kg_s(self):
+        d = self.wire_diameter_m
+        return self.mat.rho * (math.pi * (d * 0.5) ** 2) * self.wire_feed_m_s
+
+    def transfer_mode(self):
+        mode = str(self.droplet_transfer_mode).strip().lower()
+        if mode in {"globular", "spray", "pulsed"}:
+            return mode
+        if self.pulse_frequency_hz > 0.0:
+            return "pulsed"
+        return "globular" if float(self.welding_current_A) < 180.0 else "spray"
+
+    def droplet_period_s(self):
+        f = float(self.droplet_freq)
+        base = 1.0 / max(f, 1e-9) if f > 0.0 else 1.0 / 50.0
+        mode = self.transfer_mode()
+        if mode == "globular":
+            ms = 1.55
+        elif mode == "spray":
+            ms = 0.72
+        elif mode == "pulsed":
+            ms = max(0.25, min(4.0, (1.0 / max(self.pulse_frequency_hz, 1e-9)) / base))
+        else:
+            ms = 1.0
+        idx = float(self._n_droplets_fired + 1)
+        amp = max(0.0, min(float(self.droplet_size_jitter), 0.35))
+        jitter = 1.0 + amp * math.sin(0.73 * idx + 0.31)
+        return max(base * 0.20, base * ms * jitter)
+
+    def droplet_mass_kg(self):
+        if self.droplet_freq <= 0:
+            return 0.0
+        return max(0.0, self.wire_mass_flux_kg_s() * max(self.droplet_period_s(), 0.0))
+
+    def droplet_radius_cells(self, m):
+        if m <= 0:
+            return 3.0
+        n_cells = (m / self.mat.rho) / (self.g.dx ** 3)
+        r = (3.0 * n_cells / (4.0 * math.pi)) ** (1.0 / 3.0)
+        return max(1.0, min(r, 10.0))
+
+    def droplet_entry_temperature_K(self):
+        return self.mat.T_liquidus + self.deposition_superheat_K
+
+    def droplet_impact_velocity_m_s(self, m_drop):
+        drop_len = 3e-3
+        if self.droplet_freq > 0 and self.wire_feed_m_s > 0:
+            drop_len = self.wire_feed_m_s / self.droplet_freq
+        v_feed = max(self.wire_feed_m_s, 0.0)
+        v_grav = math.sqrt(max(0.0, 2.0 * 9.81 * drop_len))
+        dia_ref = max(self.wire_diameter_m, 1e-6)
+        pinch = 0.35 * math.sqrt(max(self.welding_current_A, 0.0) / 200.0) * math.sqrt(1.2e-3 / dia_ref)
+        ms = {"globular": 0.78, "spray": 1.18, "pulsed": 1.05}.get(self.transfer_mode(), 1.0)
+        mass_term = 1.0
+        if m_drop > 0:
+            mass_term = max(0.8, min(1.25, math.sqrt(self.droplet_mass_kg() / m_drop)))
+        return min(5.0, ms * max(v_feed, v_grav + pinch) * mass_term)
+
+    def droplet_velocity_lu(self, m_drop):
+        g = self.g
+        v = self.droplet_impact_velocity_m_s(m_drop)
+        lead = math.radians(float(self.impact_lead_angle_deg))
+        dxd, dyd, _ = self._torch_dir_xyz
+        dxy = math.sqrt(dxd * dxd + dyd * dyd)
+        if dxy < 1e-9:
+            dxd, dyd, dxy = 1.0, 0.0, 1.0
+        dxd /= dxy
+        dyd /= dxy
+        v_xy = max(self.travel_speed_m_s, abs(v) * math.tan(lead))
+        v_xy = min(v * 0.65, v_xy)
+        v_z = -math.sqrt(max(v * v - v_xy * v_xy, v * v * 0.20))
+        return (v_xy * dxd * g.dt / g.dx, v_xy * dyd * g.dt / g.dx, v_z * g.dt / g.dx)
+
+    def droplet_impact_pressure_pa(self, m, r_m):
+        if m <= 0:
+            return 0.0
+        v = self.droplet_impact_velocity_m_s(m)
+        return (0.5 * m * v * v) / max(math.pi * r_m * r_m, 1e-12)
+
+    # ---------------------------------------------------------------- step
+    def _resolve_arc_k(self, arc_i, arc_j):
+        g = self.g
+        i0 = max(0, min(int(arc_i), g.nx - 1))
+        j0 = max(0, min(int(arc_j), g.ny - 1))
+        k = float(self.lib.or_surface_height_at(ptr(g.phi), ptr(g.flags), i0, j0, self.nz_solid,
+                                                *g.dims))
+        if k < 1.0:
+            k = float(max(1, self.nz_solid - 1))
+        return k
+
+    def _ceiling(self):
+        if not self.enable_enthalpy_cap:
+            return
+        g = self.g
+        self.lib.or_clamp_enthalpy_ceiling_scalar(ptr(g.H), ptr(g.flags), self.cp_rho,
+                                                  self.mat.T_solidus, self.mat.T_liquidus,
+                                                  self.T_vapor_cap_K, self.L_rho, *g.dims)
+
+    def _phase(self):
+        g = self.g
+        self.lib.or_update_phase(ptr(g.H), ptr(g.T), ptr(g.f_l), self.cp_rho, self.L_rho,
+                                 self.mat.T_solidus, self.mat.T_liquidus, *g.dims)
+
+    def _inject_arc(self, arc_i, arc_j, arc_k):
+        g, L = self.g, self.lib
+        pen = self.arc_penetration_m / g.dx
+        en = 1 if self.arc_surface_weighting else 0
+        key = self.heat_source_name.lower().replace("-", "").replace("_", "")
+        if key in ("goldak", "goldak3d", "doubleellipsoid"):
+            ff, fr, af, ar, b, c = self.goldak
+            sgn = 1.0 if self.travel_speed_m_s >= 0.0 else -1.0
+            L.or_inject_goldak_heat(ptr(g.H), ptr(g.flags), ptr(g.f_l), None, arc_i, arc_j,
+                                    arc_k, self.Q_w, g.dt, g.dx ** 3, self.eta, sgn, ff, fr, af,
+                                    ar, b, c, pen, en, *g.dims)
+        elif key in ("conical", "conicalvolume", "cone"):
+            L.or_inject_arc_heat(ptr(g.H), ptr(g.flags), ptr(g.f_l), None, arc_i, arc_j,
+                                 arc_k - 1.5, self.Q_w, self.sigma_cells * 1.2, g.dt, g.dx ** 3,
+                                 self.eta, pen, en, *g.dims)
+        else:
+            L.or_inject_arc_heat(ptr(g.H), ptr(g.flags), ptr(g.f_l), None, arc_i, arc_j, arc_k,
+                                 self.Q_w, self.sigma_cells, g.dt, g.dx ** 3, self.eta, pen, en,
+                                 *g.dims)
+
+    def _droplet(self, x_m, y_m, arc_i, arc_j, arc_k):
+        g, L = self.g, self.lib
+        sim_time = self._step_n * g.dt
+        if self.droplet_freq <= 0:
+            return
+        period = self.droplet_period_s()
+        if sim_time - self._last_droplet_time < period:
+            return
+        drop_dt = sim_time - self._last_droplet_time
+        self._last_droplet_time = sim_time
+        drop_mass = max(0.0, self.wire_mass_flux_kg_s() * max(drop_dt, 0.0))
+        drop_r = self.droplet_radius_cells(drop_mass)
+        drop_vol = drop_mass / self.mat.rho
+        T_drop = self.droplet_entry_temperature_K()
+        L.or_inject_tracers(ptr(g.porosity_pos), ptr(g.porosity_active), ptr(g.tracer_head),
+                            g.max_tracers, x_m, y_m, float(arc_k + 1) * g.dx,
+                            float(self.sigma_cells * g.dx), 50, self.tracer_seed)
+        foot_r = max(drop_r, self.sigma_cells * self.deposition_footprint_sigma_scale)
+        r_try = foot_r
+        placed = 0.0
+        for _ in range(10):
+            g.deposit_vol[0] = placed
+            g.deposit_real[0] = placed
+            L.or_feed_wire_surface(ptr(g.f_src), ptr(g.flags), ptr(g.f_l), ptr(g.phi), ptr(g.H),
+                                   ptr(g.T), ptr(g.rho), arc_i, arc_j, arc_k, r_try, drop_r,
+                                   drop_vol, T_drop, self.cp_rho, self.L_rho, 1.0,
+                                   ptr(g.deposit_vol), ptr(g.deposit_real), g.dx ** 3, *g.dims)
+            placed = float(g.deposit_real[0])
+            if placed >= drop_vol * 0.98:
+                break
+            r_try = min(r_try + 1.5, 16.0)
+        if placed < drop_vol * 0.98:
+            self._deposition_overflow += 1
+        self._deposited_volume_m3 += placed
+        self._n_droplets_fired += 1
+        if self.enable_deposition_momentum:
+            vx, vy, vz = self.droplet_velocity_lu(drop_mass)
+            L.or_feed_wire_momentum_impact(ptr(g.f_src), ptr(g.rho), ptr(g.ux), ptr(g.uy),
+                                           ptr(g.uz), ptr(g.flags), ptr(g.f_l), arc_i, arc_j,
+                                           arc_k, drop_r, vx, vy, vz, *g.dims)
+            if self.enable_droplet_impact_pressure:
+                L.or_apply_droplet_impact_pressure(
+                    ptr(g.Fz), ptr(g.flags), ptr(g.phi), ptr(g.f_l), arc_i, arc_j, arc_k, drop_r,
+                    self.droplet_impact_pressure_pa(drop_mass, drop_r * g.dx), g.dt, g.dx,
+                    self.mat.rho, *g.dims)
+
+    def step(self, torch_x_m, torch_y_m, is_welding=True, torch_z_m=None):
+        # torch direction (twin.py:540-549)
+        z_now = 0.0 if torch_z_m is None else float(torch_z_m)
+        if self._last_torch_pos_m is not None:
+            px, py, pz = self._last_torch_pos_m
+            ddx, ddy, ddz = torch_x_m - px, torch_y_m - py, z_now - pz
+            nrm = math.sqrt(ddx * ddx + ddy * ddy + ddz * ddz)
+            if nrm > 1e-12:
+                self._torch_dir_xyz = (ddx / nrm, ddy / nrm, ddz / nrm)
+        self._last_torch_pos_m = (torch_x_m, torch_y_m, z_now)
+        self.coupled_step(torch_x_m, torch_y_m, is_welding)
+
+    def coupled_step(self, x_m, y_m, is_welding):
+        g, L, m = self.g, self.lib, self.mat
+        d = g.dims
+        arc_i = x_m / g.dx
+        arc_j = y_m / g.dx
+        arc_k = self._resolve_arc_k(arc_i, arc_j)
+        self.last_arc_ijk = (arc_i, arc_j, arc_k)
+        g.Fx.fill(0)
+        g.Fy.fill(0)
+        g.Fz.fill(0)
+        if is_welding:
+            self._inject_arc(arc_i, arc_j, arc_k)
+            self._ceiling()
+            self._droplet(x_m, y_m, arc_i, arc_j, arc_k)
+        if self.mode == "serial":
+            L.or_advect_diffuse_temperature_serial(ptr(g.H), ptr(g.T), ptr(g.ux), ptr(g.uy),
+                                                   ptr(g.uz), ptr(g.flags), self.alpha_lu, 1.0,
+                                                   self.cp_rho, *d)
+        else:
+            H_in = g.H.copy()
+            L.or_advect_diffuse_temperature_jacobi(ptr(H_in), ptr(g.H), ptr(g.T), ptr(g.ux),
+                                                   ptr(g.uy), ptr(g.uz), ptr(g.flags),
+                                                   self.alpha_lu, 1.0, self.cp_rho, *d)
+        if self.enable_heat_loss:
+            L.or_apply_thermal_boundary_losses(ptr(g.H), ptr(g.T), ptr(g.flags), self.T_amb,
+                                               self.h_conv, self.eps_rad,
+                                               1 if self.enable_convection else 0,
+                                               1 if self.enable_radiation else 0, self.cp_rho,
+                                               g.dt, g.dx, self.sigma_sb, *d)
+            L.or_clamp_enthalpy_floor_scalar(ptr(g.H), self.cp_rho, ptr(g.flags), self.T_amb, *d)
+        self._phase()
+        self._ceiling()
+        self._phase()
+        L.or_update_T_max(ptr(g.T), ptr(g.T_max), ptr(g.flags), *d)
+        L.or_update_cooling_rate(ptr(g.T), ptr(g.T_prev), ptr(g.dT_dt), ptr(g.flags), g.dt, *d)
+        if self.enable_vof:
+            L.or_advect_phi(ptr(g.phi_tmp), ptr(g.phi), ptr(g.ux), ptr(g.uy), ptr(g.uz),
+                            ptr(g.flags), *d)
+            g.phi[...] = g.phi_tmp
+            L.or_reinitialize_phi(ptr(g.phi), ptr(g.flags), *d)
+            if self.enable_wetting:
+                ct, st = cos_sin_f32(self.theta_rad)
+                if self.mode == "serial":
+                    L.or_apply_contact_angle_phi_bc_serial(ptr(g.phi), ptr(g.flags), ct, st, *d)
+                else:
+                    p_in = g.phi.copy()
+                    L.or_apply_contact_angle_phi_bc_jacobi(ptr(p_in), ptr(g.phi), ptr(g.flags),
+                                                           ct, st, *d)
+            L.or_update_flags_from_phi(ptr(g.phi), ptr(g.f_l), ptr(g.flags), self.nz_solid, *d)
+            L.or_sync_phi_liquid_fraction(ptr(g.phi), ptr(g.f_l), ptr(g.flags), *d)
+        if self.enable_csf_tension:
+            ct, st = cos_sin_f32(self.theta_rad)
+            L.or_compute_csf_tension(ptr(g.phi), ptr(g.flags), ptr(g.Fx), ptr(g.Fy), ptr(g.Fz),
+                                     self.gamma_lu, 1 if self.enable_wetting else 0, ct, st, *d)
+        L.or_compute_marangoni_force(ptr(g.T), ptr(g.phi), ptr(g.f_l), ptr(g.Fx), ptr(g.Fy),
+                                     ptr(g.Fz), ptr(g.flags), self.dgamma_dT_lu, *d)
+        if is_welding:
+            L.or_apply_arc_pressure(ptr(g.Fz), ptr(g.flags), ptr(g.phi), arc_i, arc_j, arc_k,
+                                    self.arc_sigma_m / g.dx, float(self.arc_pressure), g.dt,
+                                    g.dx, m.rho, *d)
+        if self.enable_hydrostatic_gravity:
+            L.or_add_hydrostatic_gravity(ptr(g.Fz), ptr(g.f_l), ptr(g.flags), 1.0, self.g_lu, *d)
+        L.or_add_buoyancy(ptr(g.T), ptr(g.Fz), ptr(g.f_l), ptr(g.flags), self.g_lu, self.beta_T,
+                          m.T_liquidus, 1.0, *d)
+        if self.force_limit_lu > 0.0:
+            L.or_clamp_body_force_magnitude(ptr(g.Fx), ptr(g.Fy), ptr(g.Fz), ptr(g.flags),
+                                            self.force_limit_lu, *d)
+        if self.use_srt:
+            L.or_collide_srt(ptr(g.f_src), ptr(g.f_dst), ptr(g.rho), ptr(g.ux), ptr(g.uy),
+                             ptr(g.uz), ptr(g.Fx), ptr(g.Fy), ptr(g.Fz), ptr(g.f_l), ptr(g.flags),
+                             g.tau, self.omega, 1.0, self.C_darcy, *d)
+        else:
+            L.or_collide_mrt(ptr(g.f_src), ptr(g.f_dst), ptr(g.rho), ptr(g.ux), ptr(g.uy),
+                             ptr(g.uz), ptr(g.Fx), ptr(g.Fy), ptr(g.Fz), ptr(g.f_l), ptr(g.flags),
+                             self.omega, self.omega_bulk, self.C_darcy, *d)
+        if self.u_mach_limit_lu > 0.0:
+            L.or_clamp_velocity_mach(ptr(g.ux), ptr(g.uy), ptr(g.uz), ptr(g.flags),
+                                     self.u_mach_limit_lu, *d)
+        L.or_stream(ptr(g.f_dst), ptr(g.f_src), ptr(g.flags), *d)
+        L.or_advect_tracers(ptr(g.porosity_pos), ptr(g.porosity_active), ptr(g.ux), ptr(g.uy),
+                            ptr(g.uz), ptr(g.f_l), ptr(g.flags), g.dx, g.dt, g.max_tracers, *d)
+        L.or_update_time_above_T(ptr(g.T), ptr(g.flags), ptr(g.time_above_800_s),
+                                 ptr(g.time_above_1100_s), ptr(g.time_above_solidus_s), g.dt,
+                                 800.0 + 273.15, 1100.0 + 273.15, m.T_solidus, *d)
+        g.Fx_snap[...] = g.Fx
+        g.Fy_snap[...] = g.Fy
+        g.Fz_snap[...] = g.Fz
+        if self.enable_substrate_growth or self.enable_bead_freeze:
+            L.or_remelt_hot_solid_scalar(ptr(g.T), ptr(g.H), ptr(g.f_l), ptr(g.phi), ptr(g.flags),
+                                         self.cp_rho, m.T_solidus, m.T_liquidus, self.L_rho, *d)
+            if is_welding and self.enable_bead_freeze:
+                dx_, dy_, dz_ = self._torch_dir_xyz
+                lookback = max(2.0, self.trailing_solidify_lookback_mm / (g.dx * 1000.0))
+                T_freeze = m.T_liquidus + self.trailing_solidify_temp_margin_K
+                L.or_solidify_trailing_pool_scalar(
+                    ptr(g.T), ptr(g.H), ptr(g.f_l), ptr(g.phi), ptr(g.flags), ptr(g.ux),
+                    ptr(g.uy), ptr(g.uz), self.cp_rho, arc_i, arc_j, arc_k, dx_, dy_, dz_,
+                    lookback, T_freeze, m.T_solidus, *d)
+            L.or_solidify_cooled_metal(ptr(g.T), ptr(g.f_l), ptr(g.phi), ptr(g.flags), ptr(g.ux),
+                                       ptr(g.uy), ptr(g.uz), m.T_solidus,
+                                       1 if (self.enable_bead_freeze or self.enable_substrate_growth) else 0,
+                                       *d)
+        g.swap_buffers()
+        self._step_n += 1
+
+    # ---------------------------------------------------------------- telemetry (twin.py:716-744)
+    def pool_metrics(self):
+        g = self.g
+        out = np.zeros(8, np.float64)
+        self.lib.or_telemetry_pool_reduce(ptr(g.T), ptr(g.f_l), ptr(g.phi), ptr(g.dT_dt),
+                                          ptr(g.ux), ptr(g.uy), ptr(g.uz), ptr(out), *g.dims)
+        n = int(out[0])
+        res = {"n_liquid_cells": n, "T_global": float(out[7]), "cool": float(out[2])}
+        if n > 0:
+            i_c = int(round(float(np.float32(out[4])) / n))
+            i_c = max(0, min(g.nx - 1, i_c))
+            sect = g.f_l[i_c] > 0.5
+            if sect.any():
+                y_idx = np.where(sect.any(axis=1))[0]
+                z_idx = np.where(sect.any(axis=0))[0]
+                W_m = (y_idx[-1] - y_idx[0] + 1) * g.dx
+                D_m = max(0, self.nz_solid - z_idx[0]) * g.dx
+            else:
+                W_m = D_m = 0.0
+            res.update(T_peak=float(out[1]), i_c=i_c, pool_width_m=W_m, pool_depth_m=D_m,
+                       pool_length_m=(float(out[6]) - float(out[5]) + 1.0) * g.dx,
+                       u2_max=float(out[3]))
+        else:
+            res.update(T_peak=float(out[7]), pool_width_m=0.0, pool_depth_m=0.0,
+                       pool_length_m=0.0, u2_max=0.0)
+        return res
+
+
+def cos_sin_f32(theta_rad: float) -> tuple[float, float]:
+    """cos/sin of the fp32-rounded contact angle, rounded to fp32 (the kernels' ti.cos/ti.sin)."""
+    t = float(np.float32(theta_rad))
+    return float(np.float32(math.cos(t))), float(np.float32(math.sin(t)))
