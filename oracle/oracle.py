@@ -94,7 +94,7 @@ _SIGS = {
     "or_sync_phi_liquid_fraction": [_P, _P, _P, _I, _I, _I],
     "or_solidify_cooled_metal": [_P] * 7 + [_F, _I, _I, _I, _I],
     "or_remelt_hot_solid_scalar": [_P] * 5 + [_F, _F, _F, _F, _I, _I, _I],
-    "or_solidify_trailing_pool_scalar": [_P] * 8 + [_F] * 11 + [_I, _I, _I],
+    "or_solidify_trailing_pool_scalar": [_P] * 8 + [_F] * 10 + [_I, _I, _I],
     "or_collide_srt": [_P] * 11 + [_F, _F, _F, _F, _I, _I, _I],
     "or_collide_srt_variable_tau": [_P] * 12 + [_F, _I, _I, _I],
     "or_collide_mrt": [_P] * 11 + [_F, _F, _F, _I, _I, _I],
@@ -359,7 +359,6 @@ class OracleTwin:
         g.porosity_active.fill(0)
         g.tracer_head[0] = 0
         L.or_stream(ptr(g.f_a), ptr(g.f_b), ptr(g.flags), *g.dims)
-        g.read_a = True
         self._reset_counters()
         g.deposit_vol[0] = 0
         g.deposit_real[0] = 0

@@ -15,7 +15,7 @@
  *   - x**2 = x*x, x**3 = x*(x*x), x**4 = (x*x)*(x*x) (exponentiation by squaring);
  *   - ti.math.exp is waam_expf (waam_math.h), shared with the GPU code so that both
  *     evaluate it identically; cos/sin of the contact angle are passed in precomputed;
- *   - float atomics that accumulate a sum (arc weight total) are a double-precision sum
+ *   - float atomics that accumulate a sum (arc weight total) are a compensated double sum
  *     rounded once to fp32; integer-valued sums are exact.
  * Layout: every scalar field is C-order (nx,ny,nz) with k fastest; populations are
  * (19,nx,ny,nz); flags are uint8 with FLUID=0, SOLID=1, GAS=2, IFACE=4 (grid.py:63-66).
@@ -109,6 +109,18 @@ void or_init_grid(float *f, float *rho, float *ux, float *uy, float *uz, float *
  * The reference accumulates the weight total with a float atomic; here it is a double sum
  * rounded once to fp32 (order independent up to double rounding).
  * ---------------------------------------------------------------------------------------- */
+/* Compensated (TwoSum) accumulation: the returned double is the exact sum of the terms to
+ * within a couple of double ulps, whatever the order, so rounding it to fp32 is order
+ * independent in practice. */
+typedef struct { double s, c; } csum_t;
+static inline void csum_add(csum_t *a, double x)
+{
+    double t = a->s + x;
+    double bb = t - a->s;
+    a->c += (a->s - (t - bb)) + (x - bb);
+    a->s = t;
+}
+
 static inline float arc_deposition_weight(int k, float arc_k, float fl, float pen, int enable)
 {
     float w = 1.0f;
@@ -133,7 +145,7 @@ float or_arc_weight_sum(const uint8_t *flags, const float *f_l, float arc_i, flo
                         float arc_k, float sigma, float pen, int enable, int nx, int ny, int nz)
 {
     float inv2s2 = 1.0f / (2.0f * sigma * sigma);
-    double acc = 0.0;
+    csum_t acc = {0.0, 0.0};
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < ny; ++j)
             for (int k = 0; k < nz; ++k) {
@@ -143,9 +155,9 @@ float or_arc_weight_sum(const uint8_t *flags, const float *f_l, float arc_i, flo
                 if (w_dep < 1e-6f) continue;
                 float di = (float)i - arc_i, dj = (float)j - arc_j, dk = (float)k - arc_k;
                 float r2 = di * di + dj * dj + dk * dk * 0.25f;
-                acc += (double)(waam_expf(-r2 * inv2s2) * w_dep);
+                csum_add(&acc, (double)(waam_expf(-r2 * inv2s2) * w_dep));
             }
-    return (float)acc;
+    return (float)(acc.s + acc.c);
 }
 
 void or_inject_arc_heat(float *H, const uint8_t *flags, const float *f_l, float *norm_out,
@@ -188,7 +200,7 @@ void or_inject_goldak_heat(float *H, const uint8_t *flags, const float *f_l, flo
                            float eta, float sgn, float ff, float fr, float af, float ar, float b,
                            float cax, float pen, int enable, int nx, int ny, int nz)
 {
-    double acc = 0.0;
+    csum_t acc = {0.0, 0.0};
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < ny; ++j)
             for (int k = 0; k < nz; ++k) {
@@ -198,9 +210,9 @@ void or_inject_goldak_heat(float *H, const uint8_t *flags, const float *f_l, flo
                 if (w_dep < 1e-6f) continue;
                 float w = goldak_pdf_weight((float)i - arc_i, (float)j - arc_j, (float)k - arc_k,
                                             sgn, ff, fr, af, ar, b, cax);
-                acc += (double)(w * w_dep);
+                csum_add(&acc, (double)(w * w_dep));
             }
-    float w_total = (float)acc;
+    float w_total = (float)(acc.s + acc.c);
     if (norm_out) *norm_out = w_total;
     float epw = 0.0f;
     if (w_total > 1e-9f) epw = Q_w * eta * dt / (w_total * dx3);

@@ -1,0 +1,320 @@
+"""Per-kernel parity: every HIP entry point against the oracle, bit for bit, on seeded states.
+
+Each case builds a seeded random state (plate, gas, liquid patch, interface band and sprinkled
+random flags so every branch runs), hands identical bytes to the GPU kernel and to the oracle's
+restatement of the same reference kernel, and requires identical output bits (fp32 fields,
+uint8 flags). Both sides evaluate fp32 expressions in the reference's operand order with FMA
+contraction off and the shared exp(), so any difference is a defect.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from tests.gpu_common import bits_equal, mismatch_report, random_state
+
+pytestmark = pytest.mark.gpu
+
+DIMS = [(24, 20, 16), (17, 13, 70)]
+
+
+@pytest.fixture(scope="module")
+def env(hip_device):
+    import torch
+
+    from oracle.oracle import lib as olib
+    from waam_solver_amd import kernels as K
+
+    return dict(torch=torch, K=K, O=olib(), dev=hip_device)
+
+
+def up(env, a):
+    return env["torch"].from_numpy(np.ascontiguousarray(a)).to(env["dev"])
+
+
+def down(t):
+    return t.detach().cpu().numpy()
+
+
+def P(a):
+    from oracle.oracle import ptr
+
+    return ptr(a)
+
+
+def check(name, gpu, ora):
+    assert bits_equal(gpu, ora), mismatch_report(name, gpu, ora)
+
+
+@pytest.mark.parametrize("dims", DIMS)
+@pytest.mark.parametrize("which", ["mrt", "srt"])
+def test_collide(env, dims, which):
+    s = random_state(*dims, seed=11)
+    n = dims
+    g = {k: up(env, s[k]) for k in ("f", "f2", "rho", "ux", "uy", "uz", "Fx", "Fy", "Fz", "f_l", "flags")}
+    o = {k: s[k].copy() for k in g}
+    K, O = env["K"], env["O"]
+    if which == "mrt":
+        K.collide_mrt(g["f"], g["f2"], g["rho"], g["ux"], g["uy"], g["uz"], g["Fx"], g["Fy"], g["Fz"],
+                      g["f_l"], g["flags"], None, None, None, None, None, 1.7, 1.3, 1.6e5, 1, 2, *n)
+        O.or_collide_mrt(P(o["f"]), P(o["f2"]), P(o["rho"]), P(o["ux"]), P(o["uy"]), P(o["uz"]),
+                         P(o["Fx"]), P(o["Fy"]), P(o["Fz"]), P(o["f_l"]), P(o["flags"]), 1.7, 1.3,
+                         1.6e5, *n)
+    else:
+        K.collide_srt(g["f"], g["f2"], g["rho"], g["ux"], g["uy"], g["uz"], g["Fx"], g["Fy"], g["Fz"],
+                      g["f_l"], g["flags"], 0.6, 1.0 / 0.6, 1.0, 1.6e5, 1, 2, *n)
+        O.or_collide_srt(P(o["f"]), P(o["f2"]), P(o["rho"]), P(o["ux"]), P(o["uy"]), P(o["uz"]),
+                         P(o["Fx"]), P(o["Fy"]), P(o["Fz"]), P(o["f_l"]), P(o["flags"]), 0.6,
+                         1.0 / 0.6, 1.0, 1.6e5, *n)
+    for k in ("f2", "rho", "ux", "uy", "uz"):
+        check(k, down(g[k]), o[k])
+
+
+@pytest.mark.parametrize("dims", DIMS)
+def test_stream(env, dims):
+    s = random_state(*dims, seed=12)
+    g = {k: up(env, s[k]) for k in ("f", "f2", "flags")}
+    o = {k: s[k].copy() for k in g}
+    env["K"].stream(g["f"], g["f2"], g["flags"], 1, 2, *dims)
+    env["O"].or_stream(P(o["f"]), P(o["f2"]), P(o["flags"]), *dims)
+    check("f_dst", down(g["f2"]), o["f2"])
+
+
+@pytest.mark.parametrize("dims", DIMS)
+def test_thermal_advect_diffuse(env, dims):
+    s = random_state(*dims, seed=13)
+    g = {k: up(env, s[k]) for k in ("H", "T", "ux", "uy", "uz", "flags")}
+    o = {k: s[k].copy() for k in g}
+    alpha, cp = 0.05, 5.3e6
+    env["K"].advect_diffuse_temperature(g["H"], g["T"], g["ux"], g["uy"], g["uz"], g["flags"],
+                                        alpha, 1.0, 1, 2, cp, *dims)
+    H_in = o["H"].copy()
+    env["O"].or_advect_diffuse_temperature_jacobi(P(H_in), P(o["H"]), P(o["T"]), P(o["ux"]),
+                                                  P(o["uy"]), P(o["uz"]), P(o["flags"]), alpha,
+                                                  1.0, cp, *dims)
+    check("H", down(g["H"]), o["H"])
+
+
+@pytest.mark.parametrize("dims", DIMS)
+def test_boundary_losses_floor_ceiling_phase(env, dims):
+    s = random_state(*dims, seed=14)
+    g = {k: up(env, s[k]) for k in ("H", "T", "flags", "f_l")}
+    o = {k: s[k].copy() for k in g}
+    K, O = env["K"], env["O"]
+    cp, L, Ts, Tl = 5.3e6, 2.1e9, 1723.0, 1780.0
+    K.apply_thermal_boundary_losses(g["H"], g["T"], g["flags"], 300.0, 25.0, 0.3, 1, 1, cp, 2e-5,
+                                    2e-4, 5.670374419e-8, 1, 2, *dims)
+    O.or_apply_thermal_boundary_losses(P(o["H"]), P(o["T"]), P(o["flags"]), 300.0, 25.0, 0.3, 1,
+                                       1, cp, 2e-5, 2e-4, 5.670374419e-8, *dims)
+    check("H losses", down(g["H"]), o["H"])
+    K.clamp_enthalpy_floor_scalar(g["H"], cp, g["flags"], 1200.0, 2)
+    O.or_clamp_enthalpy_floor_scalar(P(o["H"]), cp, P(o["flags"]), 1200.0, *dims)
+    check("H floor", down(g["H"]), o["H"])
+    K.clamp_enthalpy_ceiling_scalar(g["H"], g["flags"], cp, Ts, Tl, 2500.0, L, 2)
+    O.or_clamp_enthalpy_ceiling_scalar(P(o["H"]), P(o["flags"]), cp, Ts, Tl, 2500.0, L, *dims)
+    check("H ceiling", down(g["H"]), o["H"])
+    K.update_phase(g["H"], g["T"], g["f_l"], cp, L, Ts, Tl)
+    O.or_update_phase(P(o["H"]), P(o["T"]), P(o["f_l"]), cp, L, Ts, Tl, *dims)
+    check("T", down(g["T"]), o["T"])
+    check("f_l", down(g["f_l"]), o["f_l"])
+
+
+def test_haz_diagnostics(env):
+    dims = DIMS[0]
+    s = random_state(*dims, seed=15)
+    rng = np.random.default_rng(1)
+    extra = {k: rng.uniform(0, 3000, dims).astype(np.float32) for k in ("Tm", "Tp", "d", "a", "b", "c")}
+    g = {k: up(env, v) for k, v in {**s, **extra}.items() if isinstance(v, np.ndarray) and k != "f" and k != "f2"}
+    o = {k: v.copy() for k, v in {**s, **extra}.items() if isinstance(v, np.ndarray)}
+    K, O = env["K"], env["O"]
+    K.update_T_max(g["T"], g["Tm"], g["flags"], 2)
+    O.or_update_T_max(P(o["T"]), P(o["Tm"]), P(o["flags"]), *dims)
+    K.update_cooling_rate(g["T"], g["Tp"], g["d"], g["flags"], 2e-5, 2)
+    O.or_update_cooling_rate(P(o["T"]), P(o["Tp"]), P(o["d"]), P(o["flags"]), 2e-5, *dims)
+    K.update_time_above_T(g["T"], g["flags"], g["a"], g["b"], g["c"], 2e-5, 1073.15, 1373.15,
+                          1723.0, 2)
+    O.or_update_time_above_T(P(o["T"]), P(o["flags"]), P(o["a"]), P(o["b"]), P(o["c"]), 2e-5,
+                             1073.15, 1373.15, 1723.0, *dims)
+    for k in ("Tm", "Tp", "d", "a", "b", "c"):
+        check(k, down(g[k]), o[k])
+
+
+@pytest.mark.parametrize("dims", DIMS)
+@pytest.mark.parametrize("source", ["gauss", "goldak"])
+def test_arc_heat(env, dims, source):
+    s = random_state(*dims, seed=16)
+    g = {k: up(env, s[k]) for k in ("H", "flags", "f_l")}
+    o = {k: s[k].copy() for k in g}
+    ai, aj, ak = dims[0] * 0.45, dims[1] * 0.55, float(s["nz_solid"])
+    norm_g = env["torch"].zeros((), dtype=env["torch"].float32, device=env["dev"])
+    norm_o = np.zeros(1, np.float32)
+    if source == "gauss":
+        env["K"].inject_arc_heat(g["H"], g["flags"], None, g["f_l"], norm_g, ai, aj, ak, 4000.0,
+                                 6.0, 2e-5, 8e-12, 0.8, 10.0, 1, 1, 2)
+        env["O"].or_inject_arc_heat(P(o["H"]), P(o["flags"]), P(o["f_l"]), P(norm_o), ai, aj, ak,
+                                    4000.0, 6.0, 2e-5, 8e-12, 0.8, 10.0, 1, *dims)
+    else:
+        env["K"].inject_goldak_heat(g["H"], g["flags"], None, g["f_l"], norm_g, ai, aj, ak,
+                                    4000.0, 2e-5, 8e-12, 0.8, 1.0, 0.6, 1.4, 4.0, 8.0, 4.0, 3.0,
+                                    10.0, 1, 1, 2)
+        env["O"].or_inject_goldak_heat(P(o["H"]), P(o["flags"]), P(o["f_l"]), P(norm_o), ai, aj,
+                                       ak, 4000.0, 2e-5, 8e-12, 0.8, 1.0, 0.6, 1.4, 4.0, 8.0, 4.0,
+                                       3.0, 10.0, 1, *dims)
+    assert float(norm_g.item()) == float(norm_o[0])
+    check("H", down(g["H"]), o["H"])
+
+
+@pytest.mark.parametrize("dims", DIMS)
+def test_vof_chain(env, dims):
+    s = random_state(*dims, seed=17)
+    g = {k: up(env, s[k]) for k in ("phi", "ux", "uy", "uz", "flags", "f_l")}
+    o = {k: s[k].copy() for k in g}
+    K, O = env["K"], env["O"]
+    torch = env["torch"]
+    tmp_g = torch.empty_like(g["phi"])
+    tmp_o = np.empty_like(o["phi"])
+    K.advect_phi(tmp_g, g["phi"], g["ux"], g["uy"], g["uz"], g["flags"], 1, 2, *dims)
+    O.or_advect_phi(P(tmp_o), P(o["phi"]), P(o["ux"]), P(o["uy"]), P(o["uz"]), P(o["flags"]), *dims)
+    check("phi advect", down(tmp_g), tmp_o)
+    K.reinitialize_phi(tmp_g, g["flags"], 1, 2, 0)
+    O.or_reinitialize_phi(P(tmp_o), P(o["flags"]), *dims)
+    check("phi reinit", down(tmp_g), tmp_o)
+    theta = math.radians(80.0)
+    K.apply_contact_angle_phi_bc_into(tmp_g, g["phi"], g["flags"], theta, *dims)
+    c, sn = K.contact_angle_cos_sin(theta)
+    O.or_apply_contact_angle_phi_bc_jacobi(P(tmp_o), P(o["phi"]), P(o["flags"]), c, sn, *dims)
+    check("phi contact", down(g["phi"]), o["phi"])
+    K.update_flags_from_phi(g["phi"], g["f_l"], g["flags"], s["nz_solid"], 0, 1, 2, 4)
+    O.or_update_flags_from_phi(P(o["phi"]), P(o["f_l"]), P(o["flags"]), s["nz_solid"], *dims)
+    check("flags", down(g["flags"]), o["flags"])
+    K.sync_phi_liquid_fraction(g["phi"], g["f_l"], g["flags"], 2, 1)
+    O.or_sync_phi_liquid_fraction(P(o["phi"]), P(o["f_l"]), P(o["flags"]), *dims)
+    check("f_l", down(g["f_l"]), o["f_l"])
+
+
+@pytest.mark.parametrize("dims", DIMS)
+@pytest.mark.parametrize("wetting", [0, 1])
+def test_forces(env, dims, wetting):
+    s = random_state(*dims, seed=18)
+    g = {k: up(env, s[k]) for k in ("phi", "flags", "Fx", "Fy", "Fz", "T", "f_l")}
+    o = {k: s[k].copy() for k in g}
+    K, O = env["K"], env["O"]
+    theta = math.radians(70.0)
+    c, sn = K.contact_angle_cos_sin(theta)
+    K.compute_csf_tension(g["phi"], g["flags"], g["Fx"], g["Fy"], g["Fz"], 3e-3, 1, 2, *dims,
+                          wetting, theta)
+    O.or_compute_csf_tension(P(o["phi"]), P(o["flags"]), P(o["Fx"]), P(o["Fy"]), P(o["Fz"]), 3e-3,
+                             wetting, c, sn, *dims)
+    K.compute_marangoni_force(g["T"], g["phi"], g["f_l"], g["Fx"], g["Fy"], g["Fz"], g["flags"],
+                              -2e-7, 2e-4, 1, 2, *dims)
+    O.or_compute_marangoni_force(P(o["T"]), P(o["phi"]), P(o["f_l"]), P(o["Fx"]), P(o["Fy"]),
+                                 P(o["Fz"]), P(o["flags"]), -2e-7, *dims)
+    ai, aj, ak = dims[0] * 0.5, dims[1] * 0.5, float(s["nz_solid"])
+    K.apply_arc_pressure(g["Fz"], g["flags"], g["phi"], ai, aj, ak, 6.0, 500.0, 2e-5, 2e-4,
+                         7800.0, 1, 2)
+    O.or_apply_arc_pressure(P(o["Fz"]), P(o["flags"]), P(o["phi"]), ai, aj, ak, 6.0, 500.0, 2e-5,
+                            2e-4, 7800.0, *dims)
+    K.add_hydrostatic_gravity(g["Fz"], g["f_l"], g["flags"], 1.0, 3e-7, 1, 2)
+    O.or_add_hydrostatic_gravity(P(o["Fz"]), P(o["f_l"]), P(o["flags"]), 1.0, 3e-7, *dims)
+    K.add_buoyancy(g["T"], g["Fz"], g["f_l"], g["flags"], 3e-7, 1.2e-4, 1780.0, 1.0, 1, 2)
+    O.or_add_buoyancy(P(o["T"]), P(o["Fz"]), P(o["f_l"]), P(o["flags"]), 3e-7, 1.2e-4, 1780.0,
+                      1.0, *dims)
+    K.apply_droplet_impact_pressure(g["Fz"], g["flags"], g["phi"], g["f_l"], ai, aj, ak, 3.0,
+                                    2000.0, 2e-5, 2e-4, 7800.0, 1, 2)
+    O.or_apply_droplet_impact_pressure(P(o["Fz"]), P(o["flags"]), P(o["phi"]), P(o["f_l"]), ai, aj,
+                                       ak, 3.0, 2000.0, 2e-5, 2e-4, 7800.0, *dims)
+    K.clamp_body_force_magnitude(g["Fx"], g["Fy"], g["Fz"], g["flags"], 0.02, 1, 2)
+    O.or_clamp_body_force_magnitude(P(o["Fx"]), P(o["Fy"]), P(o["Fz"]), P(o["flags"]), 0.02, *dims)
+    for k in ("Fx", "Fy", "Fz"):
+        check(k, down(g[k]), o[k])
+
+
+def test_curvature_and_velocity_clamp(env):
+    dims = DIMS[0]
+    s = random_state(*dims, seed=19)
+    g = {k: up(env, s[k]) for k in ("phi", "flags", "ux", "uy", "uz")}
+    o = {k: s[k].copy() for k in g}
+    kap_g = env["torch"].zeros_like(g["phi"])
+    kap_o = np.zeros_like(o["phi"])
+    env["K"].compute_curvature_field(g["phi"], g["flags"], kap_g, 2, *dims)
+    env["O"].or_compute_curvature_field(P(o["phi"]), P(o["flags"]), P(kap_o), *dims)
+    check("kappa", down(kap_g), kap_o)
+    env["K"].clamp_velocity_mach(g["ux"], g["uy"], g["uz"], g["flags"], 0.04, 1, 2)
+    env["O"].or_clamp_velocity_mach(P(o["ux"]), P(o["uy"]), P(o["uz"]), P(o["flags"]), 0.04, *dims)
+    for k in ("ux", "uy", "uz"):
+        check(k, down(g[k]), o[k])
+
+
+def test_freeze_kernels(env):
+    dims = DIMS[0]
+    s = random_state(*dims, seed=20)
+    s["T"] = np.random.default_rng(2).uniform(1500, 2000, dims).astype(np.float32)
+    keys = ("T", "H", "f_l", "phi", "flags", "ux", "uy", "uz")
+    g = {k: up(env, s[k]) for k in keys}
+    o = {k: s[k].copy() for k in keys}
+    K, O = env["K"], env["O"]
+    cp, Ts, Tl, L = 5.3e6, 1723.0, 1780.0, 2.1e9
+    K.remelt_hot_solid_scalar(g["T"], g["H"], g["f_l"], g["phi"], g["flags"], cp, Ts, Tl, L, 1, 0)
+    O.or_remelt_hot_solid_scalar(P(o["T"]), P(o["H"]), P(o["f_l"]), P(o["phi"]), P(o["flags"]), cp,
+                                 Ts, Tl, L, *dims)
+    K.solidify_trailing_pool_scalar(g["T"], g["H"], g["f_l"], g["phi"], g["flags"], g["ux"],
+                                    g["uy"], g["uz"], cp, 14.0, 10.0, 4.0, 1.0, 0.0, 0.0, 3.0,
+                                    1815.0, Ts, 1, 0, 2)
+    O.or_solidify_trailing_pool_scalar(P(o["T"]), P(o["H"]), P(o["f_l"]), P(o["phi"]),
+                                       P(o["flags"]), P(o["ux"]), P(o["uy"]), P(o["uz"]), cp, 14.0,
+                                       10.0, 4.0, 1.0, 0.0, 0.0, 3.0, 1815.0, Ts, *dims)
+    K.solidify_cooled_metal(g["T"], g["f_l"], g["phi"], g["flags"], g["ux"], g["uy"], g["uz"], Ts,
+                            True, 1, 0, 2)
+    O.or_solidify_cooled_metal(P(o["T"]), P(o["f_l"]), P(o["phi"]), P(o["flags"]), P(o["ux"]),
+                               P(o["uy"]), P(o["uz"]), Ts, 1, *dims)
+    for k in keys:
+        check(k, down(g[k]), o[k])
+
+
+@pytest.mark.parametrize("footprint", [4.0, 9.5])
+def test_feed_wire_surface_single_call(env, footprint):
+    """One reference feed_wire_surface call: serial-sweep semantics on the GPU wavefront."""
+    dims = (40, 36, 30)
+    s = random_state(*dims, seed=21, nz_solid=10)
+    keys = ("f", "flags", "f_l", "phi", "H", "T", "rho")
+    g = {k: up(env, s[k]) for k in keys}
+    o = {k: s[k].copy() for k in keys}
+    torch = env["torch"]
+    ai, aj, ak = 19.3, 17.6, 10.0
+    cell_vol = 8e-12
+    for target in (40 * cell_vol, 400 * cell_vol):
+        vg = torch.zeros((), dtype=torch.float32, device=env["dev"])
+        rg = torch.zeros((), dtype=torch.float32, device=env["dev"])
+        vo = np.zeros(1, np.float32)
+        ro = np.zeros(1, np.float32)
+        env["K"].feed_wire_surface(g["f"], g["flags"], g["f_l"], g["phi"], g["H"], g["T"], g["rho"],
+                                   ai, aj, ak, footprint, 2.5, target, 2280.0, 5.3e6, 2.1e9, 1.0, vg,
+                                   rg, cell_vol, 2, 0, 1, *dims)
+        env["O"].or_feed_wire_surface(P(o["f"]), P(o["flags"]), P(o["f_l"]), P(o["phi"]), P(o["H"]),
+                                      P(o["T"]), P(o["rho"]), ai, aj, ak, footprint, 2.5, target,
+                                      2280.0, 5.3e6, 2.1e9, 1.0, P(vo), P(ro), cell_vol, *dims)
+        assert float(vg.item()) == float(vo[0]) and float(rg.item()) == float(ro[0])
+        for k in keys:
+            check(k, down(g[k]), o[k])
+
+
+def test_telemetry_reduce(env):
+    dims = DIMS[1]
+    s = random_state(*dims, seed=22)
+    rng = np.random.default_rng(3)
+    dT = (rng.standard_normal(dims) * 1e5).astype(np.float32)
+    g = {k: up(env, s[k]) for k in ("T", "f_l", "phi", "ux", "uy", "uz")}
+    torch = env["torch"]
+    out_g = torch.zeros(8, dtype=torch.float64, device=env["dev"])
+    sc = torch.zeros(__import__("waam_solver_amd._lib", fromlist=["lib"]).lib().scratch_bytes(),
+                     dtype=torch.uint8, device=env["dev"])
+    env["K"].telemetry_pool_reduce_into(g["T"], g["f_l"], g["phi"], up(env, dT), g["ux"], g["uy"],
+                                        g["uz"], out_g, sc)
+    out_o = np.zeros(8, np.float64)
+    o = {k: s[k].copy() for k in g}
+    env["O"].or_telemetry_pool_reduce(P(o["T"]), P(o["f_l"]), P(o["phi"]), P(dT), P(o["ux"]),
+                                      P(o["uy"]), P(o["uz"]), P(out_o), *dims)
+    assert np.array_equal(down(out_g), out_o), (down(out_g), out_o)

@@ -1,0 +1,449 @@
+"""Positional kernel API of the reference (kernels.py / cumulant_kernel.py) on the HIP library.
+
+Every function keeps the reference kernel's name and positional parameters — fields first,
+then scalars, FLAG_* constants and grid sizes — and launches the matching libwaam_hip entry
+point on the current torch stream. Fields are DeviceFields (or contiguous CUDA tensors). FLAG_*
+arguments are checked against the fixed flag values of the library. Sizes are checked against
+the tensors before any launch.
+
+Two reference kernels update H or phi in place while reading neighbouring values of the same
+field (advect_diffuse_temperature, apply_contact_angle_phi_bc); here they read a copy of the
+pre-kernel field (Jacobi order), which is the one deterministic order a parallel GPU kernel has.
+"""
+
+from __future__ import annotations
+
+import math
+
+import torch
+
+from ._lib import FLAG_FLUID, FLAG_GAS, FLAG_IFACE, FLAG_SOLID, WaamError, lib
+from .field import DeviceField, P, tensor_of
+
+_FLAGS = {"FLAG_FLUID": FLAG_FLUID, "FLAG_SOLID": FLAG_SOLID, "FLAG_GAS": FLAG_GAS,
+          "FLAG_IFACE": FLAG_IFACE}
+
+
+def _stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _flags_ok(**kw) -> None:
+    for k, v in kw.items():
+        if int(v) != _FLAGS[k]:
+            raise ValueError(f"{k}={v} differs from the library's fixed value {_FLAGS[k]}")
+
+
+def _need(n: int, *fields) -> None:
+    for f in fields:
+        if f is None:
+            continue
+        t = tensor_of(f)
+        if t.numel() < n:
+            raise ValueError(f"field of {t.numel()} elements is smaller than the {n} the call needs")
+
+
+def _need_pop(n: int, *fields) -> None:
+    _need(19 * n, *fields)
+
+
+def _n(x) -> int:
+    return tensor_of(x).numel()
+
+
+def bind_velocity_set(grid) -> None:
+    """The velocity set is compiled into the library (kept for API compatibility)."""
+    return None
+
+
+def _call(name, *args):
+    lib().call(name, *args)
+
+
+def _scratch_of(*fields):
+    """Workspace tensor for reducing entries: the grid's when reachable, else a fresh one."""
+    return torch.zeros(lib().scratch_bytes(), dtype=torch.uint8, device=tensor_of(fields[0]).device)
+
+
+# ------------------------------------------------------------------ init / forces reset
+def init_grid(f, rho, ux, uy, uz, T, H, f_l, phi, flags, T_ambient, rho0, cp_rho, nz_solid,
+              plate_i0, plate_i1, plate_j0, plate_j1, FLAG_FLUID, FLAG_SOLID, FLAG_GAS):
+    _flags_ok(FLAG_FLUID=FLAG_FLUID, FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
+    nx, ny, nz = tensor_of(rho).shape
+    n = nx * ny * nz
+    _need_pop(n, f)
+    _need(n, rho, ux, uy, uz, T, H, f_l, phi, flags)
+    _call("waam_init_grid", P(f), P(rho), P(ux), P(uy), P(uz), P(T), P(H), P(f_l), P(phi),
+          P(flags), T_ambient, rho0, cp_rho, nz_solid, plate_i0, plate_i1, plate_j0, plate_j1,
+          nx, ny, nz, _stream())
+
+
+def init_aux_fields(T_max, T_prev, dT_dt, time_above_800, time_above_1100, time_above_solidus,
+                    Fx_snap, Fy_snap, Fz_snap, Fx, Fy, Fz, T_ambient, porosity_active,
+                    tracer_head, max_tracers):
+    for fld in (T_max, T_prev):
+        tensor_of(fld).fill_(float(T_ambient))
+    for fld in (dT_dt, time_above_800, time_above_1100, time_above_solidus, Fx_snap, Fy_snap,
+                Fz_snap, Fx, Fy, Fz):
+        tensor_of(fld).zero_()
+    tensor_of(porosity_active).zero_()
+    tensor_of(tracer_head).zero_()
+
+
+def clear_forces(Fx, Fy, Fz):
+    for fld in (Fx, Fy, Fz):
+        tensor_of(fld).zero_()
+
+
+# ------------------------------------------------------------------ arc heat
+def inject_arc_heat(H, flags, phi, f_l, norm_buf, arc_i, arc_j, arc_k, Q_w, sigma, dt, dx3, eta,
+                    penetration_cells, enable_surface_weight, FLAG_SOLID, FLAG_GAS, scratch=None):
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
+    nx, ny, nz = tensor_of(H).shape
+    _need(nx * ny * nz, flags, f_l)
+    sc = scratch if scratch is not None else _scratch_of(H)
+    _call("waam_inject_arc_heat", P(H), P(flags), P(f_l), P(norm_buf), arc_i, arc_j, arc_k, Q_w,
+          sigma, dt, dx3, eta, penetration_cells, int(enable_surface_weight), nx, ny, nz, P(sc),
+          _stream())
+
+
+def inject_goldak_heat(H, flags, phi, f_l, norm_buf, arc_i, arc_j, arc_k, Q_w, dt, dx3, eta,
+                       travel_sign, ff, fr, a_front, a_rear, b_axis, c_axis, penetration_cells,
+                       enable_surface_weight, FLAG_SOLID, FLAG_GAS, scratch=None):
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
+    nx, ny, nz = tensor_of(H).shape
+    _need(nx * ny * nz, flags, f_l)
+    sc = scratch if scratch is not None else _scratch_of(H)
+    _call("waam_inject_goldak_heat", P(H), P(flags), P(f_l), P(norm_buf), arc_i, arc_j, arc_k,
+          Q_w, dt, dx3, eta, travel_sign, ff, fr, a_front, a_rear, b_axis, c_axis,
+          penetration_cells, int(enable_surface_weight), nx, ny, nz, P(sc), _stream())
+
+
+def surface_height_at(phi, flags, out, i0, j0, nz_fallback, FLAG_GAS, nz):
+    _flags_ok(FLAG_GAS=FLAG_GAS)
+    nx, ny, nz_ = tensor_of(phi).shape
+    if nz != nz_:
+        raise ValueError("nz does not match phi")
+    _call("waam_surface_height_at", P(phi), P(flags), P(out), int(i0), int(j0), int(nz_fallback),
+          nx, ny, nz, _stream())
+
+
+# ------------------------------------------------------------------ thermal
+def clamp_enthalpy_ceiling_scalar(H, flags, cp_rho, T_solidus, T_liquidus, T_vapor_cap, L_rho,
+                                  FLAG_GAS):
+    _flags_ok(FLAG_GAS=FLAG_GAS)
+    _need(_n(H), flags)
+    _call("waam_clamp_enthalpy_ceiling_scalar", P(H), P(flags), cp_rho, T_solidus, T_liquidus,
+          T_vapor_cap, L_rho, _n(H), _stream())
+
+
+def clamp_enthalpy_floor_scalar(H, cp_rho, flags, T_floor, FLAG_GAS):
+    _flags_ok(FLAG_GAS=FLAG_GAS)
+    _need(_n(H), flags)
+    _call("waam_clamp_enthalpy_floor_scalar", P(H), cp_rho, P(flags), T_floor, _n(H), _stream())
+
+
+def advect_diffuse_temperature_into(H_in, H_out, T, ux, uy, uz, flags, alpha_lu, dt, cp_rho, nx,
+                                    ny, nz):
+    _need(nx * ny * nz, H_in, H_out, T, ux, uy, uz, flags)
+    _call("waam_advect_diffuse_temperature", P(H_in), P(H_out), P(T), P(ux), P(uy), P(uz),
+          P(flags), alpha_lu, dt, cp_rho, nx, ny, nz, _stream())
+
+
+def advect_diffuse_temperature(H, T, ux, uy, uz, flags, alpha_lu, dt, FLAG_SOLID, FLAG_GAS,
+                               cp_rho, nx, ny, nz):
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
+    H_in = tensor_of(H).clone()
+    advect_diffuse_temperature_into(H_in, H, T, ux, uy, uz, flags, alpha_lu, dt, cp_rho, nx, ny,
+                                    nz)
+
+
+def apply_thermal_boundary_losses(H, T, flags, T_amb, h_conv, eps_rad, enable_conv, enable_rad,
+                                  cp_rho, dt, dx, sigma_sb, FLAG_SOLID, FLAG_GAS, nx, ny, nz):
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
+    _need(nx * ny * nz, H, T, flags)
+    _call("waam_apply_thermal_boundary_losses", P(H), P(T), P(flags), T_amb, h_conv, eps_rad,
+          int(enable_conv), int(enable_rad), cp_rho, dt, dx, sigma_sb, nx, ny, nz, _stream())
+
+
+def update_phase(H, T, f_l, cp_rho, L_rho, T_solidus, T_liquidus):
+    _need(_n(H), T, f_l)
+    _call("waam_update_phase", P(H), P(T), P(f_l), cp_rho, L_rho, T_solidus, T_liquidus, _n(H),
+          _stream())
+
+
+def update_T_max(T, T_max, flags, FLAG_GAS):
+    _flags_ok(FLAG_GAS=FLAG_GAS)
+    _need(_n(T), T_max, flags)
+    _call("waam_update_T_max", P(T), P(T_max), P(flags), _n(T), _stream())
+
+
+def update_cooling_rate(T, T_prev, dT_dt, flags, dt_phys, FLAG_GAS):
+    _flags_ok(FLAG_GAS=FLAG_GAS)
+    _need(_n(T), T_prev, dT_dt, flags)
+    _call("waam_update_cooling_rate", P(T), P(T_prev), P(dT_dt), P(flags), dt_phys, _n(T),
+          _stream())
+
+
+def update_time_above_T(T, flags, time_above_800, time_above_1100, time_above_solidus, dt,
+                        T_800, T_1100, T_solidus, FLAG_GAS):
+    _flags_ok(FLAG_GAS=FLAG_GAS)
+    _need(_n(T), flags, time_above_800, time_above_1100, time_above_solidus)
+    _call("waam_update_time_above_T", P(T), P(flags), P(time_above_800), P(time_above_1100),
+          P(time_above_solidus), dt, T_800, T_1100, T_solidus, _n(T), _stream())
+
+
+def sync_T_from_H(H, T, cp_rho):
+    """T = H / (rho cp) (reference kernels.py:3119-3122; a verification helper)."""
+    tensor_of(T).copy_(tensor_of(H) / torch.tensor(cp_rho, dtype=torch.float32))
+
+
+# ------------------------------------------------------------------ VOF
+def advect_phi(phi_dst, phi_src, ux, uy, uz, flags, FLAG_SOLID, FLAG_GAS, nx, ny, nz):
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
+    _need(nx * ny * nz, phi_dst, phi_src, ux, uy, uz, flags)
+    _call("waam_advect_phi", P(phi_dst), P(phi_src), P(ux), P(uy), P(uz), P(flags), nx, ny, nz,
+          _stream())
+
+
+def reinitialize_phi(phi, flags, FLAG_SOLID, FLAG_GAS, FLAG_FLUID):
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS, FLAG_FLUID=FLAG_FLUID)
+    _need(_n(phi), flags)
+    _call("waam_reinitialize_phi", P(phi), P(flags), _n(phi), _stream())
+
+
+def contact_angle_cos_sin(theta_rad: float) -> tuple[float, float]:
+    """cos/sin of the fp32-rounded angle, rounded to fp32 (the reference's ti.cos/ti.sin)."""
+    import numpy as np
+
+    t = float(np.float32(theta_rad))
+    return float(np.float32(math.cos(t))), float(np.float32(math.sin(t)))
+
+
+def apply_contact_angle_phi_bc_into(phi_in, phi_out, flags, theta_rad, nx, ny, nz):
+    _need(nx * ny * nz, phi_in, phi_out, flags)
+    c, s = contact_angle_cos_sin(theta_rad)
+    _call("waam_apply_contact_angle_phi_bc", P(phi_in), P(phi_out), P(flags), c, s, nx, ny, nz,
+          _stream())
+
+
+def apply_contact_angle_phi_bc(phi, flags, theta_rad, FLAG_SOLID, FLAG_GAS, nx, ny, nz):
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
+    p_in = tensor_of(phi).clone()
+    apply_contact_angle_phi_bc_into(p_in, phi, flags, theta_rad, nx, ny, nz)
+
+
+def update_flags_from_phi(phi, f_l, flags, nz_solid, FLAG_FLUID, FLAG_SOLID, FLAG_GAS,
+                          FLAG_IFACE):
+    _flags_ok(FLAG_FLUID=FLAG_FLUID, FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS,
+              FLAG_IFACE=FLAG_IFACE)
+    nx, ny, nz = tensor_of(phi).shape
+    _need(nx * ny * nz, f_l, flags)
+    _call("waam_update_flags_from_phi", P(phi), P(f_l), P(flags), int(nz_solid), nx, ny, nz,
+          _stream())
+
+
+def sync_phi_liquid_fraction(phi, f_l, flags, FLAG_GAS, FLAG_SOLID):
+    _flags_ok(FLAG_GAS=FLAG_GAS, FLAG_SOLID=FLAG_SOLID)
+    _need(_n(phi), f_l, flags)
+    _call("waam_sync_phi_liquid_fraction", P(phi), P(f_l), P(flags), _n(phi), _stream())
+
+
+def solidify_cooled_metal(T, f_l, phi, flags, ux, uy, uz, T_solidus, zero_velocity, FLAG_SOLID,
+                          FLAG_FLUID, FLAG_GAS):
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_FLUID=FLAG_FLUID, FLAG_GAS=FLAG_GAS)
+    _need(_n(T), f_l, phi, flags, ux, uy, uz)
+    _call("waam_solidify_cooled_metal", P(T), P(f_l), P(phi), P(flags), P(ux), P(uy), P(uz),
+          T_solidus, 1 if zero_velocity else 0, _n(T), _stream())
+
+
+def remelt_hot_solid_scalar(T, H, f_l, phi, flags, cp_rho, T_solidus, T_liquidus, L_rho,
+                            FLAG_SOLID, FLAG_FLUID):
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_FLUID=FLAG_FLUID)
+    _need(_n(T), H, f_l, phi, flags)
+    _call("waam_remelt_hot_solid_scalar", P(T), P(H), P(f_l), P(phi), P(flags), cp_rho, T_solidus,
+          T_liquidus, L_rho, _n(T), _stream())
+
+
+def solidify_trailing_pool_scalar(T, H, f_l, phi, flags, ux, uy, uz, cp_rho, arc_i, arc_j, arc_k,
+                                  dir_x, dir_y, dir_z, lookback_cells, T_freeze, T_solidus,
+                                  FLAG_SOLID, FLAG_FLUID, FLAG_GAS):
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_FLUID=FLAG_FLUID, FLAG_GAS=FLAG_GAS)
+    nx, ny, nz = tensor_of(T).shape
+    _need(nx * ny * nz, H, f_l, phi, flags, ux, uy, uz)
+    _call("waam_solidify_trailing_pool_scalar", P(T), P(H), P(f_l), P(phi), P(flags), P(ux),
+          P(uy), P(uz), cp_rho, arc_i, arc_j, arc_k, dir_x, dir_y, dir_z, lookback_cells,
+          T_freeze, T_solidus, nx, ny, nz, _stream())
+
+
+# ------------------------------------------------------------------ forces
+def compute_csf_tension(phi, flags, Fx, Fy, Fz, gamma_lu, FLAG_SOLID, FLAG_GAS, nx, ny, nz,
+                        enable_wetting=0, theta_rad=0.0):
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
+    _need(nx * ny * nz, phi, flags, Fx, Fy, Fz)
+    c, s = contact_angle_cos_sin(theta_rad)
+    _call("waam_compute_csf_tension", P(phi), P(flags), P(Fx), P(Fy), P(Fz), gamma_lu,
+          int(enable_wetting), c, s, nx, ny, nz, _stream())
+
+
+compute_csf_wetting = compute_csf_tension
+
+
+def compute_curvature_field(phi, flags, kappa_out, FLAG_GAS, nx, ny, nz):
+    _flags_ok(FLAG_GAS=FLAG_GAS)
+    _need(nx * ny * nz, phi, flags, kappa_out)
+    _call("waam_compute_curvature_field", P(phi), P(flags), P(kappa_out), nx, ny, nz, _stream())
+
+
+def compute_marangoni_force(T, phi, f_l, Fx, Fy, Fz, flags, dgamma_dT, dx, FLAG_SOLID, FLAG_GAS,
+                            nx, ny, nz):
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
+    _need(nx * ny * nz, T, phi, f_l, Fx, Fy, Fz, flags)
+    _call("waam_compute_marangoni_force", P(T), P(phi), P(f_l), P(Fx), P(Fy), P(Fz), P(flags),
+          dgamma_dT, nx, ny, nz, _stream())
+
+
+def add_buoyancy(T, Fz, f_l, flags, g_lu, beta, T_ref, rho_ref, FLAG_SOLID, FLAG_GAS):
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
+    _need(_n(T), Fz, f_l, flags)
+    _call("waam_add_buoyancy", P(T), P(Fz), P(f_l), P(flags), g_lu, beta, T_ref, rho_ref, _n(T),
+          _stream())
+
+
+def add_hydrostatic_gravity(Fz, f_l, flags, rho_ref, g_lu, FLAG_SOLID, FLAG_GAS):
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
+    _need(_n(Fz), f_l, flags)
+    _call("waam_add_hydrostatic_gravity", P(Fz), P(f_l), P(flags), rho_ref, g_lu, _n(Fz),
+          _stream())
+
+
+def apply_arc_pressure(Fz, flags, phi, arc_i, arc_j, arc_k, sigma, pressure_pa, dt, dx, rho_ref,
+                       FLAG_SOLID, FLAG_GAS):
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
+    nx, ny, nz = tensor_of(Fz).shape
+    _need(nx * ny * nz, flags, phi)
+    _call("waam_apply_arc_pressure", P(Fz), P(flags), P(phi), arc_i, arc_j, arc_k, sigma,
+          pressure_pa, dt, dx, rho_ref, nx, ny, nz, _stream())
+
+
+def clamp_body_force_magnitude(Fx, Fy, Fz, flags, F_max_lu, FLAG_SOLID, FLAG_GAS):
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
+    _need(_n(Fx), Fy, Fz, flags)
+    _call("waam_clamp_body_force_magnitude", P(Fx), P(Fy), P(Fz), P(flags), F_max_lu, _n(Fx),
+          _stream())
+
+
+def apply_droplet_impact_pressure(Fz, flags, phi, f_l, arc_i, arc_j, arc_k, drop_radius,
+                                  impact_pa, dt, dx, rho_ref, FLAG_SOLID, FLAG_GAS):
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
+    nx, ny, nz = tensor_of(Fz).shape
+    _need(nx * ny * nz, flags, phi, f_l)
+    _call("waam_apply_droplet_impact_pressure", P(Fz), P(flags), P(phi), P(f_l), arc_i, arc_j,
+          arc_k, drop_radius, impact_pa, dt, dx, rho_ref, nx, ny, nz, _stream())
+
+
+def snapshot_forces(Fx, Fy, Fz, Fx_snap, Fy_snap, Fz_snap):
+    for a, b in ((Fx, Fx_snap), (Fy, Fy_snap), (Fz, Fz_snap)):
+        tensor_of(b).copy_(tensor_of(a))
+
+
+# ------------------------------------------------------------------ LBM
+def collide_mrt(f_src, f_dst, rho, ux, uy, uz, Fx, Fy, Fz, f_l, flags, EX, EY, EZ, W, OPP,
+                omega_s, omega_b, C_darcy, FLAG_SOLID, FLAG_GAS, nx, ny, nz):
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
+    n = nx * ny * nz
+    _need_pop(n, f_src, f_dst)
+    _need(n, rho, ux, uy, uz, Fx, Fy, Fz, f_l, flags)
+    _call("waam_collide_mrt", P(f_src), P(f_dst), P(rho), P(ux), P(uy), P(uz), P(Fx), P(Fy),
+          P(Fz), P(f_l), P(flags), omega_s, omega_b, C_darcy, nx, ny, nz, _stream())
+
+
+def collide_srt(f_src, f_dst, rho, ux, uy, uz, Fx, Fy, Fz, f_l, flags, tau, omega, dt_lu, C_darcy,
+                FLAG_SOLID, FLAG_GAS, nx, ny, nz):
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
+    n = nx * ny * nz
+    _need_pop(n, f_src, f_dst)
+    _need(n, rho, ux, uy, uz, Fx, Fy, Fz, f_l, flags)
+    _call("waam_collide_srt", P(f_src), P(f_dst), P(rho), P(ux), P(uy), P(uz), P(Fx), P(Fy),
+          P(Fz), P(f_l), P(flags), tau, omega, dt_lu, C_darcy, nx, ny, nz, _stream())
+
+
+def collide_srt_variable_tau(f_src, f_dst, rho, ux, uy, uz, Fx, Fy, Fz, f_l, flags, tau_field,
+                             C_darcy, FLAG_SOLID, FLAG_GAS, nx, ny, nz):
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
+    n = nx * ny * nz
+    _need_pop(n, f_src, f_dst)
+    _need(n, rho, ux, uy, uz, Fx, Fy, Fz, f_l, flags, tau_field)
+    _call("waam_collide_srt_variable_tau", P(f_src), P(f_dst), P(rho), P(ux), P(uy), P(uz), P(Fx),
+          P(Fy), P(Fz), P(f_l), P(flags), P(tau_field), C_darcy, nx, ny, nz, _stream())
+
+
+def stream(f_src, f_dst, flags, FLAG_SOLID, FLAG_GAS, nx, ny, nz):
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
+    n = nx * ny * nz
+    _need_pop(n, f_src, f_dst)
+    _need(n, flags)
+    _call("waam_stream", P(f_src), P(f_dst), P(flags), nx, ny, nz, _stream())
+
+
+def clamp_velocity_mach(ux, uy, uz, flags, u_max_lu, FLAG_SOLID, FLAG_GAS):
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
+    _need(_n(ux), uy, uz, flags)
+    _call("waam_clamp_velocity_mach", P(ux), P(uy), P(uz), P(flags), u_max_lu, _n(ux), _stream())
+
+
+# ------------------------------------------------------------------ deposition, tracers
+def feed_wire_surface(f_src, flags, f_l, phi, H, T, rho, arc_i, arc_j, arc_k, footprint_r,
+                      droplet_radius, target_vol, T_drop, cp_rho, L_rho, rho0, vol_acc, real_acc,
+                      cell_vol, FLAG_GAS, FLAG_FLUID, FLAG_SOLID, nx, ny, nz, scratch=None):
+    _flags_ok(FLAG_GAS=FLAG_GAS, FLAG_FLUID=FLAG_FLUID, FLAG_SOLID=FLAG_SOLID)
+    n = nx * ny * nz
+    _need_pop(n, f_src)
+    _need(n, flags, f_l, phi, H, T, rho)
+    sc = scratch if scratch is not None else _scratch_of(H)
+    _call("waam_feed_wire_surface", P(f_src), P(flags), P(f_l), P(phi), P(H), P(T), P(rho), arc_i,
+          arc_j, arc_k, footprint_r, droplet_radius, target_vol, T_drop, cp_rho, L_rho, rho0,
+          P(vol_acc), P(real_acc), cell_vol, nx, ny, nz, P(sc), _stream())
+
+
+def feed_wire_momentum_impact(f_src, rho, ux, uy, uz, flags, f_l, arc_i, arc_j, arc_k,
+                              droplet_radius, vx_lu, vy_lu, vz_lu, FLAG_GAS, nx, ny, nz):
+    _flags_ok(FLAG_GAS=FLAG_GAS)
+    n = nx * ny * nz
+    _need_pop(n, f_src)
+    _need(n, rho, ux, uy, uz, flags, f_l)
+    _call("waam_feed_wire_momentum_impact", P(f_src), P(rho), P(ux), P(uy), P(uz), P(flags),
+          P(f_l), arc_i, arc_j, arc_k, droplet_radius, vx_lu, vy_lu, vz_lu, nx, ny, nz, _stream())
+
+
+def inject_tracers(pos, active, head, max_tracers, arc_x, arc_y, arc_z, sigma_m, spawn_count,
+                   seed=0):
+    if tensor_of(pos).numel() < 3 * max_tracers or tensor_of(active).numel() < max_tracers:
+        raise ValueError("tracer buffers smaller than max_tracers")
+    _call("waam_inject_tracers", P(pos), P(active), P(head), int(max_tracers), arc_x, arc_y,
+          arc_z, sigma_m, int(spawn_count), int(seed) & 0xFFFFFFFF, _stream())
+
+
+def advect_tracers(pos, active, ux, uy, uz, f_l, flags, dx, dt, max_tracers, FLAG_SOLID,
+                   FLAG_GAS):
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
+    nx, ny, nz = tensor_of(ux).shape
+    if tensor_of(pos).numel() < 3 * max_tracers or tensor_of(active).numel() < max_tracers:
+        raise ValueError("tracer buffers smaller than max_tracers")
+    _call("waam_advect_tracers", P(pos), P(active), P(ux), P(uy), P(uz), P(f_l), P(flags), dx, dt,
+          int(max_tracers), nx, ny, nz, _stream())
+
+
+# ------------------------------------------------------------------ telemetry
+def telemetry_pool_reduce_into(T, f_l, phi, dT_dt, ux, uy, uz, out, scratch):
+    nx, ny, nz = tensor_of(T).shape
+    _need(nx * ny * nz, f_l, phi, dT_dt, ux, uy, uz)
+    _call("waam_telemetry_pool_reduce", P(T), P(f_l), P(phi), P(dT_dt), P(ux), P(uy), P(uz),
+          P(out), nx, ny, nz, P(scratch), _stream())
+
+
+def extract_fl_yz_slice(f_l, out, i_slice, ny, nz):
+    tensor_of(out).copy_(tensor_of(f_l)[int(i_slice)])
+
+
+__all__ = [n for n in dir() if not n.startswith("_") and n not in ("math", "torch")]

@@ -1,0 +1,92 @@
+"""Arc pressure and droplet impact helpers (reference physics/weld_forces.py:22-193)."""
+
+from __future__ import annotations
+
+import math
+
+from .. import kernels
+from .deposition_balance import droplet_mass_kg, infer_transfer_mode
+
+MU0 = 4.0e-7 * math.pi
+
+
+def lin_eagar_peak_pa(current_A: float, sigma_p_m: float) -> float:
+    """Lin & Eagar (1986) peak arc pressure mu0 I^2 / (4 pi^2 sigma^2) [Pa]."""
+    sig = max(float(sigma_p_m), 1.0e-6)
+    i = max(float(current_A), 0.0)
+    return MU0 * i * i / (4.0 * math.pi * math.pi * sig * sig)
+
+
+def arc_pressure_sigma_m(twin) -> float:
+    sig = getattr(twin, "pressure_sigma_m", None)
+    if sig is not None and sig > 0.0:
+        return float(sig)
+    return float(twin.arc_sigma_m)
+
+
+def arc_pressure_sigma_cells(twin) -> float:
+    return arc_pressure_sigma_m(twin) / twin.grid.dx
+
+
+def arc_pressure_peak_pa(twin) -> float:
+    model = str(getattr(twin, "arc_pressure_model", "constant")).strip().lower()
+    if model in ("lin_eagar", "lineagar", "lin-eagar"):
+        return lin_eagar_peak_pa(twin.welding_current_A, arc_pressure_sigma_m(twin))
+    return float(twin.arc_pressure)
+
+
+def droplet_impact_velocity_m_s(twin, m_drop_kg: float | None = None) -> float:
+    drop_len = 3e-3
+    if twin.droplet_freq > 0 and twin.wire_feed_m_s > 0:
+        drop_len = twin.wire_feed_m_s / twin.droplet_freq
+    if m_drop_kg is None:
+        m_drop_kg = droplet_mass_kg(twin)
+    v_feed = max(twin.wire_feed_m_s, 0.0)
+    v_grav = math.sqrt(max(0.0, 2.0 * 9.81 * drop_len))
+    dia_ref = max(twin.wire_diameter_m, 1e-6)
+    pinch = 0.35 * math.sqrt(max(twin.welding_current_A, 0.0) / 200.0) * math.sqrt(1.2e-3 / dia_ref)
+    scale = {"globular": 0.78, "spray": 1.18, "pulsed": 1.05}.get(infer_transfer_mode(twin), 1.0)
+    mass_term = 1.0
+    if m_drop_kg > 0:
+        mass_term = max(0.8, min(1.25, math.sqrt(droplet_mass_kg(twin) / m_drop_kg)))
+    return min(5.0, scale * max(v_feed, v_grav + pinch) * mass_term)
+
+
+def droplet_velocity_lu(twin, m_drop_kg: float | None = None) -> tuple[float, float, float]:
+    g = twin.grid
+    v = droplet_impact_velocity_m_s(twin, m_drop_kg)
+    lead = math.radians(float(getattr(twin, "impact_lead_angle_deg", 0.0)))
+    dxd, dyd, _ = getattr(twin, "_torch_dir_xyz", (1.0, 0.0, 0.0))
+    dxy = math.sqrt(dxd * dxd + dyd * dyd)
+    if dxy < 1e-9:
+        dxd, dyd, dxy = 1.0, 0.0, 1.0
+    dxd /= dxy
+    dyd /= dxy
+    v_xy = max(twin.travel_speed_m_s, abs(v) * math.tan(lead))
+    v_xy = min(v * 0.65, v_xy)
+    v_z = -math.sqrt(max(v * v - v_xy * v_xy, v * v * 0.20))
+    return (v_xy * dxd * g.dt / g.dx, v_xy * dyd * g.dt / g.dx, v_z * g.dt / g.dx)
+
+
+def droplet_impact_pressure_pa(twin, m_drop_kg: float | None = None,
+                               drop_r_m: float | None = None) -> float:
+    m = droplet_mass_kg(twin) if m_drop_kg is None else m_drop_kg
+    if m <= 0:
+        return 0.0
+    v = droplet_impact_velocity_m_s(twin, m)
+    r = drop_r_m if drop_r_m is not None else max(twin.grid.dx * 1.5, twin.wire_diameter_m * 0.5)
+    return (0.5 * m * v * v) / max(math.pi * r * r, 1e-12)
+
+
+def apply_droplet_impact(twin, g, arc_i, arc_j, arc_k, drop_r, drop_mass_kg=None) -> None:
+    if not twin.enable_deposition_momentum:
+        return
+    vx, vy, vz = droplet_velocity_lu(twin, drop_mass_kg)
+    kernels.feed_wire_momentum_impact(g.f_src, g.rho, g.ux, g.uy, g.uz, g.flags, g.f_l, arc_i,
+                                      arc_j, arc_k, drop_r, vx, vy, vz, g.FLAG_GAS, g.nx, g.ny,
+                                      g.nz)
+    if twin.enable_droplet_impact_pressure:
+        kernels.apply_droplet_impact_pressure(
+            g.Fz, g.flags, g.phi, g.f_l, arc_i, arc_j, arc_k, drop_r,
+            droplet_impact_pressure_pa(twin, drop_mass_kg, drop_r * g.dx), g.dt, g.dx,
+            twin.mat.rho, g.FLAG_SOLID, g.FLAG_GAS)

@@ -1,0 +1,211 @@
+"""One coupled physics timestep (reference solvers/coupled_step.py:17-464).
+
+``coupled_step`` launches the HIP kernels in the reference's order, one launch per reference
+kernel, on the current torch stream. Differences from the reference's host code, all of them
+keeping its results:
+
+* in-place neighbour-reading kernels (thermal advection, contact-angle BC) write a partner
+  buffer that is then swapped with the field, as is ``phi_tmp`` after VOF advection (the
+  reference copies it back);
+* the droplet placement loop (up to ten feed_wire_surface calls with a growing footprint) runs
+  in a single device launch; the host reads back the placed volume once per droplet for the
+  mass ledger;
+* arc_k is read back once per step, as in the reference.
+"""
+
+from __future__ import annotations
+
+import logging
+
+import numpy as np
+
+from .. import kernels
+from ..field import P
+from ..physics import deposition, weld_forces
+from ..physics.electrical_stickout import droplet_entry_temperature_K
+
+log = logging.getLogger("waam_solver_amd")
+
+
+def _swap(g, a: str, b: str) -> None:
+    fa, fb = getattr(g, a), getattr(g, b)
+    setattr(g, a, fb)
+    setattr(g, b, fa)
+
+
+def _clamp_enthalpy_ceiling(twin, g) -> None:
+    if not twin.enable_enthalpy_cap:
+        return
+    kernels.clamp_enthalpy_ceiling_scalar(g.H, g.flags, twin.cp_rho, twin.mat.T_solidus,
+                                          twin.mat.T_liquidus, twin.T_vapor_cap_K, twin.L_rho,
+                                          g.FLAG_GAS)
+
+
+def resolve_arc_k(twin, g, arc_i: float, arc_j: float, torch_z_m: float | None = None) -> float:
+    """Top metal cell in the torch column (reference coupled_step.py:34-59)."""
+    i0 = max(0, min(int(arc_i), g.nx - 1))
+    j0 = max(0, min(int(arc_j), g.ny - 1))
+    kernels.surface_height_at(g.phi, g.flags, g.surface_k_buf, i0, j0, twin.nz_solid, g.FLAG_GAS,
+                              g.nz)
+    k_surface = float(g.surface_k_buf[None])
+    if k_surface < 1.0:
+        k_surface = float(max(1, twin.nz_solid - 1))
+    if getattr(twin, "use_torch_z", False) and torch_z_m is not None:
+        z_bead = max(twin.substrate_z_m, torch_z_m - twin.ctwd_m)
+        k_robot = twin.nz_solid + (z_bead - twin.substrate_z_m) / g.dx - 1.0
+        k_robot = max(float(twin.nz_solid) + 0.5, min(k_robot, float(g.nz) - 2.0))
+        return max(k_surface, k_robot)
+    return k_surface
+
+
+def _droplet(twin, g, x_m: float, y_m: float, arc_i: float, arc_j: float, arc_k: float) -> None:
+    """Droplet block (reference coupled_step.py:110-166)."""
+    sim_time = twin._step_n * g.dt
+    if twin.droplet_freq <= 0:
+        return
+    period = deposition.droplet_period_s(twin)
+    if sim_time - twin._last_droplet_time < period:
+        return
+    drop_dt = sim_time - twin._last_droplet_time
+    twin._last_droplet_time = sim_time
+    drop_mass = deposition.droplet_mass_for_interval_kg(twin, drop_dt)
+    drop_r = deposition.droplet_radius_cells_from_mass_kg(twin, drop_mass)
+    drop_vol = drop_mass / twin.mat.rho
+    T_drop = droplet_entry_temperature_K(twin)
+    kernels.inject_tracers(g.porosity_pos, g.porosity_active, g.tracer_head, g.max_tracers, x_m,
+                           y_m, float(arc_k + 1) * g.dx, float(twin.sigma_cells * g.dx), 50,
+                           seed=twin.tracer_seed)
+    foot_r = deposition.deposition_footprint_cells(twin, drop_r)
+    from .._lib import lib
+    import torch
+
+    lib().call("waam_deposit_droplet", P(g.f_src), P(g.flags), P(g.f_l), P(g.phi), P(g.H), P(g.T),
+               P(g.rho), arc_i, arc_j, arc_k, float(foot_r), drop_r, float(drop_vol), T_drop,
+               twin.cp_rho, twin.L_rho, 1.0, P(g.deposit_vol_buf), P(g.deposit_real_buf),
+               g.dx ** 3, P(g.deposit_result), g.nx, g.ny, g.nz, P(g.scratch),
+               torch.cuda.current_stream().cuda_stream)
+    placed = float(g.deposit_result[0])
+    if placed < drop_vol * 0.98:
+        twin._deposition_overflow += 1
+        if not twin._warned_overflow and twin._deposition_overflow >= 20:
+            twin._warned_overflow = True
+            log.warning(f"[coupled_step] {twin._deposition_overflow} droplets deposited less "
+                        "volume than the wire feed supplies; mass_balance_ratio will drift")
+    twin._deposited_volume_m3 += placed
+    twin._n_droplets_fired += 1
+    weld_forces.apply_droplet_impact(twin, g, arc_i, arc_j, arc_k, drop_r, drop_mass)
+
+
+def coupled_step(twin, torch_x_m: float, torch_y_m: float, is_welding: bool,
+                 torch_z_m: float | None = None) -> None:
+    if twin.use_material_tables:
+        raise NotImplementedError("temperature-dependent material tables are not built; "
+                                  "use a scalar (constants-only) material")
+    g = twin.grid
+    n = g.nx, g.ny, g.nz
+    arc_i = torch_x_m / g.dx
+    arc_j = torch_y_m / g.dx
+    arc_k = resolve_arc_k(twin, g, arc_i, arc_j, torch_z_m)
+    twin._last_arc_ijk = (float(arc_i), float(arc_j), float(arc_k))
+
+    kernels.clear_forces(g.Fx, g.Fy, g.Fz)
+    current_pressure = weld_forces.arc_pressure_peak_pa(twin)
+    pressure_sigma_cells = weld_forces.arc_pressure_sigma_cells(twin)
+
+    if is_welding:
+        twin.arc_source.inject(twin, g, arc_i, arc_j, arc_k)
+        _clamp_enthalpy_ceiling(twin, g)
+        _droplet(twin, g, torch_x_m, torch_y_m, arc_i, arc_j, arc_k)
+
+    # thermal advection-diffusion, Jacobi: H_tmp <- H + dH(H); swap
+    kernels.advect_diffuse_temperature_into(g.H, g.H_tmp, g.T, g.ux, g.uy, g.uz, g.flags,
+                                            twin.alpha_lu, 1.0, twin.cp_rho, *n)
+    _swap(g, "H", "H_tmp")
+    if twin.enable_heat_loss:
+        kernels.apply_thermal_boundary_losses(
+            g.H, g.T, g.flags, twin.T_amb, twin.h_conv, twin.eps_rad,
+            1 if twin.enable_convection else 0, 1 if twin.enable_radiation else 0, twin.cp_rho,
+            g.dt, g.dx, twin.sigma_sb, g.FLAG_SOLID, g.FLAG_GAS, *n)
+        kernels.clamp_enthalpy_floor_scalar(g.H, twin.cp_rho, g.flags, twin.T_amb, g.FLAG_GAS)
+    kernels.update_phase(g.H, g.T, g.f_l, twin.cp_rho, twin.L_rho, twin.mat.T_solidus,
+                         twin.mat.T_liquidus)
+    _clamp_enthalpy_ceiling(twin, g)
+    kernels.update_phase(g.H, g.T, g.f_l, twin.cp_rho, twin.L_rho, twin.mat.T_solidus,
+                         twin.mat.T_liquidus)
+    kernels.update_T_max(g.T, g.T_max, g.flags, g.FLAG_GAS)
+    kernels.update_cooling_rate(g.T, g.T_prev, g.dT_dt, g.flags, g.dt, g.FLAG_GAS)
+
+    if twin.enable_vof:
+        g.ensure_vof_buffers()
+        kernels.advect_phi(g.phi_tmp, g.phi, g.ux, g.uy, g.uz, g.flags, g.FLAG_SOLID, g.FLAG_GAS,
+                           *n)
+        _swap(g, "phi", "phi_tmp")
+        kernels.reinitialize_phi(g.phi, g.flags, g.FLAG_SOLID, g.FLAG_GAS, g.FLAG_FLUID)
+        if twin.enable_wetting:
+            kernels.apply_contact_angle_phi_bc_into(g.phi, g.phi_tmp, g.flags, twin.theta_rad, *n)
+            _swap(g, "phi", "phi_tmp")
+        kernels.update_flags_from_phi(g.phi, g.f_l, g.flags, twin.nz_solid, g.FLAG_FLUID,
+                                      g.FLAG_SOLID, g.FLAG_GAS, g.FLAG_IFACE)
+        kernels.sync_phi_liquid_fraction(g.phi, g.f_l, g.flags, g.FLAG_GAS, g.FLAG_SOLID)
+
+    if twin.enable_csf_tension:
+        kernels.compute_csf_tension(g.phi, g.flags, g.Fx, g.Fy, g.Fz, twin.gamma_lu, g.FLAG_SOLID,
+                                    g.FLAG_GAS, *n, int(twin.enable_wetting), twin.theta_rad)
+    kernels.compute_marangoni_force(g.T, g.phi, g.f_l, g.Fx, g.Fy, g.Fz, g.flags,
+                                    twin.dgamma_dT_lu, g.dx, g.FLAG_SOLID, g.FLAG_GAS, *n)
+    if is_welding and twin.enable_gas_shear:
+        raise NotImplementedError("gas shear (physics_tier: full) is not built")
+    if is_welding:
+        kernels.apply_arc_pressure(g.Fz, g.flags, g.phi, arc_i, arc_j, arc_k,
+                                   pressure_sigma_cells, current_pressure, g.dt, g.dx,
+                                   twin.mat.rho, g.FLAG_SOLID, g.FLAG_GAS)
+        if twin.enable_recoil:
+            raise NotImplementedError("vapour recoil (physics_tier: full) is not built")
+    if twin.enable_hydrostatic_gravity:
+        kernels.add_hydrostatic_gravity(g.Fz, g.f_l, g.flags, 1.0, twin.g_lu, g.FLAG_SOLID,
+                                        g.FLAG_GAS)
+    kernels.add_buoyancy(g.T, g.Fz, g.f_l, g.flags, twin.g_lu, twin.beta_T, twin.mat.T_liquidus,
+                         1.0, g.FLAG_SOLID, g.FLAG_GAS)
+    if is_welding and twin.enable_lorentz:
+        raise NotImplementedError("the Lorentz force solve (physics_tier: full) is not built")
+    u_cap = float(getattr(twin, "u_mach_limit_lu", 0.08))
+    F_cap = float(getattr(twin, "force_limit_lu", 0.05))
+    if F_cap > 0.0:
+        kernels.clamp_body_force_magnitude(g.Fx, g.Fy, g.Fz, g.flags, F_cap, g.FLAG_SOLID,
+                                           g.FLAG_GAS)
+    if twin.use_srt:
+        kernels.collide_srt(g.f_src, g.f_dst, g.rho, g.ux, g.uy, g.uz, g.Fx, g.Fy, g.Fz, g.f_l,
+                            g.flags, g.tau, twin.omega, 1.0, twin.C_darcy, g.FLAG_SOLID,
+                            g.FLAG_GAS, *n)
+    else:
+        kernels.collide_mrt(g.f_src, g.f_dst, g.rho, g.ux, g.uy, g.uz, g.Fx, g.Fy, g.Fz, g.f_l,
+                            g.flags, None, None, None, None, None, twin.omega, twin.omega_bulk,
+                            twin.C_darcy, g.FLAG_SOLID, g.FLAG_GAS, *n)
+    if u_cap > 0.0:
+        kernels.clamp_velocity_mach(g.ux, g.uy, g.uz, g.flags, u_cap, g.FLAG_SOLID, g.FLAG_GAS)
+    kernels.stream(g.f_dst, g.f_src, g.flags, g.FLAG_SOLID, g.FLAG_GAS, *n)
+    kernels.advect_tracers(g.porosity_pos, g.porosity_active, g.ux, g.uy, g.uz, g.f_l, g.flags,
+                           g.dx, g.dt, g.max_tracers, g.FLAG_SOLID, g.FLAG_GAS)
+    kernels.update_time_above_T(g.T, g.flags, g.time_above_800_s, g.time_above_1100_s,
+                                g.time_above_solidus_s, g.dt, 800.0 + 273.15, 1100.0 + 273.15,
+                                twin.mat.T_solidus, g.FLAG_GAS)
+    if twin.snapshot_forces:
+        kernels.snapshot_forces(g.Fx, g.Fy, g.Fz, g.Fx_snap, g.Fy_snap, g.Fz_snap)
+    if twin.enable_substrate_growth or twin.enable_bead_freeze:
+        kernels.remelt_hot_solid_scalar(g.T, g.H, g.f_l, g.phi, g.flags, twin.cp_rho,
+                                        twin.mat.T_solidus, twin.mat.T_liquidus, twin.L_rho,
+                                        g.FLAG_SOLID, g.FLAG_FLUID)
+        if is_welding and twin.enable_bead_freeze:
+            dir_x, dir_y, dir_z = getattr(twin, "_torch_dir_xyz", (1.0, 0.0, 0.0))
+            lookback = max(2.0, twin.trailing_solidify_lookback_mm / (g.dx * 1000.0))
+            T_freeze = twin.mat.T_liquidus + twin.trailing_solidify_temp_margin_K
+            kernels.solidify_trailing_pool_scalar(
+                g.T, g.H, g.f_l, g.phi, g.flags, g.ux, g.uy, g.uz, twin.cp_rho, arc_i, arc_j,
+                arc_k, dir_x, dir_y, dir_z, lookback, T_freeze, twin.mat.T_solidus, g.FLAG_SOLID,
+                g.FLAG_FLUID, g.FLAG_GAS)
+        kernels.solidify_cooled_metal(g.T, g.f_l, g.phi, g.flags, g.ux, g.uy, g.uz,
+                                      twin.mat.T_solidus,
+                                      twin.enable_bead_freeze or twin.enable_substrate_growth,
+                                      g.FLAG_SOLID, g.FLAG_FLUID, g.FLAG_GAS)
+    g.swap_buffers()
+    twin._step_n += 1
