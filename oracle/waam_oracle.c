@@ -170,6 +170,7 @@ void or_inject_arc_heat(float *H, const uint8_t *flags, const float *f_l, float 
     if (norm_out) *norm_out = w_total;
     float epw = 0.0f;
     if (w_total > 1e-9f) epw = Q_w * eta * dt / (w_total * dx3);
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < ny; ++j)
             for (int k = 0; k < nz; ++k) {
@@ -216,6 +217,7 @@ void or_inject_goldak_heat(float *H, const uint8_t *flags, const float *f_l, flo
     if (norm_out) *norm_out = w_total;
     float epw = 0.0f;
     if (w_total > 1e-9f) epw = Q_w * eta * dt / (w_total * dx3);
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < ny; ++j)
             for (int k = 0; k < nz; ++k) {
@@ -405,6 +407,7 @@ void or_apply_droplet_impact_pressure(float *Fz, const uint8_t *flags, const flo
     float inv2s2 = 1.0f / (2.0f * dr * dr + eps);
     float F_peak_phys = impact_pa / (rho_ref * dx);
     float F_peak_lu = F_peak_phys * dt * dt / dx;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < ny; ++j)
             for (int k = 0; k < nz; ++k) {
@@ -536,6 +539,7 @@ void or_compute_csf_tension(const float *phi, const uint8_t *flags, float *Fx, f
                             float sin_t, int nx, int ny, int nz)
 {
     const float eps = 1e-6f;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < ny; ++j)
             for (int k = 0; k < nz; ++k) {
@@ -575,6 +579,7 @@ void or_compute_curvature_field(const float *phi, const uint8_t *flags, float *k
                                 int ny, int nz)
 {
     const float eps = 1e-6f;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < ny; ++j)
             for (int k = 0; k < nz; ++k) {
@@ -647,6 +652,7 @@ void or_apply_contact_angle_phi_bc_jacobi(const float *phi_in, float *phi_out,
                                           int ny, int nz)
 {
     float film = 0.5f * (1.0f - cos_t);
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < ny; ++j)
             for (int k = 0; k < nz; ++k) {
@@ -712,6 +718,7 @@ void or_update_phase(const float *H, float *T, float *f_l, float cp_rho, float L
     float H_sol = cp_rho * T_s;
     float H_liq = H_sol + L_rho;
     size_t N = NCELL;
+#pragma omp parallel for schedule(static)
     for (size_t c = 0; c < N; ++c) {
         float h = H[c];
         if (h <= H_sol) {
@@ -874,6 +881,7 @@ void or_clamp_enthalpy_floor_scalar(float *H, float cp_rho, const uint8_t *flags
 {
     float h_min = cp_rho * T_floor;
     size_t N = NCELL;
+#pragma omp parallel for schedule(static)
     for (size_t c = 0; c < N; ++c) {
         if (flags[c] == GAS) continue;
         if (H[c] < h_min) H[c] = h_min;
@@ -888,6 +896,7 @@ void or_clamp_enthalpy_ceiling_scalar(float *H, const uint8_t *flags, float cp_r
     float h_cap_liquid = H_liq + cp_rho * (T_cap - T_l);
     float h_cap_solid = cp_rho * T_cap;
     size_t N = NCELL;
+#pragma omp parallel for schedule(static)
     for (size_t c = 0; c < N; ++c) {
         if (flags[c] == GAS) continue;
         float h = H[c];
@@ -904,6 +913,7 @@ void or_update_cooling_rate(const float *T, float *T_prev, float *dT_dt, const u
 {
     float inv_dt = 1.0f / (dt_phys + 1e-12f);
     size_t N = NCELL;
+#pragma omp parallel for schedule(static)
     for (size_t c = 0; c < N; ++c) {
         if (flags[c] == GAS) continue;
         dT_dt[c] = (T[c] - T_prev[c]) * inv_dt;
@@ -914,6 +924,7 @@ void or_update_cooling_rate(const float *T, float *T_prev, float *dT_dt, const u
 void or_update_T_max(const float *T, float *T_max, const uint8_t *flags, int nx, int ny, int nz)
 {
     size_t N = NCELL;
+#pragma omp parallel for schedule(static)
     for (size_t c = 0; c < N; ++c) {
         if (flags[c] == GAS) continue;
         if (T[c] > T_max[c]) T_max[c] = T[c];
@@ -925,6 +936,7 @@ void or_update_time_above_T(const float *T, const uint8_t *flags, float *t800, f
                             int ny, int nz)
 {
     size_t N = NCELL;
+#pragma omp parallel for schedule(static)
     for (size_t c = 0; c < N; ++c) {
         if (flags[c] == GAS) continue;
         float t = T[c];
@@ -944,6 +956,7 @@ void or_compute_marangoni_force(const float *T, const float *phi, const float *f
                                 int nx, int ny, int nz)
 {
     const float eps = 1e-6f, fl_min = 0.05f;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < ny; ++j)
             for (int k = 0; k < nz; ++k) {
@@ -975,6 +988,7 @@ void or_add_buoyancy(const float *T, float *Fz, const float *f_l, const uint8_t 
                      float g_lu, float beta, float T_ref, float rho_ref, int nx, int ny, int nz)
 {
     size_t N = NCELL;
+#pragma omp parallel for schedule(static)
     for (size_t c = 0; c < N; ++c) {
         if (flags[c] == SOLID || flags[c] == GAS) continue;
         float fl = f_l[c];
@@ -986,6 +1000,7 @@ void or_add_hydrostatic_gravity(float *Fz, const float *f_l, const uint8_t *flag
                                 float rho_ref, float g_lu, int nx, int ny, int nz)
 {
     size_t N = NCELL;
+#pragma omp parallel for schedule(static)
     for (size_t c = 0; c < N; ++c) {
         if (flags[c] == SOLID || flags[c] == GAS) continue;
         float fl = f_l[c];
@@ -1001,6 +1016,7 @@ void or_apply_arc_pressure(float *Fz, const uint8_t *flags, const float *phi, fl
     float inv2s2 = 1.0f / (2.0f * sigma * sigma + eps);
     float F_peak_phys = pressure_pa / (rho_ref * dx);
     float F_peak_lu = F_peak_phys * (dt * dt) / dx;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < ny; ++j)
             for (int k = 0; k < nz; ++k) {
@@ -1021,6 +1037,7 @@ void or_clamp_body_force_magnitude(float *Fx, float *Fy, float *Fz, const uint8_
 {
     const float eps = 1e-12f;
     size_t N = NCELL;
+#pragma omp parallel for schedule(static)
     for (size_t c = 0; c < N; ++c) {
         if (flags[c] == SOLID || flags[c] == GAS) continue;
         float fx = Fx[c], fy = Fy[c], fz = Fz[c];
@@ -1086,6 +1103,7 @@ void or_advect_phi(float *phi_dst, const float *phi_src, const float *ux, const 
 void or_reinitialize_phi(float *phi, const uint8_t *flags, int nx, int ny, int nz)
 {
     size_t N = NCELL;
+#pragma omp parallel for schedule(static)
     for (size_t c = 0; c < N; ++c) {
         uint8_t f = flags[c];
         if (f == SOLID) phi[c] = 1.0f;
@@ -1102,6 +1120,7 @@ void or_reinitialize_phi(float *phi, const uint8_t *flags, int nx, int ny, int n
 void or_update_flags_from_phi(const float *phi, const float *f_l, uint8_t *flags, int nz_solid,
                               int nx, int ny, int nz)
 {
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < ny; ++j)
             for (int k = 0; k < nz; ++k) {
@@ -1122,6 +1141,7 @@ void or_sync_phi_liquid_fraction(const float *phi, float *f_l, const uint8_t *fl
                                  int ny, int nz)
 {
     size_t N = NCELL;
+#pragma omp parallel for schedule(static)
     for (size_t c = 0; c < N; ++c) {
         uint8_t f = flags[c];
         if (f == GAS || phi[c] < 0.05f) f_l[c] = 0.0f;
@@ -1138,6 +1158,7 @@ void or_solidify_cooled_metal(const float *T, float *f_l, float *phi, uint8_t *f
                               int nz)
 {
     size_t N = NCELL;
+#pragma omp parallel for schedule(static)
     for (size_t c = 0; c < N; ++c) {
         if (flags[c] == GAS) continue;
         if (phi[c] < 0.85f) continue;
@@ -1160,6 +1181,7 @@ void or_remelt_hot_solid_scalar(const float *T, float *H, float *f_l, float *phi
 {
     const float margin = 5.0f;
     size_t N = NCELL;
+#pragma omp parallel for schedule(static)
     for (size_t c = 0; c < N; ++c) {
         if (flags[c] != SOLID) continue;
         float T_c = T[c];
@@ -1185,6 +1207,7 @@ void or_solidify_trailing_pool_scalar(float *T, float *H, float *f_l, float *phi
                                       float dir_z, float lookback, float T_freeze, float T_s,
                                       int nx, int ny, int nz)
 {
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < ny; ++j)
             for (int k = 0; k < nz; ++k) {
@@ -1397,6 +1420,7 @@ void or_clamp_velocity_mach(float *ux, float *uy, float *uz, const uint8_t *flag
 {
     const float eps = 1e-12f;
     size_t N = NCELL;
+#pragma omp parallel for schedule(static)
     for (size_t c = 0; c < N; ++c) {
         if (flags[c] == SOLID || flags[c] == GAS) continue;
         float vx = ux[c], vy = uy[c], vz = uz[c];

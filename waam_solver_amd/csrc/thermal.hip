@@ -1,5 +1,6 @@
 // thermal.hip — arc heat input, enthalpy advection-diffusion, boundary losses, enthalpy clamps,
 // the enthalpy -> (T, f_l) phase map and the heat-affected-zone diagnostics.
+#include "arc_cell.h"
 #include "thermal_cell.h"
 
 namespace waam {
@@ -11,58 +12,6 @@ namespace waam {
  * the exact sum to within a couple of double ulps, independent of scheduling, rounded once to
  * fp32. Pass 2 recomputes each weight and adds energy_per_weight * w to H.
  * ------------------------------------------------------------------------------------------- */
-struct ArcParams {
-    int kind;  // 0 Gaussian (Gaussian2D / ConicalVolume), 1 Goldak
-    float arc_i, arc_j, arc_k;
-    float inv2s2;                                  // Gaussian
-    float sgn, ff, fr, af, ar, b, cax;             // Goldak
-    float pen;
-    int enable;
-};
-
-/* Returns false for cells the reference skips (w_dep < 1e-6); otherwise the shape weight wg
- * (Gaussian exp or Goldak pdf) and the surface weight wdep, whose product is the pass-1 term and
- * (epw*wg)*wdep the pass-2 increment. */
-__device__ __forceinline__ bool arc_cell_weights(const ArcParams &p, int i, int j, int k, float fl,
-                                                 float &wg, float &wdep)
-{
-    wdep = arc_deposition_weight(k, p.arc_k, fl, p.pen, p.enable);
-    if (wdep < 1e-6f) return false;
-    float di = (float)i - p.arc_i, dj = (float)j - p.arc_j, dk = (float)k - p.arc_k;
-    if (p.kind == 0) {
-        float r2 = di * di + dj * dj + dk * dk * 0.25f;
-        wg = waam_expf(-r2 * p.inv2s2);
-        return true;
-    }
-    const float eps = 1e-6f;
-    float x = di * p.sgn;
-    bool front = x >= 0.0f;
-    float a = front ? p.af : p.ar;
-    float frac = front ? p.ff : p.fr;
-    wg = frac * waam_expf(-3.0f * ((x * x) / (a * a + eps) + (dj * dj) / (p.b * p.b + eps) +
-                                   (dk * dk) / (p.cax * p.cax + eps)));
-    return true;
-}
-
-__device__ __forceinline__ void block_reduce_pair(double &s, double &c, double *sh_s, double *sh_c)
-{
-    int t = threadIdx.x;
-    sh_s[t] = s;
-    sh_c[t] = c;
-    __syncthreads();
-    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
-        if (t < off) {
-            double a = sh_s[t], b = sh_s[t + off], ss, e;
-            two_sum(a, b, ss, e);
-            sh_s[t] = ss;
-            sh_c[t] = sh_c[t] + sh_c[t + off] + e;
-        }
-        __syncthreads();
-    }
-    s = sh_s[0];
-    c = sh_c[0];
-}
-
 __global__ void __launch_bounds__(kBlock) k_arc_partials(const uint8_t *__restrict__ flags,
                                                          const float *__restrict__ f_l,
                                                          ArcParams p, Dims d,

@@ -1,0 +1,216 @@
+"""The fused coupled step: one ``waam_step_fused`` call per timestep.
+
+Same results as ``coupled_step`` (the kernel-per-reference-kernel sequence) — the parity tests
+run both against the oracle — in eight launches and without host round trips on ordinary
+steps (arc_k is found and used on the device). Steps on which a droplet detaches go through
+the standalone arc, deposition and impact kernels first (one host read of arc_k and of the
+placed volume), then the fused kernels with the arc marked as applied.
+
+Buffer pairs: the grid holds H/H_tmp, T/T_tmp, f_l/fl_tmp, flags/flags_tmp and
+sflags/sflags_tmp; after each fused step every pair swaps, as do the population buffers. The
+population stream of the last step is deferred (``pending``); ``flush`` materialises it.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from .. import kernels
+from .._lib import lib
+from ..field import DeviceField, P
+from ..physics import deposition, weld_forces
+from ..physics.electrical_stickout import droplet_entry_temperature_K
+
+_i32, _f32, _vp = ctypes.c_int32, ctypes.c_float, ctypes.c_void_p
+
+
+class StepFields(ctypes.Structure):
+    _fields_ = [(n, _vp) for n in (
+        "f_src", "f_dst", "rho", "ux", "uy", "uz", "H_in", "H_out", "T_in", "T_out", "fl_in",
+        "fl_out", "phi", "phi_tmp", "flags_in", "flags_out", "sflags_prev", "sflags_next", "Fx",
+        "Fy", "Fz", "T_max", "T_prev", "dT_dt", "t800", "t1100", "tsol", "tracer_pos",
+        "tracer_active", "scratch")] + [(n, _i32) for n in ("max_tracers", "nx", "ny", "nz")]
+
+
+_INT_PARAMS = ("is_welding", "collision", "arc_kind", "arc_applied", "pending_stream",
+               "forces_from_fields", "store_forces", "enable_vof", "enable_wetting", "enable_csf",
+               "enable_heat_loss", "enable_conv", "enable_rad", "enable_cap", "enable_hydro",
+               "enable_freeze", "enable_trailing", "zero_velocity", "surface_weight", "nz_solid",
+               "col_i", "col_j", "use_arc_k_override", "use_k_robot")
+_FLOAT_PARAMS = ("arc_i", "arc_j", "arc_k_override", "k_robot", "Q_w", "eta", "dt", "dx", "dx3",
+                 "sigma", "pen", "g_sgn", "g_ff", "g_fr", "g_af", "g_ar", "g_b", "g_c",
+                 "alpha_lu", "cp_rho", "L_rho", "T_s", "T_l", "T_cap", "T_amb", "h_conv",
+                 "eps_rad", "sigma_sb", "T800", "T1100", "gamma_lu", "cos_t", "sin_t", "dgamma",
+                 "p_sigma", "p_pa", "rho_phys", "g_lu", "beta", "F_cap", "u_cap", "omega_s",
+                 "omega_b", "C_darcy", "dir_x", "dir_y", "dir_z", "lookback", "T_freeze")
+
+
+class StepParams(ctypes.Structure):
+    _fields_ = [(n, _i32) for n in _INT_PARAMS] + [(n, _f32) for n in _FLOAT_PARAMS]
+
+
+def _swap(g, a: str, b: str) -> None:
+    fa, fb = getattr(g, a), getattr(g, b)
+    setattr(g, a, fb)
+    setattr(g, b, fa)
+
+
+class FusedStepper:
+    def __init__(self, twin):
+        self.twin = twin
+        g = twin.grid
+        g.ensure_vof_buffers()
+        s3 = (g.nx, g.ny, g.nz)
+        for name in ("T_tmp", "fl_tmp"):
+            if not hasattr(g, name):
+                setattr(g, name, DeviceField.zeros(s3, torch.float32, g.device))
+        if not hasattr(g, "flags_tmp"):
+            g.flags_tmp = DeviceField.zeros(s3, torch.uint8, g.device, np_dtype=np.int32)
+        if not hasattr(g, "sflags"):
+            g.sflags = DeviceField.zeros(s3, torch.uint8, g.device, np_dtype=np.int32)
+            g.sflags_tmp = DeviceField.zeros(s3, torch.uint8, g.device, np_dtype=np.int32)
+        self.pending = False
+        self._arc_out = (ctypes.c_float * 3)()
+
+    # ------------------------------------------------------------------ deferred stream
+    def flush(self) -> None:
+        """Materialise the deferred population stream of the last fused step."""
+        if not self.pending:
+            return
+        g = self.twin.grid
+        lib().call("waam_stream_pending", P(g.f_src), P(g.f_dst), P(g.sflags), g.nx, g.ny, g.nz,
+                   torch.cuda.current_stream().cuda_stream)
+        self.pending = False
+
+    def reset(self) -> None:
+        self.pending = False
+
+    def last_arc_k(self) -> float:
+        torch.cuda.current_stream().synchronize()
+        lib().call("waam_step_arc_readback", P(self.twin.grid.scratch), self._arc_out)
+        return float(self._arc_out[2])
+
+    # ------------------------------------------------------------------ step
+    def _fields(self) -> StepFields:
+        g = self.twin.grid
+        sf = StepFields()
+        for name, fld in (("f_src", g.f_src), ("f_dst", g.f_dst), ("rho", g.rho), ("ux", g.ux),
+                          ("uy", g.uy), ("uz", g.uz), ("H_in", g.H), ("H_out", g.H_tmp),
+                          ("T_in", g.T), ("T_out", g.T_tmp), ("fl_in", g.f_l),
+                          ("fl_out", g.fl_tmp), ("phi", g.phi), ("phi_tmp", g.phi_tmp),
+                          ("flags_in", g.flags), ("flags_out", g.flags_tmp),
+                          ("sflags_prev", g.sflags), ("sflags_next", g.sflags_tmp),
+                          ("Fx", g.Fx), ("Fy", g.Fy), ("Fz", g.Fz), ("T_max", g.T_max),
+                          ("T_prev", g.T_prev), ("dT_dt", g.dT_dt), ("t800", g.time_above_800_s),
+                          ("t1100", g.time_above_1100_s), ("tsol", g.time_above_solidus_s),
+                          ("tracer_pos", g.porosity_pos), ("tracer_active", g.porosity_active),
+                          ("scratch", g.scratch)):
+            setattr(sf, name, P(fld))
+        sf.max_tracers, sf.nx, sf.ny, sf.nz = g.max_tracers, g.nx, g.ny, g.nz
+        return sf
+
+    def _params(self, arc_i, arc_j, is_welding, torch_z_m) -> StepParams:
+        tw, g, m = self.twin, self.twin.grid, self.twin.mat
+        p = StepParams()
+        src = tw.arc_source
+        name = type(src).__name__
+        p.is_welding = 1 if is_welding else 0
+        p.collision = 1 if tw.use_srt else 0
+        p.arc_kind = 1 if name == "Goldak3D" else 0
+        p.store_forces = 1 if tw.snapshot_forces else 0
+        p.enable_vof = 1 if tw.enable_vof else 0
+        p.enable_wetting = 1 if tw.enable_wetting else 0
+        p.enable_csf = 1 if tw.enable_csf_tension else 0
+        p.enable_heat_loss = 1 if tw.enable_heat_loss else 0
+        p.enable_conv = 1 if tw.enable_convection else 0
+        p.enable_rad = 1 if tw.enable_radiation else 0
+        p.enable_cap = 1 if tw.enable_enthalpy_cap else 0
+        p.enable_hydro = 1 if tw.enable_hydrostatic_gravity else 0
+        p.enable_freeze = 1 if (tw.enable_substrate_growth or tw.enable_bead_freeze) else 0
+        p.enable_trailing = 1 if (is_welding and tw.enable_bead_freeze) else 0
+        p.zero_velocity = 1 if (tw.enable_bead_freeze or tw.enable_substrate_growth) else 0
+        p.surface_weight = 1 if tw.arc_surface_weighting else 0
+        p.nz_solid = tw.nz_solid
+        p.col_i = max(0, min(int(arc_i), g.nx - 1))
+        p.col_j = max(0, min(int(arc_j), g.ny - 1))
+        if getattr(tw, "use_torch_z", False) and torch_z_m is not None:
+            z_bead = max(tw.substrate_z_m, torch_z_m - tw.ctwd_m)
+            k_robot = tw.nz_solid + (z_bead - tw.substrate_z_m) / g.dx - 1.0
+            p.k_robot = max(float(tw.nz_solid) + 0.5, min(k_robot, float(g.nz) - 2.0))
+            p.use_k_robot = 1
+        p.arc_i, p.arc_j = arc_i, arc_j
+        # arc source (physics/arc.py: Gaussian2D, ConicalVolume shift arc_k by -1.5 and widen)
+        pen, _ = tw.arc_penetration_m / g.dx, None
+        p.Q_w, p.eta, p.dt, p.dx, p.dx3, p.pen = tw.Q_w, tw.eta, g.dt, g.dx, g.dx ** 3, pen
+        if name == "Goldak3D":
+            a_f, a_r, b, c = src.resolved_axes(tw)
+            p.g_sgn = 1.0 if tw.travel_speed_m_s >= 0.0 else -1.0
+            p.g_ff, p.g_fr, p.g_af, p.g_ar, p.g_b, p.g_c = src.ff, src.fr, a_f, a_r, b, c
+            p.sigma = tw.sigma_cells
+        elif name == "ConicalVolume":
+            raise NotImplementedError("the conical source shifts arc_k on the host; use the "
+                                      "reference engine for it")
+        else:
+            p.sigma = tw.sigma_cells
+        p.alpha_lu, p.cp_rho, p.L_rho = tw.alpha_lu, tw.cp_rho, tw.L_rho
+        p.T_s, p.T_l, p.T_cap, p.T_amb = m.T_solidus, m.T_liquidus, tw.T_vapor_cap_K, tw.T_amb
+        p.h_conv, p.eps_rad, p.sigma_sb = tw.h_conv, tw.eps_rad, tw.sigma_sb
+        p.T800, p.T1100 = 800.0 + 273.15, 1100.0 + 273.15
+        p.gamma_lu = tw.gamma_lu
+        p.cos_t, p.sin_t = kernels.contact_angle_cos_sin(tw.theta_rad)
+        p.dgamma = tw.dgamma_dT_lu
+        p.p_sigma = weld_forces.arc_pressure_sigma_cells(tw)
+        p.p_pa = weld_forces.arc_pressure_peak_pa(tw)
+        p.rho_phys, p.g_lu, p.beta = m.rho, tw.g_lu, tw.beta_T
+        p.F_cap = float(getattr(tw, "force_limit_lu", 0.05))
+        p.u_cap = float(getattr(tw, "u_mach_limit_lu", 0.08))
+        p.omega_s = tw.omega
+        p.omega_b = tw.omega_bulk
+        p.C_darcy = tw.C_darcy
+        dir_x, dir_y, dir_z = getattr(tw, "_torch_dir_xyz", (1.0, 0.0, 0.0))
+        p.dir_x, p.dir_y, p.dir_z = dir_x, dir_y, dir_z
+        p.lookback = max(2.0, tw.trailing_solidify_lookback_mm / (g.dx * 1000.0))
+        p.T_freeze = m.T_liquidus + tw.trailing_solidify_temp_margin_K
+        return p
+
+    def step(self, torch_x_m: float, torch_y_m: float, is_welding: bool,
+             torch_z_m: float | None = None) -> None:
+        from .coupled_step import _clamp_enthalpy_ceiling, droplet_due, do_droplet, resolve_arc_k
+
+        tw, g = self.twin, self.twin.grid
+        if tw.use_material_tables:
+            raise NotImplementedError("temperature-dependent material tables are not built")
+        if tw.enable_gas_shear or tw.enable_recoil or tw.enable_lorentz:
+            raise NotImplementedError("physics_tier: full forces are not built")
+        arc_i = torch_x_m / g.dx
+        arc_j = torch_y_m / g.dx
+        p = self._params(arc_i, arc_j, is_welding, torch_z_m)
+        if is_welding and droplet_due(tw, g):
+            # the deferred stream reads f_src, which deposition is about to change
+            self.flush()
+            arc_k = resolve_arc_k(tw, g, arc_i, arc_j, torch_z_m)
+            kernels.clear_forces(g.Fx, g.Fy, g.Fz)
+            tw.arc_source.inject(tw, g, arc_i, arc_j, arc_k)
+            _clamp_enthalpy_ceiling(tw, g)
+            do_droplet(tw, g, torch_x_m, torch_y_m, arc_i, arc_j, arc_k)
+            p.arc_applied = 1
+            p.use_arc_k_override = 1
+            p.arc_k_override = arc_k
+            p.forces_from_fields = 1
+            p.pending_stream = 0
+            tw._last_arc_ijk = (float(arc_i), float(arc_j), float(arc_k))
+        else:
+            p.pending_stream = 1 if self.pending else 0
+            tw._last_arc_ijk = (float(arc_i), float(arc_j), None)
+        lib().call("waam_step_fused", ctypes.byref(p), ctypes.byref(self._fields()),
+                   torch.cuda.current_stream().cuda_stream)
+        for a, b in (("H", "H_tmp"), ("T", "T_tmp"), ("f_l", "fl_tmp"), ("flags", "flags_tmp"),
+                     ("sflags", "sflags_tmp")):
+            _swap(g, a, b)
+        g.swap_buffers()
+        self.pending = True
+        tw._step_n += 1
