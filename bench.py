@@ -109,6 +109,46 @@ def cpu_baseline(w: dict, seconds: float) -> dict:
                       f"(steps 2..{steps + 1}) on the OpenMP build of the oracle, {el:.1f} s"}
 
 
+# Algorithmic bytes of one k_step_collide launch, per cell class (each field touched once; halo
+# re-reads, the force snapshot and the F zeroing of inactive cells not counted):
+#   every cell          flags read + stream-flags write                         2 B
+#   collided cell       19 populations read + written (152), rho+u written (16),
+#                       phi, T, f_l read (12)                                  180 B
+#   pull-only cell      deferred stream: 19 populations read + written (152),
+#                       previous stream flags read (1)                         153 B
+COLLIDE_B_EVERY, COLLIDE_B_ACTIVE, COLLIDE_B_PULL = 2, 180, 153
+# Whole-step compulsory bytes per cell (fp32 populations, with the HAZ fields)
+STEP_B_PER_CELL = 258
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_step_collide_bead_on_plate_256x128x64_pmc.md")
+
+
+def collide_classes(g) -> tuple[int, int]:
+    """(collided cells, pull-only cells) of the last fused step, from its stream flags."""
+    import numpy as np
+
+    cur = g.sflags.to_numpy()
+    prev = g.sflags_tmp.to_numpy()
+    active = (cur == 0) | (cur == 4)
+    pull = (~active) & (prev != 1)
+    return int(active.sum()), int(pull.sum())
+
+
+def collide_bytes(ncell: int, active: float, pull: float) -> float:
+    return COLLIDE_B_EVERY * ncell + COLLIDE_B_ACTIVE * active + COLLIDE_B_PULL * pull
+
+
+def measured_traffic() -> float | None:
+    """HBM bytes per k_step_collide launch from the committed rocprofv3 counter summary."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +156,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--engine", choices=("fused", "reference"), default="fused")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (each box here has "
@@ -125,18 +166,46 @@ def main() -> int:
 
     import torch
 
+    import ctypes
+
+    from waam_solver_amd._lib import lib
+
     w = dict(WORKLOAD)
     tw = build_gpu_twin(w)
+    tw.engine = args.engine
     g = tw.grid
     ncell = g.nx * g.ny * g.nz
     for s in range(args.warmup):
         tw.step(*torch_xy(w, s, g.dt), True)
     torch.cuda.synchronize()
+    fused = args.engine == "fused"
+    if fused:
+        cls0 = collide_classes(g) if hasattr(g, "sflags_tmp") else None
+        lib().call("waam_timing_begin", 5, args.steps)  # WAAM_K_COLLIDE
     t0 = time.perf_counter()
     for s in range(args.warmup, args.warmup + args.steps):
         tw.step(*torch_xy(w, s, g.dt), True)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    roofline = None
+    if fused:
+        tot_ms, n_launch = ctypes.c_double(), ctypes.c_int64()
+        lib().call("waam_timing_end", ctypes.addressof(tot_ms), ctypes.addressof(n_launch))
+        cls1 = collide_classes(g)
+        cls0 = cls0 or cls1
+        active, pull = (cls0[0] + cls1[0]) / 2.0, (cls0[1] + cls1[1]) / 2.0
+        per_launch = collide_bytes(ncell, active, pull)
+        avg_s = tot_ms.value / 1e3 / max(n_launch.value, 1)
+        achieved = per_launch / avg_s / 1e9
+        roofline = {"bound": "hbm", "kernel": "k_step_collide", "achieved": round(achieved, 1),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": measured_traffic(),
+                    "algorithmic_bytes_per_launch": round(per_launch),
+                    "avg_launch_us": round(avg_s * 1e6, 2), "timed_launches": n_launch.value,
+                    "cells_collided": round(active), "cells_pull_only": round(pull),
+                    "step_bytes_per_cell": STEP_B_PER_CELL,
+                    "step_achieved": round(STEP_B_PER_CELL * ncell * args.steps / el / 1e9, 1)}
+        roofline["step_frac"] = round(roofline["step_achieved"] / HBM_PEAK_GBS, 4)
     mlups = ncell * args.steps / el / 1e6
     tel = tw.get_telemetry()
     result = {
@@ -155,9 +224,12 @@ def main() -> int:
         "config": {"workload": "bead-on-plate 256x128x64 dx=0.2mm, Gaussian2D 4kW eta0.8 "
                                "sigma2mm +x 8mm/s, wire feed 6.5m/min, flow tier, MRT",
                    "grid": [g.nx, g.ny, g.nz], "cells": ncell, "collision": "mrt",
+                   "engine": args.engine,
                    "pool_width_mm": tel["pool_width_mm"], "pool_depth_mm": tel["pool_depth_mm"],
                    "n_droplets": tel["n_droplets_fired"]},
     }
+    if roofline is not None:
+        result["roofline"] = roofline
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
     print(json.dumps(result))

@@ -278,9 +278,29 @@ int waam_step_fused(const waam_step_params *p, const waam_step_fields *f, void *
  * step that deferred it saw). */
 int waam_stream_pending(const float *f_src, float *f_dst, const uint8_t *sflags, int nx, int ny,
                         int nz, void *stream);
-/* The step's arc column height and weight total, as written into the workspace by the last
- * waam_step_fused (device floats: arc_k, w_total, energy per weight). */
+/* The step's arc weight total, energy per unit weight and arc column height, as written into
+ * the workspace by the last waam_step_fused (three device floats, in that order). Blocking. */
 int waam_step_arc_readback(const void *scratch, float *host_out3);
+
+/* ---- instrumentation (no counterpart in the reference) -------------------------------------
+ * HIP-event timing of one kernel of waam_step_fused, recorded on the stream that kernel is
+ * launched on: waam_timing_begin arms it for up to max_launches launches of kernel `which`
+ * (one of WAAM_K_*); waam_timing_end waits for the recorded events, returns the summed
+ * duration and the number of timed launches, and disarms. Process-wide, one kernel at a time;
+ * not for concurrent use from several threads. */
+enum {
+    WAAM_K_ARC_PARTIALS = 0,
+    WAAM_K_ARC_TOTAL = 1,
+    WAAM_K_THERMAL = 2,
+    WAAM_K_VOF_ADVECT = 3,
+    WAAM_K_VOF_FLAGS = 4,
+    WAAM_K_COLLIDE = 5,
+    WAAM_K_TRACERS = 6,
+    WAAM_K_FREEZE = 7,
+    WAAM_K_COUNT = 8
+};
+int waam_timing_begin(int which, int max_launches);
+int waam_timing_end(double *total_ms, int64_t *launches);
 
 #ifdef __cplusplus
 }

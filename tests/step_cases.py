@@ -53,7 +53,7 @@ def torch_x(twin_dx: float, nx: int, step: int, dt: float) -> float:
     return (nx // 3) * twin_dx + 8e-3 * step * dt
 
 
-def build_pair(name: str, mode: str = "jacobi"):
+def build_pair(name: str, mode: str = "jacobi", engine: str = "fused"):
     """(product twin, oracle twin), both reset and seeded identically."""
     from oracle.oracle import Material, OracleTwin
     from waam_solver_amd.materials import material_from_dict
@@ -64,6 +64,7 @@ def build_pair(name: str, mode: str = "jacobi"):
     prod_mat = material_from_dict({"name": mat_d["name"], "status": "synthetic",
                                    "constants": {k: v for k, v in mat_d.items() if k != "name"}})
     tw = WAAMTwin(material=prod_mat, **kw)
+    tw.engine = engine
     ot = OracleTwin(Material(**mat_d), mode=mode, **kw)
     for t in (tw, ot):
         configure(t, name)
@@ -97,6 +98,7 @@ def compare(tw, ot, fields=FIELDS) -> dict:
     from tests.gpu_common import bits_equal, rel_l2
 
     out = {}
+    tw.sync()
     g, og = tw.grid, ot.g
     for f in fields:
         a = getattr(g, f).to_numpy()

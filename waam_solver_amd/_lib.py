@@ -66,7 +66,16 @@ SIGNATURES: dict[str, str] = {
     "waam_inject_tracers": "ppp i ffff i u p",
     "waam_advect_tracers": "ppppppp ff i iii p",
     "waam_telemetry_pool_reduce": "pppppppp iii pp",
+    "waam_step_fused": "ppp",
+    "waam_stream_pending": "ppp iii p",
+    "waam_step_arc_readback": "pp",
+    "waam_timing_begin": "ii",
+    "waam_timing_end": "pp",
 }
+
+# kernel ids of waam_step_fused for waam_timing_begin (include/waam_hip.h WAAM_K_*)
+STEP_KERNELS = ("k_step_arc_partials", "k_step_arc_total", "k_step_thermal", "k_step_vof_advect",
+                "k_step_vof_flags", "k_step_collide", "k_advect_tracers_step", "k_step_freeze")
 
 
 class WaamError(RuntimeError):

@@ -4,10 +4,10 @@
 //   1 k_step_arc_partials   torch-column surface height (arc_k) + arc weight partial sums
 //   2 k_step_arc_total      fixed-order weight total, energy per unit weight
 //   3 k_step_thermal        arc + enthalpy ceiling applied on the fly, advection-diffusion,
-//                           boundary losses, floor, ceiling, phase map, T_max, cooling rate,
-//                           time above thresholds
+//                           boundary losses, floor, ceiling, phase map, T_max, cooling rate
 //   4 k_step_vof_advect     donor-cell phi advection + re-initialisation -> phi_tmp
-//   5 k_step_vof_flags      contact-angle BC, flag classification, phi/f_l consistency
+//   5 k_step_vof_flags      contact-angle BC, flag classification, phi/f_l consistency, time
+//                           above the HAZ thresholds
 //   6 k_step_collide        CSF + Marangoni + arc pressure + gravity + buoyancy, force clamp,
 //                           MRT/SRT collision, Mach clamp, and the previous step's pull stream
 //   7 k_advect_tracers      porosity tracers
@@ -23,6 +23,7 @@
 // and written once per active cell per step instead of twice.
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "arc_cell.h"
 #include "forces_cell.h"
@@ -165,7 +166,7 @@ struct HView {
 };
 
 /* 3 — thermal chain (reference kernels.py:497-510, 1953-1984, 1611-1682, 2030-2082,
- *     1937-1950, 1519-1556, 3016-3032, 2013-2027, 3261-3283 in coupled_step order) */
+ *     1937-1950, 1519-1556, 3016-3032, 2013-2027 in coupled_step order) */
 __global__ void __launch_bounds__(256) k_step_thermal(waam_step_fields f, StepConsts s)
 {
     const Dims &d = s.d;
@@ -237,10 +238,19 @@ __global__ void __launch_bounds__(256) k_step_thermal(waam_step_fields f, StepCo
     f.H_out[c] = h;
     f.T_out[c] = T;
     f.fl_out[c] = fl;
-    /* HAZ diagnostics (reference kernels.py:3027-3032, 2023-2027, 3274-3283) */
+    /* HAZ diagnostics (reference kernels.py:3027-3032, 2023-2027); they run before the VOF
+     * flag update, so they see this step's incoming flags */
     if (T > f.T_max[c]) f.T_max[c] = T;
     f.dT_dt[c] = (T - f.T_prev[c]) * s.inv_dt;
     f.T_prev[c] = T;
+}
+
+/* update_time_above_T (reference kernels.py:3261-3283) runs after the VOF flag update and
+ * skips cells that are GAS by the updated flags */
+__device__ __forceinline__ void time_above(const waam_step_fields &f, const waam_step_params &p,
+                                           size_t c)
+{
+    float T = f.T_out[c];
     if (T >= p.T800) f.t800[c] += p.dt;
     if (T >= p.T1100) f.t1100[c] += p.dt;
     if (T >= p.T_s) f.tsol[c] += p.dt;
@@ -258,8 +268,8 @@ __global__ void __launch_bounds__(256) k_step_vof_advect(waam_step_fields f, Dim
     f.phi_tmp[c] = reinit_phi(p, f.flags_in[c]);
 }
 
-/* 5 — apply_contact_angle_phi_bc (Jacobi), update_flags_from_phi, sync_phi_liquid_fraction
- *     (reference kernels.py:933-991, 2394-2419, 3484-3504) */
+/* 5 — apply_contact_angle_phi_bc (Jacobi), update_flags_from_phi, sync_phi_liquid_fraction,
+ *     update_time_above_T (reference kernels.py:933-991, 2394-2419, 3484-3504, 3261-3283) */
 __global__ void __launch_bounds__(256) k_step_vof_flags(waam_step_fields f, StepConsts s)
 {
     const Dims &d = s.d;
@@ -269,7 +279,9 @@ __global__ void __launch_bounds__(256) k_step_vof_flags(waam_step_fields f, Step
     if (k >= d.nz || j >= d.ny) return;
     size_t c = d.idx(i, j, k);
     if (!s.p.enable_vof) {
-        f.flags_out[c] = f.flags_in[c];
+        uint8_t fi = f.flags_in[c];
+        f.flags_out[c] = fi;
+        if (fi != GAS) time_above(f, s.p, c);
         return;
     }
     float p = s.p.enable_wetting
@@ -280,6 +292,7 @@ __global__ void __launch_bounds__(256) k_step_vof_flags(waam_step_fields f, Step
     f.flags_out[c] = nf;
     if (nf == GAS || p < 0.05f) f.fl_out[c] = 0.0f;
     else if (nf == SOLID) f.fl_out[c] = 0.0f;
+    if (nf != GAS) time_above(f, s.p, c);
 }
 
 /* pull of the reference stream (kernels.py:2991-3010) with the flags of the step it belongs to */
@@ -530,6 +543,27 @@ static void arc_box(StepConsts &s)
     }
 }
 
+/* HIP-event timing of one kernel kind (waam_timing_begin / waam_timing_end) */
+struct KernelTimer {
+    bool armed = false;
+    int which = -1, used = 0;
+    std::vector<hipEvent_t> ev;  // start/stop pairs
+};
+static KernelTimer g_timer;
+
+template <class Launch>
+static void timed_launch(int which, hipStream_t st, Launch &&launch)
+{
+    KernelTimer &t = g_timer;
+    bool rec = t.armed && t.which == which && 2 * t.used < (int)t.ev.size();
+    if (rec) hipEventRecord(t.ev[2 * t.used], st);
+    launch();
+    if (rec) {
+        hipEventRecord(t.ev[2 * t.used + 1], st);
+        ++t.used;
+    }
+}
+
 }  // namespace waam
 
 using namespace waam;
@@ -587,18 +621,34 @@ int waam_step_fused(const waam_step_params *pp, const waam_step_fields *ff, void
     const Dims &d = s.d;
     bool arc_sums = p.is_welding && !p.arc_applied;
     if (!p.use_arc_k_override || arc_sums)
-        hipLaunchKernelGGL(k_step_arc_partials, dim3(arc_sums ? kReduceBlocks : 1), dim3(kBlock), 0,
-                           st, f, s, arc_sums ? 1 : 0);
-    if (arc_sums) hipLaunchKernelGGL(k_step_arc_total, dim3(1), dim3(kBlock), 0, st, f, s);
-    hipLaunchKernelGGL(k_step_thermal, grid3(d), block3(), 0, st, f, s);
-    if (p.enable_vof) hipLaunchKernelGGL(k_step_vof_advect, grid3(d), block3(), 0, st, f, d);
-    hipLaunchKernelGGL(k_step_vof_flags, grid3(d), block3(), 0, st, f, s);
-    hipLaunchKernelGGL(k_step_collide, grid3(d), block3(), 0, st, f, s);
+        timed_launch(WAAM_K_ARC_PARTIALS, st, [&] {
+            hipLaunchKernelGGL(k_step_arc_partials, dim3(arc_sums ? kReduceBlocks : 1),
+                               dim3(kBlock), 0, st, f, s, arc_sums ? 1 : 0);
+        });
+    if (arc_sums)
+        timed_launch(WAAM_K_ARC_TOTAL, st, [&] {
+            hipLaunchKernelGGL(k_step_arc_total, dim3(1), dim3(kBlock), 0, st, f, s);
+        });
+    timed_launch(WAAM_K_THERMAL, st,
+                 [&] { hipLaunchKernelGGL(k_step_thermal, grid3(d), block3(), 0, st, f, s); });
+    if (p.enable_vof)
+        timed_launch(WAAM_K_VOF_ADVECT, st, [&] {
+            hipLaunchKernelGGL(k_step_vof_advect, grid3(d), block3(), 0, st, f, d);
+        });
+    timed_launch(WAAM_K_VOF_FLAGS, st,
+                 [&] { hipLaunchKernelGGL(k_step_vof_flags, grid3(d), block3(), 0, st, f, s); });
+    timed_launch(WAAM_K_COLLIDE, st,
+                 [&] { hipLaunchKernelGGL(k_step_collide, grid3(d), block3(), 0, st, f, s); });
     if (f.max_tracers > 0)
-        hipLaunchKernelGGL(k_advect_tracers_step, dim3(grid_for((size_t)f.max_tracers)), dim3(kBlock),
-                           0, st, f.tracer_pos, f.tracer_active, f.ux, f.uy, f.uz, f.fl_out,
-                           f.flags_out, p.dx, f.max_tracers, d);
-    if (p.enable_freeze) hipLaunchKernelGGL(k_step_freeze, grid3(d), block3(), 0, st, f, s);
+        timed_launch(WAAM_K_TRACERS, st, [&] {
+            hipLaunchKernelGGL(k_advect_tracers_step, dim3(grid_for((size_t)f.max_tracers)),
+                               dim3(kBlock), 0, st, f.tracer_pos, f.tracer_active, f.ux, f.uy,
+                               f.uz, f.fl_out, f.flags_out, p.dx, f.max_tracers, d);
+        });
+    if (p.enable_freeze)
+        timed_launch(WAAM_K_FREEZE, st, [&] {
+            hipLaunchKernelGGL(k_step_freeze, grid3(d), block3(), 0, st, f, s);
+        });
     return check_launch("waam_step_fused");
 }
 
@@ -621,6 +671,55 @@ int waam_step_arc_readback(const void *scratch, float *host_out3)
                              hipMemcpyDeviceToHost);
     if (e != hipSuccess) return set_error(WAAM_ERR_HIP, "waam_step_arc_readback: %s", hipGetErrorString(e));
     return WAAM_OK;
+}
+
+int waam_timing_begin(int which, int max_launches)
+{
+    KernelTimer &t = g_timer;
+    if (t.armed) return set_error(WAAM_ERR_INVALID, "waam_timing_begin: timing already armed");
+    if (which < 0 || which >= WAAM_K_COUNT)
+        return set_error(WAAM_ERR_INVALID, "waam_timing_begin: kernel id %d out of range", which);
+    if (max_launches <= 0 || max_launches > (1 << 20))
+        return set_error(WAAM_ERR_INVALID, "waam_timing_begin: max_launches %d out of range",
+                         max_launches);
+    t.ev.assign((size_t)2 * max_launches, nullptr);
+    for (auto &e : t.ev) {
+        hipError_t err = hipEventCreate(&e);
+        if (err != hipSuccess) {
+            for (auto &x : t.ev)
+                if (x) hipEventDestroy(x);
+            t.ev.clear();
+            return set_error(WAAM_ERR_HIP, "waam_timing_begin: %s", hipGetErrorString(err));
+        }
+    }
+    t.which = which;
+    t.used = 0;
+    t.armed = true;
+    return WAAM_OK;
+}
+
+int waam_timing_end(double *total_ms, int64_t *launches)
+{
+    KernelTimer &t = g_timer;
+    if (!t.armed) return set_error(WAAM_ERR_INVALID, "waam_timing_end: timing not armed");
+    double sum = 0.0;
+    int rc = WAAM_OK;
+    for (int i = 0; i < t.used; ++i) {
+        float ms = 0.0f;
+        hipError_t e = hipEventSynchronize(t.ev[2 * i + 1]);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, t.ev[2 * i], t.ev[2 * i + 1]);
+        if (e != hipSuccess) {
+            rc = set_error(WAAM_ERR_HIP, "waam_timing_end: %s", hipGetErrorString(e));
+            break;
+        }
+        sum += ms;
+    }
+    for (auto &e : t.ev) hipEventDestroy(e);
+    t.ev.clear();
+    t.armed = false;
+    if (total_ms) *total_ms = sum;
+    if (launches) *launches = t.used;
+    return rc;
 }
 
 }  // extern "C"

@@ -58,14 +58,19 @@ def resolve_arc_k(twin, g, arc_i: float, arc_j: float, torch_z_m: float | None =
     return k_surface
 
 
+def droplet_due(twin, g) -> bool:
+    """Whether a droplet detaches on this step (reference coupled_step.py:110-117)."""
+    if twin.droplet_freq <= 0:
+        return False
+    period = deposition.droplet_period_s(twin)
+    return not (twin._step_n * g.dt - twin._last_droplet_time < period)
+
+
 def _droplet(twin, g, x_m: float, y_m: float, arc_i: float, arc_j: float, arc_k: float) -> None:
     """Droplet block (reference coupled_step.py:110-166)."""
+    if not droplet_due(twin, g):
+        return
     sim_time = twin._step_n * g.dt
-    if twin.droplet_freq <= 0:
-        return
-    period = deposition.droplet_period_s(twin)
-    if sim_time - twin._last_droplet_time < period:
-        return
     drop_dt = sim_time - twin._last_droplet_time
     twin._last_droplet_time = sim_time
     drop_mass = deposition.droplet_mass_for_interval_kg(twin, drop_dt)

@@ -14,16 +14,14 @@ population stream of the last step is deferred (``pending``); ``flush`` material
 from __future__ import annotations
 
 import ctypes
-import math
-
 import numpy as np
 import torch
 
 from .. import kernels
 from .._lib import lib
 from ..field import DeviceField, P
-from ..physics import deposition, weld_forces
-from ..physics.electrical_stickout import droplet_entry_temperature_K
+from ..physics import weld_forces
+from .coupled_step import _clamp_enthalpy_ceiling, _droplet, droplet_due, resolve_arc_k
 
 _i32, _f32, _vp = ctypes.c_int32, ctypes.c_float, ctypes.c_void_p
 
@@ -91,7 +89,8 @@ class FusedStepper:
 
     def last_arc_k(self) -> float:
         torch.cuda.current_stream().synchronize()
-        lib().call("waam_step_arc_readback", P(self.twin.grid.scratch), self._arc_out)
+        lib().call("waam_step_arc_readback", P(self.twin.grid.scratch),
+                   ctypes.addressof(self._arc_out))
         return float(self._arc_out[2])
 
     # ------------------------------------------------------------------ step
@@ -143,19 +142,13 @@ class FusedStepper:
             p.k_robot = max(float(tw.nz_solid) + 0.5, min(k_robot, float(g.nz) - 2.0))
             p.use_k_robot = 1
         p.arc_i, p.arc_j = arc_i, arc_j
-        # arc source (physics/arc.py: Gaussian2D, ConicalVolume shift arc_k by -1.5 and widen)
-        pen, _ = tw.arc_penetration_m / g.dx, None
-        p.Q_w, p.eta, p.dt, p.dx, p.dx3, p.pen = tw.Q_w, tw.eta, g.dt, g.dx, g.dx ** 3, pen
+        p.Q_w, p.eta, p.dt, p.dx, p.dx3 = tw.Q_w, tw.eta, g.dt, g.dx, g.dx ** 3
+        p.pen = tw.arc_penetration_m / g.dx
+        p.sigma = tw.sigma_cells
         if name == "Goldak3D":
             a_f, a_r, b, c = src.resolved_axes(tw)
             p.g_sgn = 1.0 if tw.travel_speed_m_s >= 0.0 else -1.0
             p.g_ff, p.g_fr, p.g_af, p.g_ar, p.g_b, p.g_c = src.ff, src.fr, a_f, a_r, b, c
-            p.sigma = tw.sigma_cells
-        elif name == "ConicalVolume":
-            raise NotImplementedError("the conical source shifts arc_k on the host; use the "
-                                      "reference engine for it")
-        else:
-            p.sigma = tw.sigma_cells
         p.alpha_lu, p.cp_rho, p.L_rho = tw.alpha_lu, tw.cp_rho, tw.L_rho
         p.T_s, p.T_l, p.T_cap, p.T_amb = m.T_solidus, m.T_liquidus, tw.T_vapor_cap_K, tw.T_amb
         p.h_conv, p.eps_rad, p.sigma_sb = tw.h_conv, tw.eps_rad, tw.sigma_sb
@@ -179,8 +172,6 @@ class FusedStepper:
 
     def step(self, torch_x_m: float, torch_y_m: float, is_welding: bool,
              torch_z_m: float | None = None) -> None:
-        from .coupled_step import _clamp_enthalpy_ceiling, droplet_due, do_droplet, resolve_arc_k
-
         tw, g = self.twin, self.twin.grid
         if tw.use_material_tables:
             raise NotImplementedError("temperature-dependent material tables are not built")
@@ -189,14 +180,16 @@ class FusedStepper:
         arc_i = torch_x_m / g.dx
         arc_j = torch_y_m / g.dx
         p = self._params(arc_i, arc_j, is_welding, torch_z_m)
-        if is_welding and droplet_due(tw, g):
-            # the deferred stream reads f_src, which deposition is about to change
+        # Eager arc: droplet steps (deposition edits the fields the fused kernels read, and the
+        # deferred stream reads f_src, which deposition changes) and the conical source (whose
+        # centre sits 1.5 cells below the resolved arc_k).
+        if is_welding and (droplet_due(tw, g) or type(tw.arc_source).__name__ == "ConicalVolume"):
             self.flush()
             arc_k = resolve_arc_k(tw, g, arc_i, arc_j, torch_z_m)
             kernels.clear_forces(g.Fx, g.Fy, g.Fz)
             tw.arc_source.inject(tw, g, arc_i, arc_j, arc_k)
             _clamp_enthalpy_ceiling(tw, g)
-            do_droplet(tw, g, torch_x_m, torch_y_m, arc_i, arc_j, arc_k)
+            _droplet(tw, g, torch_x_m, torch_y_m, arc_i, arc_j, arc_k)
             p.arc_applied = 1
             p.use_arc_k_override = 1
             p.arc_k_override = arc_k
@@ -206,7 +199,8 @@ class FusedStepper:
         else:
             p.pending_stream = 1 if self.pending else 0
             tw._last_arc_ijk = (float(arc_i), float(arc_j), None)
-        lib().call("waam_step_fused", ctypes.byref(p), ctypes.byref(self._fields()),
+        fields = self._fields()
+        lib().call("waam_step_fused", ctypes.addressof(p), ctypes.addressof(fields),
                    torch.cuda.current_stream().cuda_stream)
         for a, b in (("H", "H_tmp"), ("T", "T_tmp"), ("f_l", "fl_tmp"), ("flags", "flags_tmp"),
                      ("sflags", "sflags_tmp")):

@@ -26,6 +26,7 @@ from .physics.deposition_balance import (droplet_mass_kg, expected_deposited_mas
 from .physics.electrical_stickout import droplet_entry_temperature_K
 from .physics.weld_forces import arc_pressure_peak_pa, droplet_impact_velocity_m_s
 from .solvers.coupled_step import coupled_step
+from .solvers.fused import FusedStepper
 
 log = logging.getLogger("waam_solver_amd")
 
@@ -149,6 +150,10 @@ class WAAMTwin:
         self._interpass_cooling_steps = 0
         self.snapshot_forces = True
         self.tracer_seed = 0
+        # "fused": waam_step_fused (eight launches per step, the default); "reference": one
+        # launch per reference kernel in the reference's order. Both give identical fields.
+        self.engine = "fused"
+        self._fused: FusedStepper | None = None
         self._reset_counters()
 
         g = self.grid
@@ -305,6 +310,8 @@ class WAAMTwin:
                                 g.time_above_1100_s, g.time_above_solidus_s, g.Fx_snap,
                                 g.Fy_snap, g.Fz_snap, g.Fx, g.Fy, g.Fz, T0, g.porosity_active,
                                 g.tracer_head, g.max_tracers)
+        if self._fused is not None:
+            self._fused.reset()
         kernels.stream(g.f_a, g.f_b, g.flags, g.FLAG_SOLID, g.FLAG_GAS, g.nx, g.ny, g.nz)
         self._reset_counters()
         g.deposit_vol_buf.fill(0.0)
@@ -326,8 +333,25 @@ class WAAMTwin:
             log.warning("[WAAMTwin] welding step with wire_feed_m_s == 0: no metal is deposited")
         sim_x = torch_x_m - self._window_offset_x_m - self._sim_origin_offset_x_m
         sim_y = torch_y_m - self._sim_origin_offset_y_m
-        coupled_step(self, sim_x, sim_y, is_welding, torch_z_m)
+        if self.engine == "fused":
+            if self._fused is None:
+                self._fused = FusedStepper(self)
+            self._fused.step(sim_x, sim_y, is_welding, torch_z_m)
+        elif self.engine == "reference":
+            self.sync()
+            coupled_step(self, sim_x, sim_y, is_welding, torch_z_m)
+        else:
+            raise ValueError(f"unknown engine {self.engine!r} (fused | reference)")
         self._check_strict_mode()
+
+    def sync(self) -> None:
+        """Materialise deferred device work (the fused engine's last population stream).
+
+        Fields other than the two population buffers are always current; call this before
+        reading ``grid.f_src`` / ``grid.f_dst`` on the host or handing them to other kernels.
+        """
+        if self._fused is not None:
+            self._fused.flush()
 
     def _check_strict_mode(self) -> None:
         if not self.strict_mode or self._n_droplets_fired < 20:
