@@ -109,23 +109,22 @@ def cpu_baseline(w: dict, seconds: float) -> dict:
                       f"(steps 2..{steps + 1}) on the OpenMP build of the oracle, {el:.1f} s"}
 
 
-# Algorithmic bytes of one k_step_collide launch, per cell class (each field touched once; halo
-# re-reads, the force snapshot and the F zeroing of inactive cells not counted):
-#   every cell          flags read + stream-flags write                         2 B
-#   collided cell       19 populations read + written (152), rho+u written (16),
-#                       phi, T, f_l read (12)                                  180 B
-#   pull-only cell      deferred stream: 19 populations read + written (152),
-#                       previous stream flags read (1)                         153 B
-COLLIDE_B_EVERY, COLLIDE_B_ACTIVE, COLLIDE_B_PULL = 2, 180, 153
+# The dominant kernel is k_step_stream, the deferred pull stream of the populations. Its
+# algorithmic bytes per launch (each array element touched once; neighbour re-reads and the
+# force reset of inactive cells not counted):
+#   every cell        flags read, previous stream flags read, stream flags written   3 B
+#   pulled cell       19 populations read + 19 written                              152 B
+# A pulled cell is one the collision skips this step that was not SOLID at the step whose
+# stream is being completed.
+STREAM_B_EVERY, STREAM_B_PULL = 3, 152
 # Whole-step compulsory bytes per cell (fp32 populations, with the HAZ fields)
 STEP_B_PER_CELL = 258
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_step_collide_bead_on_plate_256x128x64_pmc.md")
+DOMINANT = "k_step_stream"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_step_stream_bead_on_plate_256x128x64_pmc.md")
 
 
-def collide_classes(g) -> tuple[int, int]:
-    """(collided cells, pull-only cells) of the last fused step, from its stream flags."""
-    import numpy as np
-
+def stream_classes(g) -> tuple[int, int]:
+    """(collided cells, pulled cells) of the last fused step, from its stream flags."""
     cur = g.sflags.to_numpy()
     prev = g.sflags_tmp.to_numpy()
     active = (cur == 0) | (cur == 4)
@@ -133,12 +132,35 @@ def collide_classes(g) -> tuple[int, int]:
     return int(active.sum()), int(pull.sum())
 
 
-def collide_bytes(ncell: int, active: float, pull: float) -> float:
-    return COLLIDE_B_EVERY * ncell + COLLIDE_B_ACTIVE * active + COLLIDE_B_PULL * pull
+def stream_bytes(ncell: int, pull: float) -> float:
+    return STREAM_B_EVERY * ncell + STREAM_B_PULL * pull
+
+
+def kernel_breakdown(tw, w: dict, start: int, steps: int) -> dict:
+    """Average HIP-event duration (us) of each fused-step kernel over `steps` further steps each
+    (run after the timed region)."""
+    import ctypes
+
+    import torch
+
+    from waam_solver_amd._lib import STEP_KERNELS, lib
+
+    out, s = {}, start
+    for kid, name in enumerate(STEP_KERNELS):
+        lib().call("waam_timing_begin", kid, steps)
+        for _ in range(steps):
+            tw.step(*torch_xy(w, s, tw.grid.dt), True)
+            s += 1
+        torch.cuda.synchronize()
+        tot, n = ctypes.c_double(), ctypes.c_int64()
+        lib().call("waam_timing_end", ctypes.addressof(tot), ctypes.addressof(n))
+        if n.value:
+            out[name] = round(tot.value * 1e3 / n.value, 2)
+    return out
 
 
 def measured_traffic() -> float | None:
-    """HBM bytes per k_step_collide launch from the committed rocprofv3 counter summary."""
+    """HBM bytes per k_step_stream launch from the committed rocprofv3 counter summary."""
     try:
         with open(TRAFFIC_FILE) as fh:
             for line in fh:
@@ -157,6 +179,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--engine", choices=("fused", "reference"), default="fused")
+    ap.add_argument("--breakdown", type=int, default=50,
+                    help="steps per kernel for the per-kernel HIP-event breakdown (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (each box here has "
@@ -168,7 +192,7 @@ def main() -> int:
 
     import ctypes
 
-    from waam_solver_amd._lib import lib
+    from waam_solver_amd._lib import STEP_KERNELS, lib
 
     w = dict(WORKLOAD)
     tw = build_gpu_twin(w)
@@ -180,8 +204,8 @@ def main() -> int:
     torch.cuda.synchronize()
     fused = args.engine == "fused"
     if fused:
-        cls0 = collide_classes(g) if hasattr(g, "sflags_tmp") else None
-        lib().call("waam_timing_begin", 5, args.steps)  # WAAM_K_COLLIDE
+        cls0 = stream_classes(g) if hasattr(g, "sflags_tmp") else None
+        lib().call("waam_timing_begin", STEP_KERNELS.index(DOMINANT), args.steps)
     t0 = time.perf_counter()
     for s in range(args.warmup, args.warmup + args.steps):
         tw.step(*torch_xy(w, s, g.dt), True)
@@ -191,21 +215,24 @@ def main() -> int:
     if fused:
         tot_ms, n_launch = ctypes.c_double(), ctypes.c_int64()
         lib().call("waam_timing_end", ctypes.addressof(tot_ms), ctypes.addressof(n_launch))
-        cls1 = collide_classes(g)
+        cls1 = stream_classes(g)
         cls0 = cls0 or cls1
         active, pull = (cls0[0] + cls1[0]) / 2.0, (cls0[1] + cls1[1]) / 2.0
-        per_launch = collide_bytes(ncell, active, pull)
+        per_launch = stream_bytes(ncell, pull)
         avg_s = tot_ms.value / 1e3 / max(n_launch.value, 1)
         achieved = per_launch / avg_s / 1e9
-        roofline = {"bound": "hbm", "kernel": "k_step_collide", "achieved": round(achieved, 1),
+        roofline = {"bound": "hbm", "kernel": DOMINANT, "achieved": round(achieved, 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": measured_traffic(),
                     "algorithmic_bytes_per_launch": round(per_launch),
                     "avg_launch_us": round(avg_s * 1e6, 2), "timed_launches": n_launch.value,
-                    "cells_collided": round(active), "cells_pull_only": round(pull),
+                    "cells_collided": round(active), "cells_pulled": round(pull),
                     "step_bytes_per_cell": STEP_B_PER_CELL,
                     "step_achieved": round(STEP_B_PER_CELL * ncell * args.steps / el / 1e9, 1)}
         roofline["step_frac"] = round(roofline["step_achieved"] / HBM_PEAK_GBS, 4)
+        if args.breakdown > 0:
+            roofline["kernel_us"] = kernel_breakdown(tw, w, args.warmup + args.steps,
+                                                     args.breakdown)
     mlups = ncell * args.steps / el / 1e6
     tel = tw.get_telemetry()
     result = {

@@ -230,7 +230,7 @@ int waam_telemetry_pool_reduce(const float *T, const float *f_l, const float *ph
                                void *scratch, void *stream);
 
 /* ---- the fused step ------------------------------------------------------------------------
- * One coupled timestep (reference solvers/coupled_step.py:73-464, scalar-material path) in eight
+ * One coupled timestep (reference solvers/coupled_step.py:73-464, scalar-material path) in seven
  * launches, with results identical to the one-kernel-per-reference-kernel sequence above.
  * Fields that a kernel both reads around a cell and writes come in pairs (*_in read, *_out
  * written) that the caller swaps after the step. The pull stream of step n is deferred into the
@@ -247,6 +247,8 @@ typedef struct waam_step_fields {
     float *T_max, *T_prev, *dT_dt, *t800, *t1100, *tsol;
     float *tracer_pos;
     int32_t *tracer_active;
+    /* two int32 work lists of nx*ny*nz entries each (compacted FLUID/IFACE cell indices) */
+    int32_t *list_thermal, *list_collide;
     void *scratch;
     int32_t max_tracers, nx, ny, nz;
 } waam_step_fields;
@@ -289,15 +291,14 @@ int waam_step_arc_readback(const void *scratch, float *host_out3);
  * duration and the number of timed launches, and disarms. Process-wide, one kernel at a time;
  * not for concurrent use from several threads. */
 enum {
-    WAAM_K_ARC_PARTIALS = 0,
-    WAAM_K_ARC_TOTAL = 1,
-    WAAM_K_THERMAL = 2,
-    WAAM_K_VOF_ADVECT = 3,
-    WAAM_K_VOF_FLAGS = 4,
-    WAAM_K_COLLIDE = 5,
-    WAAM_K_TRACERS = 6,
-    WAAM_K_FREEZE = 7,
-    WAAM_K_COUNT = 8
+    WAAM_K_BEGIN = 0,
+    WAAM_K_THERMAL = 1,
+    WAAM_K_THERMAL_SURFACE = 2,
+    WAAM_K_VOF_FLAGS = 3,
+    WAAM_K_COLLIDE = 4,
+    WAAM_K_TRACERS = 5,
+    WAAM_K_STREAM = 6,
+    WAAM_K_COUNT = 7
 };
 int waam_timing_begin(int which, int max_launches);
 int waam_timing_end(double *total_ms, int64_t *launches);

@@ -10,6 +10,10 @@ from __future__ import annotations
 import ctypes
 from pathlib import Path
 
+# torch first: the library then binds to the HIP runtime torch has loaded, so device memory
+# and streams from torch are valid in it (one runtime per process)
+import torch  # noqa: F401
+
 LIB_PATH = Path(__file__).resolve().parent / "libwaam_hip.so"
 
 ABI_VERSION = 1
@@ -74,8 +78,8 @@ SIGNATURES: dict[str, str] = {
 }
 
 # kernel ids of waam_step_fused for waam_timing_begin (include/waam_hip.h WAAM_K_*)
-STEP_KERNELS = ("k_step_arc_partials", "k_step_arc_total", "k_step_thermal", "k_step_vof_advect",
-                "k_step_vof_flags", "k_step_collide", "k_advect_tracers_step", "k_step_freeze")
+STEP_KERNELS = ("k_step_begin", "k_step_thermal", "k_step_thermal_surface", "k_step_vof_flags",
+                "k_step_collide", "k_advect_tracers_step", "k_step_stream")
 
 
 class WaamError(RuntimeError):

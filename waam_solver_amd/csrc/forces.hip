@@ -17,8 +17,10 @@ __global__ void k_csf(const float *__restrict__ phi, const uint8_t *__restrict__
     CELL3D
     uint8_t f = flags[c];
     if (f == SOLID || f == GAS) return;
+    ForceStencil fs;
+    load_force_stencil(phi, phi, flags, i, j, k, d, fs);
     float fx, fy, fz;
-    if (!csf_force(phi, flags, i, j, k, d, gamma_lu, wetting, cos_t, sin_t, fx, fy, fz)) return;
+    if (!csf_force(fs, i, j, k, d, gamma_lu, wetting, cos_t, sin_t, fx, fy, fz)) return;
     Fx[c] += fx;
     Fy[c] += fy;
     Fz[c] += fz;
@@ -32,8 +34,10 @@ __global__ void k_curvature(const float *__restrict__ phi, const uint8_t *__rest
     kappa_out[c] = 0.0f;
     if (flags[c] == GAS) return;
     if (i < 1 || j < 1 || k < 1 || i > d.nx - 2 || j > d.ny - 2 || k > d.nz - 2) return;
+    PhiStencil st;
+    load_phi_stencil(phi, i, j, k, d, st);
     float kappa, g;
-    brackbill_kappa(phi, i, j, k, d, 1e-6f, kappa, g);
+    brackbill_kappa(st, 1e-6f, kappa, g);
     if (g >= 1e-6f) kappa_out[c] = kappa;
 }
 
@@ -45,8 +49,10 @@ __global__ void k_marangoni(const float *__restrict__ T, const float *__restrict
     CELL3D
     uint8_t f = flags[c];
     if (f == SOLID || f == GAS) return;
+    ForceStencil fs;
+    load_force_stencil(phi, T, flags, i, j, k, d, fs);
     float fx, fy, fz;
-    if (!marangoni_force(T, phi, f_l[c], i, j, k, d, dgamma, fx, fy, fz)) return;
+    if (!marangoni_force(fs, f_l[c], dgamma, fx, fy, fz)) return;
     Fx[c] += fx;
     Fy[c] += fy;
     Fz[c] += fz;
@@ -85,7 +91,8 @@ __global__ void k_arc_pressure(float *__restrict__ Fz, const uint8_t *__restrict
     uint8_t f = flags[c];
     if (f == SOLID || f == GAS) return;
     float fz;
-    if (arc_pressure_force(phi, i, j, k, d, ap, fz)) Fz[c] += fz;
+    float pzp = phi[d.idx(i, j, imin(k + 1, d.nz - 1))], pzm = phi[d.idx(i, j, imax(k - 1, 0))];
+    if (arc_pressure_force(pzp, pzm, i, j, k, ap, fz)) Fz[c] += fz;
 }
 
 /* clamp_body_force_magnitude — reference kernels.py:3298-3309 */

@@ -9,72 +9,108 @@
 
 namespace waam {
 
-/* _phi_unit_normal_at — reference kernels.py:769-796 */
-__device__ __forceinline__ void phi_normal(const float *__restrict__ phi, int i, int j, int k,
-                                           const Dims &d, float eps, float &nx_, float &ny_,
-                                           float &nz_, float &g)
+/* Loads first, arithmetic after: every routine below gathers all the values it may need with
+ * clamped indices before it evaluates anything, and selects between them instead of branching
+ * around loads, so a cell's loads issue together rather than as a dependent chain. */
+
+/* index offsets of the six face neighbours of (i, j, k) with the reference's clamps, in the
+ * order x+, x-, y+, y-, z+, z- */
+__device__ __forceinline__ void face_idx(int i, int j, int k, const Dims &d, size_t (&o)[6])
 {
-    float dpx = 0.5f * (phi[d.idx(imin(i + 1, d.nx - 1), j, k)] - phi[d.idx(imax(i - 1, 0), j, k)]);
-    float dpy = 0.5f * (phi[d.idx(i, imin(j + 1, d.ny - 1), k)] - phi[d.idx(i, imax(j - 1, 0), k)]);
-    float dpz = 0.5f * (phi[d.idx(i, j, imin(k + 1, d.nz - 1))] - phi[d.idx(i, j, imax(k - 1, 0))]);
+    o[0] = d.idx(imin(i + 1, d.nx - 1), j, k);
+    o[1] = d.idx(imax(i - 1, 0), j, k);
+    o[2] = d.idx(i, imin(j + 1, d.ny - 1), k);
+    o[3] = d.idx(i, imax(j - 1, 0), k);
+    o[4] = d.idx(i, j, imin(k + 1, d.nz - 1));
+    o[5] = d.idx(i, j, imax(k - 1, 0));
+}
+
+/* _phi_unit_normal_at (reference kernels.py:769-796) from the six loaded face values */
+__device__ __forceinline__ void normal_from(const float (&v)[6], float eps, float &nx_, float &ny_,
+                                            float &nz_, float &g)
+{
+    float dpx = 0.5f * (v[0] - v[1]);
+    float dpy = 0.5f * (v[2] - v[3]);
+    float dpz = 0.5f * (v[4] - v[5]);
     g = sqrtf(dpx * dpx + dpy * dpy + dpz * dpz);
-    nx_ = 0.0f;
-    ny_ = 0.0f;
-    nz_ = 0.0f;
-    if (g >= eps) {
-        nx_ = dpx / g;
-        ny_ = dpy / g;
-        nz_ = dpz / g;
-    }
+    bool ok = g >= eps;
+    nx_ = ok ? dpx / g : 0.0f;
+    ny_ = ok ? dpy / g : 0.0f;
+    nz_ = ok ? dpz / g : 0.0f;
+}
+
+/* phi on the stencil of the Brackbill curvature: the face values of the cell and of each of
+ * its six (clamped) face neighbours */
+struct PhiStencil {
+    float c[6];      // faces of the cell
+    float n[6][6];   // faces of face neighbour m (m in face_idx order)
+};
+
+__device__ __forceinline__ void load_phi_stencil(const float *__restrict__ phi, int i, int j,
+                                                 int k, const Dims &d, PhiStencil &st)
+{
+    size_t o[6];
+    face_idx(i, j, k, d, o);
+    const int ni[6] = {imin(i + 1, d.nx - 1), imax(i - 1, 0), i, i, i, i};
+    const int nj[6] = {j, j, imin(j + 1, d.ny - 1), imax(j - 1, 0), j, j};
+    const int nk[6] = {k, k, k, k, imin(k + 1, d.nz - 1), imax(k - 1, 0)};
+    size_t on[6][6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) face_idx(ni[m], nj[m], nk[m], d, on[m]);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) st.c[q] = phi[o[q]];
+#pragma unroll
+    for (int m = 0; m < 6; ++m)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) st.n[m][q] = phi[on[m][q]];
 }
 
 /* _brackbill_curvature_at — reference kernels.py:799-855 */
-__device__ __forceinline__ void brackbill_kappa(const float *__restrict__ phi, int i, int j, int k,
-                                                const Dims &d, float eps, float &kappa, float &g0)
+__device__ __forceinline__ void brackbill_kappa(const PhiStencil &st, float eps, float &kappa,
+                                                float &g0)
 {
     float n0x, n0y, n0z;
-    phi_normal(phi, i, j, k, d, eps, n0x, n0y, n0z, g0);
-    kappa = 0.0f;
-    if (g0 >= eps) {
-        float a, b, nxp, nxm, nyp, nym, nzp, nzm, gp, gm, gp2, gm2, gp3, gm3;
-        phi_normal(phi, imin(i + 1, d.nx - 1), j, k, d, eps, nxp, a, b, gp);
-        phi_normal(phi, imax(i - 1, 0), j, k, d, eps, nxm, a, b, gm);
-        phi_normal(phi, i, imin(j + 1, d.ny - 1), k, d, eps, a, nyp, b, gp2);
-        phi_normal(phi, i, imax(j - 1, 0), k, d, eps, a, nym, b, gm2);
-        phi_normal(phi, i, j, imin(k + 1, d.nz - 1), d, eps, a, b, nzp, gp3);
-        phi_normal(phi, i, j, imax(k - 1, 0), d, eps, a, b, nzm, gm3);
-        if (gp < eps) nxp = n0x;
-        if (gm < eps) nxm = n0x;
-        if (gp2 < eps) nyp = n0y;
-        if (gm2 < eps) nym = n0y;
-        if (gp3 < eps) nzp = n0z;
-        if (gm3 < eps) nzm = n0z;
-        float div_n = 0.5f * (nxp - nxm) + 0.5f * (nyp - nym) + 0.5f * (nzp - nzm);
-        kappa = -div_n;
-    }
+    normal_from(st.c, eps, n0x, n0y, n0z, g0);
+    float a, b, nxp, nxm, nyp, nym, nzp, nzm, gp, gm, gp2, gm2, gp3, gm3;
+    normal_from(st.n[0], eps, nxp, a, b, gp);
+    normal_from(st.n[1], eps, nxm, a, b, gm);
+    normal_from(st.n[2], eps, a, nyp, b, gp2);
+    normal_from(st.n[3], eps, a, nym, b, gm2);
+    normal_from(st.n[4], eps, a, b, nzp, gp3);
+    normal_from(st.n[5], eps, a, b, nzm, gm3);
+    nxp = gp < eps ? n0x : nxp;
+    nxm = gm < eps ? n0x : nxm;
+    nyp = gp2 < eps ? n0y : nyp;
+    nym = gm2 < eps ? n0y : nym;
+    nzp = gp3 < eps ? n0z : nzp;
+    nzm = gm3 < eps ? n0z : nzm;
+    float div_n = 0.5f * (nxp - nxm) + 0.5f * (nyp - nym) + 0.5f * (nzp - nzm);
+    kappa = g0 >= eps ? -div_n : 0.0f;
 }
 
-/* _solid_wall_normal — reference kernels.py:680-719 */
-__device__ __forceinline__ bool wall_normal(const uint8_t *__restrict__ flags, int i, int j, int k,
+/* _solid_wall_normal — reference kernels.py:680-719, from the six loaded face flags */
+__device__ __forceinline__ bool wall_normal(const uint8_t (&fn)[6], int i, int j, int k,
                                             const Dims &d, float &wx, float &wy, float &wz)
 {
+    bool sxm = i > 0 && fn[1] == SOLID, sxp = i < d.nx - 1 && fn[0] == SOLID;
+    bool sym = j > 0 && fn[3] == SOLID, syp = j < d.ny - 1 && fn[2] == SOLID;
+    bool szm = k > 0 && fn[5] == SOLID, szp = k < d.nz - 1 && fn[4] == SOLID;
     wx = 0.0f;
     wy = 0.0f;
     wz = 0.0f;
-    bool has = false;
-    if (i > 0 && flags[d.idx(i - 1, j, k)] == SOLID) { wx += 1.0f; has = true; }
-    if (i < d.nx - 1 && flags[d.idx(i + 1, j, k)] == SOLID) { wx -= 1.0f; has = true; }
-    if (j > 0 && flags[d.idx(i, j - 1, k)] == SOLID) { wy += 1.0f; has = true; }
-    if (j < d.ny - 1 && flags[d.idx(i, j + 1, k)] == SOLID) { wy -= 1.0f; has = true; }
-    if (k > 0 && flags[d.idx(i, j, k - 1)] == SOLID) { wz += 1.0f; has = true; }
-    if (k < d.nz - 1 && flags[d.idx(i, j, k + 1)] == SOLID) { wz -= 1.0f; has = true; }
+    if (sxm) wx += 1.0f;
+    if (sxp) wx -= 1.0f;
+    if (sym) wy += 1.0f;
+    if (syp) wy -= 1.0f;
+    if (szm) wz += 1.0f;
+    if (szp) wz -= 1.0f;
     float m = sqrtf(wx * wx + wy * wy + wz * wz);
     if (m > 1e-8f) {
         wx /= m;
         wy /= m;
         wz /= m;
     }
-    return has;
+    return sxm || sxp || sym || syp || szm || szp;
 }
 
 /* _correct_normal_contact_angle — reference kernels.py:722-766 */
@@ -110,56 +146,75 @@ __device__ __forceinline__ void contact_normal(float nxn, float nyn, float nzn, 
     }
 }
 
+/* Everything the body forces of one cell read around it: the curvature stencil of phi, and T
+ * and the flags on the six faces. */
+struct ForceStencil {
+    PhiStencil phi;
+    float T[6];
+    uint8_t flags[6];
+};
+
+__device__ __forceinline__ void load_force_stencil(const float *__restrict__ phi,
+                                                   const float *__restrict__ T,
+                                                   const uint8_t *__restrict__ flags, int i, int j,
+                                                   int k, const Dims &d, ForceStencil &st)
+{
+    load_phi_stencil(phi, i, j, k, d, st.phi);
+    size_t o[6];
+    face_idx(i, j, k, d, o);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        st.T[q] = T[o[q]];
+        st.flags[q] = flags[o[q]];
+    }
+}
+
 /* compute_csf_tension body — reference kernels.py:885-926. Returns false when the reference
  * adds nothing (rim, |grad phi| < eps). Caller has checked the cell is neither SOLID nor GAS. */
-__device__ __forceinline__ bool csf_force(const float *__restrict__ phi,
-                                          const uint8_t *__restrict__ flags, int i, int j, int k,
+__device__ __forceinline__ bool csf_force(const ForceStencil &fs, int i, int j, int k,
                                           const Dims &d, float gamma_lu, int wetting, float cos_t,
                                           float sin_t, float &fx, float &fy, float &fz)
 {
     const float eps = 1e-6f;
     if (i < 1 || j < 1 || k < 1 || i > d.nx - 2 || j > d.ny - 2 || k > d.nz - 2) return false;
+    const PhiStencil &st = fs.phi;
+    float wx = 0.0f, wy = 0.0f, wz = 0.0f;
+    bool wall = wetting != 0 && wall_normal(fs.flags, i, j, k, d, wx, wy, wz);
     float kappa, g;
-    brackbill_kappa(phi, i, j, k, d, eps, kappa, g);
+    brackbill_kappa(st, eps, kappa, g);
     if (g < eps) return false;
-    float dpx = 0.5f * (phi[d.idx(imin(i + 1, d.nx - 1), j, k)] - phi[d.idx(imax(i - 1, 0), j, k)]);
-    float dpy = 0.5f * (phi[d.idx(i, imin(j + 1, d.ny - 1), k)] - phi[d.idx(i, imax(j - 1, 0), k)]);
-    float dpz = 0.5f * (phi[d.idx(i, j, imin(k + 1, d.nz - 1))] - phi[d.idx(i, j, imax(k - 1, 0))]);
+    float dpx = 0.5f * (st.c[0] - st.c[1]);
+    float dpy = 0.5f * (st.c[2] - st.c[3]);
+    float dpz = 0.5f * (st.c[4] - st.c[5]);
     fx = gamma_lu * kappa * dpx;
     fy = gamma_lu * kappa * dpy;
     fz = gamma_lu * kappa * dpz;
-    if (wetting != 0) {
-        float wx, wy, wz;
-        if (wall_normal(flags, i, j, k, d, wx, wy, wz)) {
-            float cx, cy, cz;
-            contact_normal(dpx / g, dpy / g, dpz / g, wx, wy, wz, cos_t, sin_t, cx, cy, cz);
-            fx = gamma_lu * kappa * cx * g;
-            fy = gamma_lu * kappa * cy * g;
-            fz = gamma_lu * kappa * cz * g;
-        }
+    if (wall) {
+        float cx, cy, cz;
+        contact_normal(dpx / g, dpy / g, dpz / g, wx, wy, wz, cos_t, sin_t, cx, cy, cz);
+        fx = gamma_lu * kappa * cx * g;
+        fy = gamma_lu * kappa * cy * g;
+        fz = gamma_lu * kappa * cz * g;
     }
     return true;
 }
 
 /* compute_marangoni_force body — reference kernels.py:2187-2217 (cell neither SOLID nor GAS). */
-__device__ __forceinline__ bool marangoni_force(const float *__restrict__ T,
-                                                const float *__restrict__ phi, float fl, int i,
-                                                int j, int k, const Dims &d, float dgamma,
+__device__ __forceinline__ bool marangoni_force(const ForceStencil &fs, float fl, float dgamma,
                                                 float &fx, float &fy, float &fz)
 {
+    const float(&pv)[6] = fs.phi.c;
+    const float(&tv)[6] = fs.T;
     if (fl < 0.05f) return false;
-    int ip = imin(i + 1, d.nx - 1), im = imax(i - 1, 0);
-    int jp = imin(j + 1, d.ny - 1), jm = imax(j - 1, 0);
-    int kp = imin(k + 1, d.nz - 1), km = imax(k - 1, 0);
-    float dpx = 0.5f * (phi[d.idx(ip, j, k)] - phi[d.idx(im, j, k)]);
-    float dpy = 0.5f * (phi[d.idx(i, jp, k)] - phi[d.idx(i, jm, k)]);
-    float dpz = 0.5f * (phi[d.idx(i, j, kp)] - phi[d.idx(i, j, km)]);
+    float dpx = 0.5f * (pv[0] - pv[1]);
+    float dpy = 0.5f * (pv[2] - pv[3]);
+    float dpz = 0.5f * (pv[4] - pv[5]);
     float gm = sqrtf(dpx * dpx + dpy * dpy + dpz * dpz);
     if (gm < 1e-6f) return false;
     float nxn = dpx / gm, nyn = dpy / gm, nzn = dpz / gm;
-    float dTx = 0.5f * (T[d.idx(ip, j, k)] - T[d.idx(im, j, k)]);
-    float dTy = 0.5f * (T[d.idx(i, jp, k)] - T[d.idx(i, jm, k)]);
-    float dTz = 0.5f * (T[d.idx(i, j, kp)] - T[d.idx(i, j, km)]);
+    float dTx = 0.5f * (tv[0] - tv[1]);
+    float dTy = 0.5f * (tv[2] - tv[3]);
+    float dTz = 0.5f * (tv[4] - tv[5]);
     float ndot = nxn * dTx + nyn * dTy + nzn * dTz;
     float scale = dgamma * gm;
     fx = scale * (dTx - ndot * nxn);
@@ -173,11 +228,11 @@ struct ArcPressure {
     float arc_i, arc_j, arc_k, inv2s2, F_peak_lu;
 };
 
-__device__ __forceinline__ bool arc_pressure_force(const float *__restrict__ phi, int i, int j,
-                                                   int k, const Dims &d, const ArcPressure &ap,
-                                                   float &fz)
+/* phi_zp, phi_zm: phi at the clamped k+1 and k-1 cells */
+__device__ __forceinline__ bool arc_pressure_force(float phi_zp, float phi_zm, int i, int j, int k,
+                                                   const ArcPressure &ap, float &fz)
 {
-    float dphz = phi[d.idx(i, j, imin(k + 1, d.nz - 1))] - phi[d.idx(i, j, imax(k - 1, 0))];
+    float dphz = phi_zp - phi_zm;
     if (!(fabsf(dphz) > 0.05f)) return false;
     float di = (float)i - ap.arc_i, dj = (float)j - ap.arc_j, dk = (float)k - ap.arc_k;
     float r2 = di * di + dj * dj + dk * dk * 0.1f;

@@ -1,27 +1,45 @@
-// fused.hip — the coupled step in eight launches (waam_step_fused).
+// fused.hip — the coupled step in seven launches (waam_step_fused).
 //
 // Launch sequence for one step (reference solvers/coupled_step.py:73-464):
-//   1 k_step_arc_partials   torch-column surface height (arc_k) + arc weight partial sums
-//   2 k_step_arc_total      fixed-order weight total, energy per unit weight
-//   3 k_step_thermal        arc + enthalpy ceiling applied on the fly, advection-diffusion,
-//                           boundary losses, floor, ceiling, phase map, T_max, cooling rate
-//   4 k_step_vof_advect     donor-cell phi advection + re-initialisation -> phi_tmp
-//   5 k_step_vof_flags      contact-angle BC, flag classification, phi/f_l consistency, time
-//                           above the HAZ thresholds
-//   6 k_step_collide        CSF + Marangoni + arc pressure + gravity + buoyancy, force clamp,
-//                           MRT/SRT collision, Mach clamp, and the previous step's pull stream
-//   7 k_advect_tracers      porosity tracers
-//   8 k_step_freeze         remelt, trailing-pool and cooled-metal solidification
+//   1 k_step_begin           torch-column surface height (arc_k); arc weight partial sums and,
+//                            in the last block to finish, their fixed-order total and the
+//                            energy per unit weight
+//   2 k_step_thermal         every cell but FLUID/IFACE: arc + enthalpy ceiling applied on the
+//                            fly, conduction, boundary losses, floor, ceiling, phase map, T_max,
+//                            cooling rate, donor-cell phi advection + re-initialisation; lists
+//                            the FLUID/IFACE cells
+//   3 k_step_thermal_surface the same chain plus enthalpy advection for the listed cells
+//   4 k_step_vof_flags       contact-angle BC, flag classification, phi/f_l consistency, time
+//                            above the HAZ thresholds; lists the cells the collision updates
+//   5 k_step_collide         listed cells: CSF + Marangoni + arc pressure + gravity + buoyancy,
+//                            force clamp, MRT/SRT collision, Mach clamp
+//   6 k_advect_tracers       porosity tracers
+//   7 k_step_stream          every other cell: the previous step's pull stream, force reset;
+//                            every cell: remelt, trailing-pool and cooled-metal solidification
 // Every cell value is produced by the same fp32 expression sequence as the one-kernel-per-
 // reference-kernel path, so both paths give identical bits.
 //
+// Work split. A wavefront covers 64 cells of one (i, j) column, which on a plate crosses
+// SOLID, the free surface and GAS; the surface cells carry nearly all the arithmetic (enthalpy
+// advection, CSF, collision). The sweeps over every cell (2, 4, 6) do the cheap per-class work
+// and append the surface cells to a compacted index list; kernels 3 and 5 run the expensive
+// path on full wavefronts from that list (order-free: each cell's result depends only on the
+// kernel's inputs). Separate kernels also keep the sweeps' register count, and so their
+// occupancy, independent of the surface arithmetic.
+//
+// Loads before stores. The field pointers arrive in a struct and carry no alias information,
+// so every per-cell routine issues its loads before its first store, and selects between
+// loaded values rather than branching around loads (a branch lets the compiler sink the loads
+// behind it and serialise them).
+//
 // Deferred stream. The reference streams populations after each collide (f_dst -> f_src) and
 // then swaps, so the next collide overwrites the streamed value on every FLUID/IFACE cell. Here
-// the stream of step n is folded into kernel 6 of step n+1: a cell that step n+1 collides
+// the stream of step n is folded into kernel 4 of step n+1: a cell that step n+1 collides
 // skips the pull (its value would be overwritten), every other cell that was not SOLID at step
 // n pulls from its neighbours with step n's flags (kept in sflags). Populations are read once
 // and written once per active cell per step instead of twice.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <vector>
 
@@ -33,6 +51,27 @@
 
 namespace waam {
 
+/* blocks of the list kernels 3 and 5 (each grid-strides over its list) */
+constexpr int kListBlocks = 1024;
+/* int32 counter in the workspace: blocks of k_step_begin done */
+constexpr size_t kScratchStepCounters = kScratchTelemetry + 32;
+static_assert(kScratchStepCounters + sizeof(int) <= kScratchTelemetryParts,
+              "step counters overlap the telemetry partials");
+static_assert(kBlock == kListSegments, "the list prefix scan uses one thread per segment");
+
+/* A cell list is split into kListSegments contiguous cell ranges of seg_len cells (a multiple
+ * of 256); segment g stores its cells at [g*seg_len, g*seg_len + count_g) with its own counter,
+ * so appends spread over many addresses instead of contending on one. */
+struct CellList {
+    int32_t *cells;
+    int *counts;  // kListSegments counters, kCounterStride bytes apart
+    int seg_len;
+    __device__ __forceinline__ int &count(int g) const
+    {
+        return *reinterpret_cast<int *>((char *)counts + (size_t)g * kCounterStride);
+    }
+};
+
 struct StepConsts {
     waam_step_params p;
     ArcParams arc;
@@ -40,7 +79,8 @@ struct StepConsts {
     LossConsts lc;
     ArcPressure ap;
     float cap_l, cap_s, h_min, inv_dt;
-    int box_i0, box_i1, box_j0, box_j1, box_k0, box_k1;  // cells outside get exactly zero arc heat
+    int box_i0, box_i1, box_j0, box_j1;  // cells outside get exactly zero arc heat
+    int seg_len;                         // cells per list segment
     Dims d;
 };
 
@@ -49,16 +89,107 @@ __device__ __forceinline__ float step_arc_k(const waam_step_params &p, const flo
     return p.use_arc_k_override ? p.arc_k_override : tot[2];
 }
 
+__device__ __forceinline__ int *step_counters(const waam_step_fields &f)
+{
+    return reinterpret_cast<int *>((char *)f.scratch + kScratchStepCounters);
+}
+
+/* which = 0: thermal list (FLUID/IFACE by the incoming flags), 1: collision list */
+__device__ __forceinline__ CellList cell_list(const waam_step_fields &f, int seg_len, int which)
+{
+    CellList l;
+    l.cells = which == 0 ? f.list_thermal : f.list_collide;
+    l.counts = reinterpret_cast<int *>((char *)f.scratch + kScratchListCounters +
+                                       (size_t)which * kListSegments * kCounterStride);
+    l.seg_len = seg_len;
+    return l;
+}
+
+__device__ __forceinline__ bool is_surface_class(uint8_t fl) { return fl == FLUID || fl == IFACE; }
+
+/* Wavefront-aggregated append of cell c for the lanes with want set: one atomic per wavefront
+ * and segment touched; positions within a step are unique, their order is not fixed (consumers
+ * are order-free). */
+__device__ __forceinline__ void list_append(bool want, size_t c, const CellList &l, size_t ncell)
+{
+    int seg = want ? (int)(c / (size_t)l.seg_len) : -1;
+    int lane = (int)__lane_id();
+    for (;;) {
+        unsigned long long m = __ballot(seg >= 0);
+        if (m == 0ull) return;
+        int leader = __builtin_ctzll(m);
+        int g = __shfl(seg, leader);
+        unsigned long long mm = __ballot(seg == g);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&l.count(g), __popcll(mm));
+        base = __shfl(base, leader);
+        if (seg == g) {
+            size_t pos = (size_t)g * l.seg_len + (size_t)(base + __popcll(mm & ((1ull << lane) - 1ull)));
+            if (pos < std::min((size_t)(g + 1) * l.seg_len, ncell)) l.cells[pos] = (int32_t)c;
+            seg = -1;
+        }
+    }
+}
+
+/* Exclusive prefix of the clamped segment lengths into s_pref[0..kListSegments]; every thread
+ * of the block calls it. Returns the list length. */
+__device__ __forceinline__ int list_prefix(const CellList &l, size_t ncell, int *s_pref)
+{
+    int t = threadIdx.y * blockDim.x + threadIdx.x;
+    size_t lo = (size_t)t * l.seg_len;
+    int cap = lo >= ncell ? 0 : (int)std::min((size_t)l.seg_len, ncell - lo);
+    int v = imin(l.count(t), cap);
+    s_pref[t + 1] = v;
+    if (t == 0) s_pref[0] = 0;
+    __syncthreads();
+    for (int off = 1; off < kListSegments; off <<= 1) {
+        int add = t + 1 > off ? s_pref[t + 1 - off] : 0;
+        __syncthreads();
+        s_pref[t + 1] += add;
+        __syncthreads();
+    }
+    return s_pref[kListSegments];
+}
+
+/* cell at flat position t of the list (s_pref[lo] <= t < s_pref[lo + 1]) */
+__device__ __forceinline__ size_t list_at(const CellList &l, const int *s_pref, int t)
+{
+    int lo = 0, hi = kListSegments;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (s_pref[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    return (size_t)l.cells[(size_t)lo * l.seg_len + (t - s_pref[lo])];
+}
+
+/* (i, j, k) of cell c; the fused step limits grids to INT_MAX cells, so 32-bit division */
+__device__ __forceinline__ void cell_ijk(size_t c, const Dims &d, int &i, int &j, int &k)
+{
+    unsigned cc = (unsigned)c, nz = (unsigned)d.nz, ny = (unsigned)d.ny;
+    unsigned r = cc / nz;
+    k = (int)(cc - r * nz);
+    i = (int)(r / ny);
+    j = (int)(r - (unsigned)i * ny);
+}
+
 /* 1 — surface height in the torch column (reference kernels.py:625-641 and
- *     coupled_step.py:34-59) and per-block compensated arc weight sums. */
-__global__ void __launch_bounds__(kBlock) k_step_arc_partials(waam_step_fields f, StepConsts s,
-                                                               int do_sums)
+ *     coupled_step.py:34-59), compensated arc weight sums (kernels.py:443-496) with the total
+ *     formed by the last block; empties both cell lists for kernels 2 and 4. */
+__global__ void __launch_bounds__(kBlock) k_step_begin(waam_step_fields f, StepConsts s,
+                                                       int do_sums)
 {
     __shared__ double sh_s[kBlock], sh_c[kBlock];
     __shared__ int sh_k[kBlock];
     __shared__ float s_arck;
+    __shared__ bool s_last;
     const Dims &d = s.d;
     const waam_step_params &p = s.p;
+    int *cnt = step_counters(f);
+    if (blockIdx.x == 0) {
+        cell_list(f, s.seg_len, 0).count(threadIdx.x) = 0;
+        cell_list(f, s.seg_len, 1).count(threadIdx.x) = 0;
+    }
     int best = -1;
     for (int k = threadIdx.x; k < d.nz; k += blockDim.x) {
         size_t c = d.idx(p.col_i, p.col_j, k);
@@ -82,40 +213,54 @@ __global__ void __launch_bounds__(kBlock) k_step_arc_partials(waam_step_fields f
         }
     }
     __syncthreads();
-    if (!do_sums) return;
     ArcParams ap = s.arc;
-    ap.arc_k = s_arck;
+    ap.arc_k = p.use_arc_k_override ? p.arc_k_override : s_arck;
     double sum = 0.0, comp = 0.0;
     size_t N = d.n();
-    for (size_t cell = (size_t)blockIdx.x * blockDim.x + threadIdx.x; cell < N;
-         cell += (size_t)gridDim.x * blockDim.x) {
-        if (f.flags_in[cell] == GAS) continue;
-        int k = (int)(cell % (size_t)d.nz);
-        size_t r = cell / (size_t)d.nz;
-        int j = (int)(r % (size_t)d.ny);
-        int i = (int)(r / (size_t)d.ny);
-        float wg, wdep;
-        if (!arc_cell_weights(ap, i, j, k, f.fl_in[cell], wg, wdep)) continue;
-        double ss, e;
-        two_sum(sum, (double)(wg * wdep), ss, e);
-        sum = ss;
-        comp += e;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    /* same cell-to-thread assignment and per-thread order as the standalone arc kernel; the
+     * flags and liquid fractions of kUnroll consecutive iterations are loaded together */
+    constexpr int kUnroll = 8;
+    for (size_t base = (size_t)blockIdx.x * blockDim.x + threadIdx.x; do_sums && base < N;
+         base += kUnroll * stride) {
+        uint8_t flg[kUnroll];
+        float flv[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            size_t cell = base + u * stride;
+            bool in = cell < N;
+            flg[u] = in ? f.flags_in[cell] : (uint8_t)GAS;
+            flv[u] = in ? f.fl_in[cell] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            if (flg[u] == GAS) continue;
+            int i, j, k;
+            cell_ijk(base + u * stride, d, i, j, k);
+            float wg, wdep;
+            if (!arc_cell_weights(ap, i, j, k, flv[u], wg, wdep)) continue;
+            double ss, e;
+            two_sum(sum, (double)(wg * wdep), ss, e);
+            sum = ss;
+            comp += e;
+        }
     }
+    if (!do_sums) return;
     block_reduce_pair(sum, comp, sh_s, sh_c);
+    double *parts = reinterpret_cast<double *>((char *)f.scratch + kScratchPartials);
     if (threadIdx.x == 0) {
-        double *parts = reinterpret_cast<double *>((char *)f.scratch + kScratchPartials);
         parts[2 * blockIdx.x] = sum;
         parts[2 * blockIdx.x + 1] = comp;
+        __threadfence();
+        s_last = atomicAdd(&cnt[0], 1) == (int)gridDim.x - 1;
     }
-}
-
-/* 2 — fixed-order total of the block sums (reference kernels.py:493-496) */
-__global__ void __launch_bounds__(kBlock) k_step_arc_total(waam_step_fields f, StepConsts s)
-{
-    __shared__ double sh_s[kBlock], sh_c[kBlock];
-    const double *parts = reinterpret_cast<const double *>((char *)f.scratch + kScratchPartials);
-    double sum = 0.0, comp = 0.0;
-    for (int b = threadIdx.x; b < kReduceBlocks; b += blockDim.x) {
+    __syncthreads();
+    if (!s_last) return;
+    /* last block: fixed-order total of the block sums (reference kernels.py:493-496) */
+    __threadfence();
+    sum = 0.0;
+    comp = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
         double ss, e;
         two_sum(sum, parts[2 * b], ss, e);
         sum = ss;
@@ -126,9 +271,10 @@ __global__ void __launch_bounds__(kBlock) k_step_arc_total(waam_step_fields f, S
         float *tot = reinterpret_cast<float *>((char *)f.scratch + kScratchTelemetry);
         float w_total = (float)(sum + comp);
         float epw = 0.0f;
-        if (w_total > 1e-9f) epw = s.p.Q_w * s.p.eta * s.p.dt / (w_total * s.p.dx3);
+        if (w_total > 1e-9f) epw = p.Q_w * p.eta * p.dt / (w_total * p.dx3);
         tot[0] = w_total;
         tot[1] = epw;
+        cnt[0] = 0;
     }
 }
 
@@ -141,18 +287,21 @@ struct HView {
     float epw;
     bool arc_on, cap_on;
 
-    __device__ __forceinline__ float at(size_t c, int i, int j, int k) const
+    __device__ __forceinline__ float value(float h, float fl, int i, int j, int k) const
     {
-        float h = f->H_in[c];
-        if (arc_on && i >= s->box_i0 && i <= s->box_i1 && j >= s->box_j0 && j <= s->box_j1 &&
-            k >= s->box_k0 && k <= s->box_k1) {
+        if (arc_on && i >= s->box_i0 && i <= s->box_i1 && j >= s->box_j0 && j <= s->box_j1) {
             float wg, wdep;
-            if (arc_cell_weights(ap, i, j, k, f->fl_in[c], wg, wdep)) h += epw * wg * wdep;
+            if (arc_cell_weights(ap, i, j, k, fl, wg, wdep)) h += epw * wg * wdep;
         }
         if (cap_on) h = ceiling_clamp(h, s->pc.H_liq, s->cap_l, s->cap_s);
         return h;
     }
-    /* _masked_at on the enthalpy view: clamped index, GAS neighbours give the fallback */
+    __device__ __forceinline__ float at(size_t c, int i, int j, int k) const
+    {
+        return value(f->H_in[c], f->fl_in[c], i, j, k);
+    }
+    /* _masked_at on the enthalpy view: clamped index, GAS neighbours give the fallback (loads
+     * first, select last, so the loads are not sunk behind the flag test) */
     __device__ __forceinline__ float masked(int i, int j, int k, float fallback) const
     {
         const Dims &d = s->d;
@@ -160,23 +309,33 @@ struct HView {
         int jj = imax(0, imin(j, d.ny - 1));
         int kk = imax(0, imin(k, d.nz - 1));
         size_t c = d.idx(ii, jj, kk);
-        if (f->flags_in[c] == GAS) return fallback;
-        return at(c, ii, jj, kk);
+        uint8_t fg = f->flags_in[c];
+        float v = value(f->H_in[c], f->fl_in[c], ii, jj, kk);
+        return fg == GAS ? fallback : v;
     }
 };
 
-/* 3 — thermal chain (reference kernels.py:497-510, 1953-1984, 1611-1682, 2030-2082,
- *     1937-1950, 1519-1556, 3016-3032, 2013-2027 in coupled_step order) */
-__global__ void __launch_bounds__(256) k_step_thermal(waam_step_fields f, StepConsts s)
+__device__ __forceinline__ HView make_hview(const waam_step_fields &f, const StepConsts &s)
+{
+    const float *tot = reinterpret_cast<const float *>((char *)f.scratch + kScratchTelemetry);
+    HView hv;
+    hv.f = &f;
+    hv.s = &s;
+    hv.ap = s.arc;
+    hv.ap.arc_k = step_arc_k(s.p, tot);
+    hv.epw = tot[1];
+    hv.arc_on = s.p.is_welding && !s.p.arc_applied && hv.epw > 0.0f;
+    hv.cap_on = s.p.is_welding && !s.p.arc_applied && s.p.enable_cap;
+    return hv;
+}
+
+/* thermal chain of one cell (reference kernels.py:497-510, 1953-1984, 1611-1682, 2030-2082,
+ * 1937-1950, 1519-1556, 3016-3032, 2013-2027 in coupled_step order) */
+__device__ __forceinline__ void thermal_cell(const waam_step_fields &f, const StepConsts &s,
+                                             int i, int j, int k, size_t c, uint8_t flag)
 {
     const Dims &d = s.d;
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int i = blockIdx.z;
-    if (k >= d.nz || j >= d.ny) return;
     const waam_step_params &p = s.p;
-    size_t c = d.idx(i, j, k);
-    uint8_t flag = f.flags_in[c];
     if (flag == GAS) {
         /* GAS: only the two phase updates touch it (ceiling, thermal and losses skip GAS) */
         float h = f.H_in[c], T, fl;
@@ -186,16 +345,7 @@ __global__ void __launch_bounds__(256) k_step_thermal(waam_step_fields f, StepCo
         f.fl_out[c] = fl;
         return;
     }
-    const float *tot = reinterpret_cast<const float *>((char *)f.scratch + kScratchTelemetry);
-    HView hv;
-    hv.f = &f;
-    hv.s = &s;
-    hv.ap = s.arc;
-    hv.ap.arc_k = step_arc_k(p, tot);
-    hv.epw = tot[1];
-    hv.arc_on = p.is_welding && !p.arc_applied && hv.epw > 0.0f;
-    hv.cap_on = p.is_welding && !p.arc_applied && p.enable_cap;
-
+    HView hv = make_hview(f, s);
     /* advect_diffuse_temperature (Jacobi: every neighbour value is pre-kernel) */
     float T_c = f.T_in[c];
     float T_im = masked_at(f.T_in, f.flags_in, i - 1, j, k, T_c, d);
@@ -235,64 +385,121 @@ __global__ void __launch_bounds__(256) k_step_thermal(waam_step_fields f, StepCo
     if (p.enable_cap) h = ceiling_clamp(h, s.pc.H_liq, s.cap_l, s.cap_s);
     float T, fl;
     phase_of(h, s.pc, T, fl);
+    /* HAZ diagnostics (reference kernels.py:3027-3032, 2023-2027); they run before the VOF
+     * flag update, so they see this step's incoming flags. Every load precedes the first store
+     * (the field pointers carry no alias information). */
+    float T_max = f.T_max[c], T_prev = f.T_prev[c];
     f.H_out[c] = h;
     f.T_out[c] = T;
     f.fl_out[c] = fl;
-    /* HAZ diagnostics (reference kernels.py:3027-3032, 2023-2027); they run before the VOF
-     * flag update, so they see this step's incoming flags */
-    if (T > f.T_max[c]) f.T_max[c] = T;
-    f.dT_dt[c] = (T - f.T_prev[c]) * s.inv_dt;
+    if (T > T_max) f.T_max[c] = T;
+    f.dT_dt[c] = (T - T_prev) * s.inv_dt;
     f.T_prev[c] = T;
 }
 
-/* update_time_above_T (reference kernels.py:3261-3283) runs after the VOF flag update and
- * skips cells that are GAS by the updated flags */
-__device__ __forceinline__ void time_above(const waam_step_fields &f, const waam_step_params &p,
-                                           size_t c)
+/* advect_phi + reinitialize_phi of one cell (reference kernels.py:2320-2391) */
+__device__ __forceinline__ float vof_advect_cell(const waam_step_fields &f, const Dims &d, int i,
+                                                 int j, int k, uint8_t flag)
 {
-    float T = f.T_out[c];
-    if (T >= p.T800) f.t800[c] += p.dt;
-    if (T >= p.T1100) f.t1100[c] += p.dt;
-    if (T >= p.T_s) f.tsol[c] += p.dt;
-}
-
-/* 4 — advect_phi + reinitialize_phi (reference kernels.py:2320-2391) */
-__global__ void __launch_bounds__(256) k_step_vof_advect(waam_step_fields f, Dims d)
-{
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int i = blockIdx.z;
-    if (k >= d.nz || j >= d.ny) return;
-    size_t c = d.idx(i, j, k);
     float p = advected_phi(f.phi, f.ux, f.uy, f.uz, f.flags_in, i, j, k, d);
-    f.phi_tmp[c] = reinit_phi(p, f.flags_in[c]);
+    return reinit_phi(p, flag);
 }
 
-/* 5 — apply_contact_angle_phi_bc (Jacobi), update_flags_from_phi, sync_phi_liquid_fraction,
- *     update_time_above_T (reference kernels.py:933-991, 2394-2419, 3484-3504, 3261-3283) */
-__global__ void __launch_bounds__(256) k_step_vof_flags(waam_step_fields f, StepConsts s)
+/* thermal chain and VOF advection of one cell */
+__device__ __forceinline__ void thermal_vof_cell(const waam_step_fields &f, const StepConsts &s,
+                                                 int i, int j, int k, size_t c, uint8_t flag)
+{
+    float phi_new = s.p.enable_vof ? vof_advect_cell(f, s.d, i, j, k, flag) : 0.0f;
+    thermal_cell(f, s, i, j, k, c, flag);
+    if (s.p.enable_vof) f.phi_tmp[c] = phi_new;
+}
+
+/* 2 — thermal chain and VOF advection of every cell that is not FLUID/IFACE; lists those */
+__global__ void __launch_bounds__(256) k_step_thermal(waam_step_fields f, StepConsts s)
 {
     const Dims &d = s.d;
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     int j = blockIdx.y * blockDim.y + threadIdx.y;
     int i = blockIdx.z;
-    if (k >= d.nz || j >= d.ny) return;
-    size_t c = d.idx(i, j, k);
-    if (!s.p.enable_vof) {
-        uint8_t fi = f.flags_in[c];
-        f.flags_out[c] = fi;
-        if (fi != GAS) time_above(f, s.p, c);
-        return;
+    bool in = k < d.nz && j < d.ny;
+    size_t c = in ? d.idx(i, j, k) : 0;
+    uint8_t flag = in ? f.flags_in[c] : (uint8_t)GAS;
+    list_append(in && is_surface_class(flag), c, cell_list(f, s.seg_len, 0), d.n());
+    if (!in || is_surface_class(flag)) return;
+    thermal_vof_cell(f, s, i, j, k, c, flag);
+}
+
+/* 3 — thermal chain (with enthalpy advection) and VOF advection of the listed FLUID/IFACE
+ *     cells */
+__global__ void __launch_bounds__(256) k_step_thermal_surface(waam_step_fields f, StepConsts s)
+{
+    __shared__ int s_pref[kListSegments + 1];
+    const Dims &d = s.d;
+    CellList l = cell_list(f, s.seg_len, 0);
+    int n = list_prefix(l, d.n(), s_pref);
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        size_t c = list_at(l, s_pref, t);
+        int i, j, k;
+        cell_ijk(c, d, i, j, k);
+        thermal_vof_cell(f, s, i, j, k, c, f.flags_in[c]);
     }
-    float p = s.p.enable_wetting
-                  ? contact_bc_phi(f.phi_tmp, f.flags_in, i, j, k, s.p.cos_t, s.p.sin_t, d)
-                  : f.phi_tmp[c];
-    uint8_t nf = classify_cell(k, s.p.nz_solid, p, f.fl_out[c]);
-    f.phi[c] = p;
-    f.flags_out[c] = nf;
-    if (nf == GAS || p < 0.05f) f.fl_out[c] = 0.0f;
-    else if (nf == SOLID) f.fl_out[c] = 0.0f;
-    if (nf != GAS) time_above(f, s.p, c);
+}
+
+/* update_time_above_T (reference kernels.py:3261-3283) runs after the VOF flag update and
+ * skips cells that are GAS by the updated flags */
+struct TimeAbove {
+    float T, t800, t1100, tsol;
+};
+__device__ __forceinline__ TimeAbove time_above_load(const waam_step_fields &f, size_t c)
+{
+    return TimeAbove{f.T_out[c], f.t800[c], f.t1100[c], f.tsol[c]};
+}
+__device__ __forceinline__ void time_above_store(const waam_step_fields &f,
+                                                 const waam_step_params &p, size_t c,
+                                                 const TimeAbove &t)
+{
+    if (t.T >= p.T800) f.t800[c] = t.t800 + p.dt;
+    if (t.T >= p.T1100) f.t1100[c] = t.t1100 + p.dt;
+    if (t.T >= p.T_s) f.tsol[c] = t.tsol + p.dt;
+}
+
+/* 4 — apply_contact_angle_phi_bc (Jacobi), update_flags_from_phi, sync_phi_liquid_fraction,
+ *     update_time_above_T (reference kernels.py:933-991, 2394-2419, 3484-3504, 3261-3283);
+ *     lists the cells kernel 5 collides */
+__global__ void __launch_bounds__(256) k_step_vof_flags(waam_step_fields f, StepConsts s)
+{
+    const Dims &d = s.d;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0)  // kernel 2 has consumed the
+        cell_list(f, s.seg_len, 0).count(threadIdx.y * blockDim.x + threadIdx.x) = 0;  // list
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int i = blockIdx.z;
+    bool in = k < d.nz && j < d.ny;
+    size_t c = in ? d.idx(i, j, k) : 0;
+    uint8_t nf = GAS;
+    if (in) {
+        /* loads ahead of the stores for the cells that were not GAS (nearly all that stay so) */
+        bool pre = f.flags_in[c] != GAS;
+        TimeAbove ta{};
+        if (pre) ta = time_above_load(f, c);
+        if (!s.p.enable_vof) {
+            nf = f.flags_in[c];
+        } else {
+            float p = s.p.enable_wetting
+                          ? contact_bc_phi(f.phi_tmp, f.flags_in, i, j, k, s.p.cos_t, s.p.sin_t, d)
+                          : f.phi_tmp[c];
+            nf = classify_cell(k, s.p.nz_solid, p, f.fl_out[c]);
+            f.phi[c] = p;
+            if (nf == GAS || p < 0.05f) f.fl_out[c] = 0.0f;
+            else if (nf == SOLID) f.fl_out[c] = 0.0f;
+        }
+        f.flags_out[c] = nf;
+        if (nf != GAS) {
+            if (!pre) ta = time_above_load(f, c);
+            time_above_store(f, s.p, c, ta);
+        }
+    }
+    list_append(in && is_surface_class(nf), c, cell_list(f, s.seg_len, 1), d.n());
 }
 
 /* pull of the reference stream (kernels.py:2991-3010) with the flags of the step it belongs to */
@@ -302,75 +509,75 @@ __device__ __forceinline__ void pull_cell(const float *__restrict__ src, float *
 {
     size_t N = d.n();
     size_t c = d.idx(i, j, k);
+    /* all loads are issued before the first store (src and dst carry no alias information
+     * once inlined) */
+    size_t from[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         int ni = i - ex(q), nj = j - ey(q), nk = k - ez(q);
         ni = ni < 0 ? ni + d.nx : (ni >= d.nx ? ni - d.nx : ni);
         nj = imax(0, imin(nj, d.ny - 1));
         nk = imax(0, imin(nk, d.nz - 1));
-        size_t n = d.idx(ni, nj, nk);
-        float v;
-        if (fc == GAS) v = src[(size_t)q * N + n];
-        else if (sfl[n] == SOLID) v = src[(size_t)opp(q) * N + c];
-        else v = src[(size_t)q * N + n];
-        dst[(size_t)q * N + c] = v;
+        from[q] = (size_t)q * N + d.idx(ni, nj, nk);
     }
+    if (fc != GAS) {
+        /* bounce-back off SOLID neighbours (a GAS destination pulls unconditionally) */
+        uint8_t nfl[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) nfl[q] = sfl[from[q] - (size_t)q * N];
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+            if (nfl[q] == SOLID) from[q] = (size_t)opp(q) * N + c;
+    }
+    float v[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) v[q] = src[from[q]];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) dst[(size_t)q * N + c] = v[q];
 }
 
-/* 6 — body forces, collision, Mach clamp; pending pull stream of the previous step */
-__global__ void __launch_bounds__(256) k_step_collide(waam_step_fields f, StepConsts s)
+/* body forces, collision and Mach clamp of one FLUID/IFACE cell */
+__device__ __forceinline__ void collide_cell(const waam_step_fields &f, const StepConsts &s,
+                                             int i, int j, int k, size_t c)
 {
     const Dims &d = s.d;
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int i = blockIdx.z;
-    if (k >= d.nz || j >= d.ny) return;
     const waam_step_params &p = s.p;
-    size_t c = d.idx(i, j, k);
     size_t N = d.n();
-    uint8_t fc = f.flags_out[c];
-    f.sflags_next[c] = fc;
-    if (fc == SOLID || fc == GAS) {
-        if (p.store_forces && !p.forces_from_fields) {
-            f.Fx[c] = 0.0f;
-            f.Fy[c] = 0.0f;
-            f.Fz[c] = 0.0f;
-        }
-        if (p.pending_stream) {
-            uint8_t fp = f.sflags_prev[c];
-            if (fp != SOLID) pull_cell(f.f_src, f.f_dst, f.sflags_prev, i, j, k, fp, d);
-        }
-        return;
-    }
-    /* forces, reference order coupled_step.py:298-362 */
+    /* every load first (populations, forces, stencils), then the arithmetic */
+    float fin[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) fin[q] = f.f_src[(size_t)q * N + c];
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     if (p.forces_from_fields) {
         fx = f.Fx[c];
         fy = f.Fy[c];
         fz = f.Fz[c];
     }
-    float fl = f.fl_out[c];
+    float fl = f.fl_out[c], T_c = f.T_out[c];
+    ForceStencil fs;
+    load_force_stencil(f.phi, f.T_out, f.flags_out, i, j, k, d, fs);
+    const float *tot = reinterpret_cast<const float *>((char *)f.scratch + kScratchTelemetry);
+    float arc_k = step_arc_k(p, tot);
+    /* forces, reference order coupled_step.py:298-362 */
     float tx, ty, tz;
     if (p.enable_csf &&
-        csf_force(f.phi, f.flags_out, i, j, k, d, p.gamma_lu, p.enable_wetting, p.cos_t, p.sin_t,
-                  tx, ty, tz)) {
+        csf_force(fs, i, j, k, d, p.gamma_lu, p.enable_wetting, p.cos_t, p.sin_t, tx, ty, tz)) {
         fx += tx;
         fy += ty;
         fz += tz;
     }
-    if (marangoni_force(f.T_out, f.phi, fl, i, j, k, d, p.dgamma, tx, ty, tz)) {
+    if (marangoni_force(fs, fl, p.dgamma, tx, ty, tz)) {
         fx += tx;
         fy += ty;
         fz += tz;
     }
     if (p.is_welding) {
         ArcPressure ap = s.ap;
-        const float *tot = reinterpret_cast<const float *>((char *)f.scratch + kScratchTelemetry);
-        ap.arc_k = step_arc_k(p, tot);
-        if (arc_pressure_force(f.phi, i, j, k, d, ap, tz)) fz += tz;
+        ap.arc_k = arc_k;
+        if (arc_pressure_force(fs.phi.c[4], fs.phi.c[5], i, j, k, ap, tz)) fz += tz;
     }
     if (p.enable_hydro && fl > 0.0f) fz += -1.0f * p.g_lu * fl;
-    if (fl > 0.0f) fz += 1.0f * p.g_lu * p.beta * (f.T_out[c] - p.T_l) * fl;
+    if (fl > 0.0f) fz += 1.0f * p.g_lu * p.beta * (T_c - p.T_l) * fl;
     if (p.F_cap > 0.0f) {
         float mag = sqrtf(fx * fx + fy * fy + fz * fz);
         if (mag > p.F_cap) {
@@ -380,14 +587,6 @@ __global__ void __launch_bounds__(256) k_step_collide(waam_step_fields f, StepCo
             fz = fz * sc;
         }
     }
-    if (p.store_forces) {
-        f.Fx[c] = fx;
-        f.Fy[c] = fy;
-        f.Fz[c] = fz;
-    }
-    float fin[Q];
-#pragma unroll
-    for (int q = 0; q < Q; ++q) fin[q] = f.f_src[(size_t)q * N + c];
     CellOut o;
     if (p.collision == 0) mrt_cell(fin, fx, fy, fz, fl, p.omega_s, p.omega_b, p.C_darcy, o);
     else srt_cell(fin, fx, fy, fz, fl, p.omega_s, p.C_darcy, o);
@@ -407,67 +606,115 @@ __global__ void __launch_bounds__(256) k_step_collide(waam_step_fields f, StepCo
     f.ux[c] = u;
     f.uy[c] = v;
     f.uz[c] = w;
+    if (p.store_forces) {
+        f.Fx[c] = fx;
+        f.Fy[c] = fy;
+        f.Fz[c] = fz;
+    }
 }
 
-/* 8 — remelt_hot_solid_scalar, solidify_trailing_pool_scalar, solidify_cooled_metal
- *     (reference kernels.py:2489-2520, 2573-2617, 2422-2450), in that order per cell */
-__global__ void __launch_bounds__(256) k_step_freeze(waam_step_fields f, StepConsts s)
+/* 5 — forces + collision of the FLUID/IFACE cells listed by kernel 4 */
+__global__ void __launch_bounds__(256) k_step_collide(waam_step_fields f, StepConsts s)
+{
+    __shared__ int s_pref[kListSegments + 1];
+    const Dims &d = s.d;
+    CellList l = cell_list(f, s.seg_len, 1);
+    int n = list_prefix(l, d.n(), s_pref);
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        size_t c = list_at(l, s_pref, t);
+        int i, j, k;
+        cell_ijk(c, d, i, j, k);
+        collide_cell(f, s, i, j, k, c);
+    }
+}
+
+/* remelt_hot_solid_scalar, solidify_trailing_pool_scalar, solidify_cooled_metal (reference
+ * kernels.py:2489-2520, 2573-2617, 2422-2450) in that order for one non-GAS cell, on loaded
+ * values; a field is stored exactly when one of the three writes it, with its last value */
+__device__ __forceinline__ void freeze_cell(const waam_step_fields &f, const StepConsts &s,
+                                            int i, int j, int k, size_t c, uint8_t flag, float T,
+                                            float phi, float fl, float arc_k)
+{
+    const waam_step_params &p = s.p;
+    bool wH = false, wT = false, wfl = false, wphi = false, wflag = false, wu = false;
+    float H = 0.0f;
+    if (flag == SOLID && T > p.T_s + 5.0f) {
+        float H_sol = p.cp_rho * p.T_s;
+        if (T >= p.T_l) {
+            fl = 1.0f;
+            H = p.cp_rho * p.T_l + p.L_rho + p.cp_rho * (T - p.T_l);
+        } else {
+            fl = (T - p.T_s) / (p.T_l - p.T_s + 1e-6f);
+            H = H_sol + fl * p.L_rho;
+        }
+        phi = 1.0f;
+        flag = FLUID;
+        wH = wfl = wphi = wflag = true;
+    }
+    if (p.enable_trailing && !(phi < 0.85f || fl < 0.05f)) {
+        float di = (float)i - p.arc_i, dj = (float)j - p.arc_j, dk = (float)k - arc_k;
+        float proj = di * p.dir_x + dj * p.dir_y + dk * p.dir_z;
+        if (!(proj > -p.lookback) && !(T >= p.T_freeze)) {
+            H = p.cp_rho * (p.T_s - 1.0f);
+            T = p.T_s - 1.0f;
+            fl = 0.0f;
+            phi = 1.0f;
+            flag = SOLID;
+            wH = wT = wfl = wphi = wflag = wu = true;
+        }
+    }
+    if (!(phi < 0.85f) && fl < 0.02f && T < p.T_s) {
+        flag = SOLID;
+        phi = 1.0f;
+        fl = 0.0f;
+        wflag = wphi = wfl = true;
+        if (p.zero_velocity == 1) wu = true;
+    }
+    if (wH) f.H_out[c] = H;
+    if (wT) f.T_out[c] = T;
+    if (wfl) f.fl_out[c] = fl;
+    if (wphi) f.phi[c] = phi;
+    if (wflag) f.flags_out[c] = flag;
+    if (wu) {
+        f.ux[c] = 0.0f;
+        f.uy[c] = 0.0f;
+        f.uz[c] = 0.0f;
+    }
+}
+
+/* 7 — every cell: stream flags; cells the collision skipped: force reset and the previous
+ *     step's pull; then the freeze/remelt updates of every non-GAS cell (cell-local; the pull
+ *     reads only populations and stream flags, which they do not touch) */
+__global__ void __launch_bounds__(256) k_step_stream(waam_step_fields f, StepConsts s)
 {
     const Dims &d = s.d;
+    const waam_step_params &p = s.p;
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     int j = blockIdx.y * blockDim.y + threadIdx.y;
     int i = blockIdx.z;
     if (k >= d.nz || j >= d.ny) return;
-    const waam_step_params &p = s.p;
     size_t c = d.idx(i, j, k);
-    uint8_t flag = f.flags_out[c];
-    if (flag == GAS) return;
-    float T = f.T_out[c];
-    if (flag == SOLID && T > p.T_s + 5.0f) {
-        float H_sol = p.cp_rho * p.T_s, fl;
-        if (T >= p.T_l) {
-            fl = 1.0f;
-            f.H_out[c] = p.cp_rho * p.T_l + p.L_rho + p.cp_rho * (T - p.T_l);
-        } else {
-            fl = (T - p.T_s) / (p.T_l - p.T_s + 1e-6f);
-            f.H_out[c] = H_sol + fl * p.L_rho;
-        }
-        f.fl_out[c] = fl;
-        f.phi[c] = 1.0f;
-        flag = FLUID;
-        f.flags_out[c] = FLUID;
+    uint8_t fc = f.flags_out[c];
+    uint8_t fp = f.sflags_prev[c];
+    bool surface = is_surface_class(fc);
+    bool freeze = p.enable_freeze && fc != GAS;
+    float T = 0.0f, phi = 0.0f, fl = 0.0f;
+    if (freeze) {
+        T = f.T_out[c];
+        phi = f.phi[c];
+        fl = f.fl_out[c];
     }
-    if (p.enable_trailing) {
-        float phi = f.phi[c], fl = f.fl_out[c];
-        if (!(phi < 0.85f || fl < 0.05f)) {
-            const float *tot = reinterpret_cast<const float *>((char *)f.scratch + kScratchTelemetry);
-            float ak = step_arc_k(p, tot);
-            float di = (float)i - p.arc_i, dj = (float)j - p.arc_j, dk = (float)k - ak;
-            float proj = di * p.dir_x + dj * p.dir_y + dk * p.dir_z;
-            if (!(proj > -p.lookback) && !(T >= p.T_freeze)) {
-                f.H_out[c] = p.cp_rho * (p.T_s - 1.0f);
-                T = p.T_s - 1.0f;
-                f.T_out[c] = T;
-                f.fl_out[c] = 0.0f;
-                f.phi[c] = 1.0f;
-                flag = SOLID;
-                f.flags_out[c] = SOLID;
-                f.ux[c] = 0.0f;
-                f.uy[c] = 0.0f;
-                f.uz[c] = 0.0f;
-            }
-        }
+    if (!surface && p.pending_stream && fp != SOLID)
+        pull_cell(f.f_src, f.f_dst, f.sflags_prev, i, j, k, fp, d);
+    f.sflags_next[c] = fc;
+    if (!surface && p.store_forces && !p.forces_from_fields) {
+        f.Fx[c] = 0.0f;
+        f.Fy[c] = 0.0f;
+        f.Fz[c] = 0.0f;
     }
-    if (f.phi[c] < 0.85f) return;
-    if (f.fl_out[c] < 0.02f && T < p.T_s) {
-        f.flags_out[c] = SOLID;
-        f.phi[c] = 1.0f;
-        f.fl_out[c] = 0.0f;
-        if (p.zero_velocity == 1) {
-            f.ux[c] = 0.0f;
-            f.uy[c] = 0.0f;
-            f.uz[c] = 0.0f;
-        }
+    if (freeze) {
+        const float *tot = reinterpret_cast<const float *>((char *)f.scratch + kScratchTelemetry);
+        freeze_cell(f, s, i, j, k, c, fc, T, phi, fl, step_arc_k(p, tot));
     }
 }
 
@@ -483,6 +730,7 @@ __global__ void k_stream_pending(const float *__restrict__ src, float *__restric
     if (fp != SOLID) pull_cell(src, dst, sfl, i, j, k, fp, d);
 }
 
+/* 6 — advect_tracers (reference kernels.py:3206-3255) */
 __global__ void k_advect_tracers_step(float *__restrict__ pos, int32_t *__restrict__ active,
                                       const float *__restrict__ ux, const float *__restrict__ uy,
                                       const float *__restrict__ uz, const float *__restrict__ f_l,
@@ -516,31 +764,26 @@ static void arc_box(StepConsts &s)
 {
     const waam_step_params &p = s.p;
     /* exp(x) is exactly zero for x < -103.98; outside this box x < -110 for every cell */
-    double ri, rj, rk;
+    double ri, rj;
     if (p.arc_kind == 0) {
         double inv = (double)s.arc.inv2s2;
         ri = rj = std::sqrt(110.0 / inv);
-        rk = 2.0 * ri;
     } else {
         double a = std::max((double)p.g_af, (double)p.g_ar);
         ri = std::sqrt(110.0 / 3.0 * (a * a + 1e-6)) / std::max(std::fabs((double)p.g_sgn), 1e-30);
         rj = std::sqrt(110.0 / 3.0 * ((double)p.g_b * p.g_b + 1e-6));
-        rk = std::sqrt(110.0 / 3.0 * ((double)p.g_c * p.g_c + 1e-6));
     }
     auto lo = [](double a, double r) { return (int)std::floor(a - r) - 2; };
     auto hi = [](double a, double r) { return (int)std::ceil(a + r) + 2; };
+    if (!(ri < 1e6) || !(rj < 1e6)) {
+        s.box_i0 = s.box_j0 = -(1 << 30);
+        s.box_i1 = s.box_j1 = (1 << 30);
+        return;
+    }
     s.box_i0 = lo(p.arc_i, ri);
     s.box_i1 = hi(p.arc_i, ri);
     s.box_j0 = lo(p.arc_j, rj);
     s.box_j1 = hi(p.arc_j, rj);
-    /* arc_k is only known on the device; bound it by the grid (k spans the whole column) */
-    (void)rk;
-    s.box_k0 = -1;
-    s.box_k1 = s.d.nz + 1;
-    if (!(ri < 1e6) || !(rj < 1e6)) {
-        s.box_i0 = s.box_j0 = -(1 << 30);
-        s.box_i1 = s.box_j1 = (1 << 30);
-    }
 }
 
 /* HIP-event timing of one kernel kind (waam_timing_begin / waam_timing_end) */
@@ -580,10 +823,16 @@ int waam_step_fused(const waam_step_params *pp, const waam_step_fields *ff, void
                               {f.f_src, f.f_dst, f.rho, f.ux, f.uy, f.uz, f.H_in, f.H_out, f.T_in,
                                f.T_out, f.fl_in, f.fl_out, f.phi, f.phi_tmp, f.flags_in,
                                f.flags_out, f.sflags_prev, f.sflags_next, f.Fx, f.Fy, f.Fz,
-                               f.T_max, f.T_prev, f.dT_dt, f.t800, f.t1100, f.tsol, f.scratch}));
+                               f.T_max, f.T_prev, f.dT_dt, f.t800, f.t1100, f.tsol,
+                               f.list_thermal, f.list_collide, f.scratch}));
     if (f.f_src == f.f_dst || f.H_in == f.H_out || f.T_in == f.T_out || f.fl_in == f.fl_out ||
-        f.flags_in == f.flags_out || f.sflags_prev == f.sflags_next || f.phi == f.phi_tmp)
+        f.flags_in == f.flags_out || f.sflags_prev == f.sflags_next || f.phi == f.phi_tmp ||
+        f.list_thermal == f.list_collide)
         return set_error(WAAM_ERR_INVALID, "waam_step_fused: an in/out buffer pair aliases");
+    Dims d{f.nx, f.ny, f.nz};
+    if (d.n() > (size_t)INT_MAX)
+        return set_error(WAAM_ERR_INVALID, "waam_step_fused: %zu cells exceed the int32 cell lists",
+                         d.n());
     if (p.col_i < 0 || p.col_i >= f.nx || p.col_j < 0 || p.col_j >= f.ny)
         return set_error(WAAM_ERR_INVALID, "waam_step_fused: arc column outside the grid");
     if (f.max_tracers > 0 && (!f.tracer_pos || !f.tracer_active))
@@ -591,7 +840,7 @@ int waam_step_fused(const waam_step_params *pp, const waam_step_fields *ff, void
 
     StepConsts s{};
     s.p = p;
-    s.d = Dims{f.nx, f.ny, f.nz};
+    s.d = d;
     s.arc.kind = p.arc_kind;
     s.arc.arc_i = p.arc_i;
     s.arc.arc_j = p.arc_j;
@@ -616,39 +865,34 @@ int waam_step_fused(const waam_step_params *pp, const waam_step_fields *ff, void
     s.ap.inv2s2 = 1.0f / (2.0f * p.p_sigma * p.p_sigma + 1e-6f);
     s.ap.F_peak_lu = p.p_pa / (p.rho_phys * p.dx) * (p.dt * p.dt) / p.dx;
     arc_box(s);
+    dim3 g3 = grid3(d);
+    size_t seg = (d.n() + kListSegments - 1) / kListSegments;
+    s.seg_len = (int)((seg + 255) / 256 * 256);
 
     hipStream_t st = as_stream(stream);
-    const Dims &d = s.d;
-    bool arc_sums = p.is_welding && !p.arc_applied;
-    if (!p.use_arc_k_override || arc_sums)
-        timed_launch(WAAM_K_ARC_PARTIALS, st, [&] {
-            hipLaunchKernelGGL(k_step_arc_partials, dim3(arc_sums ? kReduceBlocks : 1),
-                               dim3(kBlock), 0, st, f, s, arc_sums ? 1 : 0);
-        });
-    if (arc_sums)
-        timed_launch(WAAM_K_ARC_TOTAL, st, [&] {
-            hipLaunchKernelGGL(k_step_arc_total, dim3(1), dim3(kBlock), 0, st, f, s);
-        });
+    int do_sums = p.is_welding && !p.arc_applied ? 1 : 0;
+    timed_launch(WAAM_K_BEGIN, st, [&] {
+        hipLaunchKernelGGL(k_step_begin, dim3(do_sums ? kReduceBlocks : 1), dim3(kBlock), 0, st, f,
+                           s, do_sums);
+    });
     timed_launch(WAAM_K_THERMAL, st,
-                 [&] { hipLaunchKernelGGL(k_step_thermal, grid3(d), block3(), 0, st, f, s); });
-    if (p.enable_vof)
-        timed_launch(WAAM_K_VOF_ADVECT, st, [&] {
-            hipLaunchKernelGGL(k_step_vof_advect, grid3(d), block3(), 0, st, f, d);
-        });
+                 [&] { hipLaunchKernelGGL(k_step_thermal, g3, block3(), 0, st, f, s); });
+    timed_launch(WAAM_K_THERMAL_SURFACE, st, [&] {
+        hipLaunchKernelGGL(k_step_thermal_surface, dim3(kListBlocks), dim3(kBlock), 0, st, f, s);
+    });
     timed_launch(WAAM_K_VOF_FLAGS, st,
-                 [&] { hipLaunchKernelGGL(k_step_vof_flags, grid3(d), block3(), 0, st, f, s); });
-    timed_launch(WAAM_K_COLLIDE, st,
-                 [&] { hipLaunchKernelGGL(k_step_collide, grid3(d), block3(), 0, st, f, s); });
+                 [&] { hipLaunchKernelGGL(k_step_vof_flags, g3, block3(), 0, st, f, s); });
+    timed_launch(WAAM_K_COLLIDE, st, [&] {
+        hipLaunchKernelGGL(k_step_collide, dim3(kListBlocks), dim3(kBlock), 0, st, f, s);
+    });
     if (f.max_tracers > 0)
         timed_launch(WAAM_K_TRACERS, st, [&] {
             hipLaunchKernelGGL(k_advect_tracers_step, dim3(grid_for((size_t)f.max_tracers)),
                                dim3(kBlock), 0, st, f.tracer_pos, f.tracer_active, f.ux, f.uy,
                                f.uz, f.fl_out, f.flags_out, p.dx, f.max_tracers, d);
         });
-    if (p.enable_freeze)
-        timed_launch(WAAM_K_FREEZE, st, [&] {
-            hipLaunchKernelGGL(k_step_freeze, grid3(d), block3(), 0, st, f, s);
-        });
+    timed_launch(WAAM_K_STREAM, st,
+                 [&] { hipLaunchKernelGGL(k_step_stream, g3, block3(), 0, st, f, s); });
     return check_launch("waam_step_fused");
 }
 

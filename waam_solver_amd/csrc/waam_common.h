@@ -103,7 +103,11 @@ constexpr size_t kScratchTelemetry = kScratchPartials + 2 * kReduceBlocks * size
 constexpr size_t kScratchTelemetryParts = kScratchTelemetry + 64;       /* 8*kReduceBlocks doubles */
 constexpr size_t kDepScratchOffset = kScratchTelemetryParts + 8 * kReduceBlocks * sizeof(double);
 constexpr size_t kDepMaxCells = 512 * 1024;                             /* deposition marks */
-constexpr size_t kScratchBytes = kDepScratchOffset + kDepMaxCells;
+/* fused step: per-segment lengths of its two cell lists, one counter per 256-byte line */
+constexpr int kListSegments = 256;
+constexpr size_t kCounterStride = 256;
+constexpr size_t kScratchListCounters = kDepScratchOffset + kDepMaxCells;
+constexpr size_t kScratchBytes = kScratchListCounters + 2 * kListSegments * kCounterStride;
 
 /* Compensated addition (Knuth TwoSum): s + e == a + b exactly. */
 __device__ __forceinline__ void two_sum(double a, double b, double &s, double &e)

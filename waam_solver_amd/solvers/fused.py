@@ -31,7 +31,8 @@ class StepFields(ctypes.Structure):
         "f_src", "f_dst", "rho", "ux", "uy", "uz", "H_in", "H_out", "T_in", "T_out", "fl_in",
         "fl_out", "phi", "phi_tmp", "flags_in", "flags_out", "sflags_prev", "sflags_next", "Fx",
         "Fy", "Fz", "T_max", "T_prev", "dT_dt", "t800", "t1100", "tsol", "tracer_pos",
-        "tracer_active", "scratch")] + [(n, _i32) for n in ("max_tracers", "nx", "ny", "nz")]
+        "tracer_active", "list_thermal", "list_collide", "scratch")] + [
+            (n, _i32) for n in ("max_tracers", "nx", "ny", "nz")]
 
 
 _INT_PARAMS = ("is_welding", "collision", "arc_kind", "arc_applied", "pending_stream",
@@ -71,8 +72,14 @@ class FusedStepper:
         if not hasattr(g, "sflags"):
             g.sflags = DeviceField.zeros(s3, torch.uint8, g.device, np_dtype=np.int32)
             g.sflags_tmp = DeviceField.zeros(s3, torch.uint8, g.device, np_dtype=np.int32)
+        if not hasattr(g, "list_thermal"):
+            # compacted FLUID/IFACE cell indices written and consumed inside each fused step
+            g.list_thermal = DeviceField.zeros((g.n_cells,), torch.int32, g.device)
+            g.list_collide = DeviceField.zeros((g.n_cells,), torch.int32, g.device)
         self.pending = False
         self._arc_out = (ctypes.c_float * 3)()
+        # the swapped pairs alternate between two pointer sets: keep one struct per set
+        self._fields_cache: dict[tuple, StepFields] = {}
 
     # ------------------------------------------------------------------ deferred stream
     def flush(self) -> None:
@@ -96,6 +103,14 @@ class FusedStepper:
     # ------------------------------------------------------------------ step
     def _fields(self) -> StepFields:
         g = self.twin.grid
+        key = (id(g.H), id(g.T), id(g.f_l), id(g.flags), id(g.sflags), id(g.f_src), id(g.phi))
+        sf = self._fields_cache.get(key)
+        if sf is None:
+            sf = self._fields_cache[key] = self._build_fields()
+        return sf
+
+    def _build_fields(self) -> StepFields:
+        g = self.twin.grid
         sf = StepFields()
         for name, fld in (("f_src", g.f_src), ("f_dst", g.f_dst), ("rho", g.rho), ("ux", g.ux),
                           ("uy", g.uy), ("uz", g.uz), ("H_in", g.H), ("H_out", g.H_tmp),
@@ -107,6 +122,7 @@ class FusedStepper:
                           ("T_prev", g.T_prev), ("dT_dt", g.dT_dt), ("t800", g.time_above_800_s),
                           ("t1100", g.time_above_1100_s), ("tsol", g.time_above_solidus_s),
                           ("tracer_pos", g.porosity_pos), ("tracer_active", g.porosity_active),
+                          ("list_thermal", g.list_thermal), ("list_collide", g.list_collide),
                           ("scratch", g.scratch)):
             setattr(sf, name, P(fld))
         sf.max_tracers, sf.nx, sf.ny, sf.nz = g.max_tracers, g.nx, g.ny, g.nz
