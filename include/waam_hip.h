@@ -5,16 +5,19 @@
  * The reference (THEDARKDEACON/waam_solver, Python + Taichi) has no FFI: its hot path is a set
  * of @ti.kernel functions called positionally from Python with Taichi fields. Each entry point
  * below replaces one of those kernels (reference file:line given per entry) and keeps its
- * positional parameter order, with three mechanical changes:
+ * positional parameter order, with four mechanical changes:
  *   - a field argument becomes a device pointer (float* or uint8_t* for flags);
  *   - FLAG_* arguments are dropped: flag values are fixed by this ABI (WAAM_FLAG_*, the
  *     reference's grid.py:63-66 values);
+ *   - the grid sizes nx, ny, nz are followed by the field layout (WAAM_LAYOUT_*);
  *   - 0-d scratch fields (norm_buf, vol_acc, ...) become device pointers, and every entry
  *     takes a trailing hipStream_t (passed as void*, NULL = the null stream).
  *
  * Conventions
- *   - Layout: scalar fields are C-order (nx, ny, nz), k fastest; populations are
- *     (19, nx, ny, nz); flags are uint8 with FLUID=0, SOLID=1, GAS=2, IFACE=4.
+ *   - Layout: every field of a grid uses one layout. WAAM_LAYOUT_XYZ is the reference's
+ *     C-order (nx, ny, nz), k fastest; WAAM_LAYOUT_ZYX is C-order (nz, ny, nx), i fastest.
+ *     Populations are 19 such arrays back to back. Flags are uint8 with FLUID=0, SOLID=1,
+ *     GAS=2, IFACE=4.
  *   - All launches are asynchronous on the given stream. Entries that return a host value
  *     (*_sync) synchronise that stream.
  *   - Return 0 on success or a negative WAAM_ERR_* code; waam_last_error() gives a message
@@ -32,9 +35,10 @@
 extern "C" {
 #endif
 
-#define WAAM_ABI_VERSION 1
+#define WAAM_ABI_VERSION 2
 
 enum { WAAM_FLAG_FLUID = 0, WAAM_FLAG_SOLID = 1, WAAM_FLAG_GAS = 2, WAAM_FLAG_IFACE = 4 };
+enum { WAAM_LAYOUT_XYZ = 0, WAAM_LAYOUT_ZYX = 1 };
 enum { WAAM_OK = 0, WAAM_ERR_INVALID = -1, WAAM_ERR_HIP = -2 };
 
 int waam_abi_version(void);
@@ -47,23 +51,23 @@ int64_t waam_scratch_bytes(void);
 int waam_init_grid(float *f, float *rho, float *ux, float *uy, float *uz, float *T, float *H,
                    float *f_l, float *phi, uint8_t *flags, float T_ambient, float rho0,
                    float cp_rho, int nz_solid, int plate_i0, int plate_i1, int plate_j0,
-                   int plate_j1, int nx, int ny, int nz, void *stream);
+                   int plate_j1, int nx, int ny, int nz, int layout, void *stream);
 
 /* ---- arc heat: inject_arc_heat kernels.py:443-510, inject_goldak_heat :548-622 --------- */
 /* norm_out (device float, may be NULL) receives the weight total; scratch is the workspace. */
 int waam_inject_arc_heat(float *H, const uint8_t *flags, const float *f_l, float *norm_out,
                          float arc_i, float arc_j, float arc_k, float Q_w, float sigma, float dt,
                          float dx3, float eta, float penetration_cells, int enable_surface_weight,
-                         int nx, int ny, int nz, void *scratch, void *stream);
+                         int nx, int ny, int nz, int layout, void *scratch, void *stream);
 int waam_inject_goldak_heat(float *H, const uint8_t *flags, const float *f_l, float *norm_out,
                             float arc_i, float arc_j, float arc_k, float Q_w, float dt,
                             float dx3, float eta, float travel_sign, float ff, float fr,
                             float a_front, float a_rear, float b_axis, float c_axis,
                             float penetration_cells, int enable_surface_weight, int nx, int ny,
-                            int nz, void *scratch, void *stream);
+                            int nz, int layout, void *scratch, void *stream);
 /* surface_height_at kernels.py:625-641 (max qualifying k; result as float in *out). */
 int waam_surface_height_at(const float *phi, const uint8_t *flags, float *out, int i0, int j0,
-                           int nz_fallback, int nx, int ny, int nz, void *stream);
+                           int nz_fallback, int nx, int ny, int nz, int layout, void *stream);
 
 /* ---- thermal ----------------------------------------------------------------------------- */
 /* clamp_enthalpy_ceiling_scalar kernels.py:1953-1984 */
@@ -78,12 +82,12 @@ int waam_clamp_enthalpy_floor_scalar(float *H, float cp_rho, const uint8_t *flag
 int waam_advect_diffuse_temperature(const float *H_in, float *H_out, const float *T,
                                     const float *ux, const float *uy, const float *uz,
                                     const uint8_t *flags, float alpha_lu, float dt,
-                                    float cp_rho, int nx, int ny, int nz, void *stream);
+                                    float cp_rho, int nx, int ny, int nz, int layout, void *stream);
 /* apply_thermal_boundary_losses kernels.py:2030-2082 */
 int waam_apply_thermal_boundary_losses(float *H, const float *T, const uint8_t *flags,
                                        float T_amb, float h_conv, float eps_rad,
                                        int enable_conv, int enable_rad, float cp_rho, float dt,
-                                       float dx, float sigma_sb, int nx, int ny, int nz,
+                                       float dx, float sigma_sb, int nx, int ny, int nz, int layout,
                                        void *stream);
 /* update_phase kernels.py:1519-1556 */
 int waam_update_phase(const float *H, float *T, float *f_l, float cp_rho, float L_rho,
@@ -102,17 +106,17 @@ int waam_update_time_above_T(const float *T, const uint8_t *flags, float *t800, 
 /* ---- VOF ----------------------------------------------------------------------------------- */
 /* advect_phi kernels.py:2320-2361 (+ _vof_face_flux :2275-2317) */
 int waam_advect_phi(float *phi_dst, const float *phi_src, const float *ux, const float *uy,
-                    const float *uz, const uint8_t *flags, int nx, int ny, int nz, void *stream);
+                    const float *uz, const uint8_t *flags, int nx, int ny, int nz, int layout, void *stream);
 /* reinitialize_phi kernels.py:2364-2391 */
 int waam_reinitialize_phi(float *phi, const uint8_t *flags, int64_t n, void *stream);
 /* apply_contact_angle_phi_bc kernels.py:933-991, Jacobi form (phi_in != phi_out); cos/sin of
  * the contact angle are passed precomputed. */
 int waam_apply_contact_angle_phi_bc(const float *phi_in, float *phi_out, const uint8_t *flags,
-                                    float cos_t, float sin_t, int nx, int ny, int nz,
+                                    float cos_t, float sin_t, int nx, int ny, int nz, int layout,
                                     void *stream);
 /* update_flags_from_phi kernels.py:2394-2419 */
 int waam_update_flags_from_phi(const float *phi, const float *f_l, uint8_t *flags, int nz_solid,
-                               int nx, int ny, int nz, void *stream);
+                               int nx, int ny, int nz, int layout, void *stream);
 /* sync_phi_liquid_fraction kernels.py:3484-3504 */
 int waam_sync_phi_liquid_fraction(const float *phi, float *f_l, const uint8_t *flags, int64_t n,
                                   void *stream);
@@ -130,20 +134,20 @@ int waam_solidify_trailing_pool_scalar(float *T, float *H, float *f_l, float *ph
                                        float cp_rho, float arc_i, float arc_j, float arc_k,
                                        float dir_x, float dir_y, float dir_z,
                                        float lookback_cells, float T_freeze, float T_solidus,
-                                       int nx, int ny, int nz, void *stream);
+                                       int nx, int ny, int nz, int layout, void *stream);
 
 /* ---- forces ---------------------------------------------------------------------------- */
 /* compute_csf_tension kernels.py:858-926 */
 int waam_compute_csf_tension(const float *phi, const uint8_t *flags, float *Fx, float *Fy,
                              float *Fz, float gamma_lu, int enable_wetting, float cos_t,
-                             float sin_t, int nx, int ny, int nz, void *stream);
+                             float sin_t, int nx, int ny, int nz, int layout, void *stream);
 /* compute_curvature_field kernels.py:3353-3376 */
 int waam_compute_curvature_field(const float *phi, const uint8_t *flags, float *kappa_out,
-                                 int nx, int ny, int nz, void *stream);
+                                 int nx, int ny, int nz, int layout, void *stream);
 /* compute_marangoni_force kernels.py:2158-2217 */
 int waam_compute_marangoni_force(const float *T, const float *phi, const float *f_l, float *Fx,
                                  float *Fy, float *Fz, const uint8_t *flags, float dgamma_dT,
-                                 int nx, int ny, int nz, void *stream);
+                                 int nx, int ny, int nz, int layout, void *stream);
 /* add_buoyancy kernels.py:2729-2753 */
 int waam_add_buoyancy(const float *T, float *Fz, const float *f_l, const uint8_t *flags,
                       float g_lu, float beta, float T_ref, float rho_ref, int64_t n,
@@ -154,7 +158,7 @@ int waam_add_hydrostatic_gravity(float *Fz, const float *f_l, const uint8_t *fla
 /* apply_arc_pressure kernels.py:3038-3083 */
 int waam_apply_arc_pressure(float *Fz, const uint8_t *flags, const float *phi, float arc_i,
                             float arc_j, float arc_k, float sigma, float pressure_pa, float dt,
-                            float dx, float rho_ref, int nx, int ny, int nz, void *stream);
+                            float dx, float rho_ref, int nx, int ny, int nz, int layout, void *stream);
 /* clamp_body_force_magnitude kernels.py:3286-3309 */
 int waam_clamp_body_force_magnitude(float *Fx, float *Fy, float *Fz, const uint8_t *flags,
                                     float F_max_lu, int64_t n, void *stream);
@@ -162,27 +166,27 @@ int waam_clamp_body_force_magnitude(float *Fx, float *Fy, float *Fz, const uint8
 int waam_apply_droplet_impact_pressure(float *Fz, const uint8_t *flags, const float *phi,
                                        const float *f_l, float arc_i, float arc_j, float arc_k,
                                        float drop_radius, float impact_pa, float dt, float dx,
-                                       float rho_ref, int nx, int ny, int nz, void *stream);
+                                       float rho_ref, int nx, int ny, int nz, int layout, void *stream);
 
 /* ---- LBM ----------------------------------------------------------------------------------- */
 /* collide_mrt cumulant_kernel.py:28-209 */
 int waam_collide_mrt(const float *f_src, float *f_dst, float *rho, float *ux, float *uy,
                      float *uz, const float *Fx, const float *Fy, const float *Fz,
                      const float *f_l, const uint8_t *flags, float omega_s, float omega_b,
-                     float C_darcy, int nx, int ny, int nz, void *stream);
+                     float C_darcy, int nx, int ny, int nz, int layout, void *stream);
 /* collide_srt kernels.py:2778-2885 */
 int waam_collide_srt(const float *f_src, float *f_dst, float *rho, float *ux, float *uy,
                      float *uz, const float *Fx, const float *Fy, const float *Fz,
                      const float *f_l, const uint8_t *flags, float tau, float omega,
-                     float dt_lu, float C_darcy, int nx, int ny, int nz, void *stream);
+                     float dt_lu, float C_darcy, int nx, int ny, int nz, int layout, void *stream);
 /* collide_srt_variable_tau kernels.py:2888-2965 */
 int waam_collide_srt_variable_tau(const float *f_src, float *f_dst, float *rho, float *ux,
                                   float *uy, float *uz, const float *Fx, const float *Fy,
                                   const float *Fz, const float *f_l, const uint8_t *flags,
-                                  const float *tau_field, float C_darcy, int nx, int ny, int nz,
+                                  const float *tau_field, float C_darcy, int nx, int ny, int nz, int layout,
                                   void *stream);
 /* stream kernels.py:2971-3010 (pull, x periodic, y/z clamped, bounce-back) */
-int waam_stream(const float *f_src, float *f_dst, const uint8_t *flags, int nx, int ny, int nz,
+int waam_stream(const float *f_src, float *f_dst, const uint8_t *flags, int nx, int ny, int nz, int layout,
                 void *stream);
 /* clamp_velocity_mach kernels.py:3312-3335 */
 int waam_clamp_velocity_mach(float *ux, float *uy, float *uz, const uint8_t *flags,
@@ -195,7 +199,7 @@ int waam_feed_wire_surface(float *f_src, uint8_t *flags, float *f_l, float *phi,
                            float *T, float *rho, float arc_i, float arc_j, float arc_k,
                            float footprint_r, float droplet_radius, float target_vol,
                            float T_drop, float cp_rho, float L_rho, float rho0, float *vol_acc,
-                           float *real_acc, float cell_vol, int nx, int ny, int nz,
+                           float *real_acc, float cell_vol, int nx, int ny, int nz, int layout,
                            void *scratch, void *stream);
 /* The droplet placement loop of solvers/coupled_step.py:126-151 (up to 10 feed_wire_surface
  * calls, footprint growing by 1.5 cells to at most 16) in one launch, with no host round trip.
@@ -205,13 +209,13 @@ int waam_deposit_droplet(float *f_src, uint8_t *flags, float *f_l, float *phi, f
                          float *T, float *rho, float arc_i, float arc_j, float arc_k,
                          double footprint_r, float droplet_radius, double drop_vol,
                          float T_drop, float cp_rho, float L_rho, float rho0, float *vol_acc,
-                         float *real_acc, float cell_vol, float *result, int nx, int ny, int nz,
+                         float *real_acc, float cell_vol, float *result, int nx, int ny, int nz, int layout,
                          void *scratch, void *stream);
 /* feed_wire_momentum_impact kernels.py:1176-1233 */
 int waam_feed_wire_momentum_impact(float *f_src, const float *rho, float *ux, float *uy,
                                    float *uz, const uint8_t *flags, const float *f_l,
                                    float arc_i, float arc_j, float arc_k, float droplet_radius,
-                                   float vx_lu, float vy_lu, float vz_lu, int nx, int ny, int nz,
+                                   float vx_lu, float vy_lu, float vz_lu, int nx, int ny, int nz, int layout,
                                    void *stream);
 /* inject_tracers kernels.py:3182-3204 (counter-hash RNG in place of ti.random) */
 int waam_inject_tracers(float *pos, int32_t *active, int32_t *head, int max_tracers,
@@ -220,13 +224,13 @@ int waam_inject_tracers(float *pos, int32_t *active, int32_t *head, int max_trac
 /* advect_tracers kernels.py:3206-3255 */
 int waam_advect_tracers(float *pos, int32_t *active, const float *ux, const float *uy,
                         const float *uz, const float *f_l, const uint8_t *flags, float dx,
-                        float dt, int max_tracers, int nx, int ny, int nz, void *stream);
+                        float dt, int max_tracers, int nx, int ny, int nz, int layout, void *stream);
 
 /* ---- telemetry: telemetry_pool_reduce kernels.py:3424-3468 ------------------------------- */
 /* out (device, 8 doubles): n_liquid, T_peak, cool, u2_max, i_sum, i_min, i_max, T_global. */
 int waam_telemetry_pool_reduce(const float *T, const float *f_l, const float *phi,
                                const float *dT_dt, const float *ux, const float *uy,
-                               const float *uz, double *out, int nx, int ny, int nz,
+                               const float *uz, double *out, int nx, int ny, int nz, int layout,
                                void *scratch, void *stream);
 
 /* ---- the fused step ------------------------------------------------------------------------
@@ -251,6 +255,7 @@ typedef struct waam_step_fields {
     int32_t *list_thermal, *list_collide;
     void *scratch;
     int32_t max_tracers, nx, ny, nz;
+    int32_t layout; /* WAAM_LAYOUT_XYZ or WAAM_LAYOUT_ZYX, shared by every field */
 } waam_step_fields;
 
 typedef struct waam_step_params {
@@ -279,7 +284,7 @@ int waam_step_fused(const waam_step_params *p, const waam_step_fields *f, void *
 /* Materialise a deferred stream: f_dst = pull(f_src) on cells not SOLID in sflags (the flags the
  * step that deferred it saw). */
 int waam_stream_pending(const float *f_src, float *f_dst, const uint8_t *sflags, int nx, int ny,
-                        int nz, void *stream);
+                        int nz, int layout, void *stream);
 /* The step's arc weight total, energy per unit weight and arc column height, as written into
  * the workspace by the last waam_step_fused (three device floats, in that order). Blocking. */
 int waam_step_arc_readback(const void *scratch, float *host_out3);

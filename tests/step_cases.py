@@ -53,7 +53,7 @@ def torch_x(twin_dx: float, nx: int, step: int, dt: float) -> float:
     return (nx // 3) * twin_dx + 8e-3 * step * dt
 
 
-def build_pair(name: str, mode: str = "jacobi", engine: str = "fused"):
+def build_pair(name: str, mode: str = "jacobi", engine: str = "fused", layout: int | None = None):
     """(product twin, oracle twin), both reset and seeded identically."""
     from oracle.oracle import Material, OracleTwin
     from waam_solver_amd.materials import material_from_dict
@@ -63,7 +63,7 @@ def build_pair(name: str, mode: str = "jacobi", engine: str = "fused"):
     mat_d = dict(LCS_BEAD_ON_PLATE)
     prod_mat = material_from_dict({"name": mat_d["name"], "status": "synthetic",
                                    "constants": {k: v for k, v in mat_d.items() if k != "name"}})
-    tw = WAAMTwin(material=prod_mat, **kw)
+    tw = WAAMTwin(material=prod_mat, field_layout=layout, **kw)
     tw.engine = engine
     ot = OracleTwin(Material(**mat_d), mode=mode, **kw)
     for t in (tw, ot):

@@ -61,11 +61,42 @@ def test_argument_validation_without_device():
     with pytest.raises(WaamError, match="NULL|null"):
         L.call("waam_step_fused", None, None, None)
     with pytest.raises(WaamError):
-        L.call("waam_stream", None, None, None, 4, 4, 4, None)
+        L.call("waam_stream", None, None, None, 4, 4, 4, 0, None)
     with pytest.raises(WaamError):
-        L.call("waam_stream", 1, 2, 3, 0, 4, 4, None)
+        L.call("waam_stream", 1, 2, 3, 0, 4, 4, 0, None)
+    with pytest.raises(WaamError, match="layout"):
+        L.call("waam_stream", 1, 2, 3, 4, 4, 4, 7, None)
     with pytest.raises(WaamError, match="range"):
         L.call("waam_timing_begin", 99, 10)
+
+
+_CODES = {"float": "f", "double": "d", "int": "i", "int32_t": "i", "int64_t": "l",
+          "uint32_t": "u"}
+
+
+def header_signatures() -> dict[str, str]:
+    """ctypes argument codes of every declaration in the header, derived from its text."""
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    out = {}
+    for name, args in re.findall(r"^\s*(?:const\s+)?[\w ]+?\**\s*\b(waam_\w+)\s*\(([^)]*)\)\s*;",
+                                 text, re.M | re.S):
+        args = args.strip()
+        if args in ("", "void"):
+            continue
+        codes = []
+        for a in args.split(","):
+            a = " ".join(a.split())
+            codes.append("p" if "*" in a else _CODES[a.rsplit(" ", 1)[0]])
+        out[name] = "".join(codes)
+    return out
+
+
+def test_ctypes_signatures_follow_the_header():
+    from waam_solver_amd._lib import SIGNATURES
+
+    hdr = header_signatures()
+    for name, sig in SIGNATURES.items():
+        assert hdr[name] == sig.replace(" ", ""), name
 
 
 _OFFSETS_SRC = r"""

@@ -27,15 +27,17 @@ def _assert_parity(res: dict, label: str) -> None:
     assert not bad, f"{label}: not bit-identical: {sorted(bad)}"
 
 
-@pytest.mark.parametrize("engine", ["reference", "fused"])
+@pytest.mark.parametrize("engine,layout", [("reference", 1), ("fused", 1), ("fused", 0),
+                                           ("reference", 0)])
 @pytest.mark.parametrize("case", ["bead_mrt", "droplets_srt"])
-def test_coupled_step_parity(hip_device, case, engine):
-    tw, ot = build_pair(case, engine=engine)
+def test_coupled_step_parity(hip_device, case, engine, layout):
+    """layout 1: fields stored (nz, ny, nx), i fastest; layout 0: the reference's (nx, ny, nz)."""
+    tw, ot = build_pair(case, engine=engine, layout=layout)
     done = 0
     for n in (1, 9, 40, 50):
         run_pair(tw, ot, n, start=done)
         done += n
-        _assert_parity(compare(tw, ot), f"{case}/{engine} step {done}")
+        _assert_parity(compare(tw, ot), f"{case}/{engine}/layout{layout} step {done}")
     if case == "droplets_srt":
         assert tw._n_droplets_fired == ot._n_droplets_fired and tw._n_droplets_fired > 0
         assert tw._deposited_volume_m3 == ot._deposited_volume_m3

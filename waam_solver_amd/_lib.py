@@ -16,8 +16,9 @@ import torch  # noqa: F401
 
 LIB_PATH = Path(__file__).resolve().parent / "libwaam_hip.so"
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 FLAG_FLUID, FLAG_SOLID, FLAG_GAS, FLAG_IFACE = 0, 1, 2, 4
+LAYOUT_XYZ, LAYOUT_ZYX = 0, 1
 
 _T = {
     "p": ctypes.c_void_p,
@@ -28,50 +29,52 @@ _T = {
     "u": ctypes.c_uint32,
 }
 
-# Argument codes per entry point, in declaration order (include/waam_hip.h); the trailing
-# "p" of every compute entry is the hipStream_t.
+# Argument codes per entry point, in declaration order (include/waam_hip.h; tests/test_abi.py
+# derives them from the header and compares): p pointer, f float, d double, i int, l int64,
+# u uint32. The trailing "p" of every compute entry is the hipStream_t; grid sizes nx, ny, nz
+# are followed by the field layout.
 SIGNATURES: dict[str, str] = {
-    "waam_init_grid": "pppppppppp fff iiiii iii p",
-    "waam_inject_arc_heat": "pppp fffffffff i iii pp",
-    "waam_inject_goldak_heat": "pppp fffffffffffffff i iii pp",
-    "waam_surface_height_at": "ppp iii iii p",
-    "waam_clamp_enthalpy_ceiling_scalar": "pp fffff l p",
-    "waam_clamp_enthalpy_floor_scalar": "pfpf l p",
-    "waam_advect_diffuse_temperature": "ppppppp fff iii p",
-    "waam_apply_thermal_boundary_losses": "ppp fff ii ffff iii p",
-    "waam_update_phase": "ppp ffff l p",
-    "waam_update_T_max": "ppp l p",
-    "waam_update_cooling_rate": "pppp f l p",
-    "waam_update_time_above_T": "ppppp ffff l p",
-    "waam_advect_phi": "pppppp iii p",
-    "waam_reinitialize_phi": "pp l p",
-    "waam_apply_contact_angle_phi_bc": "ppp ff iii p",
-    "waam_update_flags_from_phi": "ppp i iii p",
-    "waam_sync_phi_liquid_fraction": "ppp l p",
-    "waam_solidify_cooled_metal": "ppppppp f i l p",
-    "waam_remelt_hot_solid_scalar": "ppppp ffff l p",
-    "waam_solidify_trailing_pool_scalar": "pppppppp ffffffffff iii p",
-    "waam_compute_csf_tension": "ppppp f i ff iii p",
-    "waam_compute_curvature_field": "ppp iii p",
-    "waam_compute_marangoni_force": "ppppppp f iii p",
-    "waam_add_buoyancy": "pppp ffff l p",
-    "waam_add_hydrostatic_gravity": "ppp ff l p",
-    "waam_apply_arc_pressure": "ppp ffffffff iii p",
-    "waam_clamp_body_force_magnitude": "pppp f l p",
-    "waam_apply_droplet_impact_pressure": "pppp ffffffff iii p",
-    "waam_collide_mrt": "ppppppppppp fff iii p",
-    "waam_collide_srt": "ppppppppppp ffff iii p",
-    "waam_collide_srt_variable_tau": "pppppppppppp f iii p",
-    "waam_stream": "ppp iii p",
-    "waam_clamp_velocity_mach": "pppp f l p",
-    "waam_feed_wire_surface": "ppppppp ffffffffff pp f iii pp",
-    "waam_deposit_droplet": "ppppppp fff d f d ffff pp f p iii pp",
-    "waam_feed_wire_momentum_impact": "ppppppp fffffff iii p",
-    "waam_inject_tracers": "ppp i ffff i u p",
-    "waam_advect_tracers": "ppppppp ff i iii p",
-    "waam_telemetry_pool_reduce": "pppppppp iii pp",
+    "waam_init_grid": "ppppppppppfffiiiiiiiiip",
+    "waam_inject_arc_heat": "ppppfffffffffiiiiipp",
+    "waam_inject_goldak_heat": "ppppfffffffffffffffiiiiipp",
+    "waam_surface_height_at": "pppiiiiiiip",
+    "waam_clamp_enthalpy_ceiling_scalar": "ppffffflp",
+    "waam_clamp_enthalpy_floor_scalar": "pfpflp",
+    "waam_advect_diffuse_temperature": "pppppppfffiiiip",
+    "waam_apply_thermal_boundary_losses": "pppfffiiffffiiiip",
+    "waam_update_phase": "pppfffflp",
+    "waam_update_T_max": "ppplp",
+    "waam_update_cooling_rate": "ppppflp",
+    "waam_update_time_above_T": "pppppfffflp",
+    "waam_advect_phi": "ppppppiiiip",
+    "waam_reinitialize_phi": "pplp",
+    "waam_apply_contact_angle_phi_bc": "pppffiiiip",
+    "waam_update_flags_from_phi": "pppiiiiip",
+    "waam_sync_phi_liquid_fraction": "ppplp",
+    "waam_solidify_cooled_metal": "pppppppfilp",
+    "waam_remelt_hot_solid_scalar": "pppppfffflp",
+    "waam_solidify_trailing_pool_scalar": "ppppppppffffffffffiiiip",
+    "waam_compute_csf_tension": "pppppfiffiiiip",
+    "waam_compute_curvature_field": "pppiiiip",
+    "waam_compute_marangoni_force": "pppppppfiiiip",
+    "waam_add_buoyancy": "ppppfffflp",
+    "waam_add_hydrostatic_gravity": "pppfflp",
+    "waam_apply_arc_pressure": "pppffffffffiiiip",
+    "waam_clamp_body_force_magnitude": "ppppflp",
+    "waam_apply_droplet_impact_pressure": "ppppffffffffiiiip",
+    "waam_collide_mrt": "pppppppppppfffiiiip",
+    "waam_collide_srt": "pppppppppppffffiiiip",
+    "waam_collide_srt_variable_tau": "ppppppppppppfiiiip",
+    "waam_stream": "pppiiiip",
+    "waam_clamp_velocity_mach": "ppppflp",
+    "waam_feed_wire_surface": "pppppppffffffffffppfiiiipp",
+    "waam_deposit_droplet": "pppppppfffdfdffffppfpiiiipp",
+    "waam_feed_wire_momentum_impact": "pppppppfffffffiiiip",
+    "waam_inject_tracers": "pppiffffiup",
+    "waam_advect_tracers": "pppppppffiiiiip",
+    "waam_telemetry_pool_reduce": "ppppppppiiiipp",
     "waam_step_fused": "ppp",
-    "waam_stream_pending": "ppp iii p",
+    "waam_stream_pending": "pppiiiip",
     "waam_step_arc_readback": "pp",
     "waam_timing_begin": "ii",
     "waam_timing_end": "pp",

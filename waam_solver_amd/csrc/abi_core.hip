@@ -25,10 +25,12 @@ int check_launch(const char *what)
     return WAAM_OK;
 }
 
-int check_dims(const char *what, int nx, int ny, int nz)
+int check_dims(const char *what, int nx, int ny, int nz, int layout)
 {
     if (nx <= 0 || ny <= 0 || nz <= 0)
         return set_error(WAAM_ERR_INVALID, "%s: invalid dims %d x %d x %d", what, nx, ny, nz);
+    if (layout != kLayoutXYZ && layout != kLayoutZYX)
+        return set_error(WAAM_ERR_INVALID, "%s: unknown field layout %d", what, layout);
     if ((int64_t)nx * ny * nz > ((int64_t)1 << 40))
         return set_error(WAAM_ERR_INVALID, "%s: grid too large", what);
     return WAAM_OK;

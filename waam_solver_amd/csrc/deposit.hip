@@ -304,14 +304,15 @@ __global__ void __launch_bounds__(kBlock) k_telemetry(const float *__restrict__ 
     __shared__ Telem sh[kBlock];
     Telem v{0.0, 0.0, 0.0, 0.0, 0.0, 1.0e9, -1.0e9, 0.0};
     size_t N = d.n();
-    size_t plane = (size_t)d.ny * d.nz;
     for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < N;
          c += (size_t)gridDim.x * blockDim.x) {
         float Tc = T[c];
         v.Tg = fmax(v.Tg, (double)Tc);
         if (phi[c] > 0.05f) v.cool = fmax(v.cool, (double)(-dT_dt[c]));
         if (f_l[c] > 0.5f) {
-            double i = (double)(c / plane);
+            int ci, cj, ck;
+            d.ijk(c, ci, cj, ck);
+            double i = (double)ci;
             v.n += 1.0;
             v.Tp = fmax(v.Tp, (double)Tc);
             float u2 = ux[c] * ux[c] + uy[c] * uy[c] + uz[c] * uz[c];
@@ -352,7 +353,7 @@ static int deposit_launch(const char *name, float *f_src, uint8_t *flags, float 
                           float *H, float *T, float *rho, float *vol_acc, float *real_acc,
                           void *scratch, float *result, const DepositArgs &a, void *stream)
 {
-    WAAM_RETURN_IF(check_dims(name, a.d.nx, a.d.ny, a.d.nz));
+    WAAM_RETURN_IF(check_dims(name, a.d.nx, a.d.ny, a.d.nz, a.d.layout));
     WAAM_RETURN_IF(check_ptrs(name, {f_src, flags, f_l, phi, H, T, rho, vol_acc, real_acc, scratch}));
     double sr = a.r_try0 > (double)a.droplet_radius ? a.r_try0 : (double)a.droplet_radius;
     if (a.max_tries > 1 && sr < 16.0) sr = 16.0;
@@ -372,7 +373,7 @@ int waam_feed_wire_surface(float *f_src, uint8_t *flags, float *f_l, float *phi,
                            float *T, float *rho, float arc_i, float arc_j, float arc_k,
                            float footprint_r, float droplet_radius, float target_vol,
                            float T_drop, float cp_rho, float L_rho, float rho0, float *vol_acc,
-                           float *real_acc, float cell_vol, int nx, int ny, int nz,
+                           float *real_acc, float cell_vol, int nx, int ny, int nz, int layout,
                            void *scratch, void *stream)
 {
     DepositArgs a{};
@@ -389,7 +390,7 @@ int waam_feed_wire_surface(float *f_src, uint8_t *flags, float *f_l, float *phi,
     a.target_vol = (double)target_vol;
     a.r_try0 = (double)footprint_r;
     a.max_tries = 1;
-    a.d = Dims{nx, ny, nz};
+    a.d = Dims{nx, ny, nz, layout};
     return deposit_launch("waam_feed_wire_surface", f_src, flags, f_l, phi, H, T, rho, vol_acc,
                           real_acc, scratch, nullptr, a, stream);
 }
@@ -398,7 +399,7 @@ int waam_deposit_droplet(float *f_src, uint8_t *flags, float *f_l, float *phi, f
                          float *T, float *rho, float arc_i, float arc_j, float arc_k,
                          double footprint_r, float droplet_radius, double drop_vol,
                          float T_drop, float cp_rho, float L_rho, float rho0, float *vol_acc,
-                         float *real_acc, float cell_vol, float *result, int nx, int ny, int nz,
+                         float *real_acc, float cell_vol, float *result, int nx, int ny, int nz, int layout,
                          void *scratch, void *stream)
 {
     DepositArgs a{};
@@ -415,7 +416,7 @@ int waam_deposit_droplet(float *f_src, uint8_t *flags, float *f_l, float *phi, f
     a.target_vol = drop_vol;
     a.r_try0 = footprint_r;
     a.max_tries = 10;
-    a.d = Dims{nx, ny, nz};
+    a.d = Dims{nx, ny, nz, layout};
     return deposit_launch("waam_deposit_droplet", f_src, flags, f_l, phi, H, T, rho, vol_acc,
                           real_acc, scratch, result, a, stream);
 }
@@ -433,29 +434,29 @@ int waam_inject_tracers(float *pos, int32_t *active, int32_t *head, int max_trac
 
 int waam_advect_tracers(float *pos, int32_t *active, const float *ux, const float *uy,
                         const float *uz, const float *f_l, const uint8_t *flags, float dx,
-                        float dt, int max_tracers, int nx, int ny, int nz, void *stream)
+                        float dt, int max_tracers, int nx, int ny, int nz, int layout, void *stream)
 {
     (void)dt;
     if (max_tracers <= 0) return WAAM_OK;
-    WAAM_RETURN_IF(check_dims("waam_advect_tracers", nx, ny, nz));
+    WAAM_RETURN_IF(check_dims("waam_advect_tracers", nx, ny, nz, layout));
     WAAM_RETURN_IF(check_ptrs("waam_advect_tracers", {pos, active, ux, uy, uz, f_l, flags}));
     hipLaunchKernelGGL(k_advect_tracers, dim3(grid_for((size_t)max_tracers)), dim3(kBlock), 0,
                        as_stream(stream), pos, active, ux, uy, uz, f_l, flags, dx, max_tracers,
-                       Dims{nx, ny, nz});
+                       Dims{nx, ny, nz, layout});
     return check_launch("waam_advect_tracers");
 }
 
 int waam_telemetry_pool_reduce(const float *T, const float *f_l, const float *phi,
                                const float *dT_dt, const float *ux, const float *uy,
-                               const float *uz, double *out, int nx, int ny, int nz,
+                               const float *uz, double *out, int nx, int ny, int nz, int layout,
                                void *scratch, void *stream)
 {
-    WAAM_RETURN_IF(check_dims("waam_telemetry_pool_reduce", nx, ny, nz));
+    WAAM_RETURN_IF(check_dims("waam_telemetry_pool_reduce", nx, ny, nz, layout));
     WAAM_RETURN_IF(check_ptrs("waam_telemetry_pool_reduce", {T, f_l, phi, dT_dt, ux, uy, uz, out, scratch}));
     Telem *parts = reinterpret_cast<Telem *>((char *)scratch + kScratchTelemetryParts);
     hipStream_t s = as_stream(stream);
     hipLaunchKernelGGL(k_telemetry, dim3(kReduceBlocks), dim3(kBlock), 0, s, T, f_l, phi, dT_dt, ux,
-                       uy, uz, Dims{nx, ny, nz}, parts);
+                       uy, uz, Dims{nx, ny, nz, layout}, parts);
     hipLaunchKernelGGL(k_telemetry_final, dim3(1), dim3(kBlock), 0, s, parts, kReduceBlocks, out);
     return check_launch("waam_telemetry_pool_reduce");
 }

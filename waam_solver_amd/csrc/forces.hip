@@ -3,11 +3,9 @@
 
 namespace waam {
 
-#define CELL3D                                                    \
-    int k = blockIdx.x * blockDim.x + threadIdx.x;                \
-    int j = blockIdx.y * blockDim.y + threadIdx.y;                \
-    int i = blockIdx.z;                                           \
-    if (k >= d.nz || j >= d.ny) return;                           \
+#define CELL3D                    \
+    int i, j, k;                  \
+    if (!d.cell(i, j, k)) return; \
     size_t c = d.idx(i, j, k);
 
 __global__ void k_csf(const float *__restrict__ phi, const uint8_t *__restrict__ flags,
@@ -137,22 +135,22 @@ extern "C" {
 
 int waam_compute_csf_tension(const float *phi, const uint8_t *flags, float *Fx, float *Fy,
                              float *Fz, float gamma_lu, int enable_wetting, float cos_t,
-                             float sin_t, int nx, int ny, int nz, void *stream)
+                             float sin_t, int nx, int ny, int nz, int layout, void *stream)
 {
-    WAAM_RETURN_IF(check_dims("waam_compute_csf_tension", nx, ny, nz));
+    WAAM_RETURN_IF(check_dims("waam_compute_csf_tension", nx, ny, nz, layout));
     WAAM_RETURN_IF(check_ptrs("waam_compute_csf_tension", {phi, flags, Fx, Fy, Fz}));
-    Dims d{nx, ny, nz};
+    Dims d{nx, ny, nz, layout};
     hipLaunchKernelGGL(k_csf, grid3(d), block3(), 0, as_stream(stream), phi, flags, Fx, Fy, Fz,
                        gamma_lu, enable_wetting, cos_t, sin_t, d);
     return check_launch("waam_compute_csf_tension");
 }
 
 int waam_compute_curvature_field(const float *phi, const uint8_t *flags, float *kappa_out,
-                                 int nx, int ny, int nz, void *stream)
+                                 int nx, int ny, int nz, int layout, void *stream)
 {
-    WAAM_RETURN_IF(check_dims("waam_compute_curvature_field", nx, ny, nz));
+    WAAM_RETURN_IF(check_dims("waam_compute_curvature_field", nx, ny, nz, layout));
     WAAM_RETURN_IF(check_ptrs("waam_compute_curvature_field", {phi, flags, kappa_out}));
-    Dims d{nx, ny, nz};
+    Dims d{nx, ny, nz, layout};
     hipLaunchKernelGGL(k_curvature, grid3(d), block3(), 0, as_stream(stream), phi, flags,
                        kappa_out, d);
     return check_launch("waam_compute_curvature_field");
@@ -160,11 +158,11 @@ int waam_compute_curvature_field(const float *phi, const uint8_t *flags, float *
 
 int waam_compute_marangoni_force(const float *T, const float *phi, const float *f_l, float *Fx,
                                  float *Fy, float *Fz, const uint8_t *flags, float dgamma_dT,
-                                 int nx, int ny, int nz, void *stream)
+                                 int nx, int ny, int nz, int layout, void *stream)
 {
-    WAAM_RETURN_IF(check_dims("waam_compute_marangoni_force", nx, ny, nz));
+    WAAM_RETURN_IF(check_dims("waam_compute_marangoni_force", nx, ny, nz, layout));
     WAAM_RETURN_IF(check_ptrs("waam_compute_marangoni_force", {T, phi, f_l, Fx, Fy, Fz, flags}));
-    Dims d{nx, ny, nz};
+    Dims d{nx, ny, nz, layout};
     hipLaunchKernelGGL(k_marangoni, grid3(d), block3(), 0, as_stream(stream), T, phi, f_l, Fx, Fy,
                        Fz, flags, dgamma_dT, d);
     return check_launch("waam_compute_marangoni_force");
@@ -193,11 +191,11 @@ int waam_add_hydrostatic_gravity(float *Fz, const float *f_l, const uint8_t *fla
 
 int waam_apply_arc_pressure(float *Fz, const uint8_t *flags, const float *phi, float arc_i,
                             float arc_j, float arc_k, float sigma, float pressure_pa, float dt,
-                            float dx, float rho_ref, int nx, int ny, int nz, void *stream)
+                            float dx, float rho_ref, int nx, int ny, int nz, int layout, void *stream)
 {
-    WAAM_RETURN_IF(check_dims("waam_apply_arc_pressure", nx, ny, nz));
+    WAAM_RETURN_IF(check_dims("waam_apply_arc_pressure", nx, ny, nz, layout));
     WAAM_RETURN_IF(check_ptrs("waam_apply_arc_pressure", {Fz, flags, phi}));
-    Dims d{nx, ny, nz};
+    Dims d{nx, ny, nz, layout};
     ArcPressure ap;
     ap.arc_i = arc_i;
     ap.arc_j = arc_j;
@@ -223,11 +221,11 @@ int waam_clamp_body_force_magnitude(float *Fx, float *Fy, float *Fz, const uint8
 int waam_apply_droplet_impact_pressure(float *Fz, const uint8_t *flags, const float *phi,
                                        const float *f_l, float arc_i, float arc_j, float arc_k,
                                        float drop_radius, float impact_pa, float dt, float dx,
-                                       float rho_ref, int nx, int ny, int nz, void *stream)
+                                       float rho_ref, int nx, int ny, int nz, int layout, void *stream)
 {
-    WAAM_RETURN_IF(check_dims("waam_apply_droplet_impact_pressure", nx, ny, nz));
+    WAAM_RETURN_IF(check_dims("waam_apply_droplet_impact_pressure", nx, ny, nz, layout));
     WAAM_RETURN_IF(check_ptrs("waam_apply_droplet_impact_pressure", {Fz, flags, phi, f_l}));
-    Dims d{nx, ny, nz};
+    Dims d{nx, ny, nz, layout};
     float inv2s2 = 1.0f / (2.0f * drop_radius * drop_radius + 1e-6f);
     float F_peak_phys = impact_pa / (rho_ref * dx);
     float F_peak_lu = F_peak_phys * dt * dt / dx;

@@ -166,11 +166,14 @@ __device__ __forceinline__ size_t list_at(const CellList &l, const int *s_pref, 
 /* (i, j, k) of cell c; the fused step limits grids to INT_MAX cells, so 32-bit division */
 __device__ __forceinline__ void cell_ijk(size_t c, const Dims &d, int &i, int &j, int &k)
 {
-    unsigned cc = (unsigned)c, nz = (unsigned)d.nz, ny = (unsigned)d.ny;
-    unsigned r = cc / nz;
-    k = (int)(cc - r * nz);
-    i = (int)(r / ny);
-    j = (int)(r - (unsigned)i * ny);
+    unsigned cc = (unsigned)c, ny = (unsigned)d.ny;
+    unsigned nf = (unsigned)d.fast();
+    unsigned r = cc / nf;
+    int a = (int)(cc - r * nf);
+    int b = (int)(r / ny);
+    j = (int)(r - (unsigned)b * ny);
+    i = d.layout == kLayoutZYX ? a : b;
+    k = d.layout == kLayoutZYX ? b : a;
 }
 
 /* 1 — surface height in the torch column (reference kernels.py:625-641 and
@@ -418,10 +421,8 @@ __device__ __forceinline__ void thermal_vof_cell(const waam_step_fields &f, cons
 __global__ void __launch_bounds__(256) k_step_thermal(waam_step_fields f, StepConsts s)
 {
     const Dims &d = s.d;
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int i = blockIdx.z;
-    bool in = k < d.nz && j < d.ny;
+    int i, j, k;
+    bool in = d.cell(i, j, k);
     size_t c = in ? d.idx(i, j, k) : 0;
     uint8_t flag = in ? f.flags_in[c] : (uint8_t)GAS;
     list_append(in && is_surface_class(flag), c, cell_list(f, s.seg_len, 0), d.n());
@@ -471,10 +472,8 @@ __global__ void __launch_bounds__(256) k_step_vof_flags(waam_step_fields f, Step
     const Dims &d = s.d;
     if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0)  // kernel 2 has consumed the
         cell_list(f, s.seg_len, 0).count(threadIdx.y * blockDim.x + threadIdx.x) = 0;  // list
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int i = blockIdx.z;
-    bool in = k < d.nz && j < d.ny;
+    int i, j, k;
+    bool in = d.cell(i, j, k);
     size_t c = in ? d.idx(i, j, k) : 0;
     uint8_t nf = GAS;
     if (in) {
@@ -689,10 +688,8 @@ __global__ void __launch_bounds__(256) k_step_stream(waam_step_fields f, StepCon
 {
     const Dims &d = s.d;
     const waam_step_params &p = s.p;
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int i = blockIdx.z;
-    if (k >= d.nz || j >= d.ny) return;
+    int i, j, k;
+    if (!d.cell(i, j, k)) return;
     size_t c = d.idx(i, j, k);
     uint8_t fc = f.flags_out[c];
     uint8_t fp = f.sflags_prev[c];
@@ -722,10 +719,8 @@ __global__ void __launch_bounds__(256) k_step_stream(waam_step_fields f, StepCon
 __global__ void k_stream_pending(const float *__restrict__ src, float *__restrict__ dst,
                                  const uint8_t *__restrict__ sfl, Dims d)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int i = blockIdx.z;
-    if (k >= d.nz || j >= d.ny) return;
+    int i, j, k;
+    if (!d.cell(i, j, k)) return;
     uint8_t fp = sfl[d.idx(i, j, k)];
     if (fp != SOLID) pull_cell(src, dst, sfl, i, j, k, fp, d);
 }
@@ -818,7 +813,7 @@ int waam_step_fused(const waam_step_params *pp, const waam_step_fields *ff, void
     if (!pp || !ff) return set_error(WAAM_ERR_INVALID, "waam_step_fused: NULL argument");
     const waam_step_params &p = *pp;
     const waam_step_fields &f = *ff;
-    WAAM_RETURN_IF(check_dims("waam_step_fused", f.nx, f.ny, f.nz));
+    WAAM_RETURN_IF(check_dims("waam_step_fused", f.nx, f.ny, f.nz, f.layout));
     WAAM_RETURN_IF(check_ptrs("waam_step_fused",
                               {f.f_src, f.f_dst, f.rho, f.ux, f.uy, f.uz, f.H_in, f.H_out, f.T_in,
                                f.T_out, f.fl_in, f.fl_out, f.phi, f.phi_tmp, f.flags_in,
@@ -829,7 +824,7 @@ int waam_step_fused(const waam_step_params *pp, const waam_step_fields *ff, void
         f.flags_in == f.flags_out || f.sflags_prev == f.sflags_next || f.phi == f.phi_tmp ||
         f.list_thermal == f.list_collide)
         return set_error(WAAM_ERR_INVALID, "waam_step_fused: an in/out buffer pair aliases");
-    Dims d{f.nx, f.ny, f.nz};
+    Dims d{f.nx, f.ny, f.nz, f.layout};
     if (d.n() > (size_t)INT_MAX)
         return set_error(WAAM_ERR_INVALID, "waam_step_fused: %zu cells exceed the int32 cell lists",
                          d.n());
@@ -897,12 +892,12 @@ int waam_step_fused(const waam_step_params *pp, const waam_step_fields *ff, void
 }
 
 int waam_stream_pending(const float *f_src, float *f_dst, const uint8_t *sflags, int nx, int ny,
-                        int nz, void *stream)
+                        int nz, int layout, void *stream)
 {
-    WAAM_RETURN_IF(check_dims("waam_stream_pending", nx, ny, nz));
+    WAAM_RETURN_IF(check_dims("waam_stream_pending", nx, ny, nz, layout));
     WAAM_RETURN_IF(check_ptrs("waam_stream_pending", {f_src, f_dst, sflags}));
     if (f_src == f_dst) return set_error(WAAM_ERR_INVALID, "waam_stream_pending: f_src aliases f_dst");
-    Dims d{nx, ny, nz};
+    Dims d{nx, ny, nz, layout};
     hipLaunchKernelGGL(k_stream_pending, grid3(d), block3(), 0, as_stream(stream), f_src, f_dst,
                        sflags, d);
     return check_launch("waam_stream_pending");

@@ -19,10 +19,8 @@ __global__ void k_init_grid(float *__restrict__ f, float *__restrict__ rho, floa
     size_t N = d.n();
     for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < N;
          c += (size_t)gridDim.x * blockDim.x) {
-        int k = (int)(c % (size_t)d.nz);
-        size_t r = c / (size_t)d.nz;
-        int j = (int)(r % (size_t)d.ny);
-        int i = (int)(r / (size_t)d.ny);
+        int i, j, k;
+        d.ijk(c, i, j, k);
 #pragma unroll
         for (int q = 0; q < Q; ++q) f[(size_t)q * N + c] = wq(q) * rho0;
         rho[c] = rho0;
@@ -85,10 +83,8 @@ __global__ void k_collide(const float *__restrict__ f_src, float *__restrict__ f
 __global__ void k_stream(const float *__restrict__ f_src, float *__restrict__ f_dst,
                          const uint8_t *__restrict__ flags, Dims d)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int i = blockIdx.z;
-    if (k >= d.nz || j >= d.ny) return;
+    int i, j, k;
+    if (!d.cell(i, j, k)) return;
     size_t N = d.n();
     size_t c = d.idx(i, j, k);
     uint8_t flag = flags[c];
@@ -171,11 +167,11 @@ extern "C" {
 int waam_init_grid(float *f, float *rho, float *ux, float *uy, float *uz, float *T, float *H,
                    float *f_l, float *phi, uint8_t *flags, float T_ambient, float rho0,
                    float cp_rho, int nz_solid, int plate_i0, int plate_i1, int plate_j0,
-                   int plate_j1, int nx, int ny, int nz, void *stream)
+                   int plate_j1, int nx, int ny, int nz, int layout, void *stream)
 {
-    WAAM_RETURN_IF(check_dims("waam_init_grid", nx, ny, nz));
+    WAAM_RETURN_IF(check_dims("waam_init_grid", nx, ny, nz, layout));
     WAAM_RETURN_IF(check_ptrs("waam_init_grid", {f, rho, ux, uy, uz, T, H, f_l, phi, flags}));
-    Dims d{nx, ny, nz};
+    Dims d{nx, ny, nz, layout};
     unsigned g = grid_for(d.n());
     if (g > 65536u) g = 65536u;
     hipLaunchKernelGGL(k_init_grid, dim3(g), dim3(kBlock), 0, as_stream(stream), f, rho, ux, uy,
@@ -188,12 +184,12 @@ static int collide_common(int mode, const char *name, const float *f_src, float 
                           float *rho, float *ux, float *uy, float *uz, const float *Fx,
                           const float *Fy, const float *Fz, const float *f_l,
                           const uint8_t *flags, const float *tau_field, float om_s, float om_b,
-                          float C, int nx, int ny, int nz, void *stream)
+                          float C, int nx, int ny, int nz, int layout, void *stream)
 {
-    WAAM_RETURN_IF(check_dims(name, nx, ny, nz));
+    WAAM_RETURN_IF(check_dims(name, nx, ny, nz, layout));
     WAAM_RETURN_IF(check_ptrs(name, {f_src, f_dst, rho, ux, uy, uz, Fx, Fy, Fz, f_l, flags}));
     if (f_src == f_dst) return set_error(WAAM_ERR_INVALID, "%s: f_src aliases f_dst", name);
-    Dims d{nx, ny, nz};
+    Dims d{nx, ny, nz, layout};
     size_t N = d.n();
     hipStream_t s = as_stream(stream);
     if (mode == 0)
@@ -211,43 +207,43 @@ static int collide_common(int mode, const char *name, const float *f_src, float 
 int waam_collide_mrt(const float *f_src, float *f_dst, float *rho, float *ux, float *uy,
                      float *uz, const float *Fx, const float *Fy, const float *Fz,
                      const float *f_l, const uint8_t *flags, float omega_s, float omega_b,
-                     float C_darcy, int nx, int ny, int nz, void *stream)
+                     float C_darcy, int nx, int ny, int nz, int layout, void *stream)
 {
     return collide_common(0, "waam_collide_mrt", f_src, f_dst, rho, ux, uy, uz, Fx, Fy, Fz, f_l,
-                          flags, nullptr, omega_s, omega_b, C_darcy, nx, ny, nz, stream);
+                          flags, nullptr, omega_s, omega_b, C_darcy, nx, ny, nz, layout, stream);
 }
 
 int waam_collide_srt(const float *f_src, float *f_dst, float *rho, float *ux, float *uy,
                      float *uz, const float *Fx, const float *Fy, const float *Fz,
                      const float *f_l, const uint8_t *flags, float tau, float omega,
-                     float dt_lu, float C_darcy, int nx, int ny, int nz, void *stream)
+                     float dt_lu, float C_darcy, int nx, int ny, int nz, int layout, void *stream)
 {
     (void)tau;
     (void)dt_lu;
     return collide_common(1, "waam_collide_srt", f_src, f_dst, rho, ux, uy, uz, Fx, Fy, Fz, f_l,
-                          flags, nullptr, omega, omega, C_darcy, nx, ny, nz, stream);
+                          flags, nullptr, omega, omega, C_darcy, nx, ny, nz, layout, stream);
 }
 
 int waam_collide_srt_variable_tau(const float *f_src, float *f_dst, float *rho, float *ux,
                                   float *uy, float *uz, const float *Fx, const float *Fy,
                                   const float *Fz, const float *f_l, const uint8_t *flags,
-                                  const float *tau_field, float C_darcy, int nx, int ny, int nz,
+                                  const float *tau_field, float C_darcy, int nx, int ny, int nz, int layout,
                                   void *stream)
 {
     if (!tau_field)
         return set_error(WAAM_ERR_INVALID, "waam_collide_srt_variable_tau: tau_field is NULL");
     return collide_common(2, "waam_collide_srt_variable_tau", f_src, f_dst, rho, ux, uy, uz, Fx,
                           Fy, Fz, f_l, flags, tau_field, 0.0f, 0.0f, C_darcy, nx, ny, nz,
-                          stream);
+                          layout, stream);
 }
 
-int waam_stream(const float *f_src, float *f_dst, const uint8_t *flags, int nx, int ny, int nz,
+int waam_stream(const float *f_src, float *f_dst, const uint8_t *flags, int nx, int ny, int nz, int layout,
                 void *stream)
 {
-    WAAM_RETURN_IF(check_dims("waam_stream", nx, ny, nz));
+    WAAM_RETURN_IF(check_dims("waam_stream", nx, ny, nz, layout));
     WAAM_RETURN_IF(check_ptrs("waam_stream", {f_src, f_dst, flags}));
     if (f_src == f_dst) return set_error(WAAM_ERR_INVALID, "waam_stream: f_src aliases f_dst");
-    Dims d{nx, ny, nz};
+    Dims d{nx, ny, nz, layout};
     hipLaunchKernelGGL(k_stream, grid3(d), block3(), 0, as_stream(stream), f_src, f_dst, flags, d);
     return check_launch("waam_stream");
 }
@@ -265,10 +261,10 @@ int waam_clamp_velocity_mach(float *ux, float *uy, float *uz, const uint8_t *fla
 int waam_feed_wire_momentum_impact(float *f_src, const float *rho, float *ux, float *uy,
                                    float *uz, const uint8_t *flags, const float *f_l,
                                    float arc_i, float arc_j, float arc_k, float droplet_radius,
-                                   float vx_lu, float vy_lu, float vz_lu, int nx, int ny, int nz,
+                                   float vx_lu, float vy_lu, float vz_lu, int nx, int ny, int nz, int layout,
                                    void *stream)
 {
-    WAAM_RETURN_IF(check_dims("waam_feed_wire_momentum_impact", nx, ny, nz));
+    WAAM_RETURN_IF(check_dims("waam_feed_wire_momentum_impact", nx, ny, nz, layout));
     WAAM_RETURN_IF(check_ptrs("waam_feed_wire_momentum_impact", {f_src, rho, ux, uy, uz, flags, f_l}));
     const float dr = droplet_radius;
     int i_min = (int)(arc_i - dr * 2.5f); i_min = i_min > 0 ? i_min : 0;
@@ -279,7 +275,7 @@ int waam_feed_wire_momentum_impact(float *f_src, const float *rho, float *ux, fl
     int k_max = (int)(arc_k + dr * 3.0f) + 1; k_max = k_max < nz ? k_max : nz;
     int ni = i_max - i_min, nj = j_max - j_min, nk = k_max - k_min;
     if (ni <= 0 || nj <= 0 || nk <= 0) return WAAM_OK;
-    Dims d{nx, ny, nz};
+    Dims d{nx, ny, nz, layout};
     hipLaunchKernelGGL(k_momentum_impact, dim3(grid_for((size_t)ni * nj * nk)), dim3(kBlock), 0,
                        as_stream(stream), f_src, rho, ux, uy, uz, flags, f_l, arc_i, arc_j,
                        arc_k, dr, vx_lu, vy_lu, vz_lu, i_min, j_min, k_min, ni, nj, nk, d);

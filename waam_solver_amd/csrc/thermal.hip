@@ -23,10 +23,8 @@ __global__ void __launch_bounds__(kBlock) k_arc_partials(const uint8_t *__restri
     for (size_t cell = (size_t)blockIdx.x * blockDim.x + threadIdx.x; cell < N;
          cell += (size_t)gridDim.x * blockDim.x) {
         if (flags[cell] == GAS) continue;
-        int k = (int)(cell % (size_t)d.nz);
-        size_t r = cell / (size_t)d.nz;
-        int j = (int)(r % (size_t)d.ny);
-        int i = (int)(r / (size_t)d.ny);
+        int i, j, k;
+        d.ijk(cell, i, j, k);
         float wg, wdep;
         if (!arc_cell_weights(p, i, j, k, f_l[cell], wg, wdep)) continue;
         double ss, e;
@@ -76,10 +74,8 @@ __global__ void k_arc_apply(float *__restrict__ H, const uint8_t *__restrict__ f
     if (cell >= N) return;
     float epw = tot[1];
     if (epw <= 0.0f || flags[cell] == GAS) return;
-    int k = (int)(cell % (size_t)d.nz);
-    size_t r = cell / (size_t)d.nz;
-    int j = (int)(r % (size_t)d.ny);
-    int i = (int)(r / (size_t)d.ny);
+    int i, j, k;
+    d.ijk(cell, i, j, k);
     float wg, wdep;
     if (!arc_cell_weights(p, i, j, k, f_l[cell], wg, wdep)) return;
     H[cell] += epw * wg * wdep;
@@ -87,11 +83,11 @@ __global__ void k_arc_apply(float *__restrict__ H, const uint8_t *__restrict__ f
 
 static int arc_run(const char *name, float *H, const uint8_t *flags, const float *f_l,
                    float *norm_out, const ArcParams &p, float Q_w, float dt, float dx3, float eta,
-                   int nx, int ny, int nz, void *scratch, void *stream)
+                   int nx, int ny, int nz, int layout, void *scratch, void *stream)
 {
-    WAAM_RETURN_IF(check_dims(name, nx, ny, nz));
+    WAAM_RETURN_IF(check_dims(name, nx, ny, nz, layout));
     WAAM_RETURN_IF(check_ptrs(name, {H, flags, f_l, scratch}));
-    Dims d{nx, ny, nz};
+    Dims d{nx, ny, nz, layout};
     hipStream_t s = as_stream(stream);
     double *partials = reinterpret_cast<double *>((char *)scratch + kScratchPartials);
     float *tot = reinterpret_cast<float *>((char *)scratch + kScratchTelemetry);
@@ -176,10 +172,8 @@ __global__ void k_advect_diffuse(const float *__restrict__ H_in, float *__restri
                                  const uint8_t *__restrict__ flags, float alpha_lu, float dt,
                                  float cp_rho, Dims d)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int i = blockIdx.z;
-    if (k >= d.nz || j >= d.ny) return;
+    int i, j, k;
+    if (!d.cell(i, j, k)) return;
     size_t c = d.idx(i, j, k);
     uint8_t flag = flags[c];
     if (flag == GAS) {
@@ -194,10 +188,8 @@ __global__ void k_advect_diffuse(const float *__restrict__ H_in, float *__restri
 __global__ void k_losses(float *__restrict__ H, const float *__restrict__ T,
                          const uint8_t *__restrict__ flags, LossConsts lc, Dims d)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int i = blockIdx.z;
-    if (k >= d.nz || j >= d.ny) return;
+    int i, j, k;
+    if (!d.cell(i, j, k)) return;
     size_t c = d.idx(i, j, k);
     if (flags[c] == GAS) return;
     int n_exp = exposed_faces(flags, i, j, k, d);
@@ -233,7 +225,7 @@ extern "C" {
 int waam_inject_arc_heat(float *H, const uint8_t *flags, const float *f_l, float *norm_out,
                          float arc_i, float arc_j, float arc_k, float Q_w, float sigma, float dt,
                          float dx3, float eta, float penetration_cells, int enable_surface_weight,
-                         int nx, int ny, int nz, void *scratch, void *stream)
+                         int nx, int ny, int nz, int layout, void *scratch, void *stream)
 {
     ArcParams p{};
     p.kind = 0;
@@ -244,7 +236,7 @@ int waam_inject_arc_heat(float *H, const uint8_t *flags, const float *f_l, float
     p.pen = penetration_cells;
     p.enable = enable_surface_weight;
     return arc_run("waam_inject_arc_heat", H, flags, f_l, norm_out, p, Q_w, dt, dx3, eta, nx, ny,
-                   nz, scratch, stream);
+                   nz, layout, scratch, stream);
 }
 
 int waam_inject_goldak_heat(float *H, const uint8_t *flags, const float *f_l, float *norm_out,
@@ -252,7 +244,7 @@ int waam_inject_goldak_heat(float *H, const uint8_t *flags, const float *f_l, fl
                             float dx3, float eta, float travel_sign, float ff, float fr,
                             float a_front, float a_rear, float b_axis, float c_axis,
                             float penetration_cells, int enable_surface_weight, int nx, int ny,
-                            int nz, void *scratch, void *stream)
+                            int nz, int layout, void *scratch, void *stream)
 {
     ArcParams p{};
     p.kind = 1;
@@ -269,20 +261,20 @@ int waam_inject_goldak_heat(float *H, const uint8_t *flags, const float *f_l, fl
     p.pen = penetration_cells;
     p.enable = enable_surface_weight;
     return arc_run("waam_inject_goldak_heat", H, flags, f_l, norm_out, p, Q_w, dt, dx3, eta, nx,
-                   ny, nz, scratch, stream);
+                   ny, nz, layout, scratch, stream);
 }
 
 int waam_surface_height_at(const float *phi, const uint8_t *flags, float *out, int i0, int j0,
-                           int nz_fallback, int nx, int ny, int nz, void *stream)
+                           int nz_fallback, int nx, int ny, int nz, int layout, void *stream)
 {
-    WAAM_RETURN_IF(check_dims("waam_surface_height_at", nx, ny, nz));
+    WAAM_RETURN_IF(check_dims("waam_surface_height_at", nx, ny, nz, layout));
     WAAM_RETURN_IF(check_ptrs("waam_surface_height_at", {phi, flags, out}));
     if (i0 < 0 || i0 >= nx || j0 < 0 || j0 >= ny)
         return set_error(WAAM_ERR_INVALID, "waam_surface_height_at: column (%d,%d) outside grid",
                          i0, j0);
     int kf = nz_fallback - 1 > 1 ? nz_fallback - 1 : 1;
     hipLaunchKernelGGL(k_surface_height, dim3(1), dim3(kBlock), 0, as_stream(stream), phi, flags,
-                       out, i0, j0, kf, Dims{nx, ny, nz});
+                       out, i0, j0, kf, Dims{nx, ny, nz, layout});
     return check_launch("waam_surface_height_at");
 }
 
@@ -314,13 +306,13 @@ int waam_clamp_enthalpy_floor_scalar(float *H, float cp_rho, const uint8_t *flag
 int waam_advect_diffuse_temperature(const float *H_in, float *H_out, const float *T,
                                     const float *ux, const float *uy, const float *uz,
                                     const uint8_t *flags, float alpha_lu, float dt,
-                                    float cp_rho, int nx, int ny, int nz, void *stream)
+                                    float cp_rho, int nx, int ny, int nz, int layout, void *stream)
 {
-    WAAM_RETURN_IF(check_dims("waam_advect_diffuse_temperature", nx, ny, nz));
+    WAAM_RETURN_IF(check_dims("waam_advect_diffuse_temperature", nx, ny, nz, layout));
     WAAM_RETURN_IF(check_ptrs("waam_advect_diffuse_temperature", {H_in, H_out, T, ux, uy, uz, flags}));
     if (H_in == H_out)
         return set_error(WAAM_ERR_INVALID, "waam_advect_diffuse_temperature: H_in aliases H_out");
-    Dims d{nx, ny, nz};
+    Dims d{nx, ny, nz, layout};
     hipLaunchKernelGGL(k_advect_diffuse, grid3(d), block3(), 0, as_stream(stream), H_in, H_out, T,
                        ux, uy, uz, flags, alpha_lu, dt, cp_rho, d);
     return check_launch("waam_advect_diffuse_temperature");
@@ -329,13 +321,13 @@ int waam_advect_diffuse_temperature(const float *H_in, float *H_out, const float
 int waam_apply_thermal_boundary_losses(float *H, const float *T, const uint8_t *flags,
                                        float T_amb, float h_conv, float eps_rad,
                                        int enable_conv, int enable_rad, float cp_rho, float dt,
-                                       float dx, float sigma_sb, int nx, int ny, int nz,
+                                       float dx, float sigma_sb, int nx, int ny, int nz, int layout,
                                        void *stream)
 {
     (void)cp_rho;
-    WAAM_RETURN_IF(check_dims("waam_apply_thermal_boundary_losses", nx, ny, nz));
+    WAAM_RETURN_IF(check_dims("waam_apply_thermal_boundary_losses", nx, ny, nz, layout));
     WAAM_RETURN_IF(check_ptrs("waam_apply_thermal_boundary_losses", {H, T, flags}));
-    Dims d{nx, ny, nz};
+    Dims d{nx, ny, nz, layout};
     LossConsts lc{T_amb, h_conv, eps_rad, sigma_sb, dt, dx, enable_conv, enable_rad};
     hipLaunchKernelGGL(k_losses, grid3(d), block3(), 0, as_stream(stream), H, T, flags, lc, d);
     return check_launch("waam_apply_thermal_boundary_losses");

@@ -11,10 +11,8 @@ __global__ void k_advect_phi(float *__restrict__ dst, const float *__restrict__ 
                              const float *__restrict__ uz, const uint8_t *__restrict__ flags,
                              Dims d)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int i = blockIdx.z;
-    if (k >= d.nz || j >= d.ny) return;
+    int i, j, k;
+    if (!d.cell(i, j, k)) return;
     size_t c = d.idx(i, j, k);
     dst[c] = advected_phi(src, ux, uy, uz, flags, i, j, k, d);
 }
@@ -31,21 +29,20 @@ __global__ void k_reinit(float *__restrict__ phi, const uint8_t *__restrict__ fl
 __global__ void k_contact_bc(const float *__restrict__ pin, float *__restrict__ pout,
                              const uint8_t *__restrict__ flags, float cos_t, float sin_t, Dims d)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int i = blockIdx.z;
-    if (k >= d.nz || j >= d.ny) return;
+    int i, j, k;
+    if (!d.cell(i, j, k)) return;
     size_t c = d.idx(i, j, k);
     pout[c] = contact_bc_phi(pin, flags, i, j, k, cos_t, sin_t, d);
 }
 
 /* update_flags_from_phi — reference kernels.py:2394-2419 */
 __global__ void k_update_flags(const float *__restrict__ phi, const float *__restrict__ f_l,
-                               uint8_t *__restrict__ flags, int nz_solid, size_t N, int nz)
+                               uint8_t *__restrict__ flags, int nz_solid, Dims d)
 {
     size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= N) return;
-    int k = (int)(c % (size_t)nz);
+    if (c >= d.n()) return;
+    int i, j, k;
+    d.ijk(c, i, j, k);
     flags[c] = classify_cell(k, nz_solid, phi[c], f_l[c]);
 }
 
@@ -113,10 +110,8 @@ __global__ void k_trailing(float *__restrict__ T, float *__restrict__ H, float *
                            float arc_k, float dx_, float dy_, float dz_, float lookback,
                            float T_freeze, float T_s, Dims d)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int i = blockIdx.z;
-    if (k >= d.nz || j >= d.ny) return;
+    int i, j, k;
+    if (!d.cell(i, j, k)) return;
     size_t c = d.idx(i, j, k);
     if (flags[c] == GAS) return;
     if (phi[c] < 0.85f || f_l[c] < 0.05f) return;
@@ -141,13 +136,13 @@ using namespace waam;
 extern "C" {
 
 int waam_advect_phi(float *phi_dst, const float *phi_src, const float *ux, const float *uy,
-                    const float *uz, const uint8_t *flags, int nx, int ny, int nz, void *stream)
+                    const float *uz, const uint8_t *flags, int nx, int ny, int nz, int layout, void *stream)
 {
-    WAAM_RETURN_IF(check_dims("waam_advect_phi", nx, ny, nz));
+    WAAM_RETURN_IF(check_dims("waam_advect_phi", nx, ny, nz, layout));
     WAAM_RETURN_IF(check_ptrs("waam_advect_phi", {phi_dst, phi_src, ux, uy, uz, flags}));
     if (phi_dst == phi_src)
         return set_error(WAAM_ERR_INVALID, "waam_advect_phi: phi_dst aliases phi_src");
-    Dims d{nx, ny, nz};
+    Dims d{nx, ny, nz, layout};
     hipLaunchKernelGGL(k_advect_phi, grid3(d), block3(), 0, as_stream(stream), phi_dst, phi_src,
                        ux, uy, uz, flags, d);
     return check_launch("waam_advect_phi");
@@ -163,27 +158,27 @@ int waam_reinitialize_phi(float *phi, const uint8_t *flags, int64_t n, void *str
 }
 
 int waam_apply_contact_angle_phi_bc(const float *phi_in, float *phi_out, const uint8_t *flags,
-                                    float cos_t, float sin_t, int nx, int ny, int nz,
+                                    float cos_t, float sin_t, int nx, int ny, int nz, int layout,
                                     void *stream)
 {
-    WAAM_RETURN_IF(check_dims("waam_apply_contact_angle_phi_bc", nx, ny, nz));
+    WAAM_RETURN_IF(check_dims("waam_apply_contact_angle_phi_bc", nx, ny, nz, layout));
     WAAM_RETURN_IF(check_ptrs("waam_apply_contact_angle_phi_bc", {phi_in, phi_out, flags}));
     if (phi_in == phi_out)
         return set_error(WAAM_ERR_INVALID, "waam_apply_contact_angle_phi_bc: phi_in aliases phi_out");
-    Dims d{nx, ny, nz};
+    Dims d{nx, ny, nz, layout};
     hipLaunchKernelGGL(k_contact_bc, grid3(d), block3(), 0, as_stream(stream), phi_in, phi_out,
                        flags, cos_t, sin_t, d);
     return check_launch("waam_apply_contact_angle_phi_bc");
 }
 
 int waam_update_flags_from_phi(const float *phi, const float *f_l, uint8_t *flags, int nz_solid,
-                               int nx, int ny, int nz, void *stream)
+                               int nx, int ny, int nz, int layout, void *stream)
 {
-    WAAM_RETURN_IF(check_dims("waam_update_flags_from_phi", nx, ny, nz));
+    WAAM_RETURN_IF(check_dims("waam_update_flags_from_phi", nx, ny, nz, layout));
     WAAM_RETURN_IF(check_ptrs("waam_update_flags_from_phi", {phi, f_l, flags}));
-    size_t N = Dims{nx, ny, nz}.n();
-    hipLaunchKernelGGL(k_update_flags, dim3(grid_for(N)), dim3(kBlock), 0, as_stream(stream), phi,
-                       f_l, flags, nz_solid, N, nz);
+    Dims d{nx, ny, nz, layout};
+    hipLaunchKernelGGL(k_update_flags, dim3(grid_for(d.n())), dim3(kBlock), 0, as_stream(stream),
+                       phi, f_l, flags, nz_solid, d);
     return check_launch("waam_update_flags_from_phi");
 }
 
@@ -225,11 +220,11 @@ int waam_solidify_trailing_pool_scalar(float *T, float *H, float *f_l, float *ph
                                        float cp_rho, float arc_i, float arc_j, float arc_k,
                                        float dir_x, float dir_y, float dir_z,
                                        float lookback_cells, float T_freeze, float T_solidus,
-                                       int nx, int ny, int nz, void *stream)
+                                       int nx, int ny, int nz, int layout, void *stream)
 {
-    WAAM_RETURN_IF(check_dims("waam_solidify_trailing_pool_scalar", nx, ny, nz));
+    WAAM_RETURN_IF(check_dims("waam_solidify_trailing_pool_scalar", nx, ny, nz, layout));
     WAAM_RETURN_IF(check_ptrs("waam_solidify_trailing_pool_scalar", {T, H, f_l, phi, flags, ux, uy, uz}));
-    Dims d{nx, ny, nz};
+    Dims d{nx, ny, nz, layout};
     hipLaunchKernelGGL(k_trailing, grid3(d), block3(), 0, as_stream(stream), T, H, f_l, phi, flags,
                        ux, uy, uz, cp_rho, arc_i, arc_j, arc_k, dir_x, dir_y, dir_z,
                        lookback_cells, T_freeze, T_solidus, d);

@@ -55,13 +55,54 @@ __host__ __device__ constexpr float wq(int q)
     return q == 0 ? (float)(1.0 / 3.0) : (q < 7 ? (float)(1.0 / 18.0) : (float)(1.0 / 36.0));
 }
 
+/* Field layouts (every field of a grid shares one):
+ *   kLayoutXYZ  (nx, ny, nz) C order, k fastest: the reference's layout;
+ *   kLayoutZYX  (nz, ny, nx) C order, i fastest: rows along the x axis, so a wavefront covers
+ *               64 cells of one horizontal row and a flat free surface or a SOLID plate fills
+ *               whole wavefronts instead of crossing every one of them. */
+constexpr int kLayoutXYZ = WAAM_LAYOUT_XYZ;
+constexpr int kLayoutZYX = WAAM_LAYOUT_ZYX;
+
 struct Dims {
     int nx, ny, nz;
+    int layout;
     __host__ __device__ size_t n() const { return (size_t)nx * (size_t)ny * (size_t)nz; }
     __host__ __device__ size_t idx(int i, int j, int k) const
     {
-        return ((size_t)i * (size_t)ny + (size_t)j) * (size_t)nz + (size_t)k;
+        return layout == kLayoutZYX ? ((size_t)k * (size_t)ny + (size_t)j) * (size_t)nx + (size_t)i
+                                    : ((size_t)i * (size_t)ny + (size_t)j) * (size_t)nz + (size_t)k;
     }
+    /* (i, j, k) of linear index c */
+    __host__ __device__ void ijk(size_t c, int &i, int &j, int &k) const
+    {
+        if (layout == kLayoutZYX) {
+            i = (int)(c % (size_t)nx);
+            size_t r = c / (size_t)nx;
+            j = (int)(r % (size_t)ny);
+            k = (int)(r / (size_t)ny);
+        } else {
+            k = (int)(c % (size_t)nz);
+            size_t r = c / (size_t)nz;
+            j = (int)(r % (size_t)ny);
+            i = (int)(r / (size_t)ny);
+        }
+    }
+    /* extent of the contiguous axis and of the slowest axis */
+    __host__ __device__ int fast() const { return layout == kLayoutZYX ? nx : nz; }
+    __host__ __device__ int slow() const { return layout == kLayoutZYX ? nz : nx; }
+#ifdef __HIPCC__
+    /* cell of this thread in a grid3() launch (x: contiguous axis, y: j, z: slowest axis);
+     * false for the threads past the end of a row */
+    __device__ __forceinline__ bool cell(int &i, int &j, int &k) const
+    {
+        int a = blockIdx.x * blockDim.x + threadIdx.x;
+        j = blockIdx.y * blockDim.y + threadIdx.y;
+        int b = blockIdx.z;
+        i = layout == kLayoutZYX ? a : b;
+        k = layout == kLayoutZYX ? b : a;
+        return a < fast() && j < ny;
+    }
+#endif
 };
 
 __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
@@ -79,20 +120,21 @@ inline unsigned grid_for(size_t n, int block = kBlock)
     return (unsigned)(g == 0 ? 1 : g);
 }
 
-/* 3-D launch: x over k (64 lanes = one wavefront along the contiguous axis), y over j, z over i. */
-constexpr int kTileK = 64;
+/* 3-D launch: x over the contiguous axis (64 lanes = one wavefront), y over j, z over the
+ * slowest axis (Dims::cell maps a thread to its cell). */
+constexpr int kTileFast = 64;
 constexpr int kTileJ = 4;
-inline dim3 block3() { return dim3(kTileK, kTileJ, 1); }
+inline dim3 block3() { return dim3(kTileFast, kTileJ, 1); }
 inline dim3 grid3(const Dims &d)
 {
-    return dim3((unsigned)((d.nz + kTileK - 1) / kTileK), (unsigned)((d.ny + kTileJ - 1) / kTileJ),
-                (unsigned)d.nx);
+    return dim3((unsigned)((d.fast() + kTileFast - 1) / kTileFast),
+                (unsigned)((d.ny + kTileJ - 1) / kTileJ), (unsigned)d.slow());
 }
 
 /* Error plumbing (abi_core.hip) ------------------------------------------------------------ */
 int set_error(int code, const char *fmt, ...);
 int check_launch(const char *what);
-int check_dims(const char *what, int nx, int ny, int nz);
+int check_dims(const char *what, int nx, int ny, int nz, int layout);
 int check_ptrs(const char *what, std::initializer_list<const void *> ptrs);
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 

@@ -1,9 +1,13 @@
 """WAAMGrid — the coupled step's state as device tensors (reference grid.py:37-289).
 
 Every field the reference allocates with ``ti.field`` is a contiguous torch tensor on the ROCm
-device, wrapped in a DeviceField, with the same names and the same C-order (nx, ny, nz) layout
-(k fastest); populations are (19, nx, ny, nz). Flags are uint8. Lattice-unit scaling and the
-stability checks follow reference grid.py:89-126.
+device, wrapped in a DeviceField, with the same names; populations are 19 such arrays. Flags
+are uint8. Lattice-unit scaling and the stability checks follow reference grid.py:89-126.
+
+Storage layout (``layout``): ``LAYOUT_ZYX`` (default) stores every field as (nz, ny, nx), i
+fastest, so the GPU's wavefronts run along horizontal rows, which the plate, the free surface
+and the gas above fill whole; ``LAYOUT_XYZ`` is the reference's (nx, ny, nz), k fastest. The
+fields' accessors (``to_numpy``, ``from_numpy``, ``[i, j, k]``) present (nx, ny, nz) either way.
 
 Memory: 2 x 19 x 4 B of populations plus about 26 scalar fields of 4 B per cell — 4.3 GB of
 populations and ~5.9 GB of fields at 768x384x192 — well inside one MI355X's 288 GB.
@@ -14,7 +18,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-from .field import DeviceField
+from .field import LAYOUT_XYZ, LAYOUT_ZYX, DeviceField
 from .materials import MaterialProps
 
 Q = 19
@@ -40,7 +44,10 @@ class WAAMGrid:
     def __init__(self, nx: int, ny: int, nz: int, dx: float, mat: MaterialProps,
                  max_tracers: int = 50_000, allocate_lorentz: bool = False,
                  allocate_vof: bool = False, allocate_export: bool = False,
-                 device: torch.device | str | None = None):
+                 device: torch.device | str | None = None, layout: int = LAYOUT_ZYX):
+        if layout not in (LAYOUT_XYZ, LAYOUT_ZYX):
+            raise ValueError(f"unknown field layout {layout}")
+        self.layout = int(layout)
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.dx = float(dx)
         self.mat = mat
@@ -67,16 +74,16 @@ class WAAMGrid:
         s3 = (self.nx, self.ny, self.nz)
         self._shape3 = s3
         dev = self.device
-        f32 = lambda: DeviceField.zeros(s3, torch.float32, dev)  # noqa: E731
+        f32 = lambda: self.field(torch.float32)  # noqa: E731
         # populations: f_b starts at zero, as a freshly allocated Taichi field does
-        self.f_a = DeviceField.zeros((Q,) + s3, torch.float32, dev)
-        self.f_b = DeviceField.zeros((Q,) + s3, torch.float32, dev)
+        self.f_a = self.field(torch.float32, leading=(Q,))
+        self.f_b = self.field(torch.float32, leading=(Q,))
         for name in ("rho", "T", "T_max", "H", "f_l", "phi", "cp_rho_field", "alpha_lu_field",
                      "dgamma_lu_field", "tau_field", "dT_dt", "T_prev", "time_above_800_s",
                      "time_above_1100_s", "time_above_solidus_s", "Fx_snap", "Fy_snap",
                      "Fz_snap", "ux", "uy", "uz", "Fx", "Fy", "Fz"):
             setattr(self, name, f32())
-        self.flags = DeviceField.zeros(s3, torch.uint8, dev, np_dtype=np.int32)
+        self.flags = self.field(torch.uint8, np_dtype=np.int32)
         # Jacobi partner of H (thermal advection reads the pre-step enthalpy)
         self.H_tmp = f32()
         # 0-d buffers of the reference (grid.py:160-172)
@@ -106,18 +113,23 @@ class WAAMGrid:
         self._read_buf = self.f_a
         self._write_buf = self.f_b
 
+    def field(self, dtype, np_dtype=None, leading: tuple = ()) -> DeviceField:
+        """New zero grid field in this grid's layout (logical shape leading + (nx, ny, nz))."""
+        return DeviceField.zeros(tuple(leading) + self._shape3, dtype, self.device,
+                                 np_dtype=np_dtype, layout=self.layout)
+
     # ------------------------------------------------------------------ optional buffers
     def ensure_vof_buffers(self) -> None:
         if self._has_vof_buf:
             return
-        self.phi_tmp = DeviceField.zeros(self._shape3, torch.float32, self.device)
+        self.phi_tmp = self.field(torch.float32)
         self._has_vof_buf = True
         self._n_optional_volume += 1
 
     def ensure_export_buffers(self) -> None:
         if self._has_export:
             return
-        self.kappa_field = DeviceField.zeros(self._shape3, torch.float32, self.device)
+        self.kappa_field = self.field(torch.float32)
         self._has_export = True
         self._n_optional_volume += 1
 

@@ -18,7 +18,7 @@ import math
 import torch
 
 from ._lib import FLAG_FLUID, FLAG_GAS, FLAG_IFACE, FLAG_SOLID, WaamError, lib
-from .field import DeviceField, P, tensor_of
+from .field import DeviceField, P, dims_of, layout_of, tensor_of
 
 _FLAGS = {"FLAG_FLUID": FLAG_FLUID, "FLAG_SOLID": FLAG_SOLID, "FLAG_GAS": FLAG_GAS,
           "FLAG_IFACE": FLAG_IFACE}
@@ -69,13 +69,13 @@ def _scratch_of(*fields):
 def init_grid(f, rho, ux, uy, uz, T, H, f_l, phi, flags, T_ambient, rho0, cp_rho, nz_solid,
               plate_i0, plate_i1, plate_j0, plate_j1, FLAG_FLUID, FLAG_SOLID, FLAG_GAS):
     _flags_ok(FLAG_FLUID=FLAG_FLUID, FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
-    nx, ny, nz = tensor_of(rho).shape
+    nx, ny, nz = dims_of(rho)
     n = nx * ny * nz
     _need_pop(n, f)
     _need(n, rho, ux, uy, uz, T, H, f_l, phi, flags)
     _call("waam_init_grid", P(f), P(rho), P(ux), P(uy), P(uz), P(T), P(H), P(f_l), P(phi),
           P(flags), T_ambient, rho0, cp_rho, nz_solid, plate_i0, plate_i1, plate_j0, plate_j1,
-          nx, ny, nz, _stream())
+          nx, ny, nz, layout_of(f), _stream())
 
 
 def init_aux_fields(T_max, T_prev, dT_dt, time_above_800, time_above_1100, time_above_solidus,
@@ -99,11 +99,11 @@ def clear_forces(Fx, Fy, Fz):
 def inject_arc_heat(H, flags, phi, f_l, norm_buf, arc_i, arc_j, arc_k, Q_w, sigma, dt, dx3, eta,
                     penetration_cells, enable_surface_weight, FLAG_SOLID, FLAG_GAS, scratch=None):
     _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
-    nx, ny, nz = tensor_of(H).shape
+    nx, ny, nz = dims_of(H)
     _need(nx * ny * nz, flags, f_l)
     sc = scratch if scratch is not None else _scratch_of(H)
     _call("waam_inject_arc_heat", P(H), P(flags), P(f_l), P(norm_buf), arc_i, arc_j, arc_k, Q_w,
-          sigma, dt, dx3, eta, penetration_cells, int(enable_surface_weight), nx, ny, nz, P(sc),
+          sigma, dt, dx3, eta, penetration_cells, int(enable_surface_weight), nx, ny, nz, layout_of(H), P(sc),
           _stream())
 
 
@@ -111,21 +111,21 @@ def inject_goldak_heat(H, flags, phi, f_l, norm_buf, arc_i, arc_j, arc_k, Q_w, d
                        travel_sign, ff, fr, a_front, a_rear, b_axis, c_axis, penetration_cells,
                        enable_surface_weight, FLAG_SOLID, FLAG_GAS, scratch=None):
     _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
-    nx, ny, nz = tensor_of(H).shape
+    nx, ny, nz = dims_of(H)
     _need(nx * ny * nz, flags, f_l)
     sc = scratch if scratch is not None else _scratch_of(H)
     _call("waam_inject_goldak_heat", P(H), P(flags), P(f_l), P(norm_buf), arc_i, arc_j, arc_k,
           Q_w, dt, dx3, eta, travel_sign, ff, fr, a_front, a_rear, b_axis, c_axis,
-          penetration_cells, int(enable_surface_weight), nx, ny, nz, P(sc), _stream())
+          penetration_cells, int(enable_surface_weight), nx, ny, nz, layout_of(H), P(sc), _stream())
 
 
 def surface_height_at(phi, flags, out, i0, j0, nz_fallback, FLAG_GAS, nz):
     _flags_ok(FLAG_GAS=FLAG_GAS)
-    nx, ny, nz_ = tensor_of(phi).shape
+    nx, ny, nz_ = dims_of(phi)
     if nz != nz_:
         raise ValueError("nz does not match phi")
     _call("waam_surface_height_at", P(phi), P(flags), P(out), int(i0), int(j0), int(nz_fallback),
-          nx, ny, nz, _stream())
+          nx, ny, nz, layout_of(phi), _stream())
 
 
 # ------------------------------------------------------------------ thermal
@@ -147,7 +147,7 @@ def advect_diffuse_temperature_into(H_in, H_out, T, ux, uy, uz, flags, alpha_lu,
                                     ny, nz):
     _need(nx * ny * nz, H_in, H_out, T, ux, uy, uz, flags)
     _call("waam_advect_diffuse_temperature", P(H_in), P(H_out), P(T), P(ux), P(uy), P(uz),
-          P(flags), alpha_lu, dt, cp_rho, nx, ny, nz, _stream())
+          P(flags), alpha_lu, dt, cp_rho, nx, ny, nz, layout_of(H_in), _stream())
 
 
 def advect_diffuse_temperature(H, T, ux, uy, uz, flags, alpha_lu, dt, FLAG_SOLID, FLAG_GAS,
@@ -163,7 +163,7 @@ def apply_thermal_boundary_losses(H, T, flags, T_amb, h_conv, eps_rad, enable_co
     _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
     _need(nx * ny * nz, H, T, flags)
     _call("waam_apply_thermal_boundary_losses", P(H), P(T), P(flags), T_amb, h_conv, eps_rad,
-          int(enable_conv), int(enable_rad), cp_rho, dt, dx, sigma_sb, nx, ny, nz, _stream())
+          int(enable_conv), int(enable_rad), cp_rho, dt, dx, sigma_sb, nx, ny, nz, layout_of(H), _stream())
 
 
 def update_phase(H, T, f_l, cp_rho, L_rho, T_solidus, T_liquidus):
@@ -202,7 +202,7 @@ def sync_T_from_H(H, T, cp_rho):
 def advect_phi(phi_dst, phi_src, ux, uy, uz, flags, FLAG_SOLID, FLAG_GAS, nx, ny, nz):
     _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
     _need(nx * ny * nz, phi_dst, phi_src, ux, uy, uz, flags)
-    _call("waam_advect_phi", P(phi_dst), P(phi_src), P(ux), P(uy), P(uz), P(flags), nx, ny, nz,
+    _call("waam_advect_phi", P(phi_dst), P(phi_src), P(ux), P(uy), P(uz), P(flags), nx, ny, nz, layout_of(phi_dst),
           _stream())
 
 
@@ -223,7 +223,7 @@ def contact_angle_cos_sin(theta_rad: float) -> tuple[float, float]:
 def apply_contact_angle_phi_bc_into(phi_in, phi_out, flags, theta_rad, nx, ny, nz):
     _need(nx * ny * nz, phi_in, phi_out, flags)
     c, s = contact_angle_cos_sin(theta_rad)
-    _call("waam_apply_contact_angle_phi_bc", P(phi_in), P(phi_out), P(flags), c, s, nx, ny, nz,
+    _call("waam_apply_contact_angle_phi_bc", P(phi_in), P(phi_out), P(flags), c, s, nx, ny, nz, layout_of(phi_in),
           _stream())
 
 
@@ -237,9 +237,9 @@ def update_flags_from_phi(phi, f_l, flags, nz_solid, FLAG_FLUID, FLAG_SOLID, FLA
                           FLAG_IFACE):
     _flags_ok(FLAG_FLUID=FLAG_FLUID, FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS,
               FLAG_IFACE=FLAG_IFACE)
-    nx, ny, nz = tensor_of(phi).shape
+    nx, ny, nz = dims_of(phi)
     _need(nx * ny * nz, f_l, flags)
-    _call("waam_update_flags_from_phi", P(phi), P(f_l), P(flags), int(nz_solid), nx, ny, nz,
+    _call("waam_update_flags_from_phi", P(phi), P(f_l), P(flags), int(nz_solid), nx, ny, nz, layout_of(phi),
           _stream())
 
 
@@ -269,11 +269,11 @@ def solidify_trailing_pool_scalar(T, H, f_l, phi, flags, ux, uy, uz, cp_rho, arc
                                   dir_x, dir_y, dir_z, lookback_cells, T_freeze, T_solidus,
                                   FLAG_SOLID, FLAG_FLUID, FLAG_GAS):
     _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_FLUID=FLAG_FLUID, FLAG_GAS=FLAG_GAS)
-    nx, ny, nz = tensor_of(T).shape
+    nx, ny, nz = dims_of(T)
     _need(nx * ny * nz, H, f_l, phi, flags, ux, uy, uz)
     _call("waam_solidify_trailing_pool_scalar", P(T), P(H), P(f_l), P(phi), P(flags), P(ux),
           P(uy), P(uz), cp_rho, arc_i, arc_j, arc_k, dir_x, dir_y, dir_z, lookback_cells,
-          T_freeze, T_solidus, nx, ny, nz, _stream())
+          T_freeze, T_solidus, nx, ny, nz, layout_of(T), _stream())
 
 
 # ------------------------------------------------------------------ forces
@@ -283,7 +283,7 @@ def compute_csf_tension(phi, flags, Fx, Fy, Fz, gamma_lu, FLAG_SOLID, FLAG_GAS, 
     _need(nx * ny * nz, phi, flags, Fx, Fy, Fz)
     c, s = contact_angle_cos_sin(theta_rad)
     _call("waam_compute_csf_tension", P(phi), P(flags), P(Fx), P(Fy), P(Fz), gamma_lu,
-          int(enable_wetting), c, s, nx, ny, nz, _stream())
+          int(enable_wetting), c, s, nx, ny, nz, layout_of(phi), _stream())
 
 
 compute_csf_wetting = compute_csf_tension
@@ -292,7 +292,7 @@ compute_csf_wetting = compute_csf_tension
 def compute_curvature_field(phi, flags, kappa_out, FLAG_GAS, nx, ny, nz):
     _flags_ok(FLAG_GAS=FLAG_GAS)
     _need(nx * ny * nz, phi, flags, kappa_out)
-    _call("waam_compute_curvature_field", P(phi), P(flags), P(kappa_out), nx, ny, nz, _stream())
+    _call("waam_compute_curvature_field", P(phi), P(flags), P(kappa_out), nx, ny, nz, layout_of(phi), _stream())
 
 
 def compute_marangoni_force(T, phi, f_l, Fx, Fy, Fz, flags, dgamma_dT, dx, FLAG_SOLID, FLAG_GAS,
@@ -300,7 +300,7 @@ def compute_marangoni_force(T, phi, f_l, Fx, Fy, Fz, flags, dgamma_dT, dx, FLAG_
     _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
     _need(nx * ny * nz, T, phi, f_l, Fx, Fy, Fz, flags)
     _call("waam_compute_marangoni_force", P(T), P(phi), P(f_l), P(Fx), P(Fy), P(Fz), P(flags),
-          dgamma_dT, nx, ny, nz, _stream())
+          dgamma_dT, nx, ny, nz, layout_of(T), _stream())
 
 
 def add_buoyancy(T, Fz, f_l, flags, g_lu, beta, T_ref, rho_ref, FLAG_SOLID, FLAG_GAS):
@@ -320,10 +320,10 @@ def add_hydrostatic_gravity(Fz, f_l, flags, rho_ref, g_lu, FLAG_SOLID, FLAG_GAS)
 def apply_arc_pressure(Fz, flags, phi, arc_i, arc_j, arc_k, sigma, pressure_pa, dt, dx, rho_ref,
                        FLAG_SOLID, FLAG_GAS):
     _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
-    nx, ny, nz = tensor_of(Fz).shape
+    nx, ny, nz = dims_of(Fz)
     _need(nx * ny * nz, flags, phi)
     _call("waam_apply_arc_pressure", P(Fz), P(flags), P(phi), arc_i, arc_j, arc_k, sigma,
-          pressure_pa, dt, dx, rho_ref, nx, ny, nz, _stream())
+          pressure_pa, dt, dx, rho_ref, nx, ny, nz, layout_of(Fz), _stream())
 
 
 def clamp_body_force_magnitude(Fx, Fy, Fz, flags, F_max_lu, FLAG_SOLID, FLAG_GAS):
@@ -336,10 +336,10 @@ def clamp_body_force_magnitude(Fx, Fy, Fz, flags, F_max_lu, FLAG_SOLID, FLAG_GAS
 def apply_droplet_impact_pressure(Fz, flags, phi, f_l, arc_i, arc_j, arc_k, drop_radius,
                                   impact_pa, dt, dx, rho_ref, FLAG_SOLID, FLAG_GAS):
     _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
-    nx, ny, nz = tensor_of(Fz).shape
+    nx, ny, nz = dims_of(Fz)
     _need(nx * ny * nz, flags, phi, f_l)
     _call("waam_apply_droplet_impact_pressure", P(Fz), P(flags), P(phi), P(f_l), arc_i, arc_j,
-          arc_k, drop_radius, impact_pa, dt, dx, rho_ref, nx, ny, nz, _stream())
+          arc_k, drop_radius, impact_pa, dt, dx, rho_ref, nx, ny, nz, layout_of(Fz), _stream())
 
 
 def snapshot_forces(Fx, Fy, Fz, Fx_snap, Fy_snap, Fz_snap):
@@ -355,7 +355,7 @@ def collide_mrt(f_src, f_dst, rho, ux, uy, uz, Fx, Fy, Fz, f_l, flags, EX, EY, E
     _need_pop(n, f_src, f_dst)
     _need(n, rho, ux, uy, uz, Fx, Fy, Fz, f_l, flags)
     _call("waam_collide_mrt", P(f_src), P(f_dst), P(rho), P(ux), P(uy), P(uz), P(Fx), P(Fy),
-          P(Fz), P(f_l), P(flags), omega_s, omega_b, C_darcy, nx, ny, nz, _stream())
+          P(Fz), P(f_l), P(flags), omega_s, omega_b, C_darcy, nx, ny, nz, layout_of(f_src), _stream())
 
 
 def collide_srt(f_src, f_dst, rho, ux, uy, uz, Fx, Fy, Fz, f_l, flags, tau, omega, dt_lu, C_darcy,
@@ -365,7 +365,7 @@ def collide_srt(f_src, f_dst, rho, ux, uy, uz, Fx, Fy, Fz, f_l, flags, tau, omeg
     _need_pop(n, f_src, f_dst)
     _need(n, rho, ux, uy, uz, Fx, Fy, Fz, f_l, flags)
     _call("waam_collide_srt", P(f_src), P(f_dst), P(rho), P(ux), P(uy), P(uz), P(Fx), P(Fy),
-          P(Fz), P(f_l), P(flags), tau, omega, dt_lu, C_darcy, nx, ny, nz, _stream())
+          P(Fz), P(f_l), P(flags), tau, omega, dt_lu, C_darcy, nx, ny, nz, layout_of(f_src), _stream())
 
 
 def collide_srt_variable_tau(f_src, f_dst, rho, ux, uy, uz, Fx, Fy, Fz, f_l, flags, tau_field,
@@ -375,7 +375,7 @@ def collide_srt_variable_tau(f_src, f_dst, rho, ux, uy, uz, Fx, Fy, Fz, f_l, fla
     _need_pop(n, f_src, f_dst)
     _need(n, rho, ux, uy, uz, Fx, Fy, Fz, f_l, flags, tau_field)
     _call("waam_collide_srt_variable_tau", P(f_src), P(f_dst), P(rho), P(ux), P(uy), P(uz), P(Fx),
-          P(Fy), P(Fz), P(f_l), P(flags), P(tau_field), C_darcy, nx, ny, nz, _stream())
+          P(Fy), P(Fz), P(f_l), P(flags), P(tau_field), C_darcy, nx, ny, nz, layout_of(f_src), _stream())
 
 
 def stream(f_src, f_dst, flags, FLAG_SOLID, FLAG_GAS, nx, ny, nz):
@@ -383,7 +383,7 @@ def stream(f_src, f_dst, flags, FLAG_SOLID, FLAG_GAS, nx, ny, nz):
     n = nx * ny * nz
     _need_pop(n, f_src, f_dst)
     _need(n, flags)
-    _call("waam_stream", P(f_src), P(f_dst), P(flags), nx, ny, nz, _stream())
+    _call("waam_stream", P(f_src), P(f_dst), P(flags), nx, ny, nz, layout_of(f_src), _stream())
 
 
 def clamp_velocity_mach(ux, uy, uz, flags, u_max_lu, FLAG_SOLID, FLAG_GAS):
@@ -403,7 +403,7 @@ def feed_wire_surface(f_src, flags, f_l, phi, H, T, rho, arc_i, arc_j, arc_k, fo
     sc = scratch if scratch is not None else _scratch_of(H)
     _call("waam_feed_wire_surface", P(f_src), P(flags), P(f_l), P(phi), P(H), P(T), P(rho), arc_i,
           arc_j, arc_k, footprint_r, droplet_radius, target_vol, T_drop, cp_rho, L_rho, rho0,
-          P(vol_acc), P(real_acc), cell_vol, nx, ny, nz, P(sc), _stream())
+          P(vol_acc), P(real_acc), cell_vol, nx, ny, nz, layout_of(f_src), P(sc), _stream())
 
 
 def feed_wire_momentum_impact(f_src, rho, ux, uy, uz, flags, f_l, arc_i, arc_j, arc_k,
@@ -413,7 +413,7 @@ def feed_wire_momentum_impact(f_src, rho, ux, uy, uz, flags, f_l, arc_i, arc_j, 
     _need_pop(n, f_src)
     _need(n, rho, ux, uy, uz, flags, f_l)
     _call("waam_feed_wire_momentum_impact", P(f_src), P(rho), P(ux), P(uy), P(uz), P(flags),
-          P(f_l), arc_i, arc_j, arc_k, droplet_radius, vx_lu, vy_lu, vz_lu, nx, ny, nz, _stream())
+          P(f_l), arc_i, arc_j, arc_k, droplet_radius, vx_lu, vy_lu, vz_lu, nx, ny, nz, layout_of(f_src), _stream())
 
 
 def inject_tracers(pos, active, head, max_tracers, arc_x, arc_y, arc_z, sigma_m, spawn_count,
@@ -427,23 +427,24 @@ def inject_tracers(pos, active, head, max_tracers, arc_x, arc_y, arc_z, sigma_m,
 def advect_tracers(pos, active, ux, uy, uz, f_l, flags, dx, dt, max_tracers, FLAG_SOLID,
                    FLAG_GAS):
     _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
-    nx, ny, nz = tensor_of(ux).shape
+    nx, ny, nz = dims_of(ux)
     if tensor_of(pos).numel() < 3 * max_tracers or tensor_of(active).numel() < max_tracers:
         raise ValueError("tracer buffers smaller than max_tracers")
     _call("waam_advect_tracers", P(pos), P(active), P(ux), P(uy), P(uz), P(f_l), P(flags), dx, dt,
-          int(max_tracers), nx, ny, nz, _stream())
+          int(max_tracers), nx, ny, nz, layout_of(pos), _stream())
 
 
 # ------------------------------------------------------------------ telemetry
 def telemetry_pool_reduce_into(T, f_l, phi, dT_dt, ux, uy, uz, out, scratch):
-    nx, ny, nz = tensor_of(T).shape
+    nx, ny, nz = dims_of(T)
     _need(nx * ny * nz, f_l, phi, dT_dt, ux, uy, uz)
     _call("waam_telemetry_pool_reduce", P(T), P(f_l), P(phi), P(dT_dt), P(ux), P(uy), P(uz),
-          P(out), nx, ny, nz, P(scratch), _stream())
+          P(out), nx, ny, nz, layout_of(T), P(scratch), _stream())
 
 
 def extract_fl_yz_slice(f_l, out, i_slice, ny, nz):
-    tensor_of(out).copy_(tensor_of(f_l)[int(i_slice)])
+    tensor_of(out).copy_(f_l[int(i_slice)] if isinstance(f_l, DeviceField)
+                         else tensor_of(f_l)[int(i_slice)])
 
 
 __all__ = [n for n in dir() if not n.startswith("_") and n not in ("math", "torch")]

@@ -87,7 +87,7 @@ def _droplet(twin, g, x_m: float, y_m: float, arc_i: float, arc_j: float, arc_k:
     lib().call("waam_deposit_droplet", P(g.f_src), P(g.flags), P(g.f_l), P(g.phi), P(g.H), P(g.T),
                P(g.rho), arc_i, arc_j, arc_k, float(foot_r), drop_r, float(drop_vol), T_drop,
                twin.cp_rho, twin.L_rho, 1.0, P(g.deposit_vol_buf), P(g.deposit_real_buf),
-               g.dx ** 3, P(g.deposit_result), g.nx, g.ny, g.nz, P(g.scratch),
+               g.dx ** 3, P(g.deposit_result), g.nx, g.ny, g.nz, g.layout, P(g.scratch),
                torch.cuda.current_stream().cuda_stream)
     placed = float(g.deposit_result[0])
     if placed < drop_vol * 0.98:

@@ -32,7 +32,7 @@ class StepFields(ctypes.Structure):
         "fl_out", "phi", "phi_tmp", "flags_in", "flags_out", "sflags_prev", "sflags_next", "Fx",
         "Fy", "Fz", "T_max", "T_prev", "dT_dt", "t800", "t1100", "tsol", "tracer_pos",
         "tracer_active", "list_thermal", "list_collide", "scratch")] + [
-            (n, _i32) for n in ("max_tracers", "nx", "ny", "nz")]
+            (n, _i32) for n in ("max_tracers", "nx", "ny", "nz", "layout")]
 
 
 _INT_PARAMS = ("is_welding", "collision", "arc_kind", "arc_applied", "pending_stream",
@@ -63,15 +63,14 @@ class FusedStepper:
         self.twin = twin
         g = twin.grid
         g.ensure_vof_buffers()
-        s3 = (g.nx, g.ny, g.nz)
         for name in ("T_tmp", "fl_tmp"):
             if not hasattr(g, name):
-                setattr(g, name, DeviceField.zeros(s3, torch.float32, g.device))
+                setattr(g, name, g.field(torch.float32))
         if not hasattr(g, "flags_tmp"):
-            g.flags_tmp = DeviceField.zeros(s3, torch.uint8, g.device, np_dtype=np.int32)
+            g.flags_tmp = g.field(torch.uint8, np_dtype=np.int32)
         if not hasattr(g, "sflags"):
-            g.sflags = DeviceField.zeros(s3, torch.uint8, g.device, np_dtype=np.int32)
-            g.sflags_tmp = DeviceField.zeros(s3, torch.uint8, g.device, np_dtype=np.int32)
+            g.sflags = g.field(torch.uint8, np_dtype=np.int32)
+            g.sflags_tmp = g.field(torch.uint8, np_dtype=np.int32)
         if not hasattr(g, "list_thermal"):
             # compacted FLUID/IFACE cell indices written and consumed inside each fused step
             g.list_thermal = DeviceField.zeros((g.n_cells,), torch.int32, g.device)
@@ -88,7 +87,7 @@ class FusedStepper:
             return
         g = self.twin.grid
         lib().call("waam_stream_pending", P(g.f_src), P(g.f_dst), P(g.sflags), g.nx, g.ny, g.nz,
-                   torch.cuda.current_stream().cuda_stream)
+                   g.layout, torch.cuda.current_stream().cuda_stream)
         self.pending = False
 
     def reset(self) -> None:
@@ -126,6 +125,7 @@ class FusedStepper:
                           ("scratch", g.scratch)):
             setattr(sf, name, P(fld))
         sf.max_tracers, sf.nx, sf.ny, sf.nz = g.max_tracers, g.nx, g.ny, g.nz
+        sf.layout = g.layout
         return sf
 
     def _params(self, arc_i, arc_j, is_welding, torch_z_m) -> StepParams:

@@ -18,6 +18,7 @@ from typing import Any
 import numpy as np
 
 from . import kernels
+from .field import LAYOUT_ZYX
 from .grid import WAAMGrid
 from .materials import MaterialProps, load_material
 from .physics.arc import create_heat_source
@@ -65,7 +66,7 @@ class WAAMTwin:
                  enable_enthalpy_cap: bool = True, T_vapor_cap_K: float = 3200.0,
                  arc_surface_weighting: bool = True, arc_penetration_mm: float = 2.0,
                  wire_diameter_mm: float = 1.2, arc_sigma_mm: float = 2.0,
-                 bulk_tau: float | None = None, device=None):
+                 bulk_tau: float | None = None, device=None, field_layout: int | None = None):
         if arc_power_W < 0:
             raise ValueError(f"arc_power_W must be >= 0, got {arc_power_W}")
         if not (0.0 < arc_efficiency <= 1.0):
@@ -98,8 +99,11 @@ class WAAMTwin:
         self.enable_csf_tension = enable_csf_tension
         self.enable_deposition_momentum = enable_deposition_momentum
         self.enable_lorentz, self.enable_gas_shear = enable_lorentz, enable_gas_shear
+        # field storage: rows along x (default) or the reference's k-fastest order; the fields'
+        # accessors present (nx, ny, nz) either way
         self.grid = WAAMGrid(nx, ny, nz, dx, self.mat, max_tracers=max_tracers,
-                             allocate_vof=enable_vof, device=device)
+                             allocate_vof=enable_vof, device=device,
+                             layout=LAYOUT_ZYX if field_layout is None else field_layout)
         self.enable_droplet_impact_pressure = enable_droplet_impact_pressure
         self.use_recoil_clausius_clapeyron = use_recoil_clausius_clapeyron
         self.enable_substrate_growth = enable_substrate_growth
