@@ -1,0 +1,403 @@
+"""Pinning the oracle: checks that fix its values from outside the restatement.
+
+Each check compares the oracle's fp32 output with something derived independently of the
+restated code: moment identities of the collision operators worked out from their published
+definitions (SURVEY-level formulas in the reference's docstrings: cumulant_kernel.py:28-209,
+kernels.py:2778-2885; Guo et al. 2002 for the forcing), brute-force definitions on tiny grids
+(stream, telemetry), closed forms per cell (losses, arc pressure, buoyancy, clamps), and literal
+expected values at threshold boundaries (flag classification, contact-angle rules, freeze
+rules). A plausible mistake in the restatement (a dropped term, a wrong weight, a swapped
+relaxation rate, a strict/non-strict comparison, a wrong index) fails one of them.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle.oracle import EX, EY, EZ, FLUID, GAS, IFACE, OPP, SOLID, W, lib, ptr
+
+E = np.array([EX, EY, EZ], dtype=np.float64)  # (3, 19)
+Wd = np.array(W, dtype=np.float64)
+
+
+def _cells(n, seed, rho_spread=0.02, u_max=0.05, neq=0.01):
+    """n random near-equilibrium population sets (19, n) in fp32."""
+    rng = np.random.default_rng(seed)
+    rho = 1.0 + rho_spread * rng.uniform(-1, 1, n)
+    u = u_max * rng.uniform(-1, 1, (3, n))
+    eu = E.T @ u                                        # (19, n)
+    feq = Wd[:, None] * rho * (1 + 3 * eu + 4.5 * eu ** 2 - 1.5 * (u ** 2).sum(0))
+    f = feq * (1 + neq * rng.uniform(-1, 1, feq.shape))
+    return f.astype(np.float32)
+
+
+def _collide(kind, f, F, fl, omega_s, omega_b=None, C=0.0):
+    n = f.shape[1]
+    L = lib()
+    f_dst = np.zeros_like(f)
+    rho, ux, uy, uz = (np.zeros(n, np.float32) for _ in range(4))
+    Fx, Fy, Fz = (np.ascontiguousarray(F[a].astype(np.float32)) for a in range(3))
+    flv = np.ascontiguousarray(fl.astype(np.float32))
+    flags = np.full(n, FLUID, np.uint8)
+    if kind == "mrt":
+        L.or_collide_mrt(ptr(f), ptr(f_dst), ptr(rho), ptr(ux), ptr(uy), ptr(uz), ptr(Fx), ptr(Fy),
+                         ptr(Fz), ptr(flv), ptr(flags), omega_s, omega_b, C, n, 1, 1)
+    else:
+        tau = 1.0 / omega_s
+        L.or_collide_srt(ptr(f), ptr(f_dst), ptr(rho), ptr(ux), ptr(uy), ptr(uz), ptr(Fx), ptr(Fy),
+                         ptr(Fz), ptr(flv), ptr(flags), tau, omega_s, 1.0, C, n, 1, 1)
+    return f_dst, rho, np.stack([ux, uy, uz])
+
+
+# ---------------------------------------------------------------------------- collision
+@pytest.mark.parametrize("omega_s,omega_b", [(1.2, 1.2), (1.7, 0.9), (0.8, 1.5)])
+def test_mrt_moments_follow_the_regularized_reconstruction(omega_s, omega_b):
+    """Moments of f' against the closed forms of the reference's reconstruction
+    f'_q = w_q[rho + 3 rho e.u + 4.5 rho (e.u)^2 - 1.5 rho u^2 + 4.5 e e : D]
+           + w_q (1 - w_s/2)[3 e.F + 9 (e.F)(e.u) - 3 u.F],
+    D = k' - rho/3 I, k = P - rho u u, trace of k relaxed at w_b, the rest at w_s. With the
+    D3Q19 weight moments (sum w e e = I/3, sum w e_a e_b e_c e_d = (dd + dd + dd)/9):
+      sum f'     = rho + 1.5 tr D
+      sum e f'   = rho u + (1 - w_s/2) F
+      sum e e f' = rho/3 I + rho u u + D + 0.5 tr(D) I + (1 - w_s/2)(u F + F u)."""
+    n = 64
+    rng = np.random.default_rng(5)
+    f = _cells(n, 1)
+    F = 1e-4 * rng.uniform(-1, 1, (3, n))
+    fo, rho, u = _collide("mrt", f, F, np.ones(n), omega_s, omega_b)
+    fd, fo = f.astype(np.float64), fo.astype(np.float64)
+    r, u, F = rho.astype(np.float64), u.astype(np.float64), F.astype(np.float32).astype(np.float64)
+    P = np.einsum("aq,bq,qn->abn", E, E, fd)
+    k = P - r * np.einsum("an,bn->abn", u, u)
+    km = np.trace(k) / 3.0
+    kmp = km - omega_b * (km - r / 3.0)
+    I = np.eye(3)[:, :, None]
+    kp = kmp * I + (1 - omega_s) * (k - km * I)
+    D = kp - (r / 3.0) * I
+    trD = np.trace(D)
+    gs = 1 - omega_s / 2
+    m0 = fo.sum(0)
+    m1 = E @ fo
+    m2 = np.einsum("aq,bq,qn->abn", E, E, fo)
+    exp_m2 = (r / 3.0) * I + r * np.einsum("an,bn->abn", u, u) + D + 0.5 * trD * I + gs * (
+        np.einsum("an,bn->abn", u, F) + np.einsum("an,bn->abn", F, u))
+    assert np.allclose(m0, r + 1.5 * trD, rtol=0, atol=2e-6)
+    assert np.allclose(m1, r * u + gs * F, rtol=0, atol=2e-6)
+    assert np.allclose(m2, exp_m2, rtol=0, atol=3e-6)
+    assert np.allclose(r, fd.sum(0), rtol=0, atol=2e-6)
+
+
+def test_mrt_rest_state_is_fixed():
+    n = 8
+    rho = np.linspace(0.98, 1.02, n)
+    f = (Wd[:, None] * rho).astype(np.float32)
+    fo, r, u = _collide("mrt", f, np.zeros((3, n)), np.ones(n), 1.3, 0.7)
+    assert np.abs(u).max() == 0.0
+    assert np.allclose(fo, f, rtol=1e-6, atol=0)   # fp32 rounding of the moment sums
+
+
+@pytest.mark.parametrize("omega", [0.6, 1.0, 1.85])
+def test_srt_is_bgk_with_guo_forcing(omega):
+    """Guo, Zheng & Shi (2002): f' = f - w (f - f_eq(rho, u)) + (1 - w/2) w_q [3 (e - u).F
+    + 9 (e.u)(e.F)], u = (j + F/2)/rho, written from the paper in fp64; also j' = j + F."""
+    n = 64
+    rng = np.random.default_rng(9)
+    f = _cells(n, 2)
+    F = (1e-4 * rng.uniform(-1, 1, (3, n))).astype(np.float32).astype(np.float64)
+    fo, rho, u_out = _collide("srt", f, F, np.ones(n), omega)
+    fd = f.astype(np.float64)
+    r = fd.sum(0)
+    j = E @ fd
+    u = (j + 0.5 * F) / r
+    eu = E.T @ u
+    feq = Wd[:, None] * r * (1 + 3 * eu + 4.5 * eu ** 2 - 1.5 * (u ** 2).sum(0))
+    eF = E.T @ F
+    S = (1 - omega / 2) * Wd[:, None] * (3 * (eF - (u * F).sum(0)) + 9 * eu * eF)
+    expect = fd - omega * (fd - feq) + S
+    assert np.allclose(fo, expect, rtol=0, atol=3e-7)
+    assert np.allclose(u_out, u, rtol=0, atol=1e-7)
+    assert np.allclose(fo.astype(np.float64).sum(0), r, atol=2e-6)
+    assert np.allclose(E @ fo.astype(np.float64), j + F, atol=2e-6)
+
+
+def test_darcy_damping_scales_velocity():
+    """u = u_raw / (1 + C (1 - fl)^2 / (fl^3 + 1e-6)) (kernels.py:2815-2820)."""
+    n = 16
+    f = _cells(n, 3)
+    fl = np.linspace(0.05, 1.0, n)
+    C = 1.6e5
+    _, _, u = _collide("srt", f, np.zeros((3, n)), fl, 1.1, C=C)
+    _, _, u0 = _collide("srt", f, np.zeros((3, n)), np.ones(n), 1.1, C=0.0)
+    ck = 1.0 + C * (1 - fl) ** 2 / (fl ** 3 + 1e-6)
+    assert np.allclose(u, u0 / ck, rtol=3e-6, atol=1e-12)
+
+
+# ---------------------------------------------------------------------------- stream
+def test_stream_matches_brute_force_definition():
+    """Pull, x periodic, y/z clamped; GAS pulls unconditionally; other non-SOLID cells bounce
+    back off SOLID neighbours; SOLID untouched (kernels.py:2971-3010), on a 5x4x3 grid."""
+    nx, ny, nz = 5, 4, 3
+    N = nx * ny * nz
+    rng = np.random.default_rng(4)
+    f = rng.random((19, nx, ny, nz)).astype(np.float32)
+    flags = rng.choice(np.array([FLUID, SOLID, GAS, IFACE], np.uint8), (nx, ny, nz))
+    out = np.full_like(f, -7.0)
+    lib().or_stream(ptr(f), ptr(out), ptr(flags), nx, ny, nz)
+    for i in range(nx):
+        for j in range(ny):
+            for k in range(nz):
+                fl = flags[i, j, k]
+                for q in range(19):
+                    if fl == SOLID:
+                        want = -7.0
+                    else:
+                        n_ = ((i - EX[q]) % nx, min(max(j - EY[q], 0), ny - 1),
+                              min(max(k - EZ[q], 0), nz - 1))
+                        if fl != GAS and flags[n_] == SOLID:
+                            want = f[OPP[q], i, j, k]
+                        else:
+                            want = f[(q,) + n_]
+                    assert out[q, i, j, k] == np.float32(want), (q, i, j, k)
+    assert N == out[0].size
+
+
+def test_stream_of_uniform_fluid_is_identity():
+    nx, ny, nz = 6, 5, 4
+    f = np.broadcast_to(np.arange(19, dtype=np.float32)[:, None, None, None],
+                        (19, nx, ny, nz)).copy()
+    flags = np.full((nx, ny, nz), FLUID, np.uint8)
+    out = np.zeros_like(f)
+    lib().or_stream(ptr(f), ptr(out), ptr(flags), nx, ny, nz)
+    assert np.array_equal(out, f)
+
+
+# ---------------------------------------------------------------------------- flags
+# (phi, f_l, k < nz_solid) -> expected flag, at the thresholds of kernels.py:2406-2419:
+# SOLID if k < nz_solid; SOLID if phi > 0.85 and f_l < 0.05; FLUID if phi > 0.55 and
+# f_l > 0.1; GAS if phi < 0.05; else IFACE. Values written out by hand, not computed.
+FLAG_TABLE = [
+    (0.0, 0.0, True, SOLID), (0.5, 0.9, True, SOLID),
+    (0.86, 0.04, False, SOLID), (0.85, 0.04, False, IFACE), (0.86, 0.05, False, IFACE),
+    (0.86, 0.11, False, FLUID), (0.56, 0.11, False, FLUID), (0.55, 0.11, False, IFACE),
+    (0.56, 0.10, False, IFACE), (0.049, 0.0, False, GAS), (0.05, 0.0, False, IFACE),
+    (0.0, 0.9, False, GAS), (0.3, 0.3, False, IFACE), (1.0, 1.0, False, FLUID),
+]
+
+
+def test_flag_classification_truth_table():
+    n = len(FLAG_TABLE)
+    # one column per row: k = 0 is below nz_solid = 1, k = 1 is above
+    phi = np.zeros((n, 1, 2), np.float32)
+    fl = np.zeros((n, 1, 2), np.float32)
+    for r, (p, f_, below, _) in enumerate(FLAG_TABLE):
+        kk = 0 if below else 1
+        phi[r, 0, kk] = p
+        fl[r, 0, kk] = f_
+    flags = np.zeros((n, 1, 2), np.uint8)
+    lib().or_update_flags_from_phi(ptr(phi), ptr(fl), ptr(flags), 1, n, 1, 2)
+    for r, (_, _, below, want) in enumerate(FLAG_TABLE):
+        assert flags[r, 0, 0 if below else 1] == want, FLAG_TABLE[r]
+
+
+def test_phi_liquid_fraction_sync_cases():
+    """f_l := 0 where GAS, phi < 0.05 or SOLID (kernels.py:3484-3504)."""
+    phi = np.array([0.5, 0.04, 0.05, 0.9, 0.9], np.float32)
+    flags = np.array([GAS, FLUID, FLUID, SOLID, IFACE], np.uint8)
+    fl = np.full(5, 0.7, np.float32)
+    lib().or_sync_phi_liquid_fraction(ptr(phi), ptr(fl), ptr(flags), 5, 1, 1)
+    assert fl.tolist() == [0.0, 0.0, np.float32(0.7), 0.0, np.float32(0.7)]
+
+
+# ---------------------------------------------------------------------------- thermal
+def test_boundary_losses_closed_form():
+    """A FLUID cell on a 3x3x3 block with GAS above and on its +x side has two exposed faces
+    (+z, +x; the -z face never counts): dH = -(h (Te - Ta) + eps sigma (Te^4 - Ta^4)) * 2 dt/dx,
+    Te = min(T, 2500) (kernels.py:2030-2082)."""
+    nx = ny = nz = 3
+    flags = np.full((nx, ny, nz), FLUID, np.uint8)
+    flags[:, :, 2] = GAS
+    flags[2, :, :] = GAS
+    T = np.full((nx, ny, nz), 300.0, np.float32)
+    T[1, 1, 1] = 2900.0
+    H = np.zeros((nx, ny, nz), np.float32)
+    h, eps, sig, Ta, dt, dx = 25.0, 0.3, 5.670374419e-8, 300.0, 2e-5, 2e-4
+    lib().or_apply_thermal_boundary_losses(ptr(H), ptr(T), ptr(flags), Ta, h, eps, 1, 1, 1.0, dt,
+                                           dx, sig, nx, ny, nz)
+    Te = 2500.0
+    q = h * (Te - Ta) + eps * sig * (Te ** 4 - Ta ** 4)
+    assert math.isclose(H[1, 1, 1], -q * 2 * dt / dx, rel_tol=2e-6)
+    # an interior cell with no GAS neighbour loses nothing
+    assert H[0, 0, 0] == 0.0 and H[1, 1, 0] == 0.0
+
+
+def test_enthalpy_floor_and_ceiling():
+    cp_rho, Ts, Tl, L, Tcap = 7800.0 * 680.0, 1723.0, 1780.0, 7800.0 * 2.7e5, 3200.0
+    H_sol = cp_rho * Ts
+    H_liq = H_sol + L
+    cap = H_liq + cp_rho * (Tcap - Tl)
+    H = np.array([0.0, cp_rho * 250.0, cap * 1.5, cap * 0.5, cap * 2.0], np.float32)
+    flags = np.array([FLUID, SOLID, FLUID, FLUID, GAS], np.uint8)
+    Hf = H.copy()
+    lib().or_clamp_enthalpy_floor_scalar(ptr(Hf), cp_rho, ptr(flags), 300.0, 5, 1, 1)
+    assert Hf[0] == np.float32(np.float32(cp_rho) * np.float32(300.0))
+    assert Hf[1] == np.float32(np.float32(cp_rho) * np.float32(300.0))
+    assert Hf[2] == H[2] and Hf[4] == H[4]
+    Hc = H.copy()
+    lib().or_clamp_enthalpy_ceiling_scalar(ptr(Hc), ptr(flags), cp_rho, Ts, Tl, Tcap, L, 5, 1, 1)
+    assert math.isclose(Hc[2], cap, rel_tol=1e-6)       # capped at T = T_cap
+    assert Hc[3] == H[3] and Hc[4] == H[4]              # below the cap / GAS untouched
+
+
+# ---------------------------------------------------------------------------- forces
+def test_arc_pressure_closed_form():
+    """Fz -= P/(rho dx) dt^2/dx exp(-(di^2 + dj^2 + 0.1 dk^2)/(2 s^2 + 1e-6)) where
+    |phi(k+1) - phi(k-1)| > 0.05 (kernels.py:3038-3083)."""
+    nx, ny, nz = 5, 5, 5
+    flags = np.full((nx, ny, nz), FLUID, np.uint8)
+    phi = np.zeros((nx, ny, nz), np.float32)
+    phi[:, :, :2] = 1.0                  # interface between k=1 and k=2 -> k = 1, 2 qualify
+    Fz = np.zeros((nx, ny, nz), np.float32)
+    ai, aj, ak, s, Pp, dt, dx, rho = 2.0, 1.5, 2.0, 1.7, 500.0, 2e-5, 2e-4, 7800.0
+    lib().or_apply_arc_pressure(ptr(Fz), ptr(flags), ptr(phi), ai, aj, ak, s, Pp, dt, dx, rho,
+                                nx, ny, nz)
+    peak = Pp / (rho * dx) * dt * dt / dx
+    for (i, j, k) in [(2, 2, 1), (0, 4, 2), (3, 1, 2)]:
+        r2 = (i - ai) ** 2 + (j - aj) ** 2 + 0.1 * (k - ak) ** 2
+        assert math.isclose(Fz[i, j, k], -peak * math.exp(-r2 / (2 * s * s + 1e-6)), rel_tol=3e-6)
+    assert Fz[2, 2, 4] == 0.0 and Fz[2, 2, 0] == 0.0
+
+
+def test_buoyancy_and_hydrostatic_closed_forms():
+    T = np.array([1900.0, 1700.0, 2000.0], np.float32)
+    fl = np.array([0.5, 0.5, 0.0], np.float32)
+    flags = np.array([FLUID, IFACE, FLUID], np.uint8)
+    Fz = np.zeros(3, np.float32)
+    g, beta, Tl = 4.9e-5, 1.2e-4, 1780.0
+    lib().or_add_buoyancy(ptr(T), ptr(Fz), ptr(fl), ptr(flags), g, beta, Tl, 1.0, 3, 1, 1)
+    assert math.isclose(Fz[0], g * beta * (1900.0 - Tl) * 0.5, rel_tol=2e-6)
+    assert math.isclose(Fz[1], g * beta * (1700.0 - Tl) * 0.5, rel_tol=2e-6)
+    assert Fz[2] == 0.0
+    Fh = np.zeros(3, np.float32)
+    lib().or_add_hydrostatic_gravity(ptr(Fh), ptr(fl), ptr(flags), 1.0, g, 3, 1, 1)
+    assert math.isclose(Fh[0], -g * 0.5, rel_tol=1e-6) and Fh[2] == 0.0
+
+
+def test_force_and_velocity_clamps_keep_direction():
+    rng = np.random.default_rng(8)
+    v = rng.normal(size=(3, 200)).astype(np.float32) * 0.1
+    flags = np.full(200, FLUID, np.uint8)
+    flags[:5] = SOLID
+    x, y, z = (np.ascontiguousarray(v[a]) for a in range(3))
+    lib().or_clamp_body_force_magnitude(ptr(x), ptr(y), ptr(z), ptr(flags), 0.05, 200, 1, 1)
+    m0 = np.sqrt((v.astype(np.float64) ** 2).sum(0))
+    m1 = np.sqrt(x.astype(np.float64) ** 2 + y.astype(np.float64) ** 2 + z.astype(np.float64) ** 2)
+    big = (m0 > 0.05) & (flags != SOLID)
+    assert np.all(m1[big] <= 0.05 * (1 + 1e-6)) and np.allclose(m1[big], 0.05, rtol=1e-6)
+    cosang = (x * v[0] + y * v[1] + z * v[2]) / (m0 * m1)
+    assert np.allclose(cosang[big], 1.0, atol=1e-6)
+    assert np.array_equal(x[~big], v[0][~big])
+
+
+# ---------------------------------------------------------------------------- VOF rules
+def _contact(phi_col, flags_col, theta_deg, lateral=None):
+    """Jacobi contact-angle BC on a 3x3 stack of copies of one column (lateral: phi of the
+    side columns at the centre's level)."""
+    nz = len(phi_col)
+    phi = np.tile(np.asarray(phi_col, np.float32), (3, 3, 1))
+    flags = np.tile(np.asarray(flags_col, np.uint8), (3, 3, 1))
+    if lateral is not None:
+        k, val = lateral
+        phi[:, :, k] = 0.0
+        phi[0, 1, k] = val
+    out = phi.copy()
+    t = math.radians(theta_deg)
+    lib().or_apply_contact_angle_phi_bc_jacobi(ptr(phi), ptr(out), ptr(flags),
+                                               np.float32(math.cos(t)), np.float32(math.sin(t)),
+                                               3, 3, nz)
+    return out[1, 1]
+
+
+def test_contact_angle_rules():
+    """Rules of kernels.py:949-991 with film = (1 - cos t)/2, on hand-built columns."""
+    t = 120.0
+    film = 0.5 * (1 - math.cos(math.radians(t)))   # 0.75
+    s = math.sin(math.radians(t))
+    # rule 1: a GAS cell right above SOLID holds at least the film
+    out = _contact([1.0, 0.0, 0.0], [SOLID, GAS, GAS], t)
+    assert math.isclose(out[1], film, rel_tol=1e-6)
+    # rule 3: GAS above a mostly-liquid cell: max(p, clamp(phi_below - sin t, film, 0.95))
+    out = _contact([1.0, 1.0, 0.9, 0.0], [SOLID, FLUID, FLUID, GAS], t)
+    assert math.isclose(out[3], max(film, min(0.9 - s, 0.95)), rel_tol=1e-6)
+    # rule 5: a wetted cell on SOLID with phi in (0.2, 0.65) is pinned to 0.65
+    out = _contact([1.0, 0.3, 0.0], [SOLID, IFACE, GAS], 30.0)
+    assert math.isclose(out[1], 0.65, rel_tol=1e-6)
+    # rule 4: lateral spreading into an empty cell next to a liquid one
+    film30 = 0.5 * (1 - math.cos(math.radians(30.0)))
+    out = _contact([1.0, 1.0, 0.0, 0.0], [SOLID, FLUID, IFACE, GAS], 30.0, lateral=(2, 0.9))
+    assert math.isclose(out[2], film30 * (0.85 + 0.15 * math.sin(math.radians(30.0))),
+                        rel_tol=1e-6)
+    # SOLID cells are never changed
+    out = _contact([0.3, 0.0], [SOLID, GAS], t)
+    assert out[0] == np.float32(0.3)
+
+
+def test_film_threshold_identity():
+    """film > 0.2 exactly when theta > 53.13 deg (cos theta < 0.6)."""
+    for deg, above in [(53.0, False), (53.2, True), (90.0, True), (30.0, False)]:
+        film = 0.5 * (1 - math.cos(math.radians(deg)))
+        assert (film > 0.2) == above
+
+
+# ---------------------------------------------------------------------------- freeze rules
+def test_cooled_metal_and_remelt_rules():
+    """solidify_cooled_metal: phi >= 0.85, f_l < 0.02, T < T_s -> SOLID, phi 1, f_l 0, u 0;
+    remelt_hot_solid_scalar: SOLID with T > T_s + 5 -> FLUID, phi 1, lever-rule f_l and H
+    (kernels.py:2422-2450, 2489-2520)."""
+    Ts, Tl, cp_rho, L = 1723.0, 1780.0, 5.304e6, 2.106e9
+    T = np.array([1700.0, 1700.0, 1725.0, 1750.0, 1900.0], np.float32)
+    fl = np.array([0.01, 0.03, 0.01, 0.0, 0.0], np.float32)
+    phi = np.array([0.9, 0.9, 0.9, 1.0, 1.0], np.float32)
+    flags = np.array([FLUID, FLUID, FLUID, SOLID, SOLID], np.uint8)
+    ux = np.full(5, 0.01, np.float32)
+    uy, uz = ux.copy(), ux.copy()
+    f2, p2, fg2 = fl.copy(), phi.copy(), flags.copy()
+    lib().or_solidify_cooled_metal(ptr(T), ptr(f2), ptr(p2), ptr(fg2), ptr(ux), ptr(uy), ptr(uz),
+                                   Ts, 1, 5, 1, 1)
+    assert fg2[0] == SOLID and p2[0] == 1.0 and f2[0] == 0.0 and ux[0] == 0.0
+    assert fg2[1] == FLUID and fg2[2] == FLUID and ux[1] == np.float32(0.01)
+    H = np.zeros(5, np.float32)
+    lib().or_remelt_hot_solid_scalar(ptr(T), ptr(H), ptr(fl), ptr(phi), ptr(flags), cp_rho, Ts,
+                                     Tl, L, 5, 1, 1)
+    assert flags[3] == FLUID and flags[4] == FLUID and phi[3] == 1.0
+    assert math.isclose(fl[3], (1750.0 - Ts) / (Tl - Ts + 1e-6), rel_tol=1e-6)
+    assert math.isclose(H[3], cp_rho * Ts + fl[3] * L, rel_tol=1e-6)
+    assert fl[4] == 1.0 and math.isclose(H[4], cp_rho * Tl + L + cp_rho * (1900.0 - Tl),
+                                         rel_tol=1e-6)
+    assert flags[0] == FLUID and H[0] == 0.0   # not SOLID: remelt leaves it alone
+
+
+# ---------------------------------------------------------------------------- telemetry
+def test_telemetry_matches_brute_force():
+    """Pool aggregates of kernels.py:3424-3468 recomputed with numpy on a random state."""
+    nx, ny, nz = 7, 6, 5
+    rng = np.random.default_rng(12)
+    T = rng.uniform(300, 3000, (nx, ny, nz)).astype(np.float32)
+    fl = rng.uniform(0, 1, (nx, ny, nz)).astype(np.float32)
+    phi = rng.uniform(0, 1, (nx, ny, nz)).astype(np.float32)
+    dT = rng.uniform(-5e5, 5e5, (nx, ny, nz)).astype(np.float32)
+    u = [rng.uniform(-0.05, 0.05, (nx, ny, nz)).astype(np.float32) for _ in range(3)]
+    out = np.zeros(8, np.float64)
+    lib().or_telemetry_pool_reduce(ptr(T), ptr(fl), ptr(phi), ptr(dT), ptr(u[0]), ptr(u[1]),
+                                   ptr(u[2]), ptr(out), nx, ny, nz)
+    liq = fl > 0.5
+    ii = np.broadcast_to(np.arange(nx)[:, None, None], T.shape)
+    u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2]
+    assert out[0] == liq.sum()
+    assert out[1] == T[liq].max()
+    assert out[2] == max(0.0, float((-dT[phi > 0.05]).max()))
+    assert out[3] == u2[liq].max()
+    assert out[4] == ii[liq].sum() and out[5] == ii[liq].min() and out[6] == ii[liq].max()
+    assert out[7] == T.max()
