@@ -1,9 +1,8 @@
 // fused.hip — the coupled step in seven launches (waam_step_fused).
 //
 // Launch sequence for one step (reference solvers/coupled_step.py:73-464):
-//   1 k_step_begin           torch-column surface height (arc_k); arc weight partial sums and,
-//                            in the last block to finish, their fixed-order total and the
-//                            energy per unit weight
+//   1 k_step_begin           torch-column surface height (arc_k); arc weight partial sums
+//     (+ k_step_arc_total    their fixed-order total and the energy per unit weight, one block)
 //   2 k_step_thermal         every cell but FLUID/IFACE: arc + enthalpy ceiling applied on the
 //                            fly, conduction, boundary losses, floor, ceiling, phase map, T_max,
 //                            cooling rate, donor-cell phi advection + re-initialisation; lists
@@ -53,10 +52,6 @@ namespace waam {
 
 /* blocks of the list kernels 3 and 5 (each grid-strides over its list) */
 constexpr int kListBlocks = 1024;
-/* int32 counter in the workspace: blocks of k_step_begin done */
-constexpr size_t kScratchStepCounters = kScratchTelemetry + 32;
-static_assert(kScratchStepCounters + sizeof(int) <= kScratchTelemetryParts,
-              "step counters overlap the telemetry partials");
 static_assert(kBlock == kListSegments, "the list prefix scan uses one thread per segment");
 
 /* A cell list is split into kListSegments contiguous cell ranges of seg_len cells (a multiple
@@ -87,11 +82,6 @@ struct StepConsts {
 __device__ __forceinline__ float step_arc_k(const waam_step_params &p, const float *tot)
 {
     return p.use_arc_k_override ? p.arc_k_override : tot[2];
-}
-
-__device__ __forceinline__ int *step_counters(const waam_step_fields &f)
-{
-    return reinterpret_cast<int *>((char *)f.scratch + kScratchStepCounters);
 }
 
 /* which = 0: thermal list (FLUID/IFACE by the incoming flags), 1: collision list */
@@ -177,18 +167,16 @@ __device__ __forceinline__ void cell_ijk(size_t c, const Dims &d, int &i, int &j
 }
 
 /* 1 — surface height in the torch column (reference kernels.py:625-641 and
- *     coupled_step.py:34-59), compensated arc weight sums (kernels.py:443-496) with the total
- *     formed by the last block; empties both cell lists for kernels 2 and 4. */
+ *     coupled_step.py:34-59) and per-block compensated arc weight sums (kernels.py:443-496);
+ *     empties both cell lists for kernels 2 and 4. */
 __global__ void __launch_bounds__(kBlock) k_step_begin(waam_step_fields f, StepConsts s,
                                                        int do_sums)
 {
     __shared__ double sh_s[kBlock], sh_c[kBlock];
     __shared__ int sh_k[kBlock];
     __shared__ float s_arck;
-    __shared__ bool s_last;
     const Dims &d = s.d;
     const waam_step_params &p = s.p;
-    int *cnt = step_counters(f);
     if (blockIdx.x == 0) {
         cell_list(f, s.seg_len, 0).count(threadIdx.x) = 0;
         cell_list(f, s.seg_len, 1).count(threadIdx.x) = 0;
@@ -250,20 +238,21 @@ __global__ void __launch_bounds__(kBlock) k_step_begin(waam_step_fields f, StepC
     }
     if (!do_sums) return;
     block_reduce_pair(sum, comp, sh_s, sh_c);
-    double *parts = reinterpret_cast<double *>((char *)f.scratch + kScratchPartials);
     if (threadIdx.x == 0) {
+        double *parts = reinterpret_cast<double *>((char *)f.scratch + kScratchPartials);
         parts[2 * blockIdx.x] = sum;
         parts[2 * blockIdx.x + 1] = comp;
-        __threadfence();
-        s_last = atomicAdd(&cnt[0], 1) == (int)gridDim.x - 1;
     }
-    __syncthreads();
-    if (!s_last) return;
-    /* last block: fixed-order total of the block sums (reference kernels.py:493-496) */
-    __threadfence();
-    sum = 0.0;
-    comp = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
+}
+
+/* 1b — fixed-order total of the block sums and the energy per unit weight (reference
+ *      kernels.py:493-496); one block */
+__global__ void __launch_bounds__(kBlock) k_step_arc_total(waam_step_fields f, StepConsts s)
+{
+    __shared__ double sh_s[kBlock], sh_c[kBlock];
+    const double *parts = reinterpret_cast<const double *>((char *)f.scratch + kScratchPartials);
+    double sum = 0.0, comp = 0.0;
+    for (int b = threadIdx.x; b < kReduceBlocks; b += blockDim.x) {
         double ss, e;
         two_sum(sum, parts[2 * b], ss, e);
         sum = ss;
@@ -271,13 +260,13 @@ __global__ void __launch_bounds__(kBlock) k_step_begin(waam_step_fields f, StepC
     }
     block_reduce_pair(sum, comp, sh_s, sh_c);
     if (threadIdx.x == 0) {
+        const waam_step_params &p = s.p;
         float *tot = reinterpret_cast<float *>((char *)f.scratch + kScratchTelemetry);
         float w_total = (float)(sum + comp);
         float epw = 0.0f;
         if (w_total > 1e-9f) epw = p.Q_w * p.eta * p.dt / (w_total * p.dx3);
         tot[0] = w_total;
         tot[1] = epw;
-        cnt[0] = 0;
     }
 }
 
@@ -869,6 +858,7 @@ int waam_step_fused(const waam_step_params *pp, const waam_step_fields *ff, void
     timed_launch(WAAM_K_BEGIN, st, [&] {
         hipLaunchKernelGGL(k_step_begin, dim3(do_sums ? kReduceBlocks : 1), dim3(kBlock), 0, st, f,
                            s, do_sums);
+        if (do_sums) hipLaunchKernelGGL(k_step_arc_total, dim3(1), dim3(kBlock), 0, st, f, s);
     });
     timed_launch(WAAM_K_THERMAL, st,
                  [&] { hipLaunchKernelGGL(k_step_thermal, g3, block3(), 0, st, f, s); });
