@@ -490,16 +490,25 @@ __global__ void __launch_bounds__(256) k_step_vof_flags(waam_step_fields f, Step
     list_append(in && is_surface_class(nf), c, cell_list(f, s.seg_len, 1), d.n());
 }
 
-/* pull of the reference stream (kernels.py:2991-3010) with the flags of the step it belongs to */
-__device__ __forceinline__ void pull_cell(const float *__restrict__ src, float *__restrict__ dst,
+/* Pull of the reference stream (kernels.py:2991-3010) with the flags of the step it belongs
+ * to (sfl), done by a whole wavefront of a grid3() launch: every lane loads the population
+ * at its own fast-axis position in the neighbouring row (one aligned, coalesced load per
+ * direction) and takes the value of a direction with a fast-axis component from the adjacent
+ * lane; the lanes at the ends of the wavefront and at the domain's wrap (x) or clamp (z) load
+ * directly. Every active lane of the wavefront must call it (it shuffles); `need` says whether
+ * this lane stores. Values are exactly those of a cell-by-cell pull. */
+__device__ __forceinline__ void pull_rows(const float *__restrict__ src, float *__restrict__ dst,
                                           const uint8_t *__restrict__ sfl, int i, int j, int k,
-                                          uint8_t fc, const Dims &d)
+                                          bool need, uint8_t fc, const Dims &d)
 {
     size_t N = d.n();
     size_t c = d.idx(i, j, k);
-    /* all loads are issued before the first store (src and dst carry no alias information
-     * once inlined) */
+    const bool zyx = d.layout == kLayoutZYX;
+    const int lane = (int)threadIdx.x;  // block3(): one wavefront per threadIdx.y row
+    const int a = zyx ? i : k;          // this lane's fast-axis coordinate
+    const int nf = d.fast();
     size_t from[Q];
+    float v[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         int ni = i - ex(q), nj = j - ey(q), nk = k - ez(q);
@@ -507,19 +516,19 @@ __device__ __forceinline__ void pull_cell(const float *__restrict__ src, float *
         nj = imax(0, imin(nj, d.ny - 1));
         nk = imax(0, imin(nk, d.nz - 1));
         from[q] = (size_t)q * N + d.idx(ni, nj, nk);
+        const int sh = zyx ? ex(q) : ez(q);  // fast-axis offset of the source
+        float al = src[(size_t)q * N + (zyx ? d.idx(i, nj, nk) : d.idx(ni, nj, k))];
+        float nb = sh == 0 ? al : __shfl(al, lane - sh);
+        bool direct = sh != 0 && (lane - sh < 0 || lane - sh > 63 || a - sh < 0 || a - sh >= nf);
+        v[q] = direct ? src[from[q]] : nb;
     }
+    if (!need) return;
     if (fc != GAS) {
         /* bounce-back off SOLID neighbours (a GAS destination pulls unconditionally) */
-        uint8_t nfl[Q];
-#pragma unroll
-        for (int q = 0; q < Q; ++q) nfl[q] = sfl[from[q] - (size_t)q * N];
 #pragma unroll
         for (int q = 0; q < Q; ++q)
-            if (nfl[q] == SOLID) from[q] = (size_t)opp(q) * N + c;
+            if (sfl[from[q] - (size_t)q * N] == SOLID) v[q] = src[(size_t)opp(q) * N + c];
     }
-    float v[Q];
-#pragma unroll
-    for (int q = 0; q < Q; ++q) v[q] = src[from[q]];
 #pragma unroll
     for (int q = 0; q < Q; ++q) dst[(size_t)q * N + c] = v[q];
 }
@@ -690,8 +699,8 @@ __global__ void __launch_bounds__(256) k_step_stream(waam_step_fields f, StepCon
         phi = f.phi[c];
         fl = f.fl_out[c];
     }
-    if (!surface && p.pending_stream && fp != SOLID)
-        pull_cell(f.f_src, f.f_dst, f.sflags_prev, i, j, k, fp, d);
+    bool need = !surface && p.pending_stream && fp != SOLID;
+    if (__ballot(need) != 0ull) pull_rows(f.f_src, f.f_dst, f.sflags_prev, i, j, k, need, fp, d);
     f.sflags_next[c] = fc;
     if (!surface && p.store_forces && !p.forces_from_fields) {
         f.Fx[c] = 0.0f;
@@ -711,7 +720,8 @@ __global__ void k_stream_pending(const float *__restrict__ src, float *__restric
     int i, j, k;
     if (!d.cell(i, j, k)) return;
     uint8_t fp = sfl[d.idx(i, j, k)];
-    if (fp != SOLID) pull_cell(src, dst, sfl, i, j, k, fp, d);
+    bool need = fp != SOLID;
+    if (__ballot(need) != 0ull) pull_rows(src, dst, sfl, i, j, k, need, fp, d);
 }
 
 /* 6 — advect_tracers (reference kernels.py:3206-3255) */
