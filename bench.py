@@ -209,6 +209,7 @@ def main() -> int:
     t0 = time.perf_counter()
     for s in range(args.warmup, args.warmup + args.steps):
         tw.step(*torch_xy(w, s, g.dt), True)
+    t_host = time.perf_counter() - t0  # launches are asynchronous: the host's own cost per step
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     roofline = None
@@ -243,6 +244,7 @@ def main() -> int:
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(el * 1e3 / args.steps, 4),
+        "host_us_per_step": round(t_host * 1e6 / args.steps, 1),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

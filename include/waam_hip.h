@@ -226,6 +226,67 @@ int waam_advect_tracers(float *pos, int32_t *active, const float *ux, const floa
                         const float *uz, const float *f_l, const uint8_t *flags, float dx,
                         float dt, int max_tracers, int nx, int ny, int nz, int layout, void *stream);
 
+/* ---- temperature-dependent material path ---------------------------------------------------
+ * Piecewise-linear property tables (reference gpu_tables.py:11-66: at most 8 knots per
+ * property; an empty table becomes one knot at 300 K holding the constant) and the per-cell
+ * property fields they fill. Tables and reference values are passed by value. */
+#define WAAM_MAX_KNOTS 8
+typedef struct waam_material_tables {
+    float cp_T[WAAM_MAX_KNOTS], cp_V[WAAM_MAX_KNOTS];
+    float k_T[WAAM_MAX_KNOTS], k_V[WAAM_MAX_KNOTS];
+    float mu_T[WAAM_MAX_KNOTS], mu_V[WAAM_MAX_KNOTS];
+    float dgamma_T[WAAM_MAX_KNOTS], dgamma_V[WAAM_MAX_KNOTS];
+    int32_t n_cp, n_k, n_mu, n_dgamma;
+    float cp_fallback, k_fallback, mu_fallback, dgamma_fallback;
+} waam_material_tables;
+/* scalars of refresh_thermal_properties (kernels.py:1710-1745): rho, dt, dx, the Marangoni
+ * prefactor force_scale (scale_F/dx), the constant-property values for SOLID/GAS cells and the
+ * Marangoni calibration scale */
+typedef struct waam_property_refs {
+    float rho, dt, dx, force_scale;
+    float cp_rho_ref, alpha_lu_ref, dgamma_lu_ref, tau_ref, marangoni_scale;
+} waam_property_refs;
+
+/* refresh_thermal_properties kernels.py:1710-1772 (n cells; use_tables 0 gives the constants) */
+int waam_refresh_thermal_properties(const float *T, float *cp_rho_field, float *alpha_lu_field,
+                                    float *dgamma_lu_field, float *tau_field,
+                                    const waam_material_tables *tables,
+                                    const waam_property_refs *refs, int use_tables,
+                                    const uint8_t *flags, int64_t n, void *stream);
+/* advect_diffuse_temperature_variable kernels.py:1775-1836 (Jacobi, H_in -> H_out) */
+int waam_advect_diffuse_temperature_variable(const float *H_in, float *H_out, const float *T,
+                                             const float *ux, const float *uy, const float *uz,
+                                             const uint8_t *flags, const float *alpha_lu_field,
+                                             const float *cp_rho_field, float dt, int nx, int ny,
+                                             int nz, int layout, void *stream);
+/* update_phase_variable_cp kernels.py:1839-1863 */
+int waam_update_phase_variable_cp(const float *H, float *T, float *f_l, const float *cp_rho_field,
+                                  float L_rho, float T_solidus, float T_liquidus, float H_sol,
+                                  float H_liq, int64_t n, void *stream);
+/* clamp_enthalpy_floor kernels.py:1937-1950 */
+int waam_clamp_enthalpy_floor(float *H, const float *cp_rho_field, const uint8_t *flags,
+                              float T_floor, int64_t n, void *stream);
+/* clamp_enthalpy_ceiling_variable_cp kernels.py:1987-2010 */
+int waam_clamp_enthalpy_ceiling_variable_cp(float *H, const uint8_t *flags,
+                                            const float *cp_rho_field, float H_liq,
+                                            float T_liquidus, float T_vapor_cap, int64_t n,
+                                            void *stream);
+/* compute_marangoni_force_variable kernels.py:2220-2269 */
+int waam_compute_marangoni_force_variable(const float *T, const float *phi, const float *f_l,
+                                          float *Fx, float *Fy, float *Fz, const uint8_t *flags,
+                                          const float *dgamma_lu_field, int nx, int ny, int nz,
+                                          int layout, void *stream);
+/* remelt_hot_solid kernels.py:2454-2486 */
+int waam_remelt_hot_solid(const float *T, float *H, float *f_l, float *phi, uint8_t *flags,
+                          const float *cp_rho_field, float T_solidus, float T_liquidus,
+                          float L_rho, int64_t n, void *stream);
+/* solidify_trailing_pool kernels.py:2524-2570 */
+int waam_solidify_trailing_pool(float *T, float *H, float *f_l, float *phi, uint8_t *flags,
+                                float *ux, float *uy, float *uz, const float *cp_rho_field,
+                                float arc_i, float arc_j, float arc_k, float dir_x, float dir_y,
+                                float dir_z, float lookback_cells, float T_freeze,
+                                float T_solidus, int nx, int ny, int nz, int layout, void *stream);
+
 /* ---- telemetry: telemetry_pool_reduce kernels.py:3424-3468 ------------------------------- */
 /* out (device, 8 doubles): n_liquid, T_peak, cool, u2_max, i_sum, i_min, i_max, T_global. */
 int waam_telemetry_pool_reduce(const float *T, const float *f_l, const float *phi,

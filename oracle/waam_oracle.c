@@ -1540,3 +1540,268 @@ void or_advect_tracers(float *pos, int32_t *active, const float *ux, const float
         pos[3 * p + 2] += uz[c] * dx;
     }
 }
+
+/* ------------------------------------------------------------------------------------------
+ * Temperature-dependent material tables — lookup_table_1d kernels.py:1685-1707,
+ * refresh_thermal_properties :1710-1772, advect_diffuse_temperature_variable :1775-1836,
+ * update_phase_variable_cp :1839-1863, clamp_enthalpy_floor :1937-1950 (per-cell cp_rho),
+ * clamp_enthalpy_ceiling_variable_cp :1987-2010, compute_marangoni_force_variable :2220-2269,
+ * remelt_hot_solid :2454-2486, solidify_trailing_pool :2524-2570. The boundary losses of the
+ * table path (kernels.py:1866-1918) do not read cp_rho_field and equal the scalar kernel's.
+ * ---------------------------------------------------------------------------------------- */
+#define OR_MAX_KNOTS 8
+
+static float lookup_table_1d(float T, const float *t_arr, const float *v_arr, int n,
+                             float fallback)
+{
+    float result = fallback;
+    if (n > 0) {
+        if (T <= t_arr[0]) {
+            result = v_arr[0];
+        } else if (T >= t_arr[n - 1]) {
+            result = v_arr[n - 1];
+        } else {
+            /* every bracketing interval overwrites the result: the last match wins */
+            for (int idx = 0; idx < OR_MAX_KNOTS - 1; ++idx) {
+                if (idx < n - 1) {
+                    float t0 = t_arr[idx], t1 = t_arr[idx + 1];
+                    if (T >= t0 && T <= t1) {
+                        float w = (T - t0) / (t1 - t0 + 1e-8f);
+                        result = v_arr[idx] + w * (v_arr[idx + 1] - v_arr[idx]);
+                    }
+                }
+            }
+        }
+    }
+    return result;
+}
+
+/* tables: 8 arrays of OR_MAX_KNOTS floats (cp_T, cp_V, k_T, k_V, mu_T, mu_V, dg_T, dg_V) back
+ * to back; counts: n_cp, n_k, n_mu, n_dg. consts: rho, dt, dx, cp_fb, k_fb, mu_fb, dg_fb,
+ * force_scale, cp_rho_ref, alpha_ref, dgamma_ref, tau_ref, marangoni_scale. */
+void or_refresh_thermal_properties(const float *T, float *cp_rho_f, float *alpha_f,
+                                   float *dgamma_f, float *tau_f, const float *tables,
+                                   const int *counts, const float *consts, int use_tables,
+                                   const uint8_t *flags, int nx, int ny, int nz)
+{
+    const float *cp_T = tables, *cp_V = tables + 8, *k_T = tables + 16, *k_V = tables + 24;
+    const float *mu_T = tables + 32, *mu_V = tables + 40, *dg_T = tables + 48, *dg_V = tables + 56;
+    float rho = consts[0], dt = consts[1], dx = consts[2];
+    float cp_fb = consts[3], k_fb = consts[4], mu_fb = consts[5], dg_fb = consts[6];
+    float force_scale = consts[7], cp_rho_ref = consts[8], alpha_ref = consts[9];
+    float dgamma_ref = consts[10], tau_ref = consts[11], mscale = consts[12];
+    size_t N = NCELL;
+#pragma omp parallel for schedule(static)
+    for (size_t c = 0; c < N; ++c) {
+        if (flags[c] == SOLID || flags[c] == GAS) {
+            cp_rho_f[c] = cp_rho_ref;
+            alpha_f[c] = alpha_ref;
+            dgamma_f[c] = dgamma_ref;
+            tau_f[c] = tau_ref;
+            continue;
+        }
+        float T_c = T[c];
+        if (use_tables == 1) {
+            float cp = lookup_table_1d(T_c, cp_T, cp_V, counts[0], cp_fb);
+            float k_val = lookup_table_1d(T_c, k_T, k_V, counts[1], k_fb);
+            float mu_val = lookup_table_1d(T_c, mu_T, mu_V, counts[2], mu_fb);
+            float dgamma = lookup_table_1d(T_c, dg_T, dg_V, counts[3], dg_fb);
+            float cp_r = rho * cp;
+            float alpha_phys = k_val / (rho * cp + 1e-9f);
+            float nu_phys = mu_val / (rho + 1e-9f);
+            float nu_lu = nu_phys * dt / (dx * dx);
+            cp_rho_f[c] = cp_r;
+            alpha_f[c] = alpha_phys * dt / (dx * dx);
+            dgamma_f[c] = dgamma * force_scale * mscale;
+            tau_f[c] = 3.0f * nu_lu + 0.5f;
+        } else {
+            cp_rho_f[c] = cp_rho_ref;
+            alpha_f[c] = alpha_ref;
+            dgamma_f[c] = dgamma_ref * mscale;
+            tau_f[c] = tau_ref;
+        }
+    }
+}
+
+void or_advect_diffuse_temperature_variable_serial(float *H, const float *T, const float *ux,
+                                                   const float *uy, const float *uz,
+                                                   const uint8_t *flags, const float *alpha_f,
+                                                   const float *cp_rho_f, float dt, int nx,
+                                                   int ny, int nz)
+{
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                size_t c = IDX(i, j, k);
+                if (flags[c] == GAS) continue;
+                float dH = thermal_dH(H, T, ux, uy, uz, flags, i, j, k, alpha_f[c], dt,
+                                      cp_rho_f[c], nx, ny, nz);
+                H[c] += dH;
+            }
+}
+
+void or_advect_diffuse_temperature_variable_jacobi(const float *H_in, float *H_out,
+                                                   const float *T, const float *ux,
+                                                   const float *uy, const float *uz,
+                                                   const uint8_t *flags, const float *alpha_f,
+                                                   const float *cp_rho_f, float dt, int nx,
+                                                   int ny, int nz)
+{
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                size_t c = IDX(i, j, k);
+                if (flags[c] == GAS) {
+                    H_out[c] = H_in[c];
+                    continue;
+                }
+                float dH = thermal_dH(H_in, T, ux, uy, uz, flags, i, j, k, alpha_f[c], dt,
+                                      cp_rho_f[c], nx, ny, nz);
+                H_out[c] = H_in[c] + dH;
+            }
+}
+
+void or_update_phase_variable_cp(const float *H, float *T, float *f_l, const float *cp_rho_f,
+                                 float L_rho, float T_s, float T_l, float H_sol, float H_liq,
+                                 int nx, int ny, int nz)
+{
+    size_t N = NCELL;
+#pragma omp parallel for schedule(static)
+    for (size_t c = 0; c < N; ++c) {
+        float h = H[c];
+        float cp_rho = cp_rho_f[c];
+        if (h <= H_sol) {
+            f_l[c] = 0.0f;
+            T[c] = h / (cp_rho + 1e-9f);
+        } else if (h >= H_liq) {
+            f_l[c] = 1.0f;
+            T[c] = T_l + (h - H_liq) / (cp_rho + 1e-9f);
+        } else {
+            float fl = (h - H_sol) / L_rho;
+            f_l[c] = fl;
+            T[c] = T_s + fl * (T_l - T_s);
+        }
+    }
+}
+
+void or_clamp_enthalpy_floor(float *H, const float *cp_rho_f, const uint8_t *flags,
+                             float T_floor, int nx, int ny, int nz)
+{
+    size_t N = NCELL;
+#pragma omp parallel for schedule(static)
+    for (size_t c = 0; c < N; ++c) {
+        if (flags[c] == GAS) continue;
+        float h_min = cp_rho_f[c] * T_floor;
+        if (H[c] < h_min) H[c] = h_min;
+    }
+}
+
+void or_clamp_enthalpy_ceiling_variable_cp(float *H, const uint8_t *flags, const float *cp_rho_f,
+                                           float H_liq, float T_l, float T_cap, int nx, int ny,
+                                           int nz)
+{
+    size_t N = NCELL;
+#pragma omp parallel for schedule(static)
+    for (size_t c = 0; c < N; ++c) {
+        if (flags[c] == GAS) continue;
+        float cp_r = cp_rho_f[c];
+        float h_cap_liquid = H_liq + cp_r * (T_cap - T_l);
+        float h_cap_solid = cp_r * T_cap;
+        float h = H[c];
+        if (h > H_liq) {
+            if (h > h_cap_liquid) H[c] = h_cap_liquid;
+        } else {
+            if (h > h_cap_solid) H[c] = h_cap_solid;
+        }
+    }
+}
+
+void or_compute_marangoni_force_variable(const float *T, const float *phi, const float *f_l,
+                                         float *Fx, float *Fy, float *Fz, const uint8_t *flags,
+                                         const float *dgamma_f, int nx, int ny, int nz)
+{
+    const float eps = 1e-6f, fl_min = 0.05f;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                size_t c = IDX(i, j, k);
+                if (flags[c] == SOLID || flags[c] == GAS) continue;
+                if (f_l[c] < fl_min) continue;
+                int ip = imin(i + 1, nx - 1), im = imax(i - 1, 0);
+                int jp = imin(j + 1, ny - 1), jm = imax(j - 1, 0);
+                int kp = imin(k + 1, nz - 1), km = imax(k - 1, 0);
+                float dphx = 0.5f * (phi[IDX(ip, j, k)] - phi[IDX(im, j, k)]);
+                float dphy = 0.5f * (phi[IDX(i, jp, k)] - phi[IDX(i, jm, k)]);
+                float dphz = 0.5f * (phi[IDX(i, j, kp)] - phi[IDX(i, j, km)]);
+                float gm = sqrtf(dphx * dphx + dphy * dphy + dphz * dphz);
+                if (gm < eps) continue;
+                float nxn = dphx / gm, nyn = dphy / gm, nzn = dphz / gm;
+                float dTx = 0.5f * (T[IDX(ip, j, k)] - T[IDX(im, j, k)]);
+                float dTy = 0.5f * (T[IDX(i, jp, k)] - T[IDX(i, jm, k)]);
+                float dTz = 0.5f * (T[IDX(i, j, kp)] - T[IDX(i, j, km)]);
+                float ndot = nxn * dTx + nyn * dTy + nzn * dTz;
+                float sx = dTx - ndot * nxn, sy = dTy - ndot * nyn, sz = dTz - ndot * nzn;
+                float scale = dgamma_f[c] * gm;
+                Fx[c] += scale * sx;
+                Fy[c] += scale * sy;
+                Fz[c] += scale * sz;
+            }
+}
+
+void or_remelt_hot_solid(const float *T, float *H, float *f_l, float *phi, uint8_t *flags,
+                         const float *cp_rho_f, float T_s, float T_l, float L_rho, int nx, int ny,
+                         int nz)
+{
+    const float margin = 5.0f;
+    size_t N = NCELL;
+#pragma omp parallel for schedule(static)
+    for (size_t c = 0; c < N; ++c) {
+        if (flags[c] != SOLID) continue;
+        float T_c = T[c];
+        if (T_c <= T_s + margin) continue;
+        float cp_r = cp_rho_f[c];
+        float H_sol = cp_r * T_s;
+        float fl;
+        if (T_c >= T_l) {
+            fl = 1.0f;
+            H[c] = cp_r * T_l + L_rho + cp_r * (T_c - T_l);
+        } else {
+            fl = (T_c - T_s) / (T_l - T_s + 1e-6f);
+            H[c] = H_sol + fl * L_rho;
+        }
+        f_l[c] = fl;
+        phi[c] = 1.0f;
+        flags[c] = FLUID;
+    }
+}
+
+void or_solidify_trailing_pool(float *T, float *H, float *f_l, float *phi, uint8_t *flags,
+                               float *ux, float *uy, float *uz, const float *cp_rho_f,
+                               float arc_i, float arc_j, float arc_k, float dir_x, float dir_y,
+                               float dir_z, float lookback, float T_freeze, float T_s, int nx,
+                               int ny, int nz)
+{
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                size_t c = IDX(i, j, k);
+                if (flags[c] == GAS) continue;
+                if (phi[c] < 0.85f || f_l[c] < 0.05f) continue;
+                float di = (float)i - arc_i, dj = (float)j - arc_j, dk = (float)k - arc_k;
+                float proj = di * dir_x + dj * dir_y + dk * dir_z;
+                if (proj > -lookback) continue;
+                if (T[c] >= T_freeze) continue;
+                float cp_r = cp_rho_f[c];
+                H[c] = cp_r * (T_s - 1.0f);
+                T[c] = T_s - 1.0f;
+                f_l[c] = 0.0f;
+                phi[c] = 1.0f;
+                flags[c] = SOLID;
+                ux[c] = 0.0f;
+                uy[c] = 0.0f;
+                uz[c] = 0.0f;
+            }
+}

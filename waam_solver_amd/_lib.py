@@ -78,6 +78,14 @@ SIGNATURES: dict[str, str] = {
     "waam_step_arc_readback": "pp",
     "waam_timing_begin": "ii",
     "waam_timing_end": "pp",
+    "waam_refresh_thermal_properties": "pppppppiplp",
+    "waam_advect_diffuse_temperature_variable": "pppppppppfiiiip",
+    "waam_update_phase_variable_cp": "ppppffffflp",
+    "waam_clamp_enthalpy_floor": "pppflp",
+    "waam_clamp_enthalpy_ceiling_variable_cp": "pppffflp",
+    "waam_compute_marangoni_force_variable": "ppppppppiiiip",
+    "waam_remelt_hot_solid": "ppppppffflp",
+    "waam_solidify_trailing_pool": "pppppppppfffffffffiiiip",
 }
 
 # kernel ids of waam_step_fused for waam_timing_begin (include/waam_hip.h WAAM_K_*)

@@ -442,6 +442,125 @@ def telemetry_pool_reduce_into(T, f_l, phi, dT_dt, ux, uy, uz, out, scratch):
           P(out), nx, ny, nz, layout_of(T), P(scratch), _stream())
 
 
+# ------------------------------------------------------------------ material tables
+def _host_floats(x) -> list[float]:
+    if isinstance(x, DeviceField):
+        return [float(v) for v in x.to_numpy().reshape(-1)]
+    if isinstance(x, torch.Tensor):
+        return [float(v) for v in x.detach().cpu().reshape(-1)]
+    return [float(v) for v in x]
+
+
+def _host_int(x) -> int:
+    if isinstance(x, DeviceField):
+        return int(x[None])
+    return int(x)
+
+
+def refresh_thermal_properties(T, cp_rho_field, alpha_lu_field, dgamma_lu_field, tau_field, cp_T,
+                               cp_V, k_T, k_V, mu_T, mu_V, dgamma_T, dgamma_V, n_cp, n_k, n_mu,
+                               n_dgamma, rho, dt, dx, cp_fallback, k_fallback, mu_fallback,
+                               dgamma_fallback, force_scale, cp_rho_ref, alpha_lu_ref,
+                               dgamma_lu_ref, tau_ref, marangoni_scale, use_tables, flags,
+                               FLAG_SOLID, FLAG_GAS):
+    """Per-cell cp_rho, alpha_lu, dgamma_lu and tau from the knot tables (reference
+    kernels.py:1710-1772). The knot arrays may be device fields, tensors or host sequences of
+    up to 8 values."""
+    from .gpu_tables import MAX_KNOTS, MaterialTablesC, PropertyRefsC
+
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
+    n = _n(T)
+    _need(n, cp_rho_field, alpha_lu_field, dgamma_lu_field, tau_field, flags)
+    tb = MaterialTablesC()
+    for prop, t_arr, v_arr, cnt in (("cp", cp_T, cp_V, n_cp), ("k", k_T, k_V, n_k),
+                                    ("mu", mu_T, mu_V, n_mu), ("dgamma", dgamma_T, dgamma_V, n_dgamma)):
+        t = (_host_floats(t_arr) + [0.0] * MAX_KNOTS)[:MAX_KNOTS]
+        v = (_host_floats(v_arr) + [0.0] * MAX_KNOTS)[:MAX_KNOTS]
+        getattr(tb, f"{prop}_T")[:] = t
+        getattr(tb, f"{prop}_V")[:] = v
+        setattr(tb, f"n_{prop}", _host_int(cnt))
+    tb.cp_fallback, tb.k_fallback = cp_fallback, k_fallback
+    tb.mu_fallback, tb.dgamma_fallback = mu_fallback, dgamma_fallback
+    rf = PropertyRefsC(rho, dt, dx, force_scale, cp_rho_ref, alpha_lu_ref, dgamma_lu_ref, tau_ref,
+                       marangoni_scale)
+    import ctypes
+
+    _call("waam_refresh_thermal_properties", P(T), P(cp_rho_field), P(alpha_lu_field),
+          P(dgamma_lu_field), P(tau_field), ctypes.addressof(tb), ctypes.addressof(rf),
+          int(use_tables), P(flags), n, _stream())
+
+
+def advect_diffuse_temperature_variable_into(H_in, H_out, T, ux, uy, uz, flags, alpha_lu_field,
+                                             cp_rho_field, dt, nx, ny, nz):
+    _need(nx * ny * nz, H_in, H_out, T, ux, uy, uz, flags, alpha_lu_field, cp_rho_field)
+    _call("waam_advect_diffuse_temperature_variable", P(H_in), P(H_out), P(T), P(ux), P(uy),
+          P(uz), P(flags), P(alpha_lu_field), P(cp_rho_field), dt, nx, ny, nz, layout_of(H_out),
+          _stream())
+
+
+def advect_diffuse_temperature_variable(H, T, ux, uy, uz, flags, alpha_lu_field, cp_rho_field, dt,
+                                        FLAG_SOLID, FLAG_GAS, nx, ny, nz):
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
+    H_in = tensor_of(H).clone()
+    advect_diffuse_temperature_variable_into(H_in, H, T, ux, uy, uz, flags, alpha_lu_field,
+                                             cp_rho_field, dt, nx, ny, nz)
+
+
+def update_phase_variable_cp(H, T, f_l, cp_rho_field, L_rho, T_solidus, T_liquidus, H_sol, H_liq):
+    _need(_n(H), T, f_l, cp_rho_field)
+    _call("waam_update_phase_variable_cp", P(H), P(T), P(f_l), P(cp_rho_field), L_rho, T_solidus,
+          T_liquidus, H_sol, H_liq, _n(H), _stream())
+
+
+def apply_thermal_boundary_losses_variable(H, T, flags, cp_rho_field, T_amb, h_conv, eps_rad,
+                                           enable_conv, enable_rad, dt, dx, sigma_sb, FLAG_SOLID,
+                                           FLAG_GAS, nx, ny, nz):
+    """Reference kernels.py:1866-1918: the loss does not depend on cp_rho_field."""
+    apply_thermal_boundary_losses(H, T, flags, T_amb, h_conv, eps_rad, enable_conv, enable_rad,
+                                  0.0, dt, dx, sigma_sb, FLAG_SOLID, FLAG_GAS, nx, ny, nz)
+
+
+def clamp_enthalpy_floor(H, cp_rho_field, flags, T_floor, FLAG_GAS):
+    _flags_ok(FLAG_GAS=FLAG_GAS)
+    _need(_n(H), cp_rho_field, flags)
+    _call("waam_clamp_enthalpy_floor", P(H), P(cp_rho_field), P(flags), T_floor, _n(H), _stream())
+
+
+def clamp_enthalpy_ceiling_variable_cp(H, flags, cp_rho_field, H_liq, T_liquidus, T_vapor_cap,
+                                       FLAG_GAS):
+    _flags_ok(FLAG_GAS=FLAG_GAS)
+    _need(_n(H), flags, cp_rho_field)
+    _call("waam_clamp_enthalpy_ceiling_variable_cp", P(H), P(flags), P(cp_rho_field), H_liq,
+          T_liquidus, T_vapor_cap, _n(H), _stream())
+
+
+def compute_marangoni_force_variable(T, phi, f_l, Fx, Fy, Fz, flags, dgamma_lu_field, FLAG_SOLID,
+                                     FLAG_GAS, nx, ny, nz):
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
+    _need(nx * ny * nz, T, phi, f_l, Fx, Fy, Fz, flags, dgamma_lu_field)
+    _call("waam_compute_marangoni_force_variable", P(T), P(phi), P(f_l), P(Fx), P(Fy), P(Fz),
+          P(flags), P(dgamma_lu_field), nx, ny, nz, layout_of(T), _stream())
+
+
+def remelt_hot_solid(T, H, f_l, phi, flags, cp_rho_field, T_solidus, T_liquidus, L_rho,
+                     FLAG_SOLID, FLAG_FLUID):
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_FLUID=FLAG_FLUID)
+    _need(_n(T), H, f_l, phi, flags, cp_rho_field)
+    _call("waam_remelt_hot_solid", P(T), P(H), P(f_l), P(phi), P(flags), P(cp_rho_field),
+          T_solidus, T_liquidus, L_rho, _n(T), _stream())
+
+
+def solidify_trailing_pool(T, H, f_l, phi, flags, ux, uy, uz, cp_rho_field, arc_i, arc_j, arc_k,
+                           dir_x, dir_y, dir_z, lookback_cells, T_freeze, T_solidus, FLAG_SOLID,
+                           FLAG_FLUID, FLAG_GAS):
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_FLUID=FLAG_FLUID, FLAG_GAS=FLAG_GAS)
+    nx, ny, nz = dims_of(T)
+    _need(nx * ny * nz, H, f_l, phi, flags, ux, uy, uz, cp_rho_field)
+    _call("waam_solidify_trailing_pool", P(T), P(H), P(f_l), P(phi), P(flags), P(ux), P(uy),
+          P(uz), P(cp_rho_field), arc_i, arc_j, arc_k, dir_x, dir_y, dir_z, lookback_cells,
+          T_freeze, T_solidus, nx, ny, nz, layout_of(T), _stream())
+
+
 def extract_fl_yz_slice(f_l, out, i_slice, ny, nz):
     tensor_of(out).copy_(f_l[int(i_slice)] if isinstance(f_l, DeviceField)
                          else tensor_of(f_l)[int(i_slice)])
