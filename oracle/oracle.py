@@ -103,6 +103,15 @@ _SIGS = {
     "or_telemetry_pool_reduce": [_P] * 8 + [_I, _I, _I],
     "or_inject_tracers": [_P, _P, _P, _I, _F, _F, _F, _F, _I, ctypes.c_uint32],
     "or_advect_tracers": [_P] * 7 + [_F, _F, _I, _I, _I, _I],
+    "or_refresh_thermal_properties": [_P] * 8 + [_I, _P, _I, _I, _I],
+    "or_advect_diffuse_temperature_variable_serial": [_P] * 8 + [_F, _I, _I, _I],
+    "or_advect_diffuse_temperature_variable_jacobi": [_P] * 9 + [_F, _I, _I, _I],
+    "or_update_phase_variable_cp": [_P] * 4 + [_F] * 5 + [_I, _I, _I],
+    "or_clamp_enthalpy_floor": [_P, _P, _P, _F, _I, _I, _I],
+    "or_clamp_enthalpy_ceiling_variable_cp": [_P, _P, _P, _F, _F, _F, _I, _I, _I],
+    "or_compute_marangoni_force_variable": [_P] * 8 + [_I, _I, _I],
+    "or_remelt_hot_solid": [_P] * 6 + [_F, _F, _F, _I, _I, _I],
+    "or_solidify_trailing_pool": [_P] * 9 + [_F] * 9 + [_I, _I, _I],
 }
 
 
@@ -157,6 +166,9 @@ class Material:
     sulphur_ppm: float = 30.0
     rho_e_ohm_m: float = 1.5e-7
     eta_stick: float = 0.85
+    # piecewise-linear knots {"cp", "k", "mu", "dgamma_dT"} -> [(T, value), ...], given as
+    # data (already through the surfactant model); empty means the scalar path
+    tables: dict | None = None
 
     def __post_init__(self):
         if self.alpha is None:
@@ -169,6 +181,42 @@ class Material:
         else:
             scale = 1.0 - 1.35 * (s - 30.0) / 30.0
         self.dgamma_dT = -abs(self.dgamma_dT) * scale if scale < 0 else self.dgamma_dT * scale
+        self.tables = {k: [(float(t), float(v)) for t, v in (self.tables or {}).get(k, [])]
+                       for k in ("cp", "k", "mu", "dgamma_dT")}
+
+    def has_tables(self) -> bool:
+        return any(self.tables.values())
+
+
+def interp_knots(T: float, knots, fallback: float) -> float:
+    """Host piecewise-linear lookup in double precision (reference material_tables.py:42-55):
+    clamped to the end values, first bracketing interval wins, zero-width interval -> left."""
+    if not knots:
+        return fallback
+    if T <= knots[0][0]:
+        return knots[0][1]
+    if T >= knots[-1][0]:
+        return knots[-1][1]
+    for (t0, v0), (t1, v1) in zip(knots[:-1], knots[1:]):
+        if t0 <= T <= t1:
+            return v0 if t1 == t0 else v0 + (T - t0) / (t1 - t0) * (v1 - v0)
+    return fallback
+
+
+def pack_tables(mat: "Material") -> tuple[np.ndarray, np.ndarray]:
+    """(64 float32 knots, 4 int32 counts) in the order cp, k, mu, dgamma: at most 8 knots per
+    property, an empty property becomes the single knot (300 K, scalar) (reference
+    gpu_tables.py:39-58)."""
+    tab = np.zeros(64, np.float32)
+    cnt = np.zeros(4, np.int32)
+    for p, (key, fb) in enumerate((("cp", mat.cp), ("k", mat.k), ("mu", mat.mu),
+                                   ("dgamma_dT", mat.dgamma_dT))):
+        knots = mat.tables[key][:8] or [(300.0, fb)]
+        for i, (t, v) in enumerate(knots):
+            tab[16 * p + i] = t
+            tab[16 * p + 8 + i] = v
+        cnt[p] = len(knots)
+    return tab, cnt
 
 
 # --------------------------------------------------------------------------------------------
@@ -195,7 +243,8 @@ class OracleGrid:
         for name in ("rho", "T", "T_max", "H", "f_l", "phi", "dT_dt", "T_prev",
                      "time_above_800_s", "time_above_1100_s", "time_above_solidus_s",
                      "Fx_snap", "Fy_snap", "Fz_snap", "ux", "uy", "uz", "Fx", "Fy", "Fz",
-                     "phi_tmp"):
+                     "phi_tmp", "cp_rho_field", "alpha_lu_field", "dgamma_lu_field",
+                     "tau_field"):
             setattr(self, name, z())
         self.flags = np.zeros(s3, np.uint8)
         self.max_tracers = max_tracers
@@ -223,7 +272,8 @@ class OracleGrid:
 
 
 class OracleTwin:
-    """Independent restatement of WAAMTwin's constructor, reset and coupled step (scalar path)."""
+    """Independent restatement of WAAMTwin's constructor, reset and coupled step (scalar and
+    temperature-dependent table paths)."""
 
     def __init__(self, material: Material, nx=256, ny=128, nz=64, dx=2e-4, arc_power_W=2800.0,
                  arc_efficiency=0.8, T_ambient=300.0, use_srt=True, C_darcy=1.6e5,
@@ -238,7 +288,7 @@ class OracleTwin:
                  welding_current_A=140.0, travel_speed_m_s=0.0, enable_enthalpy_cap=True,
                  T_vapor_cap_K=3200.0, arc_surface_weighting=True, arc_penetration_mm=2.0,
                  wire_diameter_mm=1.2, arc_sigma_mm=2.0, bulk_tau=None, mode="jacobi",
-                 parallel=False, tracer_seed=0):
+                 parallel=False, tracer_seed=0, use_variable_tau=True):
         self.mat = material
         self.lib = lib(parallel)
         self.mode = mode
@@ -295,6 +345,13 @@ class OracleTwin:
         self.dgamma_dT_lu = material.dgamma_dT * scale_F / g.dx
         self.gamma_lu = material.gamma_0 * scale_F / g.dx
         self.marangoni_scale = 1.0
+        # temperature-dependent tables (twin.py:269-273)
+        self.use_tables = material.has_tables()
+        self.use_variable_tau = use_variable_tau
+        self.table_knots, self.table_counts = pack_tables(material)
+        cp_sol = interp_knots(material.T_solidus, material.tables["cp"], material.cp)
+        self.H_sol = material.rho * cp_sol * material.T_solidus
+        self.H_liq = self.H_sol + self.L_rho
         self.goldak = None  # (ff, fr, a_front, a_rear, b, c) in cells when heat_source is Goldak
         self._reset_counters()
 
@@ -458,12 +515,23 @@ class OracleTwin:
         if not self.enable_enthalpy_cap:
             return
         g = self.g
+        if self.use_tables:
+            self.lib.or_clamp_enthalpy_ceiling_variable_cp(
+                ptr(g.H), ptr(g.flags), ptr(g.cp_rho_field), self.H_liq, self.mat.T_liquidus,
+                self.T_vapor_cap_K, *g.dims)
+            return
         self.lib.or_clamp_enthalpy_ceiling_scalar(ptr(g.H), ptr(g.flags), self.cp_rho,
                                                   self.mat.T_solidus, self.mat.T_liquidus,
                                                   self.T_vapor_cap_K, self.L_rho, *g.dims)
 
     def _phase(self):
         g = self.g
+        if self.use_tables:
+            self.lib.or_update_phase_variable_cp(ptr(g.H), ptr(g.T), ptr(g.f_l),
+                                                 ptr(g.cp_rho_field), self.L_rho,
+                                                 self.mat.T_solidus, self.mat.T_liquidus,
+                                                 self.H_sol, self.H_liq, *g.dims)
+            return
         self.lib.or_update_phase(ptr(g.H), ptr(g.T), ptr(g.f_l), self.cp_rho, self.L_rho,
                                  self.mat.T_solidus, self.mat.T_liquidus, *g.dims)
 
@@ -559,7 +627,25 @@ class OracleTwin:
             self._inject_arc(arc_i, arc_j, arc_k)
             self._ceiling()
             self._droplet(x_m, y_m, arc_i, arc_j, arc_k)
-        if self.mode == "serial":
+        if self.use_tables:
+            m = self.mat
+            consts = np.array([m.rho, g.dt, g.dx, m.cp, m.k, m.mu, m.dgamma_dT,
+                               self.force_scale / g.dx, self.cp_rho, self.alpha_lu,
+                               self.dgamma_dT_lu, g.tau, self.marangoni_scale], np.float32)
+            L.or_refresh_thermal_properties(ptr(g.T), ptr(g.cp_rho_field), ptr(g.alpha_lu_field),
+                                            ptr(g.dgamma_lu_field), ptr(g.tau_field),
+                                            ptr(self.table_knots), ptr(self.table_counts),
+                                            ptr(consts), 1, ptr(g.flags), *d)
+            if self.mode == "serial":
+                L.or_advect_diffuse_temperature_variable_serial(
+                    ptr(g.H), ptr(g.T), ptr(g.ux), ptr(g.uy), ptr(g.uz), ptr(g.flags),
+                    ptr(g.alpha_lu_field), ptr(g.cp_rho_field), 1.0, *d)
+            else:
+                H_in = g.H.copy()
+                L.or_advect_diffuse_temperature_variable_jacobi(
+                    ptr(H_in), ptr(g.H), ptr(g.T), ptr(g.ux), ptr(g.uy), ptr(g.uz),
+                    ptr(g.flags), ptr(g.alpha_lu_field), ptr(g.cp_rho_field), 1.0, *d)
+        elif self.mode == "serial":
             L.or_advect_diffuse_temperature_serial(ptr(g.H), ptr(g.T), ptr(g.ux), ptr(g.uy),
                                                    ptr(g.uz), ptr(g.flags), self.alpha_lu, 1.0,
                                                    self.cp_rho, *d)
@@ -574,7 +660,12 @@ class OracleTwin:
                                                1 if self.enable_convection else 0,
                                                1 if self.enable_radiation else 0, self.cp_rho,
                                                g.dt, g.dx, self.sigma_sb, *d)
-            L.or_clamp_enthalpy_floor_scalar(ptr(g.H), self.cp_rho, ptr(g.flags), self.T_amb, *d)
+            if self.use_tables:
+                L.or_clamp_enthalpy_floor(ptr(g.H), ptr(g.cp_rho_field), ptr(g.flags),
+                                          self.T_amb, *d)
+            else:
+                L.or_clamp_enthalpy_floor_scalar(ptr(g.H), self.cp_rho, ptr(g.flags),
+                                                 self.T_amb, *d)
         self._phase()
         self._ceiling()
         self._phase()
@@ -599,8 +690,14 @@ class OracleTwin:
             ct, st = cos_sin_f32(self.theta_rad)
             L.or_compute_csf_tension(ptr(g.phi), ptr(g.flags), ptr(g.Fx), ptr(g.Fy), ptr(g.Fz),
                                      self.gamma_lu, 1 if self.enable_wetting else 0, ct, st, *d)
-        L.or_compute_marangoni_force(ptr(g.T), ptr(g.phi), ptr(g.f_l), ptr(g.Fx), ptr(g.Fy),
-                                     ptr(g.Fz), ptr(g.flags), self.dgamma_dT_lu, *d)
+        if self.use_tables:
+            L.or_compute_marangoni_force_variable(ptr(g.T), ptr(g.phi), ptr(g.f_l), ptr(g.Fx),
+                                                  ptr(g.Fy), ptr(g.Fz), ptr(g.flags),
+                                                  ptr(g.dgamma_lu_field), *d)
+        else:
+            L.or_compute_marangoni_force(ptr(g.T), ptr(g.phi), ptr(g.f_l), ptr(g.Fx),
+                                         ptr(g.Fy), ptr(g.Fz), ptr(g.flags), self.dgamma_dT_lu,
+                                         *d)
         if is_welding:
             L.or_apply_arc_pressure(ptr(g.Fz), ptr(g.flags), ptr(g.phi), arc_i, arc_j, arc_k,
                                     self.arc_sigma_m / g.dx, float(self.arc_pressure), g.dt,
@@ -612,7 +709,12 @@ class OracleTwin:
         if self.force_limit_lu > 0.0:
             L.or_clamp_body_force_magnitude(ptr(g.Fx), ptr(g.Fy), ptr(g.Fz), ptr(g.flags),
                                             self.force_limit_lu, *d)
-        if self.use_srt:
+        if self.use_tables and self.use_variable_tau:
+            L.or_collide_srt_variable_tau(ptr(g.f_src), ptr(g.f_dst), ptr(g.rho), ptr(g.ux),
+                                          ptr(g.uy), ptr(g.uz), ptr(g.Fx), ptr(g.Fy), ptr(g.Fz),
+                                          ptr(g.f_l), ptr(g.flags), ptr(g.tau_field),
+                                          self.C_darcy, *d)
+        elif self.use_srt:
             L.or_collide_srt(ptr(g.f_src), ptr(g.f_dst), ptr(g.rho), ptr(g.ux), ptr(g.uy),
                              ptr(g.uz), ptr(g.Fx), ptr(g.Fy), ptr(g.Fz), ptr(g.f_l), ptr(g.flags),
                              g.tau, self.omega, 1.0, self.C_darcy, *d)
@@ -633,16 +735,28 @@ class OracleTwin:
         g.Fy_snap[...] = g.Fy
         g.Fz_snap[...] = g.Fz
         if self.enable_substrate_growth or self.enable_bead_freeze:
-            L.or_remelt_hot_solid_scalar(ptr(g.T), ptr(g.H), ptr(g.f_l), ptr(g.phi), ptr(g.flags),
-                                         self.cp_rho, m.T_solidus, m.T_liquidus, self.L_rho, *d)
+            if self.use_tables:
+                L.or_remelt_hot_solid(ptr(g.T), ptr(g.H), ptr(g.f_l), ptr(g.phi), ptr(g.flags),
+                                      ptr(g.cp_rho_field), m.T_solidus, m.T_liquidus,
+                                      self.L_rho, *d)
+            else:
+                L.or_remelt_hot_solid_scalar(ptr(g.T), ptr(g.H), ptr(g.f_l), ptr(g.phi),
+                                             ptr(g.flags), self.cp_rho, m.T_solidus,
+                                             m.T_liquidus, self.L_rho, *d)
             if is_welding and self.enable_bead_freeze:
                 dx_, dy_, dz_ = self._torch_dir_xyz
                 lookback = max(2.0, self.trailing_solidify_lookback_mm / (g.dx * 1000.0))
                 T_freeze = m.T_liquidus + self.trailing_solidify_temp_margin_K
-                L.or_solidify_trailing_pool_scalar(
-                    ptr(g.T), ptr(g.H), ptr(g.f_l), ptr(g.phi), ptr(g.flags), ptr(g.ux),
-                    ptr(g.uy), ptr(g.uz), self.cp_rho, arc_i, arc_j, arc_k, dx_, dy_, dz_,
-                    lookback, T_freeze, m.T_solidus, *d)
+                if self.use_tables:
+                    L.or_solidify_trailing_pool(
+                        ptr(g.T), ptr(g.H), ptr(g.f_l), ptr(g.phi), ptr(g.flags), ptr(g.ux),
+                        ptr(g.uy), ptr(g.uz), ptr(g.cp_rho_field), arc_i, arc_j, arc_k, dx_,
+                        dy_, dz_, lookback, T_freeze, m.T_solidus, *d)
+                else:
+                    L.or_solidify_trailing_pool_scalar(
+                        ptr(g.T), ptr(g.H), ptr(g.f_l), ptr(g.phi), ptr(g.flags), ptr(g.ux),
+                        ptr(g.uy), ptr(g.uz), self.cp_rho, arc_i, arc_j, arc_k, dx_, dy_, dz_,
+                        lookback, T_freeze, m.T_solidus, *d)
             L.or_solidify_cooled_metal(ptr(g.T), ptr(g.f_l), ptr(g.phi), ptr(g.flags), ptr(g.ux),
                                        ptr(g.uy), ptr(g.uz), m.T_solidus,
                                        1 if (self.enable_bead_freeze or self.enable_substrate_growth) else 0,

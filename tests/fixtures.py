@@ -35,6 +35,14 @@ LCS_BEAD_ON_PLATE = dict(name="LCS-bead-on-plate", k=30.0, rho=7800.0, cp=680.0,
                          T_solidus=1723.0, T_liquidus=1780.0, L_fusion=2.7e5, gamma_0=1.8,
                          dgamma_dT=-4.0e-4, mu=0.0065, beta_T=1.2e-4)
 
+# Piecewise-linear knots of the reference's validated ER70S-6 v1 file (cp, k, mu) with its
+# two-knot dgamma/dT table (the file's own surfactant model replaces that table; these raw knots
+# keep the table lookups at three and two knots).
+ER70S6_V1_TABLES = dict(cp=[(300.0, 650.0), (1000.0, 680.0), (1793.0, 720.0)],
+                        k=[(300.0, 28.0), (1000.0, 30.0), (1793.0, 34.0)],
+                        mu=[(300.0, 0.0075), (1793.0, 0.0065), (2000.0, 0.0055)],
+                        dgamma_dT=[(1793.0, -4.3e-4), (2000.0, -3.8e-4)])
+
 
 def f32(x) -> np.float32:
     return np.float32(x)

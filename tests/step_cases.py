@@ -11,7 +11,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from tests.fixtures import LCS_BEAD_ON_PLATE, SOLID, seed_pool
+from tests.fixtures import ER70S6_V1_TABLES, LCS_BEAD_ON_PLATE, SOLID, seed_pool
 
 FLOW_TIER = dict(enable_vof=True, enable_csf_tension=True, enable_wetting=True,
                  enable_hydrostatic_gravity=True, enable_bead_freeze=True,
@@ -26,6 +26,9 @@ def case_kwargs(name: str) -> dict:
         return dict(base, nx=64, ny=40, nz=32, dx=2e-4, droplet_freq_hz=0.0)
     if name == "droplets_srt":
         return dict(base, nx=56, ny=40, nz=32, dx=2e-4, use_srt=True, droplet_freq_hz=2500.0)
+    if name == "tables_mrt":
+        # temperature-dependent cp, k, mu, dgamma/dT: variable-tau SRT collision, per-cell cp
+        return dict(base, nx=64, ny=40, nz=32, dx=2e-4, droplet_freq_hz=0.0)
     if name == "bench_small":
         return dict(base, nx=128, ny=64, nz=32, dx=2e-4, droplet_freq_hz=0.0)
     raise KeyError(name)
@@ -61,11 +64,14 @@ def build_pair(name: str, mode: str = "jacobi", engine: str = "fused", layout: i
 
     kw = case_kwargs(name)
     mat_d = dict(LCS_BEAD_ON_PLATE)
+    tables = ER70S6_V1_TABLES if name.startswith("tables") else {}
     prod_mat = material_from_dict({"name": mat_d["name"], "status": "synthetic",
-                                   "constants": {k: v for k, v in mat_d.items() if k != "name"}})
+                                   "constants": {k: v for k, v in mat_d.items() if k != "name"},
+                                   "tables": {k: [list(p) for p in v] for k, v in tables.items()}})
     tw = WAAMTwin(material=prod_mat, field_layout=layout, **kw)
     tw.engine = engine
-    ot = OracleTwin(Material(**mat_d), mode=mode, **kw)
+    ot = OracleTwin(Material(**mat_d, tables=tables), mode=mode, **kw)
+    assert tw.use_material_tables == ot.use_tables == bool(tables)
     for t in (tw, ot):
         configure(t, name)
     tw.reset()

@@ -318,3 +318,110 @@ def test_telemetry_reduce(env):
     env["O"].or_telemetry_pool_reduce(P(o["T"]), P(o["f_l"]), P(o["phi"]), P(dT), P(o["ux"]),
                                       P(o["uy"]), P(o["uz"]), P(out_o), *dims)
     assert np.array_equal(down(out_g), out_o), (down(out_g), out_o)
+
+
+# Knot tables for the table-path kernels: three-knot cp/k/mu, two-knot dgamma/dT, and an
+# eight-knot cp table with a zero-width interval (the lookup's last matching interval wins).
+_TABLES = dict(cp=[(300.0, 650.0), (1000.0, 680.0), (1793.0, 720.0)],
+               k=[(300.0, 28.0), (1000.0, 30.0), (1793.0, 34.0)],
+               mu=[(300.0, 0.0075), (1793.0, 0.0065), (2000.0, 0.0055)],
+               dgamma_dT=[(1793.0, -4.3e-4), (2000.0, -3.8e-4)])
+_CP8 = [(300.0, 500.0), (600.0, 560.0), (900.0, 600.0), (900.0, 640.0), (1400.0, 690.0),
+        (1700.0, 720.0), (2100.0, 760.0), (2600.0, 800.0)]
+
+
+def _packed(tables):
+    from oracle.oracle import Material, pack_tables
+
+    m = Material(name="t", k=30.0, rho=7800.0, cp=680.0, T_solidus=1723.0, T_liquidus=1780.0,
+                 L_fusion=2.7e5, gamma_0=1.8, dgamma_dT=-4.0e-4, mu=0.0065, tables=tables)
+    return m, pack_tables(m)
+
+
+@pytest.mark.parametrize("dims", DIMS)
+@pytest.mark.parametrize("cp_table", ["three", "eight"])
+@pytest.mark.parametrize("use_tables", [1, 0])
+def test_table_kernels(env, dims, cp_table, use_tables):
+    """Every temperature-dependent-material entry point on one seeded state, in step order."""
+    s = random_state(*dims, seed=21)
+    rng = np.random.default_rng(5)
+    tables = dict(_TABLES, cp=_CP8) if cp_table == "eight" else dict(_TABLES)
+    m, (knots, counts) = _packed(tables)
+    # exact knot temperatures exercise the end clamps and the zero-width interval
+    T = s["T"]
+    hit = rng.random(dims) < 0.05
+    T[hit] = rng.choice(np.array([t for t, _ in tables["cp"]] + [250.0, 2900.0], np.float32),
+                        hit.sum())
+    s["H"] = (T * np.float32(5.2e6)).astype(np.float32)
+    keys = ("T", "H", "f_l", "phi", "flags", "ux", "uy", "uz", "Fx", "Fy", "Fz", "f", "f2",
+            "rho")
+    g = {k: up(env, s[k]) for k in keys}
+    o = {k: s[k].copy() for k in keys}
+    props = ("cp_rho", "alpha", "dgamma", "tau")
+    for p in props:
+        g[p] = up(env, np.zeros(dims, np.float32))
+        o[p] = np.zeros(dims, np.float32)
+    K, O = env["K"], env["O"]
+    rho, dt, dx = 7800.0, 2e-5, 2e-4
+    fscale = dt * dt / (rho * dx * dx) / dx
+    refs = (5.3e6, 0.0471, -3.1e-7, 0.5125, 0.9)
+    t = knots
+    K.refresh_thermal_properties(
+        g["T"], g["cp_rho"], g["alpha"], g["dgamma"], g["tau"], t[0:8], t[8:16], t[16:24],
+        t[24:32], t[32:40], t[40:48], t[48:56], t[56:64], *[int(c) for c in counts], rho, dt,
+        dx, m.cp, m.k, m.mu, m.dgamma_dT, fscale, *refs, use_tables, g["flags"], 1, 2)
+    consts = np.array([rho, dt, dx, m.cp, m.k, m.mu, m.dgamma_dT, fscale, *refs], np.float32)
+    O.or_refresh_thermal_properties(P(o["T"]), P(o["cp_rho"]), P(o["alpha"]), P(o["dgamma"]),
+                                    P(o["tau"]), P(knots), P(counts), P(consts), use_tables,
+                                    P(o["flags"]), *dims)
+    for p in props:
+        check(p, down(g[p]), o[p])
+    K.advect_diffuse_temperature_variable(g["H"], g["T"], g["ux"], g["uy"], g["uz"], g["flags"],
+                                          g["alpha"], g["cp_rho"], 1.0, 1, 2, *dims)
+    H_in = o["H"].copy()
+    O.or_advect_diffuse_temperature_variable_jacobi(
+        P(H_in), P(o["H"]), P(o["T"]), P(o["ux"]), P(o["uy"]), P(o["uz"]), P(o["flags"]),
+        P(o["alpha"]), P(o["cp_rho"]), 1.0, *dims)
+    check("H advect", down(g["H"]), o["H"])
+    K.clamp_enthalpy_floor(g["H"], g["cp_rho"], g["flags"], 1200.0, 2)
+    O.or_clamp_enthalpy_floor(P(o["H"]), P(o["cp_rho"]), P(o["flags"]), 1200.0, *dims)
+    check("H floor", down(g["H"]), o["H"])
+    L_rho, Ts, Tl = 2.1e9, 1723.0, 1780.0
+    H_sol = 7800.0 * 700.0 * Ts
+    H_liq = H_sol + L_rho
+    K.update_phase_variable_cp(g["H"], g["T"], g["f_l"], g["cp_rho"], L_rho, Ts, Tl, H_sol, H_liq)
+    O.or_update_phase_variable_cp(P(o["H"]), P(o["T"]), P(o["f_l"]), P(o["cp_rho"]), L_rho, Ts,
+                                  Tl, H_sol, H_liq, *dims)
+    check("T phase", down(g["T"]), o["T"])
+    check("f_l phase", down(g["f_l"]), o["f_l"])
+    K.clamp_enthalpy_ceiling_variable_cp(g["H"], g["flags"], g["cp_rho"], H_liq, Tl, 2500.0, 2)
+    O.or_clamp_enthalpy_ceiling_variable_cp(P(o["H"]), P(o["flags"]), P(o["cp_rho"]), H_liq, Tl,
+                                            2500.0, *dims)
+    check("H ceiling", down(g["H"]), o["H"])
+    K.compute_marangoni_force_variable(g["T"], g["phi"], g["f_l"], g["Fx"], g["Fy"], g["Fz"],
+                                       g["flags"], g["dgamma"], 1, 2, *dims)
+    O.or_compute_marangoni_force_variable(P(o["T"]), P(o["phi"]), P(o["f_l"]), P(o["Fx"]),
+                                          P(o["Fy"]), P(o["Fz"]), P(o["flags"]), P(o["dgamma"]),
+                                          *dims)
+    for k in ("Fx", "Fy", "Fz"):
+        check(k, down(g[k]), o[k])
+    K.collide_srt_variable_tau(g["f"], g["f2"], g["rho"], g["ux"], g["uy"], g["uz"], g["Fx"],
+                               g["Fy"], g["Fz"], g["f_l"], g["flags"], g["tau"], 1.6e5, 1, 2,
+                               *dims)
+    O.or_collide_srt_variable_tau(P(o["f"]), P(o["f2"]), P(o["rho"]), P(o["ux"]), P(o["uy"]),
+                                  P(o["uz"]), P(o["Fx"]), P(o["Fy"]), P(o["Fz"]), P(o["f_l"]),
+                                  P(o["flags"]), P(o["tau"]), 1.6e5, *dims)
+    for k in ("f2", "rho", "ux", "uy", "uz"):
+        check(k, down(g[k]), o[k])
+    K.remelt_hot_solid(g["T"], g["H"], g["f_l"], g["phi"], g["flags"], g["cp_rho"], Ts, Tl,
+                       L_rho, 1, 0)
+    O.or_remelt_hot_solid(P(o["T"]), P(o["H"]), P(o["f_l"]), P(o["phi"]), P(o["flags"]),
+                          P(o["cp_rho"]), Ts, Tl, L_rho, *dims)
+    K.solidify_trailing_pool(g["T"], g["H"], g["f_l"], g["phi"], g["flags"], g["ux"], g["uy"],
+                             g["uz"], g["cp_rho"], dims[0] / 2, dims[1] / 2, 4.0, 1.0, 0.0, 0.0,
+                             3.0, 2400.0, Ts, 1, 0, 2)
+    O.or_solidify_trailing_pool(P(o["T"]), P(o["H"]), P(o["f_l"]), P(o["phi"]), P(o["flags"]),
+                                P(o["ux"]), P(o["uy"]), P(o["uz"]), P(o["cp_rho"]), dims[0] / 2,
+                                dims[1] / 2, 4.0, 1.0, 0.0, 0.0, 3.0, 2400.0, Ts, *dims)
+    for k in ("T", "H", "f_l", "phi", "flags", "ux", "uy", "uz"):
+        check(k, down(g[k]), o[k])

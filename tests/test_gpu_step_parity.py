@@ -11,7 +11,9 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from tests.step_cases import build_pair, compare, run_pair
+from tests.step_cases import FIELDS, build_pair, compare, run_pair
+
+TABLE_FIELDS = ("cp_rho_field", "alpha_lu_field", "dgamma_lu_field", "tau_field")
 
 pytestmark = pytest.mark.gpu
 
@@ -46,3 +48,29 @@ def test_coupled_step_parity(hip_device, case, engine, layout):
     assert tel["n_liquid_cells"] == pm["n_liquid_cells"]
     assert abs(tel["pool_width_mm"] - round(pm["pool_width_m"] * 1000, 3)) < 1e-9
     assert abs(tel["pool_depth_mm"] - round(pm["pool_depth_m"] * 1000, 3)) < 1e-9
+
+
+@pytest.mark.parametrize("layout", [1, 0])
+@pytest.mark.parametrize("first_welding", [True, False])
+def test_table_material_parity(hip_device, layout, first_welding):
+    """Temperature-dependent cp, k, mu and dgamma/dT (per-cell properties, variable-tau SRT).
+
+    The per-cell property fields start at zero and are first written by the refresh that
+    precedes the thermal update, so a first step that welds clamps enthalpy against
+    cp_rho = 0 (reference solvers/coupled_step.py:104-108 before :168-182); the other
+    variant starts with one step without the arc so the clamp sees refreshed properties."""
+    tw, ot = build_pair("tables_mrt", layout=layout)
+    g = tw.grid
+    done = 0
+    if not first_welding:
+        x, y = (g.nx // 3) * g.dx, (g.ny // 2) * g.dx
+        tw.step(x, y, False)
+        ot.step(x, y, False)
+    for n in (1, 9, 40, 50):
+        run_pair(tw, ot, n, start=done)
+        done += n
+        _assert_parity(compare(tw, ot, FIELDS + TABLE_FIELDS),
+                       f"tables/layout{layout}/weld0={first_welding} step {done}")
+    # the refreshed properties span the table range where the metal is hot
+    cp = tw.grid.cp_rho_field.to_numpy()
+    assert float(cp.max()) > float(cp.min())

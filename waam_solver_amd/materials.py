@@ -166,15 +166,27 @@ def material_from_dict(data: dict) -> MaterialProps:
     return props
 
 
+def resolve_material_path(name: str | pathlib.Path) -> pathlib.Path | None:
+    """A YAML path (contains '/' or ends in .yaml/.yml), else ``materials/validated/<name>.yaml``
+    then ``materials/<name>.yaml`` in the package (reference materials.py:153-167, :199-200)."""
+    s = str(name)
+    if "/" in s or s.endswith((".yaml", ".yml")):
+        p = pathlib.Path(s)
+        return p if p.exists() else None
+    for base in (PKG_MATERIALS / "validated", PKG_MATERIALS):
+        for cand in (base / f"{s}.yaml", base / f"{s}.yml"):
+            if cand.exists():
+                return cand
+    return None
+
+
 def load_material(name: str | pathlib.Path, high_sulphur: bool = False) -> MaterialProps:
     """Load a material by name (package materials/ directory) or by YAML path."""
     import yaml
 
-    p = pathlib.Path(name)
-    if not p.suffix:
-        p = PKG_MATERIALS / f"{name}.yaml"
-    if not p.exists():
-        raise KeyError(f"Material '{name}' not found (looked for {p})")
+    p = resolve_material_path(name)
+    if p is None:
+        raise KeyError(f"Material '{name}' not found")
     with open(p) as f:
         props = material_from_dict(yaml.safe_load(f))
     if high_sulphur:

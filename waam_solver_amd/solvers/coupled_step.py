@@ -34,11 +34,38 @@ def _swap(g, a: str, b: str) -> None:
 
 
 def _clamp_enthalpy_ceiling(twin, g) -> None:
+    """Reference coupled_step.py:17-31."""
     if not twin.enable_enthalpy_cap:
+        return
+    if twin.use_material_tables:
+        kernels.clamp_enthalpy_ceiling_variable_cp(g.H, g.flags, g.cp_rho_field, twin.H_liq,
+                                                   twin.mat.T_liquidus, twin.T_vapor_cap_K,
+                                                   g.FLAG_GAS)
         return
     kernels.clamp_enthalpy_ceiling_scalar(g.H, g.flags, twin.cp_rho, twin.mat.T_solidus,
                                           twin.mat.T_liquidus, twin.T_vapor_cap_K, twin.L_rho,
                                           g.FLAG_GAS)
+
+
+def _refresh_properties(twin, g) -> None:
+    """Per-cell properties from the material tables (reference coupled_step.py:168-182)."""
+    t = twin.gpu_tables.struct
+    kernels.refresh_thermal_properties(
+        g.T, g.cp_rho_field, g.alpha_lu_field, g.dgamma_lu_field, g.tau_field, t.cp_T, t.cp_V,
+        t.k_T, t.k_V, t.mu_T, t.mu_V, t.dgamma_T, t.dgamma_V, t.n_cp, t.n_k, t.n_mu, t.n_dgamma,
+        g.mat.rho, g.dt, g.dx, t.cp_fallback, t.k_fallback, t.mu_fallback, t.dgamma_fallback,
+        twin.force_scale / g.dx, twin.cp_rho, twin.alpha_lu, twin.dgamma_dT_lu, g.tau,
+        twin.marangoni_scale, 1, g.flags, g.FLAG_SOLID, g.FLAG_GAS)
+
+
+def _update_phase(twin, g) -> None:
+    if twin.use_material_tables:
+        kernels.update_phase_variable_cp(g.H, g.T, g.f_l, g.cp_rho_field, twin.L_rho,
+                                         twin.mat.T_solidus, twin.mat.T_liquidus, twin.H_sol,
+                                         twin.H_liq)
+    else:
+        kernels.update_phase(g.H, g.T, g.f_l, twin.cp_rho, twin.L_rho, twin.mat.T_solidus,
+                             twin.mat.T_liquidus)
 
 
 def resolve_arc_k(twin, g, arc_i: float, arc_j: float, torch_z_m: float | None = None) -> float:
@@ -103,10 +130,8 @@ def _droplet(twin, g, x_m: float, y_m: float, arc_i: float, arc_j: float, arc_k:
 
 def coupled_step(twin, torch_x_m: float, torch_y_m: float, is_welding: bool,
                  torch_z_m: float | None = None) -> None:
-    if twin.use_material_tables:
-        raise NotImplementedError("temperature-dependent material tables are not built; "
-                                  "use a scalar (constants-only) material")
     g = twin.grid
+    tables = twin.use_material_tables
     n = g.nx, g.ny, g.nz
     arc_i = torch_x_m / g.dx
     arc_j = torch_y_m / g.dx
@@ -123,20 +148,31 @@ def coupled_step(twin, torch_x_m: float, torch_y_m: float, is_welding: bool,
         _droplet(twin, g, torch_x_m, torch_y_m, arc_i, arc_j, arc_k)
 
     # thermal advection-diffusion, Jacobi: H_tmp <- H + dH(H); swap
-    kernels.advect_diffuse_temperature_into(g.H, g.H_tmp, g.T, g.ux, g.uy, g.uz, g.flags,
-                                            twin.alpha_lu, 1.0, twin.cp_rho, *n)
+    if tables:
+        _refresh_properties(twin, g)
+        kernels.advect_diffuse_temperature_variable_into(g.H, g.H_tmp, g.T, g.ux, g.uy, g.uz,
+                                                         g.flags, g.alpha_lu_field,
+                                                         g.cp_rho_field, 1.0, *n)
+    else:
+        kernels.advect_diffuse_temperature_into(g.H, g.H_tmp, g.T, g.ux, g.uy, g.uz, g.flags,
+                                                twin.alpha_lu, 1.0, twin.cp_rho, *n)
     _swap(g, "H", "H_tmp")
     if twin.enable_heat_loss:
-        kernels.apply_thermal_boundary_losses(
-            g.H, g.T, g.flags, twin.T_amb, twin.h_conv, twin.eps_rad,
-            1 if twin.enable_convection else 0, 1 if twin.enable_radiation else 0, twin.cp_rho,
-            g.dt, g.dx, twin.sigma_sb, g.FLAG_SOLID, g.FLAG_GAS, *n)
-        kernels.clamp_enthalpy_floor_scalar(g.H, twin.cp_rho, g.flags, twin.T_amb, g.FLAG_GAS)
-    kernels.update_phase(g.H, g.T, g.f_l, twin.cp_rho, twin.L_rho, twin.mat.T_solidus,
-                         twin.mat.T_liquidus)
+        conv, rad = (1 if twin.enable_convection else 0), (1 if twin.enable_radiation else 0)
+        if tables:
+            kernels.apply_thermal_boundary_losses_variable(
+                g.H, g.T, g.flags, g.cp_rho_field, twin.T_amb, twin.h_conv, twin.eps_rad, conv,
+                rad, g.dt, g.dx, twin.sigma_sb, g.FLAG_SOLID, g.FLAG_GAS, *n)
+            kernels.clamp_enthalpy_floor(g.H, g.cp_rho_field, g.flags, twin.T_amb, g.FLAG_GAS)
+        else:
+            kernels.apply_thermal_boundary_losses(
+                g.H, g.T, g.flags, twin.T_amb, twin.h_conv, twin.eps_rad, conv, rad, twin.cp_rho,
+                g.dt, g.dx, twin.sigma_sb, g.FLAG_SOLID, g.FLAG_GAS, *n)
+            kernels.clamp_enthalpy_floor_scalar(g.H, twin.cp_rho, g.flags, twin.T_amb,
+                                                g.FLAG_GAS)
+    _update_phase(twin, g)
     _clamp_enthalpy_ceiling(twin, g)
-    kernels.update_phase(g.H, g.T, g.f_l, twin.cp_rho, twin.L_rho, twin.mat.T_solidus,
-                         twin.mat.T_liquidus)
+    _update_phase(twin, g)
     kernels.update_T_max(g.T, g.T_max, g.flags, g.FLAG_GAS)
     kernels.update_cooling_rate(g.T, g.T_prev, g.dT_dt, g.flags, g.dt, g.FLAG_GAS)
 
@@ -156,8 +192,12 @@ def coupled_step(twin, torch_x_m: float, torch_y_m: float, is_welding: bool,
     if twin.enable_csf_tension:
         kernels.compute_csf_tension(g.phi, g.flags, g.Fx, g.Fy, g.Fz, twin.gamma_lu, g.FLAG_SOLID,
                                     g.FLAG_GAS, *n, int(twin.enable_wetting), twin.theta_rad)
-    kernels.compute_marangoni_force(g.T, g.phi, g.f_l, g.Fx, g.Fy, g.Fz, g.flags,
-                                    twin.dgamma_dT_lu, g.dx, g.FLAG_SOLID, g.FLAG_GAS, *n)
+    if tables:
+        kernels.compute_marangoni_force_variable(g.T, g.phi, g.f_l, g.Fx, g.Fy, g.Fz, g.flags,
+                                                 g.dgamma_lu_field, g.FLAG_SOLID, g.FLAG_GAS, *n)
+    else:
+        kernels.compute_marangoni_force(g.T, g.phi, g.f_l, g.Fx, g.Fy, g.Fz, g.flags,
+                                        twin.dgamma_dT_lu, g.dx, g.FLAG_SOLID, g.FLAG_GAS, *n)
     if is_welding and twin.enable_gas_shear:
         raise NotImplementedError("gas shear (physics_tier: full) is not built")
     if is_welding:
@@ -178,7 +218,11 @@ def coupled_step(twin, torch_x_m: float, torch_y_m: float, is_welding: bool,
     if F_cap > 0.0:
         kernels.clamp_body_force_magnitude(g.Fx, g.Fy, g.Fz, g.flags, F_cap, g.FLAG_SOLID,
                                            g.FLAG_GAS)
-    if twin.use_srt:
+    if tables and twin.use_variable_tau:
+        kernels.collide_srt_variable_tau(g.f_src, g.f_dst, g.rho, g.ux, g.uy, g.uz, g.Fx, g.Fy,
+                                         g.Fz, g.f_l, g.flags, g.tau_field, twin.C_darcy,
+                                         g.FLAG_SOLID, g.FLAG_GAS, *n)
+    elif twin.use_srt:
         kernels.collide_srt(g.f_src, g.f_dst, g.rho, g.ux, g.uy, g.uz, g.Fx, g.Fy, g.Fz, g.f_l,
                             g.flags, g.tau, twin.omega, 1.0, twin.C_darcy, g.FLAG_SOLID,
                             g.FLAG_GAS, *n)
@@ -197,17 +241,28 @@ def coupled_step(twin, torch_x_m: float, torch_y_m: float, is_welding: bool,
     if twin.snapshot_forces:
         kernels.snapshot_forces(g.Fx, g.Fy, g.Fz, g.Fx_snap, g.Fy_snap, g.Fz_snap)
     if twin.enable_substrate_growth or twin.enable_bead_freeze:
-        kernels.remelt_hot_solid_scalar(g.T, g.H, g.f_l, g.phi, g.flags, twin.cp_rho,
-                                        twin.mat.T_solidus, twin.mat.T_liquidus, twin.L_rho,
-                                        g.FLAG_SOLID, g.FLAG_FLUID)
+        if tables:
+            kernels.remelt_hot_solid(g.T, g.H, g.f_l, g.phi, g.flags, g.cp_rho_field,
+                                     twin.mat.T_solidus, twin.mat.T_liquidus, twin.L_rho,
+                                     g.FLAG_SOLID, g.FLAG_FLUID)
+        else:
+            kernels.remelt_hot_solid_scalar(g.T, g.H, g.f_l, g.phi, g.flags, twin.cp_rho,
+                                            twin.mat.T_solidus, twin.mat.T_liquidus, twin.L_rho,
+                                            g.FLAG_SOLID, g.FLAG_FLUID)
         if is_welding and twin.enable_bead_freeze:
             dir_x, dir_y, dir_z = getattr(twin, "_torch_dir_xyz", (1.0, 0.0, 0.0))
             lookback = max(2.0, twin.trailing_solidify_lookback_mm / (g.dx * 1000.0))
             T_freeze = twin.mat.T_liquidus + twin.trailing_solidify_temp_margin_K
-            kernels.solidify_trailing_pool_scalar(
-                g.T, g.H, g.f_l, g.phi, g.flags, g.ux, g.uy, g.uz, twin.cp_rho, arc_i, arc_j,
-                arc_k, dir_x, dir_y, dir_z, lookback, T_freeze, twin.mat.T_solidus, g.FLAG_SOLID,
-                g.FLAG_FLUID, g.FLAG_GAS)
+            if tables:
+                kernels.solidify_trailing_pool(
+                    g.T, g.H, g.f_l, g.phi, g.flags, g.ux, g.uy, g.uz, g.cp_rho_field, arc_i,
+                    arc_j, arc_k, dir_x, dir_y, dir_z, lookback, T_freeze, twin.mat.T_solidus,
+                    g.FLAG_SOLID, g.FLAG_FLUID, g.FLAG_GAS)
+            else:
+                kernels.solidify_trailing_pool_scalar(
+                    g.T, g.H, g.f_l, g.phi, g.flags, g.ux, g.uy, g.uz, twin.cp_rho, arc_i, arc_j,
+                    arc_k, dir_x, dir_y, dir_z, lookback, T_freeze, twin.mat.T_solidus,
+                    g.FLAG_SOLID, g.FLAG_FLUID, g.FLAG_GAS)
         kernels.solidify_cooled_metal(g.T, g.f_l, g.phi, g.flags, g.ux, g.uy, g.uz,
                                       twin.mat.T_solidus,
                                       twin.enable_bead_freeze or twin.enable_substrate_growth,

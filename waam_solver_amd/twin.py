@@ -19,6 +19,7 @@ import numpy as np
 
 from . import kernels
 from .field import LAYOUT_ZYX
+from .gpu_tables import MaterialGPUTables
 from .grid import WAAMGrid
 from .materials import MaterialProps, load_material
 from .physics.arc import create_heat_source
@@ -175,7 +176,8 @@ class WAAMTwin:
         self.force_scale = scale_F
         self.dgamma_dT_lu = mat.dgamma_dT * scale_F / g.dx
         self.gamma_lu = mat.gamma_0 * scale_F / g.dx
-        self.use_material_tables = mat.tables.has_any()
+        self.gpu_tables = MaterialGPUTables(mat)
+        self.use_material_tables = self.gpu_tables.enabled
         cp_sol = mat.cp_at(mat.T_solidus)
         self.H_sol = mat.rho * cp_sol * mat.T_solidus
         self.H_liq = self.H_sol + self.L_rho
@@ -337,7 +339,11 @@ class WAAMTwin:
             log.warning("[WAAMTwin] welding step with wire_feed_m_s == 0: no metal is deposited")
         sim_x = torch_x_m - self._window_offset_x_m - self._sim_origin_offset_x_m
         sim_y = torch_y_m - self._sim_origin_offset_y_m
-        if self.engine == "fused":
+        if self.engine == "fused" and self.use_material_tables:
+            # the fused kernels take the scalar properties; per-cell tables step through the
+            # reference-order kernels (tables.hip), which give the same fields
+            coupled_step(self, sim_x, sim_y, is_welding, torch_z_m)
+        elif self.engine == "fused":
             if self._fused is None:
                 self._fused = FusedStepper(self)
             self._fused.step(sim_x, sim_y, is_welding, torch_z_m)
