@@ -323,6 +323,8 @@ class OracleTwin:
         self.trailing_solidify_lookback_mm = 2.5
         self.trailing_solidify_temp_margin_K = 35.0
         self.u_mach_limit_lu, self.force_limit_lu = 0.08, 0.05
+        self.arc_pressure_model = "constant"
+        self.pressure_sigma_m = None
         self.enable_ctwd = False
         self.plate_thickness_mm = None
         self.plate_size_mm = None
@@ -461,6 +463,22 @@ class OracleTwin:
         n_cells = (m / self.mat.rho) / (self.g.dx ** 3)
         r = (3.0 * n_cells / (4.0 * math.pi)) ** (1.0 / 3.0)
         return max(1.0, min(r, 10.0))
+
+    def arc_pressure_sigma_m(self):
+        """Pressure Gaussian radius: its own when set, else the arc heat sigma
+        (reference physics/weld_forces.py:35-40)."""
+        if self.pressure_sigma_m is not None and self.pressure_sigma_m > 0.0:
+            return float(self.pressure_sigma_m)
+        return float(self.arc_sigma_m)
+
+    def arc_pressure_peak_pa(self):
+        """Lin & Eagar (1986) p0 = mu0 I^2 / (4 pi^2 sigma_p^2) for the "lin_eagar" model, else
+        the constant (reference physics/weld_forces.py:20-32, :47-58)."""
+        if str(self.arc_pressure_model).strip().lower() in ("lin_eagar", "lineagar", "lin-eagar"):
+            sig = max(self.arc_pressure_sigma_m(), 1.0e-6)
+            i = max(float(self.welding_current_A), 0.0)
+            return 4.0e-7 * math.pi * i * i / (4.0 * math.pi * math.pi * sig * sig)
+        return float(self.arc_pressure)
 
     def droplet_entry_temperature_K(self):
         return self.mat.T_liquidus + self.deposition_superheat_K
@@ -700,8 +718,8 @@ class OracleTwin:
                                          *d)
         if is_welding:
             L.or_apply_arc_pressure(ptr(g.Fz), ptr(g.flags), ptr(g.phi), arc_i, arc_j, arc_k,
-                                    self.arc_sigma_m / g.dx, float(self.arc_pressure), g.dt,
-                                    g.dx, m.rho, *d)
+                                    self.arc_pressure_sigma_m() / g.dx,
+                                    self.arc_pressure_peak_pa(), g.dt, g.dx, m.rho, *d)
         if self.enable_hydrostatic_gravity:
             L.or_add_hydrostatic_gravity(ptr(g.Fz), ptr(g.f_l), ptr(g.flags), 1.0, self.g_lu, *d)
         L.or_add_buoyancy(ptr(g.T), ptr(g.Fz), ptr(g.f_l), ptr(g.flags), self.g_lu, self.beta_T,

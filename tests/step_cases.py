@@ -115,3 +115,65 @@ def compare(tw, ot, fields=FIELDS) -> dict:
     for name, a, b in (("f_src", g.f_src.to_numpy(), og.f_src), ("f_dst", g.f_dst.to_numpy(), og.f_dst)):
         out[name] = (bits_equal(a, b), rel_l2(a, b))
     return out
+
+
+def oracle_like(tw, mode: str = "jacobi"):
+    """An oracle twin with every setting of the product twin ``tw`` (as built by a constructor,
+    a preset or a job file), reset is left to the caller."""
+    from oracle.oracle import Material, OracleTwin
+
+    if tw.enable_ctwd or tw.use_torch_z or tw.enable_moving_window:
+        raise NotImplementedError("the oracle twin has no CTWD, torch-z or moving-window model")
+    m, g = tw.mat, tw.grid
+    # the product material is already through its surfactant model: hand its final dgamma/dT
+    # and gamma_0 over at the 30 ppm identity of the Heiple scale
+    mat = Material(name=m.name, k=m.k, rho=m.rho, cp=m.cp, T_solidus=m.T_solidus,
+                   T_liquidus=m.T_liquidus, L_fusion=m.L_fusion, gamma_0=m.gamma_0,
+                   dgamma_dT=m.dgamma_dT, mu=m.mu, beta_T=m.beta_T, alpha=m.alpha,
+                   contact_angle_deg=m.contact_angle_deg, sulphur_ppm=30.0,
+                   rho_e_ohm_m=m.rho_e_ohm_m, eta_stick=m.eta_stick,
+                   tables={"cp": list(m.tables.cp), "k": list(m.tables.k),
+                           "mu": list(m.tables.mu), "dgamma_dT": list(m.tables.dgamma_dT)})
+    ot = OracleTwin(
+        mat, nx=g.nx, ny=g.ny, nz=g.nz, dx=g.dx, arc_power_W=tw.Q_w, arc_efficiency=tw.eta,
+        T_ambient=tw.T_amb, use_srt=tw.use_srt, C_darcy=tw.C_darcy,
+        arc_pressure_pa=tw.arc_pressure, droplet_freq_hz=tw.droplet_freq,
+        max_tracers=g.max_tracers, enable_heat_loss=tw.enable_heat_loss, h_conv=tw.h_conv,
+        eps_rad=tw.eps_rad, enable_convection=tw.enable_convection,
+        enable_radiation=tw.enable_radiation, heat_source=tw.heat_source_name,
+        enable_vof=tw.enable_vof, enable_csf_tension=tw.enable_csf_tension,
+        enable_deposition_momentum=tw.enable_deposition_momentum,
+        enable_droplet_impact_pressure=tw.enable_droplet_impact_pressure,
+        enable_substrate_growth=tw.enable_substrate_growth, enable_wetting=tw.enable_wetting,
+        enable_hydrostatic_gravity=tw.enable_hydrostatic_gravity,
+        enable_bead_freeze=tw.enable_bead_freeze, contact_angle_deg=tw.contact_angle_deg,
+        deposition_superheat_K=tw.deposition_superheat_K,
+        deposition_footprint_sigma_scale=tw.deposition_footprint_sigma_scale,
+        welding_current_A=tw.welding_current_A, travel_speed_m_s=tw.travel_speed_m_s,
+        enable_enthalpy_cap=tw.enable_enthalpy_cap, T_vapor_cap_K=tw.T_vapor_cap_K,
+        arc_surface_weighting=tw.arc_surface_weighting,
+        arc_penetration_mm=tw.arc_penetration_m * 1e3, wire_diameter_mm=tw.wire_diameter_m * 1e3,
+        arc_sigma_mm=tw.arc_sigma_m * 1e3, mode=mode, tracer_seed=tw.tracer_seed,
+        use_variable_tau=tw.use_variable_tau)
+    # settings a job or a calibration changes after construction
+    for name in ("omega_bulk", "sigma_cells", "marangoni_scale", "wire_feed_m_s",
+                 "droplet_transfer_mode", "pulse_frequency_hz", "droplet_size_jitter",
+                 "impact_lead_angle_deg", "trailing_solidify_lookback_mm",
+                 "trailing_solidify_temp_margin_K", "u_mach_limit_lu", "force_limit_lu",
+                 "arc_pressure_model", "pressure_sigma_m", "plate_thickness_mm", "plate_size_mm",
+                 "plate_origin_mm", "nz_solid"):
+        setattr(ot, name, getattr(tw, name))
+    key = tw.heat_source_name.lower().replace("-", "").replace("_", "")
+    if key in ("goldak", "goldak3d", "doubleellipsoid"):
+        src = tw.arc_source
+        ot.goldak = (src.ff, src.fr, *src.resolved_axes(tw))
+    return ot
+
+
+def derived_constants(t) -> dict:
+    """Host-side constants both twins derive from their settings (for a like-for-like check)."""
+    g = t.grid if hasattr(t, "grid") else t.g
+    return {"dt": g.dt, "tau": g.tau, "cp_rho": t.cp_rho, "L_rho": t.L_rho,
+            "alpha_lu": t.alpha_lu, "omega": t.omega, "omega_bulk": t.omega_bulk,
+            "sigma_cells": t.sigma_cells, "g_lu": t.g_lu, "force_scale": t.force_scale,
+            "dgamma_dT_lu": t.dgamma_dT_lu, "gamma_lu": t.gamma_lu, "nz_solid": t.nz_solid}

@@ -74,3 +74,42 @@ def test_table_material_parity(hip_device, layout, first_welding):
     # the refreshed properties span the table range where the metal is hot
     cp = tw.grid.cp_rho_field.to_numpy()
     assert float(cp.max()) > float(cp.min())
+
+
+@pytest.mark.parametrize("engine", ["fused", "reference"])
+def test_job_drop_in_parity(hip_device, tmp_path, engine):
+    """``WAAMTwin.from_job`` + ``run_path`` on the bead-on-plate job (table material with the
+    Sahoo dgamma/dT table, Goldak source, Lin-Eagar pressure, droplets) on a reduced domain,
+    replayed step for step on an oracle twin with the same settings."""
+    import yaml
+
+    from tests.step_cases import oracle_like
+    from waam_solver_amd.job import PKG_ROOT
+    from waam_solver_amd.twin import WAAMTwin
+
+    job = yaml.safe_load(open(PKG_ROOT / "jobs/examples/bead_on_plate.yaml"))
+    job["simulation"]["domain_mm"] = [16, 10, 8]
+    job["plate"] = {"thickness_mm": 3.0, "size_mm": [12, 8]}
+    job.pop("torch_path_csv")
+    job["torch_path"] = [{"x_mm": 4.0, "y_mm": 5.0}, {"x_mm": 12.0, "y_mm": 5.0}]
+    p = tmp_path / "job.yaml"
+    p.write_text(yaml.safe_dump(job))
+    tw = WAAMTwin.from_job(p, droplet_freq_hz=2500.0, max_tracers=4000)
+    tw.engine = engine
+    tw.reset()
+    ot = oracle_like(tw)
+    ot.reset()
+    calls = []
+    step = tw.step
+
+    def recording_step(x, y, is_welding=True, torch_z_m=None):
+        calls.append((x, y, is_welding, torch_z_m))
+        step(x, y, is_welding, torch_z_m)
+
+    tw.step = recording_step
+    tw.run_path(job, n_steps=60)
+    for x, y, w, z in calls:
+        ot.step(x, y, w, z)
+    assert len(calls) == 60
+    _assert_parity(compare(tw, ot, FIELDS + TABLE_FIELDS), f"job/{engine} step 60")
+    assert tw._n_droplets_fired == ot._n_droplets_fired and tw._n_droplets_fired > 0

@@ -1,6 +1,6 @@
 """Job files: torch path, plate/domain and process settings (reference job.py:9-404).
 
-Covers what ``WAAMTwin.from_job`` and ``run_path`` need on the scalar-material flow path:
+Covers what ``WAAMTwin.from_job`` and ``run_path`` need on the flow path (scalar or table materials):
 physics tiers (thermal | flow; ``full`` needs the Lorentz/gas-shear/recoil kernels, which are not
 built and raise), the process, deposition, wetting, arc, heat-loss and interpass sections.
 """
@@ -13,6 +13,8 @@ import pathlib
 from typing import Any
 
 log = logging.getLogger("waam_solver_amd")
+
+PKG_ROOT = pathlib.Path(__file__).resolve().parent
 
 
 def load_job_config(path) -> dict[str, Any]:
@@ -34,7 +36,12 @@ def parse_torch_path(job: dict[str, Any]) -> list[tuple[float, float, float]]:
 
         p = pathlib.Path(csv_ref)
         if not p.is_absolute():
-            p = pathlib.Path(job.get("_base_dir", ".")) / p
+            # the job's directory, then the package directory (where the reference resolves
+            # against its project root, reference job.py:18-25), then the working directory
+            for base in (pathlib.Path(job.get("_base_dir", ".")), PKG_ROOT):
+                if (base / p).exists():
+                    p = base / p
+                    break
         return load_torch_path_csv(p)
     return [(float(pt.get("x_mm", 0)) / 1000.0, float(pt.get("y_mm", 0)) / 1000.0,
              float(pt.get("z_mm", 0)) / 1000.0) for pt in (job.get("torch_path") or [])]
@@ -183,3 +190,8 @@ def apply_job_to_twin(twin, job: dict[str, Any]) -> None:
             twin.h_conv = float(heat["h_conv"])
         if "eps_rad" in heat:
             twin.eps_rad = float(heat["eps_rad"])
+    cal_path = job.get("calibration")
+    if cal_path:
+        from .calibration import apply_calibration, load_calibration
+
+        apply_calibration(twin, load_calibration(cal_path))

@@ -167,12 +167,16 @@ def material_from_dict(data: dict) -> MaterialProps:
 
 
 def resolve_material_path(name: str | pathlib.Path) -> pathlib.Path | None:
-    """A YAML path (contains '/' or ends in .yaml/.yml), else ``materials/validated/<name>.yaml``
-    then ``materials/<name>.yaml`` in the package (reference materials.py:153-167, :199-200)."""
+    """A YAML path (contains '/' or ends in .yaml/.yml; relative paths also resolve against the
+    package directory), else ``materials/validated/<name>.yaml`` then ``materials/<name>.yaml``
+    in the package (reference materials.py:153-167, :199-200)."""
     s = str(name)
     if "/" in s or s.endswith((".yaml", ".yml")):
-        p = pathlib.Path(s)
-        return p if p.exists() else None
+        # as given, else relative to the package (job files name "materials/validated/...")
+        for p in (pathlib.Path(s), PKG_MATERIALS.parent / s):
+            if p.exists():
+                return p
+        return None
     for base in (PKG_MATERIALS / "validated", PKG_MATERIALS):
         for cand in (base / f"{s}.yaml", base / f"{s}.yml"):
             if cand.exists():
