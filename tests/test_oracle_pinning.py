@@ -401,3 +401,195 @@ def test_telemetry_matches_brute_force():
     assert out[3] == u2[liq].max()
     assert out[4] == ii[liq].sum() and out[5] == ii[liq].min() and out[6] == ii[liq].max()
     assert out[7] == T.max()
+
+
+# ---------------------------------------------------------------------------- deposition
+def _plate(nx=16, ny=16, nz=12, k0=4):
+    flags = np.full((nx, ny, nz), GAS, np.uint8)
+    flags[:, :, :k0] = SOLID
+    phi = np.where(flags == GAS, 0.0, 1.0).astype(np.float32)
+    f_l = np.zeros(flags.shape, np.float32)
+    T = np.full(flags.shape, 300.0, np.float32)
+    H = (T * np.float32(5.3e6)).astype(np.float32)
+    rho = np.ones(flags.shape, np.float32)
+    f = np.zeros((19,) + flags.shape, np.float32)
+    return dict(flags=flags, phi=phi, f_l=f_l, T=T, H=H, rho=rho, f=f)
+
+
+def _feed(s, arc, foot, drop_r, target, T_drop=2200.0, cp=5.3e6, L=2.1e9):
+    acc = np.zeros(1, np.float32)
+    real = np.zeros(1, np.float32)
+    lib().or_feed_wire_surface(ptr(s["f"]), ptr(s["flags"]), ptr(s["f_l"]), ptr(s["phi"]),
+                               ptr(s["H"]), ptr(s["T"]), ptr(s["rho"]), *arc, foot, drop_r,
+                               target, T_drop, cp, L, 1.0, ptr(acc), ptr(real), 1.0,
+                               *s["flags"].shape)
+    return float(acc[0]), float(real[0])
+
+
+def test_deposition_serial_sweep_builds_columns_within_budget():
+    """A serial i, j, k sweep with a metal-neighbour test deposits whole columns: on a flat
+    plate (top at k0 = 4) each footprint column (i-8)^2 + (j-8)^2 <= 9, in lexicographic (i, j)
+    order, fills k = 4 .. k_hi - 1 with k_hi = int(arc_k + max(r_d + 3, 4)) + 1 = 10, because
+    every new cell is the metal neighbour of the one above it. Reserve-then-commit against the
+    budget stops after ceil(7.5 / 1) = 8 cells: column (5, 8) [6 cells], then (6, 6, 4..5).
+    Pass 3 re-heats existing melt (f_l >= 0.45) within 1.2 x the footprint only."""
+    s = _plate()
+    s["flags"][8, 8, 3] = FLUID
+    s["f_l"][8, 8, 3] = 0.5
+    s["flags"][1, 1, 3] = FLUID
+    s["f_l"][1, 1, 3] = 0.5
+    before = s["flags"].copy()
+    acc, real = _feed(s, (8.0, 8.0, 4.0), 3.0, 2.0, 7.5)
+    new = np.argwhere((before == GAS) & (s["flags"] == FLUID))
+    expect = [(5, 8, k) for k in range(4, 10)] + [(6, 6, 4), (6, 6, 5)]
+    assert sorted(map(tuple, new)) == sorted(expect)
+    assert acc == 8.0 and real == 8.0
+    cp, L = np.float32(5.3e6), np.float32(2.1e9)
+    for c in expect:
+        assert s["phi"][c] == 1.0 and s["f_l"][c] == 1.0 and s["T"][c] == np.float32(2200.0)
+        assert s["H"][c] == cp * np.float32(2200.0) + L
+        np.testing.assert_array_equal(s["f"][(slice(None),) + c], np.asarray(W, np.float32))
+    assert s["T"][8, 8, 3] == np.float32(2200.0) and s["T"][1, 1, 3] == np.float32(300.0)
+
+
+def test_deposition_stops_when_budget_is_met_and_never_without_metal():
+    s = _plate()
+    acc, real = _feed(s, (8.0, 8.0, 4.0), 3.0, 2.0, 1000.0)
+    new = (s["flags"] == FLUID)
+    # unlimited budget: every footprint column up to k_hi - 1, nothing outside the footprint
+    ii, jj = np.meshgrid(np.arange(16), np.arange(16), indexing="ij")
+    col = ((ii - 8) ** 2 + (jj - 8) ** 2) <= 9 * 1.44
+    assert np.all(new[~col] == False)  # noqa: E712
+    assert real == float(new.sum()) == acc
+    s2 = _plate()
+    s2["flags"][:] = GAS
+    s2["phi"][:] = 0.0
+    acc, real = _feed(s2, (8.0, 8.0, 4.0), 3.0, 2.0, 10.0)
+    assert acc == 0.0 and real == 0.0 and np.all(s2["flags"] == GAS)
+
+
+def test_momentum_impact_sets_equilibrium_at_the_droplet_velocity():
+    """Cells in the droplet region get u = v and f = f_eq(rho, v): sum f = rho and
+    sum f e = rho v (D3Q19 moment identities); GAS and mushy (f_l < 0.45) cells are untouched."""
+    rng = np.random.default_rng(9)
+    n = (12, 12, 12)
+    flags = np.full(n, FLUID, np.uint8)
+    f_l = np.ones(n, np.float32)
+    flags[6, 6, 7] = GAS
+    f_l[5, 6, 6] = 0.3
+    rho = rng.uniform(0.95, 1.05, n).astype(np.float32)
+    u = [np.zeros(n, np.float32) for _ in range(3)]
+    f = rng.uniform(0.0, 0.1, (19,) + n).astype(np.float32)
+    f0 = f.copy()
+    v = (0.01, -0.02, -0.05)
+    lib().or_feed_wire_momentum_impact(ptr(f), ptr(rho), *(ptr(a) for a in u), ptr(flags),
+                                       ptr(f_l), 6.0, 6.0, 3.0, 1.5, *v, *n)
+    hit = u[2] != 0
+    assert hit[6, 6, 5] and not hit[6, 6, 7] and not hit[5, 6, 6] and not hit[0, 0, 0]
+    for a, va in zip(u, v):
+        assert np.all(a[hit] == np.float32(va))
+    fs = f[:, hit].astype(np.float64)
+    np.testing.assert_allclose(fs.sum(0), rho[hit], rtol=2e-6)
+    for a, va in zip(E, v):
+        np.testing.assert_allclose((a[:, None] * fs).sum(0), rho[hit] * va, rtol=2e-5, atol=1e-8)
+    np.testing.assert_array_equal(f[:, ~hit], f0[:, ~hit])
+
+
+def test_droplet_impact_pressure_closed_form():
+    """Fz += -p/(rho dx) dt^2/dx exp(-r^2 / (2 r_d^2)) on metal or wetted cells within 2 r_d."""
+    n = (14, 14, 10)
+    flags = np.full(n, FLUID, np.uint8)
+    flags[:, :, 8:] = GAS
+    phi = np.ones(n, np.float32)
+    f_l = np.ones(n, np.float32)
+    f_l[3, 7, 4], phi[3, 7, 4] = 0.1, 0.1
+    Fz = np.zeros(n, np.float32)
+    p, dt, dx, rho_ref, dr = 800.0, 2e-5, 2e-4, 7800.0, 2.0
+    lib().or_apply_droplet_impact_pressure(ptr(Fz), ptr(flags), ptr(phi), ptr(f_l), 7.0, 7.0, 4.0,
+                                           dr, p, dt, dx, rho_ref, *n)
+    i, j, k = np.meshgrid(*(np.arange(m) for m in n), indexing="ij")
+    r2 = (i - 7.0) ** 2 + (j - 7.0) ** 2 + (k - 4.0) ** 2
+    expect = -p / (rho_ref * dx) * dt * dt / dx * np.exp(-r2 / (2 * dr * dr))
+    on = (r2 <= 4 * dr * dr) & (flags != GAS)
+    on[3, 7, 4] = False
+    np.testing.assert_allclose(Fz[on], expect[on], rtol=2e-6)
+    assert np.all(Fz[~on] == 0.0)
+
+
+# ---------------------------------------------------------------------------- tracers
+def test_tracer_ring_buffer_and_spawn_box():
+    """Slots wrap modulo max_tracers (the oldest are evicted); spawns sit in the +/- sigma box
+    around (x, y) at height z."""
+    pos = np.zeros((5, 3), np.float32)
+    active = np.zeros(5, np.int32)
+    head = np.zeros(1, np.int32)
+    lib().or_inject_tracers(ptr(pos), ptr(active), ptr(head), 5, 1e-3, 2e-3, 3e-4, 2e-4, 7, 11)
+    assert head[0] == 7 and np.all(active == 1)
+    assert np.all(np.abs(pos[:, 0] - np.float32(1e-3)) <= 2e-4 * (1 + 1e-6))
+    assert np.all(np.abs(pos[:, 1] - np.float32(2e-3)) <= 2e-4 * (1 + 1e-6))
+    assert np.all(pos[:, 2] == np.float32(3e-4))
+    assert len(set(map(tuple, pos[:, :2].tolist()))) == 5
+
+
+def test_tracer_advection_states():
+    """A tracer in melt moves u dx per step (u in cells/step); entering GAS or leaving the domain
+    deactivates it (0); SOLID or f_l < 0.05 traps it for good (2)."""
+    n = (6, 5, 4)
+    dx = 1e-4
+    flags = np.full(n, FLUID, np.uint8)
+    flags[0, 0, 0] = GAS
+    flags[1, 0, 0] = SOLID
+    f_l = np.ones(n, np.float32)
+    f_l[2, 0, 0] = 0.01
+    ux = np.full(n, 0.25, np.float32)
+    uy = np.full(n, -0.125, np.float32)
+    uz = np.full(n, 0.5, np.float32)
+    cells = [(0, 0, 0), (1, 0, 0), (2, 0, 0), (3, 2, 1), (9, 0, 0)]
+    pos = np.array([[(i + 0.5) * dx, (j + 0.5) * dx, (k + 0.5) * dx] for i, j, k in cells],
+                   np.float32)
+    active = np.array([1, 1, 1, 1, 1], np.int32)
+    p0 = pos.copy()
+    lib().or_advect_tracers(ptr(pos), ptr(active), ptr(ux), ptr(uy), ptr(uz), ptr(f_l),
+                            ptr(flags), dx, 2e-5, 5, *n)
+    assert list(active) == [0, 2, 2, 1, 0]
+    np.testing.assert_array_equal(pos[[0, 1, 2, 4]], p0[[0, 1, 2, 4]])
+    moved = p0[3] + np.array([0.25, -0.125, 0.5], np.float32) * np.float32(dx)
+    np.testing.assert_array_equal(pos[3], moved)
+
+
+def test_trailing_pool_rules_at_their_thresholds():
+    """solidify_trailing_pool_scalar (kernels.py:2574-2617): a non-gas cell with phi >= 0.85,
+    f_l >= 0.05, projection on the torch direction <= -lookback and T < T_freeze freezes to
+    T_s - 1 (H = cp_rho (T_s - 1)), SOLID, phi 1, f_l 0, u 0; each threshold is inclusive on the
+    freezing side and exclusive on the other."""
+    Ts, Tf, cp = 1723.0, 1800.0, 5.3e6
+    # cells along x at k = 0; arc at i = 10 moving +x, lookback 3 -> i <= 7 is behind
+    rows = [  # (i, flag, phi, f_l, T, freezes)
+        (7, FLUID, 0.85, 0.05, 1799.0, True),     # every threshold at its boundary
+        (8, FLUID, 1.0, 1.0, 1700.0, False),      # proj = -2 > -3
+        (6, FLUID, 0.84, 1.0, 1700.0, False),     # phi below 0.85
+        (5, FLUID, 1.0, 0.049, 1700.0, False),    # f_l below 0.05
+        (4, FLUID, 1.0, 1.0, 1800.0, False),      # T == T_freeze
+        (3, GAS, 1.0, 1.0, 1700.0, False),        # gas never freezes
+        (2, IFACE, 0.9, 0.5, 1000.0, True),
+        (1, SOLID, 1.0, 0.5, 1000.0, True),       # any non-gas flag
+    ]
+    n = (12, 1, 1)
+    flags = np.full(n, GAS, np.uint8)
+    phi, f_l, T = (np.zeros(n, np.float32) for _ in range(3))
+    for i, fg, p, fl, t, _ in rows:
+        flags[i, 0, 0], phi[i, 0, 0], f_l[i, 0, 0], T[i, 0, 0] = fg, p, fl, t
+    H = np.full(n, 7.0, np.float32)
+    u = [np.full(n, 0.01, np.float32) for _ in range(3)]
+    lib().or_solidify_trailing_pool_scalar(ptr(T), ptr(H), ptr(f_l), ptr(phi), ptr(flags),
+                                           *(ptr(a) for a in u), cp, 10.0, 0.0, 0.0, 1.0, 0.0,
+                                           0.0, 3.0, Tf, Ts, *n)
+    for i, fg, p, fl, t, freezes in rows:
+        c = (i, 0, 0)
+        if freezes:
+            assert flags[c] == SOLID and phi[c] == 1.0 and f_l[c] == 0.0
+            assert T[c] == np.float32(Ts - 1.0)
+            assert H[c] == np.float32(cp) * (np.float32(Ts) - np.float32(1.0))
+            assert u[0][c] == 0.0 and u[2][c] == 0.0
+        else:
+            assert flags[c] == fg and T[c] == np.float32(t) and H[c] == 7.0
