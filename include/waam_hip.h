@@ -327,6 +327,10 @@ typedef struct waam_step_params {
     int32_t enable_vof, enable_wetting, enable_csf, enable_heat_loss, enable_conv, enable_rad;
     int32_t enable_cap, enable_hydro, enable_freeze, enable_trailing, zero_velocity;
     int32_t surface_weight, nz_solid, col_i, col_j, use_arc_k_override, use_k_robot;
+    /* 1: clear the forces of every cell the collision skips; 0: only of those the previous
+     * fused step collided (sflags_prev FLUID/IFACE), valid when every other cell's forces are
+     * already zero (the step before was a fused step that did not take forces from the fields) */
+    int32_t force_reset_all;
     /* arc */
     float arc_i, arc_j, arc_k_override, k_robot;
     float Q_w, eta, dt, dx, dx3, sigma, pen;

@@ -94,7 +94,10 @@ def test_job_drop_in_parity(hip_device, tmp_path, engine):
     job["torch_path"] = [{"x_mm": 4.0, "y_mm": 5.0}, {"x_mm": 12.0, "y_mm": 5.0}]
     p = tmp_path / "job.yaml"
     p.write_text(yaml.safe_dump(job))
-    tw = WAAMTwin.from_job(p, droplet_freq_hz=2500.0, max_tracers=4000)
+    tw = WAAMTwin.from_job(p)
+    # the job's droplet length sets ~36 Hz; detach every ~16 steps instead so that 60 steps
+    # deposit
+    tw.droplet_freq = 2500.0
     tw.engine = engine
     tw.reset()
     ot = oracle_like(tw)

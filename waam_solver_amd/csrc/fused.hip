@@ -2,7 +2,9 @@
 //
 // Launch sequence for one step (reference solvers/coupled_step.py:73-464):
 //   1 k_step_begin           torch-column surface height (arc_k); arc weight partial sums
-//     (+ k_step_arc_total    their fixed-order total and the energy per unit weight, one block)
+//     (+ k_step_arc_total    their fixed-order total and the energy per unit weight, one block;
+//                            a kernel boundary orders the partials device-wide more cheaply than
+//                            a fence in every block, which writes back each XCD's L2)
 //   2 k_step_thermal         every cell but FLUID/IFACE: arc + enthalpy ceiling applied on the
 //                            fly, conduction, boundary losses, floor, ceiling, phase map, T_max,
 //                            cooling rate, donor-cell phi advection + re-initialisation; lists
@@ -20,7 +22,7 @@
 //
 // Work split. A wavefront covers 64 cells of one (i, j) column, which on a plate crosses
 // SOLID, the free surface and GAS; the surface cells carry nearly all the arithmetic (enthalpy
-// advection, CSF, collision). The sweeps over every cell (2, 4, 6) do the cheap per-class work
+// advection, CSF, collision). The sweeps over every cell (2, 4, 7) do the cheap per-class work
 // and append the surface cells to a compacted index list; kernels 3 and 5 run the expensive
 // path on full wavefronts from that list (order-free: each cell's result depends only on the
 // kernel's inputs). Separate kernels also keep the sweeps' register count, and so their
@@ -33,7 +35,7 @@
 //
 // Deferred stream. The reference streams populations after each collide (f_dst -> f_src) and
 // then swaps, so the next collide overwrites the streamed value on every FLUID/IFACE cell. Here
-// the stream of step n is folded into kernel 4 of step n+1: a cell that step n+1 collides
+// the stream of step n is folded into kernel 7 of step n+1: a cell that step n+1 collides
 // skips the pull (its value would be overwritten), every other cell that was not SOLID at step
 // n pulls from its neighbours with step n's flags (kept in sflags). Populations are read once
 // and written once per active cell per step instead of twice.
@@ -41,6 +43,8 @@
 #include <climits>
 #include <cmath>
 #include <vector>
+
+#include <hip/hip_ext.h>
 
 #include "arc_cell.h"
 #include "forces_cell.h"
@@ -702,7 +706,10 @@ __global__ void __launch_bounds__(256) k_step_stream(waam_step_fields f, StepCon
     bool need = !surface && p.pending_stream && fp != SOLID;
     if (__ballot(need) != 0ull) pull_rows(f.f_src, f.f_dst, f.sflags_prev, i, j, k, need, fp, d);
     f.sflags_next[c] = fc;
-    if (!surface && p.store_forces && !p.forces_from_fields) {
+    /* forces are non-zero only on the cells the previous step collided, unless the step before
+     * took them from the fields (then force_reset_all) */
+    if (!surface && p.store_forces && !p.forces_from_fields &&
+        (p.force_reset_all || is_surface_class(fp))) {
         f.Fx[c] = 0.0f;
         f.Fy[c] = 0.0f;
         f.Fz[c] = 0.0f;
@@ -788,17 +795,15 @@ struct KernelTimer {
 };
 static KernelTimer g_timer;
 
+/* launch(start, stop) launches the kernel(s) of one kind with hipExtLaunchKernelGGL, which
+ * records the two events with the dispatch itself (null events when not timing) */
 template <class Launch>
-static void timed_launch(int which, hipStream_t st, Launch &&launch)
+static void timed_launch(int which, Launch &&launch)
 {
     KernelTimer &t = g_timer;
     bool rec = t.armed && t.which == which && 2 * t.used < (int)t.ev.size();
-    if (rec) hipEventRecord(t.ev[2 * t.used], st);
-    launch();
-    if (rec) {
-        hipEventRecord(t.ev[2 * t.used + 1], st);
-        ++t.used;
-    }
+    launch(rec ? t.ev[2 * t.used] : nullptr, rec ? t.ev[2 * t.used + 1] : nullptr);
+    if (rec) ++t.used;
 }
 
 }  // namespace waam
@@ -865,29 +870,37 @@ int waam_step_fused(const waam_step_params *pp, const waam_step_fields *ff, void
 
     hipStream_t st = as_stream(stream);
     int do_sums = p.is_welding && !p.arc_applied ? 1 : 0;
-    timed_launch(WAAM_K_BEGIN, st, [&] {
-        hipLaunchKernelGGL(k_step_begin, dim3(do_sums ? kReduceBlocks : 1), dim3(kBlock), 0, st, f,
-                           s, do_sums);
-        if (do_sums) hipLaunchKernelGGL(k_step_arc_total, dim3(1), dim3(kBlock), 0, st, f, s);
+    timed_launch(WAAM_K_BEGIN, [&](hipEvent_t a, hipEvent_t b) {
+        hipExtLaunchKernelGGL(k_step_begin, dim3(do_sums ? kReduceBlocks : 1), dim3(kBlock), 0,
+                              st, a, do_sums ? nullptr : b, 0, f, s, do_sums);
+        if (do_sums)
+            hipExtLaunchKernelGGL(k_step_arc_total, dim3(1), dim3(kBlock), 0, st, nullptr, b, 0, f,
+                                  s);
     });
-    timed_launch(WAAM_K_THERMAL, st,
-                 [&] { hipLaunchKernelGGL(k_step_thermal, g3, block3(), 0, st, f, s); });
-    timed_launch(WAAM_K_THERMAL_SURFACE, st, [&] {
-        hipLaunchKernelGGL(k_step_thermal_surface, dim3(kListBlocks), dim3(kBlock), 0, st, f, s);
+    timed_launch(WAAM_K_THERMAL, [&](hipEvent_t a, hipEvent_t b) {
+        hipExtLaunchKernelGGL(k_step_thermal, g3, block3(), 0, st, a, b, 0, f, s);
     });
-    timed_launch(WAAM_K_VOF_FLAGS, st,
-                 [&] { hipLaunchKernelGGL(k_step_vof_flags, g3, block3(), 0, st, f, s); });
-    timed_launch(WAAM_K_COLLIDE, st, [&] {
-        hipLaunchKernelGGL(k_step_collide, dim3(kListBlocks), dim3(kBlock), 0, st, f, s);
+    timed_launch(WAAM_K_THERMAL_SURFACE, [&](hipEvent_t a, hipEvent_t b) {
+        hipExtLaunchKernelGGL(k_step_thermal_surface, dim3(kListBlocks), dim3(kBlock), 0, st, a, b,
+                              0, f, s);
+    });
+    timed_launch(WAAM_K_VOF_FLAGS, [&](hipEvent_t a, hipEvent_t b) {
+        hipExtLaunchKernelGGL(k_step_vof_flags, g3, block3(), 0, st, a, b, 0, f, s);
+    });
+    timed_launch(WAAM_K_COLLIDE, [&](hipEvent_t a, hipEvent_t b) {
+        hipExtLaunchKernelGGL(k_step_collide, dim3(kListBlocks), dim3(kBlock), 0, st, a, b, 0, f,
+                              s);
     });
     if (f.max_tracers > 0)
-        timed_launch(WAAM_K_TRACERS, st, [&] {
-            hipLaunchKernelGGL(k_advect_tracers_step, dim3(grid_for((size_t)f.max_tracers)),
-                               dim3(kBlock), 0, st, f.tracer_pos, f.tracer_active, f.ux, f.uy,
-                               f.uz, f.fl_out, f.flags_out, p.dx, f.max_tracers, d);
+        timed_launch(WAAM_K_TRACERS, [&](hipEvent_t a, hipEvent_t b) {
+            hipExtLaunchKernelGGL(k_advect_tracers_step, dim3(grid_for((size_t)f.max_tracers)),
+                                  dim3(kBlock), 0, st, a, b, 0, f.tracer_pos, f.tracer_active,
+                                  f.ux, f.uy, f.uz, (const float *)f.fl_out,
+                                  (const uint8_t *)f.flags_out, p.dx, f.max_tracers, d);
         });
-    timed_launch(WAAM_K_STREAM, st,
-                 [&] { hipLaunchKernelGGL(k_step_stream, g3, block3(), 0, st, f, s); });
+    timed_launch(WAAM_K_STREAM, [&](hipEvent_t a, hipEvent_t b) {
+        hipExtLaunchKernelGGL(k_step_stream, g3, block3(), 0, st, a, b, 0, f, s);
+    });
     return check_launch("waam_step_fused");
 }
 

@@ -39,7 +39,7 @@ _INT_PARAMS = ("is_welding", "collision", "arc_kind", "arc_applied", "pending_st
                "forces_from_fields", "store_forces", "enable_vof", "enable_wetting", "enable_csf",
                "enable_heat_loss", "enable_conv", "enable_rad", "enable_cap", "enable_hydro",
                "enable_freeze", "enable_trailing", "zero_velocity", "surface_weight", "nz_solid",
-               "col_i", "col_j", "use_arc_k_override", "use_k_robot")
+               "col_i", "col_j", "use_arc_k_override", "use_k_robot", "force_reset_all")
 _FLOAT_PARAMS = ("arc_i", "arc_j", "arc_k_override", "k_robot", "Q_w", "eta", "dt", "dx", "dx3",
                  "sigma", "pen", "g_sgn", "g_ff", "g_fr", "g_af", "g_ar", "g_b", "g_c",
                  "alpha_lu", "cp_rho", "L_rho", "T_s", "T_l", "T_cap", "T_amb", "h_conv",
@@ -76,6 +76,9 @@ class FusedStepper:
             g.list_thermal = DeviceField.zeros((g.n_cells,), torch.int32, g.device)
             g.list_collide = DeviceField.zeros((g.n_cells,), torch.int32, g.device)
         self.pending = False
+        # the last fused step took its forces from the fields (an eager-arc step): the next one
+        # clears every cell's forces, not only those of the previously collided cells
+        self._forces_from_fields = False
         self._arc_out = (ctypes.c_float * 3)()
         # the swapped pairs alternate between two pointer sets: keep one struct per set
         self._fields_cache: dict[tuple, StepFields] = {}
@@ -214,7 +217,11 @@ class FusedStepper:
             tw._last_arc_ijk = (float(arc_i), float(arc_j), float(arc_k))
         else:
             p.pending_stream = 1 if self.pending else 0
+            # without a pending stream the previous step may not have been a fused one (reset,
+            # sync, the other engine): clear every skipped cell's forces
+            p.force_reset_all = 1 if (not self.pending or self._forces_from_fields) else 0
             tw._last_arc_ijk = (float(arc_i), float(arc_j), None)
+        self._forces_from_fields = bool(p.forces_from_fields)
         fields = self._fields()
         lib().call("waam_step_fused", ctypes.addressof(p), ctypes.addressof(fields),
                    torch.cuda.current_stream().cuda_stream)
