@@ -331,6 +331,11 @@ typedef struct waam_step_params {
      * fused step collided (sflags_prev FLUID/IFACE), valid when every other cell's forces are
      * already zero (the step before was a fused step that did not take forces from the fields) */
     int32_t force_reset_all;
+    /* 1: the thermal list already holds this step's FLUID/IFACE cells, appended by the previous
+     * waam_step_fused call's last kernel (valid when nothing changed flags_in since that call);
+     * the surface cells' thermal update then runs in the same launch as the sweep. 0: the sweep
+     * builds the list and a second launch consumes it. */
+    int32_t thermal_list_ready;
     /* arc */
     float arc_i, arc_j, arc_k_override, k_robot;
     float Q_w, eta, dt, dx, dx3, sigma, pen;

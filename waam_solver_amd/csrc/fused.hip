@@ -56,6 +56,9 @@ namespace waam {
 
 /* blocks of the list kernels 3 and 5 (each grid-strides over its list) */
 constexpr int kListBlocks = 1024;
+/* list blocks sharing a launch with the thermal sweep: about one per CU, so the sweep keeps
+ * most of the slots (measured: 1024 or 512 list blocks crowd it, 64 leave the list long) */
+constexpr int kListBlocksWithSweep = 256;
 static_assert(kBlock == kListSegments, "the list prefix scan uses one thread per segment");
 
 /* A cell list is split into kListSegments contiguous cell ranges of seg_len cells (a multiple
@@ -172,7 +175,7 @@ __device__ __forceinline__ void cell_ijk(size_t c, const Dims &d, int &i, int &j
 
 /* 1 — surface height in the torch column (reference kernels.py:625-641 and
  *     coupled_step.py:34-59) and per-block compensated arc weight sums (kernels.py:443-496);
- *     empties both cell lists for kernels 2 and 4. */
+ *     empties the collision list, and the thermal list unless it was built ahead. */
 __global__ void __launch_bounds__(kBlock) k_step_begin(waam_step_fields f, StepConsts s,
                                                        int do_sums)
 {
@@ -182,7 +185,7 @@ __global__ void __launch_bounds__(kBlock) k_step_begin(waam_step_fields f, StepC
     const Dims &d = s.d;
     const waam_step_params &p = s.p;
     if (blockIdx.x == 0) {
-        cell_list(f, s.seg_len, 0).count(threadIdx.x) = 0;
+        if (!p.thermal_list_ready) cell_list(f, s.seg_len, 0).count(threadIdx.x) = 0;
         cell_list(f, s.seg_len, 1).count(threadIdx.x) = 0;
     }
     int best = -1;
@@ -275,38 +278,41 @@ __global__ void __launch_bounds__(kBlock) k_step_arc_total(waam_step_fields f, S
 }
 
 /* Enthalpy of a non-GAS cell as the thermal kernel reads it: after this step's arc injection
- * (reference kernels.py:497-510) and enthalpy ceiling (coupled_step.py:108). */
+ * (reference kernels.py:497-510) and enthalpy ceiling (coupled_step.py:108). Holds copies, not
+ * pointers into the kernel arguments: the address of an argument forces the compiler to copy
+ * the whole argument struct to private memory in some kernels. */
 struct HView {
-    const waam_step_fields *f;
-    const StepConsts *s;
+    const float *H_in, *fl_in;
+    const uint8_t *flags_in;
+    Dims d;
     ArcParams ap;
-    float epw;
+    float epw, H_liq, cap_l, cap_s;
+    int box_i0, box_i1, box_j0, box_j1;
     bool arc_on, cap_on;
 
     __device__ __forceinline__ float value(float h, float fl, int i, int j, int k) const
     {
-        if (arc_on && i >= s->box_i0 && i <= s->box_i1 && j >= s->box_j0 && j <= s->box_j1) {
+        if (arc_on && i >= box_i0 && i <= box_i1 && j >= box_j0 && j <= box_j1) {
             float wg, wdep;
             if (arc_cell_weights(ap, i, j, k, fl, wg, wdep)) h += epw * wg * wdep;
         }
-        if (cap_on) h = ceiling_clamp(h, s->pc.H_liq, s->cap_l, s->cap_s);
+        if (cap_on) h = ceiling_clamp(h, H_liq, cap_l, cap_s);
         return h;
     }
     __device__ __forceinline__ float at(size_t c, int i, int j, int k) const
     {
-        return value(f->H_in[c], f->fl_in[c], i, j, k);
+        return value(H_in[c], fl_in[c], i, j, k);
     }
     /* _masked_at on the enthalpy view: clamped index, GAS neighbours give the fallback (loads
      * first, select last, so the loads are not sunk behind the flag test) */
     __device__ __forceinline__ float masked(int i, int j, int k, float fallback) const
     {
-        const Dims &d = s->d;
         int ii = imax(0, imin(i, d.nx - 1));
         int jj = imax(0, imin(j, d.ny - 1));
         int kk = imax(0, imin(k, d.nz - 1));
         size_t c = d.idx(ii, jj, kk);
-        uint8_t fg = f->flags_in[c];
-        float v = value(f->H_in[c], f->fl_in[c], ii, jj, kk);
+        uint8_t fg = flags_in[c];
+        float v = value(H_in[c], fl_in[c], ii, jj, kk);
         return fg == GAS ? fallback : v;
     }
 };
@@ -315,11 +321,20 @@ __device__ __forceinline__ HView make_hview(const waam_step_fields &f, const Ste
 {
     const float *tot = reinterpret_cast<const float *>((char *)f.scratch + kScratchTelemetry);
     HView hv;
-    hv.f = &f;
-    hv.s = &s;
+    hv.H_in = f.H_in;
+    hv.fl_in = f.fl_in;
+    hv.flags_in = f.flags_in;
+    hv.d = s.d;
     hv.ap = s.arc;
     hv.ap.arc_k = step_arc_k(s.p, tot);
     hv.epw = tot[1];
+    hv.H_liq = s.pc.H_liq;
+    hv.cap_l = s.cap_l;
+    hv.cap_s = s.cap_s;
+    hv.box_i0 = s.box_i0;
+    hv.box_i1 = s.box_i1;
+    hv.box_j0 = s.box_j0;
+    hv.box_j1 = s.box_j1;
     hv.arc_on = s.p.is_welding && !s.p.arc_applied && hv.epw > 0.0f;
     hv.cap_on = s.p.is_welding && !s.p.arc_applied && s.p.enable_cap;
     return hv;
@@ -423,19 +438,66 @@ __global__ void __launch_bounds__(256) k_step_thermal(waam_step_fields f, StepCo
     thermal_vof_cell(f, s, i, j, k, c, flag);
 }
 
-/* 3 — thermal chain (with enthalpy advection) and VOF advection of the listed FLUID/IFACE
- *     cells */
-__global__ void __launch_bounds__(256) k_step_thermal_surface(waam_step_fields f, StepConsts s)
+/* thermal chain (with enthalpy advection) and VOF advection of the listed FLUID/IFACE cells;
+ * worker b of nb, 256 threads each */
+__device__ __forceinline__ void thermal_surface_list(const waam_step_fields &f,
+                                                     const StepConsts &s, int b, int nb)
 {
     __shared__ int s_pref[kListSegments + 1];
     const Dims &d = s.d;
     CellList l = cell_list(f, s.seg_len, 0);
     int n = list_prefix(l, d.n(), s_pref);
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    for (int t = b * kBlock + tid; t < n; t += nb * kBlock) {
         size_t c = list_at(l, s_pref, t);
         int i, j, k;
         cell_ijk(c, d, i, j, k);
         thermal_vof_cell(f, s, i, j, k, c, f.flags_in[c]);
+    }
+}
+
+/* 3 — the listed FLUID/IFACE cells (list built by kernel 2) */
+__global__ void __launch_bounds__(256) k_step_thermal_surface(waam_step_fields f, StepConsts s)
+{
+    thermal_surface_list(f, s, blockIdx.x, gridDim.x);
+}
+
+/* 2+3 in one launch, when the previous step's kernel 7 has listed this step's FLUID/IFACE
+ * cells: the first lz grid slices work through the list (dispatched first, so the
+ * latency-bound surface cells overlap the bandwidth-bound sweep), the rest sweep every other
+ * cell as kernel 2 does. One call site of the cell routine (two inlined copies make the
+ * compiler copy the argument structs to private memory). */
+__global__ void __launch_bounds__(256) k_step_thermal_all(waam_step_fields f, StepConsts s,
+                                                          int lz)
+{
+    __shared__ int s_pref[kListSegments + 1];
+    const Dims &d = s.d;
+    const bool list = (int)blockIdx.z < lz;
+    CellList l = cell_list(f, s.seg_len, 0);
+    int n = 0, t = 0, stride = 0;
+    if (list) {
+        n = list_prefix(l, d.n(), s_pref);
+        t = (((int)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x) * kBlock +
+            threadIdx.y * blockDim.x + threadIdx.x;
+        stride = lz * gridDim.y * gridDim.x * kBlock;
+    }
+    for (bool first = true;; first = false) {
+        int i, j, k;
+        size_t c;
+        uint8_t flag;
+        if (list) {
+            if (t >= n) return;
+            c = list_at(l, s_pref, t);
+            t += stride;
+            cell_ijk(c, d, i, j, k);
+            flag = f.flags_in[c];
+        } else {
+            if (!first || !d.cell(i, j, k, lz)) return;
+            c = d.idx(i, j, k);
+            flag = f.flags_in[c];
+            if (is_surface_class(flag)) return;
+        }
+        thermal_vof_cell(f, s, i, j, k, c, flag);
     }
 }
 
@@ -500,7 +562,9 @@ __global__ void __launch_bounds__(256) k_step_vof_flags(waam_step_fields f, Step
  * direction) and takes the value of a direction with a fast-axis component from the adjacent
  * lane; the lanes at the ends of the wavefront and at the domain's wrap (x) or clamp (z) load
  * directly. Every active lane of the wavefront must call it (it shuffles); `need` says whether
- * this lane stores. Values are exactly those of a cell-by-cell pull. */
+ * this lane stores. Values are exactly those of a cell-by-cell pull. Neighbour indices are
+ * recomputed for the bounce-back test rather than kept (19 64-bit indices held across the
+ * loads cost 38 registers and a wavefront per SIMD). */
 __device__ __forceinline__ void pull_rows(const float *__restrict__ src, float *__restrict__ dst,
                                           const uint8_t *__restrict__ sfl, int i, int j, int k,
                                           bool need, uint8_t fc, const Dims &d)
@@ -511,7 +575,6 @@ __device__ __forceinline__ void pull_rows(const float *__restrict__ src, float *
     const int lane = (int)threadIdx.x;  // block3(): one wavefront per threadIdx.y row
     const int a = zyx ? i : k;          // this lane's fast-axis coordinate
     const int nf = d.fast();
-    size_t from[Q];
     float v[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
@@ -519,19 +582,23 @@ __device__ __forceinline__ void pull_rows(const float *__restrict__ src, float *
         ni = ni < 0 ? ni + d.nx : (ni >= d.nx ? ni - d.nx : ni);
         nj = imax(0, imin(nj, d.ny - 1));
         nk = imax(0, imin(nk, d.nz - 1));
-        from[q] = (size_t)q * N + d.idx(ni, nj, nk);
         const int sh = zyx ? ex(q) : ez(q);  // fast-axis offset of the source
         float al = src[(size_t)q * N + (zyx ? d.idx(i, nj, nk) : d.idx(ni, nj, k))];
         float nb = sh == 0 ? al : __shfl(al, lane - sh);
         bool direct = sh != 0 && (lane - sh < 0 || lane - sh > 63 || a - sh < 0 || a - sh >= nf);
-        v[q] = direct ? src[from[q]] : nb;
+        v[q] = direct ? src[(size_t)q * N + d.idx(ni, nj, nk)] : nb;
     }
     if (!need) return;
     if (fc != GAS) {
         /* bounce-back off SOLID neighbours (a GAS destination pulls unconditionally) */
 #pragma unroll
-        for (int q = 0; q < Q; ++q)
-            if (sfl[from[q] - (size_t)q * N] == SOLID) v[q] = src[(size_t)opp(q) * N + c];
+        for (int q = 0; q < Q; ++q) {
+            int ni = i - ex(q), nj = j - ey(q), nk = k - ez(q);
+            ni = ni < 0 ? ni + d.nx : (ni >= d.nx ? ni - d.nx : ni);
+            nj = imax(0, imin(nj, d.ny - 1));
+            nk = imax(0, imin(nk, d.nz - 1));
+            if (sfl[d.idx(ni, nj, nk)] == SOLID) v[q] = src[(size_t)opp(q) * N + c];
+        }
     }
 #pragma unroll
     for (int q = 0; q < Q; ++q) dst[(size_t)q * N + c] = v[q];
@@ -632,9 +699,9 @@ __global__ void __launch_bounds__(256) k_step_collide(waam_step_fields f, StepCo
 /* remelt_hot_solid_scalar, solidify_trailing_pool_scalar, solidify_cooled_metal (reference
  * kernels.py:2489-2520, 2573-2617, 2422-2450) in that order for one non-GAS cell, on loaded
  * values; a field is stored exactly when one of the three writes it, with its last value */
-__device__ __forceinline__ void freeze_cell(const waam_step_fields &f, const StepConsts &s,
-                                            int i, int j, int k, size_t c, uint8_t flag, float T,
-                                            float phi, float fl, float arc_k)
+__device__ __forceinline__ uint8_t freeze_cell(const waam_step_fields &f, const StepConsts &s,
+                                               int i, int j, int k, size_t c, uint8_t flag,
+                                               float T, float phi, float fl, float arc_k)
 {
     const waam_step_params &p = s.p;
     bool wH = false, wT = false, wfl = false, wphi = false, wflag = false, wu = false;
@@ -681,11 +748,13 @@ __device__ __forceinline__ void freeze_cell(const waam_step_fields &f, const Ste
         f.uy[c] = 0.0f;
         f.uz[c] = 0.0f;
     }
+    return flag;
 }
 
 /* 7 — every cell: stream flags; cells the collision skipped: force reset and the previous
  *     step's pull; then the freeze/remelt updates of every non-GAS cell (cell-local; the pull
- *     reads only populations and stream flags, which they do not touch) */
+ *     reads only populations and stream flags, which they do not touch); lists the cells that
+ *     end the step FLUID/IFACE, the next step's thermal list */
 __global__ void __launch_bounds__(256) k_step_stream(waam_step_fields f, StepConsts s)
 {
     const Dims &d = s.d;
@@ -714,10 +783,12 @@ __global__ void __launch_bounds__(256) k_step_stream(waam_step_fields f, StepCon
         f.Fy[c] = 0.0f;
         f.Fz[c] = 0.0f;
     }
+    uint8_t fin = fc;
     if (freeze) {
         const float *tot = reinterpret_cast<const float *>((char *)f.scratch + kScratchTelemetry);
-        freeze_cell(f, s, i, j, k, c, fc, T, phi, fl, step_arc_k(p, tot));
+        fin = freeze_cell(f, s, i, j, k, c, fc, T, phi, fl, step_arc_k(p, tot));
     }
+    list_append(is_surface_class(fin), c, cell_list(f, s.seg_len, 0), d.n());
 }
 
 /* standalone pending stream */
@@ -877,13 +948,23 @@ int waam_step_fused(const waam_step_params *pp, const waam_step_fields *ff, void
             hipExtLaunchKernelGGL(k_step_arc_total, dim3(1), dim3(kBlock), 0, st, nullptr, b, 0, f,
                                   s);
     });
-    timed_launch(WAAM_K_THERMAL, [&](hipEvent_t a, hipEvent_t b) {
-        hipExtLaunchKernelGGL(k_step_thermal, g3, block3(), 0, st, a, b, 0, f, s);
-    });
-    timed_launch(WAAM_K_THERMAL_SURFACE, [&](hipEvent_t a, hipEvent_t b) {
-        hipExtLaunchKernelGGL(k_step_thermal_surface, dim3(kListBlocks), dim3(kBlock), 0, st, a, b,
-                              0, f, s);
-    });
+    if (p.thermal_list_ready) {
+        /* enough list slices for about kListBlocks workers */
+        int per = (int)(g3.x * g3.y);
+        int lz = std::max(1, (kListBlocksWithSweep + per - 1) / per);
+        dim3 ga(g3.x, g3.y, g3.z + (unsigned)lz);
+        timed_launch(WAAM_K_THERMAL, [&](hipEvent_t a, hipEvent_t b) {
+            hipExtLaunchKernelGGL(k_step_thermal_all, ga, block3(), 0, st, a, b, 0, f, s, lz);
+        });
+    } else {
+        timed_launch(WAAM_K_THERMAL, [&](hipEvent_t a, hipEvent_t b) {
+            hipExtLaunchKernelGGL(k_step_thermal, g3, block3(), 0, st, a, b, 0, f, s);
+        });
+        timed_launch(WAAM_K_THERMAL_SURFACE, [&](hipEvent_t a, hipEvent_t b) {
+            hipExtLaunchKernelGGL(k_step_thermal_surface, dim3(kListBlocks), dim3(kBlock), 0, st, a,
+                                  b, 0, f, s);
+        });
+    }
     timed_launch(WAAM_K_VOF_FLAGS, [&](hipEvent_t a, hipEvent_t b) {
         hipExtLaunchKernelGGL(k_step_vof_flags, g3, block3(), 0, st, a, b, 0, f, s);
     });

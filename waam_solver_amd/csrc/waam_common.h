@@ -93,11 +93,11 @@ struct Dims {
 #ifdef __HIPCC__
     /* cell of this thread in a grid3() launch (x: contiguous axis, y: j, z: slowest axis);
      * false for the threads past the end of a row */
-    __device__ __forceinline__ bool cell(int &i, int &j, int &k) const
+    __device__ __forceinline__ bool cell(int &i, int &j, int &k, int z0 = 0) const
     {
         int a = blockIdx.x * blockDim.x + threadIdx.x;
         j = blockIdx.y * blockDim.y + threadIdx.y;
-        int b = blockIdx.z;
+        int b = (int)blockIdx.z - z0;  // z0: grid slices ahead of the cells (other work)
         i = layout == kLayoutZYX ? a : b;
         k = layout == kLayoutZYX ? b : a;
         return a < fast() && j < ny;

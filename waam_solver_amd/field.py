@@ -20,6 +20,18 @@ import torch
 
 LAYOUT_XYZ, LAYOUT_ZYX = 0, 1
 
+# Count of host-initiated field writes (DeviceField setters and per-kernel library calls). The
+# fused step keeps work lists across steps only while this count is unchanged.
+_WRITES = [0]
+
+
+def write_epoch() -> int:
+    return _WRITES[0]
+
+
+def note_write() -> None:
+    _WRITES[0] += 1
+
 
 def _spatial_perm(nd: int) -> tuple[int, ...]:
     """Axis permutation between logical (..., x, y, z) and stored (..., z, y, x) order (its own
@@ -95,9 +107,12 @@ class DeviceField:
             a = a.transpose(_spatial_perm(a.ndim))
         t = torch.from_numpy(np.ascontiguousarray(a)).to(self.tensor.dtype)
         self.tensor.copy_(t)
+        note_write()
 
     def to_torch(self) -> torch.Tensor:
-        """The stored tensor (for ZYX fields its axes are (..., z, y, x))."""
+        """The stored tensor (for ZYX fields its axes are (..., z, y, x)). Writing through it is
+        not seen by the fused step's cross-step bookkeeping: call ``WAAMTwin.sync()`` first."""
+        note_write()
         return self.tensor
 
     # ---------------------------------------------------------------- element access
@@ -132,6 +147,7 @@ class DeviceField:
         return v
 
     def __setitem__(self, key, value) -> None:
+        note_write()
         v, order = self._view(key)
         if order is not None:
             v = v.permute(order)
@@ -142,11 +158,13 @@ class DeviceField:
                                     else value, device=v.device).to(v.dtype))
 
     def fill(self, value) -> None:
+        note_write()
         self.tensor.fill_(value)
 
     def copy_from(self, other: "DeviceField") -> None:
         if other.layout != self.layout and self.tensor.dim() >= 3:
             raise ValueError("copy_from between fields of different layouts")
+        note_write()
         self.tensor.copy_(other.tensor)
 
 

@@ -18,7 +18,7 @@ import math
 import torch
 
 from ._lib import FLAG_FLUID, FLAG_GAS, FLAG_IFACE, FLAG_SOLID, WaamError, lib
-from .field import DeviceField, P, dims_of, layout_of, tensor_of
+from .field import DeviceField, P, dims_of, layout_of, note_write, tensor_of
 
 _FLAGS = {"FLAG_FLUID": FLAG_FLUID, "FLAG_SOLID": FLAG_SOLID, "FLAG_GAS": FLAG_GAS,
           "FLAG_IFACE": FLAG_IFACE}
@@ -57,6 +57,7 @@ def bind_velocity_set(grid) -> None:
 
 
 def _call(name, *args):
+    note_write()  # any per-kernel entry may write fields the fused step tracks
     lib().call(name, *args)
 
 

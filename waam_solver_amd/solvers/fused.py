@@ -1,8 +1,11 @@
 """The fused coupled step: one ``waam_step_fused`` call per timestep.
 
 Same results as ``coupled_step`` (the kernel-per-reference-kernel sequence) — the parity tests
-run both against the oracle — in eight launches and without host round trips on ordinary
-steps (arc_k is found and used on the device). Steps on which a droplet detaches go through
+run both against the oracle — in seven launches and without host round trips on ordinary
+steps (arc_k is found and used on the device). The last kernel of a step lists the cells that
+end it FLUID/IFACE; the next step works through that list in the same launch as its thermal
+sweep, provided no host write (``field.write_epoch``) came in between; otherwise the sweep
+builds the list and one more launch consumes it. Steps on which a droplet detaches go through
 the standalone arc, deposition and impact kernels first (one host read of arc_k and of the
 placed volume), then the fused kernels with the arc marked as applied.
 
@@ -19,7 +22,7 @@ import torch
 
 from .. import kernels
 from .._lib import lib
-from ..field import DeviceField, P
+from ..field import DeviceField, P, write_epoch
 from ..physics import weld_forces
 from .coupled_step import _clamp_enthalpy_ceiling, _droplet, droplet_due, resolve_arc_k
 
@@ -39,7 +42,8 @@ _INT_PARAMS = ("is_welding", "collision", "arc_kind", "arc_applied", "pending_st
                "forces_from_fields", "store_forces", "enable_vof", "enable_wetting", "enable_csf",
                "enable_heat_loss", "enable_conv", "enable_rad", "enable_cap", "enable_hydro",
                "enable_freeze", "enable_trailing", "zero_velocity", "surface_weight", "nz_solid",
-               "col_i", "col_j", "use_arc_k_override", "use_k_robot", "force_reset_all")
+               "col_i", "col_j", "use_arc_k_override", "use_k_robot", "force_reset_all",
+               "thermal_list_ready")
 _FLOAT_PARAMS = ("arc_i", "arc_j", "arc_k_override", "k_robot", "Q_w", "eta", "dt", "dx", "dx3",
                  "sigma", "pen", "g_sgn", "g_ff", "g_fr", "g_af", "g_ar", "g_b", "g_c",
                  "alpha_lu", "cp_rho", "L_rho", "T_s", "T_l", "T_cap", "T_amb", "h_conv",
@@ -79,6 +83,9 @@ class FusedStepper:
         # the last fused step took its forces from the fields (an eager-arc step): the next one
         # clears every cell's forces, not only those of the previously collided cells
         self._forces_from_fields = False
+        # host write count right after the last fused step: while unchanged, the thermal list
+        # that step's last kernel built is this step's (nothing else has touched the flags)
+        self._list_epoch = None
         self._arc_out = (ctypes.c_float * 3)()
         # the swapped pairs alternate between two pointer sets: keep one struct per set
         self._fields_cache: dict[tuple, StepFields] = {}
@@ -95,6 +102,7 @@ class FusedStepper:
 
     def reset(self) -> None:
         self.pending = False
+        self._list_epoch = None
 
     def last_arc_k(self) -> float:
         torch.cuda.current_stream().synchronize()
@@ -222,6 +230,7 @@ class FusedStepper:
             p.force_reset_all = 1 if (not self.pending or self._forces_from_fields) else 0
             tw._last_arc_ijk = (float(arc_i), float(arc_j), None)
         self._forces_from_fields = bool(p.forces_from_fields)
+        p.thermal_list_ready = 1 if self._list_epoch == write_epoch() else 0
         fields = self._fields()
         lib().call("waam_step_fused", ctypes.addressof(p), ctypes.addressof(fields),
                    torch.cuda.current_stream().cuda_stream)
@@ -230,4 +239,5 @@ class FusedStepper:
             _swap(g, a, b)
         g.swap_buffers()
         self.pending = True
+        self._list_epoch = write_epoch()
         tw._step_n += 1
