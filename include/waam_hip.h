@@ -287,6 +287,22 @@ int waam_solidify_trailing_pool(float *T, float *H, float *f_l, float *phi, uint
                                 float dir_z, float lookback_cells, float T_freeze,
                                 float T_solidus, int nx, int ny, int nz, int layout, void *stream);
 
+/* ---- moving window: shift_simulation_window_x kernels.py:2620-2723 ------------------------ */
+/* Every listed field (both population buffers included) becomes new[i] = old[i + n_shift] for
+ * i < nx - n_shift (the reference's ascending serial loop), and the last n_shift columns are
+ * reset: T = T_amb, H = cp_rho T_amb, T_max = T_prev = T_amb, f_l = u = F = dT_dt = times = 0,
+ * rho = rho0, populations w_q rho0, SOLID with phi 1 below nz_solid and GAS with phi 0 above.
+ * The per-cell material fields are shifted and not reset, as in the reference. */
+int waam_shift_simulation_window_x(int n_shift, float *f_a, float *f_b, float *T, float *H,
+                                   float *f_l, float *phi, uint8_t *flags, float *T_max,
+                                   float *T_prev, float *dT_dt, float *time_above_800,
+                                   float *time_above_1100, float *time_above_solidus, float *rho,
+                                   float *ux, float *uy, float *uz, float *Fx, float *Fy,
+                                   float *Fz, float *cp_rho_field, float *alpha_lu_field,
+                                   float *dgamma_lu_field, float *tau_field, float T_amb,
+                                   float cp_rho, float rho0, int nz_solid, int nx, int ny, int nz,
+                                   int layout, void *stream);
+
 /* ---- telemetry: telemetry_pool_reduce kernels.py:3424-3468 ------------------------------- */
 /* out (device, 8 doubles): n_liquid, T_peak, cool, u2_max, i_sum, i_min, i_max, T_global. */
 int waam_telemetry_pool_reduce(const float *T, const float *f_l, const float *phi,

@@ -112,6 +112,7 @@ _SIGS = {
     "or_compute_marangoni_force_variable": [_P] * 8 + [_I, _I, _I],
     "or_remelt_hot_solid": [_P] * 6 + [_F, _F, _F, _I, _I, _I],
     "or_solidify_trailing_pool": [_P] * 9 + [_F] * 9 + [_I, _I, _I],
+    "or_shift_simulation_window_x": [_I, _P, _P, _P, _P, _F, _F, _F, _I, _I, _I, _I],
 }
 
 
@@ -288,7 +289,8 @@ class OracleTwin:
                  welding_current_A=140.0, travel_speed_m_s=0.0, enable_enthalpy_cap=True,
                  T_vapor_cap_K=3200.0, arc_surface_weighting=True, arc_penetration_mm=2.0,
                  wire_diameter_mm=1.2, arc_sigma_mm=2.0, bulk_tau=None, mode="jacobi",
-                 parallel=False, tracer_seed=0, use_variable_tau=True):
+                 parallel=False, tracer_seed=0, use_variable_tau=True,
+                 enable_moving_window=False):
         self.mat = material
         self.lib = lib(parallel)
         self.mode = mode
@@ -330,6 +332,8 @@ class OracleTwin:
         self.plate_size_mm = None
         self.plate_origin_mm = None
         self.nz_solid = max(1, nz // 5)
+        self.enable_moving_window = enable_moving_window
+        self._window_offset_x_m = 0.0
         self.tracer_seed = tracer_seed
         self.g = OracleGrid(nx, ny, nz, dx, material, 50_000 if max_tracers is None else max_tracers)
         g = self.g
@@ -629,7 +633,32 @@ class OracleTwin:
             if nrm > 1e-12:
                 self._torch_dir_xyz = (ddx / nrm, ddy / nrm, ddz / nrm)
         self._last_torch_pos_m = (torch_x_m, torch_y_m, z_now)
-        self.coupled_step(torch_x_m, torch_y_m, is_welding)
+        if self.enable_moving_window:
+            self._maybe_shift_window(torch_x_m)
+        self.coupled_step(torch_x_m - self._window_offset_x_m, torch_y_m, is_welding)
+
+    # ---------------------------------------------------------------- moving window (twin.py:599-626)
+    WINDOW_FIELDS = ("T", "H", "f_l", "phi", "T_max", "T_prev", "dT_dt", "time_above_800_s",
+                     "time_above_1100_s", "time_above_solidus_s", "rho", "ux", "uy", "uz", "Fx",
+                     "Fy", "Fz", "cp_rho_field", "alpha_lu_field", "dgamma_lu_field", "tau_field")
+
+    def shift_window(self, n_shift):
+        g = self.g
+        arrs = [getattr(g, n) for n in self.WINDOW_FIELDS]
+        table = (ctypes.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+        self.lib.or_shift_simulation_window_x(n_shift, ptr(g.f_a), ptr(g.f_b), table,
+                                              ptr(g.flags), self.T_amb, self.cp_rho, 1.0,
+                                              self.nz_solid, *g.dims)
+
+    def _maybe_shift_window(self, torch_x_world):
+        g = self.g
+        sim_x = torch_x_world - self._window_offset_x_m
+        if sim_x < 0.62 * (g.nx - 8) * g.dx:
+            return
+        n_shift = max(8, g.nx // 10)
+        self.shift_window(n_shift)
+        self._window_offset_x_m += n_shift * g.dx
+        g.porosity_pos[:, 0] -= np.float32(n_shift * g.dx)
 
     def coupled_step(self, x_m, y_m, is_welding):
         g, L, m = self.g, self.lib, self.mat

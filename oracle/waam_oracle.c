@@ -1805,3 +1805,62 @@ void or_solidify_trailing_pool(float *T, float *H, float *f_l, float *phi, uint8
                 uz[c] = 0.0f;
             }
 }
+
+/* ------------------------------------------------------------------------------------------
+ * Moving window — shift_simulation_window_x kernels.py:2620-2723: per (j, k) row, an ascending
+ * serial loop copies cell i + n into cell i (reading cells not yet written), then the last n
+ * columns are reset. fl[] holds the 21 scalar fp32 fields in the reference's argument order:
+ * T H f_l phi T_max T_prev dT_dt t800 t1100 tsol rho ux uy uz Fx Fy Fz cp_rho alpha dgamma tau.
+ * ---------------------------------------------------------------------------------------- */
+void or_shift_simulation_window_x(int n_shift, float *f_a, float *f_b, float **fl,
+                                  uint8_t *flags, float T_amb, float cp_rho, float rho0,
+                                  int nz_solid, int nx, int ny, int nz)
+{
+    size_t N = NCELL;
+    for (int j = 0; j < ny; ++j)
+        for (int k = 0; k < nz; ++k)
+            for (int i = 0; i < nx - n_shift; ++i) {
+                size_t d = IDX(i, j, k), s = IDX(i + n_shift, j, k);
+                for (int a = 0; a < 21; ++a) fl[a][d] = fl[a][s];
+                flags[d] = flags[s];
+                for (int q = 0; q < Q; ++q) {
+                    f_a[(size_t)q * N + d] = f_a[(size_t)q * N + s];
+                    f_b[(size_t)q * N + d] = f_b[(size_t)q * N + s];
+                }
+            }
+    int i0 = nx - n_shift;
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                if (i < i0) continue;
+                size_t c = IDX(i, j, k);
+                fl[0][c] = T_amb;
+                fl[1][c] = cp_rho * T_amb;
+                fl[2][c] = 0.0f;
+                fl[10][c] = rho0;
+                fl[11][c] = 0.0f;
+                fl[12][c] = 0.0f;
+                fl[13][c] = 0.0f;
+                fl[14][c] = 0.0f;
+                fl[15][c] = 0.0f;
+                fl[16][c] = 0.0f;
+                fl[5][c] = T_amb;
+                fl[6][c] = 0.0f;
+                fl[4][c] = T_amb;
+                fl[7][c] = 0.0f;
+                fl[8][c] = 0.0f;
+                fl[9][c] = 0.0f;
+                if (k < nz_solid) {
+                    flags[c] = SOLID;
+                    fl[3][c] = 1.0f;
+                } else {
+                    flags[c] = GAS;
+                    fl[3][c] = 0.0f;
+                }
+                for (int q = 0; q < Q; ++q) {
+                    float feq = WQ[q] * rho0;
+                    f_a[(size_t)q * N + c] = feq;
+                    f_b[(size_t)q * N + c] = feq;
+                }
+            }
+}

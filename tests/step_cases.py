@@ -29,6 +29,10 @@ def case_kwargs(name: str) -> dict:
     if name == "tables_mrt":
         # temperature-dependent cp, k, mu, dgamma/dT: variable-tau SRT collision, per-cell cp
         return dict(base, nx=64, ny=40, nz=32, dx=2e-4, droplet_freq_hz=0.0)
+    if name == "window_mrt":
+        # moving window: 40 cells long, shifts of 8 cells when the torch passes 62 % of 32
+        return dict(base, nx=40, ny=24, nz=24, dx=2e-4, droplet_freq_hz=0.0,
+                    enable_moving_window=True)
     if name == "bench_small":
         return dict(base, nx=128, ny=64, nz=32, dx=2e-4, droplet_freq_hz=0.0)
     raise KeyError(name)

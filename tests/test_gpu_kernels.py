@@ -425,3 +425,28 @@ def test_table_kernels(env, dims, cp_table, use_tables):
                                 dims[1] / 2, 4.0, 1.0, 0.0, 0.0, 3.0, 2400.0, Ts, *dims)
     for k in ("T", "H", "f_l", "phi", "flags", "ux", "uy", "uz"):
         check(k, down(g[k]), o[k])
+
+
+@pytest.mark.parametrize("dims", DIMS)
+@pytest.mark.parametrize("n_shift", [5, 8, 80])
+def test_shift_window(env, dims, n_shift):
+    """shift_simulation_window_x on seeded fields, n_shift below and above nx."""
+    import ctypes
+
+    from oracle.oracle import OracleTwin
+
+    rng = np.random.default_rng(n_shift)
+    names = OracleTwin.WINDOW_FIELDS
+    o = {n: rng.standard_normal(dims).astype(np.float32) for n in names}
+    o["flags"] = rng.integers(0, 5, dims).astype(np.uint8)
+    o["f_a"] = rng.standard_normal((19,) + dims).astype(np.float32)
+    o["f_b"] = rng.standard_normal((19,) + dims).astype(np.float32)
+    g = {k: up(env, v) for k, v in o.items()}
+    env["K"].shift_simulation_window_x(n_shift, g["f_a"], g["f_b"], *(g[n] for n in names[:4]),
+                                       g["flags"], *(g[n] for n in names[4:]), 293.0, 5.3e6, 1.0,
+                                       3, 1, 2, 0, *dims)
+    table = (ctypes.c_void_p * len(names))(*[o[n].ctypes.data for n in names])
+    env["O"].or_shift_simulation_window_x(n_shift, P(o["f_a"]), P(o["f_b"]), table, P(o["flags"]),
+                                          293.0, 5.3e6, 1.0, 3, *dims)
+    for k in o:
+        check(k, down(g[k]), o[k])

@@ -116,3 +116,20 @@ def test_job_drop_in_parity(hip_device, tmp_path, engine):
     assert len(calls) == 60
     _assert_parity(compare(tw, ot, FIELDS + TABLE_FIELDS), f"job/{engine} step 60")
     assert tw._n_droplets_fired == ot._n_droplets_fired and tw._n_droplets_fired > 0
+
+
+@pytest.mark.parametrize("engine,layout", [("fused", 1), ("reference", 0)])
+def test_moving_window_parity(hip_device, engine, layout):
+    """Three window shifts in 100 steps (torch at 0.2 cells per step from 18 cells; trigger at
+    0.62 x 32 cells, shifts of 8): every field, both population buffers, the torch offset and
+    the tracers follow the oracle."""
+    tw, ot = build_pair("window_mrt", engine=engine, layout=layout)
+    g = tw.grid
+    y = (g.ny // 2) * g.dx
+    for s in range(100):
+        x = (18.0 + 0.2 * s) * g.dx
+        tw.step(x, y, True)
+        ot.step(x, y, True)
+        if s in (0, 12, 55, 99):
+            _assert_parity(compare(tw, ot), f"window/{engine}/layout{layout} step {s + 1}")
+    assert tw._window_offset_x_m == ot._window_offset_x_m == 3 * 8 * g.dx

@@ -593,3 +593,47 @@ def test_trailing_pool_rules_at_their_thresholds():
             assert u[0][c] == 0.0 and u[2][c] == 0.0
         else:
             assert flags[c] == fg and T[c] == np.float32(t) and H[c] == 7.0
+
+
+# ---------------------------------------------------------------------------- moving window
+@pytest.mark.parametrize("n_shift", [3, 8, 12, 20])
+def test_window_shift_is_a_pure_shift_with_reset_strip(n_shift):
+    """An ascending serial loop that copies cell i + n into cell i reads only cells not yet
+    written, so every field becomes new[:nx-n] = old[n:] (a descending loop would re-read
+    shifted cells); the last n columns take the ambient plate/gas state (kernels.py:2620-2723).
+    n >= nx resets the whole domain."""
+    from oracle.oracle import OracleTwin
+
+    nx, ny, nz = 12, 3, 5
+    rng = np.random.default_rng(n_shift)
+    names = OracleTwin.WINDOW_FIELDS
+    fl = [rng.standard_normal((nx, ny, nz)).astype(np.float32) for _ in names]
+    flags = rng.integers(0, 5, (nx, ny, nz)).astype(np.uint8)
+    f_a = rng.standard_normal((19, nx, ny, nz)).astype(np.float32)
+    f_b = rng.standard_normal((19, nx, ny, nz)).astype(np.float32)
+    old = [a.copy() for a in fl], flags.copy(), f_a.copy(), f_b.copy()
+    import ctypes
+
+    table = (ctypes.c_void_p * len(fl))(*[a.ctypes.data for a in fl])
+    lib().or_shift_simulation_window_x(n_shift, ptr(f_a), ptr(f_b), table, ptr(flags), 293.0,
+                                       5.3e6, 1.0, 2, nx, ny, nz)
+    m = max(0, nx - n_shift)
+    for a, o in zip(fl + [flags], old[0] + [old[1]]):
+        np.testing.assert_array_equal(a[:m], o[n_shift:n_shift + m])
+    for a, o in ((f_a, old[2]), (f_b, old[3])):
+        np.testing.assert_array_equal(a[:, :m], o[:, n_shift:n_shift + m])
+        np.testing.assert_array_equal(a[:, m:], np.broadcast_to(
+            np.asarray(W, np.float32)[:, None, None, None], a[:, m:].shape))
+    d = dict(zip(names, fl))
+    strip = slice(m, nx)
+    assert np.all(d["T"][strip] == np.float32(293.0))
+    assert np.all(d["H"][strip] == np.float32(5.3e6) * np.float32(293.0))
+    assert np.all(d["T_max"][strip] == np.float32(293.0)) and np.all(d["rho"][strip] == 1.0)
+    for n in ("f_l", "ux", "uy", "uz", "Fx", "Fy", "Fz", "dT_dt", "time_above_800_s",
+              "time_above_1100_s", "time_above_solidus_s"):
+        assert np.all(d[n][strip] == 0.0), n
+    # the per-cell material fields are shifted but not reset
+    for n in ("cp_rho_field", "alpha_lu_field", "dgamma_lu_field", "tau_field"):
+        np.testing.assert_array_equal(d[n][strip], old[0][names.index(n)][strip])
+    assert np.all(flags[strip][:, :, :2] == SOLID) and np.all(flags[strip][:, :, 2:] == GAS)
+    assert np.all(d["phi"][strip][:, :, :2] == 1.0) and np.all(d["phi"][strip][:, :, 2:] == 0.0)

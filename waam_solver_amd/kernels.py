@@ -568,3 +568,25 @@ def extract_fl_yz_slice(f_l, out, i_slice, ny, nz):
 
 
 __all__ = [n for n in dir() if not n.startswith("_") and n not in ("math", "torch")]
+
+
+# ------------------------------------------------------------------ moving window
+def shift_simulation_window_x(n_shift, f_a, f_b, T, H, f_l, phi, flags, T_max, T_prev, dT_dt,
+                              time_above_800, time_above_1100, time_above_solidus, rho, ux, uy,
+                              uz, Fx, Fy, Fz, cp_rho_field, alpha_lu_field, dgamma_lu_field,
+                              tau_field, T_amb, cp_rho, rho0, nz_solid, FLAG_SOLID, FLAG_GAS,
+                              FLAG_FLUID, nx, ny, nz):
+    """Shift every field left by n_shift cells and reset the right strip (reference
+    kernels.py:2620-2723)."""
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS, FLAG_FLUID=FLAG_FLUID)
+    n = nx * ny * nz
+    _need_pop(n, f_a, f_b)
+    fields = (T, H, f_l, phi, flags, T_max, T_prev, dT_dt, time_above_800, time_above_1100,
+              time_above_solidus, rho, ux, uy, uz, Fx, Fy, Fz, cp_rho_field, alpha_lu_field,
+              dgamma_lu_field, tau_field)
+    _need(n, *fields)
+    lay = layout_of(T)
+    if any(layout_of(x) != lay for x in (f_a, f_b) + fields):
+        raise ValueError("shift_simulation_window_x: fields of different layouts")
+    _call("waam_shift_simulation_window_x", int(n_shift), P(f_a), P(f_b),
+          *(P(x) for x in fields), T_amb, cp_rho, rho0, int(nz_solid), nx, ny, nz, lay, _stream())

@@ -78,8 +78,6 @@ class WAAMTwin:
             raise ValueError(f"arc_sigma_mm must be > 0, got {arc_sigma_mm}")
         if contact_angle_deg is not None and not (0.0 < contact_angle_deg < 180.0):
             raise ValueError(f"contact_angle_deg must be in (0, 180), got {contact_angle_deg}")
-        if enable_moving_window:
-            raise NotImplementedError("the moving window is not built")
         self.mat = material if isinstance(material, MaterialProps) else load_material(material, high_sulphur)
         max_tracers = 50_000 if max_tracers is None else max_tracers
 
@@ -337,6 +335,8 @@ class WAAMTwin:
                 and not self._warned_zero_wire_feed):
             self._warned_zero_wire_feed = True
             log.warning("[WAAMTwin] welding step with wire_feed_m_s == 0: no metal is deposited")
+        if self.enable_moving_window:
+            self._maybe_shift_window(torch_x_m)
         sim_x = torch_x_m - self._window_offset_x_m - self._sim_origin_offset_x_m
         sim_y = torch_y_m - self._sim_origin_offset_y_m
         if self.engine == "fused" and self.use_material_tables:
@@ -353,6 +353,29 @@ class WAAMTwin:
         else:
             raise ValueError(f"unknown engine {self.engine!r} (fused | reference)")
         self._check_strict_mode()
+
+    def _maybe_shift_window(self, torch_x_world: float) -> None:
+        """Shift the domain by max(8, nx // 10) cells once the torch passes 62 % of the usable
+        length (reference twin.py:599-626); the torch offset and tracer positions follow."""
+        g = self.grid
+        sim_x = torch_x_world - self._window_offset_x_m
+        trigger = 0.62 * (g.nx - 8) * g.dx
+        if sim_x < trigger:
+            return
+        n_shift = max(8, g.nx // 10)
+        self.sync()  # the deferred population stream must land before the shift
+        kernels.shift_simulation_window_x(
+            n_shift, g.f_a, g.f_b, g.T, g.H, g.f_l, g.phi, g.flags, g.T_max, g.T_prev, g.dT_dt,
+            g.time_above_800_s, g.time_above_1100_s, g.time_above_solidus_s, g.rho, g.ux, g.uy,
+            g.uz, g.Fx, g.Fy, g.Fz, g.cp_rho_field, g.alpha_lu_field, g.dgamma_lu_field,
+            g.tau_field, self.T_amb, self.cp_rho, RHO_LATTICE, self.nz_solid, g.FLAG_SOLID,
+            g.FLAG_GAS, g.FLAG_FLUID, g.nx, g.ny, g.nz)
+        self._window_offset_x_m += n_shift * g.dx
+        pos = g.porosity_pos.to_numpy()
+        pos[:, 0] -= n_shift * g.dx
+        g.porosity_pos.from_numpy(pos)
+        log.info(f"[WAAMTwin] moving window shift -{n_shift} cells, "
+                 f"offset={self._window_offset_x_m * 1e3:.2f} mm")
 
     def sync(self) -> None:
         """Materialise deferred device work (the fused engine's last population stream).
