@@ -303,6 +303,12 @@ int waam_shift_simulation_window_x(int n_shift, float *f_a, float *f_b, float *T
                                    float cp_rho, float rho0, int nz_solid, int nx, int ny, int nz,
                                    int layout, void *stream);
 
+/* ---- bead height for the CTWD update: measure_bead_height_m physics/electrical_stickout.py:39-50
+ * out (device int32): the largest k of a SOLID cell with k >= nz_solid, or -1 when there is none
+ * (the reference computes it on the host from every flag). */
+int waam_bead_top_k(const uint8_t *flags, int nz_solid, int32_t *out, int nx, int ny, int nz,
+                    int layout, void *stream);
+
 /* ---- telemetry: telemetry_pool_reduce kernels.py:3424-3468 ------------------------------- */
 /* out (device, 8 doubles): n_liquid, T_peak, cool, u2_max, i_sum, i_min, i_max, T_global. */
 int waam_telemetry_pool_reduce(const float *T, const float *f_l, const float *phi,

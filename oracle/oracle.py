@@ -290,7 +290,7 @@ class OracleTwin:
                  T_vapor_cap_K=3200.0, arc_surface_weighting=True, arc_penetration_mm=2.0,
                  wire_diameter_mm=1.2, arc_sigma_mm=2.0, bulk_tau=None, mode="jacobi",
                  parallel=False, tracer_seed=0, use_variable_tau=True,
-                 enable_moving_window=False):
+                 enable_moving_window=False, enable_ctwd=False):
         self.mat = material
         self.lib = lib(parallel)
         self.mode = mode
@@ -327,7 +327,12 @@ class OracleTwin:
         self.u_mach_limit_lu, self.force_limit_lu = 0.08, 0.05
         self.arc_pressure_model = "constant"
         self.pressure_sigma_m = None
-        self.enable_ctwd = False
+        self.enable_ctwd = enable_ctwd
+        # stick-out and CTWD (twin.py:186-191; physics/electrical_stickout.py:15-59)
+        self.stickout_m, self.ctwd_nominal_m, self.layer_height_m = 12e-3, 15e-3, 1.2e-3
+        self.ctwd_m = self.ctwd_nominal_m
+        self.rho_e_ohm_m, self.eta_stick = material.rho_e_ohm_m, material.eta_stick
+        self._bead_height_m = 0.0
         self.plate_thickness_mm = None
         self.plate_size_mm = None
         self.plate_origin_mm = None
@@ -485,7 +490,28 @@ class OracleTwin:
         return float(self.arc_pressure)
 
     def droplet_entry_temperature_K(self):
-        return self.mat.T_liquidus + self.deposition_superheat_K
+        """Liquidus + superheat, plus I^2 R heating of the stick-out (rho_e L / A) spread over
+        the wire mass flux when CTWD coupling is on, capped 50 K under the vapour cap."""
+        base = self.mat.T_liquidus + self.deposition_superheat_K
+        if not self.enable_ctwd:
+            return base
+        area = math.pi * (self.wire_diameter_m * 0.5) ** 2
+        R = self.rho_e_ohm_m * max(self.stickout_m, 1e-6) / max(area, 1e-12)
+        P = self.eta_stick * self.welding_current_A ** 2 * R
+        mdot = self.wire_mass_flux_kg_s()
+        if mdot <= 0:
+            return base
+        return min(self.T_vapor_cap_K - 50.0, base + P / (mdot * self.mat.cp))
+
+    def update_ctwd(self):
+        """Bead height = (top SOLID k at or above nz_solid - nz_solid + 1) dx; CTWD follows it
+        against the nominal layer height, never below nominal."""
+        g = self.g
+        ks = np.nonzero(g.flags == SOLID)[2]
+        ks = ks[ks >= self.nz_solid]
+        h = 0.0 if ks.size == 0 else float((ks.max() - self.nz_solid + 1) * g.dx)
+        self._bead_height_m = h
+        self.ctwd_m = max(self.ctwd_nominal_m, self.ctwd_nominal_m + (h - self.layer_height_m))
 
     def droplet_impact_velocity_m_s(self, m_drop):
         drop_len = 3e-3
@@ -670,6 +696,8 @@ class OracleTwin:
         g.Fx.fill(0)
         g.Fy.fill(0)
         g.Fz.fill(0)
+        if self.enable_ctwd:
+            self.update_ctwd()
         if is_welding:
             self._inject_arc(arc_i, arc_j, arc_k)
             self._ceiling()

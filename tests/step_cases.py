@@ -29,6 +29,10 @@ def case_kwargs(name: str) -> dict:
     if name == "tables_mrt":
         # temperature-dependent cp, k, mu, dgamma/dT: variable-tau SRT collision, per-cell cp
         return dict(base, nx=64, ny=40, nz=32, dx=2e-4, droplet_freq_hz=0.0)
+    if name == "ctwd_srt":
+        # droplets with stick-out preheat and the CTWD bead-height update every step
+        return dict(base, nx=56, ny=40, nz=32, dx=2e-4, use_srt=True, droplet_freq_hz=2500.0,
+                    enable_ctwd=True)
     if name == "window_mrt":
         # moving window: 40 cells long, shifts of 8 cells when the torch passes 62 % of 32
         return dict(base, nx=40, ny=24, nz=24, dx=2e-4, droplet_freq_hz=0.0,
@@ -41,7 +45,7 @@ def case_kwargs(name: str) -> dict:
 def configure(twin, name: str) -> None:
     twin.plate_thickness_mm = 1.6
     twin.travel_speed_m_s = 8e-3
-    if name == "droplets_srt":
+    if name in ("droplets_srt", "ctwd_srt"):
         twin.wire_feed_m_s = 6.5 / 60.0
         twin.welding_current_A = 140.0
 
@@ -126,8 +130,8 @@ def oracle_like(tw, mode: str = "jacobi"):
     a preset or a job file), reset is left to the caller."""
     from oracle.oracle import Material, OracleTwin
 
-    if tw.enable_ctwd or tw.use_torch_z or tw.enable_moving_window:
-        raise NotImplementedError("the oracle twin has no CTWD, torch-z or moving-window model")
+    if tw.use_torch_z:
+        raise NotImplementedError("the oracle twin has no robot-z arc placement")
     m, g = tw.mat, tw.grid
     # the product material is already through its surfactant model: hand its final dgamma/dT
     # and gamma_0 over at the 30 ppm identity of the Heiple scale
@@ -165,7 +169,9 @@ def oracle_like(tw, mode: str = "jacobi"):
                  "impact_lead_angle_deg", "trailing_solidify_lookback_mm",
                  "trailing_solidify_temp_margin_K", "u_mach_limit_lu", "force_limit_lu",
                  "arc_pressure_model", "pressure_sigma_m", "plate_thickness_mm", "plate_size_mm",
-                 "plate_origin_mm", "nz_solid"):
+                 "plate_origin_mm", "nz_solid", "enable_ctwd", "stickout_m", "ctwd_nominal_m",
+                 "ctwd_m", "layer_height_m", "rho_e_ohm_m", "eta_stick", "enable_moving_window",
+                 "_window_offset_x_m"):
         setattr(ot, name, getattr(tw, name))
     key = tw.heat_source_name.lower().replace("-", "").replace("_", "")
     if key in ("goldak", "goldak3d", "doubleellipsoid"):

@@ -87,6 +87,7 @@ SIGNATURES: dict[str, str] = {
     "waam_remelt_hot_solid": "ppppppffflp",
     "waam_solidify_trailing_pool": "pppppppppfffffffffiiiip",
     "waam_shift_simulation_window_x": "i" + "p" * 24 + "fffiiiiip",
+    "waam_bead_top_k": "pipiiiip",
 }
 
 # kernel ids of waam_step_fused for waam_timing_begin (include/waam_hip.h WAAM_K_*)

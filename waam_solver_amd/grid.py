@@ -94,6 +94,7 @@ class WAAMGrid:
         self.arc_norm_buf = z0()
         self.deposit_result = DeviceField.zeros((2,), torch.float32, dev)
         self.telem_out = DeviceField.zeros((8,), torch.float64, dev)
+        self.bead_top_buf = DeviceField.zeros((1,), torch.int32, dev)
         self.telem_fl_slice = DeviceField.zeros((self.ny, self.nz), torch.float32, dev)
         # tracers
         self.porosity_pos = DeviceField.zeros((self.max_tracers, 3), torch.float32, dev)

@@ -61,6 +61,11 @@ def _call(name, *args):
     lib().call(name, *args)
 
 
+def _call_ro(name, *args):
+    """An entry that reads fields and writes only its own output buffer."""
+    lib().call(name, *args)
+
+
 def _scratch_of(*fields):
     """Workspace tensor for reducing entries: the grid's when reachable, else a fresh one."""
     return torch.zeros(lib().scratch_bytes(), dtype=torch.uint8, device=tensor_of(fields[0]).device)
@@ -439,7 +444,7 @@ def advect_tracers(pos, active, ux, uy, uz, f_l, flags, dx, dt, max_tracers, FLA
 def telemetry_pool_reduce_into(T, f_l, phi, dT_dt, ux, uy, uz, out, scratch):
     nx, ny, nz = dims_of(T)
     _need(nx * ny * nz, f_l, phi, dT_dt, ux, uy, uz)
-    _call("waam_telemetry_pool_reduce", P(T), P(f_l), P(phi), P(dT_dt), P(ux), P(uy), P(uz),
+    _call_ro("waam_telemetry_pool_reduce", P(T), P(f_l), P(phi), P(dT_dt), P(ux), P(uy), P(uz),
           P(out), nx, ny, nz, layout_of(T), P(scratch), _stream())
 
 
@@ -590,3 +595,12 @@ def shift_simulation_window_x(n_shift, f_a, f_b, T, H, f_l, phi, flags, T_max, T
         raise ValueError("shift_simulation_window_x: fields of different layouts")
     _call("waam_shift_simulation_window_x", int(n_shift), P(f_a), P(f_b),
           *(P(x) for x in fields), T_amb, cp_rho, rho0, int(nz_solid), nx, ny, nz, lay, _stream())
+
+
+# ------------------------------------------------------------------ CTWD bead height
+def bead_top_k(flags, nz_solid, out) -> None:
+    """out[0] = largest k of a SOLID cell with k >= nz_solid, -1 when none (the reduction
+    behind reference physics/electrical_stickout.py:39-50)."""
+    nx, ny, nz = dims_of(flags)
+    _call_ro("waam_bead_top_k", P(flags), int(nz_solid), P(out), nx, ny, nz, layout_of(flags),
+             _stream())

@@ -22,7 +22,7 @@ import numpy as np
 from .. import kernels
 from ..field import P
 from ..physics import deposition, weld_forces
-from ..physics.electrical_stickout import droplet_entry_temperature_K
+from ..physics.electrical_stickout import droplet_entry_temperature_K, update_ctwd
 
 log = logging.getLogger("waam_solver_amd")
 
@@ -139,6 +139,8 @@ def coupled_step(twin, torch_x_m: float, torch_y_m: float, is_welding: bool,
     twin._last_arc_ijk = (float(arc_i), float(arc_j), float(arc_k))
 
     kernels.clear_forces(g.Fx, g.Fy, g.Fz)
+    if twin.enable_ctwd:
+        update_ctwd(twin, g)
     current_pressure = weld_forces.arc_pressure_peak_pa(twin)
     pressure_sigma_cells = weld_forces.arc_pressure_sigma_cells(twin)
 

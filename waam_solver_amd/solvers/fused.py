@@ -24,6 +24,7 @@ from .. import kernels
 from .._lib import lib
 from ..field import DeviceField, P, write_epoch
 from ..physics import weld_forces
+from ..physics.electrical_stickout import update_ctwd
 from .coupled_step import _clamp_enthalpy_ceiling, _droplet, droplet_due, resolve_arc_k
 
 _i32, _f32, _vp = ctypes.c_int32, ctypes.c_float, ctypes.c_void_p
@@ -207,6 +208,9 @@ class FusedStepper:
         arc_i = torch_x_m / g.dx
         arc_j = torch_y_m / g.dx
         p = self._params(arc_i, arc_j, is_welding, torch_z_m)
+        if tw.enable_ctwd:
+            # after arc_k's inputs, before this step changes the flags (reference order)
+            update_ctwd(tw, g)
         # Eager arc: droplet steps (deposition edits the fields the fused kernels read, and the
         # deferred stream reads f_src, which deposition changes) and the conical source (whose
         # centre sits 1.5 cells below the resolved arc_k).
