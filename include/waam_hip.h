@@ -287,6 +287,29 @@ int waam_solidify_trailing_pool(float *T, float *H, float *f_l, float *phi, uint
                                 float dir_z, float lookback_cells, float T_freeze,
                                 float T_solidus, int nx, int ny, int nz, int layout, void *stream);
 
+/* ---- full-tier surface forces (arc_k is accepted and unused, as in the reference) ---------- */
+/* apply_vapor_recoil kernels.py:995-1030: Fz += -(p/(rho dx)) dt^2/dx min(T/T_ref, 3)
+ * exp(-r_ij^2 / (2 sigma^2)) on non-SOLID/GAS cells with |phi(k+1) - phi(k-1)| >= 0.05 */
+int waam_apply_vapor_recoil(float *Fz, const float *T, const float *phi, const uint8_t *flags,
+                            float arc_i, float arc_j, float arc_k, float sigma, float recoil_pa,
+                            float T_ref, float dt, float dx, float rho_ref, int nx, int ny,
+                            int nz, int layout, void *stream);
+/* apply_vapor_recoil_clausius_clapeyron kernels.py:1052-1090: p = C_acc P_ref
+ * exp(min(L/R (1/T_boil - 1/T), 12)) above T_boil */
+int waam_apply_vapor_recoil_clausius_clapeyron(float *Fz, const float *T, const float *phi,
+                                               const uint8_t *flags, float arc_i, float arc_j,
+                                               float arc_k, float sigma, float P_ref_Pa,
+                                               float T_boil_K, float L_vapor_J_kg,
+                                               float R_spec_J_kgK, float C_acc, float dt, float dx,
+                                               float rho_ref, int nx, int ny, int nz, int layout,
+                                               void *stream);
+/* apply_gas_shear_stress kernels.py:1093-1139: tangential (Fx, Fy) shear away from the arc
+ * axis where |grad phi| >= 0.08 */
+int waam_apply_gas_shear_stress(float *Fx, float *Fy, const float *phi, const uint8_t *flags,
+                                float arc_i, float arc_j, float arc_k, float sigma,
+                                float tau_peak_pa, float dt, float dx, float rho_ref, int nx,
+                                int ny, int nz, int layout, void *stream);
+
 /* ---- moving window: shift_simulation_window_x kernels.py:2620-2723 ------------------------ */
 /* Every listed field (both population buffers included) becomes new[i] = old[i + n_shift] for
  * i < nx - n_shift (the reference's ascending serial loop), and the last n_shift columns are
@@ -358,6 +381,9 @@ typedef struct waam_step_params {
      * the surface cells' thermal update then runs in the same launch as the sweep. 0: the sweep
      * builds the list and a second launch consumes it. */
     int32_t thermal_list_ready;
+    /* full-tier surface forces (physics/weld_forces.py:120-161): vapour recoil (clausius: the
+     * Clausius-Clapeyron form, else the constant-pressure form) and shielding-gas shear */
+    int32_t enable_recoil, recoil_clausius, enable_gas_shear;
     /* arc */
     float arc_i, arc_j, arc_k_override, k_robot;
     float Q_w, eta, dt, dx, dx3, sigma, pen;
@@ -370,6 +396,8 @@ typedef struct waam_step_params {
     float omega_s, omega_b, C_darcy;
     /* freeze */
     float dir_x, dir_y, dir_z, lookback, T_freeze;
+    /* recoil (recoil_pa and recoil_T_ref: constant form) and gas shear (peak stress, Pa) */
+    float recoil_pa, recoil_T_ref, P_vap_ref, T_boil, L_vap, R_vap, C_acc, gas_tau;
 } waam_step_params;
 
 int waam_step_fused(const waam_step_params *p, const waam_step_fields *f, void *stream);

@@ -113,6 +113,9 @@ _SIGS = {
     "or_remelt_hot_solid": [_P] * 6 + [_F, _F, _F, _I, _I, _I],
     "or_solidify_trailing_pool": [_P] * 9 + [_F] * 9 + [_I, _I, _I],
     "or_shift_simulation_window_x": [_I, _P, _P, _P, _P, _F, _F, _F, _I, _I, _I, _I],
+    "or_apply_vapor_recoil": [_P] * 4 + [_F] * 8 + [_I, _I, _I],
+    "or_apply_vapor_recoil_cc": [_P] * 4 + [_F] * 11 + [_I, _I, _I],
+    "or_apply_gas_shear_stress": [_P] * 4 + [_F] * 7 + [_I, _I, _I],
 }
 
 
@@ -290,7 +293,11 @@ class OracleTwin:
                  T_vapor_cap_K=3200.0, arc_surface_weighting=True, arc_penetration_mm=2.0,
                  wire_diameter_mm=1.2, arc_sigma_mm=2.0, bulk_tau=None, mode="jacobi",
                  parallel=False, tracer_seed=0, use_variable_tau=True,
-                 enable_moving_window=False, enable_ctwd=False):
+                 enable_moving_window=False, enable_ctwd=False, enable_recoil=False,
+                 use_recoil_clausius_clapeyron=True, recoil_pressure_pa=5000.0,
+                 enable_gas_shear=False, gas_jet_velocity_m_s=12.0, gas_shear_coeff=1.0,
+                 T_boiling_K=3100.0, P_vapor_ref_Pa=101325.0, L_vapor_J_kg=6.0e6,
+                 R_spec_vapor_J_kgK=450.0):
         self.mat = material
         self.lib = lib(parallel)
         self.mode = mode
@@ -328,6 +335,13 @@ class OracleTwin:
         self.arc_pressure_model = "constant"
         self.pressure_sigma_m = None
         self.enable_ctwd = enable_ctwd
+        # full-tier surface forces (twin.py:181-197; physics/weld_forces.py:120-161)
+        self.enable_recoil, self.use_recoil_cc = enable_recoil, use_recoil_clausius_clapeyron
+        self.recoil_pressure_pa, self.recoil_accommodation = recoil_pressure_pa, 0.54
+        self.T_boiling_K, self.P_vapor_ref_Pa = T_boiling_K, P_vapor_ref_Pa
+        self.L_vapor_J_kg, self.R_spec_vapor_J_kgK = L_vapor_J_kg, R_spec_vapor_J_kgK
+        self.enable_gas_shear = enable_gas_shear
+        self.gas_jet_velocity_m_s, self.gas_shear_coeff = gas_jet_velocity_m_s, gas_shear_coeff
         # stick-out and CTWD (twin.py:186-191; physics/electrical_stickout.py:15-59)
         self.stickout_m, self.ctwd_nominal_m, self.layer_height_m = 12e-3, 15e-3, 1.2e-3
         self.ctwd_m = self.ctwd_nominal_m
@@ -773,10 +787,25 @@ class OracleTwin:
             L.or_compute_marangoni_force(ptr(g.T), ptr(g.phi), ptr(g.f_l), ptr(g.Fx),
                                          ptr(g.Fy), ptr(g.Fz), ptr(g.flags), self.dgamma_dT_lu,
                                          *d)
+        if is_welding and self.enable_gas_shear:
+            # shielding gas: 0.5 rho_Ar v^2 c, capped at 5 kPa
+            tau = min(0.5 * 1.6 * self.gas_jet_velocity_m_s ** 2 * self.gas_shear_coeff, 5000.0)
+            L.or_apply_gas_shear_stress(ptr(g.Fx), ptr(g.Fy), ptr(g.phi), ptr(g.flags), arc_i,
+                                        arc_j, self.sigma_cells, tau, g.dt, g.dx, m.rho, *d)
         if is_welding:
             L.or_apply_arc_pressure(ptr(g.Fz), ptr(g.flags), ptr(g.phi), arc_i, arc_j, arc_k,
                                     self.arc_pressure_sigma_m() / g.dx,
                                     self.arc_pressure_peak_pa(), g.dt, g.dx, m.rho, *d)
+            if self.enable_recoil and self.use_recoil_cc:
+                L.or_apply_vapor_recoil_cc(ptr(g.Fz), ptr(g.T), ptr(g.phi), ptr(g.flags), arc_i,
+                                           arc_j, self.sigma_cells, self.P_vapor_ref_Pa,
+                                           self.T_boiling_K, self.L_vapor_J_kg,
+                                           self.R_spec_vapor_J_kgK, self.recoil_accommodation,
+                                           g.dt, g.dx, m.rho, *d)
+            elif self.enable_recoil:
+                L.or_apply_vapor_recoil(ptr(g.Fz), ptr(g.T), ptr(g.phi), ptr(g.flags), arc_i,
+                                        arc_j, self.sigma_cells, self.recoil_pressure_pa,
+                                        m.T_liquidus, g.dt, g.dx, m.rho, *d)
         if self.enable_hydrostatic_gravity:
             L.or_add_hydrostatic_gravity(ptr(g.Fz), ptr(g.f_l), ptr(g.flags), 1.0, self.g_lu, *d)
         L.or_add_buoyancy(ptr(g.T), ptr(g.Fz), ptr(g.f_l), ptr(g.flags), self.g_lu, self.beta_T,

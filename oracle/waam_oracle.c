@@ -1864,3 +1864,93 @@ void or_shift_simulation_window_x(int n_shift, float *f_a, float *f_b, float **f
                 }
             }
 }
+
+/* ------------------------------------------------------------------------------------------
+ * Full-tier surface forces — apply_vapor_recoil kernels.py:995-1030,
+ * apply_vapor_recoil_clausius_clapeyron :1052-1090 (with _wf_pressure_to_Fz_lu :1033-1035),
+ * apply_gas_shear_stress :1093-1139. Per-cell and order-free.
+ * ---------------------------------------------------------------------------------------- */
+void or_apply_vapor_recoil(float *Fz, const float *T, const float *phi, const uint8_t *flags,
+                           float arc_i, float arc_j, float sigma, float recoil_pa, float T_ref,
+                           float dt, float dx, float rho_ref, int nx, int ny, int nz)
+{
+    const float eps = 1e-6f;
+    float inv2s2 = 1.0f / (2.0f * sigma * sigma + eps);
+    float F_peak_lu = (recoil_pa / (rho_ref * dx)) * dt * dt / dx;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                size_t c = IDX(i, j, k);
+                if (flags[c] == SOLID || flags[c] == GAS) continue;
+                float dphi_z = phi[IDX(i, j, imin(k + 1, nz - 1))] - phi[IDX(i, j, imax(k - 1, 0))];
+                if (fabsf(dphi_z) < 0.05f) continue;
+                float di = (float)i - arc_i, dj = (float)j - arc_j;
+                float r2 = di * di + dj * dj;
+                float t_ratio = fminf_(T[c] / (T_ref + eps), 3.0f);
+                Fz[c] += -F_peak_lu * t_ratio * waam_expf(-r2 * inv2s2);
+            }
+}
+
+void or_apply_vapor_recoil_cc(float *Fz, const float *T, const float *phi, const uint8_t *flags,
+                              float arc_i, float arc_j, float sigma, float P_ref, float T_boil,
+                              float L_vap, float R_spec, float C_acc, float dt, float dx,
+                              float rho_ref, int nx, int ny, int nz)
+{
+    const float eps = 1e-6f;
+    float inv2s2 = 1.0f / (2.0f * sigma * sigma + eps);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                size_t c = IDX(i, j, k);
+                if (flags[c] == SOLID || flags[c] == GAS) continue;
+                float dphi_z = phi[IDX(i, j, imin(k + 1, nz - 1))] - phi[IDX(i, j, imax(k - 1, 0))];
+                if (fabsf(dphi_z) < 0.05f) continue;
+                float Tc = T[c];
+                if (Tc <= T_boil) continue;
+                float expo = (L_vap / (R_spec + eps)) * (1.0f / (T_boil + eps) - 1.0f / (Tc + eps));
+                expo = fminf_(expo, 12.0f);
+                float P_vap = C_acc * P_ref * waam_expf(expo);
+                float F_peak_phys = P_vap / (rho_ref * dx);
+                float F_peak_lu = F_peak_phys * dt * dt / dx;
+                float di = (float)i - arc_i, dj = (float)j - arc_j;
+                float r2 = di * di + dj * dj;
+                Fz[c] += -F_peak_lu * waam_expf(-r2 * inv2s2);
+            }
+}
+
+void or_apply_gas_shear_stress(float *Fx, float *Fy, const float *phi, const uint8_t *flags,
+                               float arc_i, float arc_j, float sigma, float tau_peak, float dt,
+                               float dx, float rho_ref, int nx, int ny, int nz)
+{
+    const float eps = 1e-6f;
+    float inv2s2 = 1.0f / (2.0f * sigma * sigma + eps);
+    float F_scale = (1.0f / (rho_ref * dx)) * dt * dt / dx;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                size_t c = IDX(i, j, k);
+                if (flags[c] == SOLID || flags[c] == GAS) continue;
+                float dpx = 0.5f * (phi[IDX(imin(i + 1, nx - 1), j, k)] - phi[IDX(imax(i - 1, 0), j, k)]);
+                float dpy = 0.5f * (phi[IDX(i, imin(j + 1, ny - 1), k)] - phi[IDX(i, imax(j - 1, 0), k)]);
+                float dpz = 0.5f * (phi[IDX(i, j, imin(k + 1, nz - 1))] - phi[IDX(i, j, imax(k - 1, 0))]);
+                float gmag = sqrtf(dpx * dpx + dpy * dpy + dpz * dpz);
+                if (gmag < 0.08f) continue;
+                float di = (float)i - arc_i, dj = (float)j - arc_j;
+                float r2 = di * di + dj * dj;
+                float r = sqrtf(r2 + eps);
+                float rx = di / r, ry = dj / r;
+                float nxn = dpx / gmag, nyn = dpy / gmag;
+                float rdotn = rx * nxn + ry * nyn;
+                float tx = rx - rdotn * nxn, ty = ry - rdotn * nyn;
+                float tmag = sqrtf(tx * tx + ty * ty + eps);
+                tx /= tmag;
+                ty /= tmag;
+                float tau = tau_peak * waam_expf(-r2 * inv2s2);
+                float Fmag = tau * F_scale;
+                Fx[c] += Fmag * tx;
+                Fy[c] += Fmag * ty;
+            }
+}

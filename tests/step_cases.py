@@ -29,6 +29,14 @@ def case_kwargs(name: str) -> dict:
     if name == "tables_mrt":
         # temperature-dependent cp, k, mu, dgamma/dT: variable-tau SRT collision, per-cell cp
         return dict(base, nx=64, ny=40, nz=32, dx=2e-4, droplet_freq_hz=0.0)
+    if name == "surface_forces_mrt":
+        # vapour recoil (Clausius-Clapeyron) and shielding-gas shear on the flow tier
+        return dict(base, nx=64, ny=40, nz=32, dx=2e-4, droplet_freq_hz=0.0, enable_recoil=True,
+                    enable_gas_shear=True)
+    if name == "recoil_const_srt":
+        return dict(base, nx=56, ny=40, nz=32, dx=2e-4, use_srt=True, droplet_freq_hz=2500.0,
+                    enable_recoil=True, use_recoil_clausius_clapeyron=False,
+                    enable_gas_shear=True)
     if name == "ctwd_srt":
         # droplets with stick-out preheat and the CTWD bead-height update every step
         return dict(base, nx=56, ny=40, nz=32, dx=2e-4, use_srt=True, droplet_freq_hz=2500.0,
@@ -45,7 +53,7 @@ def case_kwargs(name: str) -> dict:
 def configure(twin, name: str) -> None:
     twin.plate_thickness_mm = 1.6
     twin.travel_speed_m_s = 8e-3
-    if name in ("droplets_srt", "ctwd_srt"):
+    if name in ("droplets_srt", "ctwd_srt", "recoil_const_srt"):
         twin.wire_feed_m_s = 6.5 / 60.0
         twin.welding_current_A = 140.0
 
@@ -171,8 +179,12 @@ def oracle_like(tw, mode: str = "jacobi"):
                  "arc_pressure_model", "pressure_sigma_m", "plate_thickness_mm", "plate_size_mm",
                  "plate_origin_mm", "nz_solid", "enable_ctwd", "stickout_m", "ctwd_nominal_m",
                  "ctwd_m", "layer_height_m", "rho_e_ohm_m", "eta_stick", "enable_moving_window",
-                 "_window_offset_x_m"):
+                 "_window_offset_x_m", "enable_recoil", "recoil_pressure_pa", "T_boiling_K",
+                 "P_vapor_ref_Pa", "L_vapor_J_kg", "R_spec_vapor_J_kgK", "enable_gas_shear",
+                 "gas_jet_velocity_m_s", "gas_shear_coeff"):
         setattr(ot, name, getattr(tw, name))
+    ot.use_recoil_cc = tw.use_recoil_clausius_clapeyron
+    ot.recoil_accommodation = float(getattr(tw, "recoil_accommodation", 0.54))
     key = tw.heat_source_name.lower().replace("-", "").replace("_", "")
     if key in ("goldak", "goldak3d", "doubleellipsoid"):
         src = tw.arc_source

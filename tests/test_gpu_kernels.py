@@ -450,3 +450,33 @@ def test_shift_window(env, dims, n_shift):
                                           293.0, 5.3e6, 1.0, 3, *dims)
     for k in o:
         check(k, down(g[k]), o[k])
+
+
+@pytest.mark.parametrize("dims", DIMS)
+def test_full_tier_surface_forces(env, dims):
+    """Vapour recoil (both forms) and gas shear on a seeded state with T spread across the
+    boiling point."""
+    s = random_state(*dims, seed=31)
+    s["T"] = np.random.default_rng(3).uniform(2500.0, 4200.0, dims).astype(np.float32)
+    keys = ("Fx", "Fy", "Fz", "T", "phi", "flags")
+    g = {k: up(env, s[k]) for k in keys}
+    o = {k: s[k].copy() for k in keys}
+    K, O = env["K"], env["O"]
+    ai, aj, ak, sig = dims[0] / 2 + 0.3, dims[1] / 2 - 0.4, 6.0, 3.5
+    K.apply_vapor_recoil(g["Fz"], g["T"], g["phi"], g["flags"], ai, aj, ak, sig, 5000.0, 1780.0,
+                         2e-5, 2e-4, 7800.0, 1, 2)
+    O.or_apply_vapor_recoil(P(o["Fz"]), P(o["T"]), P(o["phi"]), P(o["flags"]), ai, aj, sig, 5000.0,
+                            1780.0, 2e-5, 2e-4, 7800.0, *dims)
+    check("Fz recoil", down(g["Fz"]), o["Fz"])
+    K.apply_vapor_recoil_clausius_clapeyron(g["Fz"], g["T"], g["phi"], g["flags"], ai, aj, ak, sig,
+                                            101325.0, 3100.0, 6e6, 450.0, 0.54, 2e-5, 2e-4,
+                                            7800.0, 1, 2)
+    O.or_apply_vapor_recoil_cc(P(o["Fz"]), P(o["T"]), P(o["phi"]), P(o["flags"]), ai, aj, sig,
+                               101325.0, 3100.0, 6e6, 450.0, 0.54, 2e-5, 2e-4, 7800.0, *dims)
+    check("Fz recoil CC", down(g["Fz"]), o["Fz"])
+    K.apply_gas_shear_stress(g["Fx"], g["Fy"], g["phi"], g["flags"], ai, aj, ak, sig, 115.2, 2e-5,
+                             2e-4, 7800.0, 1, 2)
+    O.or_apply_gas_shear_stress(P(o["Fx"]), P(o["Fy"]), P(o["phi"]), P(o["flags"]), ai, aj, sig,
+                                115.2, 2e-5, 2e-4, 7800.0, *dims)
+    check("Fx shear", down(g["Fx"]), o["Fx"])
+    check("Fy shear", down(g["Fy"]), o["Fy"])

@@ -31,7 +31,8 @@ def _assert_parity(res: dict, label: str) -> None:
 
 @pytest.mark.parametrize("engine,layout", [("reference", 1), ("fused", 1), ("fused", 0),
                                            ("reference", 0)])
-@pytest.mark.parametrize("case", ["bead_mrt", "droplets_srt", "ctwd_srt"])
+@pytest.mark.parametrize("case", ["bead_mrt", "droplets_srt", "ctwd_srt", "surface_forces_mrt",
+                                  "recoil_const_srt"])
 def test_coupled_step_parity(hip_device, case, engine, layout):
     """layout 1: fields stored (nz, ny, nx), i fastest; layout 0: the reference's (nx, ny, nz)."""
     tw, ot = build_pair(case, engine=engine, layout=layout)
@@ -40,7 +41,7 @@ def test_coupled_step_parity(hip_device, case, engine, layout):
         run_pair(tw, ot, n, start=done)
         done += n
         _assert_parity(compare(tw, ot), f"{case}/{engine}/layout{layout} step {done}")
-    if case in ("droplets_srt", "ctwd_srt"):
+    if case in ("droplets_srt", "ctwd_srt", "recoil_const_srt"):
         assert tw._n_droplets_fired == ot._n_droplets_fired and tw._n_droplets_fired > 0
         assert tw._deposited_volume_m3 == ot._deposited_volume_m3
     if case == "ctwd_srt":

@@ -637,3 +637,71 @@ def test_window_shift_is_a_pure_shift_with_reset_strip(n_shift):
         np.testing.assert_array_equal(d[n][strip], old[0][names.index(n)][strip])
     assert np.all(flags[strip][:, :, :2] == SOLID) and np.all(flags[strip][:, :, 2:] == GAS)
     assert np.all(d["phi"][strip][:, :, :2] == 1.0) and np.all(d["phi"][strip][:, :, 2:] == 0.0)
+
+
+# ---------------------------------------------------------------------------- full-tier forces
+def _surface_column(T_val, n=(9, 7, 6), kz=3):
+    """A FLUID layer at k <= kz under GAS, so |phi(k+1) - phi(k-1)| = 1 at the top metal cell."""
+    flags = np.full(n, GAS, np.uint8)
+    flags[:, :, :kz + 1] = FLUID
+    phi = np.where(flags == GAS, 0.0, 1.0).astype(np.float32)
+    T = np.full(n, T_val, np.float32)
+    return flags, phi, T
+
+
+@pytest.mark.parametrize("T_val", [1500.0, 2600.0, 9000.0])
+def test_constant_recoil_closed_form(T_val):
+    """Fz = -(p / (rho dx)) dt^2 / dx min(T / T_ref, 3) exp(-r^2 / (2 sigma^2)) at the surface
+    cells (|d phi / dz| >= 0.05), nothing elsewhere (kernels.py:995-1030)."""
+    flags, phi, T = _surface_column(T_val)
+    Fz = np.zeros(T.shape, np.float32)
+    p, T_ref, dt, dx, rho, sig, ai, aj = 5000.0, 1780.0, 2e-5, 2e-4, 7800.0, 3.0, 4.0, 3.0
+    lib().or_apply_vapor_recoil(ptr(Fz), ptr(T), ptr(phi), ptr(flags), ai, aj, sig, p, T_ref, dt,
+                                dx, rho, *T.shape)
+    i, j = np.meshgrid(np.arange(T.shape[0]), np.arange(T.shape[1]), indexing="ij")
+    r2 = (i - ai) ** 2 + (j - aj) ** 2
+    expect = -(p / (rho * dx)) * dt * dt / dx * min(T_val / T_ref, 3.0) * np.exp(-r2 / (2 * sig ** 2))
+    np.testing.assert_allclose(Fz[:, :, 3], expect, rtol=2e-6)
+    assert np.all(Fz[:, :, :3] == 0) and np.all(Fz[:, :, 4:] == 0)
+
+
+def test_clausius_clapeyron_recoil_closed_form():
+    """p = C_acc P_ref exp(min(L/R (1/T_b - 1/T), 12)) above the boiling point, zero at or below
+    it (kernels.py:1052-1090)."""
+    for T_val in (3000.0, 3100.0, 3400.0, 20000.0):
+        flags, phi, T = _surface_column(T_val)
+        Fz = np.zeros(T.shape, np.float32)
+        P_ref, Tb, L, R, C, dt, dx, rho, sig = 101325.0, 3100.0, 6e6, 450.0, 0.54, 2e-5, 2e-4, 7800.0, 3.0
+        lib().or_apply_vapor_recoil_cc(ptr(Fz), ptr(T), ptr(phi), ptr(flags), 4.0, 3.0, sig, P_ref,
+                                       Tb, L, R, C, dt, dx, rho, *T.shape)
+        if T_val <= Tb:
+            assert np.all(Fz == 0)
+            continue
+        P = C * P_ref * math.exp(min(L / R * (1 / Tb - 1 / T_val), 12.0))
+        F0 = P / (rho * dx) * dt * dt / dx
+        assert Fz[4, 3, 3] == pytest.approx(-F0, rel=2e-5)
+        assert Fz[4, 5, 3] == pytest.approx(-F0 * math.exp(-4 / 18), rel=2e-5)
+
+
+def test_gas_shear_is_tangential_and_radial():
+    """On a flat surface the shear is horizontal, points away from the arc axis and has magnitude
+    tau exp(-r^2 / (2 sigma^2)) / (rho dx) dt^2 / dx (kernels.py:1093-1139); cells with
+    |grad phi| < 0.08 get none."""
+    flags, phi, T = _surface_column(1800.0)
+    Fx, Fy = np.zeros(T.shape, np.float32), np.zeros(T.shape, np.float32)
+    tau, dt, dx, rho, sig, ai, aj = 115.2, 2e-5, 2e-4, 7800.0, 3.0, 4.0, 3.0
+    lib().or_apply_gas_shear_stress(ptr(Fx), ptr(Fy), ptr(phi), ptr(flags), ai, aj, sig, tau, dt,
+                                    dx, rho, *T.shape)
+    scale = 1.0 / (rho * dx) * dt * dt / dx
+    for i in range(T.shape[0]):
+        for j in range(T.shape[1]):
+            di, dj = i - ai, j - aj
+            r2 = di * di + dj * dj
+            mag = math.hypot(Fx[i, j, 3], Fy[i, j, 3])
+            assert mag == pytest.approx(tau * math.exp(-r2 / (2 * sig ** 2)) * scale *
+                                        math.sqrt(r2 / (r2 + 1e-6)) if r2 > 0 else mag,
+                                        rel=1e-4, abs=1e-30)
+            if r2 > 0:
+                assert Fx[i, j, 3] * di + Fy[i, j, 3] * dj > 0
+    # interior cells (grad phi = 0) and gas get nothing
+    assert np.all(Fx[:, :, :2] == 0) and np.all(Fx[:, :, 5:] == 0)

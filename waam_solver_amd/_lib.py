@@ -88,6 +88,9 @@ SIGNATURES: dict[str, str] = {
     "waam_solidify_trailing_pool": "pppppppppfffffffffiiiip",
     "waam_shift_simulation_window_x": "i" + "p" * 24 + "fffiiiiip",
     "waam_bead_top_k": "pipiiiip",
+    "waam_apply_vapor_recoil": "ppppfffffffffiiiip",
+    "waam_apply_vapor_recoil_clausius_clapeyron": "ppppffffffffffffiiiip",
+    "waam_apply_gas_shear_stress": "ppppffffffffiiiip",
 }
 
 # kernel ids of waam_step_fused for waam_timing_begin (include/waam_hip.h WAAM_K_*)

@@ -93,6 +93,40 @@ __global__ void k_arc_pressure(float *__restrict__ Fz, const uint8_t *__restrict
     if (arc_pressure_force(pzp, pzm, i, j, k, ap, fz)) Fz[c] += fz;
 }
 
+/* apply_vapor_recoil / apply_vapor_recoil_clausius_clapeyron — reference kernels.py:995-1030,
+ * 1052-1090 */
+__global__ void k_vapor_recoil(float *__restrict__ Fz, const float *__restrict__ T,
+                               const float *__restrict__ phi, const uint8_t *__restrict__ flags,
+                               Recoil r, Dims d)
+{
+    CELL3D
+    uint8_t f = flags[c];
+    if (f == SOLID || f == GAS) return;
+    float pzp = phi[d.idx(i, j, imin(k + 1, d.nz - 1))], pzm = phi[d.idx(i, j, imax(k - 1, 0))];
+    float fz;
+    if (recoil_force(pzp, pzm, T[c], i, j, r, fz)) Fz[c] += fz;
+}
+
+/* apply_gas_shear_stress — reference kernels.py:1093-1139 */
+__global__ void k_gas_shear(float *__restrict__ Fx, float *__restrict__ Fy,
+                            const float *__restrict__ phi, const uint8_t *__restrict__ flags,
+                            GasShear g, Dims d)
+{
+    CELL3D
+    uint8_t f = flags[c];
+    if (f == SOLID || f == GAS) return;
+    PhiStencil ps;
+    size_t o[6];
+    face_idx(i, j, k, d, o);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) ps.c[q] = phi[o[q]];
+    float fx, fy;
+    if (gas_shear_force(ps, i, j, g, fx, fy)) {
+        Fx[c] += fx;
+        Fy[c] += fy;
+    }
+}
+
 /* clamp_body_force_magnitude — reference kernels.py:3298-3309 */
 __global__ void k_clamp_force(float *__restrict__ Fx, float *__restrict__ Fy,
                               float *__restrict__ Fz, const uint8_t *__restrict__ flags,
@@ -206,6 +240,80 @@ int waam_apply_arc_pressure(float *Fz, const uint8_t *flags, const float *phi, f
     hipLaunchKernelGGL(k_arc_pressure, grid3(d), block3(), 0, as_stream(stream), Fz, flags, phi, ap,
                        d);
     return check_launch("waam_apply_arc_pressure");
+}
+
+static Recoil recoil_params(float arc_i, float arc_j, float sigma)
+{
+    Recoil r{};
+    r.arc_i = arc_i;
+    r.arc_j = arc_j;
+    r.inv2s2 = 1.0f / (2.0f * sigma * sigma + 1e-6f);
+    return r;
+}
+
+int waam_apply_vapor_recoil(float *Fz, const float *T, const float *phi, const uint8_t *flags,
+                            float arc_i, float arc_j, float arc_k, float sigma, float recoil_pa,
+                            float T_ref, float dt, float dx, float rho_ref, int nx, int ny,
+                            int nz, int layout, void *stream)
+{
+    (void)arc_k;
+    WAAM_RETURN_IF(check_dims("waam_apply_vapor_recoil", nx, ny, nz, layout));
+    WAAM_RETURN_IF(check_ptrs("waam_apply_vapor_recoil", {Fz, T, phi, flags}));
+    Dims d{nx, ny, nz, layout};
+    Recoil r = recoil_params(arc_i, arc_j, sigma);
+    r.clausius = 0;
+    r.F_peak_lu = recoil_pa / (rho_ref * dx) * dt * dt / dx;
+    r.T_ref = T_ref;
+    hipLaunchKernelGGL(k_vapor_recoil, grid3(d), block3(), 0, as_stream(stream), Fz, T, phi,
+                       flags, r, d);
+    return check_launch("waam_apply_vapor_recoil");
+}
+
+int waam_apply_vapor_recoil_clausius_clapeyron(float *Fz, const float *T, const float *phi,
+                                               const uint8_t *flags, float arc_i, float arc_j,
+                                               float arc_k, float sigma, float P_ref_Pa,
+                                               float T_boil_K, float L_vapor_J_kg,
+                                               float R_spec_J_kgK, float C_acc, float dt, float dx,
+                                               float rho_ref, int nx, int ny, int nz, int layout,
+                                               void *stream)
+{
+    (void)arc_k;
+    WAAM_RETURN_IF(check_dims("waam_apply_vapor_recoil_clausius_clapeyron", nx, ny, nz, layout));
+    WAAM_RETURN_IF(check_ptrs("waam_apply_vapor_recoil_clausius_clapeyron", {Fz, T, phi, flags}));
+    Dims d{nx, ny, nz, layout};
+    Recoil r = recoil_params(arc_i, arc_j, sigma);
+    r.clausius = 1;
+    r.P_ref = P_ref_Pa;
+    r.T_boil = T_boil_K;
+    r.L_vap = L_vapor_J_kg;
+    r.R_spec = R_spec_J_kgK;
+    r.C_acc = C_acc;
+    r.dt = dt;
+    r.dx = dx;
+    r.rho_ref = rho_ref;
+    hipLaunchKernelGGL(k_vapor_recoil, grid3(d), block3(), 0, as_stream(stream), Fz, T, phi,
+                       flags, r, d);
+    return check_launch("waam_apply_vapor_recoil_clausius_clapeyron");
+}
+
+int waam_apply_gas_shear_stress(float *Fx, float *Fy, const float *phi, const uint8_t *flags,
+                                float arc_i, float arc_j, float arc_k, float sigma,
+                                float tau_peak_pa, float dt, float dx, float rho_ref, int nx,
+                                int ny, int nz, int layout, void *stream)
+{
+    (void)arc_k;
+    WAAM_RETURN_IF(check_dims("waam_apply_gas_shear_stress", nx, ny, nz, layout));
+    WAAM_RETURN_IF(check_ptrs("waam_apply_gas_shear_stress", {Fx, Fy, phi, flags}));
+    Dims d{nx, ny, nz, layout};
+    GasShear g;
+    g.arc_i = arc_i;
+    g.arc_j = arc_j;
+    g.inv2s2 = 1.0f / (2.0f * sigma * sigma + 1e-6f);
+    g.tau_peak = tau_peak_pa;
+    g.F_scale = 1.0f / (rho_ref * dx) * dt * dt / dx;
+    hipLaunchKernelGGL(k_gas_shear, grid3(d), block3(), 0, as_stream(stream), Fx, Fy, phi, flags,
+                       g, d);
+    return check_launch("waam_apply_gas_shear_stress");
 }
 
 int waam_clamp_body_force_magnitude(float *Fx, float *Fy, float *Fz, const uint8_t *flags,

@@ -240,4 +240,68 @@ __device__ __forceinline__ bool arc_pressure_force(float phi_zp, float phi_zm, i
     return true;
 }
 
+/* vapour recoil on the free surface (reference kernels.py:995-1030, apply_vapor_recoil, and
+ * :1052-1090, apply_vapor_recoil_clausius_clapeyron); for a cell that is not SOLID/GAS */
+struct Recoil {
+    float arc_i, arc_j, inv2s2;
+    int clausius;          // 1: p = C_acc P_sat(T) above T_boil; 0: p_peak scaled by T/T_ref
+    float F_peak_lu;       // constant model: (p / (rho dx)) dt dt / dx
+    float T_ref;           // constant model
+    float P_ref, T_boil, L_vap, R_spec, C_acc, dt, dx, rho_ref;  // Clausius-Clapeyron model
+};
+
+__device__ __forceinline__ bool recoil_force(float phi_zp, float phi_zm, float T, int i, int j,
+                                             const Recoil &r, float &fz)
+{
+    const float eps = 1e-6f;
+    float dphi_z = phi_zp - phi_zm;
+    if (fabsf(dphi_z) < 0.05f) return false;
+    float di = (float)i - r.arc_i, dj = (float)j - r.arc_j;
+    float r2 = di * di + dj * dj;
+    if (r.clausius) {
+        if (T <= r.T_boil) return false;
+        float expo = (r.L_vap / (r.R_spec + eps)) * (1.0f / (r.T_boil + eps) - 1.0f / (T + eps));
+        expo = fmn(expo, 12.0f);
+        float P_vap = r.C_acc * r.P_ref * waam_expf(expo);
+        float F_peak = P_vap / (r.rho_ref * r.dx) * r.dt * r.dt / r.dx;
+        fz = -F_peak * waam_expf(-r2 * r.inv2s2);
+        return true;
+    }
+    float t_ratio = fmn(T / (r.T_ref + eps), 3.0f);
+    fz = -r.F_peak_lu * t_ratio * waam_expf(-r2 * r.inv2s2);
+    return true;
+}
+
+/* shielding-gas shear along the surface, away from the arc axis (reference kernels.py:1093-1139,
+ * apply_gas_shear_stress); for a cell that is not SOLID/GAS */
+struct GasShear {
+    float arc_i, arc_j, inv2s2, tau_peak, F_scale;  // F_scale = (1 / (rho dx)) dt dt / dx
+};
+
+__device__ __forceinline__ bool gas_shear_force(const PhiStencil &ps, int i, int j,
+                                                const GasShear &g, float &fx, float &fy)
+{
+    const float eps = 1e-6f;
+    float dpx = 0.5f * (ps.c[0] - ps.c[1]);
+    float dpy = 0.5f * (ps.c[2] - ps.c[3]);
+    float dpz = 0.5f * (ps.c[4] - ps.c[5]);
+    float gmag = sqrtf(dpx * dpx + dpy * dpy + dpz * dpz);
+    if (gmag < 0.08f) return false;
+    float di = (float)i - g.arc_i, dj = (float)j - g.arc_j;
+    float r2 = di * di + dj * dj;
+    float r = sqrtf(r2 + eps);
+    float rx = di / r, ry = dj / r;
+    float nx_n = dpx / gmag, ny_n = dpy / gmag;
+    float rdotn = rx * nx_n + ry * ny_n;
+    float tx = rx - rdotn * nx_n, ty = ry - rdotn * ny_n;
+    float tmag = sqrtf(tx * tx + ty * ty + eps);
+    tx /= tmag;
+    ty /= tmag;
+    float tau = g.tau_peak * waam_expf(-r2 * g.inv2s2);
+    float Fmag = tau * g.F_scale;
+    fx = Fmag * tx;
+    fy = Fmag * ty;
+    return true;
+}
+
 }  // namespace waam

@@ -80,6 +80,8 @@ struct StepConsts {
     PhaseConsts pc;
     LossConsts lc;
     ArcPressure ap;
+    Recoil rc;
+    GasShear gs;
     float cap_l, cap_s, h_min, inv_dt;
     int box_i0, box_i1, box_j0, box_j1;  // cells outside get exactly zero arc heat
     int seg_len;                         // cells per list segment
@@ -639,10 +641,16 @@ __device__ __forceinline__ void collide_cell(const waam_step_fields &f, const St
         fy += ty;
         fz += tz;
     }
+    if (p.is_welding && p.enable_gas_shear && gas_shear_force(fs.phi, i, j, s.gs, tx, ty)) {
+        fx += tx;
+        fy += ty;
+    }
     if (p.is_welding) {
         ArcPressure ap = s.ap;
         ap.arc_k = arc_k;
         if (arc_pressure_force(fs.phi.c[4], fs.phi.c[5], i, j, k, ap, tz)) fz += tz;
+        if (p.enable_recoil && recoil_force(fs.phi.c[4], fs.phi.c[5], T_c, i, j, s.rc, tz))
+            fz += tz;
     }
     if (p.enable_hydro && fl > 0.0f) fz += -1.0f * p.g_lu * fl;
     if (fl > 0.0f) fz += 1.0f * p.g_lu * p.beta * (T_c - p.T_l) * fl;
@@ -934,6 +942,28 @@ int waam_step_fused(const waam_step_params *pp, const waam_step_fields *ff, void
     s.ap.arc_j = p.arc_j;
     s.ap.inv2s2 = 1.0f / (2.0f * p.p_sigma * p.p_sigma + 1e-6f);
     s.ap.F_peak_lu = p.p_pa / (p.rho_phys * p.dx) * (p.dt * p.dt) / p.dx;
+    /* recoil and gas shear around the arc axis with the arc's sigma (reference
+     * physics/weld_forces.py:128-161) */
+    s.rc = Recoil{};
+    s.rc.arc_i = p.arc_i;
+    s.rc.arc_j = p.arc_j;
+    s.rc.inv2s2 = 1.0f / (2.0f * p.sigma * p.sigma + 1e-6f);
+    s.rc.clausius = p.recoil_clausius;
+    s.rc.F_peak_lu = p.recoil_pa / (p.rho_phys * p.dx) * p.dt * p.dt / p.dx;
+    s.rc.T_ref = p.recoil_T_ref;
+    s.rc.P_ref = p.P_vap_ref;
+    s.rc.T_boil = p.T_boil;
+    s.rc.L_vap = p.L_vap;
+    s.rc.R_spec = p.R_vap;
+    s.rc.C_acc = p.C_acc;
+    s.rc.dt = p.dt;
+    s.rc.dx = p.dx;
+    s.rc.rho_ref = p.rho_phys;
+    s.gs.arc_i = p.arc_i;
+    s.gs.arc_j = p.arc_j;
+    s.gs.inv2s2 = s.rc.inv2s2;
+    s.gs.tau_peak = p.gas_tau;
+    s.gs.F_scale = 1.0f / (p.rho_phys * p.dx) * p.dt * p.dt / p.dx;
     arc_box(s);
     dim3 g3 = grid3(d);
     size_t seg = (d.n() + kListSegments - 1) / kListSegments;

@@ -604,3 +604,36 @@ def bead_top_k(flags, nz_solid, out) -> None:
     nx, ny, nz = dims_of(flags)
     _call_ro("waam_bead_top_k", P(flags), int(nz_solid), P(out), nx, ny, nz, layout_of(flags),
              _stream())
+
+
+# ------------------------------------------------------------------ full-tier surface forces
+def apply_vapor_recoil(Fz, T, phi, flags, arc_i, arc_j, arc_k, sigma, recoil_pa, T_ref, dt, dx,
+                       rho_ref, FLAG_SOLID, FLAG_GAS):
+    """Reference kernels.py:995-1030."""
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
+    nx, ny, nz = dims_of(Fz)
+    _need(nx * ny * nz, T, phi, flags)
+    _call("waam_apply_vapor_recoil", P(Fz), P(T), P(phi), P(flags), arc_i, arc_j, arc_k, sigma,
+          recoil_pa, T_ref, dt, dx, rho_ref, nx, ny, nz, layout_of(Fz), _stream())
+
+
+def apply_vapor_recoil_clausius_clapeyron(Fz, T, phi, flags, arc_i, arc_j, arc_k, sigma, P_ref_Pa,
+                                          T_boil_K, L_vapor_J_kg, R_spec_J_kgK, C_acc, dt, dx,
+                                          rho_ref, FLAG_SOLID, FLAG_GAS):
+    """Reference kernels.py:1052-1090."""
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
+    nx, ny, nz = dims_of(Fz)
+    _need(nx * ny * nz, T, phi, flags)
+    _call("waam_apply_vapor_recoil_clausius_clapeyron", P(Fz), P(T), P(phi), P(flags), arc_i,
+          arc_j, arc_k, sigma, P_ref_Pa, T_boil_K, L_vapor_J_kg, R_spec_J_kgK, C_acc, dt, dx,
+          rho_ref, nx, ny, nz, layout_of(Fz), _stream())
+
+
+def apply_gas_shear_stress(Fx, Fy, phi, flags, arc_i, arc_j, arc_k, sigma, tau_peak_pa, dt, dx,
+                           rho_ref, FLAG_SOLID, FLAG_GAS):
+    """Reference kernels.py:1093-1139."""
+    _flags_ok(FLAG_SOLID=FLAG_SOLID, FLAG_GAS=FLAG_GAS)
+    nx, ny, nz = dims_of(Fx)
+    _need(nx * ny * nz, Fy, phi, flags)
+    _call("waam_apply_gas_shear_stress", P(Fx), P(Fy), P(phi), P(flags), arc_i, arc_j, arc_k,
+          sigma, tau_peak_pa, dt, dx, rho_ref, nx, ny, nz, layout_of(Fx), _stream())

@@ -90,3 +90,41 @@ def apply_droplet_impact(twin, g, arc_i, arc_j, arc_k, drop_r, drop_mass_kg=None
             g.Fz, g.flags, g.phi, g.f_l, arc_i, arc_j, arc_k, drop_r,
             droplet_impact_pressure_pa(twin, drop_mass_kg, drop_r * g.dx), g.dt, g.dx,
             twin.mat.rho, g.FLAG_SOLID, g.FLAG_GAS)
+
+
+def gas_shear_tau_pa(twin) -> float:
+    """Shielding-gas shear 0.5 rho_Ar v^2 c, capped at 5 kPa (reference
+    physics/weld_forces.py:120-125)."""
+    rho_gas = 1.6  # argon, kg/m^3
+    v = twin.gas_jet_velocity_m_s
+    return min(0.5 * rho_gas * v * v * twin.gas_shear_coeff, 5000.0)
+
+
+def apply_recoil(twin, g, arc_i: float, arc_j: float, arc_k: float) -> None:
+    """Vapour recoil, Clausius-Clapeyron or constant-pressure form (reference
+    physics/weld_forces.py:128-148)."""
+    from .. import kernels
+
+    if not twin.enable_recoil:
+        return
+    if twin.use_recoil_clausius_clapeyron:
+        kernels.apply_vapor_recoil_clausius_clapeyron(
+            g.Fz, g.T, g.phi, g.flags, arc_i, arc_j, arc_k, twin.sigma_cells, twin.P_vapor_ref_Pa,
+            twin.T_boiling_K, twin.L_vapor_J_kg, twin.R_spec_vapor_J_kgK,
+            float(getattr(twin, "recoil_accommodation", 0.54)), g.dt, g.dx, twin.mat.rho,
+            g.FLAG_SOLID, g.FLAG_GAS)
+    else:
+        kernels.apply_vapor_recoil(g.Fz, g.T, g.phi, g.flags, arc_i, arc_j, arc_k,
+                                   twin.sigma_cells, twin.recoil_pressure_pa, twin.mat.T_liquidus,
+                                   g.dt, g.dx, twin.mat.rho, g.FLAG_SOLID, g.FLAG_GAS)
+
+
+def apply_gas_shear(twin, g, arc_i: float, arc_j: float, arc_k: float) -> None:
+    """Reference physics/weld_forces.py:151-161."""
+    from .. import kernels
+
+    if not twin.enable_gas_shear:
+        return
+    kernels.apply_gas_shear_stress(g.Fx, g.Fy, g.phi, g.flags, arc_i, arc_j, arc_k,
+                                   twin.sigma_cells, gas_shear_tau_pa(twin), g.dt, g.dx,
+                                   twin.mat.rho, g.FLAG_SOLID, g.FLAG_GAS)

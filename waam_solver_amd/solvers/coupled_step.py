@@ -201,13 +201,12 @@ def coupled_step(twin, torch_x_m: float, torch_y_m: float, is_welding: bool,
         kernels.compute_marangoni_force(g.T, g.phi, g.f_l, g.Fx, g.Fy, g.Fz, g.flags,
                                         twin.dgamma_dT_lu, g.dx, g.FLAG_SOLID, g.FLAG_GAS, *n)
     if is_welding and twin.enable_gas_shear:
-        raise NotImplementedError("gas shear (physics_tier: full) is not built")
+        weld_forces.apply_gas_shear(twin, g, arc_i, arc_j, arc_k)
     if is_welding:
         kernels.apply_arc_pressure(g.Fz, g.flags, g.phi, arc_i, arc_j, arc_k,
                                    pressure_sigma_cells, current_pressure, g.dt, g.dx,
                                    twin.mat.rho, g.FLAG_SOLID, g.FLAG_GAS)
-        if twin.enable_recoil:
-            raise NotImplementedError("vapour recoil (physics_tier: full) is not built")
+        weld_forces.apply_recoil(twin, g, arc_i, arc_j, arc_k)
     if twin.enable_hydrostatic_gravity:
         kernels.add_hydrostatic_gravity(g.Fz, g.f_l, g.flags, 1.0, twin.g_lu, g.FLAG_SOLID,
                                         g.FLAG_GAS)

@@ -44,13 +44,15 @@ _INT_PARAMS = ("is_welding", "collision", "arc_kind", "arc_applied", "pending_st
                "enable_heat_loss", "enable_conv", "enable_rad", "enable_cap", "enable_hydro",
                "enable_freeze", "enable_trailing", "zero_velocity", "surface_weight", "nz_solid",
                "col_i", "col_j", "use_arc_k_override", "use_k_robot", "force_reset_all",
-               "thermal_list_ready")
+               "thermal_list_ready", "enable_recoil", "recoil_clausius", "enable_gas_shear")
 _FLOAT_PARAMS = ("arc_i", "arc_j", "arc_k_override", "k_robot", "Q_w", "eta", "dt", "dx", "dx3",
                  "sigma", "pen", "g_sgn", "g_ff", "g_fr", "g_af", "g_ar", "g_b", "g_c",
                  "alpha_lu", "cp_rho", "L_rho", "T_s", "T_l", "T_cap", "T_amb", "h_conv",
                  "eps_rad", "sigma_sb", "T800", "T1100", "gamma_lu", "cos_t", "sin_t", "dgamma",
                  "p_sigma", "p_pa", "rho_phys", "g_lu", "beta", "F_cap", "u_cap", "omega_s",
-                 "omega_b", "C_darcy", "dir_x", "dir_y", "dir_z", "lookback", "T_freeze")
+                 "omega_b", "C_darcy", "dir_x", "dir_y", "dir_z", "lookback", "T_freeze",
+                 "recoil_pa", "recoil_T_ref", "P_vap_ref", "T_boil", "L_vap", "R_vap", "C_acc",
+                 "gas_tau")
 
 
 class StepParams(ctypes.Structure):
@@ -196,6 +198,14 @@ class FusedStepper:
         p.dir_x, p.dir_y, p.dir_z = dir_x, dir_y, dir_z
         p.lookback = max(2.0, tw.trailing_solidify_lookback_mm / (g.dx * 1000.0))
         p.T_freeze = m.T_liquidus + tw.trailing_solidify_temp_margin_K
+        p.enable_recoil = 1 if tw.enable_recoil else 0
+        p.recoil_clausius = 1 if tw.use_recoil_clausius_clapeyron else 0
+        p.enable_gas_shear = 1 if tw.enable_gas_shear else 0
+        p.recoil_pa, p.recoil_T_ref = tw.recoil_pressure_pa, m.T_liquidus
+        p.P_vap_ref, p.T_boil = tw.P_vapor_ref_Pa, tw.T_boiling_K
+        p.L_vap, p.R_vap = tw.L_vapor_J_kg, tw.R_spec_vapor_J_kgK
+        p.C_acc = float(getattr(tw, "recoil_accommodation", 0.54))
+        p.gas_tau = weld_forces.gas_shear_tau_pa(tw)
         return p
 
     def step(self, torch_x_m: float, torch_y_m: float, is_welding: bool,
@@ -203,8 +213,8 @@ class FusedStepper:
         tw, g = self.twin, self.twin.grid
         if tw.use_material_tables:
             raise NotImplementedError("temperature-dependent material tables are not built")
-        if tw.enable_gas_shear or tw.enable_recoil or tw.enable_lorentz:
-            raise NotImplementedError("physics_tier: full forces are not built")
+        if tw.enable_lorentz:
+            raise NotImplementedError("the Lorentz force solve is not built")
         arc_i = torch_x_m / g.dx
         arc_j = torch_y_m / g.dx
         p = self._params(arc_i, arc_j, is_welding, torch_z_m)
