@@ -310,6 +310,49 @@ int waam_apply_gas_shear_stress(float *Fx, float *Fy, const float *phi, const ui
                                 float tau_peak_pa, float dt, float dx, float rho_ref, int nx,
                                 int ny, int nz, int layout, void *stream);
 
+/* ---- Lorentz force of the welding current: kernels.py:1236-1513, driven by
+ * physics/weld_forces.py:196-281 (electric potential by weighted Jacobi, plate grounded) --- */
+/* elec_build_sigma kernels.py:1236-1251: 1e-8 in GAS, sigma_liquid where f_l > 0.55 */
+int waam_elec_build_sigma(float *sigma, const float *f_l, const uint8_t *flags,
+                          float sigma_liquid, float sigma_solid, int64_t n, void *stream);
+/* elec_clear_source kernels.py:1254-1257 */
+int waam_elec_clear_source(float *source, int64_t n, void *stream);
+/* elec_inject_arc_source kernels.py:1260-1286 (cells of the slab range only) */
+int waam_elec_inject_arc_source(float *source, float arc_i, float arc_j, float arc_k,
+                                float sigma_cells, float current_A, float dx, int nx, int ny,
+                                int nz, int layout, void *stream);
+/* elec_normalize_source kernels.py:1289-1301: source *= I / max(sum source dx^3, 1e-12) */
+int waam_elec_normalize_source(float *source, float target_current_A, float dx, int64_t n,
+                               void *scratch, void *stream);
+/* elec_init_ground kernels.py:1304-1312: phi = 0 */
+int waam_elec_init_ground(float *phi, int64_t n, void *stream);
+/* elec_jacobi_step kernels.py:1315-1356 */
+int waam_elec_jacobi_step(const float *phi_in, float *phi_out, const float *sigma,
+                          const float *source, const uint8_t *flags, float dx, float omega_j,
+                          int nz_solid, int nx, int ny, int nz, int layout, void *stream);
+/* elec_l1_diff kernels.py:1384-1396: out (device, 2 floats) = (sum |a - b|, sum |b|) */
+int waam_elec_l1_diff(const float *a, const float *b, float *out, int64_t n, void *scratch,
+                      void *stream);
+/* elec_compute_J kernels.py:1359-1381 */
+int waam_elec_compute_J(float *Jx, float *Jy, float *Jz, const float *phi, const float *sigma,
+                        const uint8_t *flags, float dx, int nx, int ny, int nz, int layout,
+                        void *stream);
+/* elec_bin_axial_current kernels.py:1399-1418: bins is nz x n_bins fp32 (slab-major) */
+int waam_elec_bin_axial_current(const float *Jz, float *bins, float arc_i, float arc_j, float dx,
+                                float bin_dr_cells, int n_bins, int nx, int ny, int nz,
+                                int layout, void *stream);
+/* elec_prefix_bins kernels.py:1421-1432 */
+int waam_elec_prefix_bins(float *bins, int n_bins, int nz, void *stream);
+/* elec_B_axisymmetric kernels.py:1435-1477 */
+int waam_elec_B_axisymmetric(float *Bx, float *By, float *Bz, const float *bins, float arc_i,
+                             float arc_j, float dx, float bin_dr_cells, float mu0, int n_bins,
+                             int nx, int ny, int nz, int layout, void *stream);
+/* apply_lorentz_JxB kernels.py:1480-1513 */
+int waam_apply_lorentz_JxB(float *Fx, float *Fy, float *Fz, const float *Jx, const float *Jy,
+                           const float *Jz, const float *Bx, const float *By, const float *Bz,
+                           const float *f_l, const uint8_t *flags, float dt, float dx,
+                           float rho_ref, int64_t n, void *stream);
+
 /* ---- moving window: shift_simulation_window_x kernels.py:2620-2723 ------------------------ */
 /* Every listed field (both population buffers included) becomes new[i] = old[i + n_shift] for
  * i < nx - n_shift (the reference's ascending serial loop), and the last n_shift columns are

@@ -116,6 +116,16 @@ _SIGS = {
     "or_apply_vapor_recoil": [_P] * 4 + [_F] * 8 + [_I, _I, _I],
     "or_apply_vapor_recoil_cc": [_P] * 4 + [_F] * 11 + [_I, _I, _I],
     "or_apply_gas_shear_stress": [_P] * 4 + [_F] * 7 + [_I, _I, _I],
+    "or_elec_build_sigma": [_P, _P, _P, _F, _F, _I, _I, _I],
+    "or_elec_inject_arc_source": [_P] + [_F] * 6 + [_I, _I, _I],
+    "or_elec_normalize_source": [_P, _F, _F, _I, _I, _I],
+    "or_elec_jacobi_step": [_P] * 5 + [_F, _F, _I, _I, _I, _I],
+    "or_elec_l1_diff": [_P, _P, _P, _I, _I, _I],
+    "or_elec_compute_J": [_P] * 6 + [_F, _I, _I, _I],
+    "or_elec_bin_axial_current": [_P, _P, _F, _F, _F, _F, _I, _I, _I, _I],
+    "or_elec_prefix_bins": [_P, _I, _I],
+    "or_elec_B_axisymmetric": [_P] * 4 + [_F] * 5 + [_I, _I, _I, _I],
+    "or_apply_lorentz_JxB": [_P] * 11 + [_F, _F, _F, _I, _I, _I],
 }
 
 
@@ -297,7 +307,8 @@ class OracleTwin:
                  use_recoil_clausius_clapeyron=True, recoil_pressure_pa=5000.0,
                  enable_gas_shear=False, gas_jet_velocity_m_s=12.0, gas_shear_coeff=1.0,
                  T_boiling_K=3100.0, P_vapor_ref_Pa=101325.0, L_vapor_J_kg=6.0e6,
-                 R_spec_vapor_J_kgK=450.0):
+                 R_spec_vapor_J_kgK=450.0, enable_lorentz=False, sigma_liquid_Sm=1.0e6,
+                 sigma_solid_Sm=1.0e5, lorentz_jacobi_iters=40, lorentz_jacobi_omega=0.85):
         self.mat = material
         self.lib = lib(parallel)
         self.mode = mode
@@ -341,6 +352,12 @@ class OracleTwin:
         self.T_boiling_K, self.P_vapor_ref_Pa = T_boiling_K, P_vapor_ref_Pa
         self.L_vapor_J_kg, self.R_spec_vapor_J_kgK = L_vapor_J_kg, R_spec_vapor_J_kgK
         self.enable_gas_shear = enable_gas_shear
+        self.enable_lorentz = enable_lorentz
+        self.sigma_liquid_Sm, self.sigma_solid_Sm = sigma_liquid_Sm, sigma_solid_Sm
+        self.lorentz_jacobi_iters, self.lorentz_jacobi_omega = lorentz_jacobi_iters, lorentz_jacobi_omega
+        self.lorentz_jacobi_tol = 1e-4
+        self._lorentz_warm = False
+        self._lorentz_unconverged = 0
         self.gas_jet_velocity_m_s, self.gas_shear_coeff = gas_jet_velocity_m_s, gas_shear_coeff
         # stick-out and CTWD (twin.py:186-191; physics/electrical_stickout.py:15-59)
         self.stickout_m, self.ctwd_nominal_m, self.layer_height_m = 12e-3, 15e-3, 1.2e-3
@@ -442,6 +459,7 @@ class OracleTwin:
         g.tracer_head[0] = 0
         L.or_stream(ptr(g.f_a), ptr(g.f_b), ptr(g.flags), *g.dims)
         self._reset_counters()
+        self._lorentz_warm = False
         g.deposit_vol[0] = 0
         g.deposit_real[0] = 0
 
@@ -677,6 +695,62 @@ class OracleTwin:
             self._maybe_shift_window(torch_x_m)
         self.coupled_step(torch_x_m - self._window_offset_x_m, torch_y_m, is_welding)
 
+    # ---------------------------------------------------------------- Lorentz (weld_forces.py:196-281)
+    def solve_lorentz(self, arc_i, arc_j, arc_k):
+        """Conductivity, normalised Gaussian current source, warm-started weighted Jacobi for
+        the potential (plate grounded; a cold start allows 2 n_max^2 sweeps) stopped when the
+        relative L1 change of the last sweep of a 10-sweep chunk is below 1e-4, J, enclosed
+        axial current per radius and slab, B_theta = mu0 I / (2 pi r), J x B on the melt."""
+        g, L = self.g, self.lib
+        d = g.dims
+        s3 = d
+        if not hasattr(g, "phi_elec"):
+            for n in ("phi_elec", "phi_elec_tmp", "sigma_elec", "elec_source", "Jx", "Jy", "Jz",
+                      "Bx", "By", "Bz"):
+                setattr(g, n, np.zeros(s3, np.float32))
+            g.n_elec_bins = 64
+            g.elec_rad_bins = np.zeros((g.nz, 64), np.float32)
+        L.or_elec_build_sigma(ptr(g.sigma_elec), ptr(g.f_l), ptr(g.flags), self.sigma_liquid_Sm,
+                              self.sigma_solid_Sm, *d)
+        g.elec_source.fill(0)
+        L.or_elec_inject_arc_source(ptr(g.elec_source), arc_i, arc_j, arc_k, self.sigma_cells,
+                                    self.welding_current_A, g.dx, *d)
+        L.or_elec_normalize_source(ptr(g.elec_source), self.welding_current_A, g.dx, *d)
+        cold = not self._lorentz_warm
+        if cold:
+            g.phi_elec.fill(0)
+            self._lorentz_warm = True
+        max_iters = max(int(self.lorentz_jacobi_iters), 20)
+        if cold:
+            n_max = max(g.nx, g.ny, g.nz)
+            max_iters = max(max_iters, 2 * n_max * n_max)
+        out = np.zeros(2, np.float32)
+        done, converged = 0, False
+        while done < max_iters:
+            for _ in range(min(10, max_iters - done)):
+                L.or_elec_jacobi_step(ptr(g.phi_elec), ptr(g.phi_elec_tmp), ptr(g.sigma_elec),
+                                      ptr(g.elec_source), ptr(g.flags), g.dx,
+                                      self.lorentz_jacobi_omega, self.nz_solid, *d)
+                L.or_elec_l1_diff(ptr(g.phi_elec_tmp), ptr(g.phi_elec), ptr(out), *d)
+                g.phi_elec, g.phi_elec_tmp = g.phi_elec_tmp, g.phi_elec
+                done += 1
+            if float(out[0]) / max(float(out[1]), 1e-30) < self.lorentz_jacobi_tol:
+                converged = True
+                break
+        if not converged:
+            self._lorentz_unconverged += 1
+        L.or_elec_compute_J(ptr(g.Jx), ptr(g.Jy), ptr(g.Jz), ptr(g.phi_elec), ptr(g.sigma_elec),
+                            ptr(g.flags), g.dx, *d)
+        dr = max(float(max(g.nx, g.ny)) / (2.0 * g.n_elec_bins), 0.5)
+        L.or_elec_bin_axial_current(ptr(g.Jz), ptr(g.elec_rad_bins), arc_i, arc_j, g.dx, dr,
+                                    g.n_elec_bins, *d)
+        L.or_elec_prefix_bins(ptr(g.elec_rad_bins), g.n_elec_bins, g.nz)
+        L.or_elec_B_axisymmetric(ptr(g.Bx), ptr(g.By), ptr(g.Bz), ptr(g.elec_rad_bins), arc_i,
+                                 arc_j, g.dx, dr, 4.0e-7 * math.pi, g.n_elec_bins, *d)
+        L.or_apply_lorentz_JxB(ptr(g.Fx), ptr(g.Fy), ptr(g.Fz), ptr(g.Jx), ptr(g.Jy), ptr(g.Jz),
+                               ptr(g.Bx), ptr(g.By), ptr(g.Bz), ptr(g.f_l), ptr(g.flags), g.dt,
+                               g.dx, self.mat.rho, *d)
+
     # ---------------------------------------------------------------- moving window (twin.py:599-626)
     WINDOW_FIELDS = ("T", "H", "f_l", "phi", "T_max", "T_prev", "dT_dt", "time_above_800_s",
                      "time_above_1100_s", "time_above_solidus_s", "rho", "ux", "uy", "uz", "Fx",
@@ -810,6 +884,8 @@ class OracleTwin:
             L.or_add_hydrostatic_gravity(ptr(g.Fz), ptr(g.f_l), ptr(g.flags), 1.0, self.g_lu, *d)
         L.or_add_buoyancy(ptr(g.T), ptr(g.Fz), ptr(g.f_l), ptr(g.flags), self.g_lu, self.beta_T,
                           m.T_liquidus, 1.0, *d)
+        if is_welding and self.enable_lorentz:
+            self.solve_lorentz(arc_i, arc_j, arc_k)
         if self.force_limit_lu > 0.0:
             L.or_clamp_body_force_magnitude(ptr(g.Fx), ptr(g.Fy), ptr(g.Fz), ptr(g.flags),
                                             self.force_limit_lu, *d)

@@ -1954,3 +1954,199 @@ void or_apply_gas_shear_stress(float *Fx, float *Fy, const float *phi, const uin
                 Fy[c] += Fmag * ty;
             }
 }
+
+/* ------------------------------------------------------------------------------------------
+ * Lorentz force — elec_build_sigma kernels.py:1236-1251, elec_inject_arc_source :1260-1286,
+ * elec_normalize_source :1289-1301, elec_jacobi_step :1315-1356, elec_compute_J :1359-1381,
+ * elec_l1_diff :1384-1396, elec_bin_axial_current :1399-1418, elec_prefix_bins :1421-1432,
+ * elec_B_axisymmetric :1435-1477, apply_lorentz_JxB :1480-1513. The reference's float atomic
+ * sums are taken as the correctly rounded fp32 of the exact sum (Neumaier-compensated fp64).
+ * ---------------------------------------------------------------------------------------- */
+typedef struct {
+    double s, c;
+} or_sum;
+
+static inline void or_sum_add(or_sum *a, double x)
+{
+    double t = a->s + x;
+    if (fabs(a->s) >= fabs(x)) a->c += (a->s - t) + x;
+    else a->c += (x - t) + a->s;
+    a->s = t;
+}
+
+void or_elec_build_sigma(float *sigma, const float *f_l, const uint8_t *flags, float s_liq,
+                         float s_sol, int nx, int ny, int nz)
+{
+    size_t N = NCELL;
+    for (size_t c = 0; c < N; ++c)
+        sigma[c] = flags[c] == GAS ? 1e-8f : (f_l[c] > 0.55f ? s_liq : s_sol);
+}
+
+void or_elec_inject_arc_source(float *source, float arc_i, float arc_j, float arc_k,
+                               float sigma_cells, float current_A, float dx, int nx, int ny, int nz)
+{
+    const float eps = 1e-6f;
+    float inv2s2 = 1.0f / (2.0f * sigma_cells * sigma_cells + eps);
+    float cell_vol = dx * dx * dx;
+    int k_max = imin(nz, (int)(arc_k + sigma_cells * 4.0f) + 1);
+    int k_min = imax(0, (int)(arc_k - sigma_cells));
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                if (k < k_min || k > k_max) continue;
+                float di = (float)i - arc_i, dj = (float)j - arc_j, dk = (float)k - arc_k;
+                float r2 = di * di + dj * dj + 0.25f * dk * dk;
+                float w = waam_expf(-r2 * inv2s2);
+                source[IDX(i, j, k)] = current_A * w / (cell_vol + eps);
+            }
+}
+
+void or_elec_normalize_source(float *source, float target, float dx, int nx, int ny, int nz)
+{
+    size_t N = NCELL;
+    float cell_vol = dx * dx * dx;
+    or_sum t = {0.0, 0.0};
+    for (size_t c = 0; c < N; ++c) or_sum_add(&t, (double)(source[c] * cell_vol));
+    float total = (float)(t.s + t.c);
+    float scale = target / fmaxf_(total, 1e-12f);
+    for (size_t c = 0; c < N; ++c) source[c] *= scale;
+}
+
+static inline float or_sigma_at(const float *s, int i, int j, int k, int nx, int ny, int nz)
+{
+    return s[IDX(imax(0, imin(nx - 1, i)), imax(0, imin(ny - 1, j)), imax(0, imin(nz - 1, k)))];
+}
+
+void or_elec_jacobi_step(const float *phi_in, float *phi_out, const float *sigma,
+                         const float *source, const uint8_t *flags, float dx, float omega,
+                         int nz_solid, int nx, int ny, int nz)
+{
+    const float eps = 1e-12f;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                size_t c = IDX(i, j, k);
+                if (flags[c] == GAS || (k <= nz_solid && flags[c] == SOLID)) {
+                    phi_out[c] = 0.0f;
+                    continue;
+                }
+                float sp = or_sigma_at(sigma, i + 1, j, k, nx, ny, nz);
+                float sm = or_sigma_at(sigma, i - 1, j, k, nx, ny, nz);
+                float sn = or_sigma_at(sigma, i, j + 1, k, nx, ny, nz);
+                float ss = or_sigma_at(sigma, i, j - 1, k, nx, ny, nz);
+                float st = or_sigma_at(sigma, i, j, k + 1, nx, ny, nz);
+                float sb = or_sigma_at(sigma, i, j, k - 1, nx, ny, nz);
+                float denom = sp + sm + sn + ss + st + sb + eps;
+                float pn = sp * phi_in[IDX(imin(i + 1, nx - 1), j, k)] +
+                           sm * phi_in[IDX(imax(i - 1, 0), j, k)] +
+                           sn * phi_in[IDX(i, imin(j + 1, ny - 1), k)] +
+                           ss * phi_in[IDX(i, imax(j - 1, 0), k)] +
+                           st * phi_in[IDX(i, j, imin(k + 1, nz - 1))] +
+                           sb * phi_in[IDX(i, j, imax(k - 1, 0))];
+                float phi_new = (pn + source[c] * dx * dx) / denom;
+                phi_out[c] = (1.0f - omega) * phi_in[c] + omega * phi_new;
+            }
+}
+
+void or_elec_l1_diff(const float *a, const float *b, float *out, int nx, int ny, int nz)
+{
+    size_t N = NCELL;
+    or_sum d = {0.0, 0.0}, n = {0.0, 0.0};
+    for (size_t c = 0; c < N; ++c) {
+        or_sum_add(&d, (double)fabsf(a[c] - b[c]));
+        or_sum_add(&n, (double)fabsf(b[c]));
+    }
+    out[0] = (float)(d.s + d.c);
+    out[1] = (float)(n.s + n.c);
+}
+
+void or_elec_compute_J(float *Jx, float *Jy, float *Jz, const float *phi, const float *sigma,
+                       const uint8_t *flags, float dx, int nx, int ny, int nz)
+{
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                size_t c = IDX(i, j, k);
+                if (flags[c] == GAS) {
+                    Jx[c] = Jy[c] = Jz[c] = 0.0f;
+                    continue;
+                }
+                float sig = sigma[c];
+                float gx = (phi[IDX(imin(i + 1, nx - 1), j, k)] - phi[IDX(imax(i - 1, 0), j, k)]) / (2.0f * dx);
+                float gy = (phi[IDX(i, imin(j + 1, ny - 1), k)] - phi[IDX(i, imax(j - 1, 0), k)]) / (2.0f * dx);
+                float gz = (phi[IDX(i, j, imin(k + 1, nz - 1))] - phi[IDX(i, j, imax(k - 1, 0))]) / (2.0f * dx);
+                Jx[c] = -sig * gx;
+                Jy[c] = -sig * gy;
+                Jz[c] = -sig * gz;
+            }
+}
+
+void or_elec_bin_axial_current(const float *Jz, float *bins, float arc_i, float arc_j, float dx,
+                               float dr, int n_bins, int nx, int ny, int nz)
+{
+    or_sum *acc = (or_sum *)calloc((size_t)nz * n_bins, sizeof(or_sum));
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j) {
+            float di = (float)i - arc_i, dj = (float)j - arc_j;
+            float r = sqrtf(di * di + dj * dj);
+            int b = imin((int)(r / dr), n_bins - 1);
+            for (int k = 0; k < nz; ++k)
+                or_sum_add(&acc[(size_t)k * n_bins + b], (double)(Jz[IDX(i, j, k)] * dx * dx));
+        }
+    for (size_t t = 0; t < (size_t)nz * n_bins; ++t) bins[t] = (float)(acc[t].s + acc[t].c);
+    free(acc);
+}
+
+void or_elec_prefix_bins(float *bins, int n_bins, int nz)
+{
+    for (int k = 0; k < nz; ++k) {
+        float acc = 0.0f;
+        for (int b = 0; b < n_bins; ++b) {
+            acc += bins[(size_t)k * n_bins + b];
+            bins[(size_t)k * n_bins + b] = acc;
+        }
+    }
+}
+
+void or_elec_B_axisymmetric(float *Bx, float *By, float *Bz, const float *bins, float arc_i,
+                            float arc_j, float dx, float dr, float mu0, int n_bins, int nx,
+                            int ny, int nz)
+{
+    const float two_pi = 2.0f * 3.14159265358979323846f;
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                size_t c = IDX(i, j, k);
+                float di = (float)i - arc_i, dj = (float)j - arc_j;
+                float r_cells = sqrtf(di * di + dj * dj);
+                Bx[c] = By[c] = Bz[c] = 0.0f;
+                if (r_cells > 1e-3f) {
+                    int b = imin((int)(r_cells / dr), n_bins - 1);
+                    float I_enc = bins[(size_t)k * n_bins + b];
+                    float r_m = r_cells * dx;
+                    float B_theta = mu0 * I_enc / (two_pi * r_m);
+                    Bx[c] = -B_theta * dj / r_cells;
+                    By[c] = B_theta * di / r_cells;
+                }
+            }
+}
+
+void or_apply_lorentz_JxB(float *Fx, float *Fy, float *Fz, const float *Jx, const float *Jy,
+                          const float *Jz, const float *Bx, const float *By, const float *Bz,
+                          const float *f_l, const uint8_t *flags, float dt, float dx, float rho,
+                          int nx, int ny, int nz)
+{
+    size_t N = NCELL;
+    for (size_t c = 0; c < N; ++c) {
+        if (flags[c] == GAS || f_l[c] < 0.05f) continue;
+        float fx = Jy[c] * Bz[c] - Jz[c] * By[c];
+        float fy = Jz[c] * Bx[c] - Jx[c] * Bz[c];
+        float fz = Jx[c] * By[c] - Jy[c] * Bx[c];
+        float flw = fminf_(1.0f, f_l[c]);
+        float scale = flw * dt * dt / (rho * dx);
+        Fx[c] += fx * scale;
+        Fy[c] += fy * scale;
+        Fz[c] += fz * scale;
+    }
+}

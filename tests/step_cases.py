@@ -37,6 +37,11 @@ def case_kwargs(name: str) -> dict:
         return dict(base, nx=56, ny=40, nz=32, dx=2e-4, use_srt=True, droplet_freq_hz=2500.0,
                     enable_recoil=True, use_recoil_clausius_clapeyron=False,
                     enable_gas_shear=True)
+    if name == "full_tier_mrt":
+        # the full physics tier: Lorentz force (Jacobi potential, warm-started), gas shear,
+        # Clausius-Clapeyron recoil, Lin-Eagar arc pressure
+        return dict(base, nx=40, ny=32, nz=24, dx=2e-4, droplet_freq_hz=0.0, enable_recoil=True,
+                    enable_gas_shear=True, enable_lorentz=True)
     if name == "ctwd_srt":
         # droplets with stick-out preheat and the CTWD bead-height update every step
         return dict(base, nx=56, ny=40, nz=32, dx=2e-4, use_srt=True, droplet_freq_hz=2500.0,
@@ -53,6 +58,8 @@ def case_kwargs(name: str) -> dict:
 def configure(twin, name: str) -> None:
     twin.plate_thickness_mm = 1.6
     twin.travel_speed_m_s = 8e-3
+    if name == "full_tier_mrt":
+        twin.arc_pressure_model = "lin_eagar"
     if name in ("droplets_srt", "ctwd_srt", "recoil_const_srt"):
         twin.wire_feed_m_s = 6.5 / 60.0
         twin.welding_current_A = 140.0
@@ -181,7 +188,8 @@ def oracle_like(tw, mode: str = "jacobi"):
                  "ctwd_m", "layer_height_m", "rho_e_ohm_m", "eta_stick", "enable_moving_window",
                  "_window_offset_x_m", "enable_recoil", "recoil_pressure_pa", "T_boiling_K",
                  "P_vapor_ref_Pa", "L_vapor_J_kg", "R_spec_vapor_J_kgK", "enable_gas_shear",
-                 "gas_jet_velocity_m_s", "gas_shear_coeff"):
+                 "gas_jet_velocity_m_s", "gas_shear_coeff", "enable_lorentz", "sigma_liquid_Sm",
+                 "sigma_solid_Sm", "lorentz_jacobi_iters", "lorentz_jacobi_omega"):
         setattr(ot, name, getattr(tw, name))
     ot.use_recoil_cc = tw.use_recoil_clausius_clapeyron
     ot.recoil_accommodation = float(getattr(tw, "recoil_accommodation", 0.54))

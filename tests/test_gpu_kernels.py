@@ -480,3 +480,69 @@ def test_full_tier_surface_forces(env, dims):
                                 115.2, 2e-5, 2e-4, 7800.0, *dims)
     check("Fx shear", down(g["Fx"]), o["Fx"])
     check("Fy shear", down(g["Fy"]), o["Fy"])
+
+
+@pytest.mark.parametrize("dims", DIMS)
+def test_lorentz_kernels(env, dims):
+    """Every Lorentz entry on seeded fields, in solve order."""
+    s = random_state(*dims, seed=41)
+    rng = np.random.default_rng(41)
+    K, O = env["K"], env["O"]
+    n = dims
+    g = {k: up(env, s[k]) for k in ("f_l", "flags", "Fx", "Fy", "Fz")}
+    o = {k: s[k].copy() for k in g}
+    z = lambda: np.zeros(n, np.float32)  # noqa: E731
+    for name in ("sigma", "src", "Jx", "Jy", "Jz", "Bx", "By", "Bz"):
+        o[name] = z()
+        g[name] = up(env, o[name])
+    o["phi"] = (rng.standard_normal(n) * 1e-3).astype(np.float32)
+    g["phi"] = up(env, o["phi"])
+    o["phi2"] = z()
+    g["phi2"] = up(env, o["phi2"])
+    K.elec_build_sigma(g["sigma"], g["f_l"], g["flags"], 1e6, 1e5, 2)
+    O.or_elec_build_sigma(P(o["sigma"]), P(o["f_l"]), P(o["flags"]), 1e6, 1e5, *n)
+    check("sigma", down(g["sigma"]), o["sigma"])
+    ai, aj, ak, sig, dx = n[0] / 2 + 0.3, n[1] / 2 - 0.2, 5.5, 2.5, 2e-4
+    K.elec_clear_source(g["src"])
+    K.elec_inject_arc_source(g["src"], ai, aj, ak, sig, 140.0, dx, n[2])
+    O.or_elec_inject_arc_source(P(o["src"]), ai, aj, ak, sig, 140.0, dx, *n)
+    check("source", down(g["src"]), o["src"])
+    K.elec_normalize_source(g["src"], 140.0, dx)
+    O.or_elec_normalize_source(P(o["src"]), 140.0, dx, *n)
+    check("source normalised", down(g["src"]), o["src"])
+    K.elec_jacobi_step(g["phi"], g["phi2"], g["sigma"], g["src"], g["flags"], dx, 0.85, 2, 1, 4,
+                       *n)
+    O.or_elec_jacobi_step(P(o["phi"]), P(o["phi2"]), P(o["sigma"]), P(o["src"]), P(o["flags"]),
+                          dx, 0.85, 4, *n)
+    check("phi jacobi", down(g["phi2"]), o["phi2"])
+    out_g = up(env, np.zeros(2, np.float32))
+    out_o = np.zeros(2, np.float32)
+    K.elec_l1_diff_into(g["phi2"], g["phi"], out_g)
+    O.or_elec_l1_diff(P(o["phi2"]), P(o["phi"]), P(out_o), *n)
+    check("l1", down(out_g), out_o)
+    K.elec_compute_J(g["Jx"], g["Jy"], g["Jz"], g["phi2"], g["sigma"], g["flags"], dx, 2, *n)
+    O.or_elec_compute_J(P(o["Jx"]), P(o["Jy"]), P(o["Jz"]), P(o["phi2"]), P(o["sigma"]),
+                        P(o["flags"]), dx, *n)
+    for k in ("Jx", "Jy", "Jz"):
+        check(k, down(g[k]), o[k])
+    nb, dr = 64, max(max(n[0], n[1]) / 128.0, 0.5)
+    bins_o = np.zeros((n[2], nb), np.float32)
+    bins_g = up(env, bins_o)
+    K.elec_bin_axial_current(g["Jz"], bins_g, ai, aj, dx, dr, nb, n[2])
+    O.or_elec_bin_axial_current(P(o["Jz"]), P(bins_o), ai, aj, dx, dr, nb, *n)
+    check("bins", down(bins_g), bins_o)
+    K.elec_prefix_bins(bins_g, nb, n[2])
+    O.or_elec_prefix_bins(P(bins_o), nb, n[2])
+    check("bins prefix", down(bins_g), bins_o)
+    mu0 = 4e-7 * math.pi
+    K.elec_B_axisymmetric(g["Bx"], g["By"], g["Bz"], bins_g, ai, aj, dx, dr, mu0, nb, n[2])
+    O.or_elec_B_axisymmetric(P(o["Bx"]), P(o["By"]), P(o["Bz"]), P(bins_o), ai, aj, dx, dr, mu0,
+                             nb, *n)
+    for k in ("Bx", "By", "Bz"):
+        check(k, down(g[k]), o[k])
+    K.apply_lorentz_JxB(g["Fx"], g["Fy"], g["Fz"], g["Jx"], g["Jy"], g["Jz"], g["Bx"], g["By"],
+                        g["Bz"], g["f_l"], g["flags"], 2e-5, dx, 7800.0, 2)
+    O.or_apply_lorentz_JxB(*(P(o[k]) for k in ("Fx", "Fy", "Fz", "Jx", "Jy", "Jz", "Bx", "By",
+                                                 "Bz", "f_l", "flags")), 2e-5, dx, 7800.0, *n)
+    for k in ("Fx", "Fy", "Fz"):
+        check(k, down(g[k]), o[k])

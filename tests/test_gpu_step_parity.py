@@ -32,7 +32,7 @@ def _assert_parity(res: dict, label: str) -> None:
 @pytest.mark.parametrize("engine,layout", [("reference", 1), ("fused", 1), ("fused", 0),
                                            ("reference", 0)])
 @pytest.mark.parametrize("case", ["bead_mrt", "droplets_srt", "ctwd_srt", "surface_forces_mrt",
-                                  "recoil_const_srt"])
+                                  "recoil_const_srt", "full_tier_mrt"])
 def test_coupled_step_parity(hip_device, case, engine, layout):
     """layout 1: fields stored (nz, ny, nx), i fastest; layout 0: the reference's (nx, ny, nz)."""
     tw, ot = build_pair(case, engine=engine, layout=layout)

@@ -705,3 +705,107 @@ def test_gas_shear_is_tangential_and_radial():
                 assert Fx[i, j, 3] * di + Fy[i, j, 3] * dj > 0
     # interior cells (grad phi = 0) and gas get nothing
     assert np.all(Fx[:, :, :2] == 0) and np.all(Fx[:, :, 5:] == 0)
+
+
+# ---------------------------------------------------------------------------- Lorentz force
+def _elec_grid(n=(14, 12, 10), nzs=3):
+    flags = np.full(n, FLUID, np.uint8)
+    flags[:, :, :nzs] = SOLID
+    flags[:, :, n[2] - 2:] = GAS
+    f_l = np.where(flags == FLUID, 1.0, 0.0).astype(np.float32)
+    return flags, f_l
+
+
+def test_current_source_carries_the_welding_current():
+    """After normalisation, sum(source) dx^3 equals the welding current (closed form)."""
+    flags, f_l = _elec_grid()
+    src = np.zeros(flags.shape, np.float32)
+    dx = 2e-4
+    lib().or_elec_inject_arc_source(ptr(src), 7.0, 6.0, 7.0, 2.0, 140.0, dx, *flags.shape)
+    assert np.all(src[:, :, :2] == 0)               # below arc_k - sigma: untouched
+    lib().or_elec_normalize_source(ptr(src), 140.0, dx, *flags.shape)
+    assert float(np.sum(src.astype(np.float64)) * dx ** 3) == pytest.approx(140.0, rel=1e-5)
+
+
+def test_jacobi_converges_to_the_discrete_conduction_equation():
+    """The fixed point of the weighted Jacobi sweep solves sum_f sigma_f (phi_f - phi) +
+    s dx^2 = 0 on every non-grounded metal cell (the defining discrete equation), with phi = 0 on
+    GAS and on SOLID at k <= nz_solid."""
+    flags, f_l = _elec_grid()
+    n = flags.shape
+    L = lib()
+    sigma = np.zeros(n, np.float32)
+    L.or_elec_build_sigma(ptr(sigma), ptr(f_l), ptr(flags), 1e6, 1e5, *n)
+    assert sigma[0, 0, 0] == np.float32(1e5) and sigma[5, 5, 5] == np.float32(1e6)
+    assert sigma[0, 0, n[2] - 1] == np.float32(1e-8)
+    src = np.zeros(n, np.float32)
+    dx = 2e-4
+    L.or_elec_inject_arc_source(ptr(src), 7.0, 6.0, 7.0, 2.0, 140.0, dx, *n)
+    L.or_elec_normalize_source(ptr(src), 140.0, dx, *n)
+    a, b = np.zeros(n, np.float32), np.zeros(n, np.float32)
+    for _ in range(6000):
+        L.or_elec_jacobi_step(ptr(a), ptr(b), ptr(sigma), ptr(src), ptr(flags), dx, 0.85, 3, *n)
+        a, b = b, a
+    s = sigma.astype(np.float64)
+    phi = a.astype(np.float64)
+    pad = lambda x: np.pad(x, 1, mode="edge")  # noqa: E731  clamped neighbours
+    sp, pp = pad(s), pad(phi)
+    res = np.zeros(n)
+    denom = np.zeros(n)
+    for sl in ((slice(2, None), slice(1, -1), slice(1, -1)), (slice(0, -2), slice(1, -1), slice(1, -1)),
+               (slice(1, -1), slice(2, None), slice(1, -1)), (slice(1, -1), slice(0, -2), slice(1, -1)),
+               (slice(1, -1), slice(1, -1), slice(2, None)), (slice(1, -1), slice(1, -1), slice(0, -2))):
+        res += sp[sl] * (pp[sl] - phi)
+        denom += sp[sl]
+    res += src * dx * dx
+    k = np.arange(n[2])[None, None, :]
+    active = (flags != GAS) & ~((k <= 3) & (flags == SOLID))
+    assert np.all(phi[~active] == 0)
+    scale = np.max(np.abs(src * dx * dx))
+    assert np.max(np.abs(res[active] / denom[active])) < 1e-4 * scale / np.max(denom)
+    out = np.zeros(2, np.float32)
+    L.or_elec_jacobi_step(ptr(a), ptr(b), ptr(sigma), ptr(src), ptr(flags), dx, 0.85, 3, *n)
+    L.or_elec_l1_diff(ptr(b), ptr(a), ptr(out), *n)
+    assert out[0] / out[1] < 1e-6 and out[1] == pytest.approx(np.abs(a).astype(np.float64).sum(), rel=1e-6)
+
+
+def test_current_density_and_self_field_closed_forms():
+    """J = -sigma grad(phi) for a linear potential; for a uniform axial current density the
+    enclosed current of each radial bin is Jz dx^2 times the cells inside it, and
+    B_theta = mu0 I_enc / (2 pi r) circles the axis; J x B points inward (pinch)."""
+    n = (16, 16, 4)
+    flags = np.full(n, FLUID, np.uint8)
+    f_l = np.ones(n, np.float32)
+    sigma = np.full(n, 2.0e5, np.float32)
+    dx = 2e-4
+    k = np.arange(n[2], dtype=np.float32)[None, None, :]
+    phi = np.broadcast_to(-0.01 * k, n).astype(np.float32).copy()
+    J = [np.zeros(n, np.float32) for _ in range(3)]
+    L = lib()
+    L.or_elec_compute_J(*(ptr(x) for x in J), ptr(phi), ptr(sigma), ptr(flags), dx, *n)
+    # interior k: central difference of -0.01 k is -0.01 per cell
+    np.testing.assert_allclose(J[2][:, :, 1:3], 2.0e5 * 0.01 / dx, rtol=1e-6)
+    assert np.all(J[0] == 0) and np.all(J[1] == 0)
+    bins = np.zeros((n[2], 64), np.float32)
+    dr = 0.5
+    L.or_elec_bin_axial_current(ptr(J[2]), ptr(bins), 7.5, 7.5, dx, dr, 64, *n)
+    ii, jj = np.meshgrid(np.arange(16), np.arange(16), indexing="ij")
+    r = np.sqrt((ii - 7.5) ** 2 + (jj - 7.5) ** 2)
+    b = np.minimum((r / dr).astype(int), 63)
+    cnt = np.bincount(b.ravel(), minlength=64)
+    np.testing.assert_allclose(bins[1], J[2][0, 0, 1] * dx * dx * cnt, rtol=1e-6)
+    L.or_elec_prefix_bins(ptr(bins), 64, n[2])
+    np.testing.assert_allclose(bins[1], np.cumsum(J[2][0, 0, 1] * dx * dx * cnt), rtol=1e-5)
+    B = [np.zeros(n, np.float32) for _ in range(3)]
+    mu0 = 4e-7 * math.pi
+    L.or_elec_B_axisymmetric(*(ptr(x) for x in B), ptr(bins), 7.5, 7.5, dx, dr, mu0, 64, *n)
+    i, j = 12, 7
+    rr = math.hypot(i - 7.5, j - 7.5)
+    Bt = mu0 * bins[1, min(int(rr / dr), 63)] / (2 * math.pi * rr * dx)
+    assert B[0][i, j, 1] == pytest.approx(-Bt * (j - 7.5) / rr, rel=1e-5)
+    assert B[1][i, j, 1] == pytest.approx(Bt * (i - 7.5) / rr, rel=1e-5)
+    F = [np.zeros(n, np.float32) for _ in range(3)]
+    L.or_apply_lorentz_JxB(*(ptr(x) for x in F), *(ptr(x) for x in J), *(ptr(x) for x in B),
+                           ptr(f_l), ptr(flags), 2e-5, dx, 7800.0, *n)
+    # current along +z with B circling +z: the force points towards the axis
+    assert F[0][i, j, 1] * (i - 7.5) + F[1][i, j, 1] * (j - 7.5) < 0

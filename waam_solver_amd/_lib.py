@@ -91,6 +91,18 @@ SIGNATURES: dict[str, str] = {
     "waam_apply_vapor_recoil": "ppppfffffffffiiiip",
     "waam_apply_vapor_recoil_clausius_clapeyron": "ppppffffffffffffiiiip",
     "waam_apply_gas_shear_stress": "ppppffffffffiiiip",
+    "waam_elec_build_sigma": "pppfflp",
+    "waam_elec_clear_source": "plp",
+    "waam_elec_inject_arc_source": "pffffffiiiip",
+    "waam_elec_normalize_source": "pfflpp",
+    "waam_elec_init_ground": "plp",
+    "waam_elec_jacobi_step": "pppppffiiiiip",
+    "waam_elec_l1_diff": "ppplpp",
+    "waam_elec_compute_J": "ppppppfiiiip",
+    "waam_elec_bin_axial_current": "ppffffiiiiip",
+    "waam_elec_prefix_bins": "piip",
+    "waam_elec_B_axisymmetric": "ppppfffffiiiiip",
+    "waam_apply_lorentz_JxB": "pppppppppppffflp",
 }
 
 # kernel ids of waam_step_fused for waam_timing_begin (include/waam_hip.h WAAM_K_*)

@@ -127,6 +127,21 @@ class WAAMGrid:
         self._has_vof_buf = True
         self._n_optional_volume += 1
 
+    def ensure_lorentz_fields(self) -> None:
+        """Potential (and its Jacobi partner), conductivity, current source, J, B, the radial
+        current bins (nz x 64) and the two L1 sums (reference grid.py:235-254)."""
+        if getattr(self, "_has_lorentz", False):
+            return
+        for name in ("phi_elec", "phi_elec_tmp", "sigma_elec", "elec_source", "Jx", "Jy", "Jz",
+                     "Bx", "By", "Bz"):
+            setattr(self, name, self.field(torch.float32))
+        self.n_elec_bins = 64
+        self.elec_rad_bins = DeviceField.zeros((self.nz, self.n_elec_bins), torch.float32,
+                                               self.device)
+        self.elec_l1_buf = DeviceField.zeros((2,), torch.float32, self.device)
+        self._has_lorentz = True
+        self._n_optional_volume += 10
+
     def ensure_export_buffers(self) -> None:
         if self._has_export:
             return

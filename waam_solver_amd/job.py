@@ -1,8 +1,8 @@
 """Job files: torch path, plate/domain and process settings (reference job.py:9-404).
 
 Covers what ``WAAMTwin.from_job`` and ``run_path`` need on the flow path (scalar or table materials):
-physics tiers (thermal | flow; ``full`` needs the Lorentz/gas-shear/recoil kernels, which are not
-built and raise), the process, deposition, wetting, arc, heat-loss and interpass sections.
+physics tiers (thermal | flow | full), the process, deposition, wetting, arc, heat-loss,
+interpass and calibration sections.
 """
 
 from __future__ import annotations
@@ -93,8 +93,20 @@ def apply_physics_tier(twin, tier: str) -> None:
         twin.enable_droplet_impact_pressure = True
         twin.arc_pressure_model = "constant"
         return
-    raise NotImplementedError(f"physics_tier '{tier}' needs the Lorentz/gas-shear kernels, "
-                              "which are not built; use 'flow' or 'thermal'")
+    if key != "full":
+        log.warning(f"[job] unknown physics_tier '{tier}'; using 'full' (thermal | flow | full)")
+    # full: flow tier + Lorentz force + gas shear + Lin-Eagar arc pressure; recoil stays as the
+    # job sets it (reference job.py:104-123)
+    twin.enable_vof = True
+    twin.grid.ensure_vof_buffers()
+    twin.enable_csf_tension = twin.enable_wetting = True
+    twin.enable_hydrostatic_gravity = twin.enable_bead_freeze = True
+    twin.enable_lorentz = True
+    twin.grid.ensure_lorentz_fields()
+    twin.enable_gas_shear = True
+    twin.enable_droplet_impact_pressure = True
+    twin.arc_pressure_model = "lin_eagar"
+    twin.physics_tier = "full"
 
 
 def apply_job_to_twin(twin, job: dict[str, Any]) -> None:
@@ -117,6 +129,14 @@ def apply_job_to_twin(twin, job: dict[str, Any]) -> None:
             setattr(twin, attr, conv(process[key]))
     if process.get("travel_speed_mm_s") is not None:
         twin.travel_speed_m_s = float(process["travel_speed_mm_s"]) / 1000.0
+    adv = job.get("advanced_physics", {}) or {}
+    for key, cast in (("gas_jet_velocity_m_s", float), ("gas_shear_coeff", float),
+                      ("sigma_liquid_Sm", float), ("sigma_solid_Sm", float),
+                      ("lorentz_jacobi_iters", int), ("T_boiling_K", float),
+                      ("L_vapor_J_kg", float), ("R_spec_vapor_J_kgK", float),
+                      ("recoil_accommodation", float)):
+        if key in adv:
+            setattr(twin, key, cast(adv[key]))
     sim = job.get("simulation", {}) or {}
     if "physics_tier" in sim:
         apply_physics_tier(twin, str(sim["physics_tier"]))

@@ -637,3 +637,100 @@ def apply_gas_shear_stress(Fx, Fy, phi, flags, arc_i, arc_j, arc_k, sigma, tau_p
     _need(nx * ny * nz, Fy, phi, flags)
     _call("waam_apply_gas_shear_stress", P(Fx), P(Fy), P(phi), P(flags), arc_i, arc_j, arc_k,
           sigma, tau_peak_pa, dt, dx, rho_ref, nx, ny, nz, layout_of(Fx), _stream())
+
+
+# ------------------------------------------------------------------ Lorentz force
+def _scalar_out(buf):
+    """Device address of a 0-d/1-element result buffer."""
+    return P(buf)
+
+
+def elec_build_sigma(sigma, f_l, flags, sigma_liquid, sigma_solid, FLAG_GAS):
+    """Reference kernels.py:1236-1251."""
+    _flags_ok(FLAG_GAS=FLAG_GAS)
+    _need(_n(sigma), f_l, flags)
+    _call("waam_elec_build_sigma", P(sigma), P(f_l), P(flags), sigma_liquid, sigma_solid,
+          _n(sigma), _stream())
+
+
+def elec_clear_source(source):
+    _call("waam_elec_clear_source", P(source), _n(source), _stream())
+
+
+def elec_inject_arc_source(source, arc_i, arc_j, arc_k, sigma_cells, current_A, dx, nz):
+    """Reference kernels.py:1260-1286."""
+    nx, ny, nz_ = dims_of(source)
+    if nz != nz_:
+        raise ValueError("nz does not match the source field")
+    _call("waam_elec_inject_arc_source", P(source), arc_i, arc_j, arc_k, sigma_cells, current_A,
+          dx, nx, ny, nz, layout_of(source), _stream())
+
+
+def elec_normalize_source(source, target_current_A, dx, scratch=None):
+    """Reference kernels.py:1289-1301."""
+    sc = scratch if scratch is not None else _scratch_of(source)
+    _call("waam_elec_normalize_source", P(source), target_current_A, dx, _n(source), P(sc),
+          _stream())
+
+
+def elec_init_ground(phi, flags, nz_solid, FLAG_SOLID):
+    """Reference kernels.py:1304-1312 (phi = 0 everywhere)."""
+    _flags_ok(FLAG_SOLID=FLAG_SOLID)
+    _call("waam_elec_init_ground", P(phi), _n(phi), _stream())
+
+
+def elec_jacobi_step(phi_in, phi_out, sigma, source, flags, dx, omega_j, FLAG_GAS, FLAG_SOLID,
+                     nz_solid, nx, ny, nz):
+    """Reference kernels.py:1315-1356."""
+    _flags_ok(FLAG_GAS=FLAG_GAS, FLAG_SOLID=FLAG_SOLID)
+    _need(nx * ny * nz, phi_in, phi_out, sigma, source, flags)
+    _call("waam_elec_jacobi_step", P(phi_in), P(phi_out), P(sigma), P(source), P(flags), dx,
+          omega_j, int(nz_solid), nx, ny, nz, layout_of(phi_in), _stream())
+
+
+def elec_l1_diff_into(a, b, out, scratch=None):
+    """out[0] = sum |a - b|, out[1] = sum |b| (reference kernels.py:1384-1396)."""
+    sc = scratch if scratch is not None else _scratch_of(a)
+    _call_ro("waam_elec_l1_diff", P(a), P(b), P(out), _n(a), P(sc), _stream())
+
+
+def elec_compute_J(Jx, Jy, Jz, phi, sigma, flags, dx, FLAG_GAS, nx, ny, nz):
+    """Reference kernels.py:1359-1381."""
+    _flags_ok(FLAG_GAS=FLAG_GAS)
+    _need(nx * ny * nz, Jx, Jy, Jz, phi, sigma, flags)
+    _call("waam_elec_compute_J", P(Jx), P(Jy), P(Jz), P(phi), P(sigma), P(flags), dx, nx, ny, nz,
+          layout_of(Jz), _stream())
+
+
+def elec_bin_axial_current(Jz, bins, arc_i, arc_j, dx, bin_dr_cells, n_bins, nz):
+    """Reference kernels.py:1399-1418; bins is (nz, n_bins)."""
+    nx, ny, nz_ = dims_of(Jz)
+    if nz != nz_ or _n(bins) < nz * n_bins:
+        raise ValueError("bins must hold nz x n_bins values")
+    _call("waam_elec_bin_axial_current", P(Jz), P(bins), arc_i, arc_j, dx, bin_dr_cells,
+          int(n_bins), nx, ny, nz, layout_of(Jz), _stream())
+
+
+def elec_prefix_bins(bins, n_bins, nz):
+    """Reference kernels.py:1421-1432."""
+    if _n(bins) < nz * n_bins:
+        raise ValueError("bins must hold nz x n_bins values")
+    _call("waam_elec_prefix_bins", P(bins), int(n_bins), int(nz), _stream())
+
+
+def elec_B_axisymmetric(Bx, By, Bz, bins, arc_i, arc_j, dx, bin_dr_cells, mu0, n_bins, nz):
+    """Reference kernels.py:1435-1477."""
+    nx, ny, nz_ = dims_of(Bx)
+    if nz != nz_ or _n(bins) < nz * n_bins:
+        raise ValueError("bins must hold nz x n_bins values")
+    _need(nx * ny * nz, By, Bz)
+    _call("waam_elec_B_axisymmetric", P(Bx), P(By), P(Bz), P(bins), arc_i, arc_j, dx,
+          bin_dr_cells, mu0, int(n_bins), nx, ny, nz, layout_of(Bx), _stream())
+
+
+def apply_lorentz_JxB(Fx, Fy, Fz, Jx, Jy, Jz, Bx, By, Bz, f_l, flags, dt, dx, rho_ref, FLAG_GAS):
+    """Reference kernels.py:1480-1513."""
+    _flags_ok(FLAG_GAS=FLAG_GAS)
+    _need(_n(Fx), Fy, Fz, Jx, Jy, Jz, Bx, By, Bz, f_l, flags)
+    _call("waam_apply_lorentz_JxB", P(Fx), P(Fy), P(Fz), P(Jx), P(Jy), P(Jz), P(Bx), P(By), P(Bz),
+          P(f_l), P(flags), dt, dx, rho_ref, _n(Fx), _stream())

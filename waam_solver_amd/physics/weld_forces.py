@@ -128,3 +128,76 @@ def apply_gas_shear(twin, g, arc_i: float, arc_j: float, arc_k: float) -> None:
     kernels.apply_gas_shear_stress(g.Fx, g.Fy, g.phi, g.flags, arc_i, arc_j, arc_k,
                                    twin.sigma_cells, gas_shear_tau_pa(twin), g.dt, g.dx,
                                    twin.mat.rho, g.FLAG_SOLID, g.FLAG_GAS)
+
+
+def solve_lorentz(twin, g, arc_i: float, arc_j: float, arc_k: float) -> None:
+    """J x B force of the welding current (reference physics/weld_forces.py:196-281).
+
+    Conductivity from the phase state, a Gaussian current source normalised to the welding
+    current, the potential by weighted Jacobi with the plate grounded (warm-started from the
+    previous step; a cold start may run 2 n_max^2 sweeps), checked every 10 sweeps against the
+    relative L1 change; then J = -sigma grad(phi), the enclosed axial current per radius and
+    slab, the axisymmetric B_theta and the force on the melt.
+    """
+    from .. import kernels
+
+    if not twin.enable_lorentz:
+        return
+    g.ensure_lorentz_fields()
+    kernels.elec_build_sigma(g.sigma_elec, g.f_l, g.flags, twin.sigma_liquid_Sm,
+                             twin.sigma_solid_Sm, g.FLAG_GAS)
+    kernels.elec_clear_source(g.elec_source)
+    kernels.elec_inject_arc_source(g.elec_source, arc_i, arc_j, arc_k, twin.sigma_cells,
+                                   twin.welding_current_A, g.dx, g.nz)
+    kernels.elec_normalize_source(g.elec_source, twin.welding_current_A, g.dx, scratch=g.scratch)
+    cold_start = not getattr(twin, "_lorentz_warm", False)
+    if cold_start:
+        kernels.elec_init_ground(g.phi_elec, g.flags, twin.nz_solid, g.FLAG_SOLID)
+        twin._lorentz_warm = True
+    max_iters = max(int(twin.lorentz_jacobi_iters), 20)
+    if cold_start:
+        n_max = max(g.nx, g.ny, g.nz)
+        max_iters = max(max_iters, 2 * n_max * n_max)
+    tol = float(getattr(twin, "lorentz_jacobi_tol", 1e-4))
+    check_every = 10
+    converged = False
+    iters_done = 0
+    rel = float("inf")
+    while iters_done < max_iters:
+        chunk = min(check_every, max_iters - iters_done)
+        for it in range(chunk):
+            kernels.elec_jacobi_step(g.phi_elec, g.phi_elec_tmp, g.sigma_elec, g.elec_source,
+                                     g.flags, g.dx, twin.lorentz_jacobi_omega, g.FLAG_GAS,
+                                     g.FLAG_SOLID, twin.nz_solid, g.nx, g.ny, g.nz)
+            if it == chunk - 1:
+                # only the last sweep's change is read (the reference overwrites the others)
+                kernels.elec_l1_diff_into(g.phi_elec_tmp, g.phi_elec, g.elec_l1_buf, g.scratch)
+            g.phi_elec, g.phi_elec_tmp = g.phi_elec_tmp, g.phi_elec
+            iters_done += 1
+        res, norm = (float(v) for v in g.elec_l1_buf.to_numpy())
+        rel = res / max(norm, 1e-30)
+        if rel < tol:
+            converged = True
+            break
+    if not converged:
+        twin._lorentz_unconverged = getattr(twin, "_lorentz_unconverged", 0) + 1
+        twin._lorentz_unconverged_streak = getattr(twin, "_lorentz_unconverged_streak", 0) + 1
+        n = twin._lorentz_unconverged
+        if n in (1, 10, 100) or n % 1000 == 0:
+            import logging
+
+            logging.getLogger("waam_solver_amd").warning(
+                f"[weld_forces] Lorentz Jacobi hit {max_iters} iterations without converging "
+                f"(rel change {rel:.2e} > {tol:.0e}); raise lorentz_jacobi_iters ({n} times)")
+    else:
+        twin._lorentz_unconverged_streak = 0
+    kernels.elec_compute_J(g.Jx, g.Jy, g.Jz, g.phi_elec, g.sigma_elec, g.flags, g.dx, g.FLAG_GAS,
+                           g.nx, g.ny, g.nz)
+    bin_dr_cells = max(float(max(g.nx, g.ny)) / (2.0 * g.n_elec_bins), 0.5)
+    kernels.elec_bin_axial_current(g.Jz, g.elec_rad_bins, arc_i, arc_j, g.dx, bin_dr_cells,
+                                   g.n_elec_bins, g.nz)
+    kernels.elec_prefix_bins(g.elec_rad_bins, g.n_elec_bins, g.nz)
+    kernels.elec_B_axisymmetric(g.Bx, g.By, g.Bz, g.elec_rad_bins, arc_i, arc_j, g.dx,
+                                bin_dr_cells, MU0, g.n_elec_bins, g.nz)
+    kernels.apply_lorentz_JxB(g.Fx, g.Fy, g.Fz, g.Jx, g.Jy, g.Jz, g.Bx, g.By, g.Bz, g.f_l,
+                              g.flags, g.dt, g.dx, twin.mat.rho, g.FLAG_GAS)

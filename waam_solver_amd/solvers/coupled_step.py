@@ -213,7 +213,7 @@ def coupled_step(twin, torch_x_m: float, torch_y_m: float, is_welding: bool,
     kernels.add_buoyancy(g.T, g.Fz, g.f_l, g.flags, twin.g_lu, twin.beta_T, twin.mat.T_liquidus,
                          1.0, g.FLAG_SOLID, g.FLAG_GAS)
     if is_welding and twin.enable_lorentz:
-        raise NotImplementedError("the Lorentz force solve (physics_tier: full) is not built")
+        weld_forces.solve_lorentz(twin, g, arc_i, arc_j, arc_k)
     u_cap = float(getattr(twin, "u_mach_limit_lu", 0.08))
     F_cap = float(getattr(twin, "force_limit_lu", 0.05))
     if F_cap > 0.0:

@@ -156,6 +156,9 @@ class WAAMTwin:
         # "fused": waam_step_fused (eight launches per step, the default); "reference": one
         # launch per reference kernel in the reference's order. Both give identical fields.
         self.engine = "fused"
+        self._lorentz_unconverged = 0
+        self._lorentz_unconverged_streak = 0
+        self._lorentz_warm = False
         self._fused: FusedStepper | None = None
         self._reset_counters()
 
@@ -318,6 +321,7 @@ class WAAMTwin:
             self._fused.reset()
         kernels.stream(g.f_a, g.f_b, g.flags, g.FLAG_SOLID, g.FLAG_GAS, g.nx, g.ny, g.nz)
         self._reset_counters()
+        self._lorentz_warm = False
         g.deposit_vol_buf.fill(0.0)
         g.deposit_real_buf.fill(0.0)
 
@@ -339,9 +343,11 @@ class WAAMTwin:
             self._maybe_shift_window(torch_x_m)
         sim_x = torch_x_m - self._window_offset_x_m - self._sim_origin_offset_x_m
         sim_y = torch_y_m - self._sim_origin_offset_y_m
-        if self.engine == "fused" and self.use_material_tables:
-            # the fused kernels take the scalar properties; per-cell tables step through the
-            # reference-order kernels (tables.hip), which give the same fields
+        if self.engine == "fused" and (self.use_material_tables or self.enable_lorentz):
+            # the fused kernels take scalar properties and the surface forces; per-cell tables
+            # and the Lorentz solve (a Jacobi iteration over the whole grid that dominates such
+            # a step) run through the reference-order kernels, which give the same fields
+            self.sync()
             coupled_step(self, sim_x, sim_y, is_welding, torch_z_m)
         elif self.engine == "fused":
             if self._fused is None:
@@ -387,7 +393,14 @@ class WAAMTwin:
             self._fused.flush()
 
     def _check_strict_mode(self) -> None:
-        if not self.strict_mode or self._n_droplets_fired < 20:
+        """Reference twin.py:569-597: a Lorentz solve that failed for more than 50 consecutive
+        steps, or a drifting deposited-mass ledger, stops the run."""
+        if not self.strict_mode:
+            return
+        if self.enable_lorentz and self._lorentz_unconverged_streak > 50:
+            raise RuntimeError(f"[strict_mode] Lorentz Jacobi failed to converge for "
+                               f"{self._lorentz_unconverged_streak} consecutive steps")
+        if self._n_droplets_fired < 20:
             return
         ratio = (self._deposited_volume_m3 * self.mat.rho) / max(
             self._n_droplets_fired * droplet_mass_kg(self), 1e-12)
