@@ -109,6 +109,8 @@ class WAAMGrid:
         self._n_optional_volume = 0
         if allocate_vof:
             self.ensure_vof_buffers()
+        if allocate_lorentz:
+            self.ensure_lorentz_fields()
         if allocate_export:
             self.ensure_export_buffers()
         self._read_buf = self.f_a
@@ -148,9 +150,6 @@ class WAAMGrid:
         self.kappa_field = self.field(torch.float32)
         self._has_export = True
         self._n_optional_volume += 1
-
-    def ensure_lorentz_fields(self) -> None:
-        raise NotImplementedError("the Lorentz Jacobi solve (physics_tier: full) is not built")
 
     # ------------------------------------------------------------------ buffers
     def swap_buffers(self) -> None:

@@ -101,7 +101,8 @@ class WAAMTwin:
         # field storage: rows along x (default) or the reference's k-fastest order; the fields'
         # accessors present (nx, ny, nz) either way
         self.grid = WAAMGrid(nx, ny, nz, dx, self.mat, max_tracers=max_tracers,
-                             allocate_vof=enable_vof, device=device,
+                             allocate_vof=enable_vof, allocate_lorentz=enable_lorentz,
+                             device=device,
                              layout=LAYOUT_ZYX if field_layout is None else field_layout)
         self.enable_droplet_impact_pressure = enable_droplet_impact_pressure
         self.use_recoil_clausius_clapeyron = use_recoil_clausius_clapeyron
