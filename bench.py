@@ -28,6 +28,10 @@ WORKLOAD = dict(nx=256, ny=128, nz=64, dx=2e-4, plate_thickness_mm=4.8, arc_powe
                 arc_efficiency=0.8, arc_sigma_mm=2.0, travel_speed_m_s=8e-3, x0_mm=10.0,
                 wire_feed_m_min=6.5, wire_diameter_mm=1.2, droplet_length_mm=3.0,
                 welding_current_A=140.0, steps=4000)
+# The same physics on 768 x 384 x 192 cells (56.6 M, 153.6 x 76.8 x 38.4 mm at 0.2 mm), the
+# largest single-GPU configuration named for this path (fp32 state: about 16 GB of HBM).
+LARGE = dict(WORKLOAD, nx=768, ny=384, nz=192, x0_mm=30.0, steps=400)
+WORKLOADS = {"bead_on_plate": WORKLOAD, "large": LARGE}
 
 
 def twin_kwargs(w: dict) -> dict:
@@ -174,7 +178,10 @@ def measured_traffic() -> float | None:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=WORKLOAD["steps"])
+    ap.add_argument("--workload", choices=tuple(WORKLOADS), default="bead_on_plate",
+                    help="bead_on_plate: 256x128x64 (the metric's configuration); "
+                         "large: 768x384x192")
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -194,7 +201,9 @@ def main() -> int:
 
     from waam_solver_amd._lib import STEP_KERNELS, lib
 
-    w = dict(WORKLOAD)
+    w = dict(WORKLOADS[args.workload])
+    if args.steps is None:
+        args.steps = w["steps"]
     tw = build_gpu_twin(w)
     tw.engine = args.engine
     g = tw.grid
@@ -224,7 +233,7 @@ def main() -> int:
         achieved = per_launch / avg_s / 1e9
         roofline = {"bound": "hbm", "kernel": DOMINANT, "achieved": round(achieved, 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": measured_traffic(),
+                    "traffic": measured_traffic() if args.workload == "bead_on_plate" else None,
                     "algorithmic_bytes_per_launch": round(per_launch),
                     "avg_launch_us": round(avg_s * 1e6, 2), "timed_launches": n_launch.value,
                     "cells_collided": round(active), "cells_pulled": round(pull),
@@ -237,7 +246,8 @@ def main() -> int:
     mlups = ncell * args.steps / el / 1e6
     tel = tw.get_telemetry()
     result = {
-        "metric": "coupled-step MLUPS (bead-on-plate 256x128x64, fp32)",
+        "metric": "coupled-step MLUPS (bead-on-plate 256x128x64, fp32)" if args.workload ==
+                  "bead_on_plate" else f"coupled-step MLUPS (bead-on-plate {g.nx}x{g.ny}x{g.nz}, fp32)",
         "value": round(mlups, 2),
         "unit": "MLUPS",
         "n_gpus": 1,
@@ -250,8 +260,8 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "bead-on-plate 256x128x64 dx=0.2mm, Gaussian2D 4kW eta0.8 "
-                               "sigma2mm +x 8mm/s, wire feed 6.5m/min, flow tier, MRT",
+        "config": {"workload": f"bead-on-plate {g.nx}x{g.ny}x{g.nz} dx=0.2mm, Gaussian2D 4kW "
+                               "eta0.8 sigma2mm +x 8mm/s, wire feed 6.5m/min, flow tier, MRT",
                    "grid": [g.nx, g.ny, g.nz], "cells": ncell, "collision": "mrt",
                    "engine": args.engine,
                    "pool_width_mm": tel["pool_width_mm"], "pool_depth_mm": tel["pool_depth_mm"],

@@ -419,10 +419,11 @@ typedef struct waam_step_params {
      * fused step collided (sflags_prev FLUID/IFACE), valid when every other cell's forces are
      * already zero (the step before was a fused step that did not take forces from the fields) */
     int32_t force_reset_all;
-    /* 1: the thermal list already holds this step's FLUID/IFACE cells, appended by the previous
+    /* 0: the thermal sweep builds the FLUID/IFACE list and a second launch consumes it.
+     * 1 or 2: the list already holds this step's FLUID/IFACE cells, appended by the previous
      * waam_step_fused call's last kernel (valid when nothing changed flags_in since that call);
-     * the surface cells' thermal update then runs in the same launch as the sweep. 0: the sweep
-     * builds the list and a second launch consumes it. */
+     * 1 runs the surface cells in the sweep's launch, 2 in a launch of their own (the shared
+     * launch lowers the sweep's occupancy: it pays on short sweeps, not on long ones). */
     int32_t thermal_list_ready;
     /* full-tier surface forces (physics/weld_forces.py:120-161): vapour recoil (clausius: the
      * Clausius-Clapeyron form, else the constant-pressure form) and shielding-gas shear */

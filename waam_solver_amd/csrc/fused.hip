@@ -59,6 +59,7 @@ constexpr int kListBlocks = 1024;
 /* list blocks sharing a launch with the thermal sweep: about one per CU, so the sweep keeps
  * most of the slots (measured: 1024 or 512 list blocks crowd it, 64 leave the list long) */
 constexpr int kListBlocksWithSweep = 256;
+
 static_assert(kBlock == kListSegments, "the list prefix scan uses one thread per segment");
 
 /* A cell list is split into kListSegments contiguous cell ranges of seg_len cells (a multiple
@@ -427,7 +428,9 @@ __device__ __forceinline__ void thermal_vof_cell(const waam_step_fields &f, cons
     if (s.p.enable_vof) f.phi_tmp[c] = phi_new;
 }
 
-/* 2 — thermal chain and VOF advection of every cell that is not FLUID/IFACE; lists those */
+/* 2 — thermal chain and VOF advection of every cell that is not FLUID/IFACE; lists those
+ *     (LIST) unless the previous step's kernel 7 already has */
+template <bool LIST>
 __global__ void __launch_bounds__(256) k_step_thermal(waam_step_fields f, StepConsts s)
 {
     const Dims &d = s.d;
@@ -435,7 +438,7 @@ __global__ void __launch_bounds__(256) k_step_thermal(waam_step_fields f, StepCo
     bool in = d.cell(i, j, k);
     size_t c = in ? d.idx(i, j, k) : 0;
     uint8_t flag = in ? f.flags_in[c] : (uint8_t)GAS;
-    list_append(in && is_surface_class(flag), c, cell_list(f, s.seg_len, 0), d.n());
+    if (LIST) list_append(in && is_surface_class(flag), c, cell_list(f, s.seg_len, 0), d.n());
     if (!in || is_surface_class(flag)) return;
     thermal_vof_cell(f, s, i, j, k, c, flag);
 }
@@ -978,8 +981,8 @@ int waam_step_fused(const waam_step_params *pp, const waam_step_fields *ff, void
             hipExtLaunchKernelGGL(k_step_arc_total, dim3(1), dim3(kBlock), 0, st, nullptr, b, 0, f,
                                   s);
     });
-    if (p.thermal_list_ready) {
-        /* enough list slices for about kListBlocks workers */
+    if (p.thermal_list_ready == 1) {
+        /* enough list slices for about kListBlocksWithSweep workers */
         int per = (int)(g3.x * g3.y);
         int lz = std::max(1, (kListBlocksWithSweep + per - 1) / per);
         dim3 ga(g3.x, g3.y, g3.z + (unsigned)lz);
@@ -988,7 +991,10 @@ int waam_step_fused(const waam_step_params *pp, const waam_step_fields *ff, void
         });
     } else {
         timed_launch(WAAM_K_THERMAL, [&](hipEvent_t a, hipEvent_t b) {
-            hipExtLaunchKernelGGL(k_step_thermal, g3, block3(), 0, st, a, b, 0, f, s);
+            if (p.thermal_list_ready)
+                hipExtLaunchKernelGGL(k_step_thermal<false>, g3, block3(), 0, st, a, b, 0, f, s);
+            else
+                hipExtLaunchKernelGGL(k_step_thermal<true>, g3, block3(), 0, st, a, b, 0, f, s);
         });
         timed_launch(WAAM_K_THERMAL_SURFACE, [&](hipEvent_t a, hipEvent_t b) {
             hipExtLaunchKernelGGL(k_step_thermal_surface, dim3(kListBlocks), dim3(kBlock), 0, st, a,
