@@ -375,6 +375,21 @@ int waam_shift_simulation_window_x(int n_shift, float *f_a, float *f_b, float *T
 int waam_bead_top_k(const uint8_t *flags, int nz_solid, int32_t *out, int nx, int ny, int nz,
                     int layout, void *stream);
 
+/* ---- export-side fields and reductions -------------------------------------------------- */
+/* compute_vorticity_magnitude kernels.py:3380-3419: |curl u| / dt on non-GAS cells, 0 on GAS */
+int waam_compute_vorticity_magnitude(const float *ux, const float *uy, const float *uz,
+                                     const uint8_t *flags, float *vort_out, float dx, float dt,
+                                     int nx, int ny, int nz, int layout, void *stream);
+/* out (device float) = max |(fx, fy, fz)| over n cells (force_linf_lu,
+ * physics/force_diagnostics.py:19-24) */
+int waam_max_norm3(const float *fx, const float *fy, const float *fz, float *out, int64_t n,
+                   void *stream);
+/* ProbeRecorder.record_step export/probes.py:95-113: out[2p], out[2p+1] = T, f_l at linear cell
+ * cells[p] (in the fields' layout); a negative entry (probe outside the window) gives NaN. The
+ * caller points out at a row of a device ring of samples and reads the ring back in one copy. */
+int waam_probe_gather(const float *T, const float *f_l, const int64_t *cells, int n_probes,
+                      int64_t n_cells, float *out, void *stream);
+
 /* ---- telemetry: telemetry_pool_reduce kernels.py:3424-3468 ------------------------------- */
 /* out (device, 8 doubles): n_liquid, T_peak, cool, u2_max, i_sum, i_min, i_max, T_global. */
 int waam_telemetry_pool_reduce(const float *T, const float *f_l, const float *phi,

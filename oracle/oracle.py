@@ -126,6 +126,7 @@ _SIGS = {
     "or_elec_prefix_bins": [_P, _I, _I],
     "or_elec_B_axisymmetric": [_P] * 4 + [_F] * 5 + [_I, _I, _I, _I],
     "or_apply_lorentz_JxB": [_P] * 11 + [_F, _F, _F, _I, _I, _I],
+    "or_compute_vorticity_magnitude": [_P] * 5 + [_F, _I, _I, _I],
 }
 
 
@@ -535,15 +536,101 @@ class OracleTwin:
             return base
         return min(self.T_vapor_cap_K - 50.0, base + P / (mdot * self.mat.cp))
 
-    def update_ctwd(self):
-        """Bead height = (top SOLID k at or above nz_solid - nz_solid + 1) dx; CTWD follows it
-        against the nominal layer height, never below nominal."""
+    def bead_height_m(self) -> float:
+        """(top SOLID k at or above nz_solid - nz_solid + 1) dx, 0 without one (reference
+        physics/electrical_stickout.py:39-50)."""
         g = self.g
         ks = np.nonzero(g.flags == SOLID)[2]
         ks = ks[ks >= self.nz_solid]
-        h = 0.0 if ks.size == 0 else float((ks.max() - self.nz_solid + 1) * g.dx)
+        return 0.0 if ks.size == 0 else float((ks.max() - self.nz_solid + 1) * g.dx)
+
+    def update_ctwd(self):
+        """CTWD follows the bead height against the nominal layer height, never below nominal
+        (reference physics/electrical_stickout.py:53-59)."""
+        h = self.bead_height_m()
         self._bead_height_m = h
         self.ctwd_m = max(self.ctwd_nominal_m, self.ctwd_nominal_m + (h - self.layer_height_m))
+
+    def toe_angle_deg(self) -> float:
+        """Reference physics/bead_geometry.py:13-40 on the oracle's arrays: in column
+        (nx // 3, ny // 2), cells k0 = nz_solid - 1 .. nz_solid + 7, the last k > k0 with
+        phi > 0.45 and |phi_k - phi_k-1| > 1e-4 sets atan(1 / slope) clamped to [10, 89] deg."""
+        g = self.g
+        phi = g.phi
+        i, j = g.nx // 3, g.ny // 2
+        k0 = max(0, self.nz_solid - 1)
+        k1 = min(g.nz - 1, self.nz_solid + 8)
+        best = float(self.contact_angle_deg)
+        for k in range(k0, k1):
+            if phi[i, j, k] > 0.45 and k > k0:
+                dphi = phi[i, j, k] - phi[i, j, k - 1]
+                if abs(dphi) > 1e-4:
+                    slope = abs(dphi) * g.dx / max(g.dx, 1e-9)
+                    best = min(89.0, max(10.0, math.degrees(math.atan(1.0 / max(slope, 1e-6)))))
+        return best
+
+    def force_diagnostics(self) -> dict:
+        """Reference physics/force_diagnostics.py:36-142: max |F| of each force applied alone to
+        the current state at the last arc position, then of the restored live forces."""
+        g, L, m = self.g, self.lib, self.mat
+        d = g.dims
+        arc_i, arc_j, arc_k = self.last_arc_ijk
+        saved = (g.Fx.copy(), g.Fy.copy(), g.Fz.copy())
+        out = dict.fromkeys(("f_csf_max", "f_marangoni_max", "f_arc_max", "f_lorentz_max",
+                             "f_buoyancy_max", "f_gravity_max", "f_gas_shear_max",
+                             "f_recoil_max", "f_total_snap_max"), 0.0)
+
+        def linf():
+            return float(np.sqrt(g.Fx * g.Fx + g.Fy * g.Fy + g.Fz * g.Fz).max())
+
+        def measure(key, fn):
+            for a in (g.Fx, g.Fy, g.Fz):
+                a.fill(0)
+            fn()
+            out[key] = linf()
+
+        if self.enable_csf_tension:
+            ct, st = cos_sin_f32(self.theta_rad)
+            measure("f_csf_max", lambda: L.or_compute_csf_tension(
+                ptr(g.phi), ptr(g.flags), ptr(g.Fx), ptr(g.Fy), ptr(g.Fz), self.gamma_lu,
+                1 if self.enable_wetting else 0, ct, st, *d))
+        if self.use_tables:
+            measure("f_marangoni_max", lambda: L.or_compute_marangoni_force_variable(
+                ptr(g.T), ptr(g.phi), ptr(g.f_l), ptr(g.Fx), ptr(g.Fy), ptr(g.Fz), ptr(g.flags),
+                ptr(g.dgamma_lu_field), *d))
+        else:
+            measure("f_marangoni_max", lambda: L.or_compute_marangoni_force(
+                ptr(g.T), ptr(g.phi), ptr(g.f_l), ptr(g.Fx), ptr(g.Fy), ptr(g.Fz), ptr(g.flags),
+                self.dgamma_dT_lu, *d))
+        if self.enable_gas_shear:
+            tau = min(0.5 * 1.6 * self.gas_jet_velocity_m_s ** 2 * self.gas_shear_coeff, 5000.0)
+            measure("f_gas_shear_max", lambda: L.or_apply_gas_shear_stress(
+                ptr(g.Fx), ptr(g.Fy), ptr(g.phi), ptr(g.flags), arc_i, arc_j, self.sigma_cells,
+                tau, g.dt, g.dx, m.rho, *d))
+        measure("f_arc_max", lambda: L.or_apply_arc_pressure(
+            ptr(g.Fz), ptr(g.flags), ptr(g.phi), arc_i, arc_j, arc_k,
+            self.arc_pressure_sigma_m() / g.dx, self.arc_pressure_peak_pa(), g.dt, g.dx, m.rho,
+            *d))
+        if self.enable_recoil and self.use_recoil_cc:
+            measure("f_recoil_max", lambda: L.or_apply_vapor_recoil_cc(
+                ptr(g.Fz), ptr(g.T), ptr(g.phi), ptr(g.flags), arc_i, arc_j, self.sigma_cells,
+                self.P_vapor_ref_Pa, self.T_boiling_K, self.L_vapor_J_kg, self.R_spec_vapor_J_kgK,
+                self.recoil_accommodation, g.dt, g.dx, m.rho, *d))
+        elif self.enable_recoil:
+            measure("f_recoil_max", lambda: L.or_apply_vapor_recoil(
+                ptr(g.Fz), ptr(g.T), ptr(g.phi), ptr(g.flags), arc_i, arc_j, self.sigma_cells,
+                self.recoil_pressure_pa, m.T_liquidus, g.dt, g.dx, m.rho, *d))
+        if self.enable_hydrostatic_gravity:
+            measure("f_gravity_max", lambda: L.or_add_hydrostatic_gravity(
+                ptr(g.Fz), ptr(g.f_l), ptr(g.flags), 1.0, self.g_lu, *d))
+        measure("f_buoyancy_max", lambda: L.or_add_buoyancy(
+            ptr(g.T), ptr(g.Fz), ptr(g.f_l), ptr(g.flags), self.g_lu, self.beta_T, m.T_liquidus,
+            1.0, *d))
+        if self.enable_lorentz:
+            measure("f_lorentz_max", lambda: self.solve_lorentz(arc_i, arc_j, arc_k))
+        g.Fx[...], g.Fy[...], g.Fz[...] = saved
+        out["f_total_snap_max"] = linf()
+        return {k: round(float(v), 6) for k, v in out.items()}
 
     def droplet_impact_velocity_m_s(self, m_drop):
         drop_len = 3e-3

@@ -2150,3 +2150,31 @@ void or_apply_lorentz_JxB(float *Fx, float *Fy, float *Fz, const float *Jx, cons
         Fz[c] += fz * scale;
     }
 }
+
+/* compute_vorticity_magnitude kernels.py:3380-3419 */
+void or_compute_vorticity_magnitude(const float *ux, const float *uy, const float *uz,
+                                    const uint8_t *flags, float *out, float dt, int nx, int ny,
+                                    int nz)
+{
+    float scale = 1.0f / (dt + 1e-12f);
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                size_t c = IDX(i, j, k);
+                out[c] = 0.0f;
+                if (flags[c] == GAS) continue;
+                int ip = imin(i + 1, nx - 1), im = imax(i - 1, 0);
+                int jp = imin(j + 1, ny - 1), jm = imax(j - 1, 0);
+                int kp = imin(k + 1, nz - 1), km = imax(k - 1, 0);
+                float duz_dy = 0.5f * (uz[IDX(i, jp, k)] - uz[IDX(i, jm, k)]);
+                float duy_dz = 0.5f * (uy[IDX(i, j, kp)] - uy[IDX(i, j, km)]);
+                float dux_dz = 0.5f * (ux[IDX(i, j, kp)] - ux[IDX(i, j, km)]);
+                float duz_dx = 0.5f * (uz[IDX(ip, j, k)] - uz[IDX(im, j, k)]);
+                float duy_dx = 0.5f * (uy[IDX(ip, j, k)] - uy[IDX(im, j, k)]);
+                float dux_dy = 0.5f * (ux[IDX(i, jp, k)] - ux[IDX(i, jm, k)]);
+                float wx = (duz_dy - duy_dz) * scale;
+                float wy = (dux_dz - duz_dx) * scale;
+                float wz = (duy_dx - dux_dy) * scale;
+                out[c] = sqrtf(wx * wx + wy * wy + wz * wz);
+            }
+}

@@ -103,6 +103,9 @@ SIGNATURES: dict[str, str] = {
     "waam_elec_prefix_bins": "piip",
     "waam_elec_B_axisymmetric": "ppppfffffiiiiip",
     "waam_apply_lorentz_JxB": "pppppppppppffflp",
+    "waam_compute_vorticity_magnitude": "pppppffiiiip",
+    "waam_max_norm3": "pppplp",
+    "waam_probe_gather": "pppilpp",
 }
 
 # kernel ids of waam_step_fused for waam_timing_begin (include/waam_hip.h WAAM_K_*)

@@ -148,8 +148,9 @@ class WAAMGrid:
         if self._has_export:
             return
         self.kappa_field = self.field(torch.float32)
+        self.vorticity_mag = self.field(torch.float32)
         self._has_export = True
-        self._n_optional_volume += 1
+        self._n_optional_volume += 2
 
     # ------------------------------------------------------------------ buffers
     def swap_buffers(self) -> None:

@@ -734,3 +734,27 @@ def apply_lorentz_JxB(Fx, Fy, Fz, Jx, Jy, Jz, Bx, By, Bz, f_l, flags, dt, dx, rh
     _need(_n(Fx), Fy, Fz, Jx, Jy, Jz, Bx, By, Bz, f_l, flags)
     _call("waam_apply_lorentz_JxB", P(Fx), P(Fy), P(Fz), P(Jx), P(Jy), P(Jz), P(Bx), P(By), P(Bz),
           P(f_l), P(flags), dt, dx, rho_ref, _n(Fx), _stream())
+
+
+# ------------------------------------------------------------------ export-side fields
+def compute_vorticity_magnitude(ux, uy, uz, flags, vort_out, dx, dt, FLAG_GAS, nx, ny, nz):
+    """Reference kernels.py:3380-3419."""
+    _flags_ok(FLAG_GAS=FLAG_GAS)
+    _need(nx * ny * nz, ux, uy, uz, flags, vort_out)
+    _call("waam_compute_vorticity_magnitude", P(ux), P(uy), P(uz), P(flags), P(vort_out), dx, dt,
+          nx, ny, nz, layout_of(vort_out), _stream())
+
+
+def max_norm3_into(fx, fy, fz, out) -> None:
+    """out[0] = max |(fx, fy, fz)| (device reduction)."""
+    _need(_n(fx), fy, fz)
+    _call_ro("waam_max_norm3", P(fx), P(fy), P(fz), P(out), _n(fx), _stream())
+
+
+def probe_gather_into(T, f_l, cells, out) -> None:
+    """out[2p], out[2p + 1] = T, f_l at linear cell cells[p] (int64 device tensor, storage order
+    of the fields; negative -> NaN)."""
+    n = _n(T)
+    _need(n, f_l)
+    _call_ro("waam_probe_gather", P(T), P(f_l), P(cells), int(cells.numel()), n, P(out),
+             _stream())

@@ -25,7 +25,9 @@ from .materials import MaterialProps, load_material
 from .physics.arc import create_heat_source
 from .physics.deposition_balance import (droplet_mass_kg, expected_deposited_mass_kg,
                                          infer_transfer_mode, wire_mass_flux_kg_s)
+from .physics.bead_geometry import bead_reinforcement_height_mm, estimate_toe_angle_deg
 from .physics.electrical_stickout import droplet_entry_temperature_K
+from .physics.force_diagnostics import sample_force_diagnostics
 from .physics.weld_forces import arc_pressure_peak_pa, droplet_impact_velocity_m_s
 from .solvers.coupled_step import coupled_step
 from .solvers.fused import FusedStepper
@@ -161,6 +163,10 @@ class WAAMTwin:
         self._lorentz_unconverged_streak = 0
         self._lorentz_warm = False
         self._fused: FusedStepper | None = None
+        self.probe_recorder = None
+        self._job_path: str | None = None
+        self.enable_force_diagnostics = True
+        self._force_diag: dict = {}
         self._reset_counters()
 
         g = self.grid
@@ -297,6 +303,10 @@ class WAAMTwin:
                                   plate_origin_mm=grid["plate_origin_mm"])
         twin._job_config = job
         twin._job_path = str(job_path)
+        if job.get("probes"):
+            from .export.probes import ProbeRecorder
+
+            twin.probe_recorder = ProbeRecorder.from_job_list(job["probes"], twin)
         return twin
 
     # ------------------------------------------------------------------ state
@@ -359,6 +369,8 @@ class WAAMTwin:
             coupled_step(self, sim_x, sim_y, is_welding, torch_z_m)
         else:
             raise ValueError(f"unknown engine {self.engine!r} (fused | reference)")
+        if self.probe_recorder is not None:
+            self.probe_recorder.record_step(self)
         self._check_strict_mode()
 
     def _maybe_shift_window(self, torch_x_world: float) -> None:
@@ -488,6 +500,15 @@ class WAAMTwin:
             pool_width_m = pool_depth_m = pool_length_m = u_max_phys = 0.0
         active = g.porosity_active.to_numpy()
         n_trapped, n_active = int((active == 2).sum()), int((active == 1).sum())
+        bead_h = bead_reinforcement_height_mm(self, g)
+        toe_deg = estimate_toe_angle_deg(self, g) if self.enable_wetting else 0.0
+        force_diag = dict(self._force_diag or {})
+        if self.enable_force_diagnostics and self._step_n > 0:
+            try:
+                force_diag = sample_force_diagnostics(self)
+                self._force_diag = force_diag
+            except Exception as exc:  # reported in the telemetry, as the reference does
+                force_diag = {"error": str(exc)}
         dep_mass = self._deposited_volume_m3 * self.mat.rho
         m_drop = droplet_mass_kg(self)
         exp_drop = self._n_droplets_fired * m_drop
@@ -508,8 +529,10 @@ class WAAMTwin:
             "material_tables": self.use_material_tables,
             "vof_enabled": self.enable_vof,
             "travel_speed_mm_s": round(self.travel_speed_m_s * 1000, 3),
+            "variable_tau": self.use_variable_tau and self.use_material_tables,
             "porosity_pct": round(100.0 * n_trapped / max(n_trapped + n_active, 1), 3),
             "n_trapped_tracers": n_trapped,
+            "window_offset_x_mm": round(self._window_offset_x_m * 1000, 3),
             "deposited_mass_g": round(dep_mass * 1000, 4),
             "expected_wire_mass_g": round(expected_deposited_mass_kg(self) * 1000, 4),
             "expected_drop_mass_g": round(exp_drop * 1000, 4),
@@ -520,12 +543,63 @@ class WAAMTwin:
             "mass_balance_ratio": round(dep_mass / max(exp_drop, 1e-12), 3),
             "wire_mass_flux_g_s": round(wire_mass_flux_kg_s(self) * 1000, 4),
             "T_vapor_cap_K": self.T_vapor_cap_K,
+            "arc_surface_weighting": self.arc_surface_weighting,
+            "enable_enthalpy_cap": self.enable_enthalpy_cap,
+            "enable_wetting": self.enable_wetting,
+            "enable_hydrostatic_gravity": self.enable_hydrostatic_gravity,
+            "enable_bead_freeze": self.enable_bead_freeze,
+            "enable_ctwd": self.enable_ctwd,
+            "contact_angle_deg": round(self.contact_angle_deg, 1),
+            "bead_height_mm": round(bead_h, 3),
+            "toe_angle_deg": round(toe_deg, 1),
+            "ctwd_mm": round(self.ctwd_m * 1000, 3),
             "T_drop_K": round(droplet_entry_temperature_K(self), 1),
             "deposition_overflow_count": self._deposition_overflow,
             "physics_tier": self.physics_tier,
+            "strict_mode": bool(self.strict_mode),
             "arc_pressure_model": self.arc_pressure_model,
             "arc_pressure_peak_pa": round(arc_pressure_peak_pa(self), 1),
-            "enable_wetting": self.enable_wetting,
-            "enable_bead_freeze": self.enable_bead_freeze,
-            "contact_angle_deg": round(self.contact_angle_deg, 1),
+            "enable_lorentz": self.enable_lorentz,
+            "enable_gas_shear": self.enable_gas_shear,
+            "lorentz_unconverged_count": int(self._lorentz_unconverged or 0),
+            "lorentz_unconverged_streak": int(self._lorentz_unconverged_streak or 0),
+            "force_diagnostics": force_diag,
         }
+
+    # ------------------------------------------------------------------ export
+    def export_vtk(self, path: str = "weld_pool.vti"):
+        """Core + derived tiers (reference twin.py:823-825)."""
+        return self.export_vtk_full(path, tiers=(0, 3), crop_liquid=False)
+
+    def export_vtk_full(self, path: str = "weld_pool_full.vti", tiers=(0, 1, 2, 3),
+                        crop_liquid: bool = False):
+        from .export import vtk_io
+
+        return vtk_io.export_volume(self, path, tiers=tuple(vtk_io.TIER_MAP.get(t, t)
+                                                            for t in tiers),
+                                    crop_liquid=crop_liquid)
+
+    def export_surface_vtk(self, path: str = "bead_surface.vtp"):
+        from .export.vtk_io import export_surface
+
+        return export_surface(self, path, include_kappa=True)
+
+    def export_tracers_vtk(self, path: str = "tracers.vtp"):
+        from .export.vtk_io import export_tracers
+
+        return export_tracers(self, path)
+
+    def export_research_bundle(self, out_dir, tag: str | None = None, tiers=(0, 1, 3),
+                               include_surface: bool = True, include_tracers: bool = True,
+                               crop_liquid: bool = False) -> dict[str, str]:
+        from .export.bundle import export_research_bundle
+
+        return export_research_bundle(self, out_dir, tag=tag, tiers=tiers,
+                                      include_surface=include_surface,
+                                      include_tracers=include_tracers, crop_liquid=crop_liquid,
+                                      job_path=self._job_path)
+
+    def export_research_sequence(self, *args, **kwargs):
+        from .export.bundle import export_research_sequence
+
+        return export_research_sequence(self, *args, job_path=self._job_path, **kwargs)
