@@ -66,10 +66,10 @@ def torch_xy(w: dict, step: int, dt: float) -> tuple[float, float]:
             (w["ny"] // 2) * w["dx"])
 
 
-def build_gpu_twin(w: dict):
+def build_gpu_twin(w: dict, pop: str = "fp32"):
     from waam_solver_amd.twin import WAAMTwin
 
-    tw = WAAMTwin(material="LCS-bead-on-plate", **twin_kwargs(w))
+    tw = WAAMTwin(material="LCS-bead-on-plate", population_storage=pop, **twin_kwargs(w))
     configure(tw, w)
     tw.reset()
     g = tw.grid
@@ -120,9 +120,11 @@ def cpu_baseline(w: dict, seconds: float) -> dict:
 #   pulled cell       19 populations read + 19 written                              152 B
 # A pulled cell is one the collision skips this step that was not SOLID at the step whose
 # stream is being completed.
-STREAM_B_EVERY, STREAM_B_PULL = 3, 152
-# Whole-step compulsory bytes per cell (fp32 populations, with the HAZ fields)
-STEP_B_PER_CELL = 258
+STREAM_B_EVERY = 3
+STREAM_B_PULL = {"fp32": 152, "fp16": 76, "bf16": 76}
+# Whole-step compulsory bytes per cell with the HAZ fields: populations 2 x 19 x 4 B (fp32) or
+# 2 x 19 x 2 B (16-bit storage), H T f_l phi r/w 32 B, u r/w 24 B, flags 2 B, rho 4 B, HAZ 44 B
+STEP_B_PER_CELL = {"fp32": 258, "fp16": 182, "bf16": 182}
 DOMINANT = "k_step_stream"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_step_stream_bead_on_plate_256x128x64_pmc.md")
 
@@ -136,8 +138,60 @@ def stream_classes(g) -> tuple[int, int]:
     return int(active.sum()), int(pull.sum())
 
 
-def stream_bytes(ncell: int, pull: float) -> float:
-    return STREAM_B_EVERY * ncell + STREAM_B_PULL * pull
+def stream_bytes(ncell: int, pull: float, pop: str = "fp32") -> float:
+    return STREAM_B_EVERY * ncell + STREAM_B_PULL[pop] * pull
+
+
+def rel_l2_device(a, b) -> float:
+    """||a - b|| / ||b|| in float64 on the device."""
+    a, b = a.double(), b.double()
+    return float(((a - b).norm() / b.norm().clamp_min(1e-300)).item())
+
+
+def precision_form(w: dict, args, ref_tw, ref_mlups: float, pop: str) -> dict:
+    """The same workload, warm-up and timed steps with 16-bit population storage: MLUPS, speedup
+    over the fp32 run and the relative L2 of T, phi, f_l and |u| against the fp32 run's final
+    state (both runs start from the same seeded state). pop "fp32_ulp" is the fp32 run with half
+    of its initial populations moved by one ulp: the divergence any perturbation of this flow
+    reaches over the same steps, the floor against which the 16-bit errors read."""
+    import torch
+
+    from waam_solver_amd.field import note_write, tensor_of
+
+    tw = build_gpu_twin(w, "fp32" if pop == "fp32_ulp" else pop)
+    g = tw.grid
+    if pop == "fp32_ulp":
+        f = tensor_of(g.f_src)
+        gen = torch.Generator(device=f.device).manual_seed(5)
+        m = torch.rand(f.shape, generator=gen, device=f.device) < 0.5
+        f[m] = torch.nextafter(f[m], torch.full_like(f[m], 2.0))
+        note_write()
+    for s in range(args.warmup):
+        tw.step(*torch_xy(w, s, g.dt), True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.warmup, args.warmup + args.steps):
+        tw.step(*torch_xy(w, s, g.dt), True)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    mlups = g.n_cells * args.steps / el / 1e6
+    r = ref_tw.grid
+    err = {k: round(rel_l2_device(tensor_of(getattr(g, k)), tensor_of(getattr(r, k))), 8)
+           for k in ("T", "phi", "f_l")}
+    speed = lambda gg: torch.sqrt(sum(tensor_of(getattr(gg, c)).double() ** 2  # noqa: E731
+                                      for c in ("ux", "uy", "uz")))
+    err["|u|"] = round(rel_l2_device(speed(g), speed(r)), 8)
+    flips = int((tensor_of(g.flags) != tensor_of(r.flags)).sum().item())
+    out = {"value": round(mlups, 2), "speedup_vs_fp32": round(mlups / ref_mlups, 3),
+           "ms_per_step": round(el * 1e3 / args.steps, 4),
+           "rel_l2_vs_fp32": err, "flags_differing": flips}
+    if pop != "fp32_ulp":
+        out["step_bytes_per_cell"] = STEP_B_PER_CELL[pop]
+        out["step_frac"] = round(STEP_B_PER_CELL[pop] * g.n_cells * args.steps / el / 1e9
+                                 / HBM_PEAK_GBS, 4)
+    del tw, g
+    torch.cuda.empty_cache()
+    return out
 
 
 def kernel_breakdown(tw, w: dict, start: int, steps: int) -> dict:
@@ -188,6 +242,10 @@ def main() -> int:
     ap.add_argument("--engine", choices=("fused", "reference"), default="fused")
     ap.add_argument("--breakdown", type=int, default=50,
                     help="steps per kernel for the per-kernel HIP-event breakdown (0: skip)")
+    ap.add_argument("--pop", choices=("fp32", "fp16", "bf16"), default="fp32",
+                    help="population storage of the measured run (16-bit forms store f - w_q)")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the 16-bit storage runs that an fp32 run reports beside its value")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (each box here has "
@@ -204,7 +262,7 @@ def main() -> int:
     w = dict(WORKLOADS[args.workload])
     if args.steps is None:
         args.steps = w["steps"]
-    tw = build_gpu_twin(w)
+    tw = build_gpu_twin(w, args.pop)
     tw.engine = args.engine
     g = tw.grid
     ncell = g.nx * g.ny * g.nz
@@ -228,26 +286,34 @@ def main() -> int:
         cls1 = stream_classes(g)
         cls0 = cls0 or cls1
         active, pull = (cls0[0] + cls1[0]) / 2.0, (cls0[1] + cls1[1]) / 2.0
-        per_launch = stream_bytes(ncell, pull)
+        per_launch = stream_bytes(ncell, pull, args.pop)
         avg_s = tot_ms.value / 1e3 / max(n_launch.value, 1)
         achieved = per_launch / avg_s / 1e9
         roofline = {"bound": "hbm", "kernel": DOMINANT, "achieved": round(achieved, 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": measured_traffic() if args.workload == "bead_on_plate" else None,
+                    "traffic": (measured_traffic() if args.workload == "bead_on_plate"
+                                and args.pop == "fp32" else None),
                     "algorithmic_bytes_per_launch": round(per_launch),
                     "avg_launch_us": round(avg_s * 1e6, 2), "timed_launches": n_launch.value,
                     "cells_collided": round(active), "cells_pulled": round(pull),
-                    "step_bytes_per_cell": STEP_B_PER_CELL,
-                    "step_achieved": round(STEP_B_PER_CELL * ncell * args.steps / el / 1e9, 1)}
+                    "step_bytes_per_cell": STEP_B_PER_CELL[args.pop],
+                    "step_achieved": round(STEP_B_PER_CELL[args.pop] * ncell * args.steps / el
+                                           / 1e9, 1)}
         roofline["step_frac"] = round(roofline["step_achieved"] / HBM_PEAK_GBS, 4)
+    mlups = ncell * args.steps / el / 1e6
+    precision = None
+    if args.pop == "fp32" and fused and not args.no_precision:
+        # the precision form of the metric: 16-bit shifted population storage, same steps from
+        # the same seeded state, against this run's state at the end of its timed steps
+        precision = {pop: precision_form(w, args, tw, mlups, pop)
+                     for pop in ("fp16", "bf16", "fp32_ulp")}
+    if fused:
         if args.breakdown > 0:
             roofline["kernel_us"] = kernel_breakdown(tw, w, args.warmup + args.steps,
                                                      args.breakdown)
-    mlups = ncell * args.steps / el / 1e6
     tel = tw.get_telemetry()
     result = {
-        "metric": "coupled-step MLUPS (bead-on-plate 256x128x64, fp32)" if args.workload ==
-                  "bead_on_plate" else f"coupled-step MLUPS (bead-on-plate {g.nx}x{g.ny}x{g.nz}, fp32)",
+        "metric": f"coupled-step MLUPS (bead-on-plate {g.nx}x{g.ny}x{g.nz}, {args.pop})",
         "value": round(mlups, 2),
         "unit": "MLUPS",
         "n_gpus": 1,
@@ -258,17 +324,19 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32" if args.pop == "fp32" else f"f32 arithmetic, {args.pop} populations",
         "data": "synthetic",
         "config": {"workload": f"bead-on-plate {g.nx}x{g.ny}x{g.nz} dx=0.2mm, Gaussian2D 4kW "
                                "eta0.8 sigma2mm +x 8mm/s, wire feed 6.5m/min, flow tier, MRT",
                    "grid": [g.nx, g.ny, g.nz], "cells": ncell, "collision": "mrt",
-                   "engine": args.engine,
+                   "engine": args.engine, "population_storage": args.pop,
                    "pool_width_mm": tel["pool_width_mm"], "pool_depth_mm": tel["pool_depth_mm"],
                    "n_droplets": tel["n_droplets_fired"]},
     }
     if roofline is not None:
         result["roofline"] = roofline
+    if precision is not None:
+        result["precision"] = precision
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
     print(json.dumps(result))

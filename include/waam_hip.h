@@ -35,10 +35,13 @@
 extern "C" {
 #endif
 
-#define WAAM_ABI_VERSION 2
+#define WAAM_ABI_VERSION 3
 
 enum { WAAM_FLAG_FLUID = 0, WAAM_FLAG_SOLID = 1, WAAM_FLAG_GAS = 2, WAAM_FLAG_IFACE = 4 };
 enum { WAAM_LAYOUT_XYZ = 0, WAAM_LAYOUT_ZYX = 1 };
+/* Population storage of the fused step: fp32 populations, or f - w_q in binary16 / bfloat16
+ * (round to nearest even on store, fp32 arithmetic in between). */
+enum { WAAM_POP_F32 = 0, WAAM_POP_F16 = 1, WAAM_POP_BF16 = 2 };
 enum { WAAM_OK = 0, WAAM_ERR_INVALID = -1, WAAM_ERR_HIP = -2 };
 
 int waam_abi_version(void);
@@ -406,7 +409,8 @@ int waam_telemetry_pool_reduce(const float *T, const float *f_l, const float *ph
  * step n+1 does not overwrite it, using the flags of step n (sflags_prev). See
  * waam_stream_pending for materialising it on demand. */
 typedef struct waam_step_fields {
-    float *f_src, *f_dst;
+    /* populations in the form pop_dtype names (float, or 16-bit words of f - w_q) */
+    void *f_src, *f_dst;
     float *rho, *ux, *uy, *uz;
     float *H_in, *H_out, *T_in, *T_out, *fl_in, *fl_out;
     float *phi, *phi_tmp;
@@ -420,6 +424,7 @@ typedef struct waam_step_fields {
     void *scratch;
     int32_t max_tracers, nx, ny, nz;
     int32_t layout; /* WAAM_LAYOUT_XYZ or WAAM_LAYOUT_ZYX, shared by every field */
+    int32_t pop_dtype; /* WAAM_POP_* of f_src and f_dst */
 } waam_step_fields;
 
 typedef struct waam_step_params {
@@ -461,9 +466,18 @@ typedef struct waam_step_params {
 
 int waam_step_fused(const waam_step_params *p, const waam_step_fields *f, void *stream);
 /* Materialise a deferred stream: f_dst = pull(f_src) on cells not SOLID in sflags (the flags the
- * step that deferred it saw). */
-int waam_stream_pending(const float *f_src, float *f_dst, const uint8_t *sflags, int nx, int ny,
-                        int nz, int layout, void *stream);
+ * step that deferred it saw); populations in the WAAM_POP_* form pop_dtype. */
+int waam_stream_pending(const void *f_src, void *f_dst, const uint8_t *sflags, int nx, int ny,
+                        int nz, int layout, int pop_dtype, void *stream);
+/* Convert n_cells x 19 populations between fp32 and the storage form pop_dtype (WAAM_POP_F16 or
+ * WAAM_POP_BF16: f - w_q, rounded to nearest even; unpack adds w_q back in fp32). q-major, as
+ * the population arrays are: element q * n_cells + c has weight w_q. Unpacking then packing
+ * returns the same 16-bit words for every word that packing (or the fused collision's store)
+ * produces. */
+int waam_pack_populations(const float *src, void *dst, int64_t n_cells, int pop_dtype,
+                          void *stream);
+int waam_unpack_populations(const void *src, float *dst, int64_t n_cells, int pop_dtype,
+                            void *stream);
 /* The step's arc weight total, energy per unit weight and arc column height, as written into
  * the workspace by the last waam_step_fused (three device floats, in that order). Blocking. */
 int waam_step_arc_readback(const void *scratch, float *host_out3);

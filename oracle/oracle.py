@@ -309,10 +309,14 @@ class OracleTwin:
                  enable_gas_shear=False, gas_jet_velocity_m_s=12.0, gas_shear_coeff=1.0,
                  T_boiling_K=3100.0, P_vapor_ref_Pa=101325.0, L_vapor_J_kg=6.0e6,
                  R_spec_vapor_J_kgK=450.0, enable_lorentz=False, sigma_liquid_Sm=1.0e6,
-                 sigma_solid_Sm=1.0e5, lorentz_jacobi_iters=40, lorentz_jacobi_omega=0.85):
+                 sigma_solid_Sm=1.0e5, lorentz_jacobi_iters=40, lorentz_jacobi_omega=0.85,
+                 population_storage="fp32"):
         self.mat = material
         self.lib = lib(parallel)
         self.mode = mode
+        if population_storage not in ("fp32", "fp16", "bf16"):
+            raise ValueError(f"population_storage {population_storage!r}")
+        self.population_storage = population_storage
         self.Q_w, self.eta, self.T_amb = arc_power_W, arc_efficiency, T_ambient
         self.use_srt, self.C_darcy, self.arc_pressure = use_srt, C_darcy, arc_pressure_pa
         self.droplet_freq = droplet_freq_hz
@@ -976,6 +980,12 @@ class OracleTwin:
         if self.force_limit_lu > 0.0:
             L.or_clamp_body_force_magnitude(ptr(g.Fx), ptr(g.Fy), ptr(g.Fz), ptr(g.flags),
                                             self.force_limit_lu, *d)
+        if self.population_storage != "fp32":
+            # the 16-bit forms hold every stored population at a representable value: values
+            # written since the last step (reset, deposition, impact) are narrowed before the
+            # collision reads them, and the collision's outputs as they are stored
+            quantize_populations(g.f_src, self.population_storage)
+            quantize_populations(g.f_dst, self.population_storage)
         if self.use_tables and self.use_variable_tau:
             L.or_collide_srt_variable_tau(ptr(g.f_src), ptr(g.f_dst), ptr(g.rho), ptr(g.ux),
                                           ptr(g.uy), ptr(g.uz), ptr(g.Fx), ptr(g.Fy), ptr(g.Fz),
@@ -989,6 +999,8 @@ class OracleTwin:
             L.or_collide_mrt(ptr(g.f_src), ptr(g.f_dst), ptr(g.rho), ptr(g.ux), ptr(g.uy),
                              ptr(g.uz), ptr(g.Fx), ptr(g.Fy), ptr(g.Fz), ptr(g.f_l), ptr(g.flags),
                              self.omega, self.omega_bulk, self.C_darcy, *d)
+        if self.population_storage != "fp32":
+            quantize_populations(g.f_dst, self.population_storage)
         if self.u_mach_limit_lu > 0.0:
             L.or_clamp_velocity_mach(ptr(g.ux), ptr(g.uy), ptr(g.uz), ptr(g.flags),
                                      self.u_mach_limit_lu, *d)
@@ -1057,6 +1069,36 @@ class OracleTwin:
             res.update(T_peak=float(out[7]), pool_width_m=0.0, pool_depth_m=0.0,
                        pool_length_m=0.0, u2_max=0.0)
         return res
+
+
+def float_to_bf16_bits(x: np.ndarray) -> np.ndarray:
+    """bfloat16 words of fp32 values, rounded to nearest even (ties to the even significand);
+    NaN stays a quiet NaN."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    nan = (u & 0x7FFFFFFF) > 0x7F800000
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) & 0xFFFF
+    r = np.where(nan, ((u >> 16) | 0x40) & 0xFFFF, r)
+    return r.astype(np.uint16)
+
+
+def bf16_bits_to_float(b: np.ndarray) -> np.ndarray:
+    return (np.asarray(b, np.uint16).astype(np.uint32) << 16).view(np.float32)
+
+
+def quantize_populations(f: np.ndarray, form: str) -> None:
+    """In place: each population f[q] becomes the fp32 value its 16-bit storage form holds,
+    (f - w_q) narrowed to binary16 or bfloat16 (round to nearest even) and widened back with
+    w_q added, every operation in fp32."""
+    for q in range(Q):
+        w = np.float32(W[q])
+        d = (f[q] - w).astype(np.float32)
+        if form == "fp16":
+            back = d.astype(np.float16).astype(np.float32)
+        elif form == "bf16":
+            back = bf16_bits_to_float(float_to_bf16_bits(d))
+        else:
+            raise ValueError(form)
+        f[q] = (back + w).astype(np.float32)
 
 
 def cos_sin_f32(theta_rad: float) -> tuple[float, float]:

@@ -79,7 +79,8 @@ def torch_x(twin_dx: float, nx: int, step: int, dt: float) -> float:
     return (nx // 3) * twin_dx + 8e-3 * step * dt
 
 
-def build_pair(name: str, mode: str = "jacobi", engine: str = "fused", layout: int | None = None):
+def build_pair(name: str, mode: str = "jacobi", engine: str = "fused", layout: int | None = None,
+               population_storage: str = "fp32"):
     """(product twin, oracle twin), both reset and seeded identically."""
     from oracle.oracle import Material, OracleTwin
     from waam_solver_amd.materials import material_from_dict
@@ -91,9 +92,11 @@ def build_pair(name: str, mode: str = "jacobi", engine: str = "fused", layout: i
     prod_mat = material_from_dict({"name": mat_d["name"], "status": "synthetic",
                                    "constants": {k: v for k, v in mat_d.items() if k != "name"},
                                    "tables": {k: [list(p) for p in v] for k, v in tables.items()}})
-    tw = WAAMTwin(material=prod_mat, field_layout=layout, **kw)
+    tw = WAAMTwin(material=prod_mat, field_layout=layout, population_storage=population_storage,
+                  **kw)
     tw.engine = engine
-    ot = OracleTwin(Material(**mat_d, tables=tables), mode=mode, **kw)
+    ot = OracleTwin(Material(**mat_d, tables=tables), mode=mode,
+                    population_storage=population_storage, **kw)
     assert tw.use_material_tables == ot.use_tables == bool(tables)
     for t in (tw, ot):
         configure(t, name)

@@ -1,0 +1,141 @@
+"""Reduced-precision population storage (f - w_q in binary16 / bfloat16, fp32 collision
+arithmetic) of the fused step.
+
+The storage conversions are checked on every 16-bit word against numpy / torch; the whole step
+in each 16-bit form against the oracle twin run with the same storage (its populations narrowed
+to the form exactly where the GPU stores them), bit for bit; a sync in the middle of a run must
+not change it (unpack-then-pack is the identity). The error of each form against the fp32 run
+of the same case is measured and bounded in ``test_storage_error_against_fp32``.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from tests.gpu_common import bits_equal, mismatch_report
+from tests.step_cases import FIELDS, build_pair, compare, run_pair
+
+pytestmark = pytest.mark.gpu
+
+W = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12, np.float32)
+
+
+def _widen(words: np.ndarray, form: str) -> np.ndarray:
+    from oracle.oracle import bf16_bits_to_float
+
+    if form == "fp16":
+        return words.view(np.float16).astype(np.float32)
+    return bf16_bits_to_float(words)
+
+
+@pytest.mark.parametrize("form", ["fp16", "bf16"])
+def test_pack_unpack_every_word(hip_device, form):
+    import torch
+
+    from waam_solver_amd._lib import lib
+
+    code = {"fp16": 1, "bf16": 2}[form]
+    n = 65536
+    words = np.tile(np.arange(n, dtype=np.uint16), (19, 1))
+    h = torch.from_numpy(words.view(np.int16).copy()).to(hip_device)
+    f = torch.zeros((19, n), dtype=torch.float32, device=hip_device)
+    st = torch.cuda.current_stream().cuda_stream
+    lib().call("waam_unpack_populations", h.data_ptr(), f.data_ptr(), n, code, st)
+    got = f.cpu().numpy()
+    want = (_widen(words, form) + W[:, None]).astype(np.float32)
+    assert bits_equal(got, want), mismatch_report("unpack", got, want)
+    # unpack-then-pack returns every word that packing can produce: one round maps each word to
+    # such a word (e.g. bfloat16 words far below w_q's fp32 spacing, and -0, become +0), a
+    # second round changes nothing
+    h2, h3 = torch.zeros_like(h), torch.zeros_like(h)
+    lib().call("waam_pack_populations", f.data_ptr(), h2.data_ptr(), n, code, st)
+    f2 = torch.zeros_like(f)
+    lib().call("waam_unpack_populations", h2.data_ptr(), f2.data_ptr(), n, code, st)
+    lib().call("waam_pack_populations", f2.data_ptr(), h3.data_ptr(), n, code, st)
+    once, twice = h2.cpu().numpy().view(np.uint16), h3.cpu().numpy().view(np.uint16)
+    finite = np.isfinite(_widen(words[0], form))
+    np.testing.assert_array_equal(twice[:, finite], once[:, finite])
+    # a word whose value x has x + w_q exact in fp32 (all that packing produces) stays put
+    xs = _widen(words, form)
+    fixed = ((((xs + W[:, None]).astype(np.float32) - W[:, None]).astype(np.float32) == xs)
+             & ~((xs == 0) & np.signbit(xs)))
+    moved = (once != words) & finite[None, :]
+    assert not np.any(moved & fixed)
+    # narrowing of arbitrary fp32 values against numpy (fp16) and torch (bf16) rounding
+    rng = np.random.default_rng(4)
+    x = (W[:, None] + rng.standard_normal((19, 4096)).astype(np.float32)
+         * np.float32(10.0) ** rng.integers(-9, 0, (19, 4096))).astype(np.float32)
+    xt = torch.from_numpy(x).to(hip_device)
+    h3 = torch.zeros((19, 4096), dtype=torch.int16, device=hip_device)
+    lib().call("waam_pack_populations", xt.data_ptr(), h3.data_ptr(), 4096, code, st)
+    d = (x - W[:, None]).astype(np.float32)
+    if form == "fp16":
+        want3 = d.astype(np.float16).view(np.uint16)
+    else:
+        want3 = torch.from_numpy(d).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    np.testing.assert_array_equal(h3.cpu().numpy().view(np.uint16), want3)
+
+
+@pytest.mark.parametrize("case", ["bead_mrt", "droplets_srt"])
+@pytest.mark.parametrize("form", ["fp16", "bf16"])
+def test_reduced_storage_step_parity(hip_device, case, form):
+    print(f"[precision] {case} {form}", flush=True)
+    tw, ot = build_pair(case, population_storage=form)
+    run_pair(tw, ot, 12)
+    res = compare(tw, ot)
+    bad = {k: v for k, v in res.items() if not v[0]}
+    assert not bad, bad
+    assert tw._n_droplets_fired == ot._n_droplets_fired
+
+
+def test_sync_does_not_perturb_a_fp16_run(hip_device):
+    print("[precision] sync invariance", flush=True)
+    a, _ = build_pair("bead_mrt", population_storage="fp16")
+    b, _ = build_pair("bead_mrt", population_storage="fp16")
+    g = a.grid
+    for s in range(20):
+        from tests.step_cases import torch_x
+
+        x, y = torch_x(g.dx, g.nx, s, g.dt), (g.ny // 2) * g.dx
+        a.step(x, y, True)
+        b.step(x, y, True)
+        if s in (5, 11):
+            b.sync()
+    a.sync()
+    b.sync()
+    for f in FIELDS + ("f_a", "f_b"):
+        x, y = getattr(a.grid, f).to_numpy(), getattr(b.grid, f).to_numpy()
+        assert bits_equal(x, y), f
+
+
+@pytest.mark.parametrize("form,bound_T,bound_phi", [("fp16", 1e-3, 1e-2), ("bf16", 1e-3, 1e-2)])
+def test_storage_error_against_fp32(hip_device, form, bound_T, bound_phi):
+    """Relative L2 of T, phi, f_l and |u| after 100 steps of the seeded-pool bead case against
+    the fp32 run. The pool is a handful of cells across, so any perturbation grows: the fp32
+    run perturbed by one ulp in half of its populations reaches |u| 7.5e-3, phi 1.3e-4 and one
+    flipped flag over the same 100 steps (measured on the oracle). The bounds here are the
+    measured errors with headroom, for regressions; the stated errors are printed."""
+    print(f"[precision] error vs fp32 {form}", flush=True)
+    ref, _ = build_pair("bead_mrt")
+    low, _ = build_pair("bead_mrt", population_storage=form)
+    g = ref.grid
+    from tests.step_cases import torch_x
+
+    for s in range(100):
+        x, y = torch_x(g.dx, g.nx, s, g.dt), (g.ny // 2) * g.dx
+        ref.step(x, y, True)
+        low.step(x, y, True)
+
+    def rl2(a, b):
+        a, b = a.astype(np.float64), b.astype(np.float64)
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+    r, l_ = ref.grid, low.grid
+    err = {k: rl2(getattr(l_, k).to_numpy(), getattr(r, k).to_numpy()) for k in ("T", "phi", "f_l")}
+    sp = lambda gg: np.sqrt(gg.ux.to_numpy().astype(np.float64) ** 2 + gg.uy.to_numpy() ** 2  # noqa
+                            + gg.uz.to_numpy() ** 2)
+    err["|u|"] = rl2(sp(l_), sp(r))
+    print(f"[precision] {form} vs fp32 after 100 steps: " +
+          ", ".join(f"{k} {v:.2e}" for k, v in err.items()), flush=True)
+    assert err["T"] <= bound_T and err["phi"] <= bound_phi

@@ -16,7 +16,7 @@ import torch  # noqa: F401
 
 LIB_PATH = Path(__file__).resolve().parent / "libwaam_hip.so"
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 FLAG_FLUID, FLAG_SOLID, FLAG_GAS, FLAG_IFACE = 0, 1, 2, 4
 LAYOUT_XYZ, LAYOUT_ZYX = 0, 1
 
@@ -74,7 +74,9 @@ SIGNATURES: dict[str, str] = {
     "waam_advect_tracers": "pppppppffiiiiip",
     "waam_telemetry_pool_reduce": "ppppppppiiiipp",
     "waam_step_fused": "ppp",
-    "waam_stream_pending": "pppiiiip",
+    "waam_stream_pending": "pppiiiiip",
+    "waam_pack_populations": "pplip",
+    "waam_unpack_populations": "pplip",
     "waam_step_arc_readback": "pp",
     "waam_timing_begin": "ii",
     "waam_timing_end": "pp",

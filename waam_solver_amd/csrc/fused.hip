@@ -49,6 +49,7 @@
 #include "arc_cell.h"
 #include "forces_cell.h"
 #include "lbm_cell.h"
+#include "pop_store.h"
 #include "thermal_cell.h"
 #include "vof_cell.h"
 
@@ -570,7 +571,8 @@ __global__ void __launch_bounds__(256) k_step_vof_flags(waam_step_fields f, Step
  * this lane stores. Values are exactly those of a cell-by-cell pull. Neighbour indices are
  * recomputed for the bounce-back test rather than kept (19 64-bit indices held across the
  * loads cost 38 registers and a wavefront per SIMD). */
-__device__ __forceinline__ void pull_rows(const float *__restrict__ src, float *__restrict__ dst,
+template <class E>
+__device__ __forceinline__ void pull_rows(const E *__restrict__ src, E *__restrict__ dst,
                                           const uint8_t *__restrict__ sfl, int i, int j, int k,
                                           bool need, uint8_t fc, const Dims &d)
 {
@@ -580,7 +582,7 @@ __device__ __forceinline__ void pull_rows(const float *__restrict__ src, float *
     const int lane = (int)threadIdx.x;  // block3(): one wavefront per threadIdx.y row
     const int a = zyx ? i : k;          // this lane's fast-axis coordinate
     const int nf = d.fast();
-    float v[Q];
+    E v[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         int ni = i - ex(q), nj = j - ey(q), nk = k - ez(q);
@@ -588,8 +590,8 @@ __device__ __forceinline__ void pull_rows(const float *__restrict__ src, float *
         nj = imax(0, imin(nj, d.ny - 1));
         nk = imax(0, imin(nk, d.nz - 1));
         const int sh = zyx ? ex(q) : ez(q);  // fast-axis offset of the source
-        float al = src[(size_t)q * N + (zyx ? d.idx(i, nj, nk) : d.idx(ni, nj, k))];
-        float nb = sh == 0 ? al : __shfl(al, lane - sh);
+        E al = src[(size_t)q * N + (zyx ? d.idx(i, nj, nk) : d.idx(ni, nj, k))];
+        E nb = sh == 0 ? al : shfl_elem(al, lane - sh);
         bool direct = sh != 0 && (lane - sh < 0 || lane - sh > 63 || a - sh < 0 || a - sh >= nf);
         v[q] = direct ? src[(size_t)q * N + d.idx(ni, nj, nk)] : nb;
     }
@@ -609,17 +611,22 @@ __device__ __forceinline__ void pull_rows(const float *__restrict__ src, float *
     for (int q = 0; q < Q; ++q) dst[(size_t)q * N + c] = v[q];
 }
 
-/* body forces, collision and Mach clamp of one FLUID/IFACE cell */
+/* body forces, collision and Mach clamp of one FLUID/IFACE cell; populations in storage form
+ * PD (fp32 arithmetic either way) */
+template <int PD>
 __device__ __forceinline__ void collide_cell(const waam_step_fields &f, const StepConsts &s,
                                              int i, int j, int k, size_t c)
 {
+    using PS = PopStore<PD>;
+    const typename PS::E *f_src = static_cast<const typename PS::E *>(f.f_src);
+    typename PS::E *f_dst = static_cast<typename PS::E *>(f.f_dst);
     const Dims &d = s.d;
     const waam_step_params &p = s.p;
     size_t N = d.n();
     /* every load first (populations, forces, stencils), then the arithmetic */
     float fin[Q];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) fin[q] = f.f_src[(size_t)q * N + c];
+    for (int q = 0; q < Q; ++q) fin[q] = PS::load(f_src, (size_t)q * N + c, q);
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     if (p.forces_from_fields) {
         fx = f.Fx[c];
@@ -670,7 +677,7 @@ __device__ __forceinline__ void collide_cell(const waam_step_fields &f, const St
     if (p.collision == 0) mrt_cell(fin, fx, fy, fz, fl, p.omega_s, p.omega_b, p.C_darcy, o);
     else srt_cell(fin, fx, fy, fz, fl, p.omega_s, p.C_darcy, o);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) f.f_dst[(size_t)q * N + c] = o.f[q];
+    for (int q = 0; q < Q; ++q) PS::store(f_dst, (size_t)q * N + c, q, o.f[q]);
     float u = o.u, v = o.v, w = o.w;
     if (p.u_cap > 0.0f) {
         float mag = sqrtf(u * u + v * v + w * w);
@@ -693,6 +700,7 @@ __device__ __forceinline__ void collide_cell(const waam_step_fields &f, const St
 }
 
 /* 5 — forces + collision of the FLUID/IFACE cells listed by kernel 4 */
+template <int PD>
 __global__ void __launch_bounds__(256) k_step_collide(waam_step_fields f, StepConsts s)
 {
     __shared__ int s_pref[kListSegments + 1];
@@ -703,7 +711,7 @@ __global__ void __launch_bounds__(256) k_step_collide(waam_step_fields f, StepCo
         size_t c = list_at(l, s_pref, t);
         int i, j, k;
         cell_ijk(c, d, i, j, k);
-        collide_cell(f, s, i, j, k, c);
+        collide_cell<PD>(f, s, i, j, k, c);
     }
 }
 
@@ -766,6 +774,7 @@ __device__ __forceinline__ uint8_t freeze_cell(const waam_step_fields &f, const 
  *     step's pull; then the freeze/remelt updates of every non-GAS cell (cell-local; the pull
  *     reads only populations and stream flags, which they do not touch); lists the cells that
  *     end the step FLUID/IFACE, the next step's thermal list */
+template <class E>
 __global__ void __launch_bounds__(256) k_step_stream(waam_step_fields f, StepConsts s)
 {
     const Dims &d = s.d;
@@ -784,7 +793,9 @@ __global__ void __launch_bounds__(256) k_step_stream(waam_step_fields f, StepCon
         fl = f.fl_out[c];
     }
     bool need = !surface && p.pending_stream && fp != SOLID;
-    if (__ballot(need) != 0ull) pull_rows(f.f_src, f.f_dst, f.sflags_prev, i, j, k, need, fp, d);
+    if (__ballot(need) != 0ull)
+        pull_rows(static_cast<const E *>(f.f_src), static_cast<E *>(f.f_dst), f.sflags_prev, i, j,
+                  k, need, fp, d);
     f.sflags_next[c] = fc;
     /* forces are non-zero only on the cells the previous step collided, unless the step before
      * took them from the fields (then force_reset_all) */
@@ -803,7 +814,8 @@ __global__ void __launch_bounds__(256) k_step_stream(waam_step_fields f, StepCon
 }
 
 /* standalone pending stream */
-__global__ void k_stream_pending(const float *__restrict__ src, float *__restrict__ dst,
+template <class E>
+__global__ void k_stream_pending(const E *__restrict__ src, E *__restrict__ dst,
                                  const uint8_t *__restrict__ sfl, Dims d)
 {
     int i, j, k;
@@ -918,6 +930,9 @@ int waam_step_fused(const waam_step_params *pp, const waam_step_fields *ff, void
         return set_error(WAAM_ERR_INVALID, "waam_step_fused: arc column outside the grid");
     if (f.max_tracers > 0 && (!f.tracer_pos || !f.tracer_active))
         return set_error(WAAM_ERR_INVALID, "waam_step_fused: tracer buffers missing");
+    if (f.pop_dtype != WAAM_POP_F32 && f.pop_dtype != WAAM_POP_F16 && f.pop_dtype != WAAM_POP_BF16)
+        return set_error(WAAM_ERR_INVALID, "waam_step_fused: unknown population form %d",
+                         f.pop_dtype);
 
     StepConsts s{};
     s.p = p;
@@ -1005,8 +1020,15 @@ int waam_step_fused(const waam_step_params *pp, const waam_step_fields *ff, void
         hipExtLaunchKernelGGL(k_step_vof_flags, g3, block3(), 0, st, a, b, 0, f, s);
     });
     timed_launch(WAAM_K_COLLIDE, [&](hipEvent_t a, hipEvent_t b) {
-        hipExtLaunchKernelGGL(k_step_collide, dim3(kListBlocks), dim3(kBlock), 0, st, a, b, 0, f,
-                              s);
+        if (f.pop_dtype == WAAM_POP_F16)
+            hipExtLaunchKernelGGL(k_step_collide<WAAM_POP_F16>, dim3(kListBlocks), dim3(kBlock), 0,
+                                  st, a, b, 0, f, s);
+        else if (f.pop_dtype == WAAM_POP_BF16)
+            hipExtLaunchKernelGGL(k_step_collide<WAAM_POP_BF16>, dim3(kListBlocks), dim3(kBlock), 0,
+                                  st, a, b, 0, f, s);
+        else
+            hipExtLaunchKernelGGL(k_step_collide<WAAM_POP_F32>, dim3(kListBlocks), dim3(kBlock), 0,
+                                  st, a, b, 0, f, s);
     });
     if (f.max_tracers > 0)
         timed_launch(WAAM_K_TRACERS, [&](hipEvent_t a, hipEvent_t b) {
@@ -1016,21 +1038,106 @@ int waam_step_fused(const waam_step_params *pp, const waam_step_fields *ff, void
                                   (const uint8_t *)f.flags_out, p.dx, f.max_tracers, d);
         });
     timed_launch(WAAM_K_STREAM, [&](hipEvent_t a, hipEvent_t b) {
-        hipExtLaunchKernelGGL(k_step_stream, g3, block3(), 0, st, a, b, 0, f, s);
+        if (f.pop_dtype == WAAM_POP_F32)
+            hipExtLaunchKernelGGL(k_step_stream<float>, g3, block3(), 0, st, a, b, 0, f, s);
+        else
+            hipExtLaunchKernelGGL(k_step_stream<uint16_t>, g3, block3(), 0, st, a, b, 0, f, s);
     });
     return check_launch("waam_step_fused");
 }
 
-int waam_stream_pending(const float *f_src, float *f_dst, const uint8_t *sflags, int nx, int ny,
-                        int nz, int layout, void *stream)
+int waam_stream_pending(const void *f_src, void *f_dst, const uint8_t *sflags, int nx, int ny,
+                        int nz, int layout, int pop_dtype, void *stream)
 {
     WAAM_RETURN_IF(check_dims("waam_stream_pending", nx, ny, nz, layout));
     WAAM_RETURN_IF(check_ptrs("waam_stream_pending", {f_src, f_dst, sflags}));
     if (f_src == f_dst) return set_error(WAAM_ERR_INVALID, "waam_stream_pending: f_src aliases f_dst");
     Dims d{nx, ny, nz, layout};
-    hipLaunchKernelGGL(k_stream_pending, grid3(d), block3(), 0, as_stream(stream), f_src, f_dst,
-                       sflags, d);
+    if (pop_dtype == WAAM_POP_F32)
+        hipLaunchKernelGGL(k_stream_pending<float>, grid3(d), block3(), 0, as_stream(stream),
+                           static_cast<const float *>(f_src), static_cast<float *>(f_dst), sflags,
+                           d);
+    else if (pop_dtype == WAAM_POP_F16 || pop_dtype == WAAM_POP_BF16)
+        hipLaunchKernelGGL(k_stream_pending<uint16_t>, grid3(d), block3(), 0, as_stream(stream),
+                           static_cast<const uint16_t *>(f_src), static_cast<uint16_t *>(f_dst),
+                           sflags, d);
+    else
+        return set_error(WAAM_ERR_INVALID, "waam_stream_pending: unknown population form %d",
+                         pop_dtype);
     return check_launch("waam_stream_pending");
+}
+
+}  // extern "C"
+
+namespace waam {
+
+/* q-major population arrays: element e belongs to direction e / n_cells */
+template <int PD>
+__global__ void k_pack_pops(const float *__restrict__ src, uint16_t *__restrict__ dst, size_t n,
+                            size_t n_cells)
+{
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x)
+        PopStore<PD>::store(dst, e, (int)(e / n_cells), src[e]);
+}
+
+template <int PD>
+__global__ void k_unpack_pops(const uint16_t *__restrict__ src, float *__restrict__ dst, size_t n,
+                              size_t n_cells)
+{
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x)
+        dst[e] = PopStore<PD>::load(src, e, (int)(e / n_cells));
+}
+
+}  // namespace waam
+
+extern "C" {
+
+int waam_pack_populations(const float *src, void *dst, int64_t n_cells, int pop_dtype,
+                          void *stream)
+{
+    WAAM_RETURN_IF(check_ptrs("waam_pack_populations", {src, dst}));
+    if (n_cells <= 0)
+        return set_error(WAAM_ERR_INVALID, "waam_pack_populations: n_cells %lld",
+                         (long long)n_cells);
+    size_t n = (size_t)n_cells * Q;
+    unsigned g = grid_for(n);
+    if (g > 65536u) g = 65536u;
+    uint16_t *o = static_cast<uint16_t *>(dst);
+    if (pop_dtype == WAAM_POP_F16)
+        hipLaunchKernelGGL(k_pack_pops<WAAM_POP_F16>, dim3(g), dim3(kBlock), 0, as_stream(stream),
+                           src, o, n, (size_t)n_cells);
+    else if (pop_dtype == WAAM_POP_BF16)
+        hipLaunchKernelGGL(k_pack_pops<WAAM_POP_BF16>, dim3(g), dim3(kBlock), 0, as_stream(stream),
+                           src, o, n, (size_t)n_cells);
+    else
+        return set_error(WAAM_ERR_INVALID, "waam_pack_populations: pop_dtype %d is not a 16-bit "
+                         "form", pop_dtype);
+    return check_launch("waam_pack_populations");
+}
+
+int waam_unpack_populations(const void *src, float *dst, int64_t n_cells, int pop_dtype,
+                            void *stream)
+{
+    WAAM_RETURN_IF(check_ptrs("waam_unpack_populations", {src, dst}));
+    if (n_cells <= 0)
+        return set_error(WAAM_ERR_INVALID, "waam_unpack_populations: n_cells %lld",
+                         (long long)n_cells);
+    size_t n = (size_t)n_cells * Q;
+    unsigned g = grid_for(n);
+    if (g > 65536u) g = 65536u;
+    const uint16_t *i = static_cast<const uint16_t *>(src);
+    if (pop_dtype == WAAM_POP_F16)
+        hipLaunchKernelGGL(k_unpack_pops<WAAM_POP_F16>, dim3(g), dim3(kBlock), 0,
+                           as_stream(stream), i, dst, n, (size_t)n_cells);
+    else if (pop_dtype == WAAM_POP_BF16)
+        hipLaunchKernelGGL(k_unpack_pops<WAAM_POP_BF16>, dim3(g), dim3(kBlock), 0,
+                           as_stream(stream), i, dst, n, (size_t)n_cells);
+    else
+        return set_error(WAAM_ERR_INVALID, "waam_unpack_populations: pop_dtype %d is not a "
+                         "16-bit form", pop_dtype);
+    return check_launch("waam_unpack_populations");
 }
 
 int waam_step_arc_readback(const void *scratch, float *host_out3)

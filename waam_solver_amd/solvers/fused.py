@@ -12,6 +12,15 @@ placed volume), then the fused kernels with the arc marked as applied.
 Buffer pairs: the grid holds H/H_tmp, T/T_tmp, f_l/fl_tmp, flags/flags_tmp and
 sflags/sflags_tmp; after each fused step every pair swaps, as do the population buffers. The
 population stream of the last step is deferred (``pending``); ``flush`` materialises it.
+
+Population storage (``twin.population_storage``): "fp32" steps the grid's fp32 population
+buffers. "fp16" / "bf16" step a pair of 16-bit buffers holding f - w_q (fp32 arithmetic in the
+collision, round-to-nearest-even stores); the fp32 buffers stay the state every other kernel,
+the host and the exports see. ``flush`` (``twin.sync()``) unpacks the 16-bit pair into them;
+the next fused step packs them again, which returns the same 16-bit words (unpack-then-pack is
+the identity on every word a pack or a collision store produces: those are fp32 differences
+f - w_q, multiples of w_q's fp32 spacing, which fp32 holds exactly again once w_q is added), so
+syncing does not perturb a reduced-precision run.
 """
 
 from __future__ import annotations
@@ -22,7 +31,7 @@ import torch
 
 from .. import kernels
 from .._lib import lib
-from ..field import DeviceField, P, write_epoch
+from ..field import DeviceField, P, tensor_of, write_epoch
 from ..physics import weld_forces
 from ..physics.electrical_stickout import update_ctwd
 from .coupled_step import _clamp_enthalpy_ceiling, _droplet, droplet_due, resolve_arc_k
@@ -36,7 +45,11 @@ class StepFields(ctypes.Structure):
         "fl_out", "phi", "phi_tmp", "flags_in", "flags_out", "sflags_prev", "sflags_next", "Fx",
         "Fy", "Fz", "T_max", "T_prev", "dT_dt", "t800", "t1100", "tsol", "tracer_pos",
         "tracer_active", "list_thermal", "list_collide", "scratch")] + [
-            (n, _i32) for n in ("max_tracers", "nx", "ny", "nz", "layout")]
+            (n, _i32) for n in ("max_tracers", "nx", "ny", "nz", "layout", "pop_dtype")]
+
+# population storage forms (WAAM_POP_*): fp32, or f - w_q in binary16 / bfloat16
+POP_FORMS = {"fp32": 0, "fp16": 1, "bf16": 2}
+_POP_TORCH = {"fp16": torch.float16, "bf16": torch.bfloat16}
 
 
 _INT_PARAMS = ("is_welding", "collision", "arc_kind", "arc_applied", "pending_stream",
@@ -94,21 +107,54 @@ class FusedStepper:
         # launches take 773 us and the shared one 940 us)
         self.shared_thermal_max_cells = 1 << 23
         self._arc_out = (ctypes.c_float * 3)()
+        form = getattr(twin, "population_storage", "fp32")
+        if form not in POP_FORMS:
+            raise ValueError(f"population_storage {form!r} (fp32 | fp16 | bf16)")
+        self.pop_form, self.pop_code = form, POP_FORMS[form]
+        # the 16-bit pair holds the populations (the fp32 buffers are stale) between a fused
+        # step and the next flush
+        self.h_live = False
+        if self.pop_code:
+            shape = tensor_of(g.f_a).shape
+            dt = _POP_TORCH[form]
+            g.hf_a = torch.zeros(shape, dtype=dt, device=g.device)
+            g.hf_b = torch.zeros(shape, dtype=dt, device=g.device)
         # the swapped pairs alternate between two pointer sets: keep one struct per set
         self._fields_cache: dict[tuple, StepFields] = {}
 
     # ------------------------------------------------------------------ deferred stream
-    def flush(self) -> None:
-        """Materialise the deferred population stream of the last fused step."""
-        if not self.pending:
-            return
+    def _pop_buffers(self):
+        """(src, dst) population buffers the fused kernels use."""
         g = self.twin.grid
-        lib().call("waam_stream_pending", P(g.f_src), P(g.f_dst), P(g.sflags), g.nx, g.ny, g.nz,
-                   g.layout, torch.cuda.current_stream().cuda_stream)
-        self.pending = False
+        if not self.pop_code:
+            return g.f_src, g.f_dst
+        return (g.hf_a, g.hf_b) if g.f_src is g.f_a else (g.hf_b, g.hf_a)
+
+    def flush(self) -> None:
+        """Materialise the deferred population stream of the last fused step and, with 16-bit
+        storage, unpack the populations into the grid's fp32 buffers."""
+        g = self.twin.grid
+        st = torch.cuda.current_stream().cuda_stream
+        if self.pending:
+            src, dst = self._pop_buffers()
+            lib().call("waam_stream_pending", P(src), P(dst), P(g.sflags), g.nx, g.ny, g.nz,
+                       g.layout, self.pop_code, st)
+            self.pending = False
+        if self.h_live:
+            for h, f in ((g.hf_a, g.f_a), (g.hf_b, g.f_b)):
+                lib().call("waam_unpack_populations", P(h), P(f), g.n_cells, self.pop_code, st)
+            self.h_live = False
+
+    def _pack(self) -> None:
+        g = self.twin.grid
+        st = torch.cuda.current_stream().cuda_stream
+        for f, h in ((g.f_a, g.hf_a), (g.f_b, g.hf_b)):
+            lib().call("waam_pack_populations", P(f), P(h), g.n_cells, self.pop_code, st)
+        self.h_live = True
 
     def reset(self) -> None:
         self.pending = False
+        self.h_live = False
         self._list_epoch = None
 
     def last_arc_k(self) -> float:
@@ -129,7 +175,8 @@ class FusedStepper:
     def _build_fields(self) -> StepFields:
         g = self.twin.grid
         sf = StepFields()
-        for name, fld in (("f_src", g.f_src), ("f_dst", g.f_dst), ("rho", g.rho), ("ux", g.ux),
+        src, dst = self._pop_buffers()
+        for name, fld in (("f_src", src), ("f_dst", dst), ("rho", g.rho), ("ux", g.ux),
                           ("uy", g.uy), ("uz", g.uz), ("H_in", g.H), ("H_out", g.H_tmp),
                           ("T_in", g.T), ("T_out", g.T_tmp), ("fl_in", g.f_l),
                           ("fl_out", g.fl_tmp), ("phi", g.phi), ("phi_tmp", g.phi_tmp),
@@ -144,6 +191,7 @@ class FusedStepper:
             setattr(sf, name, P(fld))
         sf.max_tracers, sf.nx, sf.ny, sf.nz = g.max_tracers, g.nx, g.ny, g.nz
         sf.layout = g.layout
+        sf.pop_dtype = self.pop_code
         return sf
 
     def _params(self, arc_i, arc_j, is_welding, torch_z_m) -> StepParams:
@@ -252,6 +300,8 @@ class FusedStepper:
             p.thermal_list_ready = 0
         else:
             p.thermal_list_ready = 1 if g.n_cells <= self.shared_thermal_max_cells else 2
+        if self.pop_code and not self.h_live:
+            self._pack()
         fields = self._fields()
         lib().call("waam_step_fused", ctypes.addressof(p), ctypes.addressof(fields),
                    torch.cuda.current_stream().cuda_stream)

@@ -69,7 +69,11 @@ class WAAMTwin:
                  enable_enthalpy_cap: bool = True, T_vapor_cap_K: float = 3200.0,
                  arc_surface_weighting: bool = True, arc_penetration_mm: float = 2.0,
                  wire_diameter_mm: float = 1.2, arc_sigma_mm: float = 2.0,
-                 bulk_tau: float | None = None, device=None, field_layout: int | None = None):
+                 bulk_tau: float | None = None, device=None, field_layout: int | None = None,
+                 population_storage: str = "fp32"):
+        if population_storage not in ("fp32", "fp16", "bf16"):
+            raise ValueError(f"population_storage must be fp32, fp16 or bf16, got "
+                             f"{population_storage!r}")
         if arc_power_W < 0:
             raise ValueError(f"arc_power_W must be >= 0, got {arc_power_W}")
         if not (0.0 < arc_efficiency <= 1.0):
@@ -159,6 +163,10 @@ class WAAMTwin:
         # "fused": waam_step_fused (eight launches per step, the default); "reference": one
         # launch per reference kernel in the reference's order. Both give identical fields.
         self.engine = "fused"
+        # fused steps keep the populations as fp32 ("fp32") or as f - w_q in 16 bits ("fp16",
+        # "bf16": fp32 collision arithmetic, half the population traffic); every other path and
+        # the host see fp32 populations (twin.sync() unpacks)
+        self.population_storage = population_storage
         self._lorentz_unconverged = 0
         self._lorentz_unconverged_streak = 0
         self._lorentz_warm = False
