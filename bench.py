@@ -113,20 +113,20 @@ def cpu_baseline(w: dict, seconds: float) -> dict:
                       f"(steps 2..{steps + 1}) on the OpenMP build of the oracle, {el:.1f} s"}
 
 
-# The dominant kernel is k_step_stream, the deferred pull stream of the populations. Its
-# algorithmic bytes per launch (each array element touched once; neighbour re-reads and the
-# force reset of inactive cells not counted):
-#   every cell        flags read, previous stream flags read, stream flags written   3 B
-#   pulled cell       19 populations read + 19 written                              152 B
+# The dominant kernel is k_step_pull, the deferred pull stream of the populations (a launch of
+# its own ahead of k_step_stream). Its algorithmic bytes per launch (each array element touched
+# once; neighbour re-reads not counted):
+#   every cell        flags read, previous stream flags read                         2 B
+#   pulled cell       19 populations read + 19 written: fp32 152 B, 16-bit storage  76 B
 # A pulled cell is one the collision skips this step that was not SOLID at the step whose
 # stream is being completed.
-STREAM_B_EVERY = 3
+STREAM_B_EVERY = 2
 STREAM_B_PULL = {"fp32": 152, "fp16": 76, "bf16": 76}
 # Whole-step compulsory bytes per cell with the HAZ fields: populations 2 x 19 x 4 B (fp32) or
 # 2 x 19 x 2 B (16-bit storage), H T f_l phi r/w 32 B, u r/w 24 B, flags 2 B, rho 4 B, HAZ 44 B
 STEP_B_PER_CELL = {"fp32": 258, "fp16": 182, "bf16": 182}
-DOMINANT = "k_step_stream"
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_step_stream_bead_on_plate_256x128x64_pmc.md")
+DOMINANT = "k_step_pull"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_step_pull_bead_on_plate_256x128x64_pmc.md")
 
 
 def stream_classes(g) -> tuple[int, int]:
@@ -218,7 +218,7 @@ def kernel_breakdown(tw, w: dict, start: int, steps: int) -> dict:
 
 
 def measured_traffic() -> float | None:
-    """HBM bytes per k_step_stream launch from the committed rocprofv3 counter summary."""
+    """HBM bytes per k_step_pull launch from the committed rocprofv3 counter summary."""
     try:
         with open(TRAFFIC_FILE) as fh:
             for line in fh:

@@ -496,7 +496,8 @@ enum {
     WAAM_K_COLLIDE = 4,
     WAAM_K_TRACERS = 5,
     WAAM_K_STREAM = 6,
-    WAAM_K_COUNT = 7
+    WAAM_K_PULL = 7, /* the population pull when it runs ahead of kernel 7 in a launch of its own */
+    WAAM_K_COUNT = 8
 };
 int waam_timing_begin(int which, int max_launches);
 int waam_timing_end(double *total_ms, int64_t *launches);

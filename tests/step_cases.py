@@ -50,6 +50,9 @@ def case_kwargs(name: str) -> dict:
         # moving window: 40 cells long, shifts of 8 cells when the torch passes 62 % of 32
         return dict(base, nx=40, ny=24, nz=24, dx=2e-4, droplet_freq_hz=0.0,
                     enable_moving_window=True)
+    if name == "bead_odd_mrt":
+        # an odd contiguous extent (nx = 63): the 16-bit storage pulls cell by cell
+        return dict(base, nx=63, ny=40, nz=32, dx=2e-4, droplet_freq_hz=0.0)
     if name == "bench_small":
         return dict(base, nx=128, ny=64, nz=32, dx=2e-4, droplet_freq_hz=0.0)
     raise KeyError(name)

@@ -77,11 +77,14 @@ def test_pack_unpack_every_word(hip_device, form):
     np.testing.assert_array_equal(h3.cpu().numpy().view(np.uint16), want3)
 
 
-@pytest.mark.parametrize("case", ["bead_mrt", "droplets_srt"])
+@pytest.mark.parametrize("case,layout", [("bead_mrt", None), ("droplets_srt", None),
+                                         ("bead_mrt", 0), ("bead_odd_mrt", None)])
 @pytest.mark.parametrize("form", ["fp16", "bf16"])
-def test_reduced_storage_step_parity(hip_device, case, form):
-    print(f"[precision] {case} {form}", flush=True)
-    tw, ot = build_pair(case, population_storage=form)
+def test_reduced_storage_step_parity(hip_device, case, layout, form):
+    """Pairs of cells per lane on an even contiguous extent (x in the default layout, z in the
+    reference layout, where z clamps instead of wrapping), cell by cell on an odd one."""
+    print(f"[precision] {case} layout={layout} {form}", flush=True)
+    tw, ot = build_pair(case, layout=layout, population_storage=form)
     run_pair(tw, ot, 12)
     res = compare(tw, ot)
     bad = {k: v for k, v in res.items() if not v[0]}

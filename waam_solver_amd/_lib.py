@@ -112,7 +112,7 @@ SIGNATURES: dict[str, str] = {
 
 # kernel ids of waam_step_fused for waam_timing_begin (include/waam_hip.h WAAM_K_*)
 STEP_KERNELS = ("k_step_begin", "k_step_thermal", "k_step_thermal_surface", "k_step_vof_flags",
-                "k_step_collide", "k_advect_tracers_step", "k_step_stream")
+                "k_step_collide", "k_advect_tracers_step", "k_step_stream", "k_step_pull")
 
 
 class WaamError(RuntimeError):

@@ -60,6 +60,13 @@ constexpr int kListBlocks = 1024;
 /* list blocks sharing a launch with the thermal sweep: about one per CU, so the sweep keeps
  * most of the slots (measured: 1024 or 512 list blocks crowd it, 64 leave the list long) */
 constexpr int kListBlocksWithSweep = 256;
+/* the pull of fp32 (16-bit per-cell) populations in a launch of its own rather than inside
+ * kernel 7 */
+#ifndef WAAM_SEPARATE_PULL32
+#define WAAM_SEPARATE_PULL32 1
+#endif
+constexpr bool kSeparatePull32 = WAAM_SEPARATE_PULL32 != 0;
+constexpr bool kSeparatePull16 = false;
 
 static_assert(kBlock == kListSegments, "the list prefix scan uses one thread per segment");
 
@@ -774,13 +781,12 @@ __device__ __forceinline__ uint8_t freeze_cell(const waam_step_fields &f, const 
  *     step's pull; then the freeze/remelt updates of every non-GAS cell (cell-local; the pull
  *     reads only populations and stream flags, which they do not touch); lists the cells that
  *     end the step FLUID/IFACE, the next step's thermal list */
-template <class E>
-__global__ void __launch_bounds__(256) k_step_stream(waam_step_fields f, StepConsts s)
+template <class E, bool PULL>
+__device__ __forceinline__ void stream_cell(const waam_step_fields &f, const StepConsts &s, int i,
+                                            int j, int k)
 {
     const Dims &d = s.d;
     const waam_step_params &p = s.p;
-    int i, j, k;
-    if (!d.cell(i, j, k)) return;
     size_t c = d.idx(i, j, k);
     uint8_t fc = f.flags_out[c];
     uint8_t fp = f.sflags_prev[c];
@@ -793,7 +799,7 @@ __global__ void __launch_bounds__(256) k_step_stream(waam_step_fields f, StepCon
         fl = f.fl_out[c];
     }
     bool need = !surface && p.pending_stream && fp != SOLID;
-    if (__ballot(need) != 0ull)
+    if (PULL && __ballot(need) != 0ull)
         pull_rows(static_cast<const E *>(f.f_src), static_cast<E *>(f.f_dst), f.sflags_prev, i, j,
                   k, need, fp, d);
     f.sflags_next[c] = fc;
@@ -811,6 +817,195 @@ __global__ void __launch_bounds__(256) k_step_stream(waam_step_fields f, StepCon
         fin = freeze_cell(f, s, i, j, k, c, fc, T, phi, fl, step_arc_k(p, tot));
     }
     list_append(is_surface_class(fin), c, cell_list(f, s.seg_len, 0), d.n());
+}
+
+/* kernel 7 without the pull: a lane takes kStreamSlices cells along the slowest axis and issues
+ * the flag loads of all of them, then the freeze inputs of all of them, before any of the
+ * per-cell work: the sweep moves a few bytes per cell and is otherwise bound by the two
+ * dependent memory round trips of each cell */
+constexpr int kStreamSlices = 4;
+
+__device__ __forceinline__ void stream_rest_cell(const waam_step_fields &f, const StepConsts &s,
+                                                 int i, int j, int k, size_t c, uint8_t fc,
+                                                 uint8_t fp, float T, float phi, float fl)
+{
+    const waam_step_params &p = s.p;
+    bool surface = is_surface_class(fc);
+    f.sflags_next[c] = fc;
+    if (!surface && p.store_forces && !p.forces_from_fields &&
+        (p.force_reset_all || is_surface_class(fp))) {
+        f.Fx[c] = 0.0f;
+        f.Fy[c] = 0.0f;
+        f.Fz[c] = 0.0f;
+    }
+    uint8_t fin = fc;
+    if (p.enable_freeze && fc != GAS) {
+        const float *tot = reinterpret_cast<const float *>((char *)f.scratch + kScratchTelemetry);
+        fin = freeze_cell(f, s, i, j, k, c, fc, T, phi, fl, step_arc_k(p, tot));
+    }
+    list_append(is_surface_class(fin), c, cell_list(f, s.seg_len, 0), s.d.n());
+}
+
+template <class E, bool PULL>
+__global__ void __launch_bounds__(256) k_step_stream(waam_step_fields f, StepConsts s)
+{
+    const Dims &d = s.d;
+    int i, j, k;
+    if (PULL) {
+        if (!d.cell(i, j, k)) return;
+        stream_cell<E, true>(f, s, i, j, k);
+        return;
+    }
+    const int a = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    if (a >= d.fast() || j >= d.ny) return;
+    const bool zyx = d.layout == kLayoutZYX;
+    const int b0 = (int)blockIdx.z * kStreamSlices;
+    const bool freeze_on = s.p.enable_freeze != 0;
+    size_t c[kStreamSlices];
+    uint8_t fc[kStreamSlices], fp[kStreamSlices];
+    float T[kStreamSlices], phi[kStreamSlices], fl[kStreamSlices];
+#pragma unroll
+    for (int t = 0; t < kStreamSlices; ++t) {
+        const int b = imin(b0 + t, d.slow() - 1);
+        c[t] = zyx ? d.idx(a, j, b) : d.idx(b, j, a);
+        fc[t] = f.flags_out[c[t]];
+        fp[t] = f.sflags_prev[c[t]];
+    }
+#pragma unroll
+    for (int t = 0; t < kStreamSlices; ++t) {
+        const bool fr = freeze_on && fc[t] != GAS;
+        T[t] = fr ? f.T_out[c[t]] : 0.0f;
+        phi[t] = fr ? f.phi[c[t]] : 0.0f;
+        fl[t] = fr ? f.fl_out[c[t]] : 0.0f;
+    }
+#pragma unroll
+    for (int t = 0; t < kStreamSlices; ++t) {
+        const int b = b0 + t;
+        if (b >= d.slow()) break;
+        stream_rest_cell(f, s, zyx ? a : b, j, zyx ? b : a, c[t], fc[t], fp[t], T[t], phi[t],
+                         fl[t]);
+    }
+}
+
+/* 7a — the previous step's pull in a launch of its own, ahead of kernel 7 (which changes
+ *      flags_out): every cell this step does not collide and that was not SOLID at the step
+ *      being completed pulls with that step's flags. A pure streaming pass keeps more loads in
+ *      flight than the pull inside kernel 7's per-cell work. */
+template <class E>
+__global__ void __launch_bounds__(256) k_step_pull(waam_step_fields f, StepConsts s)
+{
+    const Dims &d = s.d;
+    int i, j, k;
+    if (!d.cell(i, j, k)) return;
+    size_t c = d.idx(i, j, k);
+    uint8_t fc = f.flags_out[c];
+    uint8_t fp = f.sflags_prev[c];
+    bool need = !is_surface_class(fc) && fp != SOLID;
+    if (__ballot(need) != 0ull)
+        pull_rows(static_cast<const E *>(f.f_src), static_cast<E *>(f.f_dst), f.sflags_prev, i, j,
+                  k, need, fp, d);
+}
+
+/* neighbour's position along the contiguous axis: x wraps, z clamps (reference stream,
+ * kernels.py:2991-3010) */
+__device__ __forceinline__ int fast_neighbour(int a, int sh, const Dims &d)
+{
+    int n = a - sh, nf = d.fast();
+    if (d.layout == kLayoutZYX) return n < 0 ? n + nf : (n >= nf ? n - nf : n);
+    return imax(0, imin(n, nf - 1));
+}
+
+/* 7a with 16-bit storage — the pull on pairs of cells along the contiguous axis: each lane
+ * moves one 32-bit word (the two cells' populations) per direction, and the neighbouring pair's
+ * word comes from the adjacent lane, so every load and store is a full dword of a coalesced
+ * row. Lanes whose neighbour word lies outside the wavefront or across the domain edge load
+ * their two values one by one. Values are those of the cell-by-cell pull. Needs an even
+ * contiguous extent (the host launches k_step_pull<uint16_t> otherwise). */
+__global__ void __launch_bounds__(256) k_step_pull_pair16(waam_step_fields f, StepConsts s)
+{
+    const Dims &d = s.d;
+    const int lane = (int)threadIdx.x;
+    const int nw = d.fast() >> 1;
+    const int w = (int)(blockIdx.x * blockDim.x) + lane;
+    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    const int b = (int)blockIdx.z;
+    if (w >= nw || j >= d.ny) return;
+    const bool zyx = d.layout == kLayoutZYX;
+    const int a0 = 2 * w;
+    const size_t N = d.n();
+    const size_t c0 = zyx ? d.idx(a0, j, b) : d.idx(b, j, a0);
+    const uint16_t fc2 = *reinterpret_cast<const uint16_t *>(f.flags_out + c0);
+    const uint16_t fp2 = *reinterpret_cast<const uint16_t *>(f.sflags_prev + c0);
+    const uint8_t fc0 = (uint8_t)(fc2 & 0xffu), fc1 = (uint8_t)(fc2 >> 8);
+    const uint8_t fp0 = (uint8_t)(fp2 & 0xffu), fp1 = (uint8_t)(fp2 >> 8);
+    const bool need0 = !is_surface_class(fc0) && fp0 != SOLID;
+    const bool need1 = !is_surface_class(fc1) && fp1 != SOLID;
+    if (__ballot(need0 || need1) == 0ull) return;
+    const uint16_t *src = static_cast<const uint16_t *>(f.f_src);
+    uint16_t *dst = static_cast<uint16_t *>(f.f_dst);
+    /* the row each direction pulls from (the other two axes' neighbours) */
+    auto row_of = [&](int q) {
+        int nj = imax(0, imin(j - ey(q), d.ny - 1));
+        int nb = zyx ? imax(0, imin(b - ez(q), d.nz - 1))
+                     : (b - ex(q) < 0 ? b - ex(q) + d.nx
+                                      : (b - ex(q) >= d.nx ? b - ex(q) - d.nx : b - ex(q)));
+        return src + (size_t)q * N + (zyx ? d.idx(0, nj, nb) : d.idx(nb, nj, 0));
+    };
+    /* every load first, then the shuffles: all 19 words in flight at once */
+    uint32_t v[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) v[q] = reinterpret_cast<const uint32_t *>(row_of(q))[w];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int sh = zyx ? ex(q) : ez(q);
+        if (sh == 0) continue;
+        const uint32_t aw = v[q];
+        const uint32_t ow = (uint32_t)__shfl((int)aw, lane - sh);
+        v[q] = sh > 0 ? ((aw << 16) | (ow >> 16)) : ((aw >> 16) | (ow << 16));
+    }
+    /* lanes whose neighbour word is in another wavefront or across the domain edge */
+    if (lane == 0 || lane == 63 || w == 0 || w == nw - 1) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int sh = zyx ? ex(q) : ez(q);
+            const bool direct = sh > 0 ? (lane == 0 || w == 0)
+                                       : (sh < 0 && (lane == 63 || w == nw - 1));
+            if (direct) {
+                const uint16_t *row = row_of(q);
+                v[q] = (uint32_t)row[fast_neighbour(a0, sh, d)] |
+                       ((uint32_t)row[fast_neighbour(a0 + 1, sh, d)] << 16);
+            }
+        }
+    }
+    /* bounce-back off SOLID neighbours for a non-GAS destination (a cell that is neither GAS nor
+     * collided this step: rare) */
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const bool nh = h == 0 ? need0 : need1;
+        const uint8_t fch = h == 0 ? fc0 : fc1;
+        if (!nh || fch == GAS) continue;
+        const int a = a0 + h;
+        const int i = zyx ? a : b, k = zyx ? b : a;
+        const size_t c = c0 + (size_t)h;
+        for (int q = 0; q < Q; ++q) {
+            int ni = i - ex(q), nj = j - ey(q), nk = k - ez(q);
+            ni = ni < 0 ? ni + d.nx : (ni >= d.nx ? ni - d.nx : ni);
+            nj = imax(0, imin(nj, d.ny - 1));
+            nk = imax(0, imin(nk, d.nz - 1));
+            if (f.sflags_prev[d.idx(ni, nj, nk)] == SOLID) {
+                uint32_t r = src[(size_t)opp(q) * N + c];
+                v[q] = h == 0 ? ((v[q] & 0xffff0000u) | r) : ((v[q] & 0xffffu) | (r << 16));
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        uint16_t *o = dst + (size_t)q * N + c0;
+        if (need0 && need1) *reinterpret_cast<uint32_t *>(o) = v[q];
+        else if (need0) o[0] = (uint16_t)(v[q] & 0xffffu);
+        else if (need1) o[1] = (uint16_t)(v[q] >> 16);
+    }
 }
 
 /* standalone pending stream */
@@ -1037,11 +1232,31 @@ int waam_step_fused(const waam_step_params *pp, const waam_step_fields *ff, void
                                   f.ux, f.uy, f.uz, (const float *)f.fl_out,
                                   (const uint8_t *)f.flags_out, p.dx, f.max_tracers, d);
         });
+    /* the pull runs in a launch of its own ahead of kernel 7 (pairs of cells for 16-bit
+     * storage on an even contiguous extent), or inside kernel 7 */
+    const bool f32 = f.pop_dtype == WAAM_POP_F32;
+    const bool pair16 = !f32 && d.fast() % 2 == 0;
+    const bool separate = pair16 || (f32 ? kSeparatePull32 : kSeparatePull16);
+    if (separate && p.pending_stream)
+        timed_launch(WAAM_K_PULL, [&](hipEvent_t a, hipEvent_t b) {
+            if (pair16) {
+                dim3 gp((unsigned)((d.fast() / 2 + kTileFast - 1) / kTileFast), g3.y, g3.z);
+                hipExtLaunchKernelGGL(k_step_pull_pair16, gp, block3(), 0, st, a, b, 0, f, s);
+            } else if (f32) {
+                hipExtLaunchKernelGGL(k_step_pull<float>, g3, block3(), 0, st, a, b, 0, f, s);
+            } else {
+                hipExtLaunchKernelGGL(k_step_pull<uint16_t>, g3, block3(), 0, st, a, b, 0, f, s);
+            }
+        });
     timed_launch(WAAM_K_STREAM, [&](hipEvent_t a, hipEvent_t b) {
-        if (f.pop_dtype == WAAM_POP_F32)
-            hipExtLaunchKernelGGL(k_step_stream<float>, g3, block3(), 0, st, a, b, 0, f, s);
+        dim3 gs(g3.x, g3.y, (unsigned)((d.slow() + kStreamSlices - 1) / kStreamSlices));
+        if (separate)
+            hipExtLaunchKernelGGL(k_step_stream<float, false>, gs, block3(), 0, st, a, b, 0, f, s);
+        else if (f32)
+            hipExtLaunchKernelGGL(k_step_stream<float, true>, g3, block3(), 0, st, a, b, 0, f, s);
         else
-            hipExtLaunchKernelGGL(k_step_stream<uint16_t>, g3, block3(), 0, st, a, b, 0, f, s);
+            hipExtLaunchKernelGGL(k_step_stream<uint16_t, true>, g3, block3(), 0, st, a, b, 0, f,
+                                  s);
     });
     return check_launch("waam_step_fused");
 }
