@@ -38,6 +38,8 @@ extern "C" {
 #define WAAM_ABI_VERSION 3
 
 enum { WAAM_FLAG_FLUID = 0, WAAM_FLAG_SOLID = 1, WAAM_FLAG_GAS = 2, WAAM_FLAG_IFACE = 4 };
+/* the moving lid of the lid-driven cavity (waam_cavity_step); not a flag of the coupled step */
+enum { WAAM_CAVITY_LID = 3 };
 enum { WAAM_LAYOUT_XYZ = 0, WAAM_LAYOUT_ZYX = 1 };
 /* Population storage of the fused step: fp32 populations, or f - w_q in binary16 / bfloat16
  * (round to nearest even on store, fp32 arithmetic in between). */
@@ -377,6 +379,17 @@ int waam_shift_simulation_window_x(int n_shift, float *f_a, float *f_b, float *T
  * (the reference computes it on the host from every flag). */
 int waam_bead_top_k(const uint8_t *flags, int nz_solid, int32_t *out, int nx, int ny, int nz,
                     int layout, void *stream);
+
+/* ---- lid-driven cavity --------------------------------------------------------------------
+ * No counterpart in the reference: its stream (kernels.py:2982-3010) has bounce-back only and
+ * its buffer choreography moves no momentum between fluid cells. One step of the plain D3Q19
+ * cavity: every FLUID cell pulls from its neighbours (all axes periodic; walls are cells),
+ * half-way bounce-back off other cells, plus 6 w_q (e_q . u_w) off WAAM_CAVITY_LID cells
+ * (Ladd's moving wall, wall density 1), then collide_mrt's map (cumulant_kernel.py:28-209)
+ * without force or Darcy damping. Writes f_dst, rho and u of FLUID cells only. */
+int waam_cavity_step(const float *f_src, float *f_dst, float *rho, float *ux, float *uy,
+                     float *uz, const uint8_t *flags, float omega_s, float omega_b, float uwx,
+                     float uwy, float uwz, int nx, int ny, int nz, int layout, void *stream);
 
 /* ---- export-side fields and reductions -------------------------------------------------- */
 /* compute_vorticity_magnitude kernels.py:3380-3419: |curl u| / dt on non-GAS cells, 0 on GAS */

@@ -127,6 +127,7 @@ _SIGS = {
     "or_elec_B_axisymmetric": [_P] * 4 + [_F] * 5 + [_I, _I, _I, _I],
     "or_apply_lorentz_JxB": [_P] * 11 + [_F, _F, _F, _I, _I, _I],
     "or_compute_vorticity_magnitude": [_P] * 5 + [_F, _I, _I, _I],
+    "or_cavity_step": [_P] * 7 + [_F] * 5 + [_I, _I, _I],
 }
 
 

@@ -36,6 +36,8 @@
 #define SOLID 1
 #define GAS 2
 #define IFACE 4
+/* the moving lid of the cavity case (not a reference flag) */
+#define CAVITY_LID 3
 #define Q 19
 
 /* D3Q19 velocity set, weights and opposites (grid.py:19-34, kernels.py:42-51). */
@@ -1306,89 +1308,145 @@ void or_collide_srt_variable_tau(const float *f_src, float *f_dst, float *rho, f
     }
 }
 
-void or_collide_mrt(const float *f_src, float *f_dst, float *rho, float *ux, float *uy,
-                    float *uz, const float *Fx, const float *Fy, const float *Fz,
-                    const float *f_l, const uint8_t *flags, float omega_s, float omega_b,
-                    float C_darcy, int nx, int ny, int nz)
+/* collide_mrt's per-cell map (cumulant_kernel.py:28-209) on the populations f of one cell:
+ * post-collision populations into fo, density and velocity into ro/uo/vo/wo */
+static void mrt_cell_o(const float f[Q], float fx, float fy, float fz, float fl, float omega_s,
+                       float omega_b, float C_darcy, float fo[Q], float *ro, float *uo, float *vo,
+                       float *wo)
 {
     const float cs2 = (float)(1.0 / 3.0);
     const float inv_cs2 = 3.0f;
     const float eps = 1e-6f;
     /* truncated-decimal weights of the reference expansion (cumulant_kernel.py:147-202) */
     const float W0 = 0.3333333333f, W1 = 0.0555555556f, W7 = 0.0277777778f;
+    float r = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8] + f[9] + f[10] +
+              f[11] + f[12] + f[13] + f[14] + f[15] + f[16] + f[17] + f[18];
+    float inv_rho = 1.0f / (r + eps);
+    float jx = f[1] * 1.0f + f[2] * -1.0f + f[7] * 1.0f + f[8] * -1.0f + f[9] * 1.0f +
+               f[10] * -1.0f + f[11] * 1.0f + f[12] * -1.0f + f[13] * 1.0f + f[14] * -1.0f;
+    float jy = f[3] * 1.0f + f[4] * -1.0f + f[7] * 1.0f + f[8] * -1.0f + f[9] * -1.0f +
+               f[10] * 1.0f + f[15] * 1.0f + f[16] * -1.0f + f[17] * 1.0f + f[18] * -1.0f;
+    float jz = f[5] * 1.0f + f[6] * -1.0f + f[11] * 1.0f + f[12] * -1.0f + f[13] * -1.0f +
+               f[14] * 1.0f + f[15] * 1.0f + f[16] * -1.0f + f[17] * -1.0f + f[18] * 1.0f;
+    float uxr = (jx + 0.5f * fx) * inv_rho;
+    float uyr = (jy + 0.5f * fy) * inv_rho;
+    float uzr = (jz + 0.5f * fz) * inv_rho;
+    float om = 1.0f - fl;
+    float ck = 1.0f + C_darcy * (om * om) / (fl * (fl * fl) + eps);
+    float u = uxr / ck, v = uyr / ck, w = uzr / ck;
+    float Pxx = f[1] + f[2] + f[7] + f[8] + f[9] + f[10] + f[11] + f[12] + f[13] + f[14];
+    float Pyy = f[3] + f[4] + f[7] + f[8] + f[9] + f[10] + f[15] + f[16] + f[17] + f[18];
+    float Pzz = f[5] + f[6] + f[11] + f[12] + f[13] + f[14] + f[15] + f[16] + f[17] + f[18];
+    float Pxy = f[7] * 1.0f * 1.0f + f[8] * -1.0f * -1.0f + f[9] * 1.0f * -1.0f +
+                f[10] * -1.0f * 1.0f;
+    float Pxz = f[11] * 1.0f * 1.0f + f[12] * -1.0f * -1.0f + f[13] * 1.0f * -1.0f +
+                f[14] * -1.0f * 1.0f;
+    float Pyz = f[15] * 1.0f * 1.0f + f[16] * -1.0f * -1.0f + f[17] * 1.0f * -1.0f +
+                f[18] * -1.0f * 1.0f;
+    float kxx = Pxx - r * u * u, kyy = Pyy - r * v * v, kzz = Pzz - r * w * w;
+    float kxy = Pxy - r * u * v, kxz = Pxz - r * u * w, kyz = Pyz - r * v * w;
+    float km = (kxx + kyy + kzz) / 3.0f;
+    float kmp = km - omega_b * (km - r * cs2);
+    float kxxp = kmp + (1.0f - omega_s) * (kxx - km);
+    float kyyp = kmp + (1.0f - omega_s) * (kyy - km);
+    float kzzp = kmp + (1.0f - omega_s) * (kzz - km);
+    float kxyp = kxy - omega_s * kxy;
+    float kxzp = kxz - omega_s * kxz;
+    float kyzp = kyz - omega_s * kyz;
+    float u2 = u * u + v * v + w * w;
+    float uF = u * fx + v * fy + w * fz;
+    float dxx = 1.0f * (kxxp - r * cs2), dyy = 1.0f * (kyyp - r * cs2),
+          dzz = 1.0f * (kzzp - r * cs2);
+    float Kq[Q];
+    Kq[0] = 0.0f;
+    Kq[1] = Kq[2] = dxx;
+    Kq[3] = Kq[4] = dyy;
+    Kq[5] = Kq[6] = dzz;
+    Kq[7] = Kq[8] = (dxx + dyy) + (2.0f * kxyp);
+    Kq[9] = Kq[10] = (dxx + dyy) + (-2.0f * kxyp);
+    Kq[11] = Kq[12] = (dxx + dzz) + (2.0f * kxzp);
+    Kq[13] = Kq[14] = (dxx + dzz) + (-2.0f * kxzp);
+    Kq[15] = Kq[16] = (dyy + dzz) + (2.0f * kyzp);
+    Kq[17] = Kq[18] = (dyy + dzz) + (-2.0f * kyzp);
+    for (int q = 0; q < Q; ++q) {
+        float wq = q == 0 ? W0 : (q < 7 ? W1 : W7);
+        float ex = (float)EX[q], ey = (float)EY[q], ez = (float)EZ[q];
+        float eu = ex * u + ey * v + ez * w;
+        float eF = ex * fx + ey * fy + ez * fz;
+        float feq = wq * (r + r * inv_cs2 * (eu) + r * inv_cs2 * inv_cs2 * 0.5f * (eu) * (eu) -
+                          r * 0.5f * inv_cs2 * u2 + 0.5f * inv_cs2 * inv_cs2 * (Kq[q]));
+        float S = wq * (1.0f - 0.5f * omega_s) *
+                  (inv_cs2 * (eF) + inv_cs2 * inv_cs2 * (eF) * (eu) - inv_cs2 * (uF));
+        fo[q] = feq + S;
+    }
+    *ro = r;
+    *uo = u;
+    *vo = v;
+    *wo = w;
+}
+
+void or_collide_mrt(const float *f_src, float *f_dst, float *rho, float *ux, float *uy,
+                    float *uz, const float *Fx, const float *Fy, const float *Fz,
+                    const float *f_l, const uint8_t *flags, float omega_s, float omega_b,
+                    float C_darcy, int nx, int ny, int nz)
+{
     size_t N = NCELL;
 #pragma omp parallel for schedule(static)
     for (long cc = 0; cc < (long)N; ++cc) {
         size_t c = (size_t)cc;
         if (flags[c] == SOLID || flags[c] == GAS) continue;
-        float fl = f_l[c], fx = Fx[c], fy = Fy[c], fz = Fz[c];
-        float f[Q];
+        float f[Q], fo[Q], r, u, v, w;
         for (int q = 0; q < Q; ++q) f[q] = f_src[(size_t)q * N + c];
-        float r = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8] + f[9] + f[10] +
-                  f[11] + f[12] + f[13] + f[14] + f[15] + f[16] + f[17] + f[18];
-        float inv_rho = 1.0f / (r + eps);
-        float jx = f[1] * 1.0f + f[2] * -1.0f + f[7] * 1.0f + f[8] * -1.0f + f[9] * 1.0f +
-                   f[10] * -1.0f + f[11] * 1.0f + f[12] * -1.0f + f[13] * 1.0f + f[14] * -1.0f;
-        float jy = f[3] * 1.0f + f[4] * -1.0f + f[7] * 1.0f + f[8] * -1.0f + f[9] * -1.0f +
-                   f[10] * 1.0f + f[15] * 1.0f + f[16] * -1.0f + f[17] * 1.0f + f[18] * -1.0f;
-        float jz = f[5] * 1.0f + f[6] * -1.0f + f[11] * 1.0f + f[12] * -1.0f + f[13] * -1.0f +
-                   f[14] * 1.0f + f[15] * 1.0f + f[16] * -1.0f + f[17] * -1.0f + f[18] * 1.0f;
-        float uxr = (jx + 0.5f * fx) * inv_rho;
-        float uyr = (jy + 0.5f * fy) * inv_rho;
-        float uzr = (jz + 0.5f * fz) * inv_rho;
-        float om = 1.0f - fl;
-        float ck = 1.0f + C_darcy * (om * om) / (fl * (fl * fl) + eps);
-        float u = uxr / ck, v = uyr / ck, w = uzr / ck;
-        float Pxx = f[1] + f[2] + f[7] + f[8] + f[9] + f[10] + f[11] + f[12] + f[13] + f[14];
-        float Pyy = f[3] + f[4] + f[7] + f[8] + f[9] + f[10] + f[15] + f[16] + f[17] + f[18];
-        float Pzz = f[5] + f[6] + f[11] + f[12] + f[13] + f[14] + f[15] + f[16] + f[17] + f[18];
-        float Pxy = f[7] * 1.0f * 1.0f + f[8] * -1.0f * -1.0f + f[9] * 1.0f * -1.0f +
-                    f[10] * -1.0f * 1.0f;
-        float Pxz = f[11] * 1.0f * 1.0f + f[12] * -1.0f * -1.0f + f[13] * 1.0f * -1.0f +
-                    f[14] * -1.0f * 1.0f;
-        float Pyz = f[15] * 1.0f * 1.0f + f[16] * -1.0f * -1.0f + f[17] * 1.0f * -1.0f +
-                    f[18] * -1.0f * 1.0f;
-        float kxx = Pxx - r * u * u, kyy = Pyy - r * v * v, kzz = Pzz - r * w * w;
-        float kxy = Pxy - r * u * v, kxz = Pxz - r * u * w, kyz = Pyz - r * v * w;
-        float km = (kxx + kyy + kzz) / 3.0f;
-        float kmp = km - omega_b * (km - r * cs2);
-        float kxxp = kmp + (1.0f - omega_s) * (kxx - km);
-        float kyyp = kmp + (1.0f - omega_s) * (kyy - km);
-        float kzzp = kmp + (1.0f - omega_s) * (kzz - km);
-        float kxyp = kxy - omega_s * kxy;
-        float kxzp = kxz - omega_s * kxz;
-        float kyzp = kyz - omega_s * kyz;
-        float u2 = u * u + v * v + w * w;
-        float uF = u * fx + v * fy + w * fz;
-        float dxx = 1.0f * (kxxp - r * cs2), dyy = 1.0f * (kyyp - r * cs2),
-              dzz = 1.0f * (kzzp - r * cs2);
-        float Kq[Q];
-        Kq[0] = 0.0f;
-        Kq[1] = Kq[2] = dxx;
-        Kq[3] = Kq[4] = dyy;
-        Kq[5] = Kq[6] = dzz;
-        Kq[7] = Kq[8] = (dxx + dyy) + (2.0f * kxyp);
-        Kq[9] = Kq[10] = (dxx + dyy) + (-2.0f * kxyp);
-        Kq[11] = Kq[12] = (dxx + dzz) + (2.0f * kxzp);
-        Kq[13] = Kq[14] = (dxx + dzz) + (-2.0f * kxzp);
-        Kq[15] = Kq[16] = (dyy + dzz) + (2.0f * kyzp);
-        Kq[17] = Kq[18] = (dyy + dzz) + (-2.0f * kyzp);
-        for (int q = 0; q < Q; ++q) {
-            float wq = q == 0 ? W0 : (q < 7 ? W1 : W7);
-            float ex = (float)EX[q], ey = (float)EY[q], ez = (float)EZ[q];
-            float eu = ex * u + ey * v + ez * w;
-            float eF = ex * fx + ey * fy + ez * fz;
-            float feq = wq * (r + r * inv_cs2 * (eu) + r * inv_cs2 * inv_cs2 * 0.5f * (eu) * (eu) -
-                              r * 0.5f * inv_cs2 * u2 + 0.5f * inv_cs2 * inv_cs2 * (Kq[q]));
-            float S = wq * (1.0f - 0.5f * omega_s) *
-                      (inv_cs2 * (eF) + inv_cs2 * inv_cs2 * (eF) * (eu) - inv_cs2 * (uF));
-            f_dst[(size_t)q * N + c] = feq + S;
-        }
+        mrt_cell_o(f, Fx[c], Fy[c], Fz[c], f_l[c], omega_s, omega_b, C_darcy, fo, &r, &u, &v, &w);
+        for (int q = 0; q < Q; ++q) f_dst[(size_t)q * N + c] = fo[q];
         rho[c] = r;
         ux[c] = u;
         uy[c] = v;
         uz[c] = w;
     }
+}
+
+/* Lid-driven cavity step (no counterpart in the reference, whose stream has no moving wall,
+ * kernels.py:2982-3010): every FLUID cell pulls from its neighbours (all three axes periodic),
+ * half-way bounce-back off wall cells, with Ladd's moving-wall term 6 w_q (e_q . u_w) (wall
+ * density 1) off LID cells, then collide_mrt's map without force or Darcy damping. */
+void or_cavity_step(const float *f_src, float *f_dst, float *rho, float *ux, float *uy,
+                    float *uz, const uint8_t *flags, float omega_s, float omega_b, float uwx,
+                    float uwy, float uwz, int nx, int ny, int nz)
+{
+    size_t N = NCELL;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                size_t c = IDX(i, j, k);
+                if (flags[c] != FLUID) continue;
+                float f[Q], fo[Q], r, u, v, w;
+                for (int q = 0; q < Q; ++q) {
+                    int ni = (i - EX[q] + nx) % nx, nj = (j - EY[q] + ny) % ny;
+                    int nk = (k - EZ[q] + nz) % nz;
+                    size_t n = IDX(ni, nj, nk);
+                    uint8_t fn = flags[n];
+                    if (fn == FLUID) {
+                        f[q] = f_src[(size_t)q * N + n];
+                    } else {
+                        float b = f_src[(size_t)OPP[q] * N + c];
+                        if (fn == CAVITY_LID) {
+                            float wq = q == 0 ? (float)(1.0 / 3.0)
+                                              : (q < 7 ? (float)(1.0 / 18.0) : (float)(1.0 / 36.0));
+                            float eu = (float)EX[q] * uwx + (float)EY[q] * uwy + (float)EZ[q] * uwz;
+                            b = b + 6.0f * wq * eu;
+                        }
+                        f[q] = b;
+                    }
+                }
+                mrt_cell_o(f, 0.0f, 0.0f, 0.0f, 1.0f, omega_s, omega_b, 0.0f, fo, &r, &u, &v, &w);
+                for (int q = 0; q < Q; ++q) f_dst[(size_t)q * N + c] = fo[q];
+                rho[c] = r;
+                ux[c] = u;
+                uy[c] = v;
+                uz[c] = w;
+            }
 }
 
 void or_stream(const float *f_src, float *f_dst, const uint8_t *flags, int nx, int ny, int nz)

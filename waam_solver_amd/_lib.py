@@ -108,6 +108,7 @@ SIGNATURES: dict[str, str] = {
     "waam_compute_vorticity_magnitude": "pppppffiiiip",
     "waam_max_norm3": "pppplp",
     "waam_probe_gather": "pppilpp",
+    "waam_cavity_step": "pppppppfffffiiiip",
 }
 
 # kernel ids of waam_step_fused for waam_timing_begin (include/waam_hip.h WAAM_K_*)

@@ -751,6 +751,15 @@ def max_norm3_into(fx, fy, fz, out) -> None:
     _call_ro("waam_max_norm3", P(fx), P(fy), P(fz), P(out), _n(fx), _stream())
 
 
+def cavity_step(f_src, f_dst, rho, ux, uy, uz, flags, omega_s, omega_b, u_wall, nx, ny, nz):
+    """One lid-driven cavity step (pull stream with bounce-back and a moving lid, MRT)."""
+    _need(nx * ny * nz, rho, ux, uy, uz, flags)
+    _need_pop(nx * ny * nz, f_src, f_dst)
+    uwx, uwy, uwz = (float(v) for v in u_wall)
+    _call("waam_cavity_step", P(f_src), P(f_dst), P(rho), P(ux), P(uy), P(uz), P(flags),
+          float(omega_s), float(omega_b), uwx, uwy, uwz, nx, ny, nz, layout_of(rho), _stream())
+
+
 def probe_gather_into(T, f_l, cells, out) -> None:
     """out[2p], out[2p + 1] = T, f_l at linear cell cells[p] (int64 device tensor, storage order
     of the fields; negative -> NaN)."""
