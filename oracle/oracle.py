@@ -132,14 +132,15 @@ _SIGS = {
 
 
 class _Lib:
-    def __init__(self, name: str):
+    def __init__(self, name: str, f64: bool = False):
         self.dll = _load(name)
+        real = ctypes.c_double if f64 else _F
         for fn, sig in _SIGS.items():
             f = getattr(self.dll, fn)
-            f.argtypes = sig
+            f.argtypes = [real if t is _F else t for t in sig]
             f.restype = None
-        self.dll.or_arc_weight_sum.restype = _F
-        self.dll.or_surface_height_at.restype = _F
+        self.dll.or_arc_weight_sum.restype = real
+        self.dll.or_surface_height_at.restype = real
 
     def __getattr__(self, item):
         return getattr(self.dll, item)
@@ -148,10 +149,12 @@ class _Lib:
 _LIBS: dict[str, _Lib] = {}
 
 
-def lib(parallel: bool = False) -> _Lib:
-    name = "liboracle_omp.so" if parallel else "liboracle.so"
+def lib(parallel: bool = False, f64: bool = False) -> _Lib:
+    """The oracle library: fp32 (single-threaded, or OpenMP with ``parallel``) or the binary64
+    build of the same kernels (``f64``, OpenMP; arrays float64, scalars double)."""
+    name = "liboracle_f64.so" if f64 else ("liboracle_omp.so" if parallel else "liboracle.so")
     if name not in _LIBS:
-        _LIBS[name] = _Lib(name)
+        _LIBS[name] = _Lib(name, f64=f64)
     return _LIBS[name]
 
 
@@ -240,7 +243,9 @@ def pack_tables(mat: "Material") -> tuple[np.ndarray, np.ndarray]:
 # field; f_a is written by init_grid.
 # --------------------------------------------------------------------------------------------
 class OracleGrid:
-    def __init__(self, nx: int, ny: int, nz: int, dx: float, mat: Material, max_tracers: int):
+    def __init__(self, nx: int, ny: int, nz: int, dx: float, mat: Material, max_tracers: int,
+                 dtype=np.float32):
+        self.dtype = dtype
         self.nx, self.ny, self.nz, self.dx = nx, ny, nz, dx
         self.u_ref_phys, self.u_ref_lu = 0.5, 0.05
         self.dt = dx * self.u_ref_lu / self.u_ref_phys
@@ -253,9 +258,9 @@ class OracleGrid:
         if self.alpha_lu > 1.0 / 6.0:
             raise ValueError(f"Thermal step unstable: alpha_lu={self.alpha_lu:.4f} > 1/6")
         s3 = (nx, ny, nz)
-        z = lambda: np.zeros(s3, np.float32)  # noqa: E731
-        self.f_a = np.zeros((Q,) + s3, np.float32)
-        self.f_b = np.zeros((Q,) + s3, np.float32)
+        z = lambda: np.zeros(s3, dtype)  # noqa: E731
+        self.f_a = np.zeros((Q,) + s3, dtype)
+        self.f_b = np.zeros((Q,) + s3, dtype)
         for name in ("rho", "T", "T_max", "H", "f_l", "phi", "dT_dt", "T_prev",
                      "time_above_800_s", "time_above_1100_s", "time_above_solidus_s",
                      "Fx_snap", "Fy_snap", "Fz_snap", "ux", "uy", "uz", "Fx", "Fy", "Fz",
@@ -264,11 +269,11 @@ class OracleGrid:
             setattr(self, name, z())
         self.flags = np.zeros(s3, np.uint8)
         self.max_tracers = max_tracers
-        self.porosity_pos = np.zeros((max_tracers, 3), np.float32)
+        self.porosity_pos = np.zeros((max_tracers, 3), dtype)
         self.porosity_active = np.zeros(max_tracers, np.int32)
         self.tracer_head = np.zeros(1, np.int32)
-        self.deposit_vol = np.zeros(1, np.float32)
-        self.deposit_real = np.zeros(1, np.float32)
+        self.deposit_vol = np.zeros(1, dtype)
+        self.deposit_real = np.zeros(1, dtype)
         self.read_a = True
 
     @property
@@ -311,9 +316,12 @@ class OracleTwin:
                  T_boiling_K=3100.0, P_vapor_ref_Pa=101325.0, L_vapor_J_kg=6.0e6,
                  R_spec_vapor_J_kgK=450.0, enable_lorentz=False, sigma_liquid_Sm=1.0e6,
                  sigma_solid_Sm=1.0e5, lorentz_jacobi_iters=40, lorentz_jacobi_omega=0.85,
-                 population_storage="fp32"):
+                 population_storage="fp32", precision="fp32"):
         self.mat = material
-        self.lib = lib(parallel)
+        if precision not in ("fp32", "fp64"):
+            raise ValueError(f"precision {precision!r}")
+        self.precision = precision
+        self.lib = lib(parallel, f64=precision == "fp64")
         self.mode = mode
         if population_storage not in ("fp32", "fp16", "bf16"):
             raise ValueError(f"population_storage {population_storage!r}")
@@ -377,7 +385,8 @@ class OracleTwin:
         self.enable_moving_window = enable_moving_window
         self._window_offset_x_m = 0.0
         self.tracer_seed = tracer_seed
-        self.g = OracleGrid(nx, ny, nz, dx, material, 50_000 if max_tracers is None else max_tracers)
+        self.g = OracleGrid(nx, ny, nz, dx, material, 50_000 if max_tracers is None else max_tracers,
+                            dtype=np.float64 if precision == "fp64" else np.float32)
         g = self.g
         self.cp_rho = material.rho * material.cp
         self.L_rho = material.rho * material.L_fusion
@@ -457,7 +466,7 @@ class OracleTwin:
                        ptr(g.H), ptr(g.f_l), ptr(g.phi), ptr(g.flags), T0, 1.0, self.cp_rho,
                        nzs, i0, i1, j0, j1, *g.dims)
         for a in (g.T_max, g.T_prev):
-            a.fill(np.float32(T0))
+            a.fill(g.dtype(T0))
         for a in (g.dT_dt, g.time_above_800_s, g.time_above_1100_s, g.time_above_solidus_s,
                   g.Fx, g.Fy, g.Fz, g.Fx_snap, g.Fy_snap, g.Fz_snap):
             a.fill(0)
@@ -799,9 +808,9 @@ class OracleTwin:
         if not hasattr(g, "phi_elec"):
             for n in ("phi_elec", "phi_elec_tmp", "sigma_elec", "elec_source", "Jx", "Jy", "Jz",
                       "Bx", "By", "Bz"):
-                setattr(g, n, np.zeros(s3, np.float32))
+                setattr(g, n, np.zeros(s3, g.dtype))
             g.n_elec_bins = 64
-            g.elec_rad_bins = np.zeros((g.nz, 64), np.float32)
+            g.elec_rad_bins = np.zeros((g.nz, 64), g.dtype)
         L.or_elec_build_sigma(ptr(g.sigma_elec), ptr(g.f_l), ptr(g.flags), self.sigma_liquid_Sm,
                               self.sigma_solid_Sm, *d)
         g.elec_source.fill(0)
@@ -816,7 +825,7 @@ class OracleTwin:
         if cold:
             n_max = max(g.nx, g.ny, g.nz)
             max_iters = max(max_iters, 2 * n_max * n_max)
-        out = np.zeros(2, np.float32)
+        out = np.zeros(2, g.dtype)
         done, converged = 0, False
         while done < max_iters:
             for _ in range(min(10, max_iters - done)):
@@ -864,7 +873,7 @@ class OracleTwin:
         n_shift = max(8, g.nx // 10)
         self.shift_window(n_shift)
         self._window_offset_x_m += n_shift * g.dx
-        g.porosity_pos[:, 0] -= np.float32(n_shift * g.dx)
+        g.porosity_pos[:, 0] -= g.dtype(n_shift * g.dx)
 
     def coupled_step(self, x_m, y_m, is_welding):
         g, L, m = self.g, self.lib, self.mat
@@ -886,10 +895,11 @@ class OracleTwin:
             m = self.mat
             consts = np.array([m.rho, g.dt, g.dx, m.cp, m.k, m.mu, m.dgamma_dT,
                                self.force_scale / g.dx, self.cp_rho, self.alpha_lu,
-                               self.dgamma_dT_lu, g.tau, self.marangoni_scale], np.float32)
+                               self.dgamma_dT_lu, g.tau, self.marangoni_scale], g.dtype)
+            knots = self.table_knots.astype(g.dtype)
             L.or_refresh_thermal_properties(ptr(g.T), ptr(g.cp_rho_field), ptr(g.alpha_lu_field),
                                             ptr(g.dgamma_lu_field), ptr(g.tau_field),
-                                            ptr(self.table_knots), ptr(self.table_counts),
+                                            ptr(knots), ptr(self.table_counts),
                                             ptr(consts), 1, ptr(g.flags), *d)
             if self.mode == "serial":
                 L.or_advect_diffuse_temperature_variable_serial(

@@ -32,6 +32,18 @@
 
 #include "../waam_solver_amd/csrc/waam_math.h"
 
+#ifdef ORACLE_F64
+/* The same kernels in binary64: every fp32 variable, argument and cast below becomes a double
+ * and the fp32 math calls their double forms (exp for the shared fp32 exponential). Literals
+ * written as fp32 constants keep their fp32 values; the operation sequence is unchanged. */
+#define float double
+#define sqrtf sqrt
+#define fabsf fabs
+#define fmaxf fmax
+#define fminf fmin
+#define waam_expf exp
+#endif
+
 #define FLUID 0
 #define SOLID 1
 #define GAS 2
