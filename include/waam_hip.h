@@ -391,6 +391,38 @@ int waam_cavity_step(const float *f_src, float *f_dst, float *rho, float *ux, fl
                      float *uz, const uint8_t *flags, float omega_s, float omega_b, float uwx,
                      float uwy, float uwz, int nx, int ny, int nz, int layout, void *stream);
 
+/* ---- the differentiation form: adjoint of the thermal sub-step ----------------------------
+ * No counterpart in the reference (it has no derivatives). The forward map is the coupled
+ * step's thermal chain (solvers/coupled_step.py:97-160) with frozen flags and zero velocity:
+ * Gaussian2D arc injection, enthalpy ceiling, conduction from the step's starting temperature,
+ * boundary losses, enthalpy floor, phase map, ceiling, phase map. Parameters: the arc
+ * efficiency eta, a factor loss_scale on both loss coefficients (h_conv0, eps_rad0 are the
+ * unscaled ones) and a factor sigma / sigma0 on the arc radius (cells). */
+typedef struct waam_adj_consts {
+    float cp_rho, L_rho, T_s, T_l;       /* phase map */
+    float T_cap, T_floor, alpha_lu;      /* ceiling temperature, floor temperature, conduction */
+    float T_amb, h_conv0, eps_rad0, sigma_sb, dt, dx, loss_scale;
+    float arc_i, arc_j, arc_k, sigma, sigma0, Q_w, eta, dx3, pen;
+    int32_t enable_cap, enable_loss, enable_arc, enable_conv, enable_rad, surface_weight;
+} waam_adj_consts;
+/* out2 (device, 2 doubles) = sum over weighted non-GAS cells of wg wdep and of wg wdep r^2 */
+int waam_adj_arc_sums(const uint8_t *flags, const float *f_l, const waam_adj_consts *c,
+                      double *out2, int nx, int ny, int nz, int layout, void *scratch,
+                      void *stream);
+/* lam = grad_T dT/dH + grad_fl df_l/dH at enthalpy H (0 on GAS cells) */
+int waam_adj_terminal(const float *H, const float *grad_T, const float *grad_fl,
+                      const uint8_t *flags, const waam_adj_consts *c, float *lam, int64_t n,
+                      void *stream);
+/* One step back: lam_out = dL/dH_n from lam_next = dL/dH_{n+1}, given the step's enthalpy at
+ * its start (H_n), after the arc (A), before the floor (E) and after it (F), the starting
+ * f_l and the step's arc sums; adds the step's dL/d(eta, loss_scale, sigma scale) to grads
+ * (device, 3 doubles). lamC is an n-cell work buffer. */
+int waam_adj_thermal_step(const float *lam_next, float *lam_out, const float *H_n, const float *A,
+                          const float *E, const float *F, const float *f_l_n,
+                          const uint8_t *flags, const double *arc_sums,
+                          const waam_adj_consts *c, double *grads, float *lamC, int nx, int ny,
+                          int nz, int layout, void *scratch, void *stream);
+
 /* ---- export-side fields and reductions -------------------------------------------------- */
 /* compute_vorticity_magnitude kernels.py:3380-3419: |curl u| / dt on non-GAS cells, 0 on GAS */
 int waam_compute_vorticity_magnitude(const float *ux, const float *uy, const float *uz,

@@ -108,16 +108,20 @@ int main(void) {
 %s
 printf("size waam_step_fields %zu\n", sizeof(waam_step_fields));
 printf("size waam_step_params %zu\n", sizeof(waam_step_params));
+printf("size waam_adj_consts %zu\n", sizeof(waam_adj_consts));
 return 0; }
 """
 
 
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc not available")
 def test_struct_layouts_match_header(tmp_path):
+    from waam_solver_amd.adjoint import AdjConsts
     from waam_solver_amd.solvers.fused import StepFields, StepParams
 
+    structs = (("waam_step_fields", StepFields), ("waam_step_params", StepParams),
+               ("waam_adj_consts", AdjConsts))
     lines = []
-    for cname, py in (("waam_step_fields", StepFields), ("waam_step_params", StepParams)):
+    for cname, py in structs:
         for field, _ in py._fields_:
             lines.append(f"F({cname}, {field})")
     src = tmp_path / "offsets.c"
@@ -130,7 +134,7 @@ def test_struct_layouts_match_header(tmp_path):
     for line in out.splitlines():
         a, b, c = line.split()
         c_off[(a, b)] = int(c)
-    for cname, py in (("waam_step_fields", StepFields), ("waam_step_params", StepParams)):
+    for cname, py in structs:
         for field, _ in py._fields_:
             assert getattr(py, field).offset == c_off[(cname, field)], (cname, field)
         # same size: the C struct has no member the ctypes one lacks (all members are 4 or 8 B,

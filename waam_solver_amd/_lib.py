@@ -109,6 +109,9 @@ SIGNATURES: dict[str, str] = {
     "waam_max_norm3": "pppplp",
     "waam_probe_gather": "pppilpp",
     "waam_cavity_step": "pppppppfffffiiiip",
+    "waam_adj_arc_sums": "ppppiiiipp",
+    "waam_adj_terminal": "pppppplp",
+    "waam_adj_thermal_step": "ppppppppppppiiiipp",
 }
 
 # kernel ids of waam_step_fused for waam_timing_begin (include/waam_hip.h WAAM_K_*)

@@ -1,0 +1,248 @@
+"""The differentiation form: gradients of the thermal sub-step through many steps.
+
+Forward: the coupled step's thermal chain with frozen flags (and phi) and zero velocity, on a
+copy of a twin's state (reference solvers/coupled_step.py:97-160, the flow and the free surface
+held): Gaussian2D arc injection, enthalpy ceiling, conduction from the step's starting
+temperature, boundary losses, enthalpy floor, phase map, ceiling, phase map — the product's own
+kernels, one launch each. Parameters theta = (eta, s_h, s_sigma): the arc efficiency, a factor
+on both loss coefficients (the calibration overlay's heat-loss factor) and a factor on the arc
+radius (its sigma scale).
+
+Backward: hand-written adjoint kernels (``csrc/adjoint.hip``) step the adjoint state
+lambda = dL/dH back through each step and accumulate dL/dtheta in binary64. The forward keeps
+the enthalpy every ``checkpoint_every`` steps; the backward recomputes each segment from its
+checkpoint (keeping the step's enthalpy after the arc, before and after the floor, and its
+starting f_l), then runs the segment's adjoint steps in reverse. ``ThermalRolloutFn`` wraps the
+pair as a ``torch.autograd.Function`` of theta returning the final (T, f_l), so any loss written
+in torch differentiates through it; ``fit_calibration`` runs gradient descent on it.
+
+Derivatives at the piecewise points are one-sided (a clamped value has derivative 0; the phase
+map's slope is the one of the branch the enthalpy is on); f_l enters the arc's surface weight
+only through a step at 0.55, whose derivative is 0.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import kernels
+from ._lib import lib
+from .field import P, tensor_of
+from .solvers.coupled_step import resolve_arc_k
+
+_f = ctypes.c_float
+
+
+class AdjConsts(ctypes.Structure):
+    _fields_ = [(n, _f) for n in (
+        "cp_rho", "L_rho", "T_s", "T_l", "T_cap", "T_floor", "alpha_lu", "T_amb", "h_conv0",
+        "eps_rad0", "sigma_sb", "dt", "dx", "loss_scale", "arc_i", "arc_j", "arc_k", "sigma",
+        "sigma0", "Q_w", "eta", "dx3", "pen")] + [(n, ctypes.c_int32) for n in (
+            "enable_cap", "enable_loss", "enable_arc", "enable_conv", "enable_rad",
+            "surface_weight")]
+
+
+class ThermalRollout:
+    """n_steps of the thermal sub-step from the twin's current state along ``path``
+    (step -> torch (x_m, y_m)), differentiable in theta."""
+
+    def __init__(self, twin, n_steps: int, path, checkpoint_every: int = 50,
+                 is_welding: bool = True):
+        if type(twin.arc_source).__name__ != "Gaussian2D":
+            raise NotImplementedError("the differentiation form covers the Gaussian2D arc")
+        twin.sync()
+        self.tw, g = twin, twin.grid
+        self.n_steps, self.K = int(n_steps), max(1, int(checkpoint_every))
+        self.path, self.is_welding = path, bool(is_welding)
+        f = lambda: g.field(torch.float32)  # noqa: E731
+        self.H, self.T, self.fl, self.Htmp = f(), f(), f(), f()
+        # the starting state is the twin's enthalpy with T and f_l from the phase map, as at
+        # every later step boundary (and as the checkpoints restore it)
+        self.H0, self.T0, self.fl0 = f(), f(), f()
+        tensor_of(self.H0).copy_(tensor_of(g.H))
+        tensor_of(self.H).copy_(tensor_of(g.H))
+        self._phase()
+        tensor_of(self.T0).copy_(tensor_of(self.T))
+        tensor_of(self.fl0).copy_(tensor_of(self.fl))
+        self.zero_u = f()
+        self.flags, self.phi = g.flags, g.phi
+        self.h_conv0, self.eps0 = float(twin.h_conv), float(twin.eps_rad)
+        self.sigma0 = float(twin.sigma_cells)
+        x0, y0 = path(0)
+        self.arc_k = resolve_arc_k(twin, g, x0 / g.dx, y0 / g.dx)
+        self.norm_buf = torch.zeros((), dtype=torch.float32, device=g.device)
+        self.ckpt: list[tuple[torch.Tensor, torch.Tensor, torch.Tensor]] = []
+        self.theta = None
+
+    # ------------------------------------------------------------------ forward
+    def _step(self, n: int, theta, save=None) -> None:
+        tw, g = self.tw, self.tw.grid
+        eta, s_h, s_sig = (float(v) for v in theta)
+        H, T, fl = self.H, self.T, self.fl
+        x, y = self.path(n)
+        if self.is_welding:
+            kernels.inject_arc_heat(H, self.flags, self.phi, fl, self.norm_buf, x / g.dx, y / g.dx,
+                                    self.arc_k, tw.Q_w, self.sigma0 * s_sig, g.dt, g.dx ** 3, eta,
+                                    tw.arc_penetration_m / g.dx,
+                                    1 if tw.arc_surface_weighting else 0, g.FLAG_SOLID,
+                                    g.FLAG_GAS, scratch=g.scratch)
+        if save is not None:
+            save["A"].copy_(tensor_of(H))
+        self._ceiling(H)
+        kernels.advect_diffuse_temperature_into(H, self.Htmp, T, self.zero_u, self.zero_u,
+                                                self.zero_u, self.flags, tw.alpha_lu, 1.0,
+                                                tw.cp_rho, g.nx, g.ny, g.nz)
+        self.H, self.Htmp = self.Htmp, self.H
+        H = self.H
+        if tw.enable_heat_loss:
+            kernels.apply_thermal_boundary_losses(
+                H, T, self.flags, tw.T_amb, self.h_conv0 * s_h, self.eps0 * s_h,
+                1 if tw.enable_convection else 0, 1 if tw.enable_radiation else 0, tw.cp_rho,
+                g.dt, g.dx, tw.sigma_sb, g.FLAG_SOLID, g.FLAG_GAS, g.nx, g.ny, g.nz)
+        if save is not None:
+            save["E"].copy_(tensor_of(H))
+        if tw.enable_heat_loss:
+            kernels.clamp_enthalpy_floor_scalar(H, tw.cp_rho, self.flags, tw.T_amb, g.FLAG_GAS)
+        if save is not None:
+            save["F"].copy_(tensor_of(H))
+        self._phase()
+        self._ceiling(H)
+        self._phase()
+
+    def _ceiling(self, H) -> None:
+        tw, g = self.tw, self.tw.grid
+        if tw.enable_enthalpy_cap:
+            kernels.clamp_enthalpy_ceiling_scalar(H, self.flags, tw.cp_rho, tw.mat.T_solidus,
+                                                  tw.mat.T_liquidus, tw.T_vapor_cap_K, tw.L_rho,
+                                                  g.FLAG_GAS)
+
+    def _phase(self) -> None:
+        tw = self.tw
+        kernels.update_phase(self.H, self.T, self.fl, tw.cp_rho, tw.L_rho, tw.mat.T_solidus,
+                             tw.mat.T_liquidus)
+
+    def _load(self, H) -> None:
+        tensor_of(self.H).copy_(H)
+        self._phase()
+
+    def forward(self, theta) -> tuple[torch.Tensor, torch.Tensor]:
+        """Final (T, f_l) (storage order) after n_steps with parameters theta."""
+        self.theta = tuple(float(v) for v in theta)
+        self.ckpt = []
+        tensor_of(self.H).copy_(tensor_of(self.H0))
+        tensor_of(self.T).copy_(tensor_of(self.T0))
+        tensor_of(self.fl).copy_(tensor_of(self.fl0))
+        for n in range(self.n_steps):
+            if n % self.K == 0:
+                self.ckpt.append(tensor_of(self.H).clone())
+            self._step(n, self.theta)
+        self.H_final = tensor_of(self.H).clone()
+        return tensor_of(self.T).clone(), tensor_of(self.fl).clone()
+
+    # ------------------------------------------------------------------ backward
+    def _consts(self, n: int) -> AdjConsts:
+        tw, g = self.tw, self.tw.grid
+        eta, s_h, s_sig = self.theta
+        x, y = self.path(n)
+        m = tw.mat
+        c = AdjConsts(cp_rho=tw.cp_rho, L_rho=tw.L_rho, T_s=m.T_solidus, T_l=m.T_liquidus,
+                      T_cap=tw.T_vapor_cap_K, T_floor=tw.T_amb, alpha_lu=tw.alpha_lu,
+                      T_amb=tw.T_amb, h_conv0=self.h_conv0, eps_rad0=self.eps0,
+                      sigma_sb=tw.sigma_sb, dt=g.dt, dx=g.dx, loss_scale=s_h, arc_i=x / g.dx,
+                      arc_j=y / g.dx, arc_k=self.arc_k, sigma=self.sigma0 * s_sig,
+                      sigma0=self.sigma0, Q_w=tw.Q_w, eta=eta, dx3=g.dx ** 3,
+                      pen=tw.arc_penetration_m / g.dx)
+        c.enable_cap = 1 if tw.enable_enthalpy_cap else 0
+        c.enable_loss = 1 if tw.enable_heat_loss else 0
+        c.enable_arc = 1 if self.is_welding else 0
+        c.enable_conv = 1 if tw.enable_convection else 0
+        c.enable_rad = 1 if tw.enable_radiation else 0
+        c.surface_weight = 1 if tw.arc_surface_weighting else 0
+        return c
+
+    def backward(self, grad_T: torch.Tensor, grad_fl: torch.Tensor) -> np.ndarray:
+        """dL/dtheta (float64) given dL/dT and dL/df_l at the final step."""
+        if self.theta is None:
+            raise RuntimeError("backward before forward")
+        tw, g = self.tw, self.tw.grid
+        st = torch.cuda.current_stream().cuda_stream
+        dev = tensor_of(self.H).device
+        n_cells = g.n_cells
+        shp = tensor_of(self.H).shape
+        lam = torch.empty(shp, dtype=torch.float32, device=dev)
+        lam2 = torch.empty_like(lam)
+        lamC = torch.empty_like(lam)
+        grads = torch.zeros(3, dtype=torch.float64, device=dev)
+        sums = torch.zeros(2, dtype=torch.float64, device=dev)
+        c_last = self._consts(self.n_steps - 1)
+        lib().call("waam_adj_terminal", P(self.H_final), P(grad_T.contiguous()),
+                   P(grad_fl.contiguous()), P(self.flags), ctypes.addressof(c_last), P(lam),
+                   n_cells, st)
+        seg_len = self.K
+        for s_idx in reversed(range(len(self.ckpt))):
+            s0 = s_idx * seg_len
+            s1 = min(self.n_steps, s0 + seg_len)
+            L = s1 - s0
+            Hs = torch.empty((L,) + tuple(shp), dtype=torch.float32, device=dev)
+            fls = torch.empty_like(Hs)
+            As, Es, Fs = torch.empty_like(Hs), torch.empty_like(Hs), torch.empty_like(Hs)
+            self._load(self.ckpt[s_idx])
+            for t in range(L):
+                Hs[t].copy_(tensor_of(self.H))
+                fls[t].copy_(tensor_of(self.fl))
+                self._step(s0 + t, self.theta, save={"A": As[t], "E": Es[t], "F": Fs[t]})
+            for t in reversed(range(L)):
+                n = s0 + t
+                c = self._consts(n)
+                if self.is_welding:
+                    lib().call("waam_adj_arc_sums", P(self.flags), P(fls[t]), ctypes.addressof(c),
+                               P(sums), g.nx, g.ny, g.nz, g.layout, P(g.scratch), st)
+                lib().call("waam_adj_thermal_step", P(lam), P(lam2), P(Hs[t]), P(As[t]),
+                           P(Es[t]), P(Fs[t]), P(fls[t]), P(self.flags), P(sums),
+                           ctypes.addressof(c), P(grads), P(lamC), g.nx, g.ny, g.nz, g.layout,
+                           P(g.scratch), st)
+                lam, lam2 = lam2, lam
+        self.lam0 = lam
+        return grads.cpu().numpy()
+
+
+class ThermalRolloutFn(torch.autograd.Function):
+    """(T_final, f_l_final) = rollout(theta), differentiable in theta (a length-3 tensor)."""
+
+    @staticmethod
+    def forward(ctx, theta: torch.Tensor, rollout: ThermalRollout):
+        T, fl = rollout.forward(theta.detach().double().cpu().numpy())
+        ctx.rollout = rollout
+        ctx.theta_dtype, ctx.theta_device = theta.dtype, theta.device
+        return T, fl
+
+    @staticmethod
+    def backward(ctx, grad_T, grad_fl):
+        g = ctx.rollout.backward(grad_T.float(), grad_fl.float())
+        return torch.tensor(g, dtype=ctx.theta_dtype, device=ctx.theta_device), None
+
+
+def field_loss(T, fl, T_target, fl_target, T_scale: float = 100.0):
+    """Mean squared error of the final temperature (scaled) and liquid fraction fields."""
+    return (((T - T_target) / T_scale) ** 2).mean() + ((fl - fl_target) ** 2).mean()
+
+
+def fit_calibration(rollout: ThermalRollout, T_target, fl_target, theta0, iters: int = 20,
+                    lr: float = 0.02) -> dict:
+    """Gradient descent (Adam) on theta = (eta, s_h, s_sigma) against target fields; returns the
+    parameter and loss history."""
+    theta = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
+    opt = torch.optim.Adam([theta], lr=lr)
+    hist = {"theta": [], "loss": []}
+    for _ in range(iters):
+        opt.zero_grad()
+        T, fl = ThermalRolloutFn.apply(theta, rollout)
+        loss = field_loss(T, fl, T_target, fl_target)
+        loss.backward()
+        hist["theta"].append(theta.detach().numpy().copy())
+        hist["loss"].append(float(loss))
+        opt.step()
+    return hist
