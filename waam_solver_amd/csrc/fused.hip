@@ -545,17 +545,24 @@ __global__ void __launch_bounds__(256) k_step_vof_flags(waam_step_fields f, Step
     size_t c = in ? d.idx(i, j, k) : 0;
     uint8_t nf = GAS;
     if (in) {
-        /* loads ahead of the stores for the cells that were not GAS (nearly all that stay so) */
-        bool pre = f.flags_in[c] != GAS;
+        /* the flag, the contact rule's operands and f_l are requested together (none depends on
+         * another), so a GAS cell waits for one memory round trip; the time-above fields of
+         * the cells that were not GAS (nearly all that stay so) follow the flag */
+        uint8_t f_in = f.flags_in[c];
+        float p = 0.0f, flo = 0.0f;
+        if (s.p.enable_vof) {
+            p = s.p.enable_wetting
+                    ? contact_bc_phi(f.phi_tmp, f.flags_in, i, j, k, s.p.cos_t, s.p.sin_t, d)
+                    : f.phi_tmp[c];
+            flo = f.fl_out[c];
+        }
+        bool pre = f_in != GAS;
         TimeAbove ta{};
         if (pre) ta = time_above_load(f, c);
         if (!s.p.enable_vof) {
-            nf = f.flags_in[c];
+            nf = f_in;
         } else {
-            float p = s.p.enable_wetting
-                          ? contact_bc_phi(f.phi_tmp, f.flags_in, i, j, k, s.p.cos_t, s.p.sin_t, d)
-                          : f.phi_tmp[c];
-            nf = classify_cell(k, s.p.nz_solid, p, f.fl_out[c]);
+            nf = classify_cell(k, s.p.nz_solid, p, flo);
             f.phi[c] = p;
             if (nf == GAS || p < 0.05f) f.fl_out[c] = 0.0f;
             else if (nf == SOLID) f.fl_out[c] = 0.0f;
