@@ -354,13 +354,15 @@ __device__ __forceinline__ HView make_hview(const waam_step_fields &f, const Ste
 /* thermal chain of one cell (reference kernels.py:497-510, 1953-1984, 1611-1682, 2030-2082,
  * 1937-1950, 1519-1556, 3016-3032, 2013-2027 in coupled_step order) */
 __device__ __forceinline__ void thermal_cell(const waam_step_fields &f, const StepConsts &s,
-                                             int i, int j, int k, size_t c, uint8_t flag)
+                                             int i, int j, int k, size_t c, uint8_t flag,
+                                             float h_gas)
 {
     const Dims &d = s.d;
     const waam_step_params &p = s.p;
     if (flag == GAS) {
-        /* GAS: only the two phase updates touch it (ceiling, thermal and losses skip GAS) */
-        float h = f.H_in[c], T, fl;
+        /* GAS: only the two phase updates touch it (ceiling, thermal and losses skip GAS);
+         * h_gas is the cell's enthalpy, loaded by the caller */
+        float h = h_gas, T, fl;
         phase_of(h, s.pc, T, fl);
         f.H_out[c] = h;
         f.T_out[c] = T;
@@ -432,7 +434,7 @@ __device__ __forceinline__ void thermal_vof_cell(const waam_step_fields &f, cons
                                                  int i, int j, int k, size_t c, uint8_t flag)
 {
     float phi_new = s.p.enable_vof ? vof_advect_cell(f, s.d, i, j, k, flag) : 0.0f;
-    thermal_cell(f, s, i, j, k, c, flag);
+    thermal_cell(f, s, i, j, k, c, flag, f.H_in[c]);
     if (s.p.enable_vof) f.phi_tmp[c] = phi_new;
 }
 
@@ -445,10 +447,16 @@ __global__ void __launch_bounds__(256) k_step_thermal(waam_step_fields f, StepCo
     int i, j, k;
     bool in = d.cell(i, j, k);
     size_t c = in ? d.idx(i, j, k) : 0;
+    /* the flag, the cell's enthalpy and the phi-advection operands are requested together (a
+     * GAS cell, most of a grid, then waits for one memory round trip); a FLUID/IFACE cell's
+     * advected phi is computed and dropped (kernel 3 owns those cells) */
     uint8_t flag = in ? f.flags_in[c] : (uint8_t)GAS;
+    float h_in = in ? f.H_in[c] : 0.0f;
+    float phi_new = (in && s.p.enable_vof) ? vof_advect_cell(f, d, i, j, k, flag) : 0.0f;
     if (LIST) list_append(in && is_surface_class(flag), c, cell_list(f, s.seg_len, 0), d.n());
     if (!in || is_surface_class(flag)) return;
-    thermal_vof_cell(f, s, i, j, k, c, flag);
+    thermal_cell(f, s, i, j, k, c, flag, h_in);
+    if (s.p.enable_vof) f.phi_tmp[c] = phi_new;
 }
 
 /* thermal chain (with enthalpy advection) and VOF advection of the listed FLUID/IFACE cells;
@@ -508,9 +516,13 @@ __global__ void __launch_bounds__(256) k_step_thermal_all(waam_step_fields f, St
             if (!first || !d.cell(i, j, k, lz)) return;
             c = d.idx(i, j, k);
             flag = f.flags_in[c];
-            if (is_surface_class(flag)) return;
         }
-        thermal_vof_cell(f, s, i, j, k, c, flag);
+        /* as kernel 2: the enthalpy and the phi-advection operands requested with the flag */
+        float h_in = f.H_in[c];
+        float phi_new = s.p.enable_vof ? vof_advect_cell(f, d, i, j, k, flag) : 0.0f;
+        if (!list && is_surface_class(flag)) return;
+        thermal_cell(f, s, i, j, k, c, flag, h_in);
+        if (s.p.enable_vof) f.phi_tmp[c] = phi_new;
     }
 }
 
