@@ -226,32 +226,80 @@ __global__ void __launch_bounds__(kBlock) k_step_begin(waam_step_fields f, StepC
     ap.arc_k = p.use_arc_k_override ? p.arc_k_override : s_arck;
     double sum = 0.0, comp = 0.0;
     size_t N = d.n();
+    /* the arc's weights are exactly zero outside its (i, j) box (exp underflows to 0 there), so
+     * on a grid wider than the box the sums run over the box's columns only (the compensated
+     * sum does not depend on the order of its terms); the flags and liquid fractions of kUnroll
+     * iterations are loaded together */
+    const int bi0 = imax(0, s.box_i0), bi1 = imin(d.nx - 1, s.box_i1);
+    const int bj0 = imax(0, s.box_j0), bj1 = imin(d.ny - 1, s.box_j1);
+    const unsigned bi = bi1 >= bi0 ? (unsigned)(bi1 - bi0 + 1) : 0u;
+    const unsigned bj = bj1 >= bj0 ? (unsigned)(bj1 - bj0 + 1) : 0u;
+    const size_t nbox = (size_t)bi * bj * (size_t)d.nz;
+    const bool whole = bi == (unsigned)d.nx && bj == (unsigned)d.ny;
     size_t stride = (size_t)gridDim.x * blockDim.x;
-    /* same cell-to-thread assignment and per-thread order as the standalone arc kernel; the
-     * flags and liquid fractions of kUnroll consecutive iterations are loaded together */
     constexpr int kUnroll = 8;
-    for (size_t base = (size_t)blockIdx.x * blockDim.x + threadIdx.x; do_sums && base < N;
-         base += kUnroll * stride) {
-        uint8_t flg[kUnroll];
-        float flv[kUnroll];
+    if (whole) {
+        for (size_t base = (size_t)blockIdx.x * blockDim.x + threadIdx.x; do_sums && base < N;
+             base += kUnroll * stride) {
+            uint8_t flg[kUnroll];
+            float flv[kUnroll];
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
-            size_t cell = base + u * stride;
-            bool in = cell < N;
-            flg[u] = in ? f.flags_in[cell] : (uint8_t)GAS;
-            flv[u] = in ? f.fl_in[cell] : 0.0f;
+            for (int u = 0; u < kUnroll; ++u) {
+                size_t cell = base + u * stride;
+                bool in = cell < N;
+                flg[u] = in ? f.flags_in[cell] : (uint8_t)GAS;
+                flv[u] = in ? f.fl_in[cell] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                if (flg[u] == GAS) continue;
+                int i, j, k;
+                cell_ijk(base + u * stride, d, i, j, k);
+                float wg, wdep;
+                if (!arc_cell_weights(ap, i, j, k, flv[u], wg, wdep)) continue;
+                double ss, e;
+                two_sum(sum, (double)(wg * wdep), ss, e);
+                sum = ss;
+                comp += e;
+            }
         }
+    } else {
+        /* box cell t: the box's x (ZYX) or z (XYZ) extent fastest; 32-bit arithmetic (the
+         * box holds fewer cells than the grid, which the cell lists already bound by 2^31) */
+        const bool zyx = d.layout == kLayoutZYX;
+        const unsigned ua = zyx ? bi : (unsigned)d.nz;
+        for (size_t base = (size_t)blockIdx.x * blockDim.x + threadIdx.x; do_sums && base < nbox;
+             base += kUnroll * stride) {
+            uint8_t flg[kUnroll];
+            float flv[kUnroll];
+            unsigned cells[kUnroll];
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
-            if (flg[u] == GAS) continue;
-            int i, j, k;
-            cell_ijk(base + u * stride, d, i, j, k);
-            float wg, wdep;
-            if (!arc_cell_weights(ap, i, j, k, flv[u], wg, wdep)) continue;
-            double ss, e;
-            two_sum(sum, (double)(wg * wdep), ss, e);
-            sum = ss;
-            comp += e;
+            for (int u = 0; u < kUnroll; ++u) {
+                size_t t = base + u * stride;
+                bool in = t < nbox;
+                unsigned tt = in ? (unsigned)t : 0u;
+                unsigned r = tt / ua;
+                int a = (int)(tt - r * ua);
+                unsigned b = r / bj;
+                int j = bj0 + (int)(r - b * bj);
+                int i = zyx ? bi0 + a : bi0 + (int)b;
+                int k = zyx ? (int)b : a;
+                cells[u] = (unsigned)d.idx(i, j, k);
+                flg[u] = in ? f.flags_in[cells[u]] : (uint8_t)GAS;
+                flv[u] = in ? f.fl_in[cells[u]] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                if (flg[u] == GAS) continue;
+                int i, j, k;
+                cell_ijk((size_t)cells[u], d, i, j, k);
+                float wg, wdep;
+                if (!arc_cell_weights(ap, i, j, k, flv[u], wg, wdep)) continue;
+                double ss, e;
+                two_sum(sum, (double)(wg * wdep), ss, e);
+                sum = ss;
+                comp += e;
+            }
         }
     }
     if (!do_sums) return;
