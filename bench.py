@@ -217,6 +217,31 @@ def kernel_breakdown(tw, w: dict, start: int, steps: int) -> dict:
     return out
 
 
+def captured_form(tw, w: dict, start: int, steps: int) -> dict:
+    """The same step count through WAAMTwin.step_many: runs of steady steps go to the GPU as
+    captured graphs (droplet steps and the step after each go directly). Graphs are captured in
+    an untimed first pass."""
+    import torch
+
+    g = tw.grid
+    st = tw._fused
+    warm = 4 * st.graph_steps
+    pos = [torch_xy(w, s, g.dt) for s in range(start, start + warm + steps)]
+    tw.step_many([p[0] for p in pos[:warm]], [p[1] for p in pos[:warm]], True)
+    torch.cuda.synchronize()
+    c0 = st.captured_steps
+    t0 = time.perf_counter()
+    tw.step_many([p[0] for p in pos[warm:]], [p[1] for p in pos[warm:]], True)
+    t_host = time.perf_counter() - t0
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    return {"value": round(g.n_cells * steps / el / 1e6, 2),
+            "ms_per_step": round(el * 1e3 / steps, 4),
+            "host_us_per_step": round(t_host * 1e6 / steps, 1),
+            "steps_per_graph": st.graph_steps,
+            "captured_steps": st.captured_steps - c0, "steps": steps}
+
+
 def measured_traffic() -> float | None:
     """HBM bytes per k_step_pull launch from the committed rocprofv3 counter summary."""
     try:
@@ -244,6 +269,8 @@ def main() -> int:
                     help="steps per kernel for the per-kernel HIP-event breakdown (0: skip)")
     ap.add_argument("--pop", choices=("fp32", "fp16", "bf16"), default="fp32",
                     help="population storage of the measured run (16-bit forms store f - w_q)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="skip the captured-graph run reported beside the value")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the 16-bit storage runs that an fp32 run reports beside its value")
     args = ap.parse_args()
@@ -307,10 +334,14 @@ def main() -> int:
         # the same seeded state, against this run's state at the end of its timed steps
         precision = {pop: precision_form(w, args, tw, mlups, pop)
                      for pop in ("fp16", "bf16", "fp32_ulp")}
+    graph = None
     if fused:
+        after = args.warmup + args.steps
         if args.breakdown > 0:
-            roofline["kernel_us"] = kernel_breakdown(tw, w, args.warmup + args.steps,
-                                                     args.breakdown)
+            roofline["kernel_us"] = kernel_breakdown(tw, w, after, args.breakdown)
+            after += args.breakdown * len(STEP_KERNELS)
+        if not args.no_graph:
+            graph = captured_form(tw, w, after, args.steps)
     tel = tw.get_telemetry()
     result = {
         "metric": f"coupled-step MLUPS (bead-on-plate {g.nx}x{g.ny}x{g.nz}, {args.pop})",
@@ -337,6 +368,8 @@ def main() -> int:
         result["roofline"] = roofline
     if precision is not None:
         result["precision"] = precision
+    if graph is not None:
+        result["graph"] = graph
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
     print(json.dumps(result))

@@ -527,6 +527,31 @@ int waam_unpack_populations(const void *src, float *dst, int64_t n_cells, int po
  * the workspace by the last waam_step_fused (three device floats, in that order). Blocking. */
 int waam_step_arc_readback(const void *scratch, float *host_out3);
 
+/* ---- captured steps: a run of n_steps consecutive waam_step_fused calls as one hipGraph.
+ * Replaces n_steps calls of coupled_step (reference solvers/coupled_step.py:73) made by
+ * WAAMTwin.step (twin.py:533-566) on steps with no droplet; the reference launches each kernel
+ * from Python every step.
+ * waam_graph_capture records the steps on a stream of its own (nothing runs): step k uses
+ * f_even for even k and f_odd for odd k (f_odd must be f_even with every in/out pair swapped,
+ * the state after one step), and p for everything but the arc, which each step reads from a
+ * device track. p must describe a steady step (pending_stream 1, thermal_list_ready 1 or 2,
+ * arc_applied 0, use_arc_k_override 0, force_reset_all 0, forces_from_fields 0): the captured
+ * run must follow a fused step with nothing written in between. 1 <= n_steps <= 4096.
+ * waam_graph_launch fills the track with the n_steps arc centres (cells, as the host computes
+ * p.arc_i / p.arc_j; the columns are their truncations clamped to the grid), robot heights
+ * k_robot (NULL: p.k_robot every step) and torch directions dir (3 per step; NULL: p.dir_*),
+ * copies it on `stream` and launches the graph there.
+ * It blocks only until the copy made two launches earlier has run. Launches of one graph
+ * must use one stream; they run the same kernels, with the same results, as n_steps
+ * waam_step_fused calls. Kernel timing (waam_timing_begin) does not see captured kernels. */
+typedef struct waam_step_graph waam_step_graph;
+int waam_graph_capture(const waam_step_params *p, const waam_step_fields *f_even,
+                       const waam_step_fields *f_odd, int n_steps, waam_step_graph **out);
+int waam_graph_launch(waam_step_graph *g, const double *arc_i, const double *arc_j,
+                      const float *k_robot, const float *dir, void *stream);
+int waam_graph_steps(const waam_step_graph *g);
+int waam_graph_destroy(waam_step_graph *g);
+
 /* ---- instrumentation (no counterpart in the reference) -------------------------------------
  * HIP-event timing of one kernel of waam_step_fused, recorded on the stream that kernel is
  * launched on: waam_timing_begin arms it for up to max_launches launches of kernel `which`

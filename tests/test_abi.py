@@ -70,6 +70,34 @@ def test_argument_validation_without_device():
         L.call("waam_timing_begin", 99, 10)
 
 
+def test_graph_capture_rejects_unsteady_steps_without_device():
+    """waam_graph_capture takes only steady steps and a bounded run; the checks come before
+    any HIP call."""
+    import ctypes
+
+    from waam_solver_amd._lib import WaamError, lib
+    from waam_solver_amd.solvers.fused import StepFields, StepParams
+
+    L = lib()
+    p, f = StepParams(), StepFields()
+    h = ctypes.c_void_p()
+    args = (ctypes.addressof(p), ctypes.addressof(f), ctypes.addressof(f))
+    with pytest.raises(WaamError, match="NULL"):
+        L.call("waam_graph_capture", *args, 8, None)
+    with pytest.raises(WaamError, match="steps"):
+        L.call("waam_graph_capture", *args, 0, ctypes.byref(h))
+    with pytest.raises(WaamError, match="steady"):
+        L.call("waam_graph_capture", *args, 8, ctypes.byref(h))  # no pending stream
+    p.pending_stream, p.thermal_list_ready, p.force_reset_all = 1, 1, 1
+    with pytest.raises(WaamError, match="steady"):
+        L.call("waam_graph_capture", *args, 8, ctypes.byref(h))
+    assert h.value is None
+    with pytest.raises(WaamError, match="NULL"):
+        L.call("waam_graph_launch", None, None, None, None, None, None)
+    assert L.fns["waam_graph_steps"](None) == 0
+    assert L.fns["waam_graph_destroy"](None) == 0
+
+
 _CODES = {"float": "f", "double": "d", "int": "i", "int32_t": "i", "int64_t": "l",
           "uint32_t": "u"}
 

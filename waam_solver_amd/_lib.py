@@ -78,6 +78,10 @@ SIGNATURES: dict[str, str] = {
     "waam_pack_populations": "pplip",
     "waam_unpack_populations": "pplip",
     "waam_step_arc_readback": "pp",
+    "waam_graph_capture": "pppip",
+    "waam_graph_launch": "pppppp",
+    "waam_graph_steps": "p",
+    "waam_graph_destroy": "p",
     "waam_timing_begin": "ii",
     "waam_timing_end": "pp",
     "waam_refresh_thermal_properties": "pppppppiplp",

@@ -67,6 +67,8 @@ constexpr int kListBlocksWithSweep = 256;
 #endif
 constexpr bool kSeparatePull32 = WAAM_SEPARATE_PULL32 != 0;
 constexpr bool kSeparatePull16 = false;
+/* steps one captured graph may hold */
+constexpr int kGraphMaxSteps = 4096;
 
 static_assert(kBlock == kListSegments, "the list prefix scan uses one thread per segment");
 
@@ -83,6 +85,16 @@ struct CellList {
     }
 };
 
+/* what moves with the torch from one step to the next: the arc centre, its column, the robot's
+ * height floor for arc_k, the (i, j) box outside which the arc heat is exactly zero and the
+ * torch's direction of travel */
+struct ArcStep {
+    float arc_i, arc_j, k_robot;
+    int col_i, col_j;
+    int box_i0, box_i1, box_j0, box_j1;
+    float dir_x, dir_y, dir_z;
+};
+
 struct StepConsts {
     waam_step_params p;
     ArcParams arc;
@@ -92,10 +104,35 @@ struct StepConsts {
     Recoil rc;
     GasShear gs;
     float cap_l, cap_s, h_min, inv_dt;
-    int box_i0, box_i1, box_j0, box_j1;  // cells outside get exactly zero arc heat
-    int seg_len;                         // cells per list segment
+    ArcStep at;            // this step's arc (a direct launch)
+    const ArcStep *track;  // or, in a captured step, its entry of the graph's device track
+    int seg_len;           // cells per list segment
     Dims d;
 };
+
+/* the step's arc: by value for a direct launch, from device memory for a captured one (the
+ * pointer is uniform, so these are scalar loads) */
+__device__ __forceinline__ ArcStep arc_step(const StepConsts &s)
+{
+    if (s.track) {
+        const ArcStep *t = s.track;
+        ArcStep a;
+        a.arc_i = t->arc_i;
+        a.arc_j = t->arc_j;
+        a.k_robot = t->k_robot;
+        a.col_i = t->col_i;
+        a.col_j = t->col_j;
+        a.box_i0 = t->box_i0;
+        a.box_i1 = t->box_i1;
+        a.box_j0 = t->box_j0;
+        a.box_j1 = t->box_j1;
+        a.dir_x = t->dir_x;
+        a.dir_y = t->dir_y;
+        a.dir_z = t->dir_z;
+        return a;
+    }
+    return s.at;
+}
 
 __device__ __forceinline__ float step_arc_k(const waam_step_params &p, const float *tot)
 {
@@ -195,13 +232,14 @@ __global__ void __launch_bounds__(kBlock) k_step_begin(waam_step_fields f, StepC
     __shared__ float s_arck;
     const Dims &d = s.d;
     const waam_step_params &p = s.p;
+    const ArcStep at = arc_step(s);
     if (blockIdx.x == 0) {
         if (!p.thermal_list_ready) cell_list(f, s.seg_len, 0).count(threadIdx.x) = 0;
         cell_list(f, s.seg_len, 1).count(threadIdx.x) = 0;
     }
     int best = -1;
     for (int k = threadIdx.x; k < d.nz; k += blockDim.x) {
-        size_t c = d.idx(p.col_i, p.col_j, k);
+        size_t c = d.idx(at.col_i, at.col_j, k);
         if (f.flags_in[c] != GAS && f.phi[c] > 0.05f) best = k > best ? k : best;
     }
     sh_k[threadIdx.x] = best;
@@ -214,7 +252,7 @@ __global__ void __launch_bounds__(kBlock) k_step_begin(waam_step_fields f, StepC
         int kf = imax(1, p.nz_solid - 1);
         float ks = (float)(sh_k[0] >= 0 ? sh_k[0] : kf);
         if (ks < 1.0f) ks = (float)imax(1, p.nz_solid - 1);
-        if (p.use_k_robot) ks = fmx(ks, p.k_robot);
+        if (p.use_k_robot) ks = fmx(ks, at.k_robot);
         s_arck = ks;
         if (blockIdx.x == 0) {
             float *tot = reinterpret_cast<float *>((char *)f.scratch + kScratchTelemetry);
@@ -223,6 +261,8 @@ __global__ void __launch_bounds__(kBlock) k_step_begin(waam_step_fields f, StepC
     }
     __syncthreads();
     ArcParams ap = s.arc;
+    ap.arc_i = at.arc_i;
+    ap.arc_j = at.arc_j;
     ap.arc_k = p.use_arc_k_override ? p.arc_k_override : s_arck;
     double sum = 0.0, comp = 0.0;
     size_t N = d.n();
@@ -230,8 +270,8 @@ __global__ void __launch_bounds__(kBlock) k_step_begin(waam_step_fields f, StepC
      * on a grid wider than the box the sums run over the box's columns only (the compensated
      * sum does not depend on the order of its terms); the flags and liquid fractions of kUnroll
      * iterations are loaded together */
-    const int bi0 = imax(0, s.box_i0), bi1 = imin(d.nx - 1, s.box_i1);
-    const int bj0 = imax(0, s.box_j0), bj1 = imin(d.ny - 1, s.box_j1);
+    const int bi0 = imax(0, at.box_i0), bi1 = imin(d.nx - 1, at.box_i1);
+    const int bj0 = imax(0, at.box_j0), bj1 = imin(d.ny - 1, at.box_j1);
     const unsigned bi = bi1 >= bi0 ? (unsigned)(bi1 - bi0 + 1) : 0u;
     const unsigned bj = bj1 >= bj0 ? (unsigned)(bj1 - bj0 + 1) : 0u;
     const size_t nbox = (size_t)bi * bj * (size_t)d.nz;
@@ -384,16 +424,19 @@ __device__ __forceinline__ HView make_hview(const waam_step_fields &f, const Ste
     hv.fl_in = f.fl_in;
     hv.flags_in = f.flags_in;
     hv.d = s.d;
+    const ArcStep at = arc_step(s);
     hv.ap = s.arc;
+    hv.ap.arc_i = at.arc_i;
+    hv.ap.arc_j = at.arc_j;
     hv.ap.arc_k = step_arc_k(s.p, tot);
     hv.epw = tot[1];
     hv.H_liq = s.pc.H_liq;
     hv.cap_l = s.cap_l;
     hv.cap_s = s.cap_s;
-    hv.box_i0 = s.box_i0;
-    hv.box_i1 = s.box_i1;
-    hv.box_j0 = s.box_j0;
-    hv.box_j1 = s.box_j1;
+    hv.box_i0 = at.box_i0;
+    hv.box_i1 = at.box_i1;
+    hv.box_j0 = at.box_j0;
+    hv.box_j1 = at.box_j1;
     hv.arc_on = s.p.is_welding && !s.p.arc_applied && hv.epw > 0.0f;
     hv.cap_on = s.p.is_welding && !s.p.arc_applied && s.p.enable_cap;
     return hv;
@@ -725,16 +768,28 @@ __device__ __forceinline__ void collide_cell(const waam_step_fields &f, const St
         fy += ty;
         fz += tz;
     }
-    if (p.is_welding && p.enable_gas_shear && gas_shear_force(fs.phi, i, j, s.gs, tx, ty)) {
-        fx += tx;
-        fy += ty;
+    const ArcStep at = arc_step(s);
+    if (p.is_welding && p.enable_gas_shear) {
+        GasShear gs = s.gs;
+        gs.arc_i = at.arc_i;
+        gs.arc_j = at.arc_j;
+        if (gas_shear_force(fs.phi, i, j, gs, tx, ty)) {
+            fx += tx;
+            fy += ty;
+        }
     }
     if (p.is_welding) {
         ArcPressure ap = s.ap;
+        ap.arc_i = at.arc_i;
+        ap.arc_j = at.arc_j;
         ap.arc_k = arc_k;
         if (arc_pressure_force(fs.phi.c[4], fs.phi.c[5], i, j, k, ap, tz)) fz += tz;
-        if (p.enable_recoil && recoil_force(fs.phi.c[4], fs.phi.c[5], T_c, i, j, s.rc, tz))
-            fz += tz;
+        if (p.enable_recoil) {
+            Recoil rc = s.rc;
+            rc.arc_i = at.arc_i;
+            rc.arc_j = at.arc_j;
+            if (recoil_force(fs.phi.c[4], fs.phi.c[5], T_c, i, j, rc, tz)) fz += tz;
+        }
     }
     if (p.enable_hydro && fl > 0.0f) fz += -1.0f * p.g_lu * fl;
     if (fl > 0.0f) fz += 1.0f * p.g_lu * p.beta * (T_c - p.T_l) * fl;
@@ -813,8 +868,9 @@ __device__ __forceinline__ uint8_t freeze_cell(const waam_step_fields &f, const 
         wH = wfl = wphi = wflag = true;
     }
     if (p.enable_trailing && !(phi < 0.85f || fl < 0.05f)) {
-        float di = (float)i - p.arc_i, dj = (float)j - p.arc_j, dk = (float)k - arc_k;
-        float proj = di * p.dir_x + dj * p.dir_y + dk * p.dir_z;
+        const ArcStep at = arc_step(s);
+        float di = (float)i - at.arc_i, dj = (float)j - at.arc_j, dk = (float)k - arc_k;
+        float proj = di * at.dir_x + dj * at.dir_y + dk * at.dir_z;
         if (!(proj > -p.lookback) && !(T >= p.T_freeze)) {
             H = p.cp_rho * (p.T_s - 1.0f);
             T = p.T_s - 1.0f;
@@ -1117,11 +1173,12 @@ __global__ void k_advect_tracers_step(float *__restrict__ pos, int32_t *__restri
     pos[3 * p + 2] += uz[c] * dx;
 }
 
-static void arc_box(StepConsts &s)
+/* half-widths (cells) of the (i, j) box outside which the arc heat is exactly zero: exp(x) is
+ * exactly zero for x < -103.98, and outside the box x < -110 for every cell; r < 0 when the
+ * shape has no finite box */
+static void arc_radii(const StepConsts &s, double &ri, double &rj)
 {
     const waam_step_params &p = s.p;
-    /* exp(x) is exactly zero for x < -103.98; outside this box x < -110 for every cell */
-    double ri, rj;
     if (p.arc_kind == 0) {
         double inv = (double)s.arc.inv2s2;
         ri = rj = std::sqrt(110.0 / inv);
@@ -1130,17 +1187,23 @@ static void arc_box(StepConsts &s)
         ri = std::sqrt(110.0 / 3.0 * (a * a + 1e-6)) / std::max(std::fabs((double)p.g_sgn), 1e-30);
         rj = std::sqrt(110.0 / 3.0 * ((double)p.g_b * p.g_b + 1e-6));
     }
-    auto lo = [](double a, double r) { return (int)std::floor(a - r) - 2; };
-    auto hi = [](double a, double r) { return (int)std::ceil(a + r) + 2; };
-    if (!(ri < 1e6) || !(rj < 1e6)) {
-        s.box_i0 = s.box_j0 = -(1 << 30);
-        s.box_i1 = s.box_j1 = (1 << 30);
+    if (!(ri < 1e6) || !(rj < 1e6)) ri = rj = -1.0;
+}
+
+/* the arc box of a centre (arc_i, arc_j), as the kernels compare it with cell indices */
+static void arc_box(double ri, double rj, float arc_i, float arc_j, ArcStep &a)
+{
+    if (ri < 0.0) {
+        a.box_i0 = a.box_j0 = -(1 << 30);
+        a.box_i1 = a.box_j1 = (1 << 30);
         return;
     }
-    s.box_i0 = lo(p.arc_i, ri);
-    s.box_i1 = hi(p.arc_i, ri);
-    s.box_j0 = lo(p.arc_j, rj);
-    s.box_j1 = hi(p.arc_j, rj);
+    auto lo = [](double c, double r) { return (int)std::floor(c - r) - 2; };
+    auto hi = [](double c, double r) { return (int)std::ceil(c + r) + 2; };
+    a.box_i0 = lo(arc_i, ri);
+    a.box_i1 = hi(arc_i, ri);
+    a.box_j0 = lo(arc_j, rj);
+    a.box_j1 = hi(arc_j, rj);
 }
 
 /* HIP-event timing of one kernel kind (waam_timing_begin / waam_timing_end) */
@@ -1162,40 +1225,32 @@ static void timed_launch(int which, Launch &&launch)
     if (rec) ++t.used;
 }
 
-}  // namespace waam
-
-using namespace waam;
-
-extern "C" {
-
-int waam_step_fused(const waam_step_params *pp, const waam_step_fields *ff, void *stream)
+/* the checks waam_step_fused makes on a field set */
+static int check_fields(const char *who, const waam_step_fields &f)
 {
-    if (!pp || !ff) return set_error(WAAM_ERR_INVALID, "waam_step_fused: NULL argument");
-    const waam_step_params &p = *pp;
-    const waam_step_fields &f = *ff;
-    WAAM_RETURN_IF(check_dims("waam_step_fused", f.nx, f.ny, f.nz, f.layout));
-    WAAM_RETURN_IF(check_ptrs("waam_step_fused",
-                              {f.f_src, f.f_dst, f.rho, f.ux, f.uy, f.uz, f.H_in, f.H_out, f.T_in,
-                               f.T_out, f.fl_in, f.fl_out, f.phi, f.phi_tmp, f.flags_in,
-                               f.flags_out, f.sflags_prev, f.sflags_next, f.Fx, f.Fy, f.Fz,
-                               f.T_max, f.T_prev, f.dT_dt, f.t800, f.t1100, f.tsol,
-                               f.list_thermal, f.list_collide, f.scratch}));
+    WAAM_RETURN_IF(check_dims(who, f.nx, f.ny, f.nz, f.layout));
+    WAAM_RETURN_IF(check_ptrs(who, {f.f_src, f.f_dst, f.rho, f.ux, f.uy, f.uz, f.H_in, f.H_out,
+                                    f.T_in, f.T_out, f.fl_in, f.fl_out, f.phi, f.phi_tmp,
+                                    f.flags_in, f.flags_out, f.sflags_prev, f.sflags_next, f.Fx,
+                                    f.Fy, f.Fz, f.T_max, f.T_prev, f.dT_dt, f.t800, f.t1100, f.tsol,
+                                    f.list_thermal, f.list_collide, f.scratch}));
     if (f.f_src == f.f_dst || f.H_in == f.H_out || f.T_in == f.T_out || f.fl_in == f.fl_out ||
         f.flags_in == f.flags_out || f.sflags_prev == f.sflags_next || f.phi == f.phi_tmp ||
         f.list_thermal == f.list_collide)
-        return set_error(WAAM_ERR_INVALID, "waam_step_fused: an in/out buffer pair aliases");
+        return set_error(WAAM_ERR_INVALID, "%s: an in/out buffer pair aliases", who);
     Dims d{f.nx, f.ny, f.nz, f.layout};
     if (d.n() > (size_t)INT_MAX)
-        return set_error(WAAM_ERR_INVALID, "waam_step_fused: %zu cells exceed the int32 cell lists",
-                         d.n());
-    if (p.col_i < 0 || p.col_i >= f.nx || p.col_j < 0 || p.col_j >= f.ny)
-        return set_error(WAAM_ERR_INVALID, "waam_step_fused: arc column outside the grid");
+        return set_error(WAAM_ERR_INVALID, "%s: %zu cells exceed the int32 cell lists", who, d.n());
     if (f.max_tracers > 0 && (!f.tracer_pos || !f.tracer_active))
-        return set_error(WAAM_ERR_INVALID, "waam_step_fused: tracer buffers missing");
+        return set_error(WAAM_ERR_INVALID, "%s: tracer buffers missing", who);
     if (f.pop_dtype != WAAM_POP_F32 && f.pop_dtype != WAAM_POP_F16 && f.pop_dtype != WAAM_POP_BF16)
-        return set_error(WAAM_ERR_INVALID, "waam_step_fused: unknown population form %d",
-                         f.pop_dtype);
+        return set_error(WAAM_ERR_INVALID, "%s: unknown population form %d", who, f.pop_dtype);
+    return 0;
+}
 
+/* kernel arguments of a step: the scalar groups each kernel reads, derived once on the host */
+static StepConsts make_step_consts(const waam_step_params &p, const Dims &d)
+{
     StepConsts s{};
     s.p = p;
     s.d = d;
@@ -1244,19 +1299,51 @@ int waam_step_fused(const waam_step_params *pp, const waam_step_fields *ff, void
     s.gs.inv2s2 = s.rc.inv2s2;
     s.gs.tau_peak = p.gas_tau;
     s.gs.F_scale = 1.0f / (p.rho_phys * p.dx) * p.dt * p.dt / p.dx;
-    arc_box(s);
-    dim3 g3 = grid3(d);
+    s.at.arc_i = p.arc_i;
+    s.at.arc_j = p.arc_j;
+    s.at.k_robot = p.k_robot;
+    s.at.col_i = p.col_i;
+    s.at.col_j = p.col_j;
+    s.at.dir_x = p.dir_x;
+    s.at.dir_y = p.dir_y;
+    s.at.dir_z = p.dir_z;
+    double ri, rj;
+    arc_radii(s, ri, rj);
+    arc_box(ri, rj, p.arc_i, p.arc_j, s.at);
+    s.track = nullptr;
     size_t seg = (d.n() + kListSegments - 1) / kListSegments;
     s.seg_len = (int)((seg + 255) / 256 * 256);
+    return s;
+}
 
-    hipStream_t st = as_stream(stream);
+/* one launch: timed with events (hipExtLaunchKernel) in a direct step, a plain launch into a
+ * stream being captured */
+template <class T> struct ident { using type = T; };
+
+template <typename... KArgs>
+static void launch_k(bool capture, void (*k)(KArgs...), dim3 grid, dim3 block, hipStream_t st,
+                     hipEvent_t e0, hipEvent_t e1, typename ident<KArgs>::type... args)
+{
+    if (capture) {
+        void *argv[] = {static_cast<void *>(&args)...};
+        (void)hipLaunchKernel(reinterpret_cast<const void *>(k), grid, block, argv, 0, st);
+    } else {
+        hipExtLaunchKernelGGL(k, grid, block, 0, st, e0, e1, 0, args...);
+    }
+}
+
+/* the step's launches in order (kernels 1-7) */
+static void launch_step(const StepConsts &s, const waam_step_fields &f, hipStream_t st,
+                        bool capture)
+{
+    const waam_step_params &p = s.p;
+    const Dims &d = s.d;
+    dim3 g3 = grid3(d);
     int do_sums = p.is_welding && !p.arc_applied ? 1 : 0;
     timed_launch(WAAM_K_BEGIN, [&](hipEvent_t a, hipEvent_t b) {
-        hipExtLaunchKernelGGL(k_step_begin, dim3(do_sums ? kReduceBlocks : 1), dim3(kBlock), 0,
-                              st, a, do_sums ? nullptr : b, 0, f, s, do_sums);
-        if (do_sums)
-            hipExtLaunchKernelGGL(k_step_arc_total, dim3(1), dim3(kBlock), 0, st, nullptr, b, 0, f,
-                                  s);
+        launch_k(capture, k_step_begin, dim3(do_sums ? kReduceBlocks : 1), dim3(kBlock), st, a,
+                 do_sums ? nullptr : b, f, s, do_sums);
+        if (do_sums) launch_k(capture, k_step_arc_total, dim3(1), dim3(kBlock), st, nullptr, b, f, s);
     });
     if (p.thermal_list_ready == 1) {
         /* enough list slices for about kListBlocksWithSweep workers */
@@ -1264,40 +1351,39 @@ int waam_step_fused(const waam_step_params *pp, const waam_step_fields *ff, void
         int lz = std::max(1, (kListBlocksWithSweep + per - 1) / per);
         dim3 ga(g3.x, g3.y, g3.z + (unsigned)lz);
         timed_launch(WAAM_K_THERMAL, [&](hipEvent_t a, hipEvent_t b) {
-            hipExtLaunchKernelGGL(k_step_thermal_all, ga, block3(), 0, st, a, b, 0, f, s, lz);
+            launch_k(capture, k_step_thermal_all, ga, block3(), st, a, b, f, s, lz);
         });
     } else {
         timed_launch(WAAM_K_THERMAL, [&](hipEvent_t a, hipEvent_t b) {
             if (p.thermal_list_ready)
-                hipExtLaunchKernelGGL(k_step_thermal<false>, g3, block3(), 0, st, a, b, 0, f, s);
+                launch_k(capture, k_step_thermal<false>, g3, block3(), st, a, b, f, s);
             else
-                hipExtLaunchKernelGGL(k_step_thermal<true>, g3, block3(), 0, st, a, b, 0, f, s);
+                launch_k(capture, k_step_thermal<true>, g3, block3(), st, a, b, f, s);
         });
         timed_launch(WAAM_K_THERMAL_SURFACE, [&](hipEvent_t a, hipEvent_t b) {
-            hipExtLaunchKernelGGL(k_step_thermal_surface, dim3(kListBlocks), dim3(kBlock), 0, st, a,
-                                  b, 0, f, s);
+            launch_k(capture, k_step_thermal_surface, dim3(kListBlocks), dim3(kBlock), st, a, b, f,
+                     s);
         });
     }
     timed_launch(WAAM_K_VOF_FLAGS, [&](hipEvent_t a, hipEvent_t b) {
-        hipExtLaunchKernelGGL(k_step_vof_flags, g3, block3(), 0, st, a, b, 0, f, s);
+        launch_k(capture, k_step_vof_flags, g3, block3(), st, a, b, f, s);
     });
     timed_launch(WAAM_K_COLLIDE, [&](hipEvent_t a, hipEvent_t b) {
         if (f.pop_dtype == WAAM_POP_F16)
-            hipExtLaunchKernelGGL(k_step_collide<WAAM_POP_F16>, dim3(kListBlocks), dim3(kBlock), 0,
-                                  st, a, b, 0, f, s);
+            launch_k(capture, k_step_collide<WAAM_POP_F16>, dim3(kListBlocks), dim3(kBlock), st, a,
+                     b, f, s);
         else if (f.pop_dtype == WAAM_POP_BF16)
-            hipExtLaunchKernelGGL(k_step_collide<WAAM_POP_BF16>, dim3(kListBlocks), dim3(kBlock), 0,
-                                  st, a, b, 0, f, s);
+            launch_k(capture, k_step_collide<WAAM_POP_BF16>, dim3(kListBlocks), dim3(kBlock), st,
+                     a, b, f, s);
         else
-            hipExtLaunchKernelGGL(k_step_collide<WAAM_POP_F32>, dim3(kListBlocks), dim3(kBlock), 0,
-                                  st, a, b, 0, f, s);
+            launch_k(capture, k_step_collide<WAAM_POP_F32>, dim3(kListBlocks), dim3(kBlock), st, a,
+                     b, f, s);
     });
     if (f.max_tracers > 0)
         timed_launch(WAAM_K_TRACERS, [&](hipEvent_t a, hipEvent_t b) {
-            hipExtLaunchKernelGGL(k_advect_tracers_step, dim3(grid_for((size_t)f.max_tracers)),
-                                  dim3(kBlock), 0, st, a, b, 0, f.tracer_pos, f.tracer_active,
-                                  f.ux, f.uy, f.uz, (const float *)f.fl_out,
-                                  (const uint8_t *)f.flags_out, p.dx, f.max_tracers, d);
+            launch_k(capture, k_advect_tracers_step, dim3(grid_for((size_t)f.max_tracers)),
+                     dim3(kBlock), st, a, b, f.tracer_pos, f.tracer_active, f.ux, f.uy, f.uz,
+                     f.fl_out, f.flags_out, p.dx, f.max_tracers, d);
         });
     /* the pull runs in a launch of its own ahead of kernel 7 (pairs of cells for 16-bit
      * storage on an even contiguous extent), or inside kernel 7 */
@@ -1308,25 +1394,203 @@ int waam_step_fused(const waam_step_params *pp, const waam_step_fields *ff, void
         timed_launch(WAAM_K_PULL, [&](hipEvent_t a, hipEvent_t b) {
             if (pair16) {
                 dim3 gp((unsigned)((d.fast() / 2 + kTileFast - 1) / kTileFast), g3.y, g3.z);
-                hipExtLaunchKernelGGL(k_step_pull_pair16, gp, block3(), 0, st, a, b, 0, f, s);
+                launch_k(capture, k_step_pull_pair16, gp, block3(), st, a, b, f, s);
             } else if (f32) {
-                hipExtLaunchKernelGGL(k_step_pull<float>, g3, block3(), 0, st, a, b, 0, f, s);
+                launch_k(capture, k_step_pull<float>, g3, block3(), st, a, b, f, s);
             } else {
-                hipExtLaunchKernelGGL(k_step_pull<uint16_t>, g3, block3(), 0, st, a, b, 0, f, s);
+                launch_k(capture, k_step_pull<uint16_t>, g3, block3(), st, a, b, f, s);
             }
         });
     timed_launch(WAAM_K_STREAM, [&](hipEvent_t a, hipEvent_t b) {
         dim3 gs(g3.x, g3.y, (unsigned)((d.slow() + kStreamSlices - 1) / kStreamSlices));
         if (separate)
-            hipExtLaunchKernelGGL(k_step_stream<float, false>, gs, block3(), 0, st, a, b, 0, f, s);
+            launch_k(capture, k_step_stream<float, false>, gs, block3(), st, a, b, f, s);
         else if (f32)
-            hipExtLaunchKernelGGL(k_step_stream<float, true>, g3, block3(), 0, st, a, b, 0, f, s);
+            launch_k(capture, k_step_stream<float, true>, g3, block3(), st, a, b, f, s);
         else
-            hipExtLaunchKernelGGL(k_step_stream<uint16_t, true>, g3, block3(), 0, st, a, b, 0, f,
-                                  s);
+            launch_k(capture, k_step_stream<uint16_t, true>, g3, block3(), st, a, b, f, s);
     });
+}
+
+}  // namespace waam
+
+/* a captured run of fused steps (waam_graph_capture) */
+struct waam_step_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int n_steps = 0, device = 0;
+    int nx = 0, ny = 0;
+    waam::StepConsts base{};            // the captured steps' scalars (arc of step 0)
+    double ri = 0.0, rj = 0.0;          // arc box half-widths
+    waam::ArcStep *track = nullptr;     // device: one entry per captured step
+    waam::ArcStep *stage[2] = {nullptr, nullptr};  // pinned host staging, used in turn
+    hipEvent_t staged[2] = {nullptr, nullptr};     // the copy out of each staging buffer
+    int next = 0;
+};
+
+using namespace waam;
+
+static void graph_free(waam_step_graph *g)
+{
+    if (!g) return;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    for (int b = 0; b < 2; ++b) {
+        if (g->staged[b]) {
+            (void)hipEventSynchronize(g->staged[b]);
+            (void)hipEventDestroy(g->staged[b]);
+        }
+        if (g->stage[b]) (void)hipHostFree(g->stage[b]);
+    }
+    if (g->track) (void)hipFree(g->track);
+    delete g;
+}
+
+extern "C" {
+
+int waam_step_fused(const waam_step_params *pp, const waam_step_fields *ff, void *stream)
+{
+    if (!pp || !ff) return set_error(WAAM_ERR_INVALID, "waam_step_fused: NULL argument");
+    const waam_step_params &p = *pp;
+    const waam_step_fields &f = *ff;
+    WAAM_RETURN_IF(check_fields("waam_step_fused", f));
+    if (p.col_i < 0 || p.col_i >= f.nx || p.col_j < 0 || p.col_j >= f.ny)
+        return set_error(WAAM_ERR_INVALID, "waam_step_fused: arc column outside the grid");
+    Dims d{f.nx, f.ny, f.nz, f.layout};
+    StepConsts s = make_step_consts(p, d);
+    launch_step(s, f, as_stream(stream), false);
     return check_launch("waam_step_fused");
 }
+
+int waam_graph_capture(const waam_step_params *pp, const waam_step_fields *f_even,
+                       const waam_step_fields *f_odd, int n_steps, waam_step_graph **out)
+{
+    if (!pp || !f_even || !f_odd || !out)
+        return set_error(WAAM_ERR_INVALID, "waam_graph_capture: NULL argument");
+    *out = nullptr;
+    const waam_step_params &p = *pp;
+    if (n_steps < 1 || n_steps > kGraphMaxSteps)
+        return set_error(WAAM_ERR_INVALID, "waam_graph_capture: %d steps (1..%d)", n_steps,
+                         kGraphMaxSteps);
+    /* a captured step is an ordinary fused step that follows a fused step: its population
+     * stream is pending, its thermal list was built by that step, nothing was applied eagerly */
+    if (p.pending_stream != 1 || (p.thermal_list_ready != 1 && p.thermal_list_ready != 2) ||
+        p.arc_applied || p.use_arc_k_override || p.force_reset_all || p.forces_from_fields)
+        return set_error(WAAM_ERR_INVALID,
+                         "waam_graph_capture: only steady steps are captured (pending stream, "
+                         "thermal list ready, no eager arc, no force reset)");
+    if (g_timer.armed)
+        return set_error(WAAM_ERR_INVALID, "waam_graph_capture: kernel timing is armed");
+    WAAM_RETURN_IF(check_fields("waam_graph_capture", *f_even));
+    WAAM_RETURN_IF(check_fields("waam_graph_capture", *f_odd));
+    const waam_step_fields &a = *f_even, &b = *f_odd;
+    if (a.nx != b.nx || a.ny != b.ny || a.nz != b.nz || a.layout != b.layout ||
+        a.pop_dtype != b.pop_dtype || a.max_tracers != b.max_tracers)
+        return set_error(WAAM_ERR_INVALID, "waam_graph_capture: the two field sets differ in shape");
+    /* the odd set is the even set after one step's swaps */
+    if (b.f_src != a.f_dst || b.f_dst != a.f_src || b.H_in != a.H_out || b.H_out != a.H_in ||
+        b.T_in != a.T_out || b.T_out != a.T_in || b.fl_in != a.fl_out || b.fl_out != a.fl_in ||
+        b.flags_in != a.flags_out || b.flags_out != a.flags_in ||
+        b.sflags_prev != a.sflags_next || b.sflags_next != a.sflags_prev || b.phi != a.phi ||
+        b.phi_tmp != a.phi_tmp || b.list_thermal != a.list_thermal ||
+        b.list_collide != a.list_collide || b.scratch != a.scratch)
+        return set_error(WAAM_ERR_INVALID,
+                         "waam_graph_capture: the odd field set is not the even set swapped");
+    if (p.col_i < 0 || p.col_i >= a.nx || p.col_j < 0 || p.col_j >= a.ny)
+        return set_error(WAAM_ERR_INVALID, "waam_graph_capture: arc column outside the grid");
+
+    waam_step_graph *g = new waam_step_graph();
+    Dims d{a.nx, a.ny, a.nz, a.layout};
+    g->n_steps = n_steps;
+    g->nx = a.nx;
+    g->ny = a.ny;
+    g->base = make_step_consts(p, d);
+    arc_radii(g->base, g->ri, g->rj);
+    hipStream_t cs = nullptr;
+    hipError_t e = hipGetDevice(&g->device);
+    const size_t tb = sizeof(ArcStep) * (size_t)n_steps;
+    if (e == hipSuccess) e = hipMalloc(&g->track, tb);
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+        e = hipHostMalloc(&g->stage[k], tb, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&g->staged[k], hipEventDisableTiming);
+    }
+    if (e == hipSuccess) {
+        /* every captured step reads the arc from the track until the first launch fills it:
+         * start it as step 0 repeated */
+        for (int k = 0; k < n_steps; ++k) g->stage[0][k] = g->base.at;
+        e = hipMemcpy(g->track, g->stage[0], tb, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        for (int k = 0; k < n_steps; ++k) {
+            StepConsts s = g->base;
+            s.track = g->track + k;
+            launch_step(s, (k & 1) ? b : a, cs, true);
+        }
+        hipError_t le = hipGetLastError();
+        e = hipStreamEndCapture(cs, &g->graph);
+        if (e == hipSuccess) e = le;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (cs) (void)hipStreamDestroy(cs);
+    if (e != hipSuccess) {
+        graph_free(g);
+        return set_error(WAAM_ERR_HIP, "waam_graph_capture: %s", hipGetErrorString(e));
+    }
+    *out = g;
+    return 0;
+}
+
+int waam_graph_launch(waam_step_graph *g, const double *arc_i, const double *arc_j,
+                      const float *k_robot, const float *dir, void *stream)
+{
+    if (!g || !arc_i || !arc_j) return set_error(WAAM_ERR_INVALID, "waam_graph_launch: NULL argument");
+    const int n = g->n_steps;
+    const int b = g->next;
+    /* the copy that last read this staging buffer has run before it is refilled */
+    hipError_t e = hipEventSynchronize(g->staged[b]);
+    if (e != hipSuccess)
+        return set_error(WAAM_ERR_HIP, "waam_graph_launch: %s", hipGetErrorString(e));
+    ArcStep *t = g->stage[b];
+    for (int k = 0; k < n; ++k) {
+        const double x = arc_i[k], y = arc_j[k];
+        if (!(x > -1e9 && x < 1e9) || !(y > -1e9 && y < 1e9))
+            return set_error(WAAM_ERR_INVALID, "waam_graph_launch: arc centre of step %d", k);
+        ArcStep a = g->base.at;
+        /* the column as the host computes it for a direct step: truncation of the double
+         * centre, clamped to the grid */
+        a.col_i = std::max(0, std::min((int)x, g->nx - 1));
+        a.col_j = std::max(0, std::min((int)y, g->ny - 1));
+        a.arc_i = (float)x;
+        a.arc_j = (float)y;
+        if (k_robot) a.k_robot = k_robot[k];
+        if (dir) {
+            a.dir_x = dir[3 * k];
+            a.dir_y = dir[3 * k + 1];
+            a.dir_z = dir[3 * k + 2];
+        }
+        arc_box(g->ri, g->rj, a.arc_i, a.arc_j, a);
+        t[k] = a;
+    }
+    hipStream_t st = as_stream(stream);
+    e = hipMemcpyAsync(g->track, t, sizeof(ArcStep) * (size_t)n, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipEventRecord(g->staged[b], st);
+    if (e == hipSuccess) e = hipGraphLaunch(g->exec, st);
+    if (e != hipSuccess)
+        return set_error(WAAM_ERR_HIP, "waam_graph_launch: %s", hipGetErrorString(e));
+    g->next = b ^ 1;
+    return 0;
+}
+
+int waam_graph_steps(const waam_step_graph *g) { return g ? g->n_steps : 0; }
+
+int waam_graph_destroy(waam_step_graph *g)
+{
+    graph_free(g);
+    return 0;
+}
+
 
 int waam_stream_pending(const void *f_src, void *f_dst, const uint8_t *sflags, int nx, int ny,
                         int nz, int layout, int pop_dtype, void *stream)

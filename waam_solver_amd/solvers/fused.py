@@ -25,7 +25,10 @@ syncing does not perturb a reduced-precision run.
 
 from __future__ import annotations
 
+import collections
 import ctypes
+import sys
+
 import numpy as np
 import torch
 
@@ -79,6 +82,10 @@ def _swap(g, a: str, b: str) -> None:
 
 
 class FusedStepper:
+    # steps per captured graph (WAAMTwin.step_many); graphs kept per stepper
+    graph_steps = 32
+    graph_cache = 4
+
     def __init__(self, twin):
         self.twin = twin
         g = twin.grid
@@ -121,6 +128,9 @@ class FusedStepper:
             g.hf_b = torch.zeros(shape, dtype=dt, device=g.device)
         # the swapped pairs alternate between two pointer sets: keep one struct per set
         self._fields_cache: dict[tuple, StepFields] = {}
+        # captured runs of steps (waam_graph_capture), least recently used first
+        self._graphs: collections.OrderedDict = collections.OrderedDict()
+        self.captured_steps = 0  # steps run through captured graphs
 
     # ------------------------------------------------------------------ deferred stream
     def _pop_buffers(self):
@@ -218,10 +228,9 @@ class FusedStepper:
         p.nz_solid = tw.nz_solid
         p.col_i = max(0, min(int(arc_i), g.nx - 1))
         p.col_j = max(0, min(int(arc_j), g.ny - 1))
-        if getattr(tw, "use_torch_z", False) and torch_z_m is not None:
-            z_bead = max(tw.substrate_z_m, torch_z_m - tw.ctwd_m)
-            k_robot = tw.nz_solid + (z_bead - tw.substrate_z_m) / g.dx - 1.0
-            p.k_robot = max(float(tw.nz_solid) + 0.5, min(k_robot, float(g.nz) - 2.0))
+        k_robot = self._k_robot(torch_z_m)
+        if k_robot is not None:
+            p.k_robot = k_robot
             p.use_k_robot = 1
         p.arc_i, p.arc_j = arc_i, arc_j
         p.Q_w, p.eta, p.dt, p.dx, p.dx3 = tw.Q_w, tw.eta, g.dt, g.dx, g.dx ** 3
@@ -259,6 +268,15 @@ class FusedStepper:
         p.C_acc = float(getattr(tw, "recoil_accommodation", 0.54))
         p.gas_tau = weld_forces.gas_shear_tau_pa(tw)
         return p
+
+    def _k_robot(self, torch_z_m) -> float | None:
+        """Floor for arc_k from the robot's torch height (None: not used)."""
+        tw, g = self.twin, self.twin.grid
+        if not (getattr(tw, "use_torch_z", False) and torch_z_m is not None):
+            return None
+        z_bead = max(tw.substrate_z_m, torch_z_m - tw.ctwd_m)
+        k_robot = tw.nz_solid + (z_bead - tw.substrate_z_m) / g.dx - 1.0
+        return max(float(tw.nz_solid) + 0.5, min(k_robot, float(g.nz) - 2.0))
 
     def step(self, torch_x_m: float, torch_y_m: float, is_welding: bool,
              torch_z_m: float | None = None) -> None:
@@ -312,3 +330,90 @@ class FusedStepper:
         self.pending = True
         self._list_epoch = write_epoch()
         tw._step_n += 1
+
+    # ------------------------------------------------------------------ captured steps
+    def steady(self) -> bool:
+        """Whether the next step may be a captured one: the last step was a fused step that
+        left its population stream pending and its thermal list built, and did not take its
+        forces from the fields, and nothing was written since."""
+        return (self.pending and not self._forces_from_fields
+                and self._list_epoch == write_epoch())
+
+    def _odd_fields(self, fe: StepFields) -> StepFields:
+        fo = StepFields.from_buffer_copy(fe)
+        for a, b in (("f_src", "f_dst"), ("H_in", "H_out"), ("T_in", "T_out"),
+                     ("fl_in", "fl_out"), ("flags_in", "flags_out"),
+                     ("sflags_prev", "sflags_next")):
+            setattr(fo, a, getattr(fe, b))
+            setattr(fo, b, getattr(fe, a))
+        return fo
+
+    def _graph(self, p: StepParams, n: int):
+        """The captured graph of n steps for these parameters and the current buffers."""
+        fe = self._fields()
+        fo = self._odd_fields(fe)
+        key = (bytes(p), bytes(fe), n)
+        gr = self._graphs.get(key)
+        if gr is None:
+            while len(self._graphs) >= self.graph_cache:
+                _, old = self._graphs.popitem(last=False)
+                lib().call("waam_graph_destroy", old)
+            h = ctypes.c_void_p()
+            lib().call("waam_graph_capture", ctypes.addressof(p), ctypes.addressof(fe),
+                       ctypes.addressof(fo), n, ctypes.byref(h))
+            gr = self._graphs[key] = h
+        else:
+            self._graphs.move_to_end(key)
+        return gr
+
+    def step_captured(self, arc_i: np.ndarray, arc_j: np.ndarray, is_welding: bool,
+                      k_robot: np.ndarray | None, dirs: np.ndarray) -> None:
+        """len(arc_i) steady fused steps as one graph launch: arc centres (cells, float64),
+        robot arc_k floors (float32 or None) and torch directions (float32, (n, 3)) per step;
+        the same kernels and results as that many ``step`` calls. The caller checks
+        ``steady()`` and that no step of the run is a droplet step."""
+        tw, g = self.twin, self.twin.grid
+        n = len(arc_i)
+        p = self._params(float(arc_i[0]), float(arc_j[0]), is_welding, None)
+        if k_robot is not None:
+            p.use_k_robot = 1
+        p.pending_stream = 1
+        p.force_reset_all = 0
+        p.thermal_list_ready = 1 if g.n_cells <= self.shared_thermal_max_cells else 2
+        # the moving parts come from the track: zero them in the graph's key
+        p.arc_i = p.arc_j = p.k_robot = p.dir_x = p.dir_y = p.dir_z = 0.0
+        p.col_i = p.col_j = 0
+        if self.pop_code and not self.h_live:
+            self._pack()
+        gr = self._graph(p, n)
+        ai = np.ascontiguousarray(arc_i, dtype=np.float64)
+        aj = np.ascontiguousarray(arc_j, dtype=np.float64)
+        kr = None if k_robot is None else np.ascontiguousarray(k_robot, dtype=np.float32)
+        dv = np.ascontiguousarray(dirs, dtype=np.float32).reshape(n, 3)
+        lib().call("waam_graph_launch", gr, ai.ctypes.data, aj.ctypes.data,
+                   None if kr is None else kr.ctypes.data, dv.ctypes.data,
+                   torch.cuda.current_stream().cuda_stream)
+        for _ in range(n):
+            for a, b in (("H", "H_tmp"), ("T", "T_tmp"), ("f_l", "fl_tmp"),
+                         ("flags", "flags_tmp"), ("sflags", "sflags_tmp")):
+                _swap(g, a, b)
+            g.swap_buffers()
+        self.pending = True
+        self._forces_from_fields = False
+        self._list_epoch = write_epoch()
+        tw._step_n += n
+        self.captured_steps += n
+
+    def close(self) -> None:
+        """Destroy the captured graphs (their device track and pinned staging)."""
+        graphs, self._graphs = getattr(self, "_graphs", {}), collections.OrderedDict()
+        for h in graphs.values():
+            lib().call("waam_graph_destroy", h)
+
+    def __del__(self):
+        if sys.is_finalizing():  # the runtime may be gone; the process frees everything
+            return
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -23,8 +23,9 @@ from .gpu_tables import MaterialGPUTables
 from .grid import WAAMGrid
 from .materials import MaterialProps, load_material
 from .physics.arc import create_heat_source
-from .physics.deposition_balance import (droplet_mass_kg, expected_deposited_mass_kg,
-                                         infer_transfer_mode, wire_mass_flux_kg_s)
+from .physics.deposition_balance import (droplet_mass_kg, droplet_period_s,
+                                         expected_deposited_mass_kg, infer_transfer_mode,
+                                         wire_mass_flux_kg_s)
 from .physics.bead_geometry import bead_reinforcement_height_mm, estimate_toe_angle_deg
 from .physics.electrical_stickout import droplet_entry_temperature_K
 from .physics.force_diagnostics import sample_force_diagnostics
@@ -171,6 +172,8 @@ class WAAMTwin:
         self._lorentz_unconverged_streak = 0
         self._lorentz_warm = False
         self._fused: FusedStepper | None = None
+        # fused steps in runs go out as captured graphs (step_many, run_path)
+        self.use_step_graph = True
         self.probe_recorder = None
         self._job_path: str | None = None
         self.enable_force_diagnostics = True
@@ -381,6 +384,85 @@ class WAAMTwin:
             self.probe_recorder.record_step(self)
         self._check_strict_mode()
 
+    def step_many(self, torch_xs, torch_ys, is_welding: bool = True, torch_zs=None) -> None:
+        """``step`` for each position in turn, with the same results. With the fused engine,
+        runs of ``FusedStepper.graph_steps`` steady steps (no droplet, no window shift, no
+        per-step host work such as probes or CTWD) go to the GPU as one captured graph."""
+        xs = [float(v) for v in torch_xs]
+        ys = [float(v) for v in torch_ys]
+        zs = None if torch_zs is None else list(torch_zs)
+        if len(ys) != len(xs) or (zs is not None and len(zs) != len(xs)):
+            raise ValueError("step_many: position lists differ in length")
+        s, n = 0, len(xs)
+        while s < n:
+            m = self._captured_run(xs, zs, s, is_welding)
+            if m:
+                self._step_captured(xs[s:s + m], ys[s:s + m],
+                                    None if zs is None else zs[s:s + m], is_welding)
+                s += m
+            else:
+                self.step(xs[s], ys[s], is_welding, None if zs is None else zs[s])
+                s += 1
+
+    def _captured_run(self, xs, zs, s: int, is_welding: bool) -> int:
+        """Steps from position s that may go out as one captured graph (0: step directly)."""
+        st = self._fused
+        if not self.use_step_graph or self.engine != "fused" or st is None:
+            return 0
+        m = st.graph_steps
+        if m <= 0 or s + m > len(xs) or not st.steady():
+            return 0
+        if (self.use_material_tables or self.enable_lorentz or self.enable_ctwd
+                or self.probe_recorder is not None
+                or type(self.arc_source).__name__ == "ConicalVolume"):
+            return 0
+        if (is_welding and self.droplet_freq > 0 and self.wire_feed_m_s <= 0.0
+                and not self._warned_zero_wire_feed):
+            return 0
+        if self.use_torch_z and zs is not None and any(z is None for z in zs[s:s + m]):
+            return 0
+        g = self.grid
+        if is_welding and self.droplet_freq > 0:
+            period = droplet_period_s(self)
+            for k in range(m):
+                if not ((self._step_n + k) * g.dt - self._last_droplet_time < period):
+                    return 0
+        if self.enable_moving_window:
+            trigger = 0.62 * (g.nx - 8) * g.dx
+            if any(x - self._window_offset_x_m >= trigger for x in xs[s:s + m]):
+                return 0
+        return m
+
+    def _step_captured(self, xs, ys, zs, is_welding: bool) -> None:
+        """The host side of ``step`` for each position (torch direction, last position, arc
+        centre), then one captured launch of all the steps."""
+        g, st = self.grid, self._fused
+        m = len(xs)
+        arc_i = np.empty(m, dtype=np.float64)
+        arc_j = np.empty(m, dtype=np.float64)
+        dirs = np.empty((m, 3), dtype=np.float32)
+        use_kr = self.use_torch_z and zs is not None
+        k_robot = np.empty(m, dtype=np.float32) if use_kr else None
+        for k in range(m):
+            x, y = xs[k], ys[k]
+            z = None if zs is None else zs[k]
+            z_now = 0.0 if z is None else float(z)
+            if self._last_torch_pos_m is not None:
+                px, py, pz = self._last_torch_pos_m
+                ddx, ddy, ddz = x - px, y - py, z_now - pz
+                nrm = math.sqrt(ddx * ddx + ddy * ddy + ddz * ddz)
+                if nrm > 1e-12:
+                    self._torch_dir_xyz = (ddx / nrm, ddy / nrm, ddz / nrm)
+            self._last_torch_pos_m = (x, y, z_now)
+            arc_i[k] = (x - self._window_offset_x_m - self._sim_origin_offset_x_m) / g.dx
+            arc_j[k] = (y - self._sim_origin_offset_y_m) / g.dx
+            dirs[k] = self._torch_dir_xyz
+            if use_kr:
+                k_robot[k] = st._k_robot(z)
+        st.step_captured(arc_i, arc_j, is_welding, k_robot, dirs)
+        self._last_arc_ijk = (float(arc_i[-1]), float(arc_j[-1]), None)
+        self._check_strict_mode()
+
     def _maybe_shift_window(self, torch_x_world: float) -> None:
         """Shift the domain by max(8, nx // 10) cells once the torch passes 62 % of the usable
         length (reference twin.py:599-626); the torch offset and tracer positions follow."""
@@ -464,17 +546,35 @@ class WAAMTwin:
             cx, cy = clamp_torch_to_domain(x_m - off, y_m, g.nx, g.ny, g.dx)
             return cx + off, cy
 
+        # without the moving window the clamp does not depend on the steps taken, so the
+        # positions are planned first and runs of equal welding state go through step_many
+        plan: list[tuple[float, float, bool, float]] = []
+
+        def _go(x_m, y_m, welding, z_m):
+            if self.enable_moving_window:
+                cx, cy = _clamp(x_m, y_m)
+                self.step(cx, cy, welding, torch_z_m=z_m)
+            else:
+                plan.append((*_clamp(x_m, y_m), welding, z_m))
+
         for step, x, y, z in driver.positions_for_steps(n_steps, g.dt):
             seg = driver.segment_index_at_distance(driver.distance_at_step(step, g.dt))
             if interpass > 0 and seg > prev_seg and prev_seg >= 0:
                 park = driver.segment_end(prev_seg)
                 px, py, pz = park if park else (x, y, z)
                 for _ in range(interpass):
-                    cx, cy = _clamp(px, py)
-                    self.step(cx, cy, is_welding=False, torch_z_m=pz)
+                    _go(px, py, False, pz)
             prev_seg = seg
-            cx, cy = _clamp(x, y)
-            self.step(cx, cy, is_welding, torch_z_m=z)
+            _go(x, y, is_welding, z)
+        s = 0
+        while s < len(plan):
+            e = s
+            while e < len(plan) and plan[e][2] == plan[s][2]:
+                e += 1
+            run = plan[s:e]
+            self.step_many([r[0] for r in run], [r[1] for r in run], run[0][2],
+                           [r[3] for r in run])
+            s = e
 
     # ------------------------------------------------------------------ telemetry
     def pool_reduce(self) -> np.ndarray:
