@@ -31,7 +31,13 @@ WORKLOAD = dict(nx=256, ny=128, nz=64, dx=2e-4, plate_thickness_mm=4.8, arc_powe
 # The same physics on 768 x 384 x 192 cells (56.6 M, 153.6 x 76.8 x 38.4 mm at 0.2 mm), the
 # largest single-GPU configuration named for this path (fp32 state: about 16 GB of HBM).
 LARGE = dict(WORKLOAD, nx=768, ny=384, nz=192, x0_mm=30.0, steps=400)
-WORKLOADS = {"bead_on_plate": WORKLOAD, "large": LARGE}
+# Three passes on one line with interpass cooling, 20 000 steps on 768 x 384 x 192 cells at
+# dx = 1 mm (dt = 0.1 ms: two 0.5 s interpasses take 10 000 steps at this dx; at 0.2 mm they
+# alone would need 50 000): passes of 3334 welding steps (26.7 mm at 8 mm/s), interpasses of
+# 4999 steps with the arc off at the end of the pass; 16-bit populations, captured steps.
+MULTIPASS = dict(WORKLOAD, nx=768, ny=384, nz=192, dx=1e-3, plate_thickness_mm=48.0,
+                 x0_mm=200.0, passes=3, pass_steps=3334, interpass_steps=4999)
+WORKLOADS = {"bead_on_plate": WORKLOAD, "large": LARGE, "multipass": MULTIPASS}
 
 
 def twin_kwargs(w: dict) -> dict:
@@ -242,6 +248,78 @@ def captured_form(tw, w: dict, start: int, steps: int) -> dict:
             "captured_steps": st.captured_steps - c0, "steps": steps}
 
 
+def multipass(args) -> int:
+    """The three-pass run: MLUPS over all 20 000 steps (droplet steps included), fp64 sums of
+    rho over the metal cells and of the enthalpy (J) after every pass and interpass."""
+    import torch
+
+    from waam_solver_amd.field import tensor_of
+
+    w = MULTIPASS
+    pop = args.pop if args.pop != "fp32" or args.multipass_fp32 else "fp16"
+    tw = build_gpu_twin(w, pop)
+    g = tw.grid
+    y0 = (g.ny // 2) * g.dx
+
+    def sums(after: str) -> dict:
+        tw.sync()
+        metal = tensor_of(g.flags) != 2
+        rho = float(tensor_of(g.rho).double()[metal].sum().item())
+        h_j = float(tensor_of(g.H).double().sum().item()) * g.dx ** 3
+        return {"after": after, "step": tw._step_n, "sum_rho_metal": rho, "sum_H_J": h_j,
+                "metal_cells": int(metal.sum().item())}
+
+    segs = []
+    x_end = w["x0_mm"] * 1e-3
+    for p in range(w["passes"]):
+        xs = [w["x0_mm"] * 1e-3 + w["travel_speed_m_s"] * s * g.dt for s in range(w["pass_steps"])]
+        segs.append((f"pass {p + 1}", xs, True))
+        x_end = xs[-1]
+        if p < w["passes"] - 1:
+            segs.append((f"interpass {p + 1}", [x_end] * w["interpass_steps"], False))
+    ledger = [sums("start")]
+    el, steps = 0.0, 0
+    for name, xs, weld in segs:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tw.step_many(xs, [y0] * len(xs), weld)
+        torch.cuda.synchronize()
+        el += time.perf_counter() - t0
+        steps += len(xs)
+        ledger.append(sums(name))
+        print(f"[multipass] {name}: {len(xs)} steps, {el:.1f} s so far", file=sys.stderr,
+              flush=True)
+    tel = tw.get_telemetry()
+    t_weld = w["passes"] * w["pass_steps"] * g.dt
+    first, last = ledger[0], ledger[-1]
+    result = {
+        "metric": f"coupled-step MLUPS (three-pass {g.nx}x{g.ny}x{g.nz} dx=1mm, {pop})",
+        "value": round(g.n_cells * steps / el / 1e6, 2), "unit": "MLUPS", "n_gpus": 1,
+        "steps": steps, "warmup": 0, "ms_per_step": round(el * 1e3 / steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if pop == "fp32" else f"f32 arithmetic, {pop} populations",
+        "data": "synthetic",
+        "config": {"workload": f"three passes on one line, {w['pass_steps']} welding steps each "
+                               f"(8 mm/s), {w['interpass_steps']}-step interpasses, "
+                               f"{g.nx}x{g.ny}x{g.nz} dx=1mm dt=0.1ms, Gaussian2D 4kW eta0.8 "
+                               "sigma2mm, wire feed 6.5m/min, flow tier, MRT",
+                   "grid": [g.nx, g.ny, g.nz], "cells": g.n_cells,
+                   "population_storage": pop, "captured_steps": tw._fused.captured_steps,
+                   "steps_per_graph": tw._fused.graph_steps},
+        "ledger": ledger,
+        "drift": {"sum_rho_metal_rel": (last["sum_rho_metal"] - first["sum_rho_metal"])
+                  / first["sum_rho_metal"],
+                  "sum_H_rel": (last["sum_H_J"] - first["sum_H_J"]) / first["sum_H_J"],
+                  "sum_H_gain_J": last["sum_H_J"] - first["sum_H_J"],
+                  "arc_energy_J": w["arc_efficiency"] * w["arc_power_W"] * t_weld},
+        "telemetry": {k: tel[k] for k in ("pool_width_mm", "pool_depth_mm", "peak_temp_K",
+                                          "n_droplets_fired", "deposited_mass_g",
+                                          "mass_balance_ratio") if k in tel},
+    }
+    print(json.dumps(result))
+    return 0
+
+
 def measured_traffic() -> float | None:
     """HBM bytes per k_step_pull launch from the committed rocprofv3 counter summary."""
     try:
@@ -259,7 +337,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", choices=tuple(WORKLOADS), default="bead_on_plate",
                     help="bead_on_plate: 256x128x64 (the metric's configuration); "
-                         "large: 768x384x192")
+                         "large: 768x384x192; multipass: three passes, 20 000 steps on "
+                         "768x384x192 at dx=1mm")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -269,6 +348,8 @@ def main() -> int:
                     help="steps per kernel for the per-kernel HIP-event breakdown (0: skip)")
     ap.add_argument("--pop", choices=("fp32", "fp16", "bf16"), default="fp32",
                     help="population storage of the measured run (16-bit forms store f - w_q)")
+    ap.add_argument("--multipass-fp32", action="store_true",
+                    help="the multipass workload with fp32 populations (default: fp16)")
     ap.add_argument("--no-graph", action="store_true",
                     help="skip the captured-graph run reported beside the value")
     ap.add_argument("--no-precision", action="store_true",
@@ -286,6 +367,8 @@ def main() -> int:
 
     from waam_solver_amd._lib import STEP_KERNELS, lib
 
+    if args.workload == "multipass":
+        return multipass(args)
     w = dict(WORKLOADS[args.workload])
     if args.steps is None:
         args.steps = w["steps"]
