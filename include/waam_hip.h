@@ -543,7 +543,8 @@ int waam_step_arc_readback(const void *scratch, float *host_out3);
  * copies it on `stream` and launches the graph there.
  * It blocks only until the copy made two launches earlier has run. Launches of one graph
  * must use one stream; they run the same kernels, with the same results, as n_steps
- * waam_step_fused calls. Kernel timing (waam_timing_begin) does not see captured kernels. */
+ * waam_step_fused calls. Kernel timing (waam_timing_begin) does not see captured kernels.
+ * waam_graph_destroy waits for the device to drain, then frees the graph. */
 typedef struct waam_step_graph waam_step_graph;
 int waam_graph_capture(const waam_step_params *p, const waam_step_fields *f_even,
                        const waam_step_fields *f_odd, int n_steps, waam_step_graph **out);

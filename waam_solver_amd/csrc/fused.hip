@@ -1433,6 +1433,9 @@ using namespace waam;
 static void graph_free(waam_step_graph *g)
 {
     if (!g) return;
+    /* a replay may still be running or its track copy pending: let the device drain before the
+     * graph, its track and the staging buffers go */
+    if (g->exec) (void)hipDeviceSynchronize();
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
     for (int b = 0; b < 2; ++b) {
