@@ -422,6 +422,14 @@ int waam_adj_thermal_step(const float *lam_next, float *lam_out, const float *H_
                           const uint8_t *flags, const double *arc_sums,
                           const waam_adj_consts *c, double *grads, float *lamC, int nx, int ny,
                           int nz, int layout, void *scratch, void *stream);
+/* Adjoint of compute_marangoni_force (kernels.py:2158-2217; waam_compute_marangoni_force) with
+ * phi, f_l and the flags held, for the force adjoint lam = dL/dF (three n-cell fp32 fields):
+ * T_adj (n doubles) += dL/dT, and s_cell[c] (n doubles) = lam_c . F_c / dgamma, whose sum is
+ * dL/d(dgamma). G is a work buffer of 3n doubles. */
+int waam_adj_marangoni(const float *T, const float *phi, const float *f_l, const uint8_t *flags,
+                       const float *lam_x, const float *lam_y, const float *lam_z, float dgamma,
+                       double *T_adj, double *s_cell, double *G, int nx, int ny, int nz,
+                       int layout, void *stream);
 
 /* ---- export-side fields and reductions -------------------------------------------------- */
 /* compute_vorticity_magnitude kernels.py:3380-3419: |curl u| / dt on non-GAS cells, 0 on GAS */

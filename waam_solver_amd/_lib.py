@@ -78,6 +78,7 @@ SIGNATURES: dict[str, str] = {
     "waam_pack_populations": "pplip",
     "waam_unpack_populations": "pplip",
     "waam_step_arc_readback": "pp",
+    "waam_adj_marangoni": "pppppppfpppiiiip",
     "waam_graph_capture": "pppip",
     "waam_graph_launch": "pppppp",
     "waam_graph_steps": "p",

@@ -246,3 +246,57 @@ def fit_calibration(rollout: ThermalRollout, T_target, fl_target, theta0, iters:
         hist["loss"].append(float(loss))
         opt.step()
     return hist
+
+
+# ---------------------------------------------------------------------------- Marangoni forcing
+def marangoni_force(T: torch.Tensor, phi: torch.Tensor, f_l: torch.Tensor, flags: torch.Tensor,
+                    dgamma: float, dims: tuple[int, int, int], layout: int):
+    """The thermocapillary force (reference kernels.py:2158-2217) on device tensors, through the
+    product kernel: (Fx, Fy, Fz), zero where it does not apply."""
+    nx, ny, nz = dims
+    F = [torch.zeros_like(T) for _ in range(3)]
+    st = torch.cuda.current_stream().cuda_stream
+    lib().call("waam_compute_marangoni_force", T.data_ptr(), phi.data_ptr(), f_l.data_ptr(),
+               F[0].data_ptr(), F[1].data_ptr(), F[2].data_ptr(), flags.data_ptr(), float(dgamma),
+               nx, ny, nz, layout, st)
+    return tuple(F)
+
+
+def marangoni_force_vjp(T, phi, f_l, flags, lam, dgamma: float, dims, layout: int):
+    """Hand-written adjoint (``waam_adj_marangoni``) of ``marangoni_force`` with phi, f_l and
+    the flags held: for lam = (dL/dFx, dL/dFy, dL/dFz), returns dL/dT (binary64, shaped like
+    T) and dL/d(dgamma) (a binary64 sum over the cells)."""
+    nx, ny, nz = dims
+    lam = [x.contiguous().float() for x in lam]
+    T_adj = torch.zeros(T.shape, dtype=torch.float64, device=T.device)
+    s_cell = torch.empty(T.shape, dtype=torch.float64, device=T.device)
+    G = torch.empty((3,) + tuple(T.shape), dtype=torch.float64, device=T.device)
+    st = torch.cuda.current_stream().cuda_stream
+    lib().call("waam_adj_marangoni", T.data_ptr(), phi.data_ptr(), f_l.data_ptr(),
+               flags.data_ptr(), lam[0].data_ptr(), lam[1].data_ptr(), lam[2].data_ptr(),
+               float(dgamma), T_adj.data_ptr(), s_cell.data_ptr(), G.data_ptr(), nx, ny, nz,
+               layout, st)
+    return T_adj, s_cell.sum()
+
+
+class MarangoniForceFn(torch.autograd.Function):
+    """(T, dgamma) -> (Fx, Fy, Fz) as a ``torch.autograd.Function``: forward through the
+    product kernel, backward through the adjoint kernels (phi, f_l and the flags held, as the
+    thermal rollout holds them)."""
+
+    @staticmethod
+    def forward(ctx, T, dgamma, phi, f_l, flags, dims, layout):
+        ctx.save_for_backward(T, phi, f_l, flags)
+        ctx.dgamma, ctx.dims, ctx.layout = float(dgamma), dims, layout
+        return marangoni_force(T, phi, f_l, flags, float(dgamma), dims, layout)
+
+    @staticmethod
+    def backward(ctx, gx, gy, gz):
+        T, phi, f_l, flags = ctx.saved_tensors
+        zero = torch.zeros_like(T)
+        lam = [g if g is not None else zero for g in (gx, gy, gz)]
+        T_adj, d_dg = marangoni_force_vjp(T, phi, f_l, flags, lam, ctx.dgamma, ctx.dims,
+                                          ctx.layout)
+        g_T = T_adj.to(T.dtype) if ctx.needs_input_grad[0] else None
+        g_dg = d_dg.reshape(()) if ctx.needs_input_grad[1] else None
+        return g_T, g_dg, None, None, None, None, None

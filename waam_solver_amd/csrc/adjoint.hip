@@ -280,6 +280,82 @@ __global__ void __launch_bounds__(kBlock) k_adj_step2(
 
 using namespace waam;
 
+/* Adjoint of compute_marangoni_force (reference kernels.py:2158-2217) with phi, f_l and the
+ * flags held: F_c = dgamma * gm_c * (I - n_c n_c^T) grad T_c on the cells it applies to, linear
+ * in T. Pass 1, per cell c: G_c = dgamma * gm_c * (I - n n^T) lambda_c (the projection is
+ * symmetric) and s_c = lambda_c . F_c / dgamma, both in binary64 from the forward's fp32 n and
+ * gm; zero where the forward adds nothing. */
+__global__ void k_adj_marangoni_project(const float *__restrict__ T, const float *__restrict__ phi,
+                                        const float *__restrict__ f_l,
+                                        const uint8_t *__restrict__ flags,
+                                        const float *__restrict__ lx, const float *__restrict__ ly,
+                                        const float *__restrict__ lz, float dgamma,
+                                        double *__restrict__ G, double *__restrict__ s_cell,
+                                        Dims d)
+{
+    int i, j, k;
+    if (!d.cell(i, j, k)) return;
+    const size_t N = d.n();
+    const size_t c = d.idx(i, j, k);
+    double gx = 0.0, gy = 0.0, gz = 0.0, sc = 0.0;
+    const uint8_t fg = flags[c];
+    if (fg != SOLID && fg != GAS && !(f_l[c] < 0.05f)) {
+        const int ip = imin(i + 1, d.nx - 1), im = imax(i - 1, 0);
+        const int jp = imin(j + 1, d.ny - 1), jm = imax(j - 1, 0);
+        const int kp = imin(k + 1, d.nz - 1), km = imax(k - 1, 0);
+        float dpx = 0.5f * (phi[d.idx(ip, j, k)] - phi[d.idx(im, j, k)]);
+        float dpy = 0.5f * (phi[d.idx(i, jp, k)] - phi[d.idx(i, jm, k)]);
+        float dpz = 0.5f * (phi[d.idx(i, j, kp)] - phi[d.idx(i, j, km)]);
+        float gm = sqrtf(dpx * dpx + dpy * dpy + dpz * dpz);
+        if (!(gm < 1e-6f)) {
+            const double nx = dpx / gm, ny = dpy / gm, nz = dpz / gm;
+            const double ax = lx[c], ay = ly[c], az = lz[c];
+            const double nl = nx * ax + ny * ay + nz * az;
+            const double w = (double)dgamma * (double)gm;
+            gx = w * (ax - nl * nx);
+            gy = w * (ay - nl * ny);
+            gz = w * (az - nl * nz);
+            const double tx = 0.5 * ((double)T[d.idx(ip, j, k)] - (double)T[d.idx(im, j, k)]);
+            const double ty = 0.5 * ((double)T[d.idx(i, jp, k)] - (double)T[d.idx(i, jm, k)]);
+            const double tz = 0.5 * ((double)T[d.idx(i, j, kp)] - (double)T[d.idx(i, j, km)]);
+            const double nt = nx * tx + ny * ty + nz * tz;
+            sc = (double)gm * (ax * (tx - nt * nx) + ay * (ty - nt * ny) + az * (tz - nt * nz));
+        }
+    }
+    G[c] = gx;
+    G[N + c] = gy;
+    G[2 * N + c] = gz;
+    s_cell[c] = sc;
+}
+
+/* Pass 2, gather: dL/dT at m += 0.5 * (G of the cells whose clamped +/- neighbour is m, with
+ * the sign of that side). Along x: +G_x of m-1 (i >= 1) and of m itself at i = nx-1, -G_x of
+ * m+1 (i <= nx-2) and of m itself at i = 0; likewise along y and z. */
+__global__ void k_adj_marangoni_gather(const double *__restrict__ G, double *__restrict__ T_adj,
+                                       Dims d)
+{
+    int i, j, k;
+    if (!d.cell(i, j, k)) return;
+    const size_t N = d.n();
+    const size_t c = d.idx(i, j, k);
+    const int n[3] = {d.nx, d.ny, d.nz};
+    const int at[3] = {i, j, k};
+    double acc = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double *Ga = G + (size_t)a * N;
+        const int v = at[a];
+        int lo[3] = {i, j, k}, hi[3] = {i, j, k};
+        lo[a] = v - 1;
+        hi[a] = v + 1;
+        if (v >= 1) acc += Ga[d.idx(lo[0], lo[1], lo[2])];
+        if (v == n[a] - 1) acc += Ga[c];
+        if (v <= n[a] - 2) acc -= Ga[d.idx(hi[0], hi[1], hi[2])];
+        if (v == 0) acc -= Ga[c];
+    }
+    T_adj[c] += 0.5 * acc;
+}
+
 extern "C" {
 
 int waam_adj_arc_sums(const uint8_t *flags, const float *f_l, const waam_adj_consts *ac,
@@ -333,6 +409,23 @@ int waam_adj_thermal_step(const float *lam_next, float *lam_out, const float *H_
                        flags, arc_sums, a, d, lam_out, parts);
     hipLaunchKernelGGL(k_adj_total, dim3(1), dim3(kBlock), 0, s, parts, kAdjBlocks, 3, grads, 1);
     return check_launch("waam_adj_thermal_step");
+}
+
+int waam_adj_marangoni(const float *T, const float *phi, const float *f_l, const uint8_t *flags,
+                       const float *lam_x, const float *lam_y, const float *lam_z, float dgamma,
+                       double *T_adj, double *s_cell, double *G, int nx, int ny, int nz,
+                       int layout, void *stream)
+{
+    WAAM_RETURN_IF(check_dims("waam_adj_marangoni", nx, ny, nz, layout));
+    WAAM_RETURN_IF(check_ptrs("waam_adj_marangoni",
+                              {T, phi, f_l, flags, lam_x, lam_y, lam_z, T_adj, s_cell, G}));
+    Dims d{nx, ny, nz, layout};
+    hipStream_t st = as_stream(stream);
+    hipLaunchKernelGGL(k_adj_marangoni_project, grid3(d), block3(), 0, st, T, phi, f_l, flags,
+                       lam_x, lam_y, lam_z, dgamma, G, s_cell, d);
+    hipLaunchKernelGGL(k_adj_marangoni_gather, grid3(d), block3(), 0, st, (const double *)G, T_adj,
+                       d);
+    return check_launch("waam_adj_marangoni");
 }
 
 }  // extern "C"
