@@ -431,6 +431,14 @@ int waam_adj_marangoni(const float *T, const float *phi, const float *f_l, const
                        double *T_adj, double *s_cell, double *G, int nx, int ny, int nz,
                        int layout, void *stream);
 
+/* Adjoint of collide_srt (kernels.py:2815-2885; waam_collide_srt) with f_l and the flags held:
+ * for lam = dL/d(post-collision populations) (19 q-major n-cell fp32 arrays), lam_f (19n doubles)
+ * = dL/d(f_src) and lam_F (3n doubles) = dL/d(Fx, Fy, Fz); 0 on SOLID and GAS cells, which the
+ * collision does not write. */
+int waam_adj_collide_srt(const float *f_src, const float *Fx, const float *Fy, const float *Fz,
+                         const float *f_l, const uint8_t *flags, const float *lam, float omega,
+                         float C_darcy, double *lam_f, double *lam_F, int64_t n, void *stream);
+
 /* ---- export-side fields and reductions -------------------------------------------------- */
 /* compute_vorticity_magnitude kernels.py:3380-3419: |curl u| / dt on non-GAS cells, 0 on GAS */
 int waam_compute_vorticity_magnitude(const float *ux, const float *uy, const float *uz,

@@ -300,3 +300,20 @@ class MarangoniForceFn(torch.autograd.Function):
         g_T = T_adj.to(T.dtype) if ctx.needs_input_grad[0] else None
         g_dg = d_dg.reshape(()) if ctx.needs_input_grad[1] else None
         return g_T, g_dg, None, None, None, None, None
+
+
+# ---------------------------------------------------------------------------- collision
+def collide_srt_vjp(f_src, Fx, Fy, Fz, f_l, flags, lam, omega: float, C_darcy: float):
+    """Hand-written adjoint (``waam_adj_collide_srt``) of the BGK collision with Guo forcing and
+    the Carman-Kozeny drag (reference kernels.py:2815-2885), f_l and the flags held: for
+    lam = dL/d(post-collision populations) (19 q-major n-cell arrays), returns dL/d(f_src)
+    (float64, like f_src) and dL/d(Fx, Fy, Fz) (float64, (3,) + the field shape)."""
+    n = f_l.numel()
+    lam = lam.contiguous().float()
+    lam_f = torch.empty(f_src.shape, dtype=torch.float64, device=f_src.device)
+    lam_F = torch.empty((3,) + tuple(f_l.shape), dtype=torch.float64, device=f_src.device)
+    st = torch.cuda.current_stream().cuda_stream
+    lib().call("waam_adj_collide_srt", f_src.data_ptr(), Fx.data_ptr(), Fy.data_ptr(),
+               Fz.data_ptr(), f_l.data_ptr(), flags.data_ptr(), lam.data_ptr(), float(omega),
+               float(C_darcy), lam_f.data_ptr(), lam_F.data_ptr(), n, st)
+    return lam_f, lam_F

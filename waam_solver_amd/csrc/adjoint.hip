@@ -356,6 +356,79 @@ __global__ void k_adj_marangoni_gather(const double *__restrict__ G, double *__r
     T_adj[c] += 0.5 * acc;
 }
 
+/* Adjoint of collide_srt (reference kernels.py:2815-2885: BGK with Guo forcing and the
+ * Carman-Kozeny drag) with f_l and the flags held, on the cells it updates (neither SOLID nor
+ * GAS; elsewhere it writes nothing, so those cells' derivatives are 0). For the adjoint lam of
+ * the post-collision populations: lam_f = dL/df (19 q-major arrays) and lam_F = dL/dF (3 arrays),
+ * in binary64 from the fp32 inputs. With r = sum f, u = (j + F/2) / ((r + eps) ck):
+ *   out_q = (1 - omega) f_q + omega feq_q(r, u) + S_q(u, F),
+ *   feq_q = w r (1 + 3 eu + 4.5 eu^2 - 1.5 u.u), S_q = w (1 - omega/2)(3 Fe + 9 eu Fe - 3 u.F). */
+__global__ void k_adj_collide_srt(const float *__restrict__ f, const float *__restrict__ Fx,
+                                  const float *__restrict__ Fy, const float *__restrict__ Fz,
+                                  const float *__restrict__ f_l, const uint8_t *__restrict__ flags,
+                                  const float *__restrict__ lam, float omega_f, float C_darcy_f,
+                                  double *__restrict__ lam_f, double *__restrict__ lam_F, size_t N)
+{
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < N;
+         c += (size_t)gridDim.x * blockDim.x) {
+        double lf[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) lf[q] = 0.0;
+        double lFx = 0.0, lFy = 0.0, lFz = 0.0;
+        const uint8_t fg = flags[c];
+        if (fg != SOLID && fg != GAS) {
+            const double eps = 1e-6, om = omega_f;
+            double r = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const double v = f[(size_t)q * N + c];
+                r += v;
+                jx += v * ex(q);
+                jy += v * ey(q);
+                jz += v * ez(q);
+            }
+            const double fx = Fx[c], fy = Fy[c], fz = Fz[c], fl = f_l[c];
+            const double one_m = 1.0 - fl;
+            const double ck = 1.0 + (double)C_darcy_f * (one_m * one_m) / (fl * (fl * fl) + eps);
+            const double rr = r + eps, inv = 1.0 / (rr * ck);
+            const double u = (jx + 0.5 * fx) * inv, v = (jy + 0.5 * fy) * inv,
+                         w = (jz + 0.5 * fz) * inv;
+            const double u2 = u * u + v * v + w * w;
+            double lr = 0.0, lux = 0.0, luy = 0.0, luz = 0.0;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const double l = lam[(size_t)q * N + c];
+                const double e0 = ex(q), e1 = ey(q), e2 = ez(q);
+                const double W = q == 0 ? 1.0 / 3.0 : (q < 7 ? 1.0 / 18.0 : 1.0 / 36.0);
+                const double eu = e0 * u + e1 * v + e2 * w;
+                const double Fe = e0 * fx + e1 * fy + e2 * fz;
+                lf[q] = (1.0 - om) * l;
+                lr += om * l * W * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * u2);
+                const double a = om * l * W * r;            // feq's velocity terms
+                const double b = l * W * (1.0 - 0.5 * om);  // the forcing term
+                lux += a * (3.0 * e0 + 9.0 * eu * e0 - 3.0 * u) + b * (9.0 * Fe * e0 - 3.0 * fx);
+                luy += a * (3.0 * e1 + 9.0 * eu * e1 - 3.0 * v) + b * (9.0 * Fe * e1 - 3.0 * fy);
+                luz += a * (3.0 * e2 + 9.0 * eu * e2 - 3.0 * w) + b * (9.0 * Fe * e2 - 3.0 * fz);
+                lFx += b * (3.0 * e0 + 9.0 * eu * e0 - 3.0 * u);
+                lFy += b * (3.0 * e1 + 9.0 * eu * e1 - 3.0 * v);
+                lFz += b * (3.0 * e2 + 9.0 * eu * e2 - 3.0 * w);
+            }
+            /* through u: du/dj = inv, du/dF = inv / 2, du/dr = -u / (r + eps) */
+            lFx += 0.5 * lux * inv;
+            lFy += 0.5 * luy * inv;
+            lFz += 0.5 * luz * inv;
+            lr -= (lux * u + luy * v + luz * w) / rr;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) lf[q] += lr + (lux * ex(q) + luy * ey(q) + luz * ez(q)) * inv;
+        }
+#pragma unroll
+        for (int q = 0; q < Q; ++q) lam_f[(size_t)q * N + c] = lf[q];
+        lam_F[c] = lFx;
+        lam_F[N + c] = lFy;
+        lam_F[2 * N + c] = lFz;
+    }
+}
+
 extern "C" {
 
 int waam_adj_arc_sums(const uint8_t *flags, const float *f_l, const waam_adj_consts *ac,
@@ -426,6 +499,19 @@ int waam_adj_marangoni(const float *T, const float *phi, const float *f_l, const
     hipLaunchKernelGGL(k_adj_marangoni_gather, grid3(d), block3(), 0, st, (const double *)G, T_adj,
                        d);
     return check_launch("waam_adj_marangoni");
+}
+
+int waam_adj_collide_srt(const float *f_src, const float *Fx, const float *Fy, const float *Fz,
+                         const float *f_l, const uint8_t *flags, const float *lam, float omega,
+                         float C_darcy, double *lam_f, double *lam_F, int64_t n, void *stream)
+{
+    if (n <= 0) return set_error(WAAM_ERR_INVALID, "waam_adj_collide_srt: n <= 0");
+    WAAM_RETURN_IF(check_ptrs("waam_adj_collide_srt", {f_src, Fx, Fy, Fz, f_l, flags, lam, lam_f,
+                                                       lam_F}));
+    hipLaunchKernelGGL(k_adj_collide_srt, dim3(grid_for((size_t)n)), dim3(kBlock), 0,
+                       as_stream(stream), f_src, Fx, Fy, Fz, f_l, flags, lam, omega, C_darcy,
+                       lam_f, lam_F, (size_t)n);
+    return check_launch("waam_adj_collide_srt");
 }
 
 }  // extern "C"
