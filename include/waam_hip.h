@@ -439,6 +439,11 @@ int waam_adj_collide_srt(const float *f_src, const float *Fx, const float *Fy, c
                          const float *f_l, const uint8_t *flags, const float *lam, float omega,
                          float C_darcy, double *lam_f, double *lam_F, int64_t n, void *stream);
 
+/* Adjoint of stream (kernels.py:2991-3010; waam_stream), flags held: lam_in (19 q-major n-cell
+ * doubles) = dL/d(f_src) for lam_out = dL/d(f_dst); 0 for inputs no output copies. */
+int waam_adj_stream(const double *lam_out, double *lam_in, const uint8_t *flags, int nx, int ny,
+                    int nz, int layout, void *stream);
+
 /* ---- export-side fields and reductions -------------------------------------------------- */
 /* compute_vorticity_magnitude kernels.py:3380-3419: |curl u| / dt on non-GAS cells, 0 on GAS */
 int waam_compute_vorticity_magnitude(const float *ux, const float *uy, const float *uz,

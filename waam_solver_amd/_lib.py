@@ -80,6 +80,7 @@ SIGNATURES: dict[str, str] = {
     "waam_step_arc_readback": "pp",
     "waam_adj_marangoni": "pppppppfpppiiiip",
     "waam_adj_collide_srt": "pppppppffpplp",
+    "waam_adj_stream": "pppiiiip",
     "waam_graph_capture": "pppip",
     "waam_graph_launch": "pppppp",
     "waam_graph_steps": "p",

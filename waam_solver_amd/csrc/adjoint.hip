@@ -429,6 +429,57 @@ __global__ void k_adj_collide_srt(const float *__restrict__ f, const float *__re
     }
 }
 
+/* Adjoint of stream (reference kernels.py:2991-3010), flags held: the stream copies, into each
+ * population q of a non-SOLID cell c, population q of the neighbour n = c - e_q (x periodic,
+ * y and z clamped), or, for a cell that is not GAS whose neighbour is SOLID, its own
+ * population opp(q). Gather form: input (q, m) collects the adjoint of every output that
+ * copies it: the cells c with n(c, q) = m (c = m + e_q, or m itself on a clamped face) that are
+ * not SOLID and are GAS or see m not SOLID, and (opp(q), m) when m bounces back there. */
+__global__ void k_adj_stream(const double *__restrict__ lam_out, double *__restrict__ lam_in,
+                             const uint8_t *__restrict__ flags, Dims d)
+{
+    int i, j, k;
+    if (!d.cell(i, j, k)) return;
+    const size_t N = d.n();
+    const size_t m = d.idx(i, j, k);
+    const uint8_t fm = flags[m];
+    auto src_of = [&](int ci, int cj, int ck, int q, int &si, int &sj, int &sk) {
+        si = ci - ex(q);
+        si = si < 0 ? si + d.nx : (si >= d.nx ? si - d.nx : si);
+        sj = imax(0, imin(cj - ey(q), d.ny - 1));
+        sk = imax(0, imin(ck - ez(q), d.nz - 1));
+    };
+    for (int q = 0; q < Q; ++q) {
+        double acc = 0.0;
+        int ci = i + ex(q);
+        ci = ci < 0 ? ci + d.nx : (ci >= d.nx ? ci - d.nx : ci);
+        const int cy[2] = {j + ey(q), j}, cz[2] = {k + ez(q), k};
+        const int ny_c = ey(q) != 0 ? 2 : 1, nz_c = ez(q) != 0 ? 2 : 1;
+        for (int a = 0; a < ny_c; ++a) {
+            const int cj = cy[a];
+            if (cj < 0 || cj >= d.ny) continue;
+            for (int b = 0; b < nz_c; ++b) {
+                const int ck = cz[b];
+                if (ck < 0 || ck >= d.nz) continue;
+                int si, sj, sk;
+                src_of(ci, cj, ck, q, si, sj, sk);
+                if (si != i || sj != j || sk != k) continue;
+                const size_t c = d.idx(ci, cj, ck);
+                const uint8_t fc = flags[c];
+                if (fc != SOLID && (fc == GAS || fm != SOLID)) acc += lam_out[(size_t)q * N + c];
+            }
+        }
+        /* bounce-back at m: output opp(q) of m reads input q of m */
+        const int o = opp(q);
+        if (fm != SOLID && fm != GAS) {
+            int si, sj, sk;
+            src_of(i, j, k, o, si, sj, sk);
+            if (flags[d.idx(si, sj, sk)] == SOLID) acc += lam_out[(size_t)o * N + m];
+        }
+        lam_in[(size_t)q * N + m] = acc;
+    }
+}
+
 extern "C" {
 
 int waam_adj_arc_sums(const uint8_t *flags, const float *f_l, const waam_adj_consts *ac,
@@ -512,6 +563,18 @@ int waam_adj_collide_srt(const float *f_src, const float *Fx, const float *Fy, c
                        as_stream(stream), f_src, Fx, Fy, Fz, f_l, flags, lam, omega, C_darcy,
                        lam_f, lam_F, (size_t)n);
     return check_launch("waam_adj_collide_srt");
+}
+
+int waam_adj_stream(const double *lam_out, double *lam_in, const uint8_t *flags, int nx, int ny,
+                    int nz, int layout, void *stream)
+{
+    WAAM_RETURN_IF(check_dims("waam_adj_stream", nx, ny, nz, layout));
+    WAAM_RETURN_IF(check_ptrs("waam_adj_stream", {lam_out, lam_in, flags}));
+    if (lam_out == lam_in) return set_error(WAAM_ERR_INVALID, "waam_adj_stream: in place");
+    Dims d{nx, ny, nz, layout};
+    hipLaunchKernelGGL(k_adj_stream, grid3(d), block3(), 0, as_stream(stream), lam_out, lam_in,
+                       flags, d);
+    return check_launch("waam_adj_stream");
 }
 
 }  // extern "C"
