@@ -13,7 +13,9 @@ product, the host logic around those kernels:
 * reset — reference twin.py:486-531; telemetry — twin.py:706-744.
 
 ``mode="jacobi"`` selects the GPU semantics for the two order-dependent in-place kernels
-(thermal advection, contact-angle BC); ``mode="serial"`` selects in-place lexicographic order.
+(thermal advection, contact-angle BC); ``mode="serial"`` selects in-place lexicographic order
+for both, ``"serial_thermal"`` / ``"serial_contact"`` for one of them (the deviation of each
+from the Jacobi form can then be measured on its own).
 """
 
 from __future__ import annotations
@@ -901,7 +903,7 @@ class OracleTwin:
                                             ptr(g.dgamma_lu_field), ptr(g.tau_field),
                                             ptr(knots), ptr(self.table_counts),
                                             ptr(consts), 1, ptr(g.flags), *d)
-            if self.mode == "serial":
+            if self.mode in ("serial", "serial_thermal"):
                 L.or_advect_diffuse_temperature_variable_serial(
                     ptr(g.H), ptr(g.T), ptr(g.ux), ptr(g.uy), ptr(g.uz), ptr(g.flags),
                     ptr(g.alpha_lu_field), ptr(g.cp_rho_field), 1.0, *d)
@@ -910,7 +912,7 @@ class OracleTwin:
                 L.or_advect_diffuse_temperature_variable_jacobi(
                     ptr(H_in), ptr(g.H), ptr(g.T), ptr(g.ux), ptr(g.uy), ptr(g.uz),
                     ptr(g.flags), ptr(g.alpha_lu_field), ptr(g.cp_rho_field), 1.0, *d)
-        elif self.mode == "serial":
+        elif self.mode in ("serial", "serial_thermal"):
             L.or_advect_diffuse_temperature_serial(ptr(g.H), ptr(g.T), ptr(g.ux), ptr(g.uy),
                                                    ptr(g.uz), ptr(g.flags), self.alpha_lu, 1.0,
                                                    self.cp_rho, *d)
@@ -943,7 +945,7 @@ class OracleTwin:
             L.or_reinitialize_phi(ptr(g.phi), ptr(g.flags), *d)
             if self.enable_wetting:
                 ct, st = cos_sin_f32(self.theta_rad)
-                if self.mode == "serial":
+                if self.mode in ("serial", "serial_contact"):
                     L.or_apply_contact_angle_phi_bc_serial(ptr(g.phi), ptr(g.flags), ct, st, *d)
                 else:
                     p_in = g.phi.copy()

@@ -83,13 +83,14 @@ def torch_x(twin_dx: float, nx: int, step: int, dt: float) -> float:
 
 
 def build_pair(name: str, mode: str = "jacobi", engine: str = "fused", layout: int | None = None,
-               population_storage: str = "fp32"):
-    """(product twin, oracle twin), both reset and seeded identically."""
+               population_storage: str = "fp32", **overrides):
+    """(product twin, oracle twin), both reset and seeded identically; ``overrides`` replace
+    settings of the named case (e.g. the grid size)."""
     from oracle.oracle import Material, OracleTwin
     from waam_solver_amd.materials import material_from_dict
     from waam_solver_amd.twin import WAAMTwin
 
-    kw = case_kwargs(name)
+    kw = dict(case_kwargs(name), **overrides)
     mat_d = dict(LCS_BEAD_ON_PLATE)
     tables = ER70S6_V1_TABLES if name.startswith("tables") else {}
     prod_mat = material_from_dict({"name": mat_d["name"], "status": "synthetic",

@@ -188,3 +188,16 @@ def test_marching_tetrahedra_level_on_grid_points():
     # the triangles tile the 6 x 5 plate top once, all facing +z (out of the metal)
     assert area.sum() == pytest.approx(30.0)
     assert np.all(np.cross(p1 - p0, p2 - p0)[:, 2] > 0)
+
+
+def test_sequence_schedule():
+    """Frames after step 0, every multiple of every_n and the last step; at the frame cap the
+    next due frame ends the run right after its step (reference export/bundle.py:117-170)."""
+    from waam_solver_amd.export.bundle import _schedule
+
+    assert _schedule(5, 2, 200) == (5, [0, 2, 4])
+    assert _schedule(6, 2, 200) == (6, [0, 2, 4, 5])
+    assert _schedule(10, 3, 2) == (7, [0, 3])
+    assert _schedule(10, 3, 0) == (1, [])
+    assert _schedule(1, 100, 200) == (1, [0])
+    assert _schedule(0, 1, 5) == (0, [])
