@@ -1,21 +1,25 @@
-"""Coupled-step throughput on one MI355X: the 256x128x64 bead-on-plate workload.
+"""Coupled-step throughput on one MI355X: the 768x384x192 bead-on-plate step with 16-bit storage.
 
-Workload (the reference metric's configuration): low-carbon-steel plate 256x128x64 cells at
-dx = 0.2 mm (dt = 20 us), bottom 24 layers solid at 300 K with seeded +/-0.5 K noise, argon
-above; Gaussian2D arc (eta 0.8, 4 kW, sigma 2 mm) moving +x at 8 mm/s with 6.5 m/min wire feed;
-flow-tier forces, convective and radiative losses, two-rate central-moment (MRT) collision, fp32.
+Workload (the largest single-GPU configuration of the metric): low-carbon-steel plate
+768x384x192 cells (56.6 M) at dx = 0.2 mm (dt = 20 us), bottom 24 layers solid at 300 K with
+seeded +/-0.5 K noise, argon above; Gaussian2D arc (eta 0.8, 4 kW, sigma 2 mm) moving +x at
+8 mm/s with 6.5 m/min wire feed; flow-tier forces, convective and radiative losses, two-rate
+central-moment (MRT) collision; fp32 arithmetic everywhere, populations stored as f - w_q in
+binary16 (the value) and, in the same line, in fp32 and bfloat16 from the same seeded state.
 
 One step = one full coupled timestep (reference solvers/coupled_step.py:73-464). The JSON line
-reports MLUPS = nx*ny*nz*steps/s/1e6 over the timed steps (state resident in HBM), the roofline
-of the dominant kernel measured with HIP events on its stream, and a CPU baseline (the OpenMP
-build of the oracle, timed on a bounded sample of the same workload on this host).
+reports MLUPS = nx*ny*nz*steps/s/1e6 over the timed steps (state resident in HBM), the fp32
+run's MLUPS and the speed-up of the 16-bit forms, the roofline of the dominant kernel measured
+with HIP events on its stream, the whole step against its compulsory bytes by cell class, and a
+CPU baseline (the OpenMP build of the oracle and one thread of it, timed on a bounded sample of
+the same workload on this host). ``--workload bead_on_plate`` runs the same physics on
+256x128x64 cells.
 """
 
 from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -24,6 +28,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# The bead-on-plate physics on 256 x 128 x 64 cells (51.2 x 25.6 x 12.8 mm at 0.2 mm).
 WORKLOAD = dict(nx=256, ny=128, nz=64, dx=2e-4, plate_thickness_mm=4.8, arc_power_W=4000.0,
                 arc_efficiency=0.8, arc_sigma_mm=2.0, travel_speed_m_s=8e-3, x0_mm=10.0,
                 wire_feed_m_min=6.5, wire_diameter_mm=1.2, droplet_length_mm=3.0,
@@ -86,10 +91,29 @@ def build_gpu_twin(w: dict, pop: str = "fp32"):
     return tw
 
 
-def cpu_baseline(w: dict, seconds: float) -> dict:
-    """OpenMP oracle on this host over a bounded number of steps of the same workload."""
-    import numpy as np
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
+
+def _omp_threads(n: int | None) -> None:
+    """Set the OpenMP thread count of the loaded oracle (None: leave it)."""
+    if n is None:
+        return
+    import ctypes
+
+    ctypes.CDLL("libgomp.so.1").omp_set_num_threads(int(n))
+
+
+def cpu_baseline(w: dict, seconds: float, one_thread_steps: int) -> dict:
+    """The OpenMP oracle on this host over a bounded number of steps of the same workload, then
+    one thread of it over `one_thread_steps` further steps."""
     from oracle.oracle import Material, OracleTwin
     from waam_solver_amd.materials import load_material
 
@@ -102,6 +126,8 @@ def cpu_baseline(w: dict, seconds: float) -> dict:
     ot.reset()
     seed_noise(ot.g.T, ot.g.H, ot.g.flags, ot.cp_rho)
     ncell = w["nx"] * w["ny"] * w["nz"]
+    threads = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+    _omp_threads(threads)
     x, y = torch_xy(w, 0, ot.g.dt)
     ot.step(x, y, True)  # first step outside the timing (library load, page faults)
     steps, t0 = 0, time.perf_counter()
@@ -112,11 +138,24 @@ def cpu_baseline(w: dict, seconds: float) -> dict:
         el = time.perf_counter() - t0
         if (el >= seconds and steps >= 2) or steps >= 200:
             break
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": round(ncell * steps / el / 1e6, 3), "unit": "MLUPS", "cores": threads,
-            "kind": "port",
-            "sample": f"{steps} coupled steps of the same {w['nx']}x{w['ny']}x{w['nz']} workload "
-                      f"(steps 2..{steps + 1}) on the OpenMP build of the oracle, {el:.1f} s"}
+    out = {"value": round(ncell * steps / el / 1e6, 3), "unit": "MLUPS", "cores": threads,
+           "kind": "port", "cpu": cpu_model(),
+           "sample": f"{steps} coupled steps of the same {w['nx']}x{w['ny']}x{w['nz']} workload "
+                     f"(steps 2..{steps + 1}) on the OpenMP build of the oracle with {threads} "
+                     f"threads, {el:.1f} s"}
+    if one_thread_steps > 0:
+        _omp_threads(1)
+        t1 = time.perf_counter()
+        for s in range(one_thread_steps):
+            x, y = torch_xy(w, steps + 2 + s, ot.g.dt)
+            ot.step(x, y, True)
+        e1 = time.perf_counter() - t1
+        _omp_threads(threads)
+        out["one_thread"] = {"value": round(ncell * one_thread_steps / e1 / 1e6, 3),
+                             "unit": "MLUPS", "cores": 1,
+                             "sample": f"{one_thread_steps} further steps on one thread, "
+                                       f"{e1:.1f} s"}
+    return out
 
 
 # The dominant kernel is k_step_pull, the deferred pull stream of the populations (a launch of
@@ -128,11 +167,21 @@ def cpu_baseline(w: dict, seconds: float) -> dict:
 # stream is being completed.
 STREAM_B_EVERY = 2
 STREAM_B_PULL = {"fp32": 152, "fp16": 76, "bf16": 76}
-# Whole-step compulsory bytes per cell with the HAZ fields: populations 2 x 19 x 4 B (fp32) or
-# 2 x 19 x 2 B (16-bit storage), H T f_l phi r/w 32 B, u r/w 24 B, flags 2 B, rho 4 B, HAZ 44 B
-STEP_B_PER_CELL = {"fp32": 258, "fp16": 182, "bf16": 182}
+# Whole-step compulsory bytes by cell class (each persistent field read and written once):
+#   every cell          H T f_l phi r/w 32 B, u r/w 24 B, flags r/w 2 B, rho w 4 B    62 B
+#   non-SOLID cell      populations read + written: 2 x 19 x 4 B (fp32), 2 x 19 x 2 B (16-bit)
+#   non-GAS cell        HAZ fields: T_max r/w, T_prev r/w, dT_dt w, three time-above r/w  44 B
+# SOLID cells neither collide nor stream; GAS cells skip every HAZ update (reference
+# kernels.py:2971-3010, 3016-3032, 2013-2027, 3261-3283).
+STEP_B_EVERY = 62
+STEP_B_POP = {"fp32": 152, "fp16": 76, "bf16": 76}
+STEP_B_HAZ = 44
 DOMINANT = "k_step_pull"
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_step_pull_bead_on_plate_256x128x64_pmc.md")
+
+
+def traffic_file(w: dict, pop: str) -> str:
+    return os.path.join(ROOT, "profiles",
+                        f"k_step_pull_bead_on_plate_{w['nx']}x{w['ny']}x{w['nz']}_{pop}_pmc.md")
 
 
 def stream_classes(g) -> tuple[int, int]:
@@ -144,8 +193,23 @@ def stream_classes(g) -> tuple[int, int]:
     return int(active.sum()), int(pull.sum())
 
 
+def cell_classes(g) -> tuple[int, int]:
+    """(SOLID cells, GAS cells) by the grid's current flags."""
+    import torch
+
+    from waam_solver_amd.field import tensor_of
+
+    fl = tensor_of(g.flags)
+    return int((fl == 1).sum().item()), int((fl == 2).sum().item())
+
+
 def stream_bytes(ncell: int, pull: float, pop: str = "fp32") -> float:
     return STREAM_B_EVERY * ncell + STREAM_B_PULL[pop] * pull
+
+
+def step_bytes(ncell: int, solid: float, gas: float, pop: str) -> float:
+    return (STEP_B_EVERY * ncell + STEP_B_POP[pop] * (ncell - solid)
+            + STEP_B_HAZ * (ncell - gas))
 
 
 def rel_l2_device(a, b) -> float:
@@ -154,50 +218,72 @@ def rel_l2_device(a, b) -> float:
     return float(((a - b).norm() / b.norm().clamp_min(1e-300)).item())
 
 
-def precision_form(w: dict, args, ref_tw, ref_mlups: float, pop: str) -> dict:
-    """The same workload, warm-up and timed steps with 16-bit population storage: MLUPS, speedup
-    over the fp32 run and the relative L2 of T, phi, f_l and |u| against the fp32 run's final
-    state (both runs start from the same seeded state). pop "fp32_ulp" is the fp32 run with half
-    of its initial populations moved by one ulp: the divergence any perturbation of this flow
-    reaches over the same steps, the floor against which the 16-bit errors read."""
+def timed_run(w: dict, args, pop: str):
+    """Build the seeded twin with population storage `pop`, step the warm-up, then time
+    args.steps steps. Returns (twin, seconds, host seconds, (solid, gas) averaged over the
+    start and end of the timed steps, (collided, pulled) likewise)."""
     import torch
 
-    from waam_solver_amd.field import note_write, tensor_of
-
-    tw = build_gpu_twin(w, "fp32" if pop == "fp32_ulp" else pop)
+    tw = build_gpu_twin(w, pop)
+    tw.engine = args.engine
     g = tw.grid
-    if pop == "fp32_ulp":
-        f = tensor_of(g.f_src)
-        gen = torch.Generator(device=f.device).manual_seed(5)
-        m = torch.rand(f.shape, generator=gen, device=f.device) < 0.5
-        f[m] = torch.nextafter(f[m], torch.full_like(f[m], 2.0))
-        note_write()
     for s in range(args.warmup):
         tw.step(*torch_xy(w, s, g.dt), True)
     torch.cuda.synchronize()
+    fused = args.engine == "fused"
+    cls0 = cell_classes(g)
+    sc0 = stream_classes(g) if fused and hasattr(g, "sflags_tmp") else None
+    if fused:
+        from waam_solver_amd._lib import STEP_KERNELS, lib
+
+        lib().call("waam_timing_begin", STEP_KERNELS.index(DOMINANT), args.steps)
     t0 = time.perf_counter()
     for s in range(args.warmup, args.warmup + args.steps):
         tw.step(*torch_xy(w, s, g.dt), True)
+    t_host = time.perf_counter() - t0  # launches are asynchronous: the host's own cost per step
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    timing = None
+    if fused:
+        import ctypes
+
+        tot_ms, n_launch = ctypes.c_double(), ctypes.c_int64()
+        lib().call("waam_timing_end", ctypes.addressof(tot_ms), ctypes.addressof(n_launch))
+        timing = (tot_ms.value, n_launch.value)
+    cls1 = cell_classes(g)
+    classes = ((cls0[0] + cls1[0]) / 2.0, (cls0[1] + cls1[1]) / 2.0)
+    sc = None
+    if fused:
+        sc1 = stream_classes(g)
+        sc0 = sc0 or sc1
+        sc = ((sc0[0] + sc1[0]) / 2.0, (sc0[1] + sc1[1]) / 2.0)
+    return tw, el, t_host, classes, sc, timing
+
+
+def compare_form(w: dict, args, pop: str, ref):
+    """The same workload, warm-up and timed steps with population storage `pop`: (twin, MLUPS,
+    summary), the summary holding the relative L2 of T, phi, f_l and |u| between this run's final
+    state and that of the run `ref` (both start from the same seeded state)."""
+    import torch
+
+    from waam_solver_amd.field import tensor_of
+
+    tw, el, _, classes, _, _ = timed_run(w, args, pop)
+    g = tw.grid
     mlups = g.n_cells * args.steps / el / 1e6
-    r = ref_tw.grid
+    r = ref.grid
     err = {k: round(rel_l2_device(tensor_of(getattr(g, k)), tensor_of(getattr(r, k))), 8)
            for k in ("T", "phi", "f_l")}
     speed = lambda gg: torch.sqrt(sum(tensor_of(getattr(gg, c)).double() ** 2  # noqa: E731
                                       for c in ("ux", "uy", "uz")))
     err["|u|"] = round(rel_l2_device(speed(g), speed(r)), 8)
     flips = int((tensor_of(g.flags) != tensor_of(r.flags)).sum().item())
-    out = {"value": round(mlups, 2), "speedup_vs_fp32": round(mlups / ref_mlups, 3),
-           "ms_per_step": round(el * 1e3 / args.steps, 4),
-           "rel_l2_vs_fp32": err, "flags_differing": flips}
-    if pop != "fp32_ulp":
-        out["step_bytes_per_cell"] = STEP_B_PER_CELL[pop]
-        out["step_frac"] = round(STEP_B_PER_CELL[pop] * g.n_cells * args.steps / el / 1e9
-                                 / HBM_PEAK_GBS, 4)
-    del tw, g
-    torch.cuda.empty_cache()
-    return out
+    sb = step_bytes(g.n_cells, classes[0], classes[1], pop)
+    out = {"value": round(mlups, 2), "ms_per_step": round(el * 1e3 / args.steps, 4),
+           "rel_l2_vs_value_run": err, "flags_differing": flips,
+           "step_bytes_per_cell": round(sb / g.n_cells, 1),
+           "step_frac": round(sb * args.steps / el / 1e9 / HBM_PEAK_GBS, 4)}
+    return tw, mlups, out
 
 
 def kernel_breakdown(tw, w: dict, start: int, steps: int) -> dict:
@@ -224,14 +310,15 @@ def kernel_breakdown(tw, w: dict, start: int, steps: int) -> dict:
 
 
 def captured_form(tw, w: dict, start: int, steps: int) -> dict:
-    """The same step count through WAAMTwin.step_many: runs of steady steps go to the GPU as
-    captured graphs (droplet steps and the step after each go directly). Graphs are captured in
-    an untimed first pass."""
+    """The same step count, but at least four graphs' worth, through WAAMTwin.step_many: runs of
+    steady steps go to the GPU as captured graphs (droplet steps and the step after each go
+    directly). Graphs are captured in an untimed first pass."""
     import torch
 
     g = tw.grid
     st = tw._fused
     warm = 4 * st.graph_steps
+    steps = max(steps, 4 * st.graph_steps)
     pos = [torch_xy(w, s, g.dt) for s in range(start, start + warm + steps)]
     tw.step_many([p[0] for p in pos[:warm]], [p[1] for p in pos[:warm]], True)
     torch.cuda.synchronize()
@@ -320,40 +407,43 @@ def multipass(args) -> int:
     return 0
 
 
-def measured_traffic() -> float | None:
-    """HBM bytes per k_step_pull launch from the committed rocprofv3 counter summary."""
+def measured_traffic(w: dict, pop: str) -> dict | None:
+    """HBM bytes per k_step_pull launch and per step from the committed rocprofv3 counter
+    summary of this workload and population form."""
+    out = {}
     try:
-        with open(TRAFFIC_FILE) as fh:
+        with open(traffic_file(w, pop)) as fh:
             for line in fh:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return float(line.split(":", 1)[1])
+                for key in ("traffic_bytes_per_launch", "traffic_bytes_per_step"):
+                    if line.startswith(key + ":"):
+                        out[key] = float(line.split(":", 1)[1])
     except OSError:
         return None
-    return None
+    return out or None
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=tuple(WORKLOADS), default="bead_on_plate",
-                    help="bead_on_plate: 256x128x64 (the metric's configuration); "
-                         "large: 768x384x192; multipass: three passes, 20 000 steps on "
-                         "768x384x192 at dx=1mm")
+    ap.add_argument("--workload", choices=tuple(WORKLOADS), default="large",
+                    help="large: 768x384x192 (default); bead_on_plate: 256x128x64; multipass: "
+                         "three passes, 20 000 steps on 768x384x192 at dx=1mm")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-one-thread-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--engine", choices=("fused", "reference"), default="fused")
-    ap.add_argument("--breakdown", type=int, default=50,
+    ap.add_argument("--breakdown", type=int, default=20,
                     help="steps per kernel for the per-kernel HIP-event breakdown (0: skip)")
-    ap.add_argument("--pop", choices=("fp32", "fp16", "bf16"), default="fp32",
+    ap.add_argument("--pop", choices=("fp32", "fp16", "bf16"), default="fp16",
                     help="population storage of the measured run (16-bit forms store f - w_q)")
     ap.add_argument("--multipass-fp32", action="store_true",
                     help="the multipass workload with fp32 populations (default: fp16)")
     ap.add_argument("--no-graph", action="store_true",
                     help="skip the captured-graph run reported beside the value")
     ap.add_argument("--no-precision", action="store_true",
-                    help="skip the 16-bit storage runs that an fp32 run reports beside its value")
+                    help="skip the runs in the other population forms reported beside the value")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (each box here has "
@@ -363,60 +453,59 @@ def main() -> int:
 
     import torch
 
-    import ctypes
-
-    from waam_solver_amd._lib import STEP_KERNELS, lib
+    from waam_solver_amd._lib import STEP_KERNELS
 
     if args.workload == "multipass":
         return multipass(args)
     w = dict(WORKLOADS[args.workload])
     if args.steps is None:
         args.steps = w["steps"]
-    tw = build_gpu_twin(w, args.pop)
-    tw.engine = args.engine
+    fused = args.engine == "fused"
+    tw, el, t_host, classes, sc, timing = timed_run(w, args, args.pop)
     g = tw.grid
     ncell = g.nx * g.ny * g.nz
-    for s in range(args.warmup):
-        tw.step(*torch_xy(w, s, g.dt), True)
-    torch.cuda.synchronize()
-    fused = args.engine == "fused"
-    if fused:
-        cls0 = stream_classes(g) if hasattr(g, "sflags_tmp") else None
-        lib().call("waam_timing_begin", STEP_KERNELS.index(DOMINANT), args.steps)
-    t0 = time.perf_counter()
-    for s in range(args.warmup, args.warmup + args.steps):
-        tw.step(*torch_xy(w, s, g.dt), True)
-    t_host = time.perf_counter() - t0  # launches are asynchronous: the host's own cost per step
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
+    mlups = ncell * args.steps / el / 1e6
     roofline = None
     if fused:
-        tot_ms, n_launch = ctypes.c_double(), ctypes.c_int64()
-        lib().call("waam_timing_end", ctypes.addressof(tot_ms), ctypes.addressof(n_launch))
-        cls1 = stream_classes(g)
-        cls0 = cls0 or cls1
-        active, pull = (cls0[0] + cls1[0]) / 2.0, (cls0[1] + cls1[1]) / 2.0
-        per_launch = stream_bytes(ncell, pull, args.pop)
-        avg_s = tot_ms.value / 1e3 / max(n_launch.value, 1)
+        tot_ms, n_launch = timing
+        per_launch = stream_bytes(ncell, sc[1], args.pop)
+        avg_s = tot_ms / 1e3 / max(n_launch, 1)
         achieved = per_launch / avg_s / 1e9
+        meas = measured_traffic(w, args.pop)
+        sb = step_bytes(ncell, classes[0], classes[1], args.pop)
         roofline = {"bound": "hbm", "kernel": DOMINANT, "achieved": round(achieved, 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": (measured_traffic() if args.workload == "bead_on_plate"
-                                and args.pop == "fp32" else None),
+                    "traffic": meas.get("traffic_bytes_per_launch") if meas else None,
                     "algorithmic_bytes_per_launch": round(per_launch),
-                    "avg_launch_us": round(avg_s * 1e6, 2), "timed_launches": n_launch.value,
-                    "cells_collided": round(active), "cells_pulled": round(pull),
-                    "step_bytes_per_cell": STEP_B_PER_CELL[args.pop],
-                    "step_achieved": round(STEP_B_PER_CELL[args.pop] * ncell * args.steps / el
-                                           / 1e9, 1)}
-        roofline["step_frac"] = round(roofline["step_achieved"] / HBM_PEAK_GBS, 4)
-    mlups = ncell * args.steps / el / 1e6
-    precision = None
-    if args.pop == "fp32" and fused and not args.no_precision:
-        # the precision form of the metric: 16-bit shifted population storage, same steps from
-        # the same seeded state, against this run's state at the end of its timed steps
-        precision = {pop: precision_form(w, args, tw, mlups, pop)
-                     for pop in ("fp16", "bf16", "fp32_ulp")}
+                    "avg_launch_us": round(avg_s * 1e6, 2), "timed_launches": n_launch,
+                    "cells_collided": round(sc[0]), "cells_pulled": round(sc[1]),
+                    "step": {"compulsory_bytes_per_cell": round(sb / ncell, 1),
+                             "cells_solid": round(classes[0]), "cells_gas": round(classes[1]),
+                             "achieved": round(sb * args.steps / el / 1e9, 1),
+                             "frac": round(sb * args.steps / el / 1e9 / HBM_PEAK_GBS, 4),
+                             "measured_bytes_per_step": (meas.get("traffic_bytes_per_step")
+                                                         if meas else None)}}
+    forms = None
+    if fused and not args.no_precision:
+        # the other population forms from the same seeded state over the same steps: fp32 (the
+        # plain method) first, then the remaining 16-bit form
+        forms = {}
+        ref32, m32, forms["fp32"] = compare_form(w, args, "fp32", tw) if args.pop != "fp32" \
+            else (tw, mlups, None)
+        if forms["fp32"] is None:
+            del forms["fp32"]
+        for pop in ("fp16", "bf16"):
+            if pop == args.pop:
+                continue
+            other, m, out = compare_form(w, args, pop, ref32)
+            out["speedup_vs_fp32"] = round(m / m32, 3)
+            forms[pop] = out
+            del other
+            torch.cuda.empty_cache()
+        forms["value_speedup_vs_fp32"] = round(mlups / m32, 3)
+        if ref32 is not tw:
+            del ref32
+        torch.cuda.empty_cache()
     graph = None
     if fused:
         after = args.warmup + args.steps
@@ -427,7 +516,8 @@ def main() -> int:
             graph = captured_form(tw, w, after, args.steps)
     tel = tw.get_telemetry()
     result = {
-        "metric": f"coupled-step MLUPS (bead-on-plate {g.nx}x{g.ny}x{g.nz}, {args.pop})",
+        "metric": f"coupled-step MLUPS (bead-on-plate {g.nx}x{g.ny}x{g.nz}, {args.pop} "
+                  "population storage)",
         "value": round(mlups, 2),
         "unit": "MLUPS",
         "n_gpus": 1,
@@ -438,10 +528,11 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.pop == "fp32" else f"f32 arithmetic, {args.pop} populations",
+        "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"bead-on-plate {g.nx}x{g.ny}x{g.nz} dx=0.2mm, Gaussian2D 4kW "
-                               "eta0.8 sigma2mm +x 8mm/s, wire feed 6.5m/min, flow tier, MRT",
+                               "eta0.8 sigma2mm +x 8mm/s, wire feed 6.5m/min, flow tier, MRT, "
+                               f"{args.pop} populations (f - w_q), fp32 arithmetic",
                    "grid": [g.nx, g.ny, g.nz], "cells": ncell, "collision": "mrt",
                    "engine": args.engine, "population_storage": args.pop,
                    "pool_width_mm": tel["pool_width_mm"], "pool_depth_mm": tel["pool_depth_mm"],
@@ -449,12 +540,14 @@ def main() -> int:
     }
     if roofline is not None:
         result["roofline"] = roofline
-    if precision is not None:
-        result["precision"] = precision
+    if forms is not None:
+        result["forms"] = forms
     if graph is not None:
         result["graph"] = graph
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
+        del tw, g
+        torch.cuda.empty_cache()
+        result["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds, args.cpu_one_thread_steps)
     print(json.dumps(result))
     return 0
 

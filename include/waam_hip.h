@@ -35,14 +35,15 @@
 extern "C" {
 #endif
 
-#define WAAM_ABI_VERSION 3
+#define WAAM_ABI_VERSION 4
 
 enum { WAAM_FLAG_FLUID = 0, WAAM_FLAG_SOLID = 1, WAAM_FLAG_GAS = 2, WAAM_FLAG_IFACE = 4 };
 /* the moving lid of the lid-driven cavity (waam_cavity_step); not a flag of the coupled step */
 enum { WAAM_CAVITY_LID = 3 };
 enum { WAAM_LAYOUT_XYZ = 0, WAAM_LAYOUT_ZYX = 1 };
-/* Population storage of the fused step: fp32 populations, or f - w_q in binary16 / bfloat16
- * (round to nearest even on store, fp32 arithmetic in between). */
+/* Population storage of the fused step: fp32 populations, (f - w_q) * 2^15 in binary16
+ * (saturating at +-65504), or f - w_q in bfloat16 (round to nearest even on store, fp32
+ * arithmetic in between). */
 enum { WAAM_POP_F32 = 0, WAAM_POP_F16 = 1, WAAM_POP_BF16 = 2 };
 enum { WAAM_OK = 0, WAAM_ERR_INVALID = -1, WAAM_ERR_HIP = -2 };
 
@@ -444,6 +445,45 @@ int waam_adj_collide_srt(const float *f_src, const float *Fx, const float *Fy, c
 int waam_adj_stream(const double *lam_out, double *lam_in, const uint8_t *flags, int nx, int ny,
                     int nz, int layout, void *stream);
 
+/* ---- adjoint of the coupled step with frozen flags and phi (binary64 adjoint state) --------
+ * The step's thermal chain with enthalpy advection (kernels.py:1611-1682 and the chain of
+ * waam_adj_thermal_step), the body forces the temperature and liquid fraction drive
+ * (Marangoni kernels.py:2158-2217, hydrostatic and buoyancy kernels.py:2232-2268, the force
+ * clamp), the MRT collision (cumulant_kernel.py:59-209) and the Mach clamp, as the coupled step
+ * (solvers/coupled_step.py:97-409) runs them with the flags and phi held; with frozen flags the
+ * stream never feeds a collided cell. */
+/* collision: lam_f (19 q-major), lam_u (3 arrays, after the Mach clamp at u_cap) -> lam_fin
+ * (19), lam_F (3, the force the collision saw), lam_fl += (the drag's f_l); FLUID/IFACE cells */
+int waam_cadj_collide_mrt(const float *f_src, const float *Fx, const float *Fy, const float *Fz,
+                          const float *f_l, const uint8_t *flags, const double *lam_f,
+                          const double *lam_u, double omega_s, double omega_b, double C_darcy,
+                          double u_cap, double *lam_fin, double *lam_F, double *lam_fl, int64_t n,
+                          void *stream);
+/* force clamp at the raw force Fr (F_cap), hydrostatic and buoyancy terms: lam_F -> lam_Fraw
+ * (3 arrays); lam_T += and lam_fl += */
+int waam_cadj_forces(const float *Frx, const float *Fry, const float *Frz, const float *T,
+                     const float *f_l, const uint8_t *flags, const double *lam_F, double F_cap,
+                     double g_lu, double beta, double T_l, int enable_hydro, double *lam_Fraw,
+                     double *lam_T, double *lam_fl, int64_t n, void *stream);
+/* Marangoni force with binary64 lam (3 arrays): T_adj += dL/dT; G is 3n doubles of workspace */
+int waam_cadj_marangoni(const float *phi, const float *f_l, const uint8_t *flags,
+                        const double *lam, double dgamma, double *G, double *T_adj, int nx, int ny,
+                        int nz, int layout, void *stream);
+/* phase map at H: lam_H += lam_T dT/dH + lam_fl df_l/dH (0 on GAS) */
+int waam_cadj_phase(const float *H, const uint8_t *flags, const waam_adj_consts *ac,
+                    const double *lam_T, const double *lam_fl, double *lam_H, int64_t n,
+                    void *stream);
+/* thermal chain with advection: lam_next = dL/dH_{n+1} -> lam_out = dL/dH_n, lam_u (3 arrays) =
+ * dL/du_n; H_n, A (after the arc), B (after the first ceiling), E (after the losses), F (after
+ * the floor), f_l_n and u_n of the forward; grads += dL/d(eta, s_h, s_sigma); lamC and lamB are
+ * n doubles of workspace each */
+int waam_cadj_thermal_step(const double *lam_next, double *lam_out, double *lam_u,
+                           const float *H_n, const float *A, const float *B, const float *E,
+                           const float *F, const float *f_l_n, const float *ux, const float *uy,
+                           const float *uz, const uint8_t *flags, const double *arc_sums,
+                           const waam_adj_consts *ac, double *grads, double *lamC, double *lamB,
+                           int nx, int ny, int nz, int layout, void *scratch, void *stream);
+
 /* ---- export-side fields and reductions -------------------------------------------------- */
 /* compute_vorticity_magnitude kernels.py:3380-3419: |curl u| / dt on non-GAS cells, 0 on GAS */
 int waam_compute_vorticity_magnitude(const float *ux, const float *uy, const float *uz,
@@ -487,6 +527,10 @@ typedef struct waam_step_fields {
     int32_t *tracer_active;
     /* two int32 work lists of nx*ny*nz entries each (compacted FLUID/IFACE cell indices) */
     int32_t *list_thermal, *list_collide;
+    /* waam_quiet_bytes(nx, ny, nz, layout) bytes: per row segment (the 64 cells of one
+     * wavefront along the contiguous axis) whether it ended the last two steps quiet, and
+     * whether the current step skips it; written by every step, contents need no initial value */
+    uint8_t *quiet;
     void *scratch;
     int32_t max_tracers, nx, ny, nz;
     int32_t layout; /* WAAM_LAYOUT_XYZ or WAAM_LAYOUT_ZYX, shared by every field */
@@ -514,6 +558,15 @@ typedef struct waam_step_params {
     /* full-tier surface forces (physics/weld_forces.py:120-161): vapour recoil (clausius: the
      * Clausius-Clapeyron form, else the constant-pressure form) and shielding-gas shear */
     int32_t enable_recoil, recoil_clausius, enable_gas_shear;
+    /* 1: skip the cell sweeps of kernels 2, 4 and 7 on row segments whose results cannot change
+     * (every cell the segment's stencils reach ended the previous step GAS with phi exactly 0,
+     * and the segment itself ended the step before quiet too): their fields already hold those
+     * results in both buffers of every pair. Valid only when the previous two steps were
+     * waam_step_fused calls on this field set with nothing written in between; honoured only on
+     * steady steps (pending_stream 1, thermal_list_ready 1 or 2, arc_applied 0,
+     * forces_from_fields 0, force_reset_all 0). 0: every segment is swept and the quiet record
+     * restarts. The results are the same either way. */
+    int32_t quiet_skip;
     /* arc */
     float arc_i, arc_j, arc_k_override, k_robot;
     float Q_w, eta, dt, dx, dx3, sigma, pen;
@@ -531,6 +584,8 @@ typedef struct waam_step_params {
 } waam_step_params;
 
 int waam_step_fused(const waam_step_params *p, const waam_step_fields *f, void *stream);
+/* bytes of the quiet record (waam_step_fields.quiet) for a grid; 0 for invalid sizes */
+int64_t waam_quiet_bytes(int nx, int ny, int nz, int layout);
 /* Materialise a deferred stream: f_dst = pull(f_src) on cells not SOLID in sflags (the flags the
  * step that deferred it saw); populations in the WAAM_POP_* form pop_dtype. */
 int waam_stream_pending(const void *f_src, void *f_dst, const uint8_t *sflags, int nx, int ny,

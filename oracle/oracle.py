@@ -1098,15 +1098,20 @@ def bf16_bits_to_float(b: np.ndarray) -> np.ndarray:
     return (np.asarray(b, np.uint16).astype(np.uint32) << 16).view(np.float32)
 
 
+F16_SCALE = np.float32(32768.0)  # binary16 storage holds (f - w_q) * 2^15
+F16_MAX = np.float32(65504.0)
+
+
 def quantize_populations(f: np.ndarray, form: str) -> None:
     """In place: each population f[q] becomes the fp32 value its 16-bit storage form holds,
-    (f - w_q) narrowed to binary16 or bfloat16 (round to nearest even) and widened back with
-    w_q added, every operation in fp32."""
+    (f - w_q) narrowed to binary16 (scaled by 2^15, saturating at +-65504) or bfloat16 (round to
+    nearest even) and widened back with w_q added, every operation in fp32."""
     for q in range(Q):
         w = np.float32(W[q])
         d = (f[q] - w).astype(np.float32)
         if form == "fp16":
-            back = d.astype(np.float16).astype(np.float32)
+            s = np.clip((d * F16_SCALE).astype(np.float32), -F16_MAX, F16_MAX)
+            back = (s.astype(np.float16).astype(np.float32) / F16_SCALE).astype(np.float32)
         elif form == "bf16":
             back = bf16_bits_to_float(float_to_bf16_bits(d))
         else:

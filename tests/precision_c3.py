@@ -1,0 +1,128 @@
+"""The precision forms at full size: the bead-on-plate on 512x256x128 cells for 2000 steps in
+fp32, binary16 and bfloat16 population storage (GPU) against the binary64 build of the same
+kernels (the oracle compiled with every fp32 variable a double, on the CPU).
+
+Two steps, each a command:
+
+  python tests/precision_c3.py make-ref DIR     binary64 oracle run (CPU, OpenMP, Jacobi mode)
+                                                 -> DIR/{T,phi,f_l,speed}.npy (stored as fp32)
+                                                 and DIR/ref.json (pool width/depth, ledger)
+  python tests/precision_c3.py measure DIR      the three GPU forms from the same seeded state
+                                                 over the same steps -> per-field relative L2
+                                                 against DIR, bead width/depth, MLUPS; one JSON
+                                                 line
+
+The workload is bench.py's bead-on-plate physics (seeded noise, torch path) on the larger grid.
+Not collected by pytest (it needs a GPU and a binary64 run of about an hour on the CPU).
+"""
+
+from __future__ import annotations
+
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+sys.path.insert(0, ROOT)
+
+STEPS = 2000
+
+
+def workload() -> dict:
+    import bench
+
+    return dict(bench.WORKLOAD, nx=512, ny=256, nz=128, x0_mm=20.0, steps=STEPS)
+
+
+def make_ref(out_dir: str) -> int:
+    import bench
+    from oracle.oracle import Material, OracleTwin
+    from waam_solver_amd.materials import load_material
+
+    w = workload()
+    os.makedirs(out_dir, exist_ok=True)
+    m = load_material("LCS-bead-on-plate")
+    mat = Material(name=m.name, k=m.k, rho=m.rho, cp=m.cp, T_solidus=m.T_solidus,
+                   T_liquidus=m.T_liquidus, L_fusion=m.L_fusion, gamma_0=m.gamma_0,
+                   dgamma_dT=m.dgamma_dT, mu=m.mu, beta_T=m.beta_T, alpha=m.alpha)
+    ot = OracleTwin(mat, parallel=True, mode="jacobi", precision="fp64", **bench.twin_kwargs(w))
+    bench.configure(ot, w)
+    ot.reset()
+    g = ot.g
+    # the seeded state in fp32, as the GPU twin holds it, then widened
+    T, H = g.T.astype(np.float32), g.H.astype(np.float32)
+    bench.seed_noise(T, H, g.flags, np.float32(ot.cp_rho))
+    g.T[...] = T
+    g.H[...] = H
+    t0 = time.perf_counter()
+    for s in range(STEPS):
+        ot.step(*bench.torch_xy(w, s, g.dt), True)
+        if (s + 1) % 100 == 0:
+            print(f"[c3-ref] step {s + 1} ({time.perf_counter() - t0:.0f} s)", flush=True)
+    speed = np.sqrt(g.ux.astype(np.float64) ** 2 + g.uy ** 2 + g.uz ** 2)
+    for name, arr in (("T", g.T), ("phi", g.phi), ("f_l", g.f_l), ("speed", speed)):
+        np.save(os.path.join(out_dir, f"{name}.npy"), np.asarray(arr, np.float32))
+    pm = ot.pool_metrics()
+    with open(os.path.join(out_dir, "ref.json"), "w") as fh:
+        json.dump({"grid": [w["nx"], w["ny"], w["nz"]], "steps": STEPS,
+                   "pool_width_m": pm["pool_width_m"], "pool_depth_m": pm["pool_depth_m"],
+                   "n_liquid_cells": pm["n_liquid_cells"], "T_peak": pm["T_peak"],
+                   "n_droplets": ot._n_droplets_fired, "seconds": time.perf_counter() - t0}, fh)
+    return 0
+
+
+def measure(ref_dir: str) -> int:
+    import torch
+
+    import bench
+    from waam_solver_amd.field import tensor_of
+
+    w = workload()
+    with open(os.path.join(ref_dir, "ref.json")) as fh:
+        ref = json.load(fh)
+    dev = torch.device("cuda:0")
+    refs = {k: torch.from_numpy(np.load(os.path.join(ref_dir, f"{k}.npy"))).to(dev).double()
+            for k in ("T", "phi", "f_l", "speed")}
+    out = {"workload": "bead-on-plate 512x256x128 dx=0.2mm, 2000 steps (bench.py physics)",
+           "fp64": {k: ref[k] for k in ("pool_width_m", "pool_depth_m", "n_liquid_cells",
+                                        "T_peak", "n_droplets")}}
+    for pop in ("fp32", "fp16", "bf16"):
+        print(f"[c3] {pop}", flush=True)
+        tw = bench.build_gpu_twin(w, pop)
+        g = tw.grid
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s in range(STEPS):
+            tw.step(*bench.torch_xy(w, s, g.dt), True)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        tw.sync()
+        u = [tensor_of(getattr(g, c)).double() for c in ("ux", "uy", "uz")]
+        got = {"T": tensor_of(g.T).double(), "phi": tensor_of(g.phi).double(),
+               "f_l": tensor_of(g.f_l).double(), "speed": torch.sqrt(sum(x * x for x in u))}
+        err = {}
+        for k, a in got.items():
+            # the GPU twin stores (nz, ny, nx); the reference arrays are (nx, ny, nz)
+            b = refs[k].permute(2, 1, 0) if a.shape != refs[k].shape else refs[k]
+            err["|u|" if k == "speed" else k] = float(((a - b).norm() / b.norm()).item())
+        tel = tw.get_telemetry()
+        out[pop] = {"rel_l2_vs_fp64": err, "pool_width_mm": tel["pool_width_mm"],
+                    "pool_depth_mm": tel["pool_depth_mm"],
+                    "n_liquid_cells": tel["n_liquid_cells"],
+                    "MLUPS": round(g.n_cells * STEPS / el / 1e6, 1)}
+        print(f"[c3] {pop}: {out[pop]}", flush=True)
+        del tw, g, got, u
+        torch.cuda.empty_cache()
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    if len(sys.argv) != 3 or sys.argv[1] not in ("make-ref", "measure"):
+        print(__doc__)
+        sys.exit(2)
+    sys.exit(make_ref(sys.argv[2]) if sys.argv[1] == "make-ref" else measure(sys.argv[2]))

@@ -50,9 +50,17 @@ def case_kwargs(name: str) -> dict:
         # moving window: 40 cells long, shifts of 8 cells when the torch passes 62 % of 32
         return dict(base, nx=40, ny=24, nz=24, dx=2e-4, droplet_freq_hz=0.0,
                     enable_moving_window=True)
+    if name == "bead_pair_mrt":
+        # a contiguous extent even but not a multiple of four (nx = 62): the 16-bit storage
+        # pulls two cells per lane
+        return dict(base, nx=62, ny=40, nz=32, dx=2e-4, droplet_freq_hz=0.0)
     if name == "bead_odd_mrt":
         # an odd contiguous extent (nx = 63): the 16-bit storage pulls cell by cell
         return dict(base, nx=63, ny=40, nz=32, dx=2e-4, droplet_freq_hz=0.0)
+    if name == "bead_box_mrt":
+        # a narrow arc (sigma 2 cells) on a grid wider than the arc's zero-heat box in x and y:
+        # the arc sums run over the box's columns and the heat is gated by the box
+        return dict(base, nx=128, ny=64, nz=32, dx=2e-4, droplet_freq_hz=0.0, arc_sigma_mm=0.4)
     if name == "bench_small":
         return dict(base, nx=128, ny=64, nz=32, dx=2e-4, droplet_freq_hz=0.0)
     raise KeyError(name)

@@ -1,0 +1,164 @@
+"""Adjoint of the coupled step with frozen flags and phi (csrc/adjoint_coupled.hip).
+
+Per operator, against central differences of the oracle's binary64 build (liboracle_f64.so):
+- the MRT collision with the Mach clamp (waam_cadj_collide_mrt): L = <f_out, lam_f> +
+  <clamp(u), lam_u>, perturbing populations, forces and liquid fractions (the drag) together;
+- the force clamp, hydrostatic gravity and buoyancy (waam_cadj_forces), perturbing the raw
+  forces, T and f_l.
+Through the rollout: the gradient of the smoothed pool width/depth loss with respect to (eta,
+heat-loss factor, sigma scale) over coupled steps on 128x64x48 cells against central
+differences of the same forward (the product's fp32 kernels, h = 1e-3 and 1e-4).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+NX, NY, NZ = 12, 10, 8
+N = NX * NY * NZ
+E = np.array([[0, 0, 0], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1],
+              [1, 1, 0], [-1, -1, 0], [1, -1, 0], [-1, 1, 0], [1, 0, 1], [-1, 0, -1],
+              [1, 0, -1], [-1, 0, 1], [0, 1, 1], [0, -1, -1], [0, 1, -1], [0, -1, 1]], float)
+W = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+OMEGA_S, OMEGA_B, C_DARCY = 1.6, 1.1, 50.0
+
+
+def _fixture(seed=31, u_scale=0.05):
+    rng = np.random.default_rng(seed)
+    u0 = rng.uniform(-u_scale, u_scale, (3, N))
+    rho = 1.0 + rng.uniform(-0.02, 0.02, N)
+    eu = E @ u0
+    f = (W[:, None] * rho * (1 + 3 * eu + 4.5 * eu ** 2 - 1.5 * (u0 ** 2).sum(0))
+         * (1 + rng.uniform(-0.05, 0.05, (19, N))))
+    F = rng.uniform(-1e-3, 1e-3, (3, N))
+    f_l = rng.uniform(0.2, 1.0, N)
+    flags = rng.choice(np.array([0, 4, 1, 2], np.uint8), N, p=[0.5, 0.2, 0.15, 0.15])
+    lam_f = rng.standard_normal((19, N))
+    lam_u = rng.standard_normal((3, N))
+    return (f.astype(np.float32), F.astype(np.float32), f_l.astype(np.float32), flags, lam_f,
+            lam_u)
+
+
+def _oracle_collide_L(f, F, f_l, flags, lam_f, lam_u, u_cap):
+    from oracle.oracle import lib as olib, ptr
+
+    L = olib(f64=True)
+    fs = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
+    fd = np.zeros_like(fs)
+    out = [np.zeros(N) for _ in range(4)]
+    Fs = [np.ascontiguousarray(F[a], dtype=np.float64) for a in range(3)]
+    fl = np.ascontiguousarray(f_l, dtype=np.float64)
+    L.or_collide_mrt(ptr(fs), ptr(fd), ptr(out[0]), ptr(out[1]), ptr(out[2]), ptr(out[3]),
+                     ptr(Fs[0]), ptr(Fs[1]), ptr(Fs[2]), ptr(fl), ptr(flags), OMEGA_S, OMEGA_B,
+                     C_DARCY, NX, NY, NZ)
+    L.or_clamp_velocity_mach(ptr(out[1]), ptr(out[2]), ptr(out[3]), ptr(flags), u_cap, NX, NY, NZ)
+    active = (flags != 1) & (flags != 2)
+    post = fd.reshape(19, N)
+    u = np.stack(out[1:])
+    return float((post[:, active] * lam_f[:, active]).sum() + (u[:, active] * lam_u[:, active]).sum())
+
+
+@pytest.mark.parametrize("u_cap", [1.0, 0.04])
+def test_collide_mrt_adjoint_against_binary64_differences(u_cap):
+    """u_cap 1.0: no cell clamps; 0.04: the faster cells go through the Mach clamp."""
+    print(f"[cadj] mrt collision, u_cap {u_cap}", flush=True)
+    from waam_solver_amd._lib import lib
+
+    f, F, f_l, flags, lam_f, lam_u = _fixture()
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    lam_fin = torch.empty((19, N), dtype=torch.float64, device=dev)
+    lam_F = torch.empty((3, N), dtype=torch.float64, device=dev)
+    lam_fl = torch.zeros(N, dtype=torch.float64, device=dev)
+    ft, Ft, flt, fgt = t(f), t(F), t(f_l), t(flags)
+    lft, lut = t(lam_f), t(lam_u)
+    st = torch.cuda.current_stream().cuda_stream
+    lib().call("waam_cadj_collide_mrt", ft.data_ptr(), Ft[0].data_ptr(), Ft[1].data_ptr(),
+               Ft[2].data_ptr(), flt.data_ptr(), fgt.data_ptr(), lft.data_ptr(), lut.data_ptr(),
+               OMEGA_S, OMEGA_B, C_DARCY, u_cap, lam_fin.data_ptr(), lam_F.data_ptr(),
+               lam_fl.data_ptr(), N, st)
+    g_f, g_F, g_fl = lam_fin.cpu().numpy(), lam_F.cpu().numpy(), lam_fl.cpu().numpy()
+    inactive = (flags == 1) | (flags == 2)
+    assert not g_f[:, inactive].any() and not g_F[:, inactive].any() and not g_fl[inactive].any()
+    rng = np.random.default_rng(8)
+    f64, F64, fl64 = f.astype(np.float64), F.astype(np.float64), f_l.astype(np.float64)
+    for kind in ("populations", "forces", "liquid", "all"):
+        df = rng.standard_normal(f.shape) * 1e-2 if kind in ("populations", "all") else 0 * f64
+        dF = rng.standard_normal(F.shape) * 1e-3 if kind in ("forces", "all") else 0 * F64
+        dfl = rng.standard_normal(f_l.shape) * 1e-2 if kind in ("liquid", "all") else 0 * fl64
+        h = 1e-5
+        fd = (_oracle_collide_L(f64 + h * df, F64 + h * dF, fl64 + h * dfl, flags, lam_f, lam_u,
+                                u_cap)
+              - _oracle_collide_L(f64 - h * df, F64 - h * dF, fl64 - h * dfl, flags, lam_f,
+                                  lam_u, u_cap)) / (2 * h)
+        adj = float((df * g_f).sum() + (dF * g_F).sum() + (dfl * g_fl).sum())
+        print(f"[cadj] {kind}: adjoint {adj:.10e} differences {fd:.10e}", flush=True)
+        assert abs(adj - fd) <= 1e-6 * abs(fd), (kind, adj, fd)
+
+
+def _oracle_forces_L(Fr, T, f_l, flags, lam, F_cap, g_lu, beta, T_l):
+    from oracle.oracle import lib as olib, ptr
+
+    L = olib(f64=True)
+    Fx, Fy, Fz = (np.ascontiguousarray(Fr[a], dtype=np.float64) for a in range(3))
+    T64 = np.ascontiguousarray(T, dtype=np.float64)
+    fl = np.ascontiguousarray(f_l, dtype=np.float64)
+    L.or_add_hydrostatic_gravity(ptr(Fz), ptr(fl), ptr(flags), 1.0, g_lu, NX, NY, NZ)
+    L.or_add_buoyancy(ptr(T64), ptr(Fz), ptr(fl), ptr(flags), g_lu, beta, T_l, 1.0, NX, NY, NZ)
+    L.or_clamp_body_force_magnitude(ptr(Fx), ptr(Fy), ptr(Fz), ptr(flags), F_cap, NX, NY, NZ)
+    active = (flags != 1) & (flags != 2)
+    F = np.stack([Fx, Fy, Fz])
+    return float((F[:, active] * lam[:, active]).sum())
+
+
+def test_forces_adjoint_against_binary64_differences():
+    print("[cadj] forces", flush=True)
+    from waam_solver_amd._lib import lib
+
+    rng = np.random.default_rng(12)
+    Fr = rng.uniform(-2e-3, 2e-3, (3, N)).astype(np.float32)
+    T = rng.uniform(1500.0, 2500.0, N).astype(np.float32)
+    f_l = rng.uniform(0.0, 1.0, N).astype(np.float32)
+    f_l[rng.random(N) < 0.1] = 0.0
+    flags = rng.choice(np.array([0, 4, 1, 2], np.uint8), N, p=[0.5, 0.2, 0.15, 0.15])
+    lam = rng.standard_normal((3, N))
+    F_cap, g_lu, beta, T_l = 1.5e-3, 2e-4, 1.2e-4, 1780.0
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    Frt, Tt, flt, fgt, lt = t(Fr), t(T), t(f_l), t(flags), t(lam)
+    lam_Fr = torch.empty((3, N), dtype=torch.float64, device=dev)
+    lam_T = torch.zeros(N, dtype=torch.float64, device=dev)
+    lam_fl = torch.zeros(N, dtype=torch.float64, device=dev)
+    st = torch.cuda.current_stream().cuda_stream
+    # the raw force as the clamp sees it: after hydrostatic gravity and buoyancy
+    from oracle.oracle import lib as olib, ptr
+
+    O = olib()
+    Fz_raw = Fr[2].copy()
+    O.or_add_hydrostatic_gravity(ptr(Fz_raw), ptr(f_l), ptr(flags), 1.0, g_lu, NX, NY, NZ)
+    O.or_add_buoyancy(ptr(T), ptr(Fz_raw), ptr(f_l), ptr(flags), g_lu, beta, T_l, 1.0, NX, NY, NZ)
+    Fzt = t(Fz_raw)
+    lib().call("waam_cadj_forces", Frt[0].data_ptr(), Frt[1].data_ptr(), Fzt.data_ptr(),
+               Tt.data_ptr(), flt.data_ptr(), fgt.data_ptr(), lt.data_ptr(), F_cap, g_lu, beta,
+               T_l, 1, lam_Fr.data_ptr(), lam_T.data_ptr(), lam_fl.data_ptr(), N, st)
+    gF, gT, gfl = lam_Fr.cpu().numpy(), lam_T.cpu().numpy(), lam_fl.cpu().numpy()
+    L0 = abs(_oracle_forces_L(Fr.astype(np.float64), T.astype(np.float64),
+                              f_l.astype(np.float64), flags, lam, F_cap, g_lu, beta, T_l))
+    for kind, h in (("forces", 1e-6), ("T", 1e-3), ("liquid", 1e-4)):
+        dF = rng.standard_normal(Fr.shape) * 1e-3 if kind == "forces" else np.zeros(Fr.shape)
+        dT = rng.standard_normal(T.shape) * 10.0 if kind == "T" else np.zeros(T.shape)
+        dfl = rng.standard_normal(f_l.shape) * 1e-2 if kind == "liquid" else np.zeros(f_l.shape)
+        dfl[f_l == 0.0] = 0.0  # one-sided at the f_l > 0 gate
+        F64, T64, fl64 = Fr.astype(np.float64), T.astype(np.float64), f_l.astype(np.float64)
+        fd = (_oracle_forces_L(F64 + h * dF, T64 + h * dT, fl64 + h * dfl, flags, lam, F_cap,
+                               g_lu, beta, T_l)
+              - _oracle_forces_L(F64 - h * dF, T64 - h * dT, fl64 - h * dfl, flags, lam, F_cap,
+                                 g_lu, beta, T_l)) / (2 * h)
+        adj = float((dF * gF).sum() + (dT * gT).sum() + (dfl * gfl).sum())
+        print(f"[cadj] forces/{kind}: adjoint {adj:.10e} differences {fd:.10e}", flush=True)
+        # the differences carry the binary64 rounding of L over 2h besides their truncation
+        assert abs(adj - fd) <= 1e-6 * abs(fd) + 20 * 2.2e-16 * L0 / h, (kind, adj, fd)

@@ -1,5 +1,5 @@
-"""Reduced-precision population storage (f - w_q in binary16 / bfloat16, fp32 collision
-arithmetic) of the fused step.
+"""Reduced-precision population storage ((f - w_q) * 2^15 in binary16, f - w_q in bfloat16, fp32
+collision arithmetic) of the fused step.
 
 The storage conversions are checked on every 16-bit word against numpy / torch; the whole step
 in each 16-bit form against the oracle twin run with the same storage (its populations narrowed
@@ -25,7 +25,8 @@ def _widen(words: np.ndarray, form: str) -> np.ndarray:
     from oracle.oracle import bf16_bits_to_float
 
     if form == "fp16":
-        return words.view(np.float16).astype(np.float32)
+        # binary16 holds (f - w_q) * 2^15
+        return (words.view(np.float16).astype(np.float32) / np.float32(32768.0)).astype(np.float32)
     return bf16_bits_to_float(words)
 
 
@@ -71,18 +72,20 @@ def test_pack_unpack_every_word(hip_device, form):
     lib().call("waam_pack_populations", xt.data_ptr(), h3.data_ptr(), 4096, code, st)
     d = (x - W[:, None]).astype(np.float32)
     if form == "fp16":
-        want3 = d.astype(np.float16).view(np.uint16)
+        want3 = np.clip(d * np.float32(32768.0), -65504.0, 65504.0).astype(np.float16).view(np.uint16)
     else:
         want3 = torch.from_numpy(d).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
     np.testing.assert_array_equal(h3.cpu().numpy().view(np.uint16), want3)
 
 
 @pytest.mark.parametrize("case,layout", [("bead_mrt", None), ("droplets_srt", None),
-                                         ("bead_mrt", 0), ("bead_odd_mrt", None)])
+                                         ("bead_mrt", 0), ("bead_pair_mrt", None),
+                                         ("bead_odd_mrt", None)])
 @pytest.mark.parametrize("form", ["fp16", "bf16"])
 def test_reduced_storage_step_parity(hip_device, case, layout, form):
-    """Pairs of cells per lane on an even contiguous extent (x in the default layout, z in the
-    reference layout, where z clamps instead of wrapping), cell by cell on an odd one."""
+    """Four cells per lane on a contiguous extent divisible by four (x in the default layout,
+    z in the reference layout, where z clamps instead of wrapping), pairs of cells on an even
+    one, cell by cell on an odd one."""
     print(f"[precision] {case} layout={layout} {form}", flush=True)
     tw, ot = build_pair(case, layout=layout, population_storage=form)
     run_pair(tw, ot, 12)

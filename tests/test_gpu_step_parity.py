@@ -32,7 +32,8 @@ def _assert_parity(res: dict, label: str) -> None:
 @pytest.mark.parametrize("engine,layout", [("reference", 1), ("fused", 1), ("fused", 0),
                                            ("reference", 0)])
 @pytest.mark.parametrize("case", ["bead_mrt", "droplets_srt", "ctwd_srt", "surface_forces_mrt",
-                                  "recoil_const_srt", "full_tier_mrt"])
+                                  "recoil_const_srt", "full_tier_mrt", "bead_box_mrt",
+                                  "bead_odd_mrt"])
 def test_coupled_step_parity(hip_device, case, engine, layout):
     """layout 1: fields stored (nz, ny, nx), i fastest; layout 0: the reference's (nx, ny, nz).
     The fused engine in layout 0 runs the surface list in a launch of its own (the large-grid
@@ -53,6 +54,11 @@ def test_coupled_step_parity(hip_device, case, engine, layout):
         assert tw._deposited_volume_m3 == ot._deposited_volume_m3
     if case == "ctwd_srt":
         assert tw._bead_height_m == ot._bead_height_m and tw.ctwd_m == ot.ctwd_m
+    if case == "bead_box_mrt" and engine == "fused":
+        # the arc's zero-heat box (half-width sqrt(220) sigma + 2 cells) is narrower than the
+        # grid in x, so the box branch of the arc sums and the box gate ran
+        p = tw._fused._params(2.0, 3.0, True, None)
+        assert 2.0 * (220.0 ** 0.5 * p.sigma + 2.0) + 1.0 < tw.grid.nx
     tel = tw.get_telemetry()
     pm = ot.pool_metrics()
     assert tel["n_liquid_cells"] == pm["n_liquid_cells"]
@@ -143,3 +149,74 @@ def test_moving_window_parity(hip_device, engine, layout):
         if s in (0, 12, 55, 99):
             _assert_parity(compare(tw, ot), f"window/{engine}/layout{layout} step {s + 1}")
     assert tw._window_offset_x_m == ot._window_offset_x_m == 3 * 8 * g.dx
+
+
+@pytest.mark.parametrize("case", ["bench_small", "droplets_srt"])
+def test_quiet_segments_skip_with_identical_results(hip_device, case):
+    """Two fused twins from the same state, one skipping the sweeps on quiet row segments (all
+    GAS with phi 0 around them for two steps) and one sweeping every cell: every field and both
+    population buffers stay identical over 30 steps (with droplets, whose deposition writes
+    cells between steps), and the skipping twin did skip segments."""
+    import torch
+
+    from waam_solver_amd.field import tensor_of
+    from waam_solver_amd.solvers.fused import FusedStepper
+
+    tw, _ = build_pair(case)
+    ref, _ = build_pair(case)
+    ref._fused = FusedStepper(ref)
+    ref._fused.quiet_skip = False
+    g = tw.grid
+    skipped = []
+    for s in range(30):
+        x = (g.nx // 3) * g.dx + 8e-3 * s * g.dt
+        y = (g.ny // 2) * g.dx
+        tw.step(x, y, True)
+        ref.step(x, y, True)
+        q = tensor_of(g.quiet)
+        nseg = q.numel() // 2
+        skipped.append(int(q[nseg:].sum().item()) if tw._fused.steady() else 0)
+    torch.cuda.synchronize()
+    tw.sync()
+    ref.sync()
+    diff = [n for n in FIELDS + ("f_src", "f_dst")
+            if getattr(tw.grid, n).to_numpy().tobytes() != getattr(ref.grid, n).to_numpy().tobytes()]
+    print(f"[quiet] {case}: segments skipped per step {skipped}")
+    assert not diff, f"{case}: quiet skipping changed {diff}"
+    assert max(skipped) > 0
+
+
+@pytest.mark.parametrize("pop,nx", [("fp32", 64), ("fp32", 63), ("fp16", 64), ("fp16", 62),
+                                    ("fp16", 63), ("bf16", 64)])
+def test_pull_bounces_back_for_a_cell_that_turned_gas(hip_device, pop, nx):
+    """A FLUID cell in the gas under a SOLID lump turns GAS on the step after its collision:
+    the deferred pull of that collision's stream must bounce its downward population back off
+    the lump, as the reference's stream does with the flags of the step it belongs to (the cell
+    was FLUID then), whatever its flag now. Every pull variant: 8-byte words (nx 64), cell pairs
+    (16-bit storage, nx 62), cell by cell (nx 63)."""
+    print(f"[pull] turned-gas bounce-back {pop} nx={nx}", flush=True)
+    from tests.fixtures import FLUID, SOLID
+
+    tw, ot = build_pair("bead_mrt", population_storage=pop, nx=nx)
+    g, og = tw.grid, ot.g
+    k0, i0, j0 = tw.nz_solid + 6, nx // 2 + 4, 8
+    cp_rho, L_rho = ot.cp_rho, ot.L_rho
+    cells = {(i0, j0, k0 + 1): (SOLID, 1.0, 0.0, 1000.0, cp_rho * 1000.0),
+             (i0, j0, k0): (FLUID, 0.6, 1.0, 2100.0, cp_rho * 2100.0 + L_rho)}
+    arrs = {n: getattr(og, n).copy() for n in ("flags", "phi", "f_l", "T", "H")}
+    for (i, j, k), (fl, ph, liq, T, H) in cells.items():
+        arrs["flags"][i, j, k] = fl
+        arrs["phi"][i, j, k] = ph
+        arrs["f_l"][i, j, k] = liq
+        arrs["T"][i, j, k] = T
+        arrs["H"][i, j, k] = np.float32(H)
+    for n, a in arrs.items():
+        getattr(og, n)[...] = a
+        getattr(g, n).from_numpy(a)
+    run_pair(tw, ot, 1)
+    assert int(g.flags[i0, j0, k0]) == FLUID and int(og.flags[i0, j0, k0]) == FLUID
+    g.phi[i0, j0, k0] = 0.0
+    og.phi[i0, j0, k0] = 0.0
+    run_pair(tw, ot, 1, start=1)
+    assert int(og.flags[i0, j0, k0]) == 2 and int(og.flags[i0, j0, k0 + 1]) == SOLID
+    _assert_parity(compare(tw, ot), f"pull/{pop}/nx{nx}")

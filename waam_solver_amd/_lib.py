@@ -16,7 +16,7 @@ import torch  # noqa: F401
 
 LIB_PATH = Path(__file__).resolve().parent / "libwaam_hip.so"
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 FLAG_FLUID, FLAG_SOLID, FLAG_GAS, FLAG_IFACE = 0, 1, 2, 4
 LAYOUT_XYZ, LAYOUT_ZYX = 0, 1
 
@@ -81,6 +81,11 @@ SIGNATURES: dict[str, str] = {
     "waam_adj_marangoni": "pppppppfpppiiiip",
     "waam_adj_collide_srt": "pppppppffpplp",
     "waam_adj_stream": "pppiiiip",
+    "waam_cadj_collide_mrt": "ppppppppddddppplp",
+    "waam_cadj_forces": "pppppppddddippplp",
+    "waam_cadj_marangoni": "ppppdppiiiip",
+    "waam_cadj_phase": "pppppplp",
+    "waam_cadj_thermal_step": "ppppppppppppppppppiiiipp",
     "waam_graph_capture": "pppip",
     "waam_graph_launch": "pppppp",
     "waam_graph_steps": "p",
@@ -140,6 +145,8 @@ class _Library:
         self.dll.waam_abi_version.restype = ctypes.c_int
         self.dll.waam_last_error.restype = ctypes.c_char_p
         self.dll.waam_scratch_bytes.restype = ctypes.c_int64
+        self.dll.waam_quiet_bytes.restype = ctypes.c_int64
+        self.dll.waam_quiet_bytes.argtypes = [ctypes.c_int] * 4
         ver = self.dll.waam_abi_version()
         if ver != ABI_VERSION:
             raise WaamError(f"libwaam_hip ABI {ver} != expected {ABI_VERSION}")
@@ -152,6 +159,9 @@ class _Library:
 
     def scratch_bytes(self) -> int:
         return int(self.dll.waam_scratch_bytes())
+
+    def quiet_bytes(self, nx: int, ny: int, nz: int, layout: int) -> int:
+        return int(self.dll.waam_quiet_bytes(nx, ny, nz, layout))
 
     def last_error(self) -> str:
         msg = self.dll.waam_last_error()
@@ -174,4 +184,5 @@ def lib() -> _Library:
 
 
 def exported_symbols() -> list[str]:
-    return ["waam_abi_version", "waam_last_error", "waam_scratch_bytes", *SIGNATURES]
+    return ["waam_abi_version", "waam_last_error", "waam_scratch_bytes", "waam_quiet_bytes",
+            *SIGNATURES]

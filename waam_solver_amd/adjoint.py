@@ -31,6 +31,7 @@ import torch
 from . import kernels
 from ._lib import lib
 from .field import P, tensor_of
+from .physics import weld_forces
 from .solvers.coupled_step import resolve_arc_k
 
 _f = ctypes.c_float
@@ -68,7 +69,12 @@ class ThermalRollout:
         tensor_of(self.T0).copy_(tensor_of(self.T))
         tensor_of(self.fl0).copy_(tensor_of(self.fl))
         self.zero_u = f()
-        self.flags, self.phi = g.flags, g.phi
+        # the rollout's own copies of the frozen flags and phi: the twin may step (and the fused
+        # step updates phi in place and swaps its flag buffers) between forward and backward
+        self.flags = g.field(torch.uint8, np_dtype=np.int32)
+        self.phi = f()
+        tensor_of(self.flags).copy_(tensor_of(g.flags))
+        tensor_of(self.phi).copy_(tensor_of(g.phi))
         self.h_conv0, self.eps0 = float(twin.h_conv), float(twin.eps_rad)
         self.sigma0 = float(twin.sigma_cells)
         x0, y0 = path(0)
@@ -317,3 +323,293 @@ def collide_srt_vjp(f_src, Fx, Fy, Fz, f_l, flags, lam, omega: float, C_darcy: f
                Fz.data_ptr(), f_l.data_ptr(), flags.data_ptr(), lam.data_ptr(), float(omega),
                float(C_darcy), lam_f.data_ptr(), lam_F.data_ptr(), n, st)
     return lam_f, lam_F
+
+
+# ---------------------------------------------------------------------------- the coupled step
+class CoupledRollout:
+    """n_steps of the coupled step with the flags and phi frozen, from the twin's current state
+    along ``path`` (step -> torch (x_m, y_m)), differentiable in theta = (eta, s_h, s_sigma).
+
+    Forward, per step (reference solvers/coupled_step.py:97-409 with the free surface, the flags,
+    deposition and the stream held), the product's own kernels: arc injection and ceiling,
+    conduction and enthalpy advection by the step's starting velocity, boundary losses, floor,
+    phase map, ceiling, phase map; clear forces, CSF, Marangoni, arc pressure, hydrostatic
+    gravity, buoyancy, force clamp; MRT collision and Mach clamp. With frozen flags the stream
+    never feeds a collided cell (the next collision reads the cell's own post-collision
+    populations), so the populations ping-pong between two buffers without it.
+
+    Backward (``csrc/adjoint_coupled.hip``): the adjoint state (lambda_H, lambda_f, lambda_u) in
+    binary64 runs back through each step — collision with the Mach clamp, forces with the force
+    clamp, the phase map, the thermal chain with advection — and accumulates dL/dtheta in
+    binary64. The forward keeps the state every ``checkpoint_every`` steps and the backward
+    recomputes each segment from its checkpoint."""
+
+    def __init__(self, twin, n_steps: int, path, checkpoint_every: int = 50,
+                 is_welding: bool = True):
+        if type(twin.arc_source).__name__ != "Gaussian2D":
+            raise NotImplementedError("the differentiation form covers the Gaussian2D arc")
+        if twin.use_srt or twin.use_material_tables:
+            raise NotImplementedError("the coupled adjoint covers the scalar-material MRT step")
+        if twin.enable_recoil or twin.enable_gas_shear or twin.enable_lorentz:
+            raise NotImplementedError("the coupled adjoint covers the flow-tier forces")
+        twin.sync()
+        self.tw, g = twin, twin.grid
+        self.n_steps, self.K = int(n_steps), max(1, int(checkpoint_every))
+        self.path, self.is_welding = path, bool(is_welding)
+        f = lambda: g.field(torch.float32)  # noqa: E731
+        fp = lambda: g.field(torch.float32, leading=(19,))  # noqa: E731
+        self.H, self.T, self.fl, self.Htmp = f(), f(), f(), f()
+        self.rho, self.ux, self.uy, self.uz = f(), f(), f(), f()
+        self.Fx, self.Fy, self.Fz = f(), f(), f()
+        self.fa, self.fb = fp(), fp()
+        self.flags = g.field(torch.uint8, np_dtype=np.int32)
+        self.phi = f()
+        tensor_of(self.flags).copy_(tensor_of(g.flags))
+        tensor_of(self.phi).copy_(tensor_of(g.phi))
+        # the starting state: the twin's enthalpy (T, f_l from the phase map), populations and
+        # velocity
+        self.H0 = tensor_of(g.H).clone()
+        self.f0 = tensor_of(g.f_src).clone()
+        self.u0 = [tensor_of(x).clone() for x in (g.ux, g.uy, g.uz)]
+        self.h_conv0, self.eps0 = float(twin.h_conv), float(twin.eps_rad)
+        self.sigma0 = float(twin.sigma_cells)
+        x0, y0 = path(0)
+        self.arc_k = resolve_arc_k(twin, g, x0 / g.dx, y0 / g.dx)
+        self.norm_buf = torch.zeros((), dtype=torch.float32, device=g.device)
+        self.u_cap = float(getattr(twin, "u_mach_limit_lu", 0.08))
+        self.F_cap = float(getattr(twin, "force_limit_lu", 0.05))
+        self.ckpt: list = []
+        self.theta = None
+
+    # ------------------------------------------------------------------ forward
+    def _state_in(self, H, fpop, u) -> None:
+        tensor_of(self.H).copy_(H)
+        tensor_of(self.fa).copy_(fpop)
+        for d, s in zip((self.ux, self.uy, self.uz), u):
+            tensor_of(d).copy_(s)
+        self._phase()
+
+    def _state_out(self):
+        return (tensor_of(self.H).clone(), tensor_of(self.fa).clone(),
+                [tensor_of(x).clone() for x in (self.ux, self.uy, self.uz)])
+
+    def _phase(self) -> None:
+        tw = self.tw
+        kernels.update_phase(self.H, self.T, self.fl, tw.cp_rho, tw.L_rho, tw.mat.T_solidus,
+                             tw.mat.T_liquidus)
+
+    def _ceiling(self) -> None:
+        tw, g = self.tw, self.tw.grid
+        if tw.enable_enthalpy_cap:
+            kernels.clamp_enthalpy_ceiling_scalar(self.H, self.flags, tw.cp_rho,
+                                                  tw.mat.T_solidus, tw.mat.T_liquidus,
+                                                  tw.T_vapor_cap_K, tw.L_rho, g.FLAG_GAS)
+
+    def _step(self, n: int, theta, save=None) -> None:
+        tw, g = self.tw, self.tw.grid
+        eta, s_h, s_sig = (float(v) for v in theta)
+        x, y = self.path(n)
+        arc_i, arc_j = x / g.dx, y / g.dx
+        sv = (lambda k, t: save[k].copy_(tensor_of(t))) if save is not None else None
+        if sv:
+            sv("H", self.H)
+            sv("fl", self.fl)
+            for a, u in zip(("ux", "uy", "uz"), (self.ux, self.uy, self.uz)):
+                sv(a, u)
+        if self.is_welding:
+            kernels.inject_arc_heat(self.H, self.flags, self.phi, self.fl, self.norm_buf, arc_i,
+                                    arc_j, self.arc_k, tw.Q_w, self.sigma0 * s_sig, g.dt,
+                                    g.dx ** 3, eta, tw.arc_penetration_m / g.dx,
+                                    1 if tw.arc_surface_weighting else 0, g.FLAG_SOLID,
+                                    g.FLAG_GAS, scratch=g.scratch)
+        if sv:
+            sv("A", self.H)
+        self._ceiling()
+        if sv:
+            sv("B", self.H)
+        kernels.advect_diffuse_temperature_into(self.H, self.Htmp, self.T, self.ux, self.uy,
+                                                self.uz, self.flags, tw.alpha_lu, 1.0, tw.cp_rho,
+                                                g.nx, g.ny, g.nz)
+        self.H, self.Htmp = self.Htmp, self.H
+        if tw.enable_heat_loss:
+            kernels.apply_thermal_boundary_losses(
+                self.H, self.T, self.flags, tw.T_amb, self.h_conv0 * s_h, self.eps0 * s_h,
+                1 if tw.enable_convection else 0, 1 if tw.enable_radiation else 0, tw.cp_rho,
+                g.dt, g.dx, tw.sigma_sb, g.FLAG_SOLID, g.FLAG_GAS, g.nx, g.ny, g.nz)
+        if sv:
+            sv("E", self.H)
+        if tw.enable_heat_loss:
+            kernels.clamp_enthalpy_floor_scalar(self.H, tw.cp_rho, self.flags, tw.T_amb,
+                                                g.FLAG_GAS)
+        if sv:
+            sv("F", self.H)
+        self._phase()
+        self._ceiling()
+        self._phase()
+        # forces on the new temperature and liquid fraction
+        kernels.clear_forces(self.Fx, self.Fy, self.Fz)
+        if tw.enable_csf_tension:
+            kernels.compute_csf_tension(self.phi, self.flags, self.Fx, self.Fy, self.Fz,
+                                        tw.gamma_lu, g.FLAG_SOLID, g.FLAG_GAS, g.nx, g.ny, g.nz,
+                                        int(tw.enable_wetting), tw.theta_rad)
+        kernels.compute_marangoni_force(self.T, self.phi, self.fl, self.Fx, self.Fy, self.Fz,
+                                        self.flags, tw.dgamma_dT_lu, g.dx, g.FLAG_SOLID,
+                                        g.FLAG_GAS, g.nx, g.ny, g.nz)
+        if self.is_welding:
+            kernels.apply_arc_pressure(self.Fz, self.flags, self.phi, arc_i, arc_j, self.arc_k,
+                                       weld_forces.arc_pressure_sigma_cells(tw),
+                                       weld_forces.arc_pressure_peak_pa(tw), g.dt, g.dx,
+                                       tw.mat.rho, g.FLAG_SOLID, g.FLAG_GAS)
+        if tw.enable_hydrostatic_gravity:
+            kernels.add_hydrostatic_gravity(self.Fz, self.fl, self.flags, 1.0, tw.g_lu,
+                                            g.FLAG_SOLID, g.FLAG_GAS)
+        kernels.add_buoyancy(self.T, self.Fz, self.fl, self.flags, tw.g_lu, tw.beta_T,
+                             tw.mat.T_liquidus, 1.0, g.FLAG_SOLID, g.FLAG_GAS)
+        if sv:
+            for a, fld in zip(("Frx", "Fry", "Frz"), (self.Fx, self.Fy, self.Fz)):
+                sv(a, fld)
+        if self.F_cap > 0.0:
+            kernels.clamp_body_force_magnitude(self.Fx, self.Fy, self.Fz, self.flags, self.F_cap,
+                                               g.FLAG_SOLID, g.FLAG_GAS)
+        if sv:
+            for a, fld in zip(("Fx", "Fy", "Fz"), (self.Fx, self.Fy, self.Fz)):
+                sv(a, fld)
+            sv("f", self.fa)
+            sv("H1", self.H)
+            sv("T1", self.T)
+            sv("fl1", self.fl)
+        kernels.collide_mrt(self.fa, self.fb, self.rho, self.ux, self.uy, self.uz, self.Fx,
+                            self.Fy, self.Fz, self.fl, self.flags, None, None, None, None, None,
+                            tw.omega, tw.omega_bulk, tw.C_darcy, g.FLAG_SOLID, g.FLAG_GAS, g.nx,
+                            g.ny, g.nz)
+        if self.u_cap > 0.0:
+            kernels.clamp_velocity_mach(self.ux, self.uy, self.uz, self.flags, self.u_cap,
+                                        g.FLAG_SOLID, g.FLAG_GAS)
+        self.fa, self.fb = self.fb, self.fa
+
+    def forward(self, theta) -> tuple[torch.Tensor, torch.Tensor]:
+        """Final (T, f_l) (storage order) after n_steps with parameters theta."""
+        self.theta = tuple(float(v) for v in theta)
+        self.ckpt = []
+        self._state_in(self.H0, self.f0, self.u0)
+        for n in range(self.n_steps):
+            if n % self.K == 0:
+                self.ckpt.append(self._state_out())
+            self._step(n, self.theta)
+        self.H_final = tensor_of(self.H).clone()
+        return tensor_of(self.T).clone(), tensor_of(self.fl).clone()
+
+    # ------------------------------------------------------------------ backward
+    _SAVE = ("H", "fl", "ux", "uy", "uz", "A", "B", "E", "F", "Frx", "Fry", "Frz", "Fx", "Fy",
+             "Fz", "H1", "T1", "fl1")
+
+    def backward(self, grad_T: torch.Tensor, grad_fl: torch.Tensor) -> np.ndarray:
+        """dL/dtheta (float64) given dL/dT and dL/df_l at the final step."""
+        if self.theta is None:
+            raise RuntimeError("backward before forward")
+        tw, g = self.tw, self.tw.grid
+        st = torch.cuda.current_stream().cuda_stream
+        dev = tensor_of(self.H).device
+        N = g.n_cells
+        shp = tuple(tensor_of(self.H).shape)
+        d64 = dict(dtype=torch.float64, device=dev)
+        lam_H = torch.zeros(shp, **d64)
+        lam_f = torch.zeros((19,) + shp, **d64)
+        lam_u = torch.zeros((3,) + shp, **d64)
+        lam_Ho = torch.empty(shp, **d64)
+        lam_uo = torch.empty((3,) + shp, **d64)
+        lam_fin = torch.empty((19,) + shp, **d64)
+        lam_F = torch.empty((3,) + shp, **d64)
+        lam_Fr = torch.empty((3,) + shp, **d64)
+        lam_T1 = torch.empty(shp, **d64)
+        lam_fl1 = torch.empty(shp, **d64)
+        G = torch.empty((3,) + shp, **d64)
+        lamC = torch.empty(shp, **d64)
+        lamB = torch.empty(shp, **d64)
+        grads = torch.zeros(3, **d64)
+        sums = torch.zeros(2, **d64)
+        c_last = self._consts(self.n_steps - 1)
+        lib().call("waam_cadj_phase", P(self.H_final), P(self.flags), ctypes.addressof(c_last),
+                   P(grad_T.double().contiguous()), P(grad_fl.double().contiguous()), P(lam_H),
+                   N, st)
+        for s_idx in reversed(range(len(self.ckpt))):
+            s0 = s_idx * self.K
+            s1 = min(self.n_steps, s0 + self.K)
+            L = s1 - s0
+            sv = {k: torch.empty((L,) + shp, dtype=torch.float32, device=dev) for k in self._SAVE}
+            sv["f"] = torch.empty((L, 19) + shp, dtype=torch.float32, device=dev)
+            self._state_in(*self.ckpt[s_idx])
+            for t in range(L):
+                self._step(s0 + t, self.theta, save={k: v[t] for k, v in sv.items()})
+            for t in reversed(range(L)):
+                c = self._consts(s0 + t)
+                # collision (with the Mach clamp) -> lam_f of the step's populations, lam_F, f_l
+                lam_fl1.zero_()
+                lam_T1.zero_()
+                lib().call("waam_cadj_collide_mrt", P(sv["f"][t]), P(sv["Fx"][t]),
+                           P(sv["Fy"][t]), P(sv["Fz"][t]), P(sv["fl1"][t]), P(self.flags),
+                           P(lam_f), P(lam_u), tw.omega, tw.omega_bulk, tw.C_darcy, self.u_cap,
+                           P(lam_fin), P(lam_F), P(lam_fl1), N, st)
+                lib().call("waam_cadj_forces", P(sv["Frx"][t]), P(sv["Fry"][t]), P(sv["Frz"][t]),
+                           P(sv["T1"][t]), P(sv["fl1"][t]), P(self.flags), P(lam_F), self.F_cap,
+                           tw.g_lu, tw.beta_T, tw.mat.T_liquidus,
+                           1 if tw.enable_hydrostatic_gravity else 0, P(lam_Fr), P(lam_T1),
+                           P(lam_fl1), N, st)
+                lib().call("waam_cadj_marangoni", P(self.phi), P(sv["fl1"][t]), P(self.flags),
+                           P(lam_Fr), tw.dgamma_dT_lu, P(G), P(lam_T1), g.nx, g.ny, g.nz,
+                           g.layout, st)
+                lib().call("waam_cadj_phase", P(sv["H1"][t]), P(self.flags), ctypes.addressof(c),
+                           P(lam_T1), P(lam_fl1), P(lam_H), N, st)
+                # the thermal chain with advection -> lam_H, lam_u of the step's start
+                if self.is_welding:
+                    lib().call("waam_adj_arc_sums", P(self.flags), P(sv["fl"][t]),
+                               ctypes.addressof(c), P(sums), g.nx, g.ny, g.nz, g.layout,
+                               P(g.scratch), st)
+                lib().call("waam_cadj_thermal_step", P(lam_H), P(lam_Ho), P(lam_uo),
+                           P(sv["H"][t]), P(sv["A"][t]), P(sv["B"][t]), P(sv["E"][t]),
+                           P(sv["F"][t]), P(sv["fl"][t]), P(sv["ux"][t]), P(sv["uy"][t]),
+                           P(sv["uz"][t]), P(self.flags), P(sums), ctypes.addressof(c), P(grads),
+                           P(lamC), P(lamB), g.nx, g.ny, g.nz, g.layout, P(g.scratch), st)
+                lam_H, lam_Ho = lam_Ho, lam_H
+                lam_u, lam_uo = lam_uo, lam_u
+                lam_f, lam_fin = lam_fin, lam_f
+            del sv
+        self.lam0 = (lam_H, lam_f, lam_u)
+        return grads.cpu().numpy()
+
+    def _consts(self, n: int) -> AdjConsts:
+        return ThermalRollout._consts(self, n)
+
+
+class CoupledRolloutFn(torch.autograd.Function):
+    """(T_final, f_l_final) = coupled rollout(theta), differentiable in theta (length 3)."""
+
+    @staticmethod
+    def forward(ctx, theta: torch.Tensor, rollout: CoupledRollout):
+        T, fl = rollout.forward(theta.detach().double().cpu().numpy())
+        ctx.rollout = rollout
+        ctx.theta_dtype, ctx.theta_device = theta.dtype, theta.device
+        return T, fl
+
+    @staticmethod
+    def backward(ctx, grad_T, grad_fl):
+        g = ctx.rollout.backward(grad_T, grad_fl)
+        return torch.tensor(g, dtype=ctx.theta_dtype, device=ctx.theta_device), None
+
+
+def pool_wd_surrogate(fl: torch.Tensor, i_c: int, nz_solid: int, layout: int, dx: float,
+                      tau: float = 0.1) -> tuple[torch.Tensor, torch.Tensor]:
+    """Smoothed pool width and depth (m) on the cross-section x = i_c, differentiable in f_l.
+
+    The reference's telemetry (twin.py:716-744) counts, on the cross-section through the pool,
+    the y-extent of f_l > 0.5 and the plate layers down to the deepest f_l > 0.5: step functions
+    of f_l. The surrogate replaces each column's and each plate layer's indicator with a logistic
+    of its largest liquid fraction: W = dx * sum_j s((max_k f_l(i_c, j, k) - 0.5) / tau),
+    D = dx * sum_{k < nz_solid} s((max_j f_l(i_c, j, k) - 0.5) / tau), s the logistic function;
+    as tau -> 0 these become the cell counts of a pool without holes."""
+    sec = fl[:, :, i_c] if layout == 1 else fl[i_c].transpose(0, 1)  # (nz, ny)
+    col = sec.max(dim=0).values            # per j: the column's largest f_l
+    lay = sec[:nz_solid].max(dim=1).values  # per plate layer
+    W = dx * torch.sigmoid((col - 0.5) / tau).sum()
+    D = dx * torch.sigmoid((lay - 0.5) / tau).sum()
+    return W, D

@@ -42,6 +42,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include <hip/hip_ext.h>
@@ -67,6 +68,11 @@ constexpr int kListBlocksWithSweep = 256;
 #endif
 constexpr bool kSeparatePull32 = WAAM_SEPARATE_PULL32 != 0;
 constexpr bool kSeparatePull16 = false;
+/* the pull on 8-byte words per lane where the contiguous extent allows it */
+#ifndef WAAM_WIDE_PULL
+#define WAAM_WIDE_PULL 1
+#endif
+constexpr bool kWidePull = WAAM_WIDE_PULL != 0;
 /* steps one captured graph may hold */
 constexpr int kGraphMaxSteps = 4096;
 
@@ -219,6 +225,97 @@ __device__ __forceinline__ void cell_ijk(size_t c, const Dims &d, int &i, int &j
     j = (int)(r - (unsigned)b * ny);
     i = d.layout == kLayoutZYX ? a : b;
     k = d.layout == kLayoutZYX ? b : a;
+}
+
+/* Quiet row segments. A segment is the 64 cells one wavefront of a grid3() launch covers along
+ * the contiguous axis at one (j, slowest-axis) position. A segment is quiet at the end of a step
+ * when every cell in it is GAS with phi exactly 0. For such a cell, kernels 2, 4 and 7 change
+ * nothing as long as every cell their stencils read around it is quiet too: the GAS branch of
+ * the thermal chain keeps H and maps it to the same T and f_l, the advected and contact-ruled
+ * phi stay 0, the flags stay GAS, f_l is reset to 0, no force was set, and no HAZ or freeze rule
+ * applies to GAS. A segment that was quiet at the end of the previous step, with all its
+ * neighbours, and at the end of the step before as well, therefore holds this step's results in
+ * both buffers of every pair already (each buffer was last written by a step that left it
+ * quiet), and the three sweeps skip it. The record: bits[seg] bit 0 = quiet at the end of the
+ * last step, bit 1 = quiet at the end of the step before (written by kernel 7); skip[seg] =
+ * this step's decision (written by kernel 2, read by kernels 4 and 7). */
+struct QuietRec {
+    uint8_t *bits, *skip;
+    int nbx;  // segments per row
+};
+
+__host__ __device__ inline int quiet_nbx(const Dims &d) { return (d.fast() + kTileFast - 1) / kTileFast; }
+__host__ __device__ inline size_t quiet_segments(const Dims &d)
+{
+    return (size_t)quiet_nbx(d) * (size_t)d.ny * (size_t)d.slow();
+}
+
+__device__ __forceinline__ QuietRec quiet_rec(const waam_step_fields &f, const Dims &d)
+{
+    QuietRec q;
+    q.nbx = quiet_nbx(d);
+    q.bits = f.quiet;
+    q.skip = f.quiet + quiet_segments(d);
+    return q;
+}
+
+__device__ __forceinline__ size_t seg_index(const QuietRec &q, int bx, int j, int b, const Dims &d)
+{
+    return ((size_t)b * (size_t)d.ny + (size_t)j) * (size_t)q.nbx + (size_t)bx;
+}
+
+/* Whether the wavefront's segment (bx, j, b) skips this step: bit 0 on every segment the
+ * stencils of its cells reach and bit 1 on itself. The reach: one cell along the contiguous
+ * axis, one row in j, and along the slowest axis two slices below and one above in the ZYX
+ * layout (the contact rule reads k-1 and k-2, the phi advection k+1) or one slice either way in
+ * XYZ (x; there k-2 lies on the contiguous axis). x wraps, y and z clamp (a clamped neighbour is
+ * an edge cell itself). Uniform over the wavefront; every lane must call it. */
+__device__ __forceinline__ bool segment_skips(const QuietRec &q, int bx, int j, int b,
+                                              const Dims &d)
+{
+    const int lane = (int)threadIdx.x;  // block3(): the lanes run along the contiguous axis
+    bool ok = true;
+    if (lane < 36) {
+        const bool zyx = d.layout == kLayoutZYX;
+        int x = bx + lane % 3 - 1;
+        const int jj = imax(0, imin(j + (lane / 3) % 3 - 1, d.ny - 1));
+        const int db = lane / 9 - 2;
+        int bb;
+        if (zyx) {
+            x = x < 0 ? x + q.nbx : (x >= q.nbx ? x - q.nbx : x);
+            bb = imax(0, imin(b + db, d.slow() - 1));
+        } else {
+            x = imax(0, imin(x, q.nbx - 1));
+            bb = db == -2 ? b : b + db;
+            bb = bb < 0 ? bb + d.slow() : (bb >= d.slow() ? bb - d.slow() : bb);
+        }
+        const uint8_t v = q.bits[seg_index(q, x, jj, bb, d)];
+        ok = (v & 1u) != 0;
+        if (lane == 22) ok = ok && (v & 2u) != 0;  // offsets (0, 0, 0): the segment itself
+    }
+    return __ballot(!ok) == 0ull;
+}
+
+/* kernel 2's decision for the wavefront's segment, recorded for kernels 4 and 7 */
+__device__ __forceinline__ bool sweep_skips(const waam_step_fields &f, const Dims &d, int bx,
+                                            int j, int b)
+{
+    QuietRec q = quiet_rec(f, d);
+    const bool sk = segment_skips(q, bx, j, b, d);
+    if (threadIdx.x == 0) q.skip[seg_index(q, bx, j, b, d)] = sk ? 1 : 0;
+    return sk;
+}
+
+/* the wavefront's segment ends the step quiet (`quiet`: this lane's cell is, or it has none):
+ * kernel 7 records it, shifting the previous bit up when the record is continued */
+__device__ __forceinline__ void record_quiet(const QuietRec &q, size_t seg, bool quiet,
+                                             bool cont)
+{
+    const bool all = __ballot(!quiet) == 0ull;
+    if (threadIdx.x == 0) {
+        const uint8_t old = q.bits[seg];
+        q.bits[seg] = (uint8_t)((cont ? ((old & 1u) << 1) : 0u) | (all ? 1u : 0u));
+    }
 }
 
 /* 1 — surface height in the torch column (reference kernels.py:625-641 and
@@ -537,6 +634,10 @@ __global__ void __launch_bounds__(256) k_step_thermal(waam_step_fields f, StepCo
     const Dims &d = s.d;
     int i, j, k;
     bool in = d.cell(i, j, k);
+    if (!LIST && s.p.quiet_skip) {
+        if (j >= d.ny) return;  // the wavefront's whole row is past the grid
+        if (sweep_skips(f, d, (int)blockIdx.x, j, (int)blockIdx.z)) return;
+    }
     size_t c = in ? d.idx(i, j, k) : 0;
     /* the flag, the cell's enthalpy and the phi-advection operands are requested together (a
      * GAS cell, most of a grid, then waits for one memory round trip); a FLUID/IFACE cell's
@@ -604,7 +705,13 @@ __global__ void __launch_bounds__(256) k_step_thermal_all(waam_step_fields f, St
             cell_ijk(c, d, i, j, k);
             flag = f.flags_in[c];
         } else {
-            if (!first || !d.cell(i, j, k, lz)) return;
+            if (!first) return;
+            const bool in = d.cell(i, j, k, lz);
+            if (s.p.quiet_skip) {
+                if (j >= d.ny) return;  // the wavefront's whole row is past the grid
+                if (sweep_skips(f, d, (int)blockIdx.x, j, (int)blockIdx.z - lz)) return;
+            }
+            if (!in) return;
             c = d.idx(i, j, k);
             flag = f.flags_in[c];
         }
@@ -645,6 +752,11 @@ __global__ void __launch_bounds__(256) k_step_vof_flags(waam_step_fields f, Step
         cell_list(f, s.seg_len, 0).count(threadIdx.y * blockDim.x + threadIdx.x) = 0;  // list
     int i, j, k;
     bool in = d.cell(i, j, k);
+    if (s.p.quiet_skip) {
+        if (j >= d.ny) return;
+        QuietRec q = quiet_rec(f, d);
+        if (q.skip[seg_index(q, (int)blockIdx.x, j, (int)blockIdx.z, d)]) return;
+    }
     size_t c = in ? d.idx(i, j, k) : 0;
     uint8_t nf = GAS;
     if (in) {
@@ -903,10 +1015,12 @@ __device__ __forceinline__ uint8_t freeze_cell(const waam_step_fields &f, const 
 /* 7 — every cell: stream flags; cells the collision skipped: force reset and the previous
  *     step's pull; then the freeze/remelt updates of every non-GAS cell (cell-local; the pull
  *     reads only populations and stream flags, which they do not touch); lists the cells that
- *     end the step FLUID/IFACE, the next step's thermal list */
+ *     end the step FLUID/IFACE, the next step's thermal list; records which row segments end
+ *     the step quiet. `skip`: the segment is quiet and only the pull runs. Returns whether the
+ *     cell ends the step quiet. */
 template <class E, bool PULL>
-__device__ __forceinline__ void stream_cell(const waam_step_fields &f, const StepConsts &s, int i,
-                                            int j, int k)
+__device__ __forceinline__ bool stream_cell(const waam_step_fields &f, const StepConsts &s, int i,
+                                            int j, int k, bool skip)
 {
     const Dims &d = s.d;
     const waam_step_params &p = s.p;
@@ -914,17 +1028,18 @@ __device__ __forceinline__ void stream_cell(const waam_step_fields &f, const Ste
     uint8_t fc = f.flags_out[c];
     uint8_t fp = f.sflags_prev[c];
     bool surface = is_surface_class(fc);
-    bool freeze = p.enable_freeze && fc != GAS;
+    bool freeze = !skip && p.enable_freeze && fc != GAS;
     float T = 0.0f, phi = 0.0f, fl = 0.0f;
     if (freeze) {
         T = f.T_out[c];
-        phi = f.phi[c];
         fl = f.fl_out[c];
     }
+    if (!skip) phi = f.phi[c];
     bool need = !surface && p.pending_stream && fp != SOLID;
     if (PULL && __ballot(need) != 0ull)
         pull_rows(static_cast<const E *>(f.f_src), static_cast<E *>(f.f_dst), f.sflags_prev, i, j,
                   k, need, fp, d);
+    if (skip) return true;
     f.sflags_next[c] = fc;
     /* forces are non-zero only on the cells the previous step collided, unless the step before
      * took them from the fields (then force_reset_all) */
@@ -940,6 +1055,7 @@ __device__ __forceinline__ void stream_cell(const waam_step_fields &f, const Ste
         fin = freeze_cell(f, s, i, j, k, c, fc, T, phi, fl, step_arc_k(p, tot));
     }
     list_append(is_surface_class(fin), c, cell_list(f, s.seg_len, 0), d.n());
+    return fc == GAS && phi == 0.0f;  // the freeze rules never leave a cell GAS
 }
 
 /* kernel 7 without the pull: a lane takes kStreamSlices cells along the slowest axis and issues
@@ -948,7 +1064,7 @@ __device__ __forceinline__ void stream_cell(const waam_step_fields &f, const Ste
  * dependent memory round trips of each cell */
 constexpr int kStreamSlices = 4;
 
-__device__ __forceinline__ void stream_rest_cell(const waam_step_fields &f, const StepConsts &s,
+__device__ __forceinline__ bool stream_rest_cell(const waam_step_fields &f, const StepConsts &s,
                                                  int i, int j, int k, size_t c, uint8_t fc,
                                                  uint8_t fp, float T, float phi, float fl)
 {
@@ -967,47 +1083,71 @@ __device__ __forceinline__ void stream_rest_cell(const waam_step_fields &f, cons
         fin = freeze_cell(f, s, i, j, k, c, fc, T, phi, fl, step_arc_k(p, tot));
     }
     list_append(is_surface_class(fin), c, cell_list(f, s.seg_len, 0), s.d.n());
+    return fc == GAS && phi == 0.0f;  // the freeze rules never leave a cell GAS
 }
 
 template <class E, bool PULL>
 __global__ void __launch_bounds__(256) k_step_stream(waam_step_fields f, StepConsts s)
 {
     const Dims &d = s.d;
+    const QuietRec q = quiet_rec(f, d);
+    const bool cont = s.p.quiet_skip != 0;
     int i, j, k;
     if (PULL) {
-        if (!d.cell(i, j, k)) return;
-        stream_cell<E, true>(f, s, i, j, k);
+        const bool in = d.cell(i, j, k);
+        if (j >= d.ny) return;  // the wavefront's whole row is past the grid
+        const size_t seg = seg_index(q, (int)blockIdx.x, j, (int)blockIdx.z, d);
+        const bool skip = cont && q.skip[seg] != 0;
+        bool quiet = true;
+        if (in) quiet = stream_cell<E, true>(f, s, i, j, k, skip);
+        record_quiet(q, seg, quiet, cont);
         return;
     }
     const int a = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
-    if (a >= d.fast() || j >= d.ny) return;
+    if (j >= d.ny) return;  // the wavefront's whole row is past the grid
+    const bool in = a < d.fast();
     const bool zyx = d.layout == kLayoutZYX;
     const int b0 = (int)blockIdx.z * kStreamSlices;
     const bool freeze_on = s.p.enable_freeze != 0;
-    size_t c[kStreamSlices];
+    size_t c[kStreamSlices], seg[kStreamSlices];
+    bool skip[kStreamSlices];
     uint8_t fc[kStreamSlices], fp[kStreamSlices];
     float T[kStreamSlices], phi[kStreamSlices], fl[kStreamSlices];
 #pragma unroll
     for (int t = 0; t < kStreamSlices; ++t) {
         const int b = imin(b0 + t, d.slow() - 1);
-        c[t] = zyx ? d.idx(a, j, b) : d.idx(b, j, a);
-        fc[t] = f.flags_out[c[t]];
-        fp[t] = f.sflags_prev[c[t]];
+        seg[t] = seg_index(q, (int)blockIdx.x, j, b, d);
+        skip[t] = cont && q.skip[seg[t]] != 0;
     }
 #pragma unroll
     for (int t = 0; t < kStreamSlices; ++t) {
-        const bool fr = freeze_on && fc[t] != GAS;
+        const int b = imin(b0 + t, d.slow() - 1);
+        c[t] = zyx ? d.idx(in ? a : 0, j, b) : d.idx(b, j, in ? a : 0);
+        const bool ld = in && !skip[t];
+        fc[t] = ld ? f.flags_out[c[t]] : (uint8_t)GAS;
+        fp[t] = ld ? f.sflags_prev[c[t]] : (uint8_t)GAS;
+        phi[t] = ld ? f.phi[c[t]] : 0.0f;
+    }
+#pragma unroll
+    for (int t = 0; t < kStreamSlices; ++t) {
+        const bool fr = in && !skip[t] && freeze_on && fc[t] != GAS;
         T[t] = fr ? f.T_out[c[t]] : 0.0f;
-        phi[t] = fr ? f.phi[c[t]] : 0.0f;
         fl[t] = fr ? f.fl_out[c[t]] : 0.0f;
     }
 #pragma unroll
     for (int t = 0; t < kStreamSlices; ++t) {
         const int b = b0 + t;
         if (b >= d.slow()) break;
-        stream_rest_cell(f, s, zyx ? a : b, j, zyx ? b : a, c[t], fc[t], fp[t], T[t], phi[t],
-                         fl[t]);
+        if (skip[t]) {
+            if (threadIdx.x == 0) q.bits[seg[t]] = 3u;  // quiet, as it was for two steps
+            continue;
+        }
+        bool quiet = true;
+        if (in)
+            quiet = stream_rest_cell(f, s, zyx ? a : b, j, zyx ? b : a, c[t], fc[t], fp[t], T[t],
+                                     phi[t], fl[t]);
+        record_quiet(q, seg[t], quiet, cont);
     }
 }
 
@@ -1101,13 +1241,13 @@ __global__ void __launch_bounds__(256) k_step_pull_pair16(waam_step_fields f, St
             }
         }
     }
-    /* bounce-back off SOLID neighbours for a non-GAS destination (a cell that is neither GAS nor
-     * collided this step: rare) */
+    /* bounce-back off SOLID neighbours for a destination that was not GAS at the step whose
+     * stream this is (its flag then, as every other flag of that stream) */
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const bool nh = h == 0 ? need0 : need1;
-        const uint8_t fch = h == 0 ? fc0 : fc1;
-        if (!nh || fch == GAS) continue;
+        const uint8_t fph = h == 0 ? fp0 : fp1;
+        if (!nh || fph == GAS) continue;
         const int a = a0 + h;
         const int i = zyx ? a : b, k = zyx ? b : a;
         const size_t c = c0 + (size_t)h;
@@ -1128,6 +1268,113 @@ __global__ void __launch_bounds__(256) k_step_pull_pair16(waam_step_fields f, St
         if (need0 && need1) *reinterpret_cast<uint32_t *>(o) = v[q];
         else if (need0) o[0] = (uint16_t)(v[q] & 0xffffu);
         else if (need1) o[1] = (uint16_t)(v[q] >> 16);
+    }
+}
+
+/* 7a on 8-byte words — the pull with every lane moving one 64-bit word per direction: two fp32
+ * cells (EB = 32) or four 16-bit cells (EB = 16) along the contiguous axis, so each of the 19
+ * loads and stores of a lane is a dwordx2 of a coalesced row. The neighbouring word's edge
+ * element comes from the adjacent lane (one 32-bit shuffle per direction with a contiguous-axis
+ * component); lanes whose neighbour element lies in another wavefront or across the domain edge
+ * load that one element directly. Values are those of the cell-by-cell pull. Needs a contiguous
+ * extent divisible by 64 / EB (the host launches the narrower pulls otherwise). */
+template <int EB>
+__global__ void __launch_bounds__(256) k_step_pull_x2(waam_step_fields f, StepConsts s)
+{
+    constexpr int CPL = 64 / EB;  // cells per lane
+    using Elem = typename std::conditional<EB == 32, uint32_t, uint16_t>::type;
+    constexpr uint64_t kMask = EB == 32 ? 0xffffffffull : 0xffffull;
+    const Dims &d = s.d;
+    const int lane = (int)threadIdx.x;
+    const int nw = d.fast() / CPL;
+    const int w = (int)(blockIdx.x * blockDim.x) + lane;
+    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    const int b = (int)blockIdx.z;
+    if (w >= nw || j >= d.ny) return;
+    const bool zyx = d.layout == kLayoutZYX;
+    const int a0 = CPL * w;
+    const size_t N = d.n();
+    const size_t c0 = zyx ? d.idx(a0, j, b) : d.idx(b, j, a0);
+    /* the CPL cells' flags in one load each */
+    const uint32_t fcw = CPL == 4 ? *reinterpret_cast<const uint32_t *>(f.flags_out + c0)
+                                  : *reinterpret_cast<const uint16_t *>(f.flags_out + c0);
+    const uint32_t fpw = CPL == 4 ? *reinterpret_cast<const uint32_t *>(f.sflags_prev + c0)
+                                  : *reinterpret_cast<const uint16_t *>(f.sflags_prev + c0);
+    uint8_t fph[CPL];
+    bool need[CPL];
+    bool any = false, all = true;
+#pragma unroll
+    for (int h = 0; h < CPL; ++h) {
+        const uint8_t fch = (uint8_t)((fcw >> (8 * h)) & 0xffu);
+        fph[h] = (uint8_t)((fpw >> (8 * h)) & 0xffu);
+        need[h] = !is_surface_class(fch) && fph[h] != SOLID;
+        any = any || need[h];
+        all = all && need[h];
+    }
+    if (__ballot(any) == 0ull) return;
+    const Elem *src = static_cast<const Elem *>(f.f_src);
+    Elem *dst = static_cast<Elem *>(f.f_dst);
+    /* the row each direction pulls from (the other two axes' neighbours) */
+    auto row_of = [&](int q) {
+        int nj = imax(0, imin(j - ey(q), d.ny - 1));
+        int nb = zyx ? imax(0, imin(b - ez(q), d.nz - 1))
+                     : (b - ex(q) < 0 ? b - ex(q) + d.nx
+                                      : (b - ex(q) >= d.nx ? b - ex(q) - d.nx : b - ex(q)));
+        return src + (size_t)q * N + (zyx ? d.idx(0, nj, nb) : d.idx(nb, nj, 0));
+    };
+    /* every load first, then the shuffles: all 19 words in flight at once */
+    uint64_t v[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) v[q] = reinterpret_cast<const uint64_t *>(row_of(q))[w];
+    const bool first = lane == 0 || w == 0, last = lane == 63 || w == nw - 1;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int sh = zyx ? ex(q) : ez(q);
+        if (sh == 0) continue;
+        const uint64_t aw = v[q];
+        if (sh > 0) {
+            /* cell a0 takes the element below it: the top element of the lower word */
+            const uint32_t o = (uint32_t)__shfl((int)(uint32_t)(aw >> 32), lane - 1);
+            uint64_t e = (uint64_t)(o >> (32 - EB));
+            if (first) e = (uint64_t)row_of(q)[fast_neighbour(a0, sh, d)];
+            v[q] = (aw << EB) | e;
+        } else {
+            /* cell a0 + CPL - 1 takes the element above it: the bottom one of the upper word */
+            const uint32_t o = (uint32_t)__shfl((int)(uint32_t)aw, lane + 1);
+            uint64_t e = (uint64_t)o & kMask;
+            if (last) e = (uint64_t)row_of(q)[fast_neighbour(a0 + CPL - 1, sh, d)];
+            v[q] = (aw >> EB) | (e << (64 - EB));
+        }
+    }
+    /* bounce-back off SOLID neighbours for a destination that was not GAS at the step whose
+     * stream this is (its flag then, as every other flag of that stream) */
+#pragma unroll
+    for (int h = 0; h < CPL; ++h) {
+        if (!need[h] || fph[h] == GAS) continue;
+        const int a = a0 + h;
+        const int i = zyx ? a : b, k = zyx ? b : a;
+        const size_t c = c0 + (size_t)h;
+        for (int q = 0; q < Q; ++q) {
+            int ni = i - ex(q), nj = j - ey(q), nk = k - ez(q);
+            ni = ni < 0 ? ni + d.nx : (ni >= d.nx ? ni - d.nx : ni);
+            nj = imax(0, imin(nj, d.ny - 1));
+            nk = imax(0, imin(nk, d.nz - 1));
+            if (f.sflags_prev[d.idx(ni, nj, nk)] == SOLID) {
+                const uint64_t r = (uint64_t)src[(size_t)opp(q) * N + c];
+                v[q] = (v[q] & ~(kMask << (EB * h))) | (r << (EB * h));
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        Elem *o = dst + (size_t)q * N + c0;
+        if (all) {
+            *reinterpret_cast<uint64_t *>(o) = v[q];
+        } else {
+#pragma unroll
+            for (int h = 0; h < CPL; ++h)
+                if (need[h]) o[h] = (Elem)((v[q] >> (EB * h)) & kMask);
+        }
     }
 }
 
@@ -1233,7 +1480,7 @@ static int check_fields(const char *who, const waam_step_fields &f)
                                     f.T_in, f.T_out, f.fl_in, f.fl_out, f.phi, f.phi_tmp,
                                     f.flags_in, f.flags_out, f.sflags_prev, f.sflags_next, f.Fx,
                                     f.Fy, f.Fz, f.T_max, f.T_prev, f.dT_dt, f.t800, f.t1100, f.tsol,
-                                    f.list_thermal, f.list_collide, f.scratch}));
+                                    f.list_thermal, f.list_collide, f.scratch, f.quiet}));
     if (f.f_src == f.f_dst || f.H_in == f.H_out || f.T_in == f.T_out || f.fl_in == f.fl_out ||
         f.flags_in == f.flags_out || f.sflags_prev == f.sflags_next || f.phi == f.phi_tmp ||
         f.list_thermal == f.list_collide)
@@ -1254,6 +1501,11 @@ static StepConsts make_step_consts(const waam_step_params &p, const Dims &d)
     StepConsts s{};
     s.p = p;
     s.d = d;
+    /* the quiet record is continued only on steady steps (see waam_step_params.quiet_skip) */
+    s.p.quiet_skip = (p.quiet_skip && p.pending_stream == 1 && p.thermal_list_ready != 0 &&
+                      !p.arc_applied && !p.forces_from_fields && !p.force_reset_all)
+                         ? 1
+                         : 0;
     s.arc.kind = p.arc_kind;
     s.arc.arc_i = p.arc_i;
     s.arc.arc_j = p.arc_j;
@@ -1389,10 +1641,17 @@ static void launch_step(const StepConsts &s, const waam_step_fields &f, hipStrea
      * storage on an even contiguous extent), or inside kernel 7 */
     const bool f32 = f.pop_dtype == WAAM_POP_F32;
     const bool pair16 = !f32 && d.fast() % 2 == 0;
+    /* 8-byte words per lane: two fp32 cells, or four 16-bit cells */
+    const bool wide = kWidePull && (f32 ? d.fast() % 2 == 0 : d.fast() % 4 == 0);
     const bool separate = pair16 || (f32 ? kSeparatePull32 : kSeparatePull16);
     if (separate && p.pending_stream)
         timed_launch(WAAM_K_PULL, [&](hipEvent_t a, hipEvent_t b) {
-            if (pair16) {
+            if (wide) {
+                const int cpl = f32 ? 2 : 4;
+                dim3 gp((unsigned)((d.fast() / cpl + kTileFast - 1) / kTileFast), g3.y, g3.z);
+                if (f32) launch_k(capture, k_step_pull_x2<32>, gp, block3(), st, a, b, f, s);
+                else launch_k(capture, k_step_pull_x2<16>, gp, block3(), st, a, b, f, s);
+            } else if (pair16) {
                 dim3 gp((unsigned)((d.fast() / 2 + kTileFast - 1) / kTileFast), g3.y, g3.z);
                 launch_k(capture, k_step_pull_pair16, gp, block3(), st, a, b, f, s);
             } else if (f32) {
@@ -1465,6 +1724,12 @@ int waam_step_fused(const waam_step_params *pp, const waam_step_fields *ff, void
     return check_launch("waam_step_fused");
 }
 
+int64_t waam_quiet_bytes(int nx, int ny, int nz, int layout)
+{
+    if (nx <= 0 || ny <= 0 || nz <= 0 || (layout != kLayoutXYZ && layout != kLayoutZYX)) return 0;
+    return 2 * (int64_t)quiet_segments(Dims{nx, ny, nz, layout});
+}
+
 int waam_graph_capture(const waam_step_params *pp, const waam_step_fields *f_even,
                        const waam_step_fields *f_odd, int n_steps, waam_step_graph **out)
 {
@@ -1496,7 +1761,7 @@ int waam_graph_capture(const waam_step_params *pp, const waam_step_fields *f_eve
         b.flags_in != a.flags_out || b.flags_out != a.flags_in ||
         b.sflags_prev != a.sflags_next || b.sflags_next != a.sflags_prev || b.phi != a.phi ||
         b.phi_tmp != a.phi_tmp || b.list_thermal != a.list_thermal ||
-        b.list_collide != a.list_collide || b.scratch != a.scratch)
+        b.list_collide != a.list_collide || b.scratch != a.scratch || b.quiet != a.quiet)
         return set_error(WAAM_ERR_INVALID,
                          "waam_graph_capture: the odd field set is not the even set swapped");
     if (p.col_i < 0 || p.col_i >= a.nx || p.col_j < 0 || p.col_j >= a.ny)

@@ -47,7 +47,7 @@ class StepFields(ctypes.Structure):
         "f_src", "f_dst", "rho", "ux", "uy", "uz", "H_in", "H_out", "T_in", "T_out", "fl_in",
         "fl_out", "phi", "phi_tmp", "flags_in", "flags_out", "sflags_prev", "sflags_next", "Fx",
         "Fy", "Fz", "T_max", "T_prev", "dT_dt", "t800", "t1100", "tsol", "tracer_pos",
-        "tracer_active", "list_thermal", "list_collide", "scratch")] + [
+        "tracer_active", "list_thermal", "list_collide", "quiet", "scratch")] + [
             (n, _i32) for n in ("max_tracers", "nx", "ny", "nz", "layout", "pop_dtype")]
 
 # population storage forms (WAAM_POP_*): fp32, or f - w_q in binary16 / bfloat16
@@ -60,7 +60,8 @@ _INT_PARAMS = ("is_welding", "collision", "arc_kind", "arc_applied", "pending_st
                "enable_heat_loss", "enable_conv", "enable_rad", "enable_cap", "enable_hydro",
                "enable_freeze", "enable_trailing", "zero_velocity", "surface_weight", "nz_solid",
                "col_i", "col_j", "use_arc_k_override", "use_k_robot", "force_reset_all",
-               "thermal_list_ready", "enable_recoil", "recoil_clausius", "enable_gas_shear")
+               "thermal_list_ready", "enable_recoil", "recoil_clausius", "enable_gas_shear",
+               "quiet_skip")
 _FLOAT_PARAMS = ("arc_i", "arc_j", "arc_k_override", "k_robot", "Q_w", "eta", "dt", "dx", "dx3",
                  "sigma", "pen", "g_sgn", "g_ff", "g_fr", "g_af", "g_ar", "g_b", "g_c",
                  "alpha_lu", "cp_rho", "L_rho", "T_s", "T_l", "T_cap", "T_amb", "h_conv",
@@ -102,6 +103,14 @@ class FusedStepper:
             # compacted FLUID/IFACE cell indices written and consumed inside each fused step
             g.list_thermal = DeviceField.zeros((g.n_cells,), torch.int32, g.device)
             g.list_collide = DeviceField.zeros((g.n_cells,), torch.int32, g.device)
+        if not hasattr(g, "quiet"):
+            # which row segments ended the last two steps quiet (all GAS, phi 0), kept by the
+            # fused step itself
+            nq = lib().quiet_bytes(g.nx, g.ny, g.nz, g.layout)
+            g.quiet = DeviceField.zeros((nq,), torch.uint8, g.device)
+        # skip the sweeps on quiet row segments (same results; the library honours it on
+        # steady steps only)
+        self.quiet_skip = True
         self.pending = False
         # the last fused step took its forces from the fields (an eager-arc step): the next one
         # clears every cell's forces, not only those of the previously collided cells
@@ -197,7 +206,7 @@ class FusedStepper:
                           ("t1100", g.time_above_1100_s), ("tsol", g.time_above_solidus_s),
                           ("tracer_pos", g.porosity_pos), ("tracer_active", g.porosity_active),
                           ("list_thermal", g.list_thermal), ("list_collide", g.list_collide),
-                          ("scratch", g.scratch)):
+                          ("quiet", g.quiet), ("scratch", g.scratch)):
             setattr(sf, name, P(fld))
         sf.max_tracers, sf.nx, sf.ny, sf.nz = g.max_tracers, g.nx, g.ny, g.nz
         sf.layout = g.layout
@@ -267,6 +276,7 @@ class FusedStepper:
         p.L_vap, p.R_vap = tw.L_vapor_J_kg, tw.R_spec_vapor_J_kgK
         p.C_acc = float(getattr(tw, "recoil_accommodation", 0.54))
         p.gas_tau = weld_forces.gas_shear_tau_pa(tw)
+        p.quiet_skip = 1 if self.quiet_skip else 0
         return p
 
     def _k_robot(self, torch_z_m) -> float | None:
