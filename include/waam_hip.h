@@ -586,6 +586,20 @@ typedef struct waam_step_params {
 int waam_step_fused(const waam_step_params *p, const waam_step_fields *f, void *stream);
 /* bytes of the quiet record (waam_step_fields.quiet) for a grid; 0 for invalid sizes */
 int64_t waam_quiet_bytes(int nx, int ny, int nz, int layout);
+/* Allocation-free variant of waam_step_fused (no counterpart in the reference): for callers that
+ * cannot allocate device memory (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Same kernels and results as waam_step_fused; f->scratch, f->list_thermal
+ * and f->list_collide are ignored, and a device array inside the library takes their place
+ * (workspace plus two cell lists of waam_noalloc_max_cells() entries: 16.0 MiB + 0.7 MiB per
+ * GPU, held from the first call on that GPU until the process ends). Grids larger than
+ * waam_noalloc_max_cells() cells are rejected with WAAM_ERR_INVALID. The array keeps the last
+ * call's contents: state that waam_step_fused carries in the workspace from one call to the next
+ * (the thermal list when thermal_list_ready is set) is carried in it, so the first call after a
+ * waam_step_fused call, or after a call on another grid, must pass thermal_list_ready 0. Every
+ * call on one GPU shares its array: calls that overlap on one GPU (two streams, two threads)
+ * race. Calls no hipMalloc, hipFree or allocating library routine. */
+int waam_step_fused_noalloc(const waam_step_params *p, const waam_step_fields *f, void *stream);
+int64_t waam_noalloc_max_cells(void);
 /* Materialise a deferred stream: f_dst = pull(f_src) on cells not SOLID in sflags (the flags the
  * step that deferred it saw); populations in the WAAM_POP_* form pop_dtype. */
 int waam_stream_pending(const void *f_src, void *f_dst, const uint8_t *sflags, int nx, int ny,

@@ -74,6 +74,7 @@ SIGNATURES: dict[str, str] = {
     "waam_advect_tracers": "pppppppffiiiiip",
     "waam_telemetry_pool_reduce": "ppppppppiiiipp",
     "waam_step_fused": "ppp",
+    "waam_step_fused_noalloc": "ppp",
     "waam_stream_pending": "pppiiiiip",
     "waam_pack_populations": "pplip",
     "waam_unpack_populations": "pplip",
@@ -145,6 +146,7 @@ class _Library:
         self.dll.waam_abi_version.restype = ctypes.c_int
         self.dll.waam_last_error.restype = ctypes.c_char_p
         self.dll.waam_scratch_bytes.restype = ctypes.c_int64
+        self.dll.waam_noalloc_max_cells.restype = ctypes.c_int64
         self.dll.waam_quiet_bytes.restype = ctypes.c_int64
         self.dll.waam_quiet_bytes.argtypes = [ctypes.c_int] * 4
         ver = self.dll.waam_abi_version()
@@ -184,5 +186,5 @@ def lib() -> _Library:
 
 
 def exported_symbols() -> list[str]:
-    return ["waam_abi_version", "waam_last_error", "waam_scratch_bytes", "waam_quiet_bytes",
-            *SIGNATURES]
+    return ["waam_abi_version", "waam_last_error", "waam_scratch_bytes", "waam_noalloc_max_cells",
+            "waam_quiet_bytes", *SIGNATURES]

@@ -42,6 +42,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -1724,10 +1725,80 @@ int waam_step_fused(const waam_step_params *pp, const waam_step_fields *ff, void
     return check_launch("waam_step_fused");
 }
 
+}  // extern "C"
+
+namespace waam {
+
+/* Allocation-free step: the workspace and the two cell lists of waam_step_fused_noalloc live in
+ * this file-scope device array (each GPU that runs the entry holds its own copy, with its own
+ * address, until the process ends): the workspace first, then the thermal list, then the
+ * collision list. */
+constexpr size_t kNoAllocMaxCells = (size_t)1 << 22;
+constexpr size_t kNoAllocListOffset = (kScratchBytes + 255) / 256 * 256;
+constexpr size_t kNoAllocBytes = kNoAllocListOffset + 2 * kNoAllocMaxCells * sizeof(int32_t);
+__device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kNoAllocMaxDevices = 64;
+static std::mutex g_noalloc_mu;
+static void *g_noalloc_ptr[kNoAllocMaxDevices];
+
+/* the device address of `scratch` on the current device, looked up once per device */
+static int noalloc_scratch(void **out)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess)
+        return set_error(WAAM_ERR_HIP, "waam_step_fused_noalloc: %s", hipGetErrorString(e));
+    if (dev < 0 || dev >= kNoAllocMaxDevices)
+        return set_error(WAAM_ERR_INVALID, "waam_step_fused_noalloc: device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void *ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return set_error(WAAM_ERR_HIP, "waam_step_fused_noalloc: device array lookup: %s",
+                             hipGetErrorString(e));
+        g_noalloc_ptr[dev] = ptr;
+    }
+    *out = g_noalloc_ptr[dev];
+    return WAAM_OK;
+}
+
+}  // namespace waam
+
+extern "C" {
+
+int64_t waam_noalloc_max_cells(void) { return (int64_t)kNoAllocMaxCells; }
+
 int64_t waam_quiet_bytes(int nx, int ny, int nz, int layout)
 {
     if (nx <= 0 || ny <= 0 || nz <= 0 || (layout != kLayoutXYZ && layout != kLayoutZYX)) return 0;
     return 2 * (int64_t)quiet_segments(Dims{nx, ny, nz, layout});
+}
+
+int waam_step_fused_noalloc(const waam_step_params *pp, const waam_step_fields *ff, void *stream)
+{
+    if (!pp || !ff) return set_error(WAAM_ERR_INVALID, "waam_step_fused_noalloc: NULL argument");
+    const waam_step_params &p = *pp;
+    const int64_t n = (int64_t)ff->nx * ff->ny * ff->nz;
+    if (ff->nx <= 0 || ff->ny <= 0 || ff->nz <= 0 || n > (int64_t)kNoAllocMaxCells)
+        return set_error(WAAM_ERR_INVALID,
+                         "waam_step_fused_noalloc: grid %d x %d x %d (%lld cells) outside 1..%lld "
+                         "cells, the size of its device work array; use waam_step_fused",
+                         ff->nx, ff->ny, ff->nz, (long long)n, (long long)kNoAllocMaxCells);
+    void *base = nullptr;
+    WAAM_RETURN_IF(noalloc_scratch(&base));
+    waam_step_fields f = *ff;
+    f.scratch = base;
+    f.list_thermal = reinterpret_cast<int32_t *>((char *)base + kNoAllocListOffset);
+    f.list_collide = f.list_thermal + kNoAllocMaxCells;
+    WAAM_RETURN_IF(check_fields("waam_step_fused_noalloc", f));
+    if (p.col_i < 0 || p.col_i >= f.nx || p.col_j < 0 || p.col_j >= f.ny)
+        return set_error(WAAM_ERR_INVALID, "waam_step_fused_noalloc: arc column outside the grid");
+    Dims d{f.nx, f.ny, f.nz, f.layout};
+    StepConsts s = make_step_consts(p, d);
+    launch_step(s, f, as_stream(stream), false);
+    return check_launch("waam_step_fused_noalloc");
 }
 
 int waam_graph_capture(const waam_step_params *pp, const waam_step_fields *f_even,

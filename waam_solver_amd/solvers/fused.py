@@ -127,6 +127,9 @@ class FusedStepper:
         if form not in POP_FORMS:
             raise ValueError(f"population_storage {form!r} (fp32 | fp16 | bf16)")
         self.pop_form, self.pop_code = form, POP_FORMS[form]
+        # the C entry of a direct step: "waam_step_fused", or "waam_step_fused_noalloc" (the
+        # library's own device work array instead of the grid's workspace and cell lists)
+        self.entry = "waam_step_fused"
         # the 16-bit pair holds the populations (the fp32 buffers are stale) between a fused
         # step and the next flush
         self.h_live = False
@@ -331,7 +334,7 @@ class FusedStepper:
         if self.pop_code and not self.h_live:
             self._pack()
         fields = self._fields()
-        lib().call("waam_step_fused", ctypes.addressof(p), ctypes.addressof(fields),
+        lib().call(self.entry, ctypes.addressof(p), ctypes.addressof(fields),
                    torch.cuda.current_stream().cuda_stream)
         for a, b in (("H", "H_tmp"), ("T", "T_tmp"), ("f_l", "fl_tmp"), ("flags", "flags_tmp"),
                      ("sflags", "sflags_tmp")):
