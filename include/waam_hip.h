@@ -528,8 +528,9 @@ typedef struct waam_step_fields {
     /* two int32 work lists of nx*ny*nz entries each (compacted FLUID/IFACE cell indices) */
     int32_t *list_thermal, *list_collide;
     /* waam_quiet_bytes(nx, ny, nz, layout) bytes: per row segment (the 64 cells of one
-     * wavefront along the contiguous axis) whether it ended the last two steps quiet, and
-     * whether the current step skips it; written by every step, contents need no initial value */
+     * wavefront along the contiguous axis) whether it ended the last two steps quiet, and the
+     * list of the segments a steady step sweeps; written by every step, contents need no
+     * initial value */
     uint8_t *quiet;
     void *scratch;
     int32_t max_tracers, nx, ny, nz;
@@ -558,13 +559,13 @@ typedef struct waam_step_params {
     /* full-tier surface forces (physics/weld_forces.py:120-161): vapour recoil (clausius: the
      * Clausius-Clapeyron form, else the constant-pressure form) and shielding-gas shear */
     int32_t enable_recoil, recoil_clausius, enable_gas_shear;
-    /* 1: skip the cell sweeps of kernels 2, 4 and 7 on row segments whose results cannot change
-     * (every cell the segment's stencils reach ended the previous step GAS with phi exactly 0,
-     * and the segment itself ended the step before quiet too): their fields already hold those
-     * results in both buffers of every pair. Valid only when the previous two steps were
-     * waam_step_fused calls on this field set with nothing written in between; honoured only on
-     * steady steps (pending_stream 1, thermal_list_ready 1 or 2, arc_applied 0,
-     * forces_from_fields 0, force_reset_all 0). 0: every segment is swept and the quiet record
+    /* 1: run the cell sweeps of kernels 2, 4 and 7 only on the row segments whose results can
+     * change: a segment is left out when every cell its stencils reach ended the previous step
+     * GAS with phi exactly 0 and the segment itself ended the step before so too (both buffers
+     * of every pair then hold its results already). Honoured only on steady steps
+     * (pending_stream 1, thermal_list_ready 1 or 2, arc_applied 0, forces_from_fields 0,
+     * force_reset_all 0), which a caller may pass only when nothing was written between this
+     * step and the waam_step_fused call before. 0: every segment is swept and the quiet record
      * restarts. The results are the same either way. */
     int32_t quiet_skip;
     /* arc */

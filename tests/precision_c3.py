@@ -1,12 +1,14 @@
 """The precision forms at full size: the bead-on-plate on 512x256x128 cells for 2000 steps in
 fp32, binary16 and bfloat16 population storage (GPU) against the binary64 build of the same
-kernels (the oracle compiled with every fp32 variable a double, on the CPU).
+kernels (the oracle compiled with every fp32 variable a double, on the CPU). A molten dome of
+radius 8 cells is seeded under the torch's start (as in the 4000-step 256x128x64 check): from a
+cold plate the arc melts nothing in 2000 steps (peak 758 K), and without a pool nothing flows.
 
 Two steps, each a command:
 
   python tests/precision_c3.py make-ref DIR     binary64 oracle run (CPU, OpenMP, Jacobi mode)
-                                                 -> DIR/{T,phi,f_l,speed}.npy (stored as fp32)
-                                                 and DIR/ref.json (pool width/depth, ledger)
+                                                 -> DIR/ref_fields.npz (T, phi, f_l, |u| as fp32)
+                                                 and DIR/ref.json (pool width/depth)
   python tests/precision_c3.py measure DIR      the three GPU forms from the same seeded state
                                                  over the same steps -> per-field relative L2
                                                  against DIR, bead width/depth, MLUPS; one JSON
@@ -40,6 +42,7 @@ def workload() -> dict:
 
 def make_ref(out_dir: str) -> int:
     import bench
+    from tests.golden.make_c2_telemetry import seed_state
     from oracle.oracle import Material, OracleTwin
     from waam_solver_amd.materials import load_material
 
@@ -54,18 +57,20 @@ def make_ref(out_dir: str) -> int:
     ot.reset()
     g = ot.g
     # the seeded state in fp32, as the GPU twin holds it, then widened
-    T, H = g.T.astype(np.float32), g.H.astype(np.float32)
-    bench.seed_noise(T, H, g.flags, np.float32(ot.cp_rho))
-    g.T[...] = T
-    g.H[...] = H
+    arrs = [g.T.astype(np.float32), g.f_l.astype(np.float32), g.phi.astype(np.float32),
+            g.flags.copy(), g.H.astype(np.float32)]
+    seed_state(*arrs, ot.nz_solid, mat.T_liquidus, ot.cp_rho, ot.L_rho, w)
+    for n, a in zip(("T", "f_l", "phi", "flags", "H"), arrs):
+        getattr(g, n)[...] = a
     t0 = time.perf_counter()
     for s in range(STEPS):
         ot.step(*bench.torch_xy(w, s, g.dt), True)
         if (s + 1) % 100 == 0:
             print(f"[c3-ref] step {s + 1} ({time.perf_counter() - t0:.0f} s)", flush=True)
     speed = np.sqrt(g.ux.astype(np.float64) ** 2 + g.uy ** 2 + g.uz ** 2)
-    for name, arr in (("T", g.T), ("phi", g.phi), ("f_l", g.f_l), ("speed", speed)):
-        np.save(os.path.join(out_dir, f"{name}.npy"), np.asarray(arr, np.float32))
+    np.savez_compressed(os.path.join(out_dir, "ref_fields.npz"),
+                        **{name: np.asarray(arr, np.float32) for name, arr in
+                           (("T", g.T), ("phi", g.phi), ("f_l", g.f_l), ("speed", speed))})
     pm = ot.pool_metrics()
     with open(os.path.join(out_dir, "ref.json"), "w") as fh:
         json.dump({"grid": [w["nx"], w["ny"], w["nz"]], "steps": STEPS,
@@ -85,15 +90,27 @@ def measure(ref_dir: str) -> int:
     with open(os.path.join(ref_dir, "ref.json")) as fh:
         ref = json.load(fh)
     dev = torch.device("cuda:0")
-    refs = {k: torch.from_numpy(np.load(os.path.join(ref_dir, f"{k}.npy"))).to(dev).double()
-            for k in ("T", "phi", "f_l", "speed")}
-    out = {"workload": "bead-on-plate 512x256x128 dx=0.2mm, 2000 steps (bench.py physics)",
+    with np.load(os.path.join(ref_dir, "ref_fields.npz")) as z:
+        refs = {k: torch.from_numpy(z[k]).to(dev).double() for k in ("T", "phi", "f_l", "speed")}
+    out = {"workload": ("bead-on-plate 512x256x128 dx=0.2mm with a seeded molten dome, "
+                        "2000 steps (bench.py physics)"),
            "fp64": {k: ref[k] for k in ("pool_width_m", "pool_depth_m", "n_liquid_cells",
                                         "T_peak", "n_droplets")}}
+    from tests.golden.make_c2_telemetry import seed_state
+    from waam_solver_amd.twin import WAAMTwin
+
     for pop in ("fp32", "fp16", "bf16"):
         print(f"[c3] {pop}", flush=True)
-        tw = bench.build_gpu_twin(w, pop)
+        tw = WAAMTwin(material="LCS-bead-on-plate", population_storage=pop,
+                      **bench.twin_kwargs(w))
+        bench.configure(tw, w)
+        tw.reset()
         g = tw.grid
+        arrs = {n: getattr(g, n).to_numpy() for n in ("T", "f_l", "phi", "flags", "H")}
+        seed_state(arrs["T"], arrs["f_l"], arrs["phi"], arrs["flags"], arrs["H"], tw.nz_solid,
+                   tw.mat.T_liquidus, tw.cp_rho, tw.L_rho, w)
+        for n, a in arrs.items():
+            getattr(g, n).from_numpy(a)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for s in range(STEPS):

@@ -162,3 +162,161 @@ def test_forces_adjoint_against_binary64_differences():
         print(f"[cadj] forces/{kind}: adjoint {adj:.10e} differences {fd:.10e}", flush=True)
         # the differences carry the binary64 rounding of L over 2h besides their truncation
         assert abs(adj - fd) <= 1e-6 * abs(fd) + 20 * 2.2e-16 * L0 / h, (kind, adj, fd)
+
+
+# ---------------------------------------------------------------------------- through the rollout
+THETA0 = (0.8, 1.0, 1.0)
+THETA_TARGET = (0.75, 1.0, 1.1)
+
+
+def _rollout_case(nx, ny, nz, preheat_K=1650.0, warm_steps=10):
+    """The seeded-pool bead case on an (nx, ny, nz) grid in the reference layout, its plate
+    preheated to just below the solidus (so the arc melts within a few hundred steps and the
+    pool's width and depth respond to the parameters), after warm_steps coupled steps (a flow
+    field in the pool)."""
+    from tests.step_cases import build_pair, torch_x
+
+    tw, _ = build_pair("bead_mrt", layout=0, nx=nx, ny=ny, nz=nz)
+    g = tw.grid
+    T, H, fl = g.T.to_numpy(), g.H.to_numpy(), g.flags.to_numpy()
+    plate = fl == 1
+    T[plate] = preheat_K
+    H[plate] = np.float32(tw.cp_rho) * np.float32(preheat_K)
+    g.T.from_numpy(T)
+    g.H.from_numpy(H)
+    for s in range(warm_steps):
+        tw.step(torch_x(g.dx, g.nx, s, g.dt), (g.ny // 2) * g.dx, True)
+    tw.sync()
+    path = lambda n: (torch_x(g.dx, g.nx, warm_steps + n, g.dt), (g.ny // 2) * g.dx)  # noqa
+    return tw, path
+
+
+def _oracle_coupled(tw, ro, theta, f64=True):
+    """The rollout's forward on the oracle (binary64 build by default) from its starting state:
+    the same thermal chain, advection by the rollout's velocity rule, and (advection "flow")
+    the same forces, MRT collision and Mach clamp. Returns the final f_l in (nx, ny, nz) order."""
+    from oracle.oracle import cos_sin_f32, lib, ptr
+    from waam_solver_amd.field import tensor_of
+    from waam_solver_amd.physics import weld_forces
+
+    g = tw.grid
+    dt_np = np.float64 if f64 else np.float32
+    L = lib(parallel=True, f64=f64)
+    d = (g.nx, g.ny, g.nz)
+    m = tw.mat
+    H = ro.H0.cpu().numpy().astype(dt_np)
+    T, fl = np.empty_like(H), np.empty_like(H)
+    f = ro.f0.cpu().numpy().astype(dt_np)
+    f2 = np.zeros_like(f)
+    u = [x.cpu().numpy().astype(dt_np) for x in ro.u0]
+    zero = np.zeros_like(H)
+    u_adv = {"flow": u, "frozen": [x.copy() for x in u], "none": [zero, zero, zero]}[ro.advection]
+    rho = np.zeros_like(H)
+    F = [np.zeros_like(H) for _ in range(3)]
+    flags = tensor_of(ro.flags).cpu().numpy().astype(np.uint8)
+    phi = tensor_of(ro.phi).cpu().numpy().astype(dt_np)
+    eta, s_h, s_sig = theta
+    ct, st = cos_sin_f32(tw.theta_rad)
+
+    def phase():
+        L.or_update_phase(ptr(H), ptr(T), ptr(fl), tw.cp_rho, tw.L_rho, m.T_solidus,
+                          m.T_liquidus, *d)
+
+    def ceiling():
+        L.or_clamp_enthalpy_ceiling_scalar(ptr(H), ptr(flags), tw.cp_rho, m.T_solidus,
+                                           m.T_liquidus, tw.T_vapor_cap_K, tw.L_rho, *d)
+
+    phase()
+    for n in range(ro.n_steps):
+        x, y = ro.path(n)
+        ai, aj = x / g.dx, y / g.dx
+        L.or_inject_arc_heat(ptr(H), ptr(flags), ptr(fl), None, ai, aj, ro.arc_k, tw.Q_w,
+                             ro.sigma0 * s_sig, g.dt, g.dx ** 3, eta,
+                             tw.arc_penetration_m / g.dx, 1 if tw.arc_surface_weighting else 0,
+                             *d)
+        ceiling()
+        H_in = H.copy()
+        L.or_advect_diffuse_temperature_jacobi(ptr(H_in), ptr(H), ptr(T), ptr(u_adv[0]),
+                                               ptr(u_adv[1]), ptr(u_adv[2]), ptr(flags),
+                                               tw.alpha_lu, 1.0, tw.cp_rho, *d)
+        L.or_apply_thermal_boundary_losses(ptr(H), ptr(T), ptr(flags), tw.T_amb,
+                                           ro.h_conv0 * s_h, ro.eps0 * s_h,
+                                           1 if tw.enable_convection else 0,
+                                           1 if tw.enable_radiation else 0, tw.cp_rho, g.dt,
+                                           g.dx, tw.sigma_sb, *d)
+        L.or_clamp_enthalpy_floor_scalar(ptr(H), tw.cp_rho, ptr(flags), tw.T_amb, *d)
+        phase()
+        ceiling()
+        phase()
+        if ro.advection != "flow":
+            continue
+        for a in F:
+            a[...] = 0.0
+        L.or_compute_csf_tension(ptr(phi), ptr(flags), ptr(F[0]), ptr(F[1]), ptr(F[2]),
+                                 tw.gamma_lu, 1 if tw.enable_wetting else 0, ct, st, *d)
+        L.or_compute_marangoni_force(ptr(T), ptr(phi), ptr(fl), ptr(F[0]), ptr(F[1]), ptr(F[2]),
+                                     ptr(flags), tw.dgamma_dT_lu, *d)
+        L.or_apply_arc_pressure(ptr(F[2]), ptr(flags), ptr(phi), ai, aj, ro.arc_k,
+                                weld_forces.arc_pressure_sigma_cells(tw),
+                                weld_forces.arc_pressure_peak_pa(tw), g.dt, g.dx, m.rho, *d)
+        L.or_add_hydrostatic_gravity(ptr(F[2]), ptr(fl), ptr(flags), 1.0, tw.g_lu, *d)
+        L.or_add_buoyancy(ptr(T), ptr(F[2]), ptr(fl), ptr(flags), tw.g_lu, tw.beta_T,
+                          m.T_liquidus, 1.0, *d)
+        L.or_clamp_body_force_magnitude(ptr(F[0]), ptr(F[1]), ptr(F[2]), ptr(flags), ro.F_cap,
+                                        *d)
+        L.or_collide_mrt(ptr(f), ptr(f2), ptr(rho), ptr(u[0]), ptr(u[1]), ptr(u[2]), ptr(F[0]),
+                         ptr(F[1]), ptr(F[2]), ptr(fl), ptr(flags), tw.omega, tw.omega_bulk,
+                         tw.C_darcy, *d)
+        L.or_clamp_velocity_mach(ptr(u[0]), ptr(u[1]), ptr(u[2]), ptr(flags), ro.u_cap, *d)
+        f, f2 = f2, f
+    return fl
+
+
+def _wd_loss(fl, i_c, tw, target, tau):
+    from waam_solver_amd.adjoint import pool_wd_surrogate
+
+    W, D = pool_wd_surrogate(fl, i_c, tw.nz_solid, tw.grid.layout, tw.grid.dx, tau=tau)
+    dx = tw.grid.dx
+    return ((W - target[0]) / dx) ** 2 + ((D - target[1]) / dx) ** 2
+
+
+@pytest.mark.parametrize("advection,steps", [("none", 60), ("frozen", 60), ("flow", 60)])
+def test_rollout_gradient_against_binary64_differences(advection, steps):
+    """dL/d(eta, heat-loss factor, sigma scale) of the smoothed pool width/depth loss through the
+    rollout (64x40x32, checkpoints every 20 steps) against central differences (h = 1e-4) of
+    the same forward on the oracle's binary64 build."""
+    print(f"[cadj] rollout gradient, advection {advection}, {steps} steps", flush=True)
+    from waam_solver_amd.adjoint import CoupledRollout, CoupledRolloutFn
+
+    tw, path = _rollout_case(64, 40, 32)
+    g = tw.grid
+    ro = CoupledRollout(tw, steps, path, checkpoint_every=20, advection=advection)
+    i_c = int(path(steps - 1)[0] / g.dx)
+    tau = 0.1
+    fl_t = torch.from_numpy(_oracle_coupled(tw, ro, THETA_TARGET))
+    target = tuple(float(v) for v in _wd_target(fl_t, i_c, tw, tau))
+    theta = torch.tensor(THETA0, dtype=torch.float64, requires_grad=True)
+    T, fl = CoupledRolloutFn.apply(theta, ro)
+    loss = _wd_loss(fl, i_c, tw, target, tau)
+    loss.backward()
+    adj = theta.grad.numpy()
+    h = 1e-4
+    fd = np.zeros(3)
+    for a in range(3):
+        tp, tm = list(THETA0), list(THETA0)
+        tp[a] += h
+        tm[a] -= h
+        lp = float(_wd_loss(torch.from_numpy(_oracle_coupled(tw, ro, tp)), i_c, tw, target, tau))
+        lm = float(_wd_loss(torch.from_numpy(_oracle_coupled(tw, ro, tm)), i_c, tw, target, tau))
+        fd[a] = (lp - lm) / (2 * h)
+    print(f"[cadj] {advection}: loss {float(loss):.6e}, adjoint {adj}, differences {fd}",
+          flush=True)
+    scale = np.abs(fd).max()
+    assert scale > 0.0
+    assert np.all(np.abs(adj - fd) <= 1e-3 * scale), (adj, fd)
+
+
+def _wd_target(fl, i_c, tw, tau):
+    from waam_solver_amd.adjoint import pool_wd_surrogate
+
+    return pool_wd_surrogate(fl, i_c, tw.nz_solid, tw.grid.layout, tw.grid.dx, tau=tau)

@@ -184,9 +184,13 @@ class ThermalRollout:
         grads = torch.zeros(3, dtype=torch.float64, device=dev)
         sums = torch.zeros(2, dtype=torch.float64, device=dev)
         c_last = self._consts(self.n_steps - 1)
-        lib().call("waam_adj_terminal", P(self.H_final), P(grad_T.contiguous()),
-                   P(grad_fl.contiguous()), P(self.flags), ctypes.addressof(c_last), P(lam),
-                   n_cells, st)
+        # P() hands over a bare address: the converted seeds are held in locals until the
+        # kernels that read them are queued (a temporary would be freed, and its memory reused,
+        # before the call)
+        gT = grad_T.float().contiguous()
+        gfl = grad_fl.float().contiguous()
+        lib().call("waam_adj_terminal", P(self.H_final), P(gT), P(gfl), P(self.flags),
+                   ctypes.addressof(c_last), P(lam), n_cells, st)
         seg_len = self.K
         for s_idx in reversed(range(len(self.ckpt))):
             s0 = s_idx * seg_len
@@ -344,8 +348,16 @@ class CoupledRollout:
     binary64. The forward keeps the state every ``checkpoint_every`` steps and the backward
     recomputes each segment from its checkpoint."""
 
+    ADVECTION = ("flow", "frozen", "none")
+
     def __init__(self, twin, n_steps: int, path, checkpoint_every: int = 50,
-                 is_welding: bool = True):
+                 is_welding: bool = True, advection: str = "flow"):
+        """advection: "flow" — the enthalpy is advected by the velocity each step's collision
+        produces (the coupled step); "frozen" — by the starting velocity throughout, with the
+        forces and the collision left out; "none" — not at all (the thermal chain alone)."""
+        if advection not in self.ADVECTION:
+            raise ValueError(f"advection {advection!r} (one of {self.ADVECTION})")
+        self.advection = advection
         if type(twin.arc_source).__name__ != "Gaussian2D":
             raise NotImplementedError("the differentiation form covers the Gaussian2D arc")
         if twin.use_srt or twin.use_material_tables:
@@ -371,6 +383,15 @@ class CoupledRollout:
         self.H0 = tensor_of(g.H).clone()
         self.f0 = tensor_of(g.f_src).clone()
         self.u0 = [tensor_of(x).clone() for x in (g.ux, g.uy, g.uz)]
+        self.zero_u = f()
+        # the velocity the enthalpy is advected by when it does not evolve
+        self.u_adv = None
+        if advection == "frozen":
+            self.u_adv = [f() for _ in range(3)]
+            for d_, s_ in zip(self.u_adv, self.u0):
+                tensor_of(d_).copy_(s_)
+        elif advection == "none":
+            self.u_adv = [self.zero_u] * 3
         self.h_conv0, self.eps0 = float(twin.h_conv), float(twin.eps_rad)
         self.sigma0 = float(twin.sigma_cells)
         x0, y0 = path(0)
@@ -410,7 +431,8 @@ class CoupledRollout:
         eta, s_h, s_sig = (float(v) for v in theta)
         x, y = self.path(n)
         arc_i, arc_j = x / g.dx, y / g.dx
-        sv = (lambda k, t: save[k].copy_(tensor_of(t))) if save is not None else None
+        sv = ((lambda k, t: save[k].copy_(tensor_of(t)) if k in save else None)
+              if save is not None else None)
         if sv:
             sv("H", self.H)
             sv("fl", self.fl)
@@ -427,9 +449,10 @@ class CoupledRollout:
         self._ceiling()
         if sv:
             sv("B", self.H)
-        kernels.advect_diffuse_temperature_into(self.H, self.Htmp, self.T, self.ux, self.uy,
-                                                self.uz, self.flags, tw.alpha_lu, 1.0, tw.cp_rho,
-                                                g.nx, g.ny, g.nz)
+        ua = self.u_adv or (self.ux, self.uy, self.uz)
+        kernels.advect_diffuse_temperature_into(self.H, self.Htmp, self.T, ua[0], ua[1], ua[2],
+                                                self.flags, tw.alpha_lu, 1.0, tw.cp_rho, g.nx,
+                                                g.ny, g.nz)
         self.H, self.Htmp = self.Htmp, self.H
         if tw.enable_heat_loss:
             kernels.apply_thermal_boundary_losses(
@@ -446,6 +469,8 @@ class CoupledRollout:
         self._phase()
         self._ceiling()
         self._phase()
+        if self.advection != "flow":
+            return
         # forces on the new temperature and liquid fraction
         kernels.clear_forces(self.Fx, self.Fy, self.Fz)
         if tw.enable_csf_tension:
@@ -513,68 +538,80 @@ class CoupledRollout:
         N = g.n_cells
         shp = tuple(tensor_of(self.H).shape)
         d64 = dict(dtype=torch.float64, device=dev)
+        flow = self.advection == "flow"
         lam_H = torch.zeros(shp, **d64)
-        lam_f = torch.zeros((19,) + shp, **d64)
-        lam_u = torch.zeros((3,) + shp, **d64)
         lam_Ho = torch.empty(shp, **d64)
+        lam_u = torch.zeros((3,) + shp, **d64)
         lam_uo = torch.empty((3,) + shp, **d64)
-        lam_fin = torch.empty((19,) + shp, **d64)
-        lam_F = torch.empty((3,) + shp, **d64)
-        lam_Fr = torch.empty((3,) + shp, **d64)
-        lam_T1 = torch.empty(shp, **d64)
-        lam_fl1 = torch.empty(shp, **d64)
-        G = torch.empty((3,) + shp, **d64)
         lamC = torch.empty(shp, **d64)
         lamB = torch.empty(shp, **d64)
         grads = torch.zeros(3, **d64)
         sums = torch.zeros(2, **d64)
+        if flow:
+            lam_f = torch.zeros((19,) + shp, **d64)
+            lam_fin = torch.empty((19,) + shp, **d64)
+            lam_F = torch.empty((3,) + shp, **d64)
+            lam_Fr = torch.empty((3,) + shp, **d64)
+            lam_T1 = torch.empty(shp, **d64)
+            lam_fl1 = torch.empty(shp, **d64)
+            G = torch.empty((3,) + shp, **d64)
         c_last = self._consts(self.n_steps - 1)
+        # the binary64 seeds live in locals while their kernel is queued (P() keeps no reference)
+        gT = grad_T.double().contiguous()
+        gfl = grad_fl.double().contiguous()
         lib().call("waam_cadj_phase", P(self.H_final), P(self.flags), ctypes.addressof(c_last),
-                   P(grad_T.double().contiguous()), P(grad_fl.double().contiguous()), P(lam_H),
-                   N, st)
+                   P(gT), P(gfl), P(lam_H), N, st)
+        keys = self._SAVE if flow else ("H", "fl", "A", "B", "E", "F")
         for s_idx in reversed(range(len(self.ckpt))):
             s0 = s_idx * self.K
             s1 = min(self.n_steps, s0 + self.K)
             L = s1 - s0
-            sv = {k: torch.empty((L,) + shp, dtype=torch.float32, device=dev) for k in self._SAVE}
-            sv["f"] = torch.empty((L, 19) + shp, dtype=torch.float32, device=dev)
+            sv = {k: torch.empty((L,) + shp, dtype=torch.float32, device=dev) for k in keys}
+            if flow:
+                sv["f"] = torch.empty((L, 19) + shp, dtype=torch.float32, device=dev)
             self._state_in(*self.ckpt[s_idx])
             for t in range(L):
                 self._step(s0 + t, self.theta, save={k: v[t] for k, v in sv.items()})
             for t in reversed(range(L)):
                 c = self._consts(s0 + t)
-                # collision (with the Mach clamp) -> lam_f of the step's populations, lam_F, f_l
-                lam_fl1.zero_()
-                lam_T1.zero_()
-                lib().call("waam_cadj_collide_mrt", P(sv["f"][t]), P(sv["Fx"][t]),
-                           P(sv["Fy"][t]), P(sv["Fz"][t]), P(sv["fl1"][t]), P(self.flags),
-                           P(lam_f), P(lam_u), tw.omega, tw.omega_bulk, tw.C_darcy, self.u_cap,
-                           P(lam_fin), P(lam_F), P(lam_fl1), N, st)
-                lib().call("waam_cadj_forces", P(sv["Frx"][t]), P(sv["Fry"][t]), P(sv["Frz"][t]),
-                           P(sv["T1"][t]), P(sv["fl1"][t]), P(self.flags), P(lam_F), self.F_cap,
-                           tw.g_lu, tw.beta_T, tw.mat.T_liquidus,
-                           1 if tw.enable_hydrostatic_gravity else 0, P(lam_Fr), P(lam_T1),
-                           P(lam_fl1), N, st)
-                lib().call("waam_cadj_marangoni", P(self.phi), P(sv["fl1"][t]), P(self.flags),
-                           P(lam_Fr), tw.dgamma_dT_lu, P(G), P(lam_T1), g.nx, g.ny, g.nz,
-                           g.layout, st)
-                lib().call("waam_cadj_phase", P(sv["H1"][t]), P(self.flags), ctypes.addressof(c),
-                           P(lam_T1), P(lam_fl1), P(lam_H), N, st)
-                # the thermal chain with advection -> lam_H, lam_u of the step's start
+                if flow:
+                    # collision (with the Mach clamp) -> lam_f of the step's populations,
+                    # lam_F and f_l
+                    lam_fl1.zero_()
+                    lam_T1.zero_()
+                    lib().call("waam_cadj_collide_mrt", P(sv["f"][t]), P(sv["Fx"][t]),
+                               P(sv["Fy"][t]), P(sv["Fz"][t]), P(sv["fl1"][t]), P(self.flags),
+                               P(lam_f), P(lam_u), tw.omega, tw.omega_bulk, tw.C_darcy,
+                               self.u_cap, P(lam_fin), P(lam_F), P(lam_fl1), N, st)
+                    lib().call("waam_cadj_forces", P(sv["Frx"][t]), P(sv["Fry"][t]),
+                               P(sv["Frz"][t]), P(sv["T1"][t]), P(sv["fl1"][t]), P(self.flags),
+                               P(lam_F), self.F_cap, tw.g_lu, tw.beta_T, tw.mat.T_liquidus,
+                               1 if tw.enable_hydrostatic_gravity else 0, P(lam_Fr), P(lam_T1),
+                               P(lam_fl1), N, st)
+                    lib().call("waam_cadj_marangoni", P(self.phi), P(sv["fl1"][t]),
+                               P(self.flags), P(lam_Fr), tw.dgamma_dT_lu, P(G), P(lam_T1), g.nx,
+                               g.ny, g.nz, g.layout, st)
+                    lib().call("waam_cadj_phase", P(sv["H1"][t]), P(self.flags),
+                               ctypes.addressof(c), P(lam_T1), P(lam_fl1), P(lam_H), N, st)
+                    ua = (sv["ux"][t], sv["uy"][t], sv["uz"][t])
+                else:
+                    ua = tuple(tensor_of(x) for x in self.u_adv)
+                # the thermal chain with advection -> lam_H and lam_u of the step's start
                 if self.is_welding:
                     lib().call("waam_adj_arc_sums", P(self.flags), P(sv["fl"][t]),
                                ctypes.addressof(c), P(sums), g.nx, g.ny, g.nz, g.layout,
                                P(g.scratch), st)
                 lib().call("waam_cadj_thermal_step", P(lam_H), P(lam_Ho), P(lam_uo),
                            P(sv["H"][t]), P(sv["A"][t]), P(sv["B"][t]), P(sv["E"][t]),
-                           P(sv["F"][t]), P(sv["fl"][t]), P(sv["ux"][t]), P(sv["uy"][t]),
-                           P(sv["uz"][t]), P(self.flags), P(sums), ctypes.addressof(c), P(grads),
-                           P(lamC), P(lamB), g.nx, g.ny, g.nz, g.layout, P(g.scratch), st)
+                           P(sv["F"][t]), P(sv["fl"][t]), P(ua[0]), P(ua[1]), P(ua[2]),
+                           P(self.flags), P(sums), ctypes.addressof(c), P(grads), P(lamC),
+                           P(lamB), g.nx, g.ny, g.nz, g.layout, P(g.scratch), st)
                 lam_H, lam_Ho = lam_Ho, lam_H
-                lam_u, lam_uo = lam_uo, lam_u
-                lam_f, lam_fin = lam_fin, lam_f
+                if flow:
+                    lam_u, lam_uo = lam_uo, lam_u
+                    lam_f, lam_fin = lam_fin, lam_f
             del sv
-        self.lam0 = (lam_H, lam_f, lam_u)
+        self.lam0 = lam_H
         return grads.cpu().numpy()
 
     def _consts(self, n: int) -> AdjConsts:

@@ -237,11 +237,15 @@ __device__ __forceinline__ void cell_ijk(size_t c, const Dims &d, int &i, int &j
  * applies to GAS. A segment that was quiet at the end of the previous step, with all its
  * neighbours, and at the end of the step before as well, therefore holds this step's results in
  * both buffers of every pair already (each buffer was last written by a step that left it
- * quiet), and the three sweeps skip it. The record: bits[seg] bit 0 = quiet at the end of the
- * last step, bit 1 = quiet at the end of the step before (written by kernel 7); skip[seg] =
- * this step's decision (written by kernel 2, read by kernels 4 and 7). */
+ * quiet): a steady step lists the other, active, segments first (kernel 1b) and runs the three
+ * sweeps over that list only. The record: bits[seg] bit 0 = quiet at the end of the last step,
+ * bit 1 = quiet at the end of the step before (written by kernel 7 for the segments it visits;
+ * a segment left out keeps 3, which is what it would write); then the active-segment count and
+ * list. */
 struct QuietRec {
-    uint8_t *bits, *skip;
+    uint8_t *bits;
+    int *count;
+    int32_t *list;
     int nbx;  // segments per row
 };
 
@@ -250,13 +254,23 @@ __host__ __device__ inline size_t quiet_segments(const Dims &d)
 {
     return (size_t)quiet_nbx(d) * (size_t)d.ny * (size_t)d.slow();
 }
+/* byte offsets in the record: bits, the count (own 256-byte line), the list */
+__host__ __device__ inline size_t quiet_count_offset(const Dims &d)
+{
+    return (quiet_segments(d) + 255) / 256 * 256;
+}
+__host__ __device__ inline size_t quiet_bytes(const Dims &d)
+{
+    return quiet_count_offset(d) + 256 + 4 * quiet_segments(d);
+}
 
 __device__ __forceinline__ QuietRec quiet_rec(const waam_step_fields &f, const Dims &d)
 {
     QuietRec q;
     q.nbx = quiet_nbx(d);
     q.bits = f.quiet;
-    q.skip = f.quiet + quiet_segments(d);
+    q.count = reinterpret_cast<int *>(f.quiet + quiet_count_offset(d));
+    q.list = reinterpret_cast<int32_t *>(f.quiet + quiet_count_offset(d) + 256);
     return q;
 }
 
@@ -265,46 +279,82 @@ __device__ __forceinline__ size_t seg_index(const QuietRec &q, int bx, int j, in
     return ((size_t)b * (size_t)d.ny + (size_t)j) * (size_t)q.nbx + (size_t)bx;
 }
 
-/* Whether the wavefront's segment (bx, j, b) skips this step: bit 0 on every segment the
- * stencils of its cells reach and bit 1 on itself. The reach: one cell along the contiguous
+/* Whether segment (bx, j, b) must be swept this step: not unless bit 0 is set on every segment
+ * the stencils of its cells reach and bit 1 on itself. The reach: one cell along the contiguous
  * axis, one row in j, and along the slowest axis two slices below and one above in the ZYX
  * layout (the contact rule reads k-1 and k-2, the phi advection k+1) or one slice either way in
  * XYZ (x; there k-2 lies on the contiguous axis). x wraps, y and z clamp (a clamped neighbour is
- * an edge cell itself). Uniform over the wavefront; every lane must call it. */
-__device__ __forceinline__ bool segment_skips(const QuietRec &q, int bx, int j, int b,
-                                              const Dims &d)
+ * an edge cell itself). */
+__device__ __forceinline__ bool segment_active(const QuietRec &q, int bx, int j, int b,
+                                               const Dims &d)
 {
-    const int lane = (int)threadIdx.x;  // block3(): the lanes run along the contiguous axis
-    bool ok = true;
-    if (lane < 36) {
-        const bool zyx = d.layout == kLayoutZYX;
-        int x = bx + lane % 3 - 1;
-        const int jj = imax(0, imin(j + (lane / 3) % 3 - 1, d.ny - 1));
-        const int db = lane / 9 - 2;
+    const bool zyx = d.layout == kLayoutZYX;
+    /* every byte is requested before any is tested (one memory round trip) */
+    unsigned all = 1u;
+#pragma unroll
+    for (int db = -2; db <= 1; ++db) {
         int bb;
         if (zyx) {
-            x = x < 0 ? x + q.nbx : (x >= q.nbx ? x - q.nbx : x);
             bb = imax(0, imin(b + db, d.slow() - 1));
         } else {
-            x = imax(0, imin(x, q.nbx - 1));
             bb = db == -2 ? b : b + db;
             bb = bb < 0 ? bb + d.slow() : (bb >= d.slow() ? bb - d.slow() : bb);
         }
-        const uint8_t v = q.bits[seg_index(q, x, jj, bb, d)];
-        ok = (v & 1u) != 0;
-        if (lane == 22) ok = ok && (v & 2u) != 0;  // offsets (0, 0, 0): the segment itself
+#pragma unroll
+        for (int dj = -1; dj <= 1; ++dj) {
+            const int jj = imax(0, imin(j + dj, d.ny - 1));
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx) {
+                int x = bx + dx;
+                if (zyx) x = x < 0 ? x + q.nbx : (x >= q.nbx ? x - q.nbx : x);
+                else x = imax(0, imin(x, q.nbx - 1));
+                all &= (unsigned)q.bits[seg_index(q, x, jj, bb, d)];
+            }
+        }
     }
-    return __ballot(!ok) == 0ull;
+    const unsigned own = q.bits[seg_index(q, bx, j, b, d)];
+    return !((own == 3u) && (all & 1u));
 }
 
-/* kernel 2's decision for the wavefront's segment, recorded for kernels 4 and 7 */
-__device__ __forceinline__ bool sweep_skips(const waam_step_fields &f, const Dims &d, int bx,
-                                            int j, int b)
+/* 1b — the active segments of a steady step, one thread per segment, appended to the list
+ * (wavefront-aggregated; the order is free: each segment's results depend only on the
+ * sweeps' inputs) */
+__global__ void __launch_bounds__(kBlock) k_step_quiet_list(waam_step_fields f, StepConsts s)
 {
-    QuietRec q = quiet_rec(f, d);
-    const bool sk = segment_skips(q, bx, j, b, d);
-    if (threadIdx.x == 0) q.skip[seg_index(q, bx, j, b, d)] = sk ? 1 : 0;
-    return sk;
+    const Dims &d = s.d;
+    const QuietRec q = quiet_rec(f, d);
+    const size_t nseg = quiet_segments(d);
+    const size_t sg = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool act = false;
+    if (sg < nseg) {
+        const int bx = (int)(sg % (size_t)q.nbx);
+        const size_t r = sg / (size_t)q.nbx;
+        act = segment_active(q, bx, (int)(r % (size_t)d.ny), (int)(r / (size_t)d.ny), d);
+    }
+    const unsigned long long m = __ballot(act);
+    if (m == 0ull) return;
+    const int lane = (int)__lane_id();
+    const int leader = __builtin_ctzll(m);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(q.count, __popcll(m));
+    base = __shfl(base, leader);
+    if (act) q.list[base + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)sg;
+}
+
+/* the cell of this lane in the listed segment t of a sweep over the list (block3() blocks: the
+ * lanes along the contiguous axis, one wavefront per threadIdx.y) */
+__device__ __forceinline__ bool seg_cell(const QuietRec &q, int t, const Dims &d, size_t &seg,
+                                         int &i, int &j, int &k)
+{
+    seg = (size_t)q.list[t];
+    const int bx = (int)(seg % (size_t)q.nbx);
+    const size_t r = seg / (size_t)q.nbx;
+    j = (int)(r % (size_t)d.ny);
+    const int b = (int)(r / (size_t)d.ny);
+    const int a = bx * kTileFast + (int)threadIdx.x;
+    i = d.layout == kLayoutZYX ? a : b;
+    k = d.layout == kLayoutZYX ? b : a;
+    return a < d.fast();
 }
 
 /* the wavefront's segment ends the step quiet (`quiet`: this lane's cell is, or it has none):
@@ -318,6 +368,9 @@ __device__ __forceinline__ void record_quiet(const QuietRec &q, size_t seg, bool
         q.bits[seg] = (uint8_t)((cont ? ((old & 1u) << 1) : 0u) | (all ? 1u : 0u));
     }
 }
+
+/* blocks of the sweeps over the active-segment list (each grid-strides over it by wavefront) */
+constexpr int kSegBlocks = 2048;
 
 /* 1 — surface height in the torch column (reference kernels.py:625-641 and
  *     coupled_step.py:34-59) and per-block compensated arc weight sums (kernels.py:443-496);
@@ -334,6 +387,7 @@ __global__ void __launch_bounds__(kBlock) k_step_begin(waam_step_fields f, StepC
     if (blockIdx.x == 0) {
         if (!p.thermal_list_ready) cell_list(f, s.seg_len, 0).count(threadIdx.x) = 0;
         cell_list(f, s.seg_len, 1).count(threadIdx.x) = 0;
+        if (p.quiet_skip && threadIdx.x == 0) *quiet_rec(f, d).count = 0;
     }
     int best = -1;
     for (int k = threadIdx.x; k < d.nz; k += blockDim.x) {
@@ -635,10 +689,6 @@ __global__ void __launch_bounds__(256) k_step_thermal(waam_step_fields f, StepCo
     const Dims &d = s.d;
     int i, j, k;
     bool in = d.cell(i, j, k);
-    if (!LIST && s.p.quiet_skip) {
-        if (j >= d.ny) return;  // the wavefront's whole row is past the grid
-        if (sweep_skips(f, d, (int)blockIdx.x, j, (int)blockIdx.z)) return;
-    }
     size_t c = in ? d.idx(i, j, k) : 0;
     /* the flag, the cell's enthalpy and the phi-advection operands are requested together (a
      * GAS cell, most of a grid, then waits for one memory round trip); a FLUID/IFACE cell's
@@ -650,6 +700,30 @@ __global__ void __launch_bounds__(256) k_step_thermal(waam_step_fields f, StepCo
     if (!in || is_surface_class(flag)) return;
     thermal_cell(f, s, i, j, k, c, flag, h_in);
     if (s.p.enable_vof) f.phi_tmp[c] = phi_new;
+}
+
+/* 2 over the active segments of a steady step (the surface cells go to kernel 3) */
+__global__ void __launch_bounds__(256) k_step_thermal_seg(waam_step_fields f, StepConsts s)
+{
+    const Dims &d = s.d;
+    const QuietRec q = quiet_rec(f, d);
+    const int n = *q.count;
+    const int nw = (int)(gridDim.x * blockDim.y);
+    for (int t = (int)(blockIdx.x * blockDim.y + threadIdx.y); t < n; t += nw) {
+        size_t seg;
+        int i, j, k;
+        const bool in = seg_cell(q, t, d, seg, i, j, k);
+        const size_t c = in ? d.idx(i, j, k) : 0;
+        const uint8_t flag = in ? f.flags_in[c] : (uint8_t)GAS;
+        const float h_in = in ? f.H_in[c] : 0.0f;
+        /* a SOLID cell's advected phi is re-initialised to 1 whatever its neighbours hold:
+         * the plate's segments (most of a steady step's list) skip those loads */
+        float phi_new = 1.0f;
+        if (in && s.p.enable_vof && flag != SOLID) phi_new = vof_advect_cell(f, d, i, j, k, flag);
+        if (!in || is_surface_class(flag)) continue;
+        thermal_cell(f, s, i, j, k, c, flag, h_in);
+        if (s.p.enable_vof) f.phi_tmp[c] = phi_new;
+    }
 }
 
 /* thermal chain (with enthalpy advection) and VOF advection of the listed FLUID/IFACE cells;
@@ -706,13 +780,7 @@ __global__ void __launch_bounds__(256) k_step_thermal_all(waam_step_fields f, St
             cell_ijk(c, d, i, j, k);
             flag = f.flags_in[c];
         } else {
-            if (!first) return;
-            const bool in = d.cell(i, j, k, lz);
-            if (s.p.quiet_skip) {
-                if (j >= d.ny) return;  // the wavefront's whole row is past the grid
-                if (sweep_skips(f, d, (int)blockIdx.x, j, (int)blockIdx.z - lz)) return;
-            }
-            if (!in) return;
+            if (!first || !d.cell(i, j, k, lz)) return;
             c = d.idx(i, j, k);
             flag = f.flags_in[c];
         }
@@ -743,6 +811,45 @@ __device__ __forceinline__ void time_above_store(const waam_step_fields &f,
     if (t.T >= p.T_s) f.tsol[c] = t.tsol + p.dt;
 }
 
+/* the body of kernel 4 for one cell; returns its new flag. SOLID_FAST: a SOLID cell (whose
+ * contact-ruled phi is its own) skips the contact rule's neighbour loads, at the price of a
+ * second memory round trip for the other cells */
+template <bool SOLID_FAST>
+__device__ __forceinline__ uint8_t vof_flags_cell(const waam_step_fields &f, const StepConsts &s,
+                                                  int i, int j, int k, size_t c)
+{
+    const Dims &d = s.d;
+    uint8_t nf = GAS;
+    /* the flag, the contact rule's operands and f_l are requested together (none depends on
+     * another), so a GAS cell waits for one memory round trip; the time-above fields of
+     * the cells that were not GAS (nearly all that stay so) follow the flag */
+    uint8_t f_in = f.flags_in[c];
+    float p = 0.0f, flo = 0.0f;
+    if (s.p.enable_vof) {
+        p = (s.p.enable_wetting && !(SOLID_FAST && f_in == SOLID))
+                ? contact_bc_phi(f.phi_tmp, f.flags_in, i, j, k, s.p.cos_t, s.p.sin_t, d)
+                : f.phi_tmp[c];
+        flo = f.fl_out[c];
+    }
+    bool pre = f_in != GAS;
+    TimeAbove ta{};
+    if (pre) ta = time_above_load(f, c);
+    if (!s.p.enable_vof) {
+        nf = f_in;
+    } else {
+        nf = classify_cell(k, s.p.nz_solid, p, flo);
+        f.phi[c] = p;
+        if (nf == GAS || p < 0.05f) f.fl_out[c] = 0.0f;
+        else if (nf == SOLID) f.fl_out[c] = 0.0f;
+    }
+    f.flags_out[c] = nf;
+    if (nf != GAS) {
+        if (!pre) ta = time_above_load(f, c);
+        time_above_store(f, s.p, c, ta);
+    }
+    return nf;
+}
+
 /* 4 — apply_contact_angle_phi_bc (Jacobi), update_flags_from_phi, sync_phi_liquid_fraction,
  *     update_time_above_T (reference kernels.py:933-991, 2394-2419, 3484-3504, 3261-3283);
  *     lists the cells kernel 5 collides */
@@ -753,43 +860,28 @@ __global__ void __launch_bounds__(256) k_step_vof_flags(waam_step_fields f, Step
         cell_list(f, s.seg_len, 0).count(threadIdx.y * blockDim.x + threadIdx.x) = 0;  // list
     int i, j, k;
     bool in = d.cell(i, j, k);
-    if (s.p.quiet_skip) {
-        if (j >= d.ny) return;
-        QuietRec q = quiet_rec(f, d);
-        if (q.skip[seg_index(q, (int)blockIdx.x, j, (int)blockIdx.z, d)]) return;
-    }
     size_t c = in ? d.idx(i, j, k) : 0;
-    uint8_t nf = GAS;
-    if (in) {
-        /* the flag, the contact rule's operands and f_l are requested together (none depends on
-         * another), so a GAS cell waits for one memory round trip; the time-above fields of
-         * the cells that were not GAS (nearly all that stay so) follow the flag */
-        uint8_t f_in = f.flags_in[c];
-        float p = 0.0f, flo = 0.0f;
-        if (s.p.enable_vof) {
-            p = s.p.enable_wetting
-                    ? contact_bc_phi(f.phi_tmp, f.flags_in, i, j, k, s.p.cos_t, s.p.sin_t, d)
-                    : f.phi_tmp[c];
-            flo = f.fl_out[c];
-        }
-        bool pre = f_in != GAS;
-        TimeAbove ta{};
-        if (pre) ta = time_above_load(f, c);
-        if (!s.p.enable_vof) {
-            nf = f_in;
-        } else {
-            nf = classify_cell(k, s.p.nz_solid, p, flo);
-            f.phi[c] = p;
-            if (nf == GAS || p < 0.05f) f.fl_out[c] = 0.0f;
-            else if (nf == SOLID) f.fl_out[c] = 0.0f;
-        }
-        f.flags_out[c] = nf;
-        if (nf != GAS) {
-            if (!pre) ta = time_above_load(f, c);
-            time_above_store(f, s.p, c, ta);
-        }
-    }
+    uint8_t nf = in ? vof_flags_cell<false>(f, s, i, j, k, c) : (uint8_t)GAS;
     list_append(in && is_surface_class(nf), c, cell_list(f, s.seg_len, 1), d.n());
+}
+
+/* 4 over the active segments of a steady step */
+__global__ void __launch_bounds__(256) k_step_vof_flags_seg(waam_step_fields f, StepConsts s)
+{
+    const Dims &d = s.d;
+    if (blockIdx.x == 0)  // kernel 2 has consumed the list
+        cell_list(f, s.seg_len, 0).count(threadIdx.y * blockDim.x + threadIdx.x) = 0;
+    const QuietRec q = quiet_rec(f, d);
+    const int n = *q.count;
+    const int nw = (int)(gridDim.x * blockDim.y);
+    for (int t = (int)(blockIdx.x * blockDim.y + threadIdx.y); t < n; t += nw) {
+        size_t seg;
+        int i, j, k;
+        const bool in = seg_cell(q, t, d, seg, i, j, k);
+        const size_t c = in ? d.idx(i, j, k) : 0;
+        const uint8_t nf = in ? vof_flags_cell<true>(f, s, i, j, k, c) : (uint8_t)GAS;
+        list_append(in && is_surface_class(nf), c, cell_list(f, s.seg_len, 1), d.n());
+    }
 }
 
 /* Pull of the reference stream (kernels.py:2991-3010) with the flags of the step it belongs
@@ -1017,11 +1109,10 @@ __device__ __forceinline__ uint8_t freeze_cell(const waam_step_fields &f, const 
  *     step's pull; then the freeze/remelt updates of every non-GAS cell (cell-local; the pull
  *     reads only populations and stream flags, which they do not touch); lists the cells that
  *     end the step FLUID/IFACE, the next step's thermal list; records which row segments end
- *     the step quiet. `skip`: the segment is quiet and only the pull runs. Returns whether the
- *     cell ends the step quiet. */
+ *     the step quiet. Returns whether the cell ends the step quiet. */
 template <class E, bool PULL>
 __device__ __forceinline__ bool stream_cell(const waam_step_fields &f, const StepConsts &s, int i,
-                                            int j, int k, bool skip)
+                                            int j, int k)
 {
     const Dims &d = s.d;
     const waam_step_params &p = s.p;
@@ -1029,18 +1120,17 @@ __device__ __forceinline__ bool stream_cell(const waam_step_fields &f, const Ste
     uint8_t fc = f.flags_out[c];
     uint8_t fp = f.sflags_prev[c];
     bool surface = is_surface_class(fc);
-    bool freeze = !skip && p.enable_freeze && fc != GAS;
-    float T = 0.0f, phi = 0.0f, fl = 0.0f;
+    bool freeze = p.enable_freeze && fc != GAS;
+    float T = 0.0f, fl = 0.0f;
+    const float phi = f.phi[c];
     if (freeze) {
         T = f.T_out[c];
         fl = f.fl_out[c];
     }
-    if (!skip) phi = f.phi[c];
     bool need = !surface && p.pending_stream && fp != SOLID;
     if (PULL && __ballot(need) != 0ull)
         pull_rows(static_cast<const E *>(f.f_src), static_cast<E *>(f.f_dst), f.sflags_prev, i, j,
                   k, need, fp, d);
-    if (skip) return true;
     f.sflags_next[c] = fc;
     /* forces are non-zero only on the cells the previous step collided, unless the step before
      * took them from the fields (then force_reset_all) */
@@ -1092,16 +1182,13 @@ __global__ void __launch_bounds__(256) k_step_stream(waam_step_fields f, StepCon
 {
     const Dims &d = s.d;
     const QuietRec q = quiet_rec(f, d);
-    const bool cont = s.p.quiet_skip != 0;
     int i, j, k;
     if (PULL) {
         const bool in = d.cell(i, j, k);
         if (j >= d.ny) return;  // the wavefront's whole row is past the grid
-        const size_t seg = seg_index(q, (int)blockIdx.x, j, (int)blockIdx.z, d);
-        const bool skip = cont && q.skip[seg] != 0;
         bool quiet = true;
-        if (in) quiet = stream_cell<E, true>(f, s, i, j, k, skip);
-        record_quiet(q, seg, quiet, cont);
+        if (in) quiet = stream_cell<E, true>(f, s, i, j, k);
+        record_quiet(q, seg_index(q, (int)blockIdx.x, j, (int)blockIdx.z, d), quiet, false);
         return;
     }
     const int a = (int)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -1111,28 +1198,20 @@ __global__ void __launch_bounds__(256) k_step_stream(waam_step_fields f, StepCon
     const bool zyx = d.layout == kLayoutZYX;
     const int b0 = (int)blockIdx.z * kStreamSlices;
     const bool freeze_on = s.p.enable_freeze != 0;
-    size_t c[kStreamSlices], seg[kStreamSlices];
-    bool skip[kStreamSlices];
+    size_t c[kStreamSlices];
     uint8_t fc[kStreamSlices], fp[kStreamSlices];
     float T[kStreamSlices], phi[kStreamSlices], fl[kStreamSlices];
 #pragma unroll
     for (int t = 0; t < kStreamSlices; ++t) {
         const int b = imin(b0 + t, d.slow() - 1);
-        seg[t] = seg_index(q, (int)blockIdx.x, j, b, d);
-        skip[t] = cont && q.skip[seg[t]] != 0;
-    }
-#pragma unroll
-    for (int t = 0; t < kStreamSlices; ++t) {
-        const int b = imin(b0 + t, d.slow() - 1);
         c[t] = zyx ? d.idx(in ? a : 0, j, b) : d.idx(b, j, in ? a : 0);
-        const bool ld = in && !skip[t];
-        fc[t] = ld ? f.flags_out[c[t]] : (uint8_t)GAS;
-        fp[t] = ld ? f.sflags_prev[c[t]] : (uint8_t)GAS;
-        phi[t] = ld ? f.phi[c[t]] : 0.0f;
+        fc[t] = in ? f.flags_out[c[t]] : (uint8_t)GAS;
+        fp[t] = in ? f.sflags_prev[c[t]] : (uint8_t)GAS;
+        phi[t] = in ? f.phi[c[t]] : 0.0f;
     }
 #pragma unroll
     for (int t = 0; t < kStreamSlices; ++t) {
-        const bool fr = in && !skip[t] && freeze_on && fc[t] != GAS;
+        const bool fr = in && freeze_on && fc[t] != GAS;
         T[t] = fr ? f.T_out[c[t]] : 0.0f;
         fl[t] = fr ? f.fl_out[c[t]] : 0.0f;
     }
@@ -1140,15 +1219,36 @@ __global__ void __launch_bounds__(256) k_step_stream(waam_step_fields f, StepCon
     for (int t = 0; t < kStreamSlices; ++t) {
         const int b = b0 + t;
         if (b >= d.slow()) break;
-        if (skip[t]) {
-            if (threadIdx.x == 0) q.bits[seg[t]] = 3u;  // quiet, as it was for two steps
-            continue;
-        }
         bool quiet = true;
         if (in)
             quiet = stream_rest_cell(f, s, zyx ? a : b, j, zyx ? b : a, c[t], fc[t], fp[t], T[t],
                                      phi[t], fl[t]);
-        record_quiet(q, seg[t], quiet, cont);
+        record_quiet(q, seg_index(q, (int)blockIdx.x, j, b, d), quiet, false);
+    }
+}
+
+/* 7 over the active segments of a steady step (the pull ran in its own launch) */
+__global__ void __launch_bounds__(256) k_step_stream_seg(waam_step_fields f, StepConsts s)
+{
+    const Dims &d = s.d;
+    const QuietRec q = quiet_rec(f, d);
+    const int n = *q.count;
+    const int nw = (int)(gridDim.x * blockDim.y);
+    const bool freeze_on = s.p.enable_freeze != 0;
+    for (int t = (int)(blockIdx.x * blockDim.y + threadIdx.y); t < n; t += nw) {
+        size_t seg;
+        int i, j, k;
+        const bool in = seg_cell(q, t, d, seg, i, j, k);
+        const size_t c = in ? d.idx(i, j, k) : 0;
+        const uint8_t fc = in ? f.flags_out[c] : (uint8_t)GAS;
+        const uint8_t fp = in ? f.sflags_prev[c] : (uint8_t)GAS;
+        const float phi = in ? f.phi[c] : 0.0f;
+        const bool fr = in && freeze_on && fc != GAS;
+        const float T = fr ? f.T_out[c] : 0.0f;
+        const float fl = fr ? f.fl_out[c] : 0.0f;
+        bool quiet = true;
+        if (in) quiet = stream_rest_cell(f, s, i, j, k, c, fc, fp, T, phi, fl);
+        record_quiet(q, seg, quiet, true);
     }
 }
 
@@ -1585,20 +1685,36 @@ static void launch_k(bool capture, void (*k)(KArgs...), dim3 grid, dim3 block, h
     }
 }
 
-/* the step's launches in order (kernels 1-7) */
+/* the step's launches in order (kernels 1-7). A steady step (s.p.quiet_skip) lists its active
+ * row segments after kernel 1 and runs kernels 2, 4 and 7 over that list only, with the surface
+ * cells of kernel 3 and the pull in launches of their own. */
 static void launch_step(const StepConsts &s, const waam_step_fields &f, hipStream_t st,
                         bool capture)
 {
     const waam_step_params &p = s.p;
     const Dims &d = s.d;
     dim3 g3 = grid3(d);
+    const bool seg = p.quiet_skip != 0;
     int do_sums = p.is_welding && !p.arc_applied ? 1 : 0;
     timed_launch(WAAM_K_BEGIN, [&](hipEvent_t a, hipEvent_t b) {
         launch_k(capture, k_step_begin, dim3(do_sums ? kReduceBlocks : 1), dim3(kBlock), st, a,
-                 do_sums ? nullptr : b, f, s, do_sums);
-        if (do_sums) launch_k(capture, k_step_arc_total, dim3(1), dim3(kBlock), st, nullptr, b, f, s);
+                 (do_sums || seg) ? nullptr : b, f, s, do_sums);
+        if (do_sums)
+            launch_k(capture, k_step_arc_total, dim3(1), dim3(kBlock), st, nullptr,
+                     seg ? nullptr : b, f, s);
+        if (seg)
+            launch_k(capture, k_step_quiet_list, dim3(grid_for(quiet_segments(d))), dim3(kBlock),
+                     st, nullptr, b, f, s);
     });
-    if (p.thermal_list_ready == 1) {
+    if (seg) {
+        timed_launch(WAAM_K_THERMAL, [&](hipEvent_t a, hipEvent_t b) {
+            launch_k(capture, k_step_thermal_seg, dim3(kSegBlocks), block3(), st, a, b, f, s);
+        });
+        timed_launch(WAAM_K_THERMAL_SURFACE, [&](hipEvent_t a, hipEvent_t b) {
+            launch_k(capture, k_step_thermal_surface, dim3(kListBlocks), dim3(kBlock), st, a, b, f,
+                     s);
+        });
+    } else if (p.thermal_list_ready == 1) {
         /* enough list slices for about kListBlocksWithSweep workers */
         int per = (int)(g3.x * g3.y);
         int lz = std::max(1, (kListBlocksWithSweep + per - 1) / per);
@@ -1619,7 +1735,8 @@ static void launch_step(const StepConsts &s, const waam_step_fields &f, hipStrea
         });
     }
     timed_launch(WAAM_K_VOF_FLAGS, [&](hipEvent_t a, hipEvent_t b) {
-        launch_k(capture, k_step_vof_flags, g3, block3(), st, a, b, f, s);
+        if (seg) launch_k(capture, k_step_vof_flags_seg, dim3(kSegBlocks), block3(), st, a, b, f, s);
+        else launch_k(capture, k_step_vof_flags, g3, block3(), st, a, b, f, s);
     });
     timed_launch(WAAM_K_COLLIDE, [&](hipEvent_t a, hipEvent_t b) {
         if (f.pop_dtype == WAAM_POP_F16)
@@ -1638,13 +1755,13 @@ static void launch_step(const StepConsts &s, const waam_step_fields &f, hipStrea
                      dim3(kBlock), st, a, b, f.tracer_pos, f.tracer_active, f.ux, f.uy, f.uz,
                      f.fl_out, f.flags_out, p.dx, f.max_tracers, d);
         });
-    /* the pull runs in a launch of its own ahead of kernel 7 (pairs of cells for 16-bit
-     * storage on an even contiguous extent), or inside kernel 7 */
+    /* the pull runs in a launch of its own ahead of kernel 7 (8-byte words, or pairs of cells
+     * for 16-bit storage on an even contiguous extent), or inside kernel 7 */
     const bool f32 = f.pop_dtype == WAAM_POP_F32;
     const bool pair16 = !f32 && d.fast() % 2 == 0;
     /* 8-byte words per lane: two fp32 cells, or four 16-bit cells */
     const bool wide = kWidePull && (f32 ? d.fast() % 2 == 0 : d.fast() % 4 == 0);
-    const bool separate = pair16 || (f32 ? kSeparatePull32 : kSeparatePull16);
+    const bool separate = seg || pair16 || (f32 ? kSeparatePull32 : kSeparatePull16);
     if (separate && p.pending_stream)
         timed_launch(WAAM_K_PULL, [&](hipEvent_t a, hipEvent_t b) {
             if (wide) {
@@ -1663,7 +1780,9 @@ static void launch_step(const StepConsts &s, const waam_step_fields &f, hipStrea
         });
     timed_launch(WAAM_K_STREAM, [&](hipEvent_t a, hipEvent_t b) {
         dim3 gs(g3.x, g3.y, (unsigned)((d.slow() + kStreamSlices - 1) / kStreamSlices));
-        if (separate)
+        if (seg)
+            launch_k(capture, k_step_stream_seg, dim3(kSegBlocks), block3(), st, a, b, f, s);
+        else if (separate)
             launch_k(capture, k_step_stream<float, false>, gs, block3(), st, a, b, f, s);
         else if (f32)
             launch_k(capture, k_step_stream<float, true>, g3, block3(), st, a, b, f, s);
@@ -1773,7 +1892,7 @@ int64_t waam_noalloc_max_cells(void) { return (int64_t)kNoAllocMaxCells; }
 int64_t waam_quiet_bytes(int nx, int ny, int nz, int layout)
 {
     if (nx <= 0 || ny <= 0 || nz <= 0 || (layout != kLayoutXYZ && layout != kLayoutZYX)) return 0;
-    return 2 * (int64_t)quiet_segments(Dims{nx, ny, nz, layout});
+    return (int64_t)quiet_bytes(Dims{nx, ny, nz, layout});
 }
 
 int waam_step_fused_noalloc(const waam_step_params *pp, const waam_step_fields *ff, void *stream)

@@ -172,7 +172,9 @@ def P(x) -> int:
     """Device pointer of a DeviceField or tensor (0 for None).
 
     Only contiguous tensors resident on the GPU are accepted: a host pointer handed to a kernel
-    would be dereferenced on the device.
+    would be dereferenced on the device. The address keeps no reference to the tensor: pass a
+    tensor a name holds, never a temporary (``P(x.double())`` frees the copy, and the allocator
+    may hand its memory to the next argument, before the call).
     """
     if x is None:
         return 0
