@@ -191,10 +191,11 @@ def _rollout_case(nx, ny, nz, preheat_K=1650.0, warm_steps=10):
     return tw, path
 
 
-def _oracle_coupled(tw, ro, theta, f64=True):
+def _oracle_coupled(tw, ro, theta, f64=True, with_T=False):
     """The rollout's forward on the oracle (binary64 build by default) from its starting state:
     the same thermal chain, advection by the rollout's velocity rule, and (advection "flow")
-    the same forces, MRT collision and Mach clamp. Returns the final f_l in (nx, ny, nz) order."""
+    the same forces, MRT collision and Mach clamp. Returns the final f_l in (nx, ny, nz) order
+    (with_T: the final (T, f_l))."""
     from oracle.oracle import cos_sin_f32, lib, ptr
     from waam_solver_amd.field import tensor_of
     from waam_solver_amd.physics import weld_forces
@@ -269,7 +270,7 @@ def _oracle_coupled(tw, ro, theta, f64=True):
                          tw.C_darcy, *d)
         L.or_clamp_velocity_mach(ptr(u[0]), ptr(u[1]), ptr(u[2]), ptr(flags), ro.u_cap, *d)
         f, f2 = f2, f
-    return fl
+    return (T, fl) if with_T else fl
 
 
 def _wd_loss(fl, i_c, tw, target, tau):

@@ -419,8 +419,10 @@ __global__ void __launch_bounds__(kBlock) k_step_begin(waam_step_fields f, StepC
     double sum = 0.0, comp = 0.0;
     size_t N = d.n();
     /* the arc's weights are exactly zero outside its (i, j) box (exp underflows to 0 there), so
-     * on a grid wider than the box the sums run over the box's columns only (the compensated
-     * sum does not depend on the order of its terms); the flags and liquid fractions of kUnroll
+     * on a grid wider than the box the sums run over the box's columns only. The TwoSum pairs
+     * carry about 100 significand bits, so the total rounded to fp32 does not depend on the
+     * order of its terms in practice (it is not order-independent in exact arithmetic: the
+     * compensation terms are summed plainly); the flags and liquid fractions of several
      * iterations are loaded together */
     const int bi0 = imax(0, at.box_i0), bi1 = imin(d.nx - 1, at.box_i1);
     const int bj0 = imax(0, at.box_j0), bj1 = imin(d.ny - 1, at.box_j1);
@@ -453,6 +455,55 @@ __global__ void __launch_bounds__(kBlock) k_step_begin(waam_step_fields f, StepC
                 two_sum(sum, (double)(wg * wdep), ss, e);
                 sum = ss;
                 comp += e;
+            }
+        }
+    } else if (d.fast() % 4 == 0) {
+        /* the box's rows in groups of four cells along the contiguous axis (the fast extent of
+         * the box widened to multiples of four: the cells this adds have weight exactly 0, and
+         * adding 0 leaves a TwoSum pair unchanged); one 4-byte flag load per group, and the
+         * group's f_l (one 16-byte load) only when a cell of it is not GAS */
+        const bool zyx = d.layout == kLayoutZYX;
+        const int fa0 = zyx ? (bi0 & ~3) : 0;
+        const int fa1 = zyx ? imin(d.nx, (bi1 + 4) & ~3) : d.nz;
+        const unsigned ng = (unsigned)imax(0, (fa1 - fa0) >> 2);
+        const unsigned nslow = zyx ? (unsigned)d.nz : bi;
+        const size_t ngroups = (size_t)ng * bj * nslow;
+        constexpr uint32_t kAllGas = 0x01010101u * (uint32_t)GAS;
+        constexpr int kG = 4;
+        for (size_t base = (size_t)blockIdx.x * blockDim.x + threadIdx.x; do_sums && base < ngroups;
+             base += kG * stride) {
+            uint32_t fw[kG];
+            size_t c0[kG];
+#pragma unroll
+            for (int u = 0; u < kG; ++u) {
+                const size_t t = base + u * stride;
+                const bool in = t < ngroups;
+                const unsigned tt = in ? (unsigned)t : 0u;
+                const unsigned r = tt / ng;
+                const int g = (int)(tt - r * ng);
+                const unsigned sl = r / bj;
+                const int j = bj0 + (int)(r - sl * bj);
+                c0[u] = zyx ? d.idx(fa0 + 4 * g, j, (int)sl) : d.idx(bi0 + (int)sl, j, 4 * g);
+                fw[u] = in ? *reinterpret_cast<const uint32_t *>(f.flags_in + c0[u]) : kAllGas;
+            }
+#pragma unroll
+            for (int u = 0; u < kG; ++u) {
+                if (fw[u] == kAllGas) continue;
+                const float4 fl4 = *reinterpret_cast<const float4 *>(f.fl_in + c0[u]);
+                const float flv[4] = {fl4.x, fl4.y, fl4.z, fl4.w};
+                int i0, j0, k0;
+                cell_ijk(c0[u], d, i0, j0, k0);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    if (((fw[u] >> (8 * e)) & 0xffu) == GAS) continue;
+                    const int i = zyx ? i0 + e : i0, k = zyx ? k0 : k0 + e;
+                    float wg, wdep;
+                    if (!arc_cell_weights(ap, i, j0, k, flv[e], wg, wdep)) continue;
+                    double ss, er;
+                    two_sum(sum, (double)(wg * wdep), ss, er);
+                    sum = ss;
+                    comp += er;
+                }
             }
         }
     } else {
