@@ -5,9 +5,9 @@ Per operator, against central differences of the oracle's binary64 build (libora
   <clamp(u), lam_u>, perturbing populations, forces and liquid fractions (the drag) together;
 - the force clamp, hydrostatic gravity and buoyancy (waam_cadj_forces), perturbing the raw
   forces, T and f_l.
-Through the rollout: the gradient of the smoothed pool width/depth loss with respect to (eta,
-heat-loss factor, sigma scale) over coupled steps on 128x64x48 cells against central
-differences of the same forward (the product's fp32 kernels, h = 1e-3 and 1e-4).
+Through the rollout: the gradient of the smoothed pool width/depth loss (plus a temperature
+misfit) with respect to (eta, heat-loss factor, sigma scale) over coupled steps on 128x64x48
+cells against central differences of the same forward on the oracle's binary64 build.
 """
 
 from __future__ import annotations
@@ -273,48 +273,62 @@ def _oracle_coupled(tw, ro, theta, f64=True, with_T=False):
     return (T, fl) if with_T else fl
 
 
-def _wd_loss(fl, i_c, tw, target, tau):
+def _c4_loss(T, fl, i_c, tw, target, T_target, tau=0.1):
+    """The smoothed pool width/depth misfit (in cells, squared) plus the mean squared
+    temperature misfit over 100 K (the heat-loss factor moves the pool's size little, the
+    temperature field visibly)."""
     from waam_solver_amd.adjoint import pool_wd_surrogate
 
     W, D = pool_wd_surrogate(fl, i_c, tw.nz_solid, tw.grid.layout, tw.grid.dx, tau=tau)
     dx = tw.grid.dx
-    return ((W - target[0]) / dx) ** 2 + ((D - target[1]) / dx) ** 2
+    dT = (T.double() - T_target.to(T.device)) / 100.0
+    return ((W - target[0]) / dx) ** 2 + ((D - target[1]) / dx) ** 2 + (dT * dT).mean()
 
 
-@pytest.mark.parametrize("advection,steps", [("none", 60), ("frozen", 60), ("flow", 60)])
-def test_rollout_gradient_against_binary64_differences(advection, steps):
-    """dL/d(eta, heat-loss factor, sigma scale) of the smoothed pool width/depth loss through the
-    rollout (64x40x32, checkpoints every 20 steps) against central differences (h = 1e-4) of
-    the same forward on the oracle's binary64 build."""
-    print(f"[cadj] rollout gradient, advection {advection}, {steps} steps", flush=True)
+@pytest.mark.parametrize("advection,steps,h", [("none", 400, 1e-6), ("frozen", 400, 1e-6),
+                                               ("flow", 150, 1e-4)])
+def test_rollout_gradient_against_binary64_differences(advection, steps, h):
+    """dL/d(eta, heat-loss factor, sigma scale) through the rollout on 128x64x48 cells with
+    checkpoints every 50 steps, each component against central differences of the same forward
+    on the oracle's binary64 build, to 1e-3 relative.
+
+    The pool melts within the first steps (plate preheated to just below the solidus), so many
+    cells cross the phase map's kinks: differences at h = 1e-4 straddle some of them and sit
+    up to 0.5 % off the derivative over 400 steps, at h = 1e-6 they agree with it (observed
+    <= 6e-5). With the flow, the Marangoni force drives cells to the Mach clamp by step ~150 and
+    the forward turns chaotic: from ~250 steps the binary64 differences no longer converge as h
+    shrinks (the tangent map grows without bound), so the flow case stops at 150 steps, where
+    differences at h = 1e-4 ... 1e-7 agree with each other to 2e-5 and with the adjoint to 7e-4
+    (the adjoint runs along the fp32 trajectory, the differences along the binary64 one)."""
+    print(f"[cadj] rollout gradient, advection {advection}, {steps} steps, h {h}", flush=True)
     from waam_solver_amd.adjoint import CoupledRollout, CoupledRolloutFn
 
-    tw, path = _rollout_case(64, 40, 32)
+    tw, path = _rollout_case(128, 64, 48)
     g = tw.grid
-    ro = CoupledRollout(tw, steps, path, checkpoint_every=20, advection=advection)
+    ro = CoupledRollout(tw, steps, path, checkpoint_every=50, advection=advection)
     i_c = int(path(steps - 1)[0] / g.dx)
-    tau = 0.1
-    fl_t = torch.from_numpy(_oracle_coupled(tw, ro, THETA_TARGET))
-    target = tuple(float(v) for v in _wd_target(fl_t, i_c, tw, tau))
+    T_t, fl_t = (torch.from_numpy(a) for a in _oracle_coupled(tw, ro, THETA_TARGET, with_T=True))
+    target = tuple(float(v) for v in _wd_target(fl_t, i_c, tw, 0.1))
+    T_t = T_t.double()
     theta = torch.tensor(THETA0, dtype=torch.float64, requires_grad=True)
     T, fl = CoupledRolloutFn.apply(theta, ro)
-    loss = _wd_loss(fl, i_c, tw, target, tau)
+    loss = _c4_loss(T, fl, i_c, tw, target, T_t)
     loss.backward()
     adj = theta.grad.numpy()
-    h = 1e-4
     fd = np.zeros(3)
     for a in range(3):
         tp, tm = list(THETA0), list(THETA0)
         tp[a] += h
         tm[a] -= h
-        lp = float(_wd_loss(torch.from_numpy(_oracle_coupled(tw, ro, tp)), i_c, tw, target, tau))
-        lm = float(_wd_loss(torch.from_numpy(_oracle_coupled(tw, ro, tm)), i_c, tw, target, tau))
+        lp = float(_c4_loss(*(torch.from_numpy(x) for x in
+                              _oracle_coupled(tw, ro, tp, with_T=True)), i_c, tw, target, T_t))
+        lm = float(_c4_loss(*(torch.from_numpy(x) for x in
+                              _oracle_coupled(tw, ro, tm, with_T=True)), i_c, tw, target, T_t))
         fd[a] = (lp - lm) / (2 * h)
-    print(f"[cadj] {advection}: loss {float(loss):.6e}, adjoint {adj}, differences {fd}",
-          flush=True)
-    scale = np.abs(fd).max()
-    assert scale > 0.0
-    assert np.all(np.abs(adj - fd) <= 1e-3 * scale), (adj, fd)
+    print(f"[cadj] {advection}: loss {float(loss):.6e}, adjoint {adj}, differences {fd}, "
+          f"relative {np.abs(adj - fd) / np.abs(fd)}", flush=True)
+    assert np.all(np.abs(fd) > 0.0)
+    assert np.all(np.abs(adj - fd) <= 1e-3 * np.abs(fd)), (adj, fd)
 
 
 def _wd_target(fl, i_c, tw, tau):
