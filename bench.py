@@ -212,12 +212,6 @@ def step_bytes(ncell: int, solid: float, gas: float, pop: str) -> float:
             + STEP_B_HAZ * (ncell - gas))
 
 
-def rel_l2_device(a, b) -> float:
-    """||a - b|| / ||b|| in float64 on the device."""
-    a, b = a.double(), b.double()
-    return float(((a - b).norm() / b.norm().clamp_min(1e-300)).item())
-
-
 def timed_run(w: dict, args, pop: str):
     """Build the seeded twin with population storage `pop`, step the warm-up, then time
     args.steps steps. Returns (twin, seconds, host seconds, (solid, gas) averaged over the
@@ -260,27 +254,16 @@ def timed_run(w: dict, args, pop: str):
     return tw, el, t_host, classes, sc, timing
 
 
-def compare_form(w: dict, args, pop: str, ref):
+def compare_form(w: dict, args, pop: str):
     """The same workload, warm-up and timed steps with population storage `pop`: (twin, MLUPS,
-    summary), the summary holding the relative L2 of T, phi, f_l and |u| between this run's final
-    state and that of the run `ref` (both start from the same seeded state)."""
-    import torch
-
-    from waam_solver_amd.field import tensor_of
-
+    summary). Speed only: the few steps from a cold plate that the timed region covers melt
+    nothing, so the forms' states do not differ yet there; their error against a binary64
+    build is measured over 2000 steps with a molten pool by tests/precision_c3.py."""
     tw, el, _, classes, _, _ = timed_run(w, args, pop)
     g = tw.grid
     mlups = g.n_cells * args.steps / el / 1e6
-    r = ref.grid
-    err = {k: round(rel_l2_device(tensor_of(getattr(g, k)), tensor_of(getattr(r, k))), 8)
-           for k in ("T", "phi", "f_l")}
-    speed = lambda gg: torch.sqrt(sum(tensor_of(getattr(gg, c)).double() ** 2  # noqa: E731
-                                      for c in ("ux", "uy", "uz")))
-    err["|u|"] = round(rel_l2_device(speed(g), speed(r)), 8)
-    flips = int((tensor_of(g.flags) != tensor_of(r.flags)).sum().item())
     sb = step_bytes(g.n_cells, classes[0], classes[1], pop)
     out = {"value": round(mlups, 2), "ms_per_step": round(el * 1e3 / args.steps, 4),
-           "rel_l2_vs_value_run": err, "flags_differing": flips,
            "step_bytes_per_cell": round(sb / g.n_cells, 1),
            "step_frac": round(sb * args.steps / el / 1e9 / HBM_PEAK_GBS, 4)}
     return tw, mlups, out
@@ -490,22 +473,19 @@ def main() -> int:
         # the other population forms from the same seeded state over the same steps: fp32 (the
         # plain method) first, then the remaining 16-bit form
         forms = {}
-        ref32, m32, forms["fp32"] = compare_form(w, args, "fp32", tw) if args.pop != "fp32" \
-            else (tw, mlups, None)
-        if forms["fp32"] is None:
-            del forms["fp32"]
-        for pop in ("fp16", "bf16"):
+        m32 = mlups
+        for pop in ("fp32", "fp16", "bf16"):
             if pop == args.pop:
                 continue
-            other, m, out = compare_form(w, args, pop, ref32)
-            out["speedup_vs_fp32"] = round(m / m32, 3)
+            other, m, out = compare_form(w, args, pop)
+            if pop == "fp32":
+                m32 = m
+            else:
+                out["speedup_vs_fp32"] = round(m / m32, 3)
             forms[pop] = out
             del other
             torch.cuda.empty_cache()
         forms["value_speedup_vs_fp32"] = round(mlups / m32, 3)
-        if ref32 is not tw:
-            del ref32
-        torch.cuda.empty_cache()
     graph = None
     if fused:
         after = args.warmup + args.steps

@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define WAAM_ABI_VERSION 4
+#define WAAM_ABI_VERSION 5
 
 enum { WAAM_FLAG_FLUID = 0, WAAM_FLAG_SOLID = 1, WAAM_FLAG_GAS = 2, WAAM_FLAG_IFACE = 4 };
 /* the moving lid of the lid-driven cavity (waam_cavity_step); not a flag of the coupled step */
@@ -532,6 +532,11 @@ typedef struct waam_step_fields {
      * list of the segments a steady step sweeps; written by every step, contents need no
      * initial value */
     uint8_t *quiet;
+    /* temperature-dependent material (params.use_tables; NULL otherwise): cp_rho_prev holds
+     * the previous step's refresh_thermal_properties cp*rho (kernels.py:1710-1772), read by the
+     * ceiling that follows arc injection; the step writes its own refresh into cp_rho,
+     * alpha_lu, dgamma_lu and tau for every cell (cp_rho_prev and cp_rho then swap roles) */
+    float *cp_rho_prev, *cp_rho, *alpha_lu, *dgamma_lu, *tau;
     void *scratch;
     int32_t max_tracers, nx, ny, nz;
     int32_t layout; /* WAAM_LAYOUT_XYZ or WAAM_LAYOUT_ZYX, shared by every field */
@@ -540,7 +545,7 @@ typedef struct waam_step_fields {
 
 typedef struct waam_step_params {
     /* switches */
-    int32_t is_welding, collision /* 0 MRT, 1 SRT */, arc_kind /* 0 Gaussian, 1 Goldak */;
+    int32_t is_welding, collision /* 0 MRT, 1 SRT, 2 SRT, tau per cell */, arc_kind /* 0 Gaussian, 1 Goldak */;
     int32_t arc_applied /* arc + ceiling already applied this step (droplet steps) */;
     int32_t pending_stream, forces_from_fields, store_forces;
     int32_t enable_vof, enable_wetting, enable_csf, enable_heat_loss, enable_conv, enable_rad;
@@ -582,6 +587,17 @@ typedef struct waam_step_params {
     float dir_x, dir_y, dir_z, lookback, T_freeze;
     /* recoil (recoil_pa and recoil_T_ref: constant form) and gas shear (peak stress, Pa) */
     float recoil_pa, recoil_T_ref, P_vap_ref, T_boil, L_vap, R_vap, C_acc, gas_tau;
+    /* temperature-dependent material (reference coupled_step.py:168-245, 307-312, 364-373,
+     * 426-449): 1 takes each FLUID/IFACE cell's cp, k, mu and dgamma/dT from `tables` at its
+     * incoming temperature and `refs` on SOLID and GAS cells (refresh_thermal_properties), and
+     * runs the _variable forms of conduction, floor, ceilings, phase map (with H_sol_tab,
+     * H_liq_tab), Marangoni, remelt and trailing solidification; the four property fields of
+     * waam_step_fields must be set. variable_tau 1 (with collision 2) collides with tau per
+     * cell (collide_srt_variable_tau). Row-segment skipping is off on these steps. */
+    int32_t use_tables, variable_tau;
+    float H_sol_tab, H_liq_tab;
+    waam_property_refs refs;
+    waam_material_tables tables;
 } waam_step_params;
 
 int waam_step_fused(const waam_step_params *p, const waam_step_fields *f, void *stream);

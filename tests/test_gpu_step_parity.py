@@ -66,16 +66,20 @@ def test_coupled_step_parity(hip_device, case, engine, layout):
     assert abs(tel["pool_depth_mm"] - round(pm["pool_depth_m"] * 1000, 3)) < 1e-9
 
 
+@pytest.mark.parametrize("engine", ["fused", "reference"])
 @pytest.mark.parametrize("layout", [1, 0])
 @pytest.mark.parametrize("first_welding", [True, False])
-def test_table_material_parity(hip_device, layout, first_welding):
-    """Temperature-dependent cp, k, mu and dgamma/dT (per-cell properties, variable-tau SRT).
+def test_table_material_parity(hip_device, layout, first_welding, engine):
+    """Temperature-dependent cp, k, mu and dgamma/dT (per-cell properties, variable-tau SRT),
+    through the fused step (properties from the tables inside its sweeps) and the
+    reference-order kernels.
 
     The per-cell property fields start at zero and are first written by the refresh that
     precedes the thermal update, so a first step that welds clamps enthalpy against
     cp_rho = 0 (reference solvers/coupled_step.py:104-108 before :168-182); the other
     variant starts with one step without the arc so the clamp sees refreshed properties."""
-    tw, ot = build_pair("tables_mrt", layout=layout)
+    print(f"[tables] engine {engine} layout {layout} first_welding {first_welding}", flush=True)
+    tw, ot = build_pair("tables_mrt", layout=layout, engine=engine)
     g = tw.grid
     done = 0
     if not first_welding:
@@ -90,6 +94,8 @@ def test_table_material_parity(hip_device, layout, first_welding):
     # the refreshed properties span the table range where the metal is hot
     cp = tw.grid.cp_rho_field.to_numpy()
     assert float(cp.max()) > float(cp.min())
+    if engine == "fused":
+        assert tw._fused is not None  # the fused stepper ran the steps
 
 
 @pytest.mark.parametrize("engine", ["fused", "reference"])

@@ -16,7 +16,7 @@ import torch  # noqa: F401
 
 LIB_PATH = Path(__file__).resolve().parent / "libwaam_hip.so"
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 FLAG_FLUID, FLAG_SOLID, FLAG_GAS, FLAG_IFACE = 0, 1, 2, 4
 LAYOUT_XYZ, LAYOUT_ZYX = 0, 1
 

@@ -52,6 +52,7 @@
 #include "forces_cell.h"
 #include "lbm_cell.h"
 #include "pop_store.h"
+#include "tables_cell.h"
 #include "thermal_cell.h"
 #include "vof_cell.h"
 
@@ -318,7 +319,9 @@ __device__ __forceinline__ bool segment_active(const QuietRec &q, int bx, int j,
 
 /* 1b — the active segments of a steady step, one thread per segment, appended to the list
  * (wavefront-aggregated; the order is free: each segment's results depend only on the
- * sweeps' inputs) */
+ * sweeps' inputs). Appending a block's segments in order with one atomic per block built the
+ * list faster (23 against 35 us at 768x384x192) but slowed the sweeps over it by 50 us: the
+ * wavefront-sized runs in arrival order spread the sweeps' concurrent accesses wider. */
 __global__ void __launch_bounds__(kBlock) k_step_quiet_list(waam_step_fields f, StepConsts s)
 {
     const Dims &d = s.d;
@@ -341,12 +344,11 @@ __global__ void __launch_bounds__(kBlock) k_step_quiet_list(waam_step_fields f, 
     if (act) q.list[base + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)sg;
 }
 
-/* the cell of this lane in the listed segment t of a sweep over the list (block3() blocks: the
- * lanes along the contiguous axis, one wavefront per threadIdx.y) */
-__device__ __forceinline__ bool seg_cell(const QuietRec &q, int t, const Dims &d, size_t &seg,
-                                         int &i, int &j, int &k)
+/* the cell of this lane in segment seg (block3() blocks: the lanes along the contiguous axis,
+ * one wavefront per threadIdx.y) */
+__device__ __forceinline__ bool seg_cell(const QuietRec &q, size_t seg, const Dims &d, int &i,
+                                         int &j, int &k)
 {
-    seg = (size_t)q.list[t];
     const int bx = (int)(seg % (size_t)q.nbx);
     const size_t r = seg / (size_t)q.nbx;
     j = (int)(r % (size_t)d.ny);
@@ -355,6 +357,23 @@ __device__ __forceinline__ bool seg_cell(const QuietRec &q, int t, const Dims &d
     i = d.layout == kLayoutZYX ? a : b;
     k = d.layout == kLayoutZYX ? b : a;
     return a < d.fast();
+}
+
+/* a wavefront's walk over the active-segment list (entries w, w + nw, ... of a grid of nw
+ * wavefronts): the entries are loaded 64 at a time, one per lane, and handed out by
+ * shuffles, so the list load drops out of each segment's chain of memory round trips */
+template <class Body>
+__device__ __forceinline__ void for_listed_segments(const QuietRec &q, int n, Body &&body)
+{
+    const int nw = (int)(gridDim.x * blockDim.y);
+    const int w = (int)(blockIdx.x * blockDim.y + threadIdx.y);
+    const int lane = (int)threadIdx.x;
+    for (long long t0 = w; t0 < n; t0 += 64LL * nw) {
+        const long long tl = t0 + (long long)lane * nw;
+        const int mine = tl < n ? q.list[tl] : 0;
+        const int cnt = (int)std::min<long long>(64, (n - t0 + nw - 1) / nw);
+        for (int it = 0; it < cnt; ++it) body((size_t)(unsigned)__shfl(mine, it));
+    }
 }
 
 /* the wavefront's segment ends the step quiet (`quiet`: this lane's cell is, or it has none):
@@ -586,24 +605,30 @@ __global__ void __launch_bounds__(kBlock) k_step_arc_total(waam_step_fields f, S
 struct HView {
     const float *H_in, *fl_in;
     const uint8_t *flags_in;
+    /* temperature-dependent material: the ceiling's cp*rho per cell, from the previous step's
+     * refresh (clamp_enthalpy_ceiling_variable_cp, kernels.py:1987-2010) */
+    const float *cp_prev;
     Dims d;
     ArcParams ap;
-    float epw, H_liq, cap_l, cap_s;
+    float epw, H_liq, cap_l, cap_s, T_cap, T_l;
     int box_i0, box_i1, box_j0, box_j1;
-    bool arc_on, cap_on;
+    bool arc_on, cap_on, tab;
 
-    __device__ __forceinline__ float value(float h, float fl, int i, int j, int k) const
+    __device__ __forceinline__ float value(float h, float fl, float cp, int i, int j, int k) const
     {
         if (arc_on && i >= box_i0 && i <= box_i1 && j >= box_j0 && j <= box_j1) {
             float wg, wdep;
             if (arc_cell_weights(ap, i, j, k, fl, wg, wdep)) h += epw * wg * wdep;
         }
-        if (cap_on) h = ceiling_clamp(h, H_liq, cap_l, cap_s);
+        if (cap_on) {
+            if (tab) h = ceiling_clamp(h, H_liq, H_liq + cp * (T_cap - T_l), cp * T_cap);
+            else h = ceiling_clamp(h, H_liq, cap_l, cap_s);
+        }
         return h;
     }
     __device__ __forceinline__ float at(size_t c, int i, int j, int k) const
     {
-        return value(H_in[c], fl_in[c], i, j, k);
+        return value(H_in[c], fl_in[c], tab ? cp_prev[c] : 0.0f, i, j, k);
     }
     /* _masked_at on the enthalpy view: clamped index, GAS neighbours give the fallback (loads
      * first, select last, so the loads are not sunk behind the flag test) */
@@ -614,7 +639,7 @@ struct HView {
         int kk = imax(0, imin(k, d.nz - 1));
         size_t c = d.idx(ii, jj, kk);
         uint8_t fg = flags_in[c];
-        float v = value(H_in[c], fl_in[c], ii, jj, kk);
+        float v = value(H_in[c], fl_in[c], tab ? cp_prev[c] : 0.0f, ii, jj, kk);
         return fg == GAS ? fallback : v;
     }
 };
@@ -642,11 +667,32 @@ __device__ __forceinline__ HView make_hview(const waam_step_fields &f, const Ste
     hv.box_j1 = at.box_j1;
     hv.arc_on = s.p.is_welding && !s.p.arc_applied && hv.epw > 0.0f;
     hv.cap_on = s.p.is_welding && !s.p.arc_applied && s.p.enable_cap;
+    hv.tab = s.p.use_tables != 0;
+    hv.cp_prev = f.cp_rho_prev;
+    hv.T_cap = s.p.T_cap;
+    hv.T_l = s.p.T_l;
     return hv;
 }
 
 /* thermal chain of one cell (reference kernels.py:497-510, 1953-1984, 1611-1682, 2030-2082,
  * 1937-1950, 1519-1556, 3016-3032, 2013-2027 in coupled_step order) */
+/* the step's refresh_thermal_properties values of one cell, stored (temperature-dependent
+ * material) */
+__device__ __forceinline__ void store_props(const waam_step_fields &f, size_t c,
+                                            const CellProps &pr)
+{
+    f.cp_rho[c] = pr.cp_rho;
+    f.alpha_lu[c] = pr.alpha_lu;
+    f.dgamma_lu[c] = pr.dgamma_lu;
+    f.tau[c] = pr.tau;
+}
+
+/* TAB: temperature-dependent material — the cell's properties from the tables at its incoming
+ * temperature (the constants on SOLID and GAS cells; reference coupled_step.py:168-182) in
+ * place of the constants, the _variable forms of conduction, floor, ceiling and phase map
+ * (kernels.py:1775-1863, 1937-1950, 1987-2010), and the properties stored for the rest of the
+ * step and the next step's first ceiling */
+template <bool TAB>
 __device__ __forceinline__ void thermal_cell(const waam_step_fields &f, const StepConsts &s,
                                              int i, int j, int k, size_t c, uint8_t flag,
                                              float h_gas)
@@ -657,10 +703,13 @@ __device__ __forceinline__ void thermal_cell(const waam_step_fields &f, const St
         /* GAS: only the two phase updates touch it (ceiling, thermal and losses skip GAS);
          * h_gas is the cell's enthalpy, loaded by the caller */
         float h = h_gas, T, fl;
-        phase_of(h, s.pc, T, fl);
+        PhaseConsts pc = s.pc;
+        if (TAB) pc.cp_rho = p.refs.cp_rho_ref + 1e-9f;
+        phase_of(h, pc, T, fl);
         f.H_out[c] = h;
         f.T_out[c] = T;
         f.fl_out[c] = fl;
+        if (TAB) store_props(f, c, cell_props(p.tables, p.refs, 1, GAS, 0.0f));
         return;
     }
     HView hv = make_hview(f, s);
@@ -673,7 +722,9 @@ __device__ __forceinline__ void thermal_cell(const waam_step_fields &f, const St
     float T_km = masked_at(f.T_in, f.flags_in, i, j, k - 1, T_c, d);
     float T_kp = masked_at(f.T_in, f.flags_in, i, j, k + 1, T_c, d);
     float lap = T_ip + T_im + T_jp + T_jm + T_kp + T_km - 6.0f * T_c;
-    float dH = p.cp_rho * p.alpha_lu * lap * 1.0f;
+    CellProps pr{p.cp_rho, p.alpha_lu, p.dgamma, 0.0f};
+    if (TAB) pr = cell_props(p.tables, p.refs, 1, flag, T_c);
+    float dH = pr.cp_rho * pr.alpha_lu * lap * 1.0f;
     float H_c = hv.at(c, i, j, k);
     if (flag != SOLID) {
         float u = f.ux[c], v = f.uy[c], w = f.uz[c];
@@ -698,11 +749,20 @@ __device__ __forceinline__ void thermal_cell(const waam_step_fields &f, const St
     if (p.enable_heat_loss) {
         int n_exp = exposed_faces(f.flags_in, i, j, k, d);
         if (n_exp != 0) h -= loss_flux(T_c, s.lc) * (float)n_exp * s.lc.dt / s.lc.dx;
-        if (h < s.h_min) h = s.h_min;
+        const float h_min = TAB ? pr.cp_rho * p.T_amb : s.h_min;
+        if (h < h_min) h = h_min;
     }
-    if (p.enable_cap) h = ceiling_clamp(h, s.pc.H_liq, s.cap_l, s.cap_s);
+    if (p.enable_cap) {
+        if (TAB)
+            h = ceiling_clamp(h, s.pc.H_liq, s.pc.H_liq + pr.cp_rho * (p.T_cap - p.T_l),
+                              pr.cp_rho * p.T_cap);
+        else
+            h = ceiling_clamp(h, s.pc.H_liq, s.cap_l, s.cap_s);
+    }
     float T, fl;
-    phase_of(h, s.pc, T, fl);
+    PhaseConsts pc = s.pc;
+    if (TAB) pc.cp_rho = pr.cp_rho + 1e-9f;
+    phase_of(h, pc, T, fl);
     /* HAZ diagnostics (reference kernels.py:3027-3032, 2023-2027); they run before the VOF
      * flag update, so they see this step's incoming flags. Every load precedes the first store
      * (the field pointers carry no alias information). */
@@ -713,6 +773,7 @@ __device__ __forceinline__ void thermal_cell(const waam_step_fields &f, const St
     if (T > T_max) f.T_max[c] = T;
     f.dT_dt[c] = (T - T_prev) * s.inv_dt;
     f.T_prev[c] = T;
+    if (TAB) store_props(f, c, pr);
 }
 
 /* advect_phi + reinitialize_phi of one cell (reference kernels.py:2320-2391) */
@@ -724,17 +785,18 @@ __device__ __forceinline__ float vof_advect_cell(const waam_step_fields &f, cons
 }
 
 /* thermal chain and VOF advection of one cell */
+template <bool TAB>
 __device__ __forceinline__ void thermal_vof_cell(const waam_step_fields &f, const StepConsts &s,
                                                  int i, int j, int k, size_t c, uint8_t flag)
 {
     float phi_new = s.p.enable_vof ? vof_advect_cell(f, s.d, i, j, k, flag) : 0.0f;
-    thermal_cell(f, s, i, j, k, c, flag, f.H_in[c]);
+    thermal_cell<TAB>(f, s, i, j, k, c, flag, f.H_in[c]);
     if (s.p.enable_vof) f.phi_tmp[c] = phi_new;
 }
 
 /* 2 — thermal chain and VOF advection of every cell that is not FLUID/IFACE; lists those
  *     (LIST) unless the previous step's kernel 7 already has */
-template <bool LIST>
+template <bool LIST, bool TAB>
 __global__ void __launch_bounds__(256) k_step_thermal(waam_step_fields f, StepConsts s)
 {
     const Dims &d = s.d;
@@ -749,7 +811,7 @@ __global__ void __launch_bounds__(256) k_step_thermal(waam_step_fields f, StepCo
     float phi_new = (in && s.p.enable_vof) ? vof_advect_cell(f, d, i, j, k, flag) : 0.0f;
     if (LIST) list_append(in && is_surface_class(flag), c, cell_list(f, s.seg_len, 0), d.n());
     if (!in || is_surface_class(flag)) return;
-    thermal_cell(f, s, i, j, k, c, flag, h_in);
+    thermal_cell<TAB>(f, s, i, j, k, c, flag, h_in);
     if (s.p.enable_vof) f.phi_tmp[c] = phi_new;
 }
 
@@ -758,12 +820,9 @@ __global__ void __launch_bounds__(256) k_step_thermal_seg(waam_step_fields f, St
 {
     const Dims &d = s.d;
     const QuietRec q = quiet_rec(f, d);
-    const int n = *q.count;
-    const int nw = (int)(gridDim.x * blockDim.y);
-    for (int t = (int)(blockIdx.x * blockDim.y + threadIdx.y); t < n; t += nw) {
-        size_t seg;
+    for_listed_segments(q, *q.count, [&](size_t seg) {
         int i, j, k;
-        const bool in = seg_cell(q, t, d, seg, i, j, k);
+        const bool in = seg_cell(q, seg, d, i, j, k);
         const size_t c = in ? d.idx(i, j, k) : 0;
         const uint8_t flag = in ? f.flags_in[c] : (uint8_t)GAS;
         const float h_in = in ? f.H_in[c] : 0.0f;
@@ -771,14 +830,15 @@ __global__ void __launch_bounds__(256) k_step_thermal_seg(waam_step_fields f, St
          * the plate's segments (most of a steady step's list) skip those loads */
         float phi_new = 1.0f;
         if (in && s.p.enable_vof && flag != SOLID) phi_new = vof_advect_cell(f, d, i, j, k, flag);
-        if (!in || is_surface_class(flag)) continue;
-        thermal_cell(f, s, i, j, k, c, flag, h_in);
+        if (!in || is_surface_class(flag)) return;
+        thermal_cell<false>(f, s, i, j, k, c, flag, h_in);
         if (s.p.enable_vof) f.phi_tmp[c] = phi_new;
-    }
+    });
 }
 
 /* thermal chain (with enthalpy advection) and VOF advection of the listed FLUID/IFACE cells;
  * worker b of nb, 256 threads each */
+template <bool TAB>
 __device__ __forceinline__ void thermal_surface_list(const waam_step_fields &f,
                                                      const StepConsts &s, int b, int nb)
 {
@@ -791,14 +851,15 @@ __device__ __forceinline__ void thermal_surface_list(const waam_step_fields &f,
         size_t c = list_at(l, s_pref, t);
         int i, j, k;
         cell_ijk(c, d, i, j, k);
-        thermal_vof_cell(f, s, i, j, k, c, f.flags_in[c]);
+        thermal_vof_cell<TAB>(f, s, i, j, k, c, f.flags_in[c]);
     }
 }
 
 /* 3 — the listed FLUID/IFACE cells (list built by kernel 2) */
+template <bool TAB>
 __global__ void __launch_bounds__(256) k_step_thermal_surface(waam_step_fields f, StepConsts s)
 {
-    thermal_surface_list(f, s, blockIdx.x, gridDim.x);
+    thermal_surface_list<TAB>(f, s, blockIdx.x, gridDim.x);
 }
 
 /* 2+3 in one launch, when the previous step's kernel 7 has listed this step's FLUID/IFACE
@@ -806,6 +867,7 @@ __global__ void __launch_bounds__(256) k_step_thermal_surface(waam_step_fields f
  * latency-bound surface cells overlap the bandwidth-bound sweep), the rest sweep every other
  * cell as kernel 2 does. One call site of the cell routine (two inlined copies make the
  * compiler copy the argument structs to private memory). */
+template <bool TAB>
 __global__ void __launch_bounds__(256) k_step_thermal_all(waam_step_fields f, StepConsts s,
                                                           int lz)
 {
@@ -839,7 +901,7 @@ __global__ void __launch_bounds__(256) k_step_thermal_all(waam_step_fields f, St
         float h_in = f.H_in[c];
         float phi_new = s.p.enable_vof ? vof_advect_cell(f, d, i, j, k, flag) : 0.0f;
         if (!list && is_surface_class(flag)) return;
-        thermal_cell(f, s, i, j, k, c, flag, h_in);
+        thermal_cell<TAB>(f, s, i, j, k, c, flag, h_in);
         if (s.p.enable_vof) f.phi_tmp[c] = phi_new;
     }
 }
@@ -923,16 +985,13 @@ __global__ void __launch_bounds__(256) k_step_vof_flags_seg(waam_step_fields f, 
     if (blockIdx.x == 0)  // kernel 2 has consumed the list
         cell_list(f, s.seg_len, 0).count(threadIdx.y * blockDim.x + threadIdx.x) = 0;
     const QuietRec q = quiet_rec(f, d);
-    const int n = *q.count;
-    const int nw = (int)(gridDim.x * blockDim.y);
-    for (int t = (int)(blockIdx.x * blockDim.y + threadIdx.y); t < n; t += nw) {
-        size_t seg;
+    for_listed_segments(q, *q.count, [&](size_t seg) {
         int i, j, k;
-        const bool in = seg_cell(q, t, d, seg, i, j, k);
+        const bool in = seg_cell(q, seg, d, i, j, k);
         const size_t c = in ? d.idx(i, j, k) : 0;
         const uint8_t nf = in ? vof_flags_cell<true>(f, s, i, j, k, c) : (uint8_t)GAS;
         list_append(in && is_surface_class(nf), c, cell_list(f, s.seg_len, 1), d.n());
-    }
+    });
 }
 
 /* Pull of the reference stream (kernels.py:2991-3010) with the flags of the step it belongs
@@ -1019,7 +1078,10 @@ __device__ __forceinline__ void collide_cell(const waam_step_fields &f, const St
         fy += ty;
         fz += tz;
     }
-    if (marangoni_force(fs, fl, p.dgamma, tx, ty, tz)) {
+    /* temperature-dependent material: the cell's coefficient of this step's refresh
+     * (compute_marangoni_force_variable, kernels.py:2220-2269) */
+    const float dgamma = p.use_tables ? f.dgamma_lu[c] : p.dgamma;
+    if (marangoni_force(fs, fl, dgamma, tx, ty, tz)) {
         fx += tx;
         fy += ty;
         fz += tz;
@@ -1059,8 +1121,14 @@ __device__ __forceinline__ void collide_cell(const waam_step_fields &f, const St
         }
     }
     CellOut o;
-    if (p.collision == 0) mrt_cell(fin, fx, fy, fz, fl, p.omega_s, p.omega_b, p.C_darcy, o);
-    else srt_cell(fin, fx, fy, fz, fl, p.omega_s, p.C_darcy, o);
+    if (p.collision == 0) {
+        mrt_cell(fin, fx, fy, fz, fl, p.omega_s, p.omega_b, p.C_darcy, o);
+    } else {
+        /* collision 2: collide_srt_variable_tau (reference coupled_step.py:364-373), the
+         * relaxation time of the cell's refresh */
+        const float om = p.collision == 2 ? 1.0f / (f.tau[c] + 1e-6f) : p.omega_s;
+        srt_cell(fin, fx, fy, fz, fl, om, p.C_darcy, o);
+    }
 #pragma unroll
     for (int q = 0; q < Q; ++q) PS::store(f_dst, (size_t)q * N + c, q, o.f[q]);
     float u = o.u, v = o.v, w = o.w;
@@ -1110,11 +1178,14 @@ __device__ __forceinline__ uint8_t freeze_cell(const waam_step_fields &f, const 
     const waam_step_params &p = s.p;
     bool wH = false, wT = false, wfl = false, wphi = false, wflag = false, wu = false;
     float H = 0.0f;
+    /* temperature-dependent material: the cell's cp*rho of this step's refresh (the _variable
+     * remelt and trailing rules, kernels.py:2454-2486, 2524-2570) */
+    const float cp_rho = p.use_tables ? f.cp_rho[c] : p.cp_rho;
     if (flag == SOLID && T > p.T_s + 5.0f) {
-        float H_sol = p.cp_rho * p.T_s;
+        float H_sol = cp_rho * p.T_s;
         if (T >= p.T_l) {
             fl = 1.0f;
-            H = p.cp_rho * p.T_l + p.L_rho + p.cp_rho * (T - p.T_l);
+            H = cp_rho * p.T_l + p.L_rho + cp_rho * (T - p.T_l);
         } else {
             fl = (T - p.T_s) / (p.T_l - p.T_s + 1e-6f);
             H = H_sol + fl * p.L_rho;
@@ -1128,7 +1199,7 @@ __device__ __forceinline__ uint8_t freeze_cell(const waam_step_fields &f, const 
         float di = (float)i - at.arc_i, dj = (float)j - at.arc_j, dk = (float)k - arc_k;
         float proj = di * at.dir_x + dj * at.dir_y + dk * at.dir_z;
         if (!(proj > -p.lookback) && !(T >= p.T_freeze)) {
-            H = p.cp_rho * (p.T_s - 1.0f);
+            H = cp_rho * (p.T_s - 1.0f);
             T = p.T_s - 1.0f;
             fl = 0.0f;
             phi = 1.0f;
@@ -1283,13 +1354,10 @@ __global__ void __launch_bounds__(256) k_step_stream_seg(waam_step_fields f, Ste
 {
     const Dims &d = s.d;
     const QuietRec q = quiet_rec(f, d);
-    const int n = *q.count;
-    const int nw = (int)(gridDim.x * blockDim.y);
     const bool freeze_on = s.p.enable_freeze != 0;
-    for (int t = (int)(blockIdx.x * blockDim.y + threadIdx.y); t < n; t += nw) {
-        size_t seg;
+    for_listed_segments(q, *q.count, [&](size_t seg) {
         int i, j, k;
-        const bool in = seg_cell(q, t, d, seg, i, j, k);
+        const bool in = seg_cell(q, seg, d, i, j, k);
         const size_t c = in ? d.idx(i, j, k) : 0;
         const uint8_t fc = in ? f.flags_out[c] : (uint8_t)GAS;
         const uint8_t fp = in ? f.sflags_prev[c] : (uint8_t)GAS;
@@ -1300,7 +1368,7 @@ __global__ void __launch_bounds__(256) k_step_stream_seg(waam_step_fields f, Ste
         bool quiet = true;
         if (in) quiet = stream_rest_cell(f, s, i, j, k, c, fc, fp, T, phi, fl);
         record_quiet(q, seg, quiet, true);
-    }
+    });
 }
 
 /* 7a — the previous step's pull in a launch of its own, ahead of kernel 7 (which changes
@@ -1647,6 +1715,23 @@ static int check_fields(const char *who, const waam_step_fields &f)
     return 0;
 }
 
+/* the temperature-dependent material's fields, when a step uses them */
+static int check_tables(const char *who, const waam_step_params &p, const waam_step_fields &f)
+{
+    if (p.collision < 0 || p.collision > 2)
+        return set_error(WAAM_ERR_INVALID, "%s: unknown collision %d", who, p.collision);
+    if (p.collision == 2 && !p.use_tables)
+        return set_error(WAAM_ERR_INVALID, "%s: tau per cell needs use_tables", who);
+    if (!p.use_tables) return 0;
+    WAAM_RETURN_IF(check_ptrs(who, {f.cp_rho_prev, f.cp_rho, f.alpha_lu, f.dgamma_lu, f.tau}));
+    if (f.cp_rho_prev == f.cp_rho)
+        return set_error(WAAM_ERR_INVALID, "%s: cp_rho_prev aliases cp_rho", who);
+    for (int n : {p.tables.n_cp, p.tables.n_k, p.tables.n_mu, p.tables.n_dgamma})
+        if (n < 0 || n > WAAM_MAX_KNOTS)
+            return set_error(WAAM_ERR_INVALID, "%s: a material table holds %d knots", who, n);
+    return 0;
+}
+
 /* kernel arguments of a step: the scalar groups each kernel reads, derived once on the host */
 static StepConsts make_step_consts(const waam_step_params &p, const Dims &d)
 {
@@ -1655,7 +1740,8 @@ static StepConsts make_step_consts(const waam_step_params &p, const Dims &d)
     s.d = d;
     /* the quiet record is continued only on steady steps (see waam_step_params.quiet_skip) */
     s.p.quiet_skip = (p.quiet_skip && p.pending_stream == 1 && p.thermal_list_ready != 0 &&
-                      !p.arc_applied && !p.forces_from_fields && !p.force_reset_all)
+                      !p.arc_applied && !p.forces_from_fields && !p.force_reset_all &&
+                      !p.use_tables)
                          ? 1
                          : 0;
     s.arc.kind = p.arc_kind;
@@ -1672,6 +1758,12 @@ static StepConsts make_step_consts(const waam_step_params &p, const Dims &d)
     s.arc.pen = p.pen;
     s.arc.enable = p.surface_weight;
     s.pc = make_phase_consts(p.cp_rho, p.L_rho, p.T_s, p.T_l);
+    if (p.use_tables) {
+        /* update_phase_variable_cp takes the material's H_sol and H_liq (reference
+         * coupled_step.py:228-233); the cell's cp*rho comes per cell */
+        s.pc.H_sol = p.H_sol_tab;
+        s.pc.H_liq = p.H_liq_tab;
+    }
     s.lc = LossConsts{p.T_amb, p.h_conv, p.eps_rad, p.sigma_sb, p.dt, p.dx, p.enable_conv, p.enable_rad};
     s.cap_l = s.pc.H_liq + p.cp_rho * (p.T_cap - p.T_l);
     s.cap_s = p.cp_rho * p.T_cap;
@@ -1757,33 +1849,41 @@ static void launch_step(const StepConsts &s, const waam_step_fields &f, hipStrea
             launch_k(capture, k_step_quiet_list, dim3(grid_for(quiet_segments(d))), dim3(kBlock),
                      st, nullptr, b, f, s);
     });
+    /* kernel 3 (the listed surface cells), with or without the temperature-dependent material */
+    const bool tab = p.use_tables != 0;
+    auto launch_surface = [&](hipEvent_t a, hipEvent_t b) {
+        if (tab)
+            launch_k(capture, k_step_thermal_surface<true>, dim3(kListBlocks), dim3(kBlock), st, a,
+                     b, f, s);
+        else
+            launch_k(capture, k_step_thermal_surface<false>, dim3(kListBlocks), dim3(kBlock), st,
+                     a, b, f, s);
+    };
     if (seg) {
         timed_launch(WAAM_K_THERMAL, [&](hipEvent_t a, hipEvent_t b) {
             launch_k(capture, k_step_thermal_seg, dim3(kSegBlocks), block3(), st, a, b, f, s);
         });
-        timed_launch(WAAM_K_THERMAL_SURFACE, [&](hipEvent_t a, hipEvent_t b) {
-            launch_k(capture, k_step_thermal_surface, dim3(kListBlocks), dim3(kBlock), st, a, b, f,
-                     s);
-        });
+        timed_launch(WAAM_K_THERMAL_SURFACE, launch_surface);
     } else if (p.thermal_list_ready == 1) {
         /* enough list slices for about kListBlocksWithSweep workers */
         int per = (int)(g3.x * g3.y);
         int lz = std::max(1, (kListBlocksWithSweep + per - 1) / per);
         dim3 ga(g3.x, g3.y, g3.z + (unsigned)lz);
         timed_launch(WAAM_K_THERMAL, [&](hipEvent_t a, hipEvent_t b) {
-            launch_k(capture, k_step_thermal_all, ga, block3(), st, a, b, f, s, lz);
+            if (tab) launch_k(capture, k_step_thermal_all<true>, ga, block3(), st, a, b, f, s, lz);
+            else launch_k(capture, k_step_thermal_all<false>, ga, block3(), st, a, b, f, s, lz);
         });
     } else {
         timed_launch(WAAM_K_THERMAL, [&](hipEvent_t a, hipEvent_t b) {
-            if (p.thermal_list_ready)
-                launch_k(capture, k_step_thermal<false>, g3, block3(), st, a, b, f, s);
-            else
-                launch_k(capture, k_step_thermal<true>, g3, block3(), st, a, b, f, s);
+            if (p.thermal_list_ready) {
+                if (tab) launch_k(capture, k_step_thermal<false, true>, g3, block3(), st, a, b, f, s);
+                else launch_k(capture, k_step_thermal<false, false>, g3, block3(), st, a, b, f, s);
+            } else {
+                if (tab) launch_k(capture, k_step_thermal<true, true>, g3, block3(), st, a, b, f, s);
+                else launch_k(capture, k_step_thermal<true, false>, g3, block3(), st, a, b, f, s);
+            }
         });
-        timed_launch(WAAM_K_THERMAL_SURFACE, [&](hipEvent_t a, hipEvent_t b) {
-            launch_k(capture, k_step_thermal_surface, dim3(kListBlocks), dim3(kBlock), st, a, b, f,
-                     s);
-        });
+        timed_launch(WAAM_K_THERMAL_SURFACE, launch_surface);
     }
     timed_launch(WAAM_K_VOF_FLAGS, [&](hipEvent_t a, hipEvent_t b) {
         if (seg) launch_k(capture, k_step_vof_flags_seg, dim3(kSegBlocks), block3(), st, a, b, f, s);
@@ -1888,6 +1988,7 @@ int waam_step_fused(const waam_step_params *pp, const waam_step_fields *ff, void
     const waam_step_params &p = *pp;
     const waam_step_fields &f = *ff;
     WAAM_RETURN_IF(check_fields("waam_step_fused", f));
+    WAAM_RETURN_IF(check_tables("waam_step_fused", p, f));
     if (p.col_i < 0 || p.col_i >= f.nx || p.col_j < 0 || p.col_j >= f.ny)
         return set_error(WAAM_ERR_INVALID, "waam_step_fused: arc column outside the grid");
     Dims d{f.nx, f.ny, f.nz, f.layout};
@@ -1964,6 +2065,7 @@ int waam_step_fused_noalloc(const waam_step_params *pp, const waam_step_fields *
     f.list_thermal = reinterpret_cast<int32_t *>((char *)base + kNoAllocListOffset);
     f.list_collide = f.list_thermal + kNoAllocMaxCells;
     WAAM_RETURN_IF(check_fields("waam_step_fused_noalloc", f));
+    WAAM_RETURN_IF(check_tables("waam_step_fused_noalloc", p, f));
     if (p.col_i < 0 || p.col_i >= f.nx || p.col_j < 0 || p.col_j >= f.ny)
         return set_error(WAAM_ERR_INVALID, "waam_step_fused_noalloc: arc column outside the grid");
     Dims d{f.nx, f.ny, f.nz, f.layout};
@@ -1993,6 +2095,8 @@ int waam_graph_capture(const waam_step_params *pp, const waam_step_fields *f_eve
         return set_error(WAAM_ERR_INVALID, "waam_graph_capture: kernel timing is armed");
     WAAM_RETURN_IF(check_fields("waam_graph_capture", *f_even));
     WAAM_RETURN_IF(check_fields("waam_graph_capture", *f_odd));
+    WAAM_RETURN_IF(check_tables("waam_graph_capture", p, *f_even));
+    WAAM_RETURN_IF(check_tables("waam_graph_capture", p, *f_odd));
     const waam_step_fields &a = *f_even, &b = *f_odd;
     if (a.nx != b.nx || a.ny != b.ny || a.nz != b.nz || a.layout != b.layout ||
         a.pop_dtype != b.pop_dtype || a.max_tracers != b.max_tracers)
@@ -2003,7 +2107,9 @@ int waam_graph_capture(const waam_step_params *pp, const waam_step_fields *f_eve
         b.flags_in != a.flags_out || b.flags_out != a.flags_in ||
         b.sflags_prev != a.sflags_next || b.sflags_next != a.sflags_prev || b.phi != a.phi ||
         b.phi_tmp != a.phi_tmp || b.list_thermal != a.list_thermal ||
-        b.list_collide != a.list_collide || b.scratch != a.scratch || b.quiet != a.quiet)
+        b.list_collide != a.list_collide || b.scratch != a.scratch || b.quiet != a.quiet ||
+        b.cp_rho_prev != a.cp_rho || b.cp_rho != a.cp_rho_prev || b.alpha_lu != a.alpha_lu ||
+        b.dgamma_lu != a.dgamma_lu || b.tau != a.tau)
         return set_error(WAAM_ERR_INVALID,
                          "waam_graph_capture: the odd field set is not the even set swapped");
     if (p.col_i < 0 || p.col_i >= a.nx || p.col_j < 0 || p.col_j >= a.ny)

@@ -3,36 +3,10 @@
 // 2454-2486, 2524-2570). Each kernel equals its constant-property counterpart with the
 // constant replaced by the cell's value from the property fields.
 #include "forces_cell.h"
+#include "tables_cell.h"
 #include "thermal_cell.h"
 
 namespace waam {
-
-/* lookup_table_1d — reference kernels.py:1685-1707: clamp outside the knots; inside, every
- * bracketing interval overwrites the result (the last one wins, which matters on a knot). */
-__device__ __forceinline__ float lookup_table_1d(float T, const float *t_arr, const float *v_arr,
-                                                 int n, float fallback)
-{
-    float result = fallback;
-    if (n > 0) {
-        if (T <= t_arr[0]) {
-            result = v_arr[0];
-        } else if (T >= t_arr[n - 1]) {
-            result = v_arr[n - 1];
-        } else {
-#pragma unroll
-            for (int idx = 0; idx < WAAM_MAX_KNOTS - 1; ++idx) {
-                if (idx < n - 1) {
-                    float t0 = t_arr[idx], t1 = t_arr[idx + 1];
-                    if (T >= t0 && T <= t1) {
-                        float w = (T - t0) / (t1 - t0 + 1e-8f);
-                        result = v_arr[idx] + w * (v_arr[idx + 1] - v_arr[idx]);
-                    }
-                }
-            }
-        }
-    }
-    return result;
-}
 
 /* refresh_thermal_properties — reference kernels.py:1710-1772 */
 __global__ void k_refresh_properties(const float *__restrict__ T, float *__restrict__ cp_rho_f,
@@ -43,35 +17,11 @@ __global__ void k_refresh_properties(const float *__restrict__ T, float *__restr
 {
     size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= N) return;
-    uint8_t fg = flags[c];
-    if (fg == SOLID || fg == GAS) {
-        cp_rho_f[c] = rf.cp_rho_ref;
-        alpha_f[c] = rf.alpha_lu_ref;
-        dgamma_f[c] = rf.dgamma_lu_ref;
-        tau_f[c] = rf.tau_ref;
-        return;
-    }
-    float T_c = T[c];
-    if (use_tables == 1) {
-        float cp = lookup_table_1d(T_c, tb.cp_T, tb.cp_V, tb.n_cp, tb.cp_fallback);
-        float k_val = lookup_table_1d(T_c, tb.k_T, tb.k_V, tb.n_k, tb.k_fallback);
-        float mu_val = lookup_table_1d(T_c, tb.mu_T, tb.mu_V, tb.n_mu, tb.mu_fallback);
-        float dgamma = lookup_table_1d(T_c, tb.dgamma_T, tb.dgamma_V, tb.n_dgamma,
-                                       tb.dgamma_fallback);
-        float cp_r = rf.rho * cp;
-        float alpha_phys = k_val / (rf.rho * cp + 1e-9f);
-        float nu_phys = mu_val / (rf.rho + 1e-9f);
-        float nu_lu = nu_phys * rf.dt / (rf.dx * rf.dx);
-        cp_rho_f[c] = cp_r;
-        alpha_f[c] = alpha_phys * rf.dt / (rf.dx * rf.dx);
-        dgamma_f[c] = dgamma * rf.force_scale * rf.marangoni_scale;
-        tau_f[c] = 3.0f * nu_lu + 0.5f;
-    } else {
-        cp_rho_f[c] = rf.cp_rho_ref;
-        alpha_f[c] = rf.alpha_lu_ref;
-        dgamma_f[c] = rf.dgamma_lu_ref * rf.marangoni_scale;
-        tau_f[c] = rf.tau_ref;
-    }
+    const CellProps pr = cell_props(tb, rf, use_tables, flags[c], T[c]);
+    cp_rho_f[c] = pr.cp_rho;
+    alpha_f[c] = pr.alpha_lu;
+    dgamma_f[c] = pr.dgamma_lu;
+    tau_f[c] = pr.tau;
 }
 
 /* advect_diffuse_temperature_variable — reference kernels.py:1775-1836 (Jacobi, as the

@@ -37,6 +37,7 @@ from .._lib import lib
 from ..field import DeviceField, P, tensor_of, write_epoch
 from ..physics import weld_forces
 from ..physics.electrical_stickout import update_ctwd
+from ..gpu_tables import MaterialTablesC, PropertyRefsC
 from .coupled_step import _clamp_enthalpy_ceiling, _droplet, droplet_due, resolve_arc_k
 
 _i32, _f32, _vp = ctypes.c_int32, ctypes.c_float, ctypes.c_void_p
@@ -47,7 +48,8 @@ class StepFields(ctypes.Structure):
         "f_src", "f_dst", "rho", "ux", "uy", "uz", "H_in", "H_out", "T_in", "T_out", "fl_in",
         "fl_out", "phi", "phi_tmp", "flags_in", "flags_out", "sflags_prev", "sflags_next", "Fx",
         "Fy", "Fz", "T_max", "T_prev", "dT_dt", "t800", "t1100", "tsol", "tracer_pos",
-        "tracer_active", "list_thermal", "list_collide", "quiet", "scratch")] + [
+        "tracer_active", "list_thermal", "list_collide", "quiet", "cp_rho_prev", "cp_rho",
+        "alpha_lu", "dgamma_lu", "tau", "scratch")] + [
             (n, _i32) for n in ("max_tracers", "nx", "ny", "nz", "layout", "pop_dtype")]
 
 # population storage forms (WAAM_POP_*): fp32, or f - w_q in binary16 / bfloat16
@@ -73,7 +75,9 @@ _FLOAT_PARAMS = ("arc_i", "arc_j", "arc_k_override", "k_robot", "Q_w", "eta", "d
 
 
 class StepParams(ctypes.Structure):
-    _fields_ = [(n, _i32) for n in _INT_PARAMS] + [(n, _f32) for n in _FLOAT_PARAMS]
+    _fields_ = ([(n, _i32) for n in _INT_PARAMS] + [(n, _f32) for n in _FLOAT_PARAMS] +
+                [("use_tables", _i32), ("variable_tau", _i32), ("H_sol_tab", _f32),
+                 ("H_liq_tab", _f32), ("refs", PropertyRefsC), ("tables", MaterialTablesC)])
 
 
 def _swap(g, a: str, b: str) -> None:
@@ -103,6 +107,10 @@ class FusedStepper:
             # compacted FLUID/IFACE cell indices written and consumed inside each fused step
             g.list_thermal = DeviceField.zeros((g.n_cells,), torch.int32, g.device)
             g.list_collide = DeviceField.zeros((g.n_cells,), torch.int32, g.device)
+        if twin.use_material_tables and not hasattr(g, "cp_rho_field_tmp"):
+            # the fused step reads the previous step's cp*rho (its first ceiling) while it
+            # writes its own: the two alternate, g.cp_rho_field holding the latest
+            g.cp_rho_field_tmp = g.field(torch.float32)
         if not hasattr(g, "quiet"):
             # which row segments ended the last two steps quiet (all GAS, phi 0), kept by the
             # fused step itself
@@ -188,7 +196,8 @@ class FusedStepper:
     # ------------------------------------------------------------------ step
     def _fields(self) -> StepFields:
         g = self.twin.grid
-        key = (id(g.H), id(g.T), id(g.f_l), id(g.flags), id(g.sflags), id(g.f_src), id(g.phi))
+        key = (id(g.H), id(g.T), id(g.f_l), id(g.flags), id(g.sflags), id(g.f_src), id(g.phi),
+               id(getattr(g, "cp_rho_field", None)), self.twin.use_material_tables)
         sf = self._fields_cache.get(key)
         if sf is None:
             sf = self._fields_cache[key] = self._build_fields()
@@ -211,6 +220,11 @@ class FusedStepper:
                           ("list_thermal", g.list_thermal), ("list_collide", g.list_collide),
                           ("quiet", g.quiet), ("scratch", g.scratch)):
             setattr(sf, name, P(fld))
+        if self.twin.use_material_tables:
+            for name, fld in (("cp_rho_prev", g.cp_rho_field), ("cp_rho", g.cp_rho_field_tmp),
+                              ("alpha_lu", g.alpha_lu_field), ("dgamma_lu", g.dgamma_lu_field),
+                              ("tau", g.tau_field)):
+                setattr(sf, name, P(fld))
         sf.max_tracers, sf.nx, sf.ny, sf.nz = g.max_tracers, g.nx, g.ny, g.nz
         sf.layout = g.layout
         sf.pop_dtype = self.pop_code
@@ -223,6 +237,18 @@ class FusedStepper:
         name = type(src).__name__
         p.is_welding = 1 if is_welding else 0
         p.collision = 1 if tw.use_srt else 0
+        if tw.use_material_tables:
+            # per-cell properties from the material tables (reference coupled_step.py:168-182);
+            # the constants go to SOLID and GAS cells
+            p.use_tables = 1
+            p.H_sol_tab, p.H_liq_tab = tw.H_sol, tw.H_liq
+            r = p.refs
+            r.rho, r.dt, r.dx, r.force_scale = m.rho, g.dt, g.dx, tw.force_scale / g.dx
+            r.cp_rho_ref, r.alpha_lu_ref = tw.cp_rho, tw.alpha_lu
+            r.dgamma_lu_ref, r.tau_ref, r.marangoni_scale = tw.dgamma_dT_lu, g.tau, tw.marangoni_scale
+            p.tables = tw.gpu_tables.struct
+            if tw.use_variable_tau:
+                p.collision, p.variable_tau = 2, 1
         p.arc_kind = 1 if name == "Goldak3D" else 0
         p.store_forces = 1 if tw.snapshot_forces else 0
         p.enable_vof = 1 if tw.enable_vof else 0
@@ -294,8 +320,6 @@ class FusedStepper:
     def step(self, torch_x_m: float, torch_y_m: float, is_welding: bool,
              torch_z_m: float | None = None) -> None:
         tw, g = self.twin, self.twin.grid
-        if tw.use_material_tables:
-            raise NotImplementedError("temperature-dependent material tables are not built")
         if tw.enable_lorentz:
             raise NotImplementedError("the Lorentz force solve is not built")
         arc_i = torch_x_m / g.dx
@@ -339,6 +363,8 @@ class FusedStepper:
         for a, b in (("H", "H_tmp"), ("T", "T_tmp"), ("f_l", "fl_tmp"), ("flags", "flags_tmp"),
                      ("sflags", "sflags_tmp")):
             _swap(g, a, b)
+        if tw.use_material_tables:
+            _swap(g, "cp_rho_field", "cp_rho_field_tmp")
         g.swap_buffers()
         self.pending = True
         self._list_epoch = write_epoch()
@@ -356,7 +382,7 @@ class FusedStepper:
         fo = StepFields.from_buffer_copy(fe)
         for a, b in (("f_src", "f_dst"), ("H_in", "H_out"), ("T_in", "T_out"),
                      ("fl_in", "fl_out"), ("flags_in", "flags_out"),
-                     ("sflags_prev", "sflags_next")):
+                     ("sflags_prev", "sflags_next"), ("cp_rho_prev", "cp_rho")):
             setattr(fo, a, getattr(fe, b))
             setattr(fo, b, getattr(fe, a))
         return fo
@@ -410,6 +436,8 @@ class FusedStepper:
             for a, b in (("H", "H_tmp"), ("T", "T_tmp"), ("f_l", "fl_tmp"),
                          ("flags", "flags_tmp"), ("sflags", "sflags_tmp")):
                 _swap(g, a, b)
+            if tw.use_material_tables:
+                _swap(g, "cp_rho_field", "cp_rho_field_tmp")
             g.swap_buffers()
         self.pending = True
         self._forces_from_fields = False

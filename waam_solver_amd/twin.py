@@ -365,10 +365,9 @@ class WAAMTwin:
             self._maybe_shift_window(torch_x_m)
         sim_x = torch_x_m - self._window_offset_x_m - self._sim_origin_offset_x_m
         sim_y = torch_y_m - self._sim_origin_offset_y_m
-        if self.engine == "fused" and (self.use_material_tables or self.enable_lorentz):
-            # the fused kernels take scalar properties and the surface forces; per-cell tables
-            # and the Lorentz solve (a Jacobi iteration over the whole grid that dominates such
-            # a step) run through the reference-order kernels, which give the same fields
+        if self.engine == "fused" and self.enable_lorentz:
+            # the Lorentz solve (a Jacobi iteration over the whole grid that dominates such a
+            # step) runs through the reference-order kernels, which give the same fields
             self.sync()
             coupled_step(self, sim_x, sim_y, is_welding, torch_z_m)
         elif self.engine == "fused":
