@@ -42,7 +42,11 @@ LARGE = dict(WORKLOAD, nx=768, ny=384, nz=192, x0_mm=30.0, steps=400)
 # 4999 steps with the arc off at the end of the pass; 16-bit populations, captured steps.
 MULTIPASS = dict(WORKLOAD, nx=768, ny=384, nz=192, dx=1e-3, plate_thickness_mm=48.0,
                  x0_mm=200.0, passes=3, pass_steps=3334, interpass_steps=4999)
-WORKLOADS = {"bead_on_plate": WORKLOAD, "large": LARGE, "multipass": MULTIPASS}
+# The reference README's job (jobs/examples/bead_on_plate.yaml: ER70S-6 with temperature-
+# dependent cp, k, mu and Sahoo dgamma/dT, Goldak source, Lin-Eagar pressure, droplets, flow
+# tier) on its own grid, 200 x 100 x 62 cells at 0.4 mm, stepped along its torch path.
+JOB = dict(job="jobs/examples/bead_on_plate.yaml", steps=400)
+WORKLOADS = {"bead_on_plate": WORKLOAD, "large": LARGE, "multipass": MULTIPASS, "job": JOB}
 
 
 def twin_kwargs(w: dict) -> dict:
@@ -405,12 +409,63 @@ def measured_traffic(w: dict, pop: str) -> dict | None:
     return out or None
 
 
+def job_run(args, engine: str) -> tuple[float, int, int]:
+    """The README job through `engine` along its torch path (WAAMTwin.run_path for the warm-up
+    steps, then the path's next positions through step_many, timed): (seconds, cells, droplets
+    fired)."""
+    import torch
+
+    from waam_solver_amd.job import PKG_ROOT, load_job_config, parse_torch_path
+    from waam_solver_amd.torch_path import TorchPathDriver, clamp_torch_to_domain
+    from waam_solver_amd.twin import WAAMTwin
+
+    path = PKG_ROOT / JOB["job"]
+    tw = WAAMTwin.from_job(path)
+    tw.engine = engine
+    tw.reset()
+    g = tw.grid
+    job = load_job_config(path)
+    tw.run_path(job, n_steps=args.warmup)
+    driver = TorchPathDriver(parse_torch_path(job), tw.travel_speed_m_s)
+    pos = list(driver.positions_for_steps(args.warmup + args.steps, g.dt))[args.warmup:]
+    xy = [clamp_torch_to_domain(x, y, g.nx, g.ny, g.dx) for _, x, y, _ in pos]
+    zs = [z for _, _, _, z in pos]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tw.step_many([c[0] for c in xy], [c[1] for c in xy], True, zs)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, g.n_cells, int(tw._n_droplets_fired)
+
+
+def job_bench(args) -> int:
+    """MLUPS of the README job with the fused step and with the reference-order kernels."""
+    el, ncell, drops = job_run(args, "fused")
+    el_ref, _, drops_ref = job_run(args, "reference")
+    mlups = ncell * args.steps / el / 1e6
+    ref = ncell * args.steps / el_ref / 1e6
+    print(json.dumps({
+        "metric": "coupled-step MLUPS (README job: bead_on_plate.yaml, table material)",
+        "value": round(mlups, 2), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(el * 1e3 / args.steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "README job jobs/examples/bead_on_plate.yaml: ER70S-6 with "
+                               "temperature-dependent tables (Sahoo dgamma/dT), Goldak, "
+                               "Lin-Eagar pressure, droplets, flow tier, along its torch path",
+                   "cells": ncell, "engine": "fused", "droplets": drops},
+        "reference_order_engine": {"value": round(ref, 2), "ms_per_step":
+                                   round(el_ref * 1e3 / args.steps, 4), "droplets": drops_ref},
+        "speedup_vs_reference_order": round(mlups / ref, 3)}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", choices=tuple(WORKLOADS), default="large",
                     help="large: 768x384x192 (default); bead_on_plate: 256x128x64; multipass: "
-                         "three passes, 20 000 steps on 768x384x192 at dx=1mm")
+                         "three passes, 20 000 steps on 768x384x192 at dx=1mm; job: the README "
+                         "job (table material) with the fused and the reference-order engine")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -440,6 +495,13 @@ def main() -> int:
 
     if args.workload == "multipass":
         return multipass(args)
+    if args.workload == "job":
+        if args.steps is None:
+            args.steps = JOB["steps"]
+        # one direct step and two 32-step captured runs: the graphs are captured before the
+        # timed steps, which then replay them
+        args.warmup = max(args.warmup, 65)
+        return job_bench(args)
     w = dict(WORKLOADS[args.workload])
     if args.steps is None:
         args.steps = w["steps"]

@@ -10,7 +10,7 @@ from __future__ import annotations
 
 import pytest
 
-from tests.step_cases import build_pair, compare, torch_x
+from tests.step_cases import FIELDS, build_pair, compare, torch_x
 
 pytestmark = pytest.mark.gpu
 
@@ -26,6 +26,8 @@ def _positions(g, start: int, n: int):
     ("droplets_srt", "fp32", 60, 6),
     ("bead_mrt", "fp16", 40, 8),
     ("bead_odd_mrt", "bf16", 24, 7),
+    ("tables_mrt", "fp32", 40, 8),
+    ("tables_mrt", "fp16", 24, 8),
 ])
 def test_captured_steps_match_oracle(name, pop, steps, per_graph):
     print(f"[graph] {name} {pop} steps={steps} per_graph={per_graph}", flush=True)
@@ -41,7 +43,12 @@ def test_captured_steps_match_oracle(name, pop, steps, per_graph):
     assert tw._step_n == ot._step_n == steps
     captured = tw._fused.captured_steps
     assert captured >= per_graph, "no captured run"
-    res = compare(tw, ot)
+    if name == "tables_mrt":
+        # the property fields of the last refresh too
+        res = compare(tw, ot, FIELDS + ("cp_rho_field", "alpha_lu_field", "dgamma_lu_field",
+                                        "tau_field"))
+    else:
+        res = compare(tw, ot)
     bad = {k: v for k, v in res.items() if not v[0]}
     assert not bad, f"{name} {pop}: captured steps differ from the oracle: {bad}"
     assert len(tw._fused._graphs) >= 1

@@ -411,7 +411,7 @@ class WAAMTwin:
         m = st.graph_steps
         if m <= 0 or s + m > len(xs) or not st.steady():
             return 0
-        if (self.use_material_tables or self.enable_lorentz or self.enable_ctwd
+        if (self.enable_lorentz or self.enable_ctwd
                 or self.probe_recorder is not None
                 or type(self.arc_source).__name__ == "ConicalVolume"):
             return 0
