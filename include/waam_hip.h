@@ -601,6 +601,40 @@ typedef struct waam_step_params {
 } waam_step_params;
 
 int waam_step_fused(const waam_step_params *p, const waam_step_fields *f, void *stream);
+/* A step on which a droplet detaches (reference solvers/coupled_step.py:97-166 followed by the
+ * rest of the step), with everything the host would read back kept on the device: the previous
+ * step's deferred stream is materialised; the torch column's surface height (with the robot
+ * floor, as waam_step_fused resolves it) gives arc_k; arc heat and the ceiling are applied to H;
+ * the forces are cleared; tracers are injected at the arc; the droplet is deposited with the
+ * reference's retry loop (coupled_step.py:138-154: up to 10 sweeps, footprint + 1.5 cells up to
+ * 16, until 98 % of the volume is placed) and its placed volume added to `ledger`; the momentum
+ * impact and the impact pressure follow; then the fused step runs with that arc_k and those
+ * forces. Populations: the deposition and the momentum impact work on fp32, so with 16-bit
+ * storage f32_src/f32_dst (the fp32 pair of f_src/f_dst) are unpacked into and packed back
+ * from; with fp32 storage they equal f_src/f_dst. No counterpart entry in the reference: its
+ * droplet block reads the placed volume back on the host after every sweep. */
+typedef struct waam_droplet {
+    float *f32_src, *f32_dst;
+    double drop_vol;        /* m^3 of metal to place (drop_mass / rho) */
+    double footprint_r;     /* first footprint radius, cells */
+    float arc_i, arc_j;     /* cells, as the step's params */
+    float drop_r, T_drop, cp_rho, L_rho, rho0, cell_vol;
+    float *vol_acc, *real_acc, *result; /* one float each, and two floats of sweep result */
+    double *ledger;         /* [0] += placed volume (m^3); [1] += 1 when below 98 % of drop_vol */
+    /* tracers (kernels.py:3186-3203): spawn at (tracer_x, tracer_y, (arc_k + 1) * dx) */
+    float *tracer_pos;
+    int32_t *tracer_active, *tracer_head;
+    int32_t max_tracers, spawn;
+    uint32_t seed;
+    float tracer_x, tracer_y, sigma_m;
+    double dx;
+    /* impact (physics/weld_forces.py apply_droplet_impact): momentum (lattice velocity) and
+     * pressure (Pa over the droplet's radius) */
+    int32_t momentum, pressure;
+    float vx, vy, vz, impact_pa, dt, dx_f, rho_ref;
+} waam_droplet;
+int waam_step_fused_droplet(const waam_step_params *p, const waam_step_fields *f,
+                            const waam_droplet *drop, void *stream);
 /* bytes of the quiet record (waam_step_fields.quiet) for a grid; 0 for invalid sizes */
 int64_t waam_quiet_bytes(int nx, int ny, int nz, int layout);
 /* Allocation-free variant of waam_step_fused (no counterpart in the reference): for callers that
@@ -633,6 +667,8 @@ int waam_unpack_populations(const void *src, float *dst, int64_t n_cells, int po
 /* The step's arc weight total, energy per unit weight and arc column height, as written into
  * the workspace by the last waam_step_fused (three device floats, in that order). Blocking. */
 int waam_step_arc_readback(const void *scratch, float *host_out3);
+/* The arc_k the last waam_step_fused_droplet used (resolved before its deposition). Blocking. */
+int waam_step_droplet_arc_k(const void *scratch, float *host_out);
 
 /* ---- captured steps: a run of n_steps consecutive waam_step_fused calls as one hipGraph.
  * Replaces n_steps calls of coupled_step (reference solvers/coupled_step.py:73) made by
@@ -655,6 +691,14 @@ int waam_step_arc_readback(const void *scratch, float *host_out3);
 typedef struct waam_step_graph waam_step_graph;
 int waam_graph_capture(const waam_step_params *p, const waam_step_fields *f_even,
                        const waam_step_fields *f_odd, int n_steps, waam_step_graph **out);
+/* waam_graph_capture with a per-step plan (no counterpart in the reference): kinds[k] 0 is a
+ * steady step, 1 a droplet step (waam_step_fused_droplet with drops[k]; its arc still comes
+ * from the track), 2 the step after a droplet step (it clears every cell's forces). The
+ * droplet parameters are recorded into the graph: a run with a droplet is captured for that
+ * droplet. kinds NULL is waam_graph_capture. */
+int waam_graph_capture_plan(const waam_step_params *p, const waam_step_fields *f_even,
+                            const waam_step_fields *f_odd, int n_steps, const int32_t *kinds,
+                            const waam_droplet *drops, waam_step_graph **out);
 int waam_graph_launch(waam_step_graph *g, const double *arc_i, const double *arc_j,
                       const float *k_robot, const float *dir, void *stream);
 int waam_graph_steps(const waam_step_graph *g);

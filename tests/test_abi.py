@@ -137,6 +137,7 @@ int main(void) {
 printf("size waam_step_fields %zu\n", sizeof(waam_step_fields));
 printf("size waam_step_params %zu\n", sizeof(waam_step_params));
 printf("size waam_adj_consts %zu\n", sizeof(waam_adj_consts));
+printf("size waam_droplet %zu\n", sizeof(waam_droplet));
 return 0; }
 """
 
@@ -144,10 +145,10 @@ return 0; }
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc not available")
 def test_struct_layouts_match_header(tmp_path):
     from waam_solver_amd.adjoint import AdjConsts
-    from waam_solver_amd.solvers.fused import StepFields, StepParams
+    from waam_solver_amd.solvers.fused import DropletC, StepFields, StepParams
 
     structs = (("waam_step_fields", StepFields), ("waam_step_params", StepParams),
-               ("waam_adj_consts", AdjConsts))
+               ("waam_adj_consts", AdjConsts), ("waam_droplet", DropletC))
     lines = []
     for cname, py in structs:
         for field, _ in py._fields_:

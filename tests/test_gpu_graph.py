@@ -1,9 +1,10 @@
 """Captured steps (waam_graph_capture / waam_graph_launch) against direct steps and the oracle.
 
-A run of steps through ``WAAMTwin.step_many`` sends runs of steady steps to the GPU as one
-captured graph each and steps directly around droplets; every field and both population buffers
-must equal, bit for bit, those of the oracle twin stepped one step at a time on the same torch
-path (the direct fused step is held to the same bar by the step parity tests).
+A run of steps through ``WAAMTwin.step_many`` sends runs of steps to the GPU as one captured
+graph each, droplet steps included (their deposition runs on the device with the parameters
+recorded into the graph); every field and both population buffers must equal, bit for bit,
+those of the oracle twin stepped one step at a time on the same torch path (the direct fused
+step is held to the same bar by the step parity tests).
 """
 
 from __future__ import annotations
@@ -23,7 +24,8 @@ def _positions(g, start: int, n: int):
 @pytest.mark.parametrize("name,pop,steps,per_graph", [
     ("bead_mrt", "fp32", 40, 8),
     ("surface_forces_mrt", "fp32", 33, 16),
-    ("droplets_srt", "fp32", 60, 6),
+    ("droplets_srt", "fp32", 61, 6),
+    ("droplets_srt", "fp16", 61, 6),
     ("bead_mrt", "fp16", 40, 8),
     ("bead_odd_mrt", "bf16", 24, 7),
     ("tables_mrt", "fp32", 40, 8),
@@ -51,6 +53,13 @@ def test_captured_steps_match_oracle(name, pop, steps, per_graph):
         res = compare(tw, ot)
     bad = {k: v for k, v in res.items() if not v[0]}
     assert not bad, f"{name} {pop}: captured steps differ from the oracle: {bad}"
+    if name == "droplets_srt":
+        # the droplet steps ran inside the graphs too (waam_step_fused_droplet captured with
+        # its deposition parameters): every step after the first was captured
+        assert captured == steps - 1, captured
+        assert tw._n_droplets_fired == ot._n_droplets_fired > 0
+        assert tw._deposited_volume_m3 == ot._deposited_volume_m3 > 0.0
+        assert tw._deposition_overflow == ot._deposition_overflow
     assert len(tw._fused._graphs) >= 1
     tw._fused.close()
 

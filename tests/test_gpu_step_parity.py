@@ -125,17 +125,22 @@ def test_job_drop_in_parity(hip_device, tmp_path, engine):
     ot = oracle_like(tw)
     ot.reset()
     calls = []
-    step = tw.step
+    step_many = tw.step_many
 
-    def recording_step(x, y, is_welding=True, torch_z_m=None):
-        calls.append((x, y, is_welding, torch_z_m))
-        step(x, y, is_welding, torch_z_m)
+    # run_path hands its positions to step_many (runs of captured steps, droplet steps
+    # included, and direct steps); record them there
+    def recording_step_many(xs, ys, is_welding=True, zs=None):
+        for k, (x, y) in enumerate(zip(xs, ys)):
+            calls.append((x, y, is_welding, None if zs is None else zs[k]))
+        step_many(xs, ys, is_welding, zs)
 
-    tw.step = recording_step
+    tw.step_many = recording_step_many
     tw.run_path(job, n_steps=60)
     for x, y, w, z in calls:
         ot.step(x, y, w, z)
     assert len(calls) == 60
+    if engine == "fused":
+        assert tw._fused.captured_steps > 0
     _assert_parity(compare(tw, ot, FIELDS + TABLE_FIELDS), f"job/{engine} step 60")
     assert tw._n_droplets_fired == ot._n_droplets_fired and tw._n_droplets_fired > 0
 

@@ -74,11 +74,13 @@ SIGNATURES: dict[str, str] = {
     "waam_advect_tracers": "pppppppffiiiiip",
     "waam_telemetry_pool_reduce": "ppppppppiiiipp",
     "waam_step_fused": "ppp",
+    "waam_step_fused_droplet": "pppp",
     "waam_step_fused_noalloc": "ppp",
     "waam_stream_pending": "pppiiiiip",
     "waam_pack_populations": "pplip",
     "waam_unpack_populations": "pplip",
     "waam_step_arc_readback": "pp",
+    "waam_step_droplet_arc_k": "pp",
     "waam_adj_marangoni": "pppppppfpppiiiip",
     "waam_adj_collide_srt": "pppppppffpplp",
     "waam_adj_stream": "pppiiiip",
@@ -88,6 +90,7 @@ SIGNATURES: dict[str, str] = {
     "waam_cadj_phase": "pppppplp",
     "waam_cadj_thermal_step": "ppppppppppppppppppiiiipp",
     "waam_graph_capture": "pppip",
+    "waam_graph_capture_plan": "pppippp",
     "waam_graph_launch": "pppppp",
     "waam_graph_steps": "p",
     "waam_graph_destroy": "p",

@@ -24,6 +24,7 @@ struct DepositArgs {
     double r_try0;        /* first footprint radius (host double) */
     int max_tries;        /* 1: a single reference kernel call; 10: the host retry loop */
     Dims d;
+    const float *arc_k_dev; /* non-NULL: arc_k is read here (resolved on the device) */
 };
 
 __device__ __forceinline__ bool is_metal(const uint8_t *flags, const float *phi, const float *f_l,
@@ -47,6 +48,7 @@ __global__ void __launch_bounds__(kDepThreads)
     const Dims d = a.d;
     const size_t N = d.n();
     const int tid = threadIdx.x;
+    const float arc_k = a.arc_k_dev ? *a.arc_k_dev : a.arc_k;
     int tries = 0;
     if (tid == 0) {
         s_rtry = a.r_try0;
@@ -71,10 +73,10 @@ __global__ void __launch_bounds__(kDepThreads)
         const int i_max = imin(d.nx, (int)(a.arc_i + search_r) + 1);
         const int j_min = imax(0, (int)(a.arc_j - search_r));
         const int j_max = imin(d.ny, (int)(a.arc_j + search_r) + 1);
-        const int k_lo = imax(0, (int)(a.arc_k - 1.0f));
+        const int k_lo = imax(0, (int)(arc_k - 1.0f));
         const float k_span = fmx(dr + 3.0f, 4.0f);
-        const int k_hi = imin(d.nz, (int)(a.arc_k + k_span) + 1);
-        const float drop_cz = a.arc_k + dr + 0.5f;
+        const int k_hi = imin(d.nz, (int)(arc_k + k_span) + 1);
+        const float drop_cz = arc_k + dr + 0.5f;
         const float r_drop2 = dr * dr;
         const float foot2 = search_r * search_r;
         const int bi = i_max - i_min, bj = j_max - j_min, bk = k_hi - k_lo;
@@ -216,9 +218,13 @@ __device__ __forceinline__ uint32_t hash_u32(uint32_t x)
 
 __global__ void k_inject_tracers(float *__restrict__ pos, int32_t *__restrict__ active,
                                  int32_t *__restrict__ head, int max_tracers, float ax, float ay,
-                                 float az, float sigma_m, int spawn, uint32_t seed)
+                                 float az, float sigma_m, int spawn, uint32_t seed,
+                                 const float *__restrict__ arc_k_dev, double dx)
 {
     if (threadIdx.x != 0) return;
+    /* arc_k resolved on the device: the height (arc_k + 1) * dx as the host forms it, in
+     * binary64, rounded once */
+    if (arc_k_dev) az = (float)(((double)*arc_k_dev + 1.0) * dx);
     for (int s = 0; s < spawn; ++s) {
         int32_t h = head[0];
         head[0] = h + 1;
@@ -367,6 +373,45 @@ static int deposit_launch(const char *name, float *f_src, uint8_t *flags, float 
     return check_launch(name);
 }
 
+namespace waam {
+
+int launch_deposit_dev(float *f_src, uint8_t *flags, float *f_l, float *phi, float *H, float *T,
+                       float *rho, float *vol_acc, float *real_acc, void *scratch, float *result,
+                       float arc_i, float arc_j, const float *arc_k_dev, double footprint_r,
+                       float droplet_radius, double drop_vol, float T_drop, float cp_rho,
+                       float L_rho, float rho0, float cell_vol, Dims d, hipStream_t st)
+{
+    DepositArgs a{};
+    a.arc_i = arc_i;
+    a.arc_j = arc_j;
+    a.arc_k = 0.0f;
+    a.arc_k_dev = arc_k_dev;
+    a.droplet_radius = droplet_radius;
+    a.T_drop = T_drop;
+    a.cp_rho = cp_rho;
+    a.L_rho = L_rho;
+    a.rho0 = rho0;
+    a.cell_vol = cell_vol;
+    a.target_vol_f = (float)drop_vol;
+    a.target_vol = drop_vol;
+    a.r_try0 = footprint_r;
+    a.max_tries = 10;
+    a.d = d;
+    return deposit_launch("waam_step_fused_droplet", f_src, flags, f_l, phi, H, T, rho, vol_acc,
+                          real_acc, scratch, result, a, st);
+}
+
+int launch_tracers_dev(float *pos, int32_t *active, int32_t *head, int max_tracers, float ax,
+                       float ay, const float *arc_k_dev, double dx, float sigma_m, int spawn,
+                       uint32_t seed, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_inject_tracers, dim3(1), dim3(64), 0, st, pos, active, head,
+                       max_tracers, ax, ay, 0.0f, sigma_m, spawn, seed, arc_k_dev, dx);
+    return check_launch("waam_step_fused_droplet (tracers)");
+}
+
+}  // namespace waam
+
 extern "C" {
 
 int waam_feed_wire_surface(float *f_src, uint8_t *flags, float *f_l, float *phi, float *H,
@@ -428,7 +473,8 @@ int waam_inject_tracers(float *pos, int32_t *active, int32_t *head, int max_trac
     if (max_tracers <= 0) return set_error(WAAM_ERR_INVALID, "waam_inject_tracers: max_tracers <= 0");
     WAAM_RETURN_IF(check_ptrs("waam_inject_tracers", {pos, active, head}));
     hipLaunchKernelGGL(k_inject_tracers, dim3(1), dim3(64), 0, as_stream(stream), pos, active,
-                       head, max_tracers, arc_x, arc_y, arc_z, sigma_m, spawn_count, seed);
+                       head, max_tracers, arc_x, arc_y, arc_z, sigma_m, spawn_count, seed,
+                       (const float *)nullptr, 0.0);
     return check_launch("waam_inject_tracers");
 }
 

@@ -150,9 +150,10 @@ __global__ void k_clamp_force(float *__restrict__ Fx, float *__restrict__ Fy,
 __global__ void k_droplet_pressure(float *__restrict__ Fz, const uint8_t *__restrict__ flags,
                                    const float *__restrict__ phi, const float *__restrict__ f_l,
                                    float arc_i, float arc_j, float arc_k, float dr, float inv2s2,
-                                   float F_peak_lu, Dims d)
+                                   float F_peak_lu, Dims d, const float *__restrict__ arc_k_dev)
 {
     CELL3D
+    if (arc_k_dev) arc_k = *arc_k_dev;  // resolved on the device
     if (flags[c] == GAS) return;
     if (f_l[c] < 0.25f && phi[c] < 0.25f) return;
     float di = (float)i - arc_i, dj = (float)j - arc_j, dk = (float)k - arc_k;
@@ -338,8 +339,26 @@ int waam_apply_droplet_impact_pressure(float *Fz, const uint8_t *flags, const fl
     float F_peak_phys = impact_pa / (rho_ref * dx);
     float F_peak_lu = F_peak_phys * dt * dt / dx;
     hipLaunchKernelGGL(k_droplet_pressure, grid3(d), block3(), 0, as_stream(stream), Fz, flags,
-                       phi, f_l, arc_i, arc_j, arc_k, drop_radius, inv2s2, F_peak_lu, d);
+                       phi, f_l, arc_i, arc_j, arc_k, drop_radius, inv2s2, F_peak_lu, d,
+                       (const float *)nullptr);
     return check_launch("waam_apply_droplet_impact_pressure");
 }
 
 }  // extern "C"
+
+namespace waam {
+
+int launch_droplet_pressure_dev(float *Fz, const uint8_t *flags, const float *phi,
+                                const float *f_l, float arc_i, float arc_j,
+                                const float *arc_k_dev, float drop_radius, float impact_pa,
+                                float dt, float dx, float rho_ref, Dims d, hipStream_t st)
+{
+    float inv2s2 = 1.0f / (2.0f * drop_radius * drop_radius + 1e-6f);
+    float F_peak_phys = impact_pa / (rho_ref * dx);
+    float F_peak_lu = F_peak_phys * dt * dt / dx;
+    hipLaunchKernelGGL(k_droplet_pressure, grid3(d), block3(), 0, st, Fz, flags, phi, f_l, arc_i,
+                       arc_j, 0.0f, drop_radius, inv2s2, F_peak_lu, d, arc_k_dev);
+    return check_launch("waam_step_fused_droplet (impact pressure)");
+}
+
+}  // namespace waam

@@ -49,6 +49,7 @@
 #include <hip/hip_ext.h>
 
 #include "arc_cell.h"
+#include "droplet_launch.h"
 #include "forces_cell.h"
 #include "lbm_cell.h"
 #include "pop_store.h"
@@ -142,9 +143,11 @@ __device__ __forceinline__ ArcStep arc_step(const StepConsts &s)
     return s.at;
 }
 
+/* the step's arc_k: the column scan of kernel 1 (tot[2]), the host's value (override 1), or the
+ * droplet step's scan kept from before the deposition changed the column (override 2, tot[4]) */
 __device__ __forceinline__ float step_arc_k(const waam_step_params &p, const float *tot)
 {
-    return p.use_arc_k_override ? p.arc_k_override : tot[2];
+    return p.use_arc_k_override == 2 ? tot[4] : (p.use_arc_k_override ? p.arc_k_override : tot[2]);
 }
 
 /* which = 0: thermal list (FLUID/IFACE by the incoming flags), 1: collision list */
@@ -1980,6 +1983,12 @@ static void graph_free(waam_step_graph *g)
     delete g;
 }
 
+namespace waam {
+static void launch_droplet_step(const StepConsts &s0, const waam_step_fields &f,
+                                const waam_droplet &dp, hipStream_t st, bool capture);
+static int check_droplet(const char *who, const waam_step_fields &f, const waam_droplet &dp);
+}  // namespace waam
+
 extern "C" {
 
 int waam_step_fused(const waam_step_params *pp, const waam_step_fields *ff, void *stream)
@@ -2077,8 +2086,17 @@ int waam_step_fused_noalloc(const waam_step_params *pp, const waam_step_fields *
 int waam_graph_capture(const waam_step_params *pp, const waam_step_fields *f_even,
                        const waam_step_fields *f_odd, int n_steps, waam_step_graph **out)
 {
+    return waam_graph_capture_plan(pp, f_even, f_odd, n_steps, nullptr, nullptr, out);
+}
+
+int waam_graph_capture_plan(const waam_step_params *pp, const waam_step_fields *f_even,
+                            const waam_step_fields *f_odd, int n_steps, const int32_t *kinds,
+                            const waam_droplet *drops, waam_step_graph **out)
+{
     if (!pp || !f_even || !f_odd || !out)
         return set_error(WAAM_ERR_INVALID, "waam_graph_capture: NULL argument");
+    if (kinds && !drops)
+        return set_error(WAAM_ERR_INVALID, "waam_graph_capture: a plan without droplets");
     *out = nullptr;
     const waam_step_params &p = *pp;
     if (n_steps < 1 || n_steps > kGraphMaxSteps)
@@ -2114,6 +2132,18 @@ int waam_graph_capture(const waam_step_params *pp, const waam_step_fields *f_eve
                          "waam_graph_capture: the odd field set is not the even set swapped");
     if (p.col_i < 0 || p.col_i >= a.nx || p.col_j < 0 || p.col_j >= a.ny)
         return set_error(WAAM_ERR_INVALID, "waam_graph_capture: arc column outside the grid");
+    for (int k = 0; kinds && k < n_steps; ++k) {
+        if (kinds[k] < 0 || kinds[k] > 2)
+            return set_error(WAAM_ERR_INVALID, "waam_graph_capture: step %d of kind %d", k,
+                             kinds[k]);
+        if (kinds[k] == 1) {
+            if (!p.is_welding || p.arc_kind > 1)
+                return set_error(WAAM_ERR_INVALID,
+                                 "waam_graph_capture: droplet steps need a welding step with the "
+                                 "Gaussian or Goldak source");
+            WAAM_RETURN_IF(check_droplet("waam_graph_capture", (k & 1) ? b : a, drops[k]));
+        }
+    }
 
     waam_step_graph *g = new waam_step_graph();
     Dims d{a.nx, a.ny, a.nz, a.layout};
@@ -2142,7 +2172,17 @@ int waam_graph_capture(const waam_step_params *pp, const waam_step_fields *f_eve
         for (int k = 0; k < n_steps; ++k) {
             StepConsts s = g->base;
             s.track = g->track + k;
-            launch_step(s, (k & 1) ? b : a, cs, true);
+            const int kind = kinds ? kinds[k] : 0;
+            if (kind == 1) {
+                launch_droplet_step(s, (k & 1) ? b : a, drops[k], cs, true);
+            } else {
+                if (kind == 2) {
+                    /* the droplet step took its forces from the fields: not a steady step */
+                    s.p.force_reset_all = 1;
+                    s.p.quiet_skip = 0;
+                }
+                launch_step(s, (k & 1) ? b : a, cs, true);
+            }
         }
         hipError_t le = hipGetLastError();
         e = hipStreamEndCapture(cs, &g->graph);
@@ -2252,7 +2292,196 @@ __global__ void k_unpack_pops(const uint16_t *__restrict__ src, float *__restric
         dst[e] = PopStore<PD>::load(src, e, (int)(e / n_cells));
 }
 
+/* ---- the droplet step (waam_step_fused_droplet) ---------------------------------------- */
+
+/* arc heat and the ceiling applied to H in place on every non-GAS cell: the eager form of the
+ * arc kernel 2 applies on the fly (reference coupled_step.py:103-108, before the droplet
+ * block); the step's arc_k, the column scan of kernel 1 (tot[2]), is kept in tot[4] for the
+ * rest of the step, whose column the deposition may raise */
+__global__ void __launch_bounds__(256) k_droplet_arc(waam_step_fields f, StepConsts s)
+{
+    const Dims &d = s.d;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0 &&
+        threadIdx.y == 0) {
+        float *tot = reinterpret_cast<float *>((char *)f.scratch + kScratchTelemetry);
+        tot[4] = tot[2];
+    }
+    int i, j, k;
+    if (!d.cell(i, j, k)) return;
+    const HView hv = make_hview(f, s);
+    const size_t c = d.idx(i, j, k);
+    if (f.flags_in[c] == GAS) return;
+    f.H_in[c] = hv.at(c, i, j, k);
+}
+
+/* the droplet's placed volume into the run's ledger, in binary64 and droplet order, as the
+ * reference's host sums it (coupled_step.py:155-167) */
+__global__ void k_droplet_ledger(const float *__restrict__ result, double drop_vol,
+                                 double *__restrict__ ledger)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double placed = (double)result[0];
+    ledger[0] += placed;
+    if (placed < drop_vol * 0.98) ledger[1] += 1.0;
+}
+
+__global__ void k_zero3(float *__restrict__ a, float *__restrict__ b, float *__restrict__ c,
+                        size_t n)
+{
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        a[e] = 0.0f;
+        b[e] = 0.0f;
+        c[e] = 0.0f;
+    }
+}
+
+static int check_droplet(const char *who, const waam_step_fields &f, const waam_droplet &dp)
+{
+    WAAM_RETURN_IF(check_ptrs(who, {dp.f32_src, dp.f32_dst, dp.vol_acc, dp.real_acc, dp.result,
+                                    dp.ledger}));
+    if (f.pop_dtype == WAAM_POP_F32 && (dp.f32_src != f.f_src || dp.f32_dst != f.f_dst))
+        return set_error(WAAM_ERR_INVALID, "%s: fp32 storage: f32_src/f32_dst must be f_src/f_dst",
+                         who);
+    if (f.pop_dtype != WAAM_POP_F32 &&
+        (dp.f32_src == f.f_src || dp.f32_dst == f.f_dst || dp.f32_src == dp.f32_dst))
+        return set_error(WAAM_ERR_INVALID, "%s: the fp32 pair aliases the 16-bit pair", who);
+    if (dp.max_tracers > 0 && (!dp.tracer_pos || !dp.tracer_active || !dp.tracer_head))
+        return set_error(WAAM_ERR_INVALID, "%s: tracer buffers missing", who);
+    if (!(dp.drop_r >= 0.0f) || !(dp.drop_vol >= 0.0) || !(dp.footprint_r >= 0.0))
+        return set_error(WAAM_ERR_INVALID, "%s: negative droplet size", who);
+    return 0;
+}
+
+static void pops_convert(bool pack, void *p16, float *p32, size_t n_cells, int pd, hipStream_t st)
+{
+    const size_t n = n_cells * Q;
+    unsigned g = grid_for(n);
+    if (g > 65536u) g = 65536u;
+    if (pack) {
+        uint16_t *o = static_cast<uint16_t *>(p16);
+        if (pd == WAAM_POP_F16)
+            hipLaunchKernelGGL(k_pack_pops<WAAM_POP_F16>, dim3(g), dim3(kBlock), 0, st, p32, o, n,
+                               n_cells);
+        else
+            hipLaunchKernelGGL(k_pack_pops<WAAM_POP_BF16>, dim3(g), dim3(kBlock), 0, st, p32, o, n,
+                               n_cells);
+    } else {
+        const uint16_t *i = static_cast<const uint16_t *>(p16);
+        if (pd == WAAM_POP_F16)
+            hipLaunchKernelGGL(k_unpack_pops<WAAM_POP_F16>, dim3(g), dim3(kBlock), 0, st, i, p32,
+                               n, n_cells);
+        else
+            hipLaunchKernelGGL(k_unpack_pops<WAAM_POP_BF16>, dim3(g), dim3(kBlock), 0, st, i, p32,
+                               n, n_cells);
+    }
+}
+
+/* One droplet step: the droplet block of the reference with arc_k, the placed volume and the
+ * retry loop on the device, then the fused step with the arc already applied. The launch order
+ * is the reference's order of these operations (solvers/coupled_step.py:97-166 and the fused
+ * engine's eager-arc step). */
+static void launch_droplet_step(const StepConsts &s0, const waam_step_fields &f,
+                                const waam_droplet &dp, hipStream_t st, bool capture)
+{
+    const Dims &d = s0.d;
+    const size_t N = d.n();
+    const float *arc_k = reinterpret_cast<const float *>((char *)f.scratch + kScratchTelemetry) + 2;
+    const bool f32 = f.pop_dtype == WAAM_POP_F32;
+    /* the previous step's deferred stream */
+    if (s0.p.pending_stream) {
+        if (f32)
+            hipLaunchKernelGGL(k_stream_pending<float>, grid3(d), block3(), 0, st,
+                               static_cast<const float *>(f.f_src), static_cast<float *>(f.f_dst),
+                               (const uint8_t *)f.sflags_prev, d);
+        else
+            hipLaunchKernelGGL(k_stream_pending<uint16_t>, grid3(d), block3(), 0, st,
+                               static_cast<const uint16_t *>(f.f_src),
+                               static_cast<uint16_t *>(f.f_dst), (const uint8_t *)f.sflags_prev,
+                               d);
+    }
+    /* the deposition and the momentum impact write fp32 populations */
+    if (!f32) {
+        pops_convert(false, f.f_src, dp.f32_src, N, f.pop_dtype, st);
+        pops_convert(false, f.f_dst, dp.f32_dst, N, f.pop_dtype, st);
+    }
+    /* kernel 1's column scan and arc weight sums, then arc heat and the ceiling on H */
+    StepConsts sa = s0;
+    sa.p.arc_applied = 0;
+    sa.p.use_arc_k_override = 0;
+    sa.p.quiet_skip = 0;
+    sa.p.thermal_list_ready = 0;
+    hipLaunchKernelGGL(k_step_begin, dim3(kReduceBlocks), dim3(kBlock), 0, st, f, sa, 1);
+    hipLaunchKernelGGL(k_step_arc_total, dim3(1), dim3(kBlock), 0, st, f, sa);
+    hipLaunchKernelGGL(k_droplet_arc, grid3(d), block3(), 0, st, f, sa);
+    /* forces, cleared before the droplet's impact pressure adds to them */
+    {
+        unsigned g = grid_for(N);
+        if (g > 65536u) g = 65536u;
+        hipLaunchKernelGGL(k_zero3, dim3(g), dim3(kBlock), 0, st, f.Fx, f.Fy, f.Fz, N);
+    }
+    const float *arc_k4 = arc_k + 2;  // tot[4]: the scan before the deposition
+    if (dp.max_tracers > 0 && dp.spawn > 0)
+        (void)launch_tracers_dev(dp.tracer_pos, dp.tracer_active, dp.tracer_head, dp.max_tracers,
+                                 dp.tracer_x, dp.tracer_y, arc_k4, dp.dx, dp.sigma_m, dp.spawn,
+                                 dp.seed, st);
+    (void)launch_deposit_dev(dp.f32_src, f.flags_in, f.fl_in, f.phi, f.H_in, f.T_in, f.rho,
+                             dp.vol_acc, dp.real_acc, f.scratch, dp.result, dp.arc_i, dp.arc_j,
+                             arc_k4, dp.footprint_r, dp.drop_r, dp.drop_vol, dp.T_drop, dp.cp_rho,
+                             dp.L_rho, dp.rho0, dp.cell_vol, d, st);
+    hipLaunchKernelGGL(k_droplet_ledger, dim3(1), dim3(64), 0, st, (const float *)dp.result,
+                       dp.drop_vol, dp.ledger);
+    if (dp.momentum)
+        (void)launch_momentum_impact_dev(dp.f32_src, f.rho, f.ux, f.uy, f.uz, f.flags_in, f.fl_in,
+                                         dp.arc_i, dp.arc_j, arc_k4, dp.drop_r, dp.vx, dp.vy,
+                                         dp.vz, d, st);
+    if (dp.pressure)
+        (void)launch_droplet_pressure_dev(f.Fz, f.flags_in, f.phi, f.fl_in, dp.arc_i, dp.arc_j,
+                                          arc_k4, dp.drop_r, dp.impact_pa, dp.dt, dp.dx_f,
+                                          dp.rho_ref, d, st);
+    if (!f32) {
+        pops_convert(true, f.f_src, dp.f32_src, N, f.pop_dtype, st);
+        pops_convert(true, f.f_dst, dp.f32_dst, N, f.pop_dtype, st);
+    }
+    /* the step, with the arc applied, the scan kept in tot[4] and the impact's forces */
+    StepConsts s = s0;
+    s.p.arc_applied = 1;
+    s.p.use_arc_k_override = 2;
+    s.p.forces_from_fields = 1;
+    s.p.pending_stream = 0;
+    s.p.thermal_list_ready = 0;
+    s.p.force_reset_all = 0;
+    s.p.quiet_skip = 0;
+    launch_step(s, f, st, capture);
+}
+
 }  // namespace waam
+
+extern "C" {
+
+int waam_step_fused_droplet(const waam_step_params *pp, const waam_step_fields *ff,
+                            const waam_droplet *dd, void *stream)
+{
+    if (!pp || !ff || !dd)
+        return set_error(WAAM_ERR_INVALID, "waam_step_fused_droplet: NULL argument");
+    const waam_step_params &p = *pp;
+    const waam_step_fields &f = *ff;
+    WAAM_RETURN_IF(check_fields("waam_step_fused_droplet", f));
+    WAAM_RETURN_IF(check_tables("waam_step_fused_droplet", p, f));
+    WAAM_RETURN_IF(check_droplet("waam_step_fused_droplet", f, *dd));
+    if (p.col_i < 0 || p.col_i >= f.nx || p.col_j < 0 || p.col_j >= f.ny)
+        return set_error(WAAM_ERR_INVALID, "waam_step_fused_droplet: arc column outside the grid");
+    if (!p.is_welding || p.arc_kind > 1)
+        return set_error(WAAM_ERR_INVALID,
+                         "waam_step_fused_droplet: needs a welding step with the Gaussian or "
+                         "Goldak source");
+    Dims d{f.nx, f.ny, f.nz, f.layout};
+    StepConsts s = make_step_consts(p, d);
+    launch_droplet_step(s, f, *dd, as_stream(stream), false);
+    return check_launch("waam_step_fused_droplet");
+}
+
+}  // extern "C"
 
 extern "C" {
 
@@ -2300,6 +2529,17 @@ int waam_unpack_populations(const void *src, float *dst, int64_t n_cells, int po
         return set_error(WAAM_ERR_INVALID, "waam_unpack_populations: pop_dtype %d is not a "
                          "16-bit form", pop_dtype);
     return check_launch("waam_unpack_populations");
+}
+
+int waam_step_droplet_arc_k(const void *scratch, float *host_out)
+{
+    if (!scratch || !host_out)
+        return set_error(WAAM_ERR_INVALID, "waam_step_droplet_arc_k: NULL");
+    hipError_t e = hipMemcpy(host_out, (const char *)scratch + kScratchTelemetry + 4 * sizeof(float),
+                             sizeof(float), hipMemcpyDeviceToHost);
+    if (e != hipSuccess)
+        return set_error(WAAM_ERR_HIP, "waam_step_droplet_arc_k: %s", hipGetErrorString(e));
+    return 0;
 }
 
 int waam_step_arc_readback(const void *scratch, float *host_out3)

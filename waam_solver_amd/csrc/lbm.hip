@@ -129,13 +129,22 @@ __global__ void k_momentum_impact(float *__restrict__ f_src, const float *__rest
                                   float *__restrict__ uz, const uint8_t *__restrict__ flags,
                                   const float *__restrict__ f_l, float arc_i, float arc_j,
                                   float arc_k, float dr, float vx, float vy, float vz, int i0,
-                                  int j0, int k0, int ni, int nj, int nk, Dims d)
+                                  int j0, int k0, int ni, int nj, int nk, Dims d,
+                                  const float *__restrict__ arc_k_dev)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ni * nj * nk) return;
     int k = k0 + t % nk;
     int j = j0 + (t / nk) % nj;
     int i = i0 + t / (nk * nj);
+    if (arc_k_dev) {
+        /* arc_k resolved on the device: the launch covers every k, and the host's k range
+         * [(int)arc_k, (int)(arc_k + 3 dr) + 1) is applied here */
+        arc_k = *arc_k_dev;
+        const int k_min = imax((int)arc_k, 0);
+        const int k_max = imin((int)(arc_k + dr * 3.0f) + 1, d.nz);
+        if (k < k_min || k >= k_max) return;
+    }
     size_t N = d.n();
     size_t c = d.idx(i, j, k);
     if (flags[c] == GAS) return;
@@ -278,8 +287,30 @@ int waam_feed_wire_momentum_impact(float *f_src, const float *rho, float *ux, fl
     Dims d{nx, ny, nz, layout};
     hipLaunchKernelGGL(k_momentum_impact, dim3(grid_for((size_t)ni * nj * nk)), dim3(kBlock), 0,
                        as_stream(stream), f_src, rho, ux, uy, uz, flags, f_l, arc_i, arc_j,
-                       arc_k, dr, vx_lu, vy_lu, vz_lu, i_min, j_min, k_min, ni, nj, nk, d);
+                       arc_k, dr, vx_lu, vy_lu, vz_lu, i_min, j_min, k_min, ni, nj, nk, d,
+                       (const float *)nullptr);
     return check_launch("waam_feed_wire_momentum_impact");
 }
 
 }  // extern "C"
+
+namespace waam {
+
+int launch_momentum_impact_dev(float *f_src, const float *rho, float *ux, float *uy, float *uz,
+                               const uint8_t *flags, const float *f_l, float arc_i, float arc_j,
+                               const float *arc_k_dev, float dr, float vx, float vy, float vz,
+                               Dims d, hipStream_t st)
+{
+    int i_min = (int)(arc_i - dr * 2.5f); i_min = i_min > 0 ? i_min : 0;
+    int i_max = (int)(arc_i + dr * 2.5f) + 1; i_max = i_max < d.nx ? i_max : d.nx;
+    int j_min = (int)(arc_j - dr * 2.5f); j_min = j_min > 0 ? j_min : 0;
+    int j_max = (int)(arc_j + dr * 2.5f) + 1; j_max = j_max < d.ny ? j_max : d.ny;
+    const int ni = i_max - i_min, nj = j_max - j_min, nk = d.nz;
+    if (ni <= 0 || nj <= 0) return WAAM_OK;
+    hipLaunchKernelGGL(k_momentum_impact, dim3(grid_for((size_t)ni * nj * nk)), dim3(kBlock), 0,
+                       st, f_src, rho, ux, uy, uz, flags, f_l, arc_i, arc_j, 0.0f, dr, vx, vy, vz,
+                       i_min, j_min, 0, ni, nj, nk, d, arc_k_dev);
+    return check_launch("waam_step_fused_droplet (momentum impact)");
+}
+
+}  // namespace waam

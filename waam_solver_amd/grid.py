@@ -93,6 +93,9 @@ class WAAMGrid:
         self.deposit_real_buf = z0()
         self.arc_norm_buf = z0()
         self.deposit_result = DeviceField.zeros((2,), torch.float32, dev)
+        # the run's deposition ledger, kept on the device: [0] placed volume (m^3, binary64 sum
+        # in droplet order), [1] droplets that placed less than 98 % of their volume
+        self.deposit_ledger = DeviceField.zeros((2,), torch.float64, dev)
         self.telem_out = DeviceField.zeros((8,), torch.float64, dev)
         self.bead_top_buf = DeviceField.zeros((1,), torch.int32, dev)
         self.telem_fl_slice = DeviceField.zeros((self.ny, self.nz), torch.float32, dev)

@@ -93,10 +93,9 @@ def droplet_due(twin, g) -> bool:
     return not (twin._step_n * g.dt - twin._last_droplet_time < period)
 
 
-def _droplet(twin, g, x_m: float, y_m: float, arc_i: float, arc_j: float, arc_k: float) -> None:
-    """Droplet block (reference coupled_step.py:110-166)."""
-    if not droplet_due(twin, g):
-        return
+def droplet_plan(twin, g) -> tuple[float, float, float, float, float]:
+    """The host side of a detaching droplet (reference coupled_step.py:110-124, 136): advances
+    the droplet clock; (drop_mass kg, drop_r cells, drop_vol m^3, T_drop K, footprint cells)."""
     sim_time = twin._step_n * g.dt
     drop_dt = sim_time - twin._last_droplet_time
     twin._last_droplet_time = sim_time
@@ -104,10 +103,18 @@ def _droplet(twin, g, x_m: float, y_m: float, arc_i: float, arc_j: float, arc_k:
     drop_r = deposition.droplet_radius_cells_from_mass_kg(twin, drop_mass)
     drop_vol = drop_mass / twin.mat.rho
     T_drop = droplet_entry_temperature_K(twin)
+    foot_r = deposition.deposition_footprint_cells(twin, drop_r)
+    return drop_mass, drop_r, drop_vol, T_drop, foot_r
+
+
+def _droplet(twin, g, x_m: float, y_m: float, arc_i: float, arc_j: float, arc_k: float) -> None:
+    """Droplet block (reference coupled_step.py:110-166)."""
+    if not droplet_due(twin, g):
+        return
+    drop_mass, drop_r, drop_vol, T_drop, foot_r = droplet_plan(twin, g)
     kernels.inject_tracers(g.porosity_pos, g.porosity_active, g.tracer_head, g.max_tracers, x_m,
                            y_m, float(arc_k + 1) * g.dx, float(twin.sigma_cells * g.dx), 50,
                            seed=twin.tracer_seed)
-    foot_r = deposition.deposition_footprint_cells(twin, drop_r)
     from .._lib import lib
     import torch
 

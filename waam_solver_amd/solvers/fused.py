@@ -38,7 +38,8 @@ from ..field import DeviceField, P, tensor_of, write_epoch
 from ..physics import weld_forces
 from ..physics.electrical_stickout import update_ctwd
 from ..gpu_tables import MaterialTablesC, PropertyRefsC
-from .coupled_step import _clamp_enthalpy_ceiling, _droplet, droplet_due, resolve_arc_k
+from .coupled_step import (_clamp_enthalpy_ceiling, _droplet, droplet_due, droplet_plan,
+                           resolve_arc_k)
 
 _i32, _f32, _vp = ctypes.c_int32, ctypes.c_float, ctypes.c_void_p
 
@@ -78,6 +79,39 @@ class StepParams(ctypes.Structure):
     _fields_ = ([(n, _i32) for n in _INT_PARAMS] + [(n, _f32) for n in _FLOAT_PARAMS] +
                 [("use_tables", _i32), ("variable_tau", _i32), ("H_sol_tab", _f32),
                  ("H_liq_tab", _f32), ("refs", PropertyRefsC), ("tables", MaterialTablesC)])
+
+
+_f64, _u32 = ctypes.c_double, ctypes.c_uint32
+
+
+class DropletC(ctypes.Structure):
+    """include/waam_hip.h waam_droplet: one droplet step's deposition, tracers and impact."""
+
+    _fields_ = [("f32_src", _vp), ("f32_dst", _vp), ("drop_vol", _f64), ("footprint_r", _f64),
+                ("arc_i", _f32), ("arc_j", _f32), ("drop_r", _f32), ("T_drop", _f32),
+                ("cp_rho", _f32), ("L_rho", _f32), ("rho0", _f32), ("cell_vol", _f32),
+                ("vol_acc", _vp), ("real_acc", _vp), ("result", _vp), ("ledger", _vp),
+                ("tracer_pos", _vp), ("tracer_active", _vp), ("tracer_head", _vp),
+                ("max_tracers", _i32), ("spawn", _i32), ("seed", _u32), ("tracer_x", _f32),
+                ("tracer_y", _f32), ("sigma_m", _f32), ("dx", _f64), ("momentum", _i32),
+                ("pressure", _i32), ("vx", _f32), ("vy", _f32), ("vz", _f32),
+                ("impact_pa", _f32), ("dt", _f32), ("dx_f", _f32), ("rho_ref", _f32)]
+
+
+class _DropletArcK:
+    """The arc_k a droplet step resolved on the device, read back when a caller needs it."""
+
+    def __init__(self, g):
+        self._scratch = g.scratch
+        self._value = None
+
+    def __float__(self) -> float:
+        if self._value is None:
+            out = ctypes.c_float()
+            torch.cuda.current_stream().synchronize()
+            lib().call("waam_step_droplet_arc_k", P(self._scratch), ctypes.addressof(out))
+            self._value = float(out.value)
+        return self._value
 
 
 def _swap(g, a: str, b: str) -> None:
@@ -328,10 +362,18 @@ class FusedStepper:
         if tw.enable_ctwd:
             # after arc_k's inputs, before this step changes the flags (reference order)
             update_ctwd(tw, g)
-        # Eager arc: droplet steps (deposition edits the fields the fused kernels read, and the
-        # deferred stream reads f_src, which deposition changes) and the conical source (whose
-        # centre sits 1.5 cells below the resolved arc_k).
-        if is_welding and (droplet_due(tw, g) or type(tw.arc_source).__name__ == "ConicalVolume"):
+        conical = type(tw.arc_source).__name__ == "ConicalVolume"
+        drop = None
+        if is_welding and not conical and droplet_due(tw, g):
+            # a droplet step on the device (waam_step_fused_droplet): arc_k, the retry loop and
+            # the placed volume stay there
+            p.pending_stream = 1 if self.pending else 0
+            p.forces_from_fields = 1  # the impact pressure's (set again by the library)
+            drop = self.droplet_struct(torch_x_m, torch_y_m, arc_i, arc_j)
+            tw._last_arc_ijk = (float(arc_i), float(arc_j), _DropletArcK(g))
+        # Eager arc: the conical source (whose centre sits 1.5 cells below the resolved arc_k),
+        # on every welding step and with its droplets
+        elif is_welding and conical:
             self.flush()
             arc_k = resolve_arc_k(tw, g, arc_i, arc_j, torch_z_m)
             kernels.clear_forces(g.Fx, g.Fy, g.Fz)
@@ -358,8 +400,12 @@ class FusedStepper:
         if self.pop_code and not self.h_live:
             self._pack()
         fields = self._fields()
-        lib().call(self.entry, ctypes.addressof(p), ctypes.addressof(fields),
-                   torch.cuda.current_stream().cuda_stream)
+        if drop is not None:
+            lib().call("waam_step_fused_droplet", ctypes.addressof(p), ctypes.addressof(fields),
+                       ctypes.addressof(drop), torch.cuda.current_stream().cuda_stream)
+        else:
+            lib().call(self.entry, ctypes.addressof(p), ctypes.addressof(fields),
+                       torch.cuda.current_stream().cuda_stream)
         for a, b in (("H", "H_tmp"), ("T", "T_tmp"), ("f_l", "fl_tmp"), ("flags", "flags_tmp"),
                      ("sflags", "sflags_tmp")):
             _swap(g, a, b)
@@ -370,13 +416,48 @@ class FusedStepper:
         self._list_epoch = write_epoch()
         tw._step_n += 1
 
+    def droplet_struct(self, x_m: float, y_m: float, arc_i: float, arc_j: float,
+                       odd: bool = False) -> DropletC:
+        """The waam_droplet of a droplet detaching now (advances the twin's droplet clock and
+        count, as the reference's droplet block does): the deposition, tracer and impact
+        parameters the host computes in binary64 (reference coupled_step.py:110-166). odd: for
+        a step an odd number of steps ahead (its population buffers swapped)."""
+        tw, g = self.twin, self.twin.grid
+        drop_mass, drop_r, drop_vol, T_drop, foot_r = droplet_plan(tw, g)
+        d = DropletC()
+        d.f32_src, d.f32_dst = P(g.f_src), P(g.f_dst)
+        if odd:
+            d.f32_src, d.f32_dst = d.f32_dst, d.f32_src
+        d.drop_vol, d.footprint_r = float(drop_vol), float(foot_r)
+        d.arc_i, d.arc_j, d.drop_r, d.T_drop = arc_i, arc_j, drop_r, T_drop
+        d.cp_rho, d.L_rho, d.rho0, d.cell_vol = tw.cp_rho, tw.L_rho, 1.0, g.dx ** 3
+        d.vol_acc, d.real_acc = P(g.deposit_vol_buf), P(g.deposit_real_buf)
+        d.result, d.ledger = P(g.deposit_result), P(g.deposit_ledger)
+        if g.max_tracers > 0:
+            d.tracer_pos, d.tracer_active = P(g.porosity_pos), P(g.porosity_active)
+            d.tracer_head, d.max_tracers, d.spawn = P(g.tracer_head), g.max_tracers, 50
+            d.seed = int(tw.tracer_seed) & 0xFFFFFFFF
+            d.tracer_x, d.tracer_y = x_m, y_m
+            d.sigma_m, d.dx = float(tw.sigma_cells * g.dx), g.dx
+        # counted before the impact (reference coupled_step.py:162-163): the impact velocity's
+        # mass term reads the droplet period, whose jitter follows the count
+        tw._n_droplets_fired += 1
+        if tw.enable_deposition_momentum:
+            d.momentum = 1
+            d.vx, d.vy, d.vz = weld_forces.droplet_velocity_lu(tw, drop_mass)
+            if tw.enable_droplet_impact_pressure:
+                d.pressure = 1
+                d.impact_pa = weld_forces.droplet_impact_pressure_pa(tw, drop_mass, drop_r * g.dx)
+                d.dt, d.dx_f, d.rho_ref = g.dt, g.dx, tw.mat.rho
+        return d
+
     # ------------------------------------------------------------------ captured steps
     def steady(self) -> bool:
         """Whether the next step may be a captured one: the last step was a fused step that
-        left its population stream pending and its thermal list built, and did not take its
-        forces from the fields, and nothing was written since."""
-        return (self.pending and not self._forces_from_fields
-                and self._list_epoch == write_epoch())
+        left its population stream pending and its thermal list built, and nothing was written
+        since (after a droplet step the captured run starts with a step that clears every
+        cell's forces)."""
+        return self.pending and self._list_epoch == write_epoch()
 
     def _odd_fields(self, fe: StepFields) -> StepFields:
         fo = StepFields.from_buffer_copy(fe)
@@ -387,30 +468,47 @@ class FusedStepper:
             setattr(fo, b, getattr(fe, a))
         return fo
 
-    def _graph(self, p: StepParams, n: int):
-        """The captured graph of n steps for these parameters and the current buffers."""
+    def _graph(self, p: StepParams, n: int, kinds=None, drops=None):
+        """The captured graph of n steps for these parameters, the current buffers and the run's
+        plan (kinds: per-step 0 steady, 1 droplet with drops[k], 2 after a droplet; None: all
+        steady)."""
         fe = self._fields()
         fo = self._odd_fields(fe)
-        key = (bytes(p), bytes(fe), n)
+        plan = None
+        if kinds is not None and np.any(kinds):
+            dl = (DropletC * n)()
+            for k, d in enumerate(drops):
+                if d is not None:
+                    dl[k] = d
+            kn = np.ascontiguousarray(kinds, dtype=np.int32)
+            plan = (kn, dl)
+        key = (bytes(p), bytes(fe), n) + ((plan[0].tobytes(), bytes(plan[1])) if plan else ())
         gr = self._graphs.get(key)
         if gr is None:
             while len(self._graphs) >= self.graph_cache:
                 _, old = self._graphs.popitem(last=False)
                 lib().call("waam_graph_destroy", old)
             h = ctypes.c_void_p()
-            lib().call("waam_graph_capture", ctypes.addressof(p), ctypes.addressof(fe),
-                       ctypes.addressof(fo), n, ctypes.byref(h))
+            if plan:
+                lib().call("waam_graph_capture_plan", ctypes.addressof(p), ctypes.addressof(fe),
+                           ctypes.addressof(fo), n, plan[0].ctypes.data,
+                           ctypes.addressof(plan[1]), ctypes.byref(h))
+            else:
+                lib().call("waam_graph_capture", ctypes.addressof(p), ctypes.addressof(fe),
+                           ctypes.addressof(fo), n, ctypes.byref(h))
             gr = self._graphs[key] = h
         else:
             self._graphs.move_to_end(key)
         return gr
 
     def step_captured(self, arc_i: np.ndarray, arc_j: np.ndarray, is_welding: bool,
-                      k_robot: np.ndarray | None, dirs: np.ndarray) -> None:
-        """len(arc_i) steady fused steps as one graph launch: arc centres (cells, float64),
-        robot arc_k floors (float32 or None) and torch directions (float32, (n, 3)) per step;
-        the same kernels and results as that many ``step`` calls. The caller checks
-        ``steady()`` and that no step of the run is a droplet step."""
+                      k_robot: np.ndarray | None, dirs: np.ndarray, kinds=None,
+                      drops=None) -> None:
+        """len(arc_i) fused steps as one graph launch: arc centres (cells, float64), robot arc_k
+        floors (float32 or None) and torch directions (float32, (n, 3)) per step, and the run's
+        plan (kinds, drops: see _graph); the same kernels and results as that many ``step``
+        calls. The caller checks ``steady()``; with a plan, the droplet structs were made for
+        these steps (droplet_struct), in order."""
         tw, g = self.twin, self.twin.grid
         n = len(arc_i)
         p = self._params(float(arc_i[0]), float(arc_j[0]), is_welding, None)
@@ -424,7 +522,7 @@ class FusedStepper:
         p.col_i = p.col_j = 0
         if self.pop_code and not self.h_live:
             self._pack()
-        gr = self._graph(p, n)
+        gr = self._graph(p, n, kinds, drops)
         ai = np.ascontiguousarray(arc_i, dtype=np.float64)
         aj = np.ascontiguousarray(arc_j, dtype=np.float64)
         kr = None if k_robot is None else np.ascontiguousarray(k_robot, dtype=np.float32)
@@ -440,7 +538,7 @@ class FusedStepper:
                 _swap(g, "cp_rho_field", "cp_rho_field_tmp")
             g.swap_buffers()
         self.pending = True
-        self._forces_from_fields = False
+        self._forces_from_fields = kinds is not None and int(kinds[-1]) == 1
         self._list_epoch = write_epoch()
         tw._step_n += n
         self.captured_steps += n

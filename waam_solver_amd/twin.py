@@ -23,15 +23,14 @@ from .gpu_tables import MaterialGPUTables
 from .grid import WAAMGrid
 from .materials import MaterialProps, load_material
 from .physics.arc import create_heat_source
-from .physics.deposition_balance import (droplet_mass_kg, droplet_period_s,
-                                         expected_deposited_mass_kg, infer_transfer_mode,
-                                         wire_mass_flux_kg_s)
+from .physics.deposition_balance import (droplet_mass_kg, expected_deposited_mass_kg,
+                                         infer_transfer_mode, wire_mass_flux_kg_s)
 from .physics.bead_geometry import bead_reinforcement_height_mm, estimate_toe_angle_deg
 from .physics.electrical_stickout import droplet_entry_temperature_K
 from .physics.force_diagnostics import sample_force_diagnostics
 from .physics.weld_forces import arc_pressure_peak_pa, droplet_impact_velocity_m_s
-from .solvers.coupled_step import coupled_step
-from .solvers.fused import FusedStepper
+from .solvers.coupled_step import coupled_step, droplet_due
+from .solvers.fused import FusedStepper, _DropletArcK
 
 log = logging.getLogger("waam_solver_amd")
 
@@ -201,6 +200,34 @@ class WAAMTwin:
         self.H_sol = mat.rho * cp_sol * mat.T_solidus
         self.H_liq = self.H_sol + self.L_rho
         self.arc_source = create_heat_source(heat_source)
+
+    # The deposition ledger lives on the device (grid.deposit_ledger), where the fused droplet
+    # step adds each droplet's placed volume without a host read; these read and write it.
+    @property
+    def _deposited_volume_m3(self) -> float:
+        return float(self.grid.deposit_ledger.to_numpy()[0])
+
+    @_deposited_volume_m3.setter
+    def _deposited_volume_m3(self, v: float) -> None:
+        self._ledger_put(0, float(v))
+
+    @property
+    def _deposition_overflow(self) -> int:
+        n = int(self.grid.deposit_ledger.to_numpy()[1])
+        if n >= 20 and not getattr(self, "_warned_overflow", True):
+            self._warned_overflow = True
+            log.warning(f"[WAAMTwin] {n} droplets deposited less volume than the wire feed "
+                        "supplies; mass_balance_ratio will drift")
+        return n
+
+    @_deposition_overflow.setter
+    def _deposition_overflow(self, n: int) -> None:
+        self._ledger_put(1, float(n))
+
+    def _ledger_put(self, i: int, v: float) -> None:
+        a = self.grid.deposit_ledger.to_numpy()
+        a[i] = v
+        self.grid.deposit_ledger.from_numpy(a)
 
     def _reset_counters(self) -> None:
         self._step_n = 0
@@ -421,11 +448,6 @@ class WAAMTwin:
         if self.use_torch_z and zs is not None and any(z is None for z in zs[s:s + m]):
             return 0
         g = self.grid
-        if is_welding and self.droplet_freq > 0:
-            period = droplet_period_s(self)
-            for k in range(m):
-                if not ((self._step_n + k) * g.dt - self._last_droplet_time < period):
-                    return 0
         if self.enable_moving_window:
             trigger = 0.62 * (g.nx - 8) * g.dx
             if any(x - self._window_offset_x_m >= trigger for x in xs[s:s + m]):
@@ -437,6 +459,11 @@ class WAAMTwin:
         centre), then one captured launch of all the steps."""
         g, st = self.grid, self._fused
         m = len(xs)
+        base_n = self._step_n
+        # the run's plan: 0 a steady step, 1 a droplet step (its deposition parameters in
+        # drops[k]), 2 the step after a droplet step
+        kinds = np.zeros(m, dtype=np.int32)
+        drops = [None] * m
         arc_i = np.empty(m, dtype=np.float64)
         arc_j = np.empty(m, dtype=np.float64)
         dirs = np.empty((m, 3), dtype=np.float32)
@@ -458,8 +485,22 @@ class WAAMTwin:
             dirs[k] = self._torch_dir_xyz
             if use_kr:
                 k_robot[k] = st._k_robot(z)
-        st.step_captured(arc_i, arc_j, is_welding, k_robot, dirs)
-        self._last_arc_ijk = (float(arc_i[-1]), float(arc_j[-1]), None)
+            if (k == 0 and st._forces_from_fields) or (k > 0 and kinds[k - 1] == 1):
+                kinds[k] = 2
+            if is_welding and self.droplet_freq > 0:
+                # the droplet clock as the direct steps would advance it
+                self._step_n = base_n + k
+                if droplet_due(self, g):
+                    kinds[k] = 1
+                    drops[k] = st.droplet_struct(
+                        x - self._window_offset_x_m - self._sim_origin_offset_x_m,
+                        y - self._sim_origin_offset_y_m, arc_i[k], arc_j[k], odd=bool(k & 1))
+                self._step_n = base_n
+        st.step_captured(arc_i, arc_j, is_welding, k_robot, dirs, kinds, drops)
+        if kinds[-1] == 1:
+            self._last_arc_ijk = (float(arc_i[-1]), float(arc_j[-1]), _DropletArcK(g))
+        else:
+            self._last_arc_ijk = (float(arc_i[-1]), float(arc_j[-1]), None)
         self._check_strict_mode()
 
     def _maybe_shift_window(self, torch_x_world: float) -> None:
