@@ -245,12 +245,24 @@ __device__ __forceinline__ void cell_ijk(size_t c, const Dims &d, int &i, int &j
  * sweeps over that list only. The record: bits[seg] bit 0 = quiet at the end of the last step,
  * bit 1 = quiet at the end of the step before (written by kernel 7 for the segments it visits;
  * a segment left out keeps 3, which is what it would write); then the active-segment count and
- * list. */
+ * list.
+ *
+ * Cold plate. A segment is cold at the end of a step when every cell in it ends the step SOLID,
+ * was SOLID after the flag update as well, holds phi 1 and f_l 0, and is below the solidus and
+ * both HAZ thresholds. For such a cell kernels 4 and 7 write nothing that differs from what
+ * their buffers hold once the segment has been cold for two steps (the flags and stream flags
+ * of both buffers are SOLID, phi is 1, no HAZ clock runs, no freeze rule changes a value, the
+ * velocity is already zeroed and no force was set): the only per-step change is kernel 2's
+ * conduction. bits[seg] bit 2 = cold at the end of the last step, bit 3 = at the end of the
+ * step before. Kernel 2 re-checks a listed segment whose bits 2 and 3 are set on its new
+ * temperature and liquid fraction; when every cell is still cold it records the segment cold
+ * again and leaves it out of kernels 4 and 7 (need[pos] = 0 at its list position). */
 struct QuietRec {
     uint8_t *bits;
     int *count;
     int32_t *list;
-    int nbx;  // segments per row
+    uint8_t *need;  // per list position: kernels 4 and 7 visit the segment
+    int nbx;        // segments per row
 };
 
 __host__ __device__ inline int quiet_nbx(const Dims &d) { return (d.fast() + kTileFast - 1) / kTileFast; }
@@ -263,9 +275,13 @@ __host__ __device__ inline size_t quiet_count_offset(const Dims &d)
 {
     return (quiet_segments(d) + 255) / 256 * 256;
 }
-__host__ __device__ inline size_t quiet_bytes(const Dims &d)
+__host__ __device__ inline size_t quiet_need_offset(const Dims &d)
 {
     return quiet_count_offset(d) + 256 + 4 * quiet_segments(d);
+}
+__host__ __device__ inline size_t quiet_bytes(const Dims &d)
+{
+    return quiet_need_offset(d) + quiet_segments(d);
 }
 
 __device__ __forceinline__ QuietRec quiet_rec(const waam_step_fields &f, const Dims &d)
@@ -275,6 +291,7 @@ __device__ __forceinline__ QuietRec quiet_rec(const waam_step_fields &f, const D
     q.bits = f.quiet;
     q.count = reinterpret_cast<int *>(f.quiet + quiet_count_offset(d));
     q.list = reinterpret_cast<int32_t *>(f.quiet + quiet_count_offset(d) + 256);
+    q.need = f.quiet + quiet_need_offset(d);
     return q;
 }
 
@@ -316,7 +333,7 @@ __device__ __forceinline__ bool segment_active(const QuietRec &q, int bx, int j,
             }
         }
     }
-    const unsigned own = q.bits[seg_index(q, bx, j, b, d)];
+    const unsigned own = q.bits[seg_index(q, bx, j, b, d)] & 3u;
     return !((own == 3u) && (all & 1u));
 }
 
@@ -375,20 +392,52 @@ __device__ __forceinline__ void for_listed_segments(const QuietRec &q, int n, Bo
         const long long tl = t0 + (long long)lane * nw;
         const int mine = tl < n ? q.list[tl] : 0;
         const int cnt = (int)std::min<long long>(64, (n - t0 + nw - 1) / nw);
-        for (int it = 0; it < cnt; ++it) body((size_t)(unsigned)__shfl(mine, it));
+        for (int it = 0; it < cnt; ++it)
+            body((size_t)(unsigned)__shfl(mine, it), t0 + (long long)it * nw);
     }
 }
 
-/* the wavefront's segment ends the step quiet (`quiet`: this lane's cell is, or it has none):
- * kernel 7 records it, shifting the previous bit up when the record is continued */
-__device__ __forceinline__ void record_quiet(const QuietRec &q, size_t seg, bool quiet,
+/* the same walk over the list positions kernel 2 marked needed (the cold plate left out) */
+template <class Body>
+__device__ __forceinline__ void for_needed_segments(const QuietRec &q, int n, Body &&body)
+{
+    const int nw = (int)(gridDim.x * blockDim.y);
+    const int w = (int)(blockIdx.x * blockDim.y + threadIdx.y);
+    const int lane = (int)threadIdx.x;
+    for (long long t0 = w; t0 < n; t0 += 64LL * nw) {
+        const long long tl = t0 + (long long)lane * nw;
+        const bool in = tl < n;
+        const int mine = in ? q.list[tl] : 0;
+        const bool nd = in && q.need[tl] != 0;
+        unsigned long long m = __ballot(nd);
+        while (m != 0ull) {
+            const int it = __builtin_ctzll(m);
+            m &= m - 1ull;
+            body((size_t)(unsigned)__shfl(mine, it));
+        }
+    }
+}
+
+/* the wavefront's segment ends the step quiet (`quiet`: this lane's cell is, or it has none)
+ * and cold (`cold`, likewise): kernel 7 records both, shifting the previous bits up when the
+ * record is continued (a record started afresh holds no cold history) */
+__device__ __forceinline__ void record_quiet(const QuietRec &q, size_t seg, bool quiet, bool cold,
                                              bool cont)
 {
     const bool all = __ballot(!quiet) == 0ull;
+    const bool all_cold = __ballot(!cold) == 0ull;
     if (threadIdx.x == 0) {
-        const uint8_t old = q.bits[seg];
-        q.bits[seg] = (uint8_t)((cont ? ((old & 1u) << 1) : 0u) | (all ? 1u : 0u));
+        const unsigned old = q.bits[seg];
+        q.bits[seg] = (uint8_t)(cont ? (((old & 1u) << 1) | (all ? 1u : 0u) | ((old & 4u) << 1) |
+                                        (all_cold ? 4u : 0u))
+                                     : (all ? 1u : 0u));
     }
+}
+
+/* below the solidus and both HAZ thresholds: no freeze rule and no HAZ clock acts on the cell */
+__device__ __forceinline__ float cold_limit(const waam_step_params &p)
+{
+    return fminf(p.T_s, fminf(p.T800, p.T1100));
 }
 
 /* blocks of the sweeps over the active-segment list (each grid-strides over it by wavefront) */
@@ -678,7 +727,8 @@ __device__ __forceinline__ HView make_hview(const waam_step_fields &f, const Ste
 }
 
 /* thermal chain of one cell (reference kernels.py:497-510, 1953-1984, 1611-1682, 2030-2082,
- * 1937-1950, 1519-1556, 3016-3032, 2013-2027 in coupled_step order) */
+ * 1937-1950, 1519-1556, 3016-3032, 2013-2027 in coupled_step order); returns the cell's new T
+ * and f_l */
 /* the step's refresh_thermal_properties values of one cell, stored (temperature-dependent
  * material) */
 __device__ __forceinline__ void store_props(const waam_step_fields &f, size_t c,
@@ -696,7 +746,7 @@ __device__ __forceinline__ void store_props(const waam_step_fields &f, size_t c,
  * (kernels.py:1775-1863, 1937-1950, 1987-2010), and the properties stored for the rest of the
  * step and the next step's first ceiling */
 template <bool TAB>
-__device__ __forceinline__ void thermal_cell(const waam_step_fields &f, const StepConsts &s,
+__device__ __forceinline__ float2 thermal_cell(const waam_step_fields &f, const StepConsts &s,
                                              int i, int j, int k, size_t c, uint8_t flag,
                                              float h_gas)
 {
@@ -713,7 +763,7 @@ __device__ __forceinline__ void thermal_cell(const waam_step_fields &f, const St
         f.T_out[c] = T;
         f.fl_out[c] = fl;
         if (TAB) store_props(f, c, cell_props(p.tables, p.refs, 1, GAS, 0.0f));
-        return;
+        return make_float2(T, fl);
     }
     HView hv = make_hview(f, s);
     /* advect_diffuse_temperature (Jacobi: every neighbour value is pre-kernel) */
@@ -777,6 +827,7 @@ __device__ __forceinline__ void thermal_cell(const waam_step_fields &f, const St
     f.dT_dt[c] = (T - T_prev) * s.inv_dt;
     f.T_prev[c] = T;
     if (TAB) store_props(f, c, pr);
+    return make_float2(T, fl);
 }
 
 /* advect_phi + reinitialize_phi of one cell (reference kernels.py:2320-2391) */
@@ -818,24 +869,36 @@ __global__ void __launch_bounds__(256) k_step_thermal(waam_step_fields f, StepCo
     if (s.p.enable_vof) f.phi_tmp[c] = phi_new;
 }
 
-/* 2 over the active segments of a steady step (the surface cells go to kernel 3) */
+/* 2 over the active segments of a steady step (the surface cells go to kernel 3); marks which
+ * of them kernels 4 and 7 visit: all but the segments that stay cold plate */
 __global__ void __launch_bounds__(256) k_step_thermal_seg(waam_step_fields f, StepConsts s)
 {
     const Dims &d = s.d;
     const QuietRec q = quiet_rec(f, d);
-    for_listed_segments(q, *q.count, [&](size_t seg) {
+    const float T_cold = cold_limit(s.p);
+    for_listed_segments(q, *q.count, [&](size_t seg, long long pos) {
         int i, j, k;
         const bool in = seg_cell(q, seg, d, i, j, k);
         const size_t c = in ? d.idx(i, j, k) : 0;
+        const uint8_t bits = q.bits[seg];
         const uint8_t flag = in ? f.flags_in[c] : (uint8_t)GAS;
         const float h_in = in ? f.H_in[c] : 0.0f;
         /* a SOLID cell's advected phi is re-initialised to 1 whatever its neighbours hold:
          * the plate's segments (most of a steady step's list) skip those loads */
         float phi_new = 1.0f;
         if (in && s.p.enable_vof && flag != SOLID) phi_new = vof_advect_cell(f, d, i, j, k, flag);
-        if (!in || is_surface_class(flag)) return;
-        thermal_cell<false>(f, s, i, j, k, c, flag, h_in);
-        if (s.p.enable_vof) f.phi_tmp[c] = phi_new;
+        bool cold = !in;
+        if (in && !is_surface_class(flag)) {
+            const float2 tf = thermal_cell<false>(f, s, i, j, k, c, flag, h_in);
+            if (s.p.enable_vof) f.phi_tmp[c] = phi_new;
+            cold = flag == SOLID && tf.x < T_cold && tf.y == 0.0f;
+        }
+        /* cold at the end of the last two steps and still cold on the new T and f_l */
+        const bool skip = (bits & 12u) == 12u && __ballot(!cold) == 0ull;
+        if (threadIdx.x == 0) {
+            q.need[pos] = skip ? 0 : 1;
+            if (skip) q.bits[seg] = 12u;
+        }
     });
 }
 
@@ -981,14 +1044,14 @@ __global__ void __launch_bounds__(256) k_step_vof_flags(waam_step_fields f, Step
     list_append(in && is_surface_class(nf), c, cell_list(f, s.seg_len, 1), d.n());
 }
 
-/* 4 over the active segments of a steady step */
+/* 4 over the segments kernel 2 left in */
 __global__ void __launch_bounds__(256) k_step_vof_flags_seg(waam_step_fields f, StepConsts s)
 {
     const Dims &d = s.d;
     if (blockIdx.x == 0)  // kernel 2 has consumed the list
         cell_list(f, s.seg_len, 0).count(threadIdx.y * blockDim.x + threadIdx.x) = 0;
     const QuietRec q = quiet_rec(f, d);
-    for_listed_segments(q, *q.count, [&](size_t seg) {
+    for_needed_segments(q, *q.count, [&](size_t seg) {
         int i, j, k;
         const bool in = seg_cell(q, seg, d, i, j, k);
         const size_t c = in ? d.idx(i, j, k) : 0;
@@ -1313,7 +1376,8 @@ __global__ void __launch_bounds__(256) k_step_stream(waam_step_fields f, StepCon
         if (j >= d.ny) return;  // the wavefront's whole row is past the grid
         bool quiet = true;
         if (in) quiet = stream_cell<E, true>(f, s, i, j, k);
-        record_quiet(q, seg_index(q, (int)blockIdx.x, j, (int)blockIdx.z, d), quiet, false);
+        record_quiet(q, seg_index(q, (int)blockIdx.x, j, (int)blockIdx.z, d), quiet, false,
+                     false);
         return;
     }
     const int a = (int)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -1348,29 +1412,33 @@ __global__ void __launch_bounds__(256) k_step_stream(waam_step_fields f, StepCon
         if (in)
             quiet = stream_rest_cell(f, s, zyx ? a : b, j, zyx ? b : a, c[t], fc[t], fp[t], T[t],
                                      phi[t], fl[t]);
-        record_quiet(q, seg_index(q, (int)blockIdx.x, j, b, d), quiet, false);
+        record_quiet(q, seg_index(q, (int)blockIdx.x, j, b, d), quiet, false, false);
     }
 }
 
-/* 7 over the active segments of a steady step (the pull ran in its own launch) */
+/* 7 over the segments kernel 2 left in (the pull ran in its own launch) */
 __global__ void __launch_bounds__(256) k_step_stream_seg(waam_step_fields f, StepConsts s)
 {
     const Dims &d = s.d;
     const QuietRec q = quiet_rec(f, d);
-    const bool freeze_on = s.p.enable_freeze != 0;
-    for_listed_segments(q, *q.count, [&](size_t seg) {
+    const float T_cold = cold_limit(s.p);
+    for_needed_segments(q, *q.count, [&](size_t seg) {
         int i, j, k;
         const bool in = seg_cell(q, seg, d, i, j, k);
         const size_t c = in ? d.idx(i, j, k) : 0;
         const uint8_t fc = in ? f.flags_out[c] : (uint8_t)GAS;
         const uint8_t fp = in ? f.sflags_prev[c] : (uint8_t)GAS;
         const float phi = in ? f.phi[c] : 0.0f;
-        const bool fr = in && freeze_on && fc != GAS;
+        /* the freeze inputs, and the cold test's */
+        const bool fr = in && fc != GAS;
         const float T = fr ? f.T_out[c] : 0.0f;
         const float fl = fr ? f.fl_out[c] : 0.0f;
         bool quiet = true;
         if (in) quiet = stream_rest_cell(f, s, i, j, k, c, fc, fp, T, phi, fl);
-        record_quiet(q, seg, quiet, true);
+        /* SOLID after the flag update with phi 1, f_l 0 and T below the limit: no freeze rule
+         * changes it (the cooled rule rewrites the same values), so it ends the step SOLID */
+        const bool cold = !in || (fc == SOLID && phi == 1.0f && fl == 0.0f && T < T_cold);
+        record_quiet(q, seg, quiet, cold, true);
     });
 }
 
