@@ -134,7 +134,24 @@ def run_pair(tw, ot, n_steps: int, start: int = 0) -> None:
 
 
 FIELDS = ("H", "T", "f_l", "phi", "flags", "rho", "ux", "uy", "uz", "T_max", "T_prev", "dT_dt",
-          "Fx", "Fy", "Fz", "time_above_solidus_s")
+          "Fx", "Fy", "Fz", "time_above_800_s", "time_above_1100_s", "time_above_solidus_s")
+
+
+def cold_skipped_segments(tw) -> tuple[int, int]:
+    """(listed, left out as cold plate) row segments of the fused engine's last steady step.
+
+    Reads the step's segment record (``waam_step_fields.quiet``): per row segment of 64 cells
+    along the contiguous axis a byte of history, then the listed-segment count on a 256-byte
+    line, the list (int32) and one byte per list position that is 0 where the thermal sweep
+    left the segment out of the flag-update and stream sweeps."""
+    g = tw.grid
+    fast, slow = (g.nx, g.nz) if g.layout == 1 else (g.nz, g.nx)
+    nseg = -(-fast // 64) * g.ny * slow
+    rec = g.quiet.to_numpy().view(np.uint8)
+    off = -(-nseg // 256) * 256
+    count = int(rec[off:off + 4].view(np.int32)[0])
+    need = rec[off + 256 + 4 * nseg: off + 256 + 4 * nseg + count]
+    return count, int((need == 0).sum())
 
 
 def compare(tw, ot, fields=FIELDS) -> dict:

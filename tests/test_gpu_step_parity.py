@@ -11,7 +11,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from tests.step_cases import FIELDS, build_pair, compare, run_pair
+from tests.step_cases import FIELDS, build_pair, cold_skipped_segments, compare, run_pair
 
 TABLE_FIELDS = ("cp_rho_field", "alpha_lu_field", "dgamma_lu_field", "tau_field")
 
@@ -54,6 +54,14 @@ def test_coupled_step_parity(hip_device, case, engine, layout):
         assert tw._deposited_volume_m3 == ot._deposited_volume_m3
     if case == "ctwd_srt":
         assert tw._bead_height_m == ot._bead_height_m and tw.ctwd_m == ot.ctwd_m
+    if engine == "fused" and case in ("bead_mrt", "bead_box_mrt", "bead_odd_mrt"):
+        listed, skipped = cold_skipped_segments(tw)
+        print(f"[{case}] last steady step: {listed} listed segments, {skipped} cold", flush=True)
+        if layout == 1:
+            # steady steps left cold plate segments (rows along x inside the plate) out of the
+            # flag-update and stream sweeps; in layout 0 a segment runs along z, through the
+            # plate's top, and is never all SOLID on these grids
+            assert 0 < skipped < listed
     if case == "bead_box_mrt" and engine == "fused":
         # the arc's zero-heat box (half-width sqrt(220) sigma + 2 cells) is narrower than the
         # grid in x, so the box branch of the arc sums and the box gate ran

@@ -726,9 +726,6 @@ __device__ __forceinline__ HView make_hview(const waam_step_fields &f, const Ste
     return hv;
 }
 
-/* thermal chain of one cell (reference kernels.py:497-510, 1953-1984, 1611-1682, 2030-2082,
- * 1937-1950, 1519-1556, 3016-3032, 2013-2027 in coupled_step order); returns the cell's new T
- * and f_l */
 /* the step's refresh_thermal_properties values of one cell, stored (temperature-dependent
  * material) */
 __device__ __forceinline__ void store_props(const waam_step_fields &f, size_t c,
@@ -740,12 +737,16 @@ __device__ __forceinline__ void store_props(const waam_step_fields &f, size_t c,
     f.tau[c] = pr.tau;
 }
 
-/* TAB: temperature-dependent material — the cell's properties from the tables at its incoming
+/* thermal chain of one cell (reference kernels.py:497-510, 1953-1984, 1611-1682, 2030-2082,
+ * 1937-1950, 1519-1556, 3016-3032, 2013-2027 in coupled_step order); returns the cell's new T
+ * and f_l.
+ * TAB: temperature-dependent material — the cell's properties from the tables at its incoming
  * temperature (the constants on SOLID and GAS cells; reference coupled_step.py:168-182) in
  * place of the constants, the _variable forms of conduction, floor, ceiling and phase map
  * (kernels.py:1775-1863, 1937-1950, 1987-2010), and the properties stored for the rest of the
- * step and the next step's first ceiling */
-template <bool TAB>
+ * step and the next step's first ceiling.
+ * ADV false: the caller passes SOLID and GAS cells only (no enthalpy advection is compiled) */
+template <bool TAB, bool ADV = true>
 __device__ __forceinline__ float2 thermal_cell(const waam_step_fields &f, const StepConsts &s,
                                              int i, int j, int k, size_t c, uint8_t flag,
                                              float h_gas)
@@ -779,7 +780,7 @@ __device__ __forceinline__ float2 thermal_cell(const waam_step_fields &f, const 
     if (TAB) pr = cell_props(p.tables, p.refs, 1, flag, T_c);
     float dH = pr.cp_rho * pr.alpha_lu * lap * 1.0f;
     float H_c = hv.at(c, i, j, k);
-    if (flag != SOLID) {
+    if (ADV && flag != SOLID) {
         float u = f.ux[c], v = f.uy[c], w = f.uz[c];
         float H_im = hv.masked(i - 1, j, k, H_c);
         float H_ip = hv.masked(i + 1, j, k, H_c);
@@ -883,13 +884,15 @@ __global__ void __launch_bounds__(256) k_step_thermal_seg(waam_step_fields f, St
         const uint8_t bits = q.bits[seg];
         const uint8_t flag = in ? f.flags_in[c] : (uint8_t)GAS;
         const float h_in = in ? f.H_in[c] : 0.0f;
-        /* a SOLID cell's advected phi is re-initialised to 1 whatever its neighbours hold:
-         * the plate's segments (most of a steady step's list) skip those loads */
+        /* a SOLID cell's advected phi is re-initialised to 1 whatever its neighbours hold (the
+         * plate's segments, most of a steady step's list, skip those loads), and kernel 3 owns
+         * the surface cells: only GAS cells advect phi here */
         float phi_new = 1.0f;
-        if (in && s.p.enable_vof && flag != SOLID) phi_new = vof_advect_cell(f, d, i, j, k, flag);
+        if (in && s.p.enable_vof && flag == GAS) phi_new = vof_advect_cell(f, d, i, j, k, flag);
         bool cold = !in;
         if (in && !is_surface_class(flag)) {
-            const float2 tf = thermal_cell<false>(f, s, i, j, k, c, flag, h_in);
+            /* flag is SOLID or GAS: no enthalpy advection */
+            const float2 tf = thermal_cell<false, false>(f, s, i, j, k, c, flag, h_in);
             if (s.p.enable_vof) f.phi_tmp[c] = phi_new;
             cold = flag == SOLID && tf.x < T_cold && tf.y == 0.0f;
         }
