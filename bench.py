@@ -34,8 +34,11 @@ WORKLOAD = dict(nx=256, ny=128, nz=64, dx=2e-4, plate_thickness_mm=4.8, arc_powe
                 wire_feed_m_min=6.5, wire_diameter_mm=1.2, droplet_length_mm=3.0,
                 welding_current_A=140.0, steps=4000)
 # The same physics on 768 x 384 x 192 cells (56.6 M, 153.6 x 76.8 x 38.4 mm at 0.2 mm), the
-# largest single-GPU configuration named for this path (fp32 state: about 16 GB of HBM).
-LARGE = dict(WORKLOAD, nx=768, ny=384, nz=192, x0_mm=30.0, steps=400)
+# largest single-GPU configuration named for this path (fp32 state: about 16 GB of HBM). The
+# timed steps start from the state after `preroll` steps of welding (untimed, captured steps):
+# by then the plate under the arc has warmed and the first droplet (near step 2400) has been
+# placed, so the window steps a deposited bead rather than the plate right after reset.
+LARGE = dict(WORKLOAD, nx=768, ny=384, nz=192, x0_mm=30.0, steps=400, preroll=2600)
 # Three passes on one line with interpass cooling, 20 000 steps on 768 x 384 x 192 cells at
 # dx = 1 mm (dt = 0.1 ms: two 0.5 s interpasses take 10 000 steps at this dx; at 0.2 mm they
 # alone would need 50 000): passes of 3334 welding steps (26.7 mm at 8 mm/s), interpasses of
@@ -217,15 +220,19 @@ def step_bytes(ncell: int, solid: float, gas: float, pop: str) -> float:
 
 
 def timed_run(w: dict, args, pop: str):
-    """Build the seeded twin with population storage `pop`, step the warm-up, then time
-    args.steps steps. Returns (twin, seconds, host seconds, (solid, gas) averaged over the
+    """Build the seeded twin with population storage `pop`, step the pre-roll (captured steps)
+    and the warm-up, then time args.steps steps. Returns (twin, seconds, host seconds, (solid, gas) averaged over the
     start and end of the timed steps, (collided, pulled) likewise)."""
     import torch
 
     tw = build_gpu_twin(w, pop)
     tw.engine = args.engine
     g = tw.grid
-    for s in range(args.warmup):
+    s0 = args.preroll
+    if s0:
+        pos = [torch_xy(w, s, g.dt) for s in range(s0)]
+        tw.step_many([p[0] for p in pos], [p[1] for p in pos], True)
+    for s in range(s0, s0 + args.warmup):
         tw.step(*torch_xy(w, s, g.dt), True)
     torch.cuda.synchronize()
     fused = args.engine == "fused"
@@ -236,7 +243,7 @@ def timed_run(w: dict, args, pop: str):
 
         lib().call("waam_timing_begin", STEP_KERNELS.index(DOMINANT), args.steps)
     t0 = time.perf_counter()
-    for s in range(args.warmup, args.warmup + args.steps):
+    for s in range(s0 + args.warmup, s0 + args.warmup + args.steps):
         tw.step(*torch_xy(w, s, g.dt), True)
     t_host = time.perf_counter() - t0  # launches are asynchronous: the host's own cost per step
     torch.cuda.synchronize()
@@ -259,10 +266,9 @@ def timed_run(w: dict, args, pop: str):
 
 
 def compare_form(w: dict, args, pop: str):
-    """The same workload, warm-up and timed steps with population storage `pop`: (twin, MLUPS,
-    summary). Speed only: the few steps from a cold plate that the timed region covers melt
-    nothing, so the forms' states do not differ yet there; their error against a binary64
-    build is measured over 2000 steps with a molten pool by tests/precision_c3.py."""
+    """The same workload, pre-roll, warm-up and timed steps with population storage `pop`:
+    (twin, MLUPS, summary). Speed only: the forms' errors against a binary64 build are measured
+    over 2000 steps with a molten pool by tests/precision_c3.py."""
     tw, el, _, classes, _, _ = timed_run(w, args, pop)
     g = tw.grid
     mlups = g.n_cells * args.steps / el / 1e6
@@ -468,6 +474,8 @@ def main() -> int:
                          "job (table material) with the fused and the reference-order engine")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--preroll", type=int, default=None,
+                    help="untimed captured steps before the warm-up (default: the workload's)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cpu-one-thread-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
@@ -505,6 +513,8 @@ def main() -> int:
     w = dict(WORKLOADS[args.workload])
     if args.steps is None:
         args.steps = w["steps"]
+    if args.preroll is None:
+        args.preroll = w.get("preroll", 0)
     fused = args.engine == "fused"
     tw, el, t_host, classes, sc, timing = timed_run(w, args, args.pop)
     g = tw.grid
@@ -550,7 +560,7 @@ def main() -> int:
         forms["value_speedup_vs_fp32"] = round(mlups / m32, 3)
     graph = None
     if fused:
-        after = args.warmup + args.steps
+        after = args.preroll + args.warmup + args.steps
         if args.breakdown > 0:
             roofline["kernel_us"] = kernel_breakdown(tw, w, after, args.breakdown)
             after += args.breakdown * len(STEP_KERNELS)
@@ -574,11 +584,14 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"bead-on-plate {g.nx}x{g.ny}x{g.nz} dx=0.2mm, Gaussian2D 4kW "
                                "eta0.8 sigma2mm +x 8mm/s, wire feed 6.5m/min, flow tier, MRT, "
-                               f"{args.pop} populations (f - w_q), fp32 arithmetic",
+                               f"{args.pop} populations (f - w_q), fp32 arithmetic; timed from "
+                               f"the state after {args.preroll} untimed steps of welding",
+                   "preroll_steps": args.preroll,
                    "grid": [g.nx, g.ny, g.nz], "cells": ncell, "collision": "mrt",
                    "engine": args.engine, "population_storage": args.pop,
                    "pool_width_mm": tel["pool_width_mm"], "pool_depth_mm": tel["pool_depth_mm"],
-                   "n_droplets": tel["n_droplets_fired"]},
+                   "peak_temp_K": tel["peak_temp_K"], "n_droplets": tel["n_droplets_fired"],
+                   "surface_cells": round(sc[0]) if sc else None},
     }
     if roofline is not None:
         result["roofline"] = roofline

@@ -42,6 +42,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <map>
 #include <mutex>
 #include <type_traits>
 #include <vector>
@@ -440,8 +441,34 @@ __device__ __forceinline__ float cold_limit(const waam_step_params &p)
     return fminf(p.T_s, fminf(p.T800, p.T1100));
 }
 
-/* blocks of the sweeps over the active-segment list (each grid-strides over it by wavefront) */
+/* blocks of the sweeps over the active-segment list (each grid-strides over it by wavefront)
+ * when the device's residency cannot be queried */
 constexpr int kSegBlocks = 2048;
+
+/* the blocks of `threads` threads of `kernel` the device holds at once (its CUs times the
+ * blocks per CU its registers allow): a list sweep launched with exactly that many gives every
+ * wavefront an equal share of the list from the start, where a larger grid leaves its last
+ * blocks to start only as the first ones finish. Queried once per device and kernel. */
+template <class K>
+static unsigned resident_blocks(K kernel, int threads)
+{
+    static std::mutex mu;
+    static std::map<std::pair<int, const void *>, unsigned> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return kSegBlocks;
+    const auto key = std::make_pair(dev, reinterpret_cast<const void *>(kernel));
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int per_cu = 0, cus = 0;
+    unsigned n = kSegBlocks;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        per_cu > 0 && cus > 0)
+        n = (unsigned)(per_cu * cus);
+    cache.emplace(key, n);
+    return n;
+}
 
 /* 1 — surface height in the torch column (reference kernels.py:625-641 and
  *     coupled_step.py:34-59) and per-block compensated arc weight sums (kernels.py:443-496);
@@ -1571,7 +1598,9 @@ __global__ void __launch_bounds__(256) k_step_pull_pair16(waam_step_fields f, St
  * element comes from the adjacent lane (one 32-bit shuffle per direction with a contiguous-axis
  * component); lanes whose neighbour element lies in another wavefront or across the domain edge
  * load that one element directly. Values are those of the cell-by-cell pull. Needs a contiguous
- * extent divisible by 64 / EB (the host launches the narrower pulls otherwise). */
+ * extent divisible by 64 / EB (the host launches the narrower pulls otherwise). A block's
+ * wavefronts lie side by side along the row (pull_block), so the cache lines holding the edge
+ * elements a wavefront takes from beyond its own words are fetched by the same CU. */
 template <int EB>
 __global__ void __launch_bounds__(256) k_step_pull_x2(waam_step_fields f, StepConsts s)
 {
@@ -1579,9 +1608,9 @@ __global__ void __launch_bounds__(256) k_step_pull_x2(waam_step_fields f, StepCo
     using Elem = typename std::conditional<EB == 32, uint32_t, uint16_t>::type;
     constexpr uint64_t kMask = EB == 32 ? 0xffffffffull : 0xffffull;
     const Dims &d = s.d;
-    const int lane = (int)threadIdx.x;
+    const int lane = (int)(threadIdx.x & 63u);
     const int nw = d.fast() / CPL;
-    const int w = (int)(blockIdx.x * blockDim.x) + lane;
+    const int w = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
     const int b = (int)blockIdx.z;
     if (w >= nw || j >= d.ny) return;
@@ -1670,6 +1699,19 @@ __global__ void __launch_bounds__(256) k_step_pull_x2(waam_step_fields f, StepCo
                 if (need[h]) o[h] = (Elem)((v[q] >> (EB * h)) & kMask);
         }
     }
+}
+
+/* launch shape of k_step_pull_x2 with `cpl` cells per lane: m wavefronts side by side along a
+ * row per block, m the largest divisor of the row's wavefront count up to 4 (rows of 4 / m
+ * per block when m divides 4) */
+static void pull_block(const Dims &d, int cpl, dim3 &grid, dim3 &block)
+{
+    const int waves = (d.fast() / cpl + kTileFast - 1) / kTileFast;
+    int m = 4;
+    while (waves % m != 0) --m;
+    const int rows = 4 % m == 0 ? 4 / m : 1;
+    block = dim3((unsigned)(kTileFast * m), (unsigned)rows, 1);
+    grid = dim3((unsigned)(waves / m), (unsigned)((d.ny + rows - 1) / rows), (unsigned)d.slow());
 }
 
 /* standalone pending stream */
@@ -1935,7 +1977,8 @@ static void launch_step(const StepConsts &s, const waam_step_fields &f, hipStrea
     };
     if (seg) {
         timed_launch(WAAM_K_THERMAL, [&](hipEvent_t a, hipEvent_t b) {
-            launch_k(capture, k_step_thermal_seg, dim3(kSegBlocks), block3(), st, a, b, f, s);
+            launch_k(capture, k_step_thermal_seg, dim3(resident_blocks(k_step_thermal_seg, kBlock)),
+                     block3(), st, a, b, f, s);
         });
         timed_launch(WAAM_K_THERMAL_SURFACE, launch_surface);
     } else if (p.thermal_list_ready == 1) {
@@ -1960,7 +2003,9 @@ static void launch_step(const StepConsts &s, const waam_step_fields &f, hipStrea
         timed_launch(WAAM_K_THERMAL_SURFACE, launch_surface);
     }
     timed_launch(WAAM_K_VOF_FLAGS, [&](hipEvent_t a, hipEvent_t b) {
-        if (seg) launch_k(capture, k_step_vof_flags_seg, dim3(kSegBlocks), block3(), st, a, b, f, s);
+        if (seg)
+            launch_k(capture, k_step_vof_flags_seg,
+                     dim3(resident_blocks(k_step_vof_flags_seg, kBlock)), block3(), st, a, b, f, s);
         else launch_k(capture, k_step_vof_flags, g3, block3(), st, a, b, f, s);
     });
     timed_launch(WAAM_K_COLLIDE, [&](hipEvent_t a, hipEvent_t b) {
@@ -1991,10 +2036,10 @@ static void launch_step(const StepConsts &s, const waam_step_fields &f, hipStrea
     if (separate && p.pending_stream)
         timed_launch(WAAM_K_PULL, [&](hipEvent_t a, hipEvent_t b) {
             if (wide) {
-                const int cpl = f32 ? 2 : 4;
-                dim3 gp((unsigned)((d.fast() / cpl + kTileFast - 1) / kTileFast), g3.y, g3.z);
-                if (f32) launch_k(capture, k_step_pull_x2<32>, gp, block3(), st, a, b, f, s);
-                else launch_k(capture, k_step_pull_x2<16>, gp, block3(), st, a, b, f, s);
+                dim3 gp, bp;
+                pull_block(d, f32 ? 2 : 4, gp, bp);
+                if (f32) launch_k(capture, k_step_pull_x2<32>, gp, bp, st, a, b, f, s);
+                else launch_k(capture, k_step_pull_x2<16>, gp, bp, st, a, b, f, s);
             } else if (pair16) {
                 dim3 gp((unsigned)((d.fast() / 2 + kTileFast - 1) / kTileFast), g3.y, g3.z);
                 launch_k(capture, k_step_pull_pair16, gp, block3(), st, a, b, f, s);
@@ -2007,7 +2052,8 @@ static void launch_step(const StepConsts &s, const waam_step_fields &f, hipStrea
     timed_launch(WAAM_K_STREAM, [&](hipEvent_t a, hipEvent_t b) {
         dim3 gs(g3.x, g3.y, (unsigned)((d.slow() + kStreamSlices - 1) / kStreamSlices));
         if (seg)
-            launch_k(capture, k_step_stream_seg, dim3(kSegBlocks), block3(), st, a, b, f, s);
+            launch_k(capture, k_step_stream_seg, dim3(resident_blocks(k_step_stream_seg, kBlock)),
+                     block3(), st, a, b, f, s);
         else if (separate)
             launch_k(capture, k_step_stream<float, false>, gs, block3(), st, a, b, f, s);
         else if (f32)
