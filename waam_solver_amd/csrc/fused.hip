@@ -1969,11 +1969,13 @@ static void launch_step(const StepConsts &s, const waam_step_fields &f, hipStrea
     const bool tab = p.use_tables != 0;
     auto launch_surface = [&](hipEvent_t a, hipEvent_t b) {
         if (tab)
-            launch_k(capture, k_step_thermal_surface<true>, dim3(kListBlocks), dim3(kBlock), st, a,
-                     b, f, s);
-        else
-            launch_k(capture, k_step_thermal_surface<false>, dim3(kListBlocks), dim3(kBlock), st,
+            launch_k(capture, k_step_thermal_surface<true>,
+                     dim3(resident_blocks(k_step_thermal_surface<true>, kBlock)), dim3(kBlock), st,
                      a, b, f, s);
+        else
+            launch_k(capture, k_step_thermal_surface<false>,
+                     dim3(resident_blocks(k_step_thermal_surface<false>, kBlock)), dim3(kBlock),
+                     st, a, b, f, s);
     };
     if (seg) {
         timed_launch(WAAM_K_THERMAL, [&](hipEvent_t a, hipEvent_t b) {
@@ -2010,14 +2012,17 @@ static void launch_step(const StepConsts &s, const waam_step_fields &f, hipStrea
     });
     timed_launch(WAAM_K_COLLIDE, [&](hipEvent_t a, hipEvent_t b) {
         if (f.pop_dtype == WAAM_POP_F16)
-            launch_k(capture, k_step_collide<WAAM_POP_F16>, dim3(kListBlocks), dim3(kBlock), st, a,
-                     b, f, s);
+            launch_k(capture, k_step_collide<WAAM_POP_F16>,
+                     dim3(resident_blocks(k_step_collide<WAAM_POP_F16>, kBlock)), dim3(kBlock), st,
+                     a, b, f, s);
         else if (f.pop_dtype == WAAM_POP_BF16)
-            launch_k(capture, k_step_collide<WAAM_POP_BF16>, dim3(kListBlocks), dim3(kBlock), st,
+            launch_k(capture, k_step_collide<WAAM_POP_BF16>,
+                     dim3(resident_blocks(k_step_collide<WAAM_POP_BF16>, kBlock)), dim3(kBlock), st,
                      a, b, f, s);
         else
-            launch_k(capture, k_step_collide<WAAM_POP_F32>, dim3(kListBlocks), dim3(kBlock), st, a,
-                     b, f, s);
+            launch_k(capture, k_step_collide<WAAM_POP_F32>,
+                     dim3(resident_blocks(k_step_collide<WAAM_POP_F32>, kBlock)), dim3(kBlock), st,
+                     a, b, f, s);
     });
     if (f.max_tracers > 0)
         timed_launch(WAAM_K_TRACERS, [&](hipEvent_t a, hipEvent_t b) {
