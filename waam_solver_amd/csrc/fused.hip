@@ -82,16 +82,41 @@ constexpr int kGraphMaxSteps = 4096;
 
 static_assert(kBlock == kListSegments, "the list prefix scan uses one thread per segment");
 
-/* A cell list is split into kListSegments contiguous cell ranges of seg_len cells (a multiple
- * of 256); segment g stores its cells at [g*seg_len, g*seg_len + count_g) with its own counter,
- * so appends spread over many addresses instead of contending on one. */
+/* A cell list is split into kListSegments groups with a counter each, so appends spread over
+ * many addresses instead of contending on one. Group g takes the cells of the 64-cell blocks
+ * b = c / 64 with b mod kListSegments = g, interleaved so that cells that sit together (the
+ * surface of a plate fills a few contiguous slices) spread over every group; group g stores
+ * its cells at [start(g), start(g) + count_g), a region exactly as long as the number of grid
+ * cells in the group, so the groups tile the n-entry list. */
+constexpr int kListBlockCells = 64;
 struct CellList {
     int32_t *cells;
     int *counts;  // kListSegments counters, kCounterStride bytes apart
-    int seg_len;
+    size_t n;     // grid cells
     __device__ __forceinline__ int &count(int g) const
     {
         return *reinterpret_cast<int *>((char *)counts + (size_t)g * kCounterStride);
+    }
+    __device__ __forceinline__ static int group(size_t c)
+    {
+        return (int)((c / kListBlockCells) % (size_t)kListSegments);
+    }
+    /* every round of kListSegments blocks gives each group one block; the partial last round
+     * gives the first groups theirs */
+    __device__ __forceinline__ size_t start(int g) const
+    {
+        const size_t round = (size_t)kListBlockCells * kListSegments;
+        const size_t full = n / round, rest = n % round;
+        return full * kListBlockCells * (size_t)g +
+               std::min(rest, (size_t)kListBlockCells * (size_t)g);
+    }
+    __device__ __forceinline__ size_t cap(int g) const
+    {
+        const size_t round = (size_t)kListBlockCells * kListSegments;
+        const size_t full = n / round, rest = n % round;
+        const size_t lo = (size_t)kListBlockCells * (size_t)g;
+        const size_t extra = rest > lo ? std::min(rest - lo, (size_t)kListBlockCells) : 0;
+        return full * kListBlockCells + extra;
     }
 };
 
@@ -116,7 +141,6 @@ struct StepConsts {
     float cap_l, cap_s, h_min, inv_dt;
     ArcStep at;            // this step's arc (a direct launch)
     const ArcStep *track;  // or, in a captured step, its entry of the graph's device track
-    int seg_len;           // cells per list segment
     Dims d;
 };
 
@@ -152,13 +176,13 @@ __device__ __forceinline__ float step_arc_k(const waam_step_params &p, const flo
 }
 
 /* which = 0: thermal list (FLUID/IFACE by the incoming flags), 1: collision list */
-__device__ __forceinline__ CellList cell_list(const waam_step_fields &f, int seg_len, int which)
+__device__ __forceinline__ CellList cell_list(const waam_step_fields &f, size_t n, int which)
 {
     CellList l;
     l.cells = which == 0 ? f.list_thermal : f.list_collide;
     l.counts = reinterpret_cast<int *>((char *)f.scratch + kScratchListCounters +
                                        (size_t)which * kListSegments * kCounterStride);
-    l.seg_len = seg_len;
+    l.n = n;
     return l;
 }
 
@@ -167,9 +191,9 @@ __device__ __forceinline__ bool is_surface_class(uint8_t fl) { return fl == FLUI
 /* Wavefront-aggregated append of cell c for the lanes with want set: one atomic per wavefront
  * and segment touched; positions within a step are unique, their order is not fixed (consumers
  * are order-free). */
-__device__ __forceinline__ void list_append(bool want, size_t c, const CellList &l, size_t ncell)
+__device__ __forceinline__ void list_append(bool want, size_t c, const CellList &l)
 {
-    int seg = want ? (int)(c / (size_t)l.seg_len) : -1;
+    int seg = want ? CellList::group(c) : -1;
     int lane = (int)__lane_id();
     for (;;) {
         unsigned long long m = __ballot(seg >= 0);
@@ -181,21 +205,19 @@ __device__ __forceinline__ void list_append(bool want, size_t c, const CellList 
         if (lane == leader) base = atomicAdd(&l.count(g), __popcll(mm));
         base = __shfl(base, leader);
         if (seg == g) {
-            size_t pos = (size_t)g * l.seg_len + (size_t)(base + __popcll(mm & ((1ull << lane) - 1ull)));
-            if (pos < std::min((size_t)(g + 1) * l.seg_len, ncell)) l.cells[pos] = (int32_t)c;
+            const size_t r = (size_t)(base + __popcll(mm & ((1ull << lane) - 1ull)));
+            if (r < l.cap(g)) l.cells[l.start(g) + r] = (int32_t)c;
             seg = -1;
         }
     }
 }
 
-/* Exclusive prefix of the clamped segment lengths into s_pref[0..kListSegments]; every thread
+/* Exclusive prefix of the clamped group lengths into s_pref[0..kListSegments]; every thread
  * of the block calls it. Returns the list length. */
-__device__ __forceinline__ int list_prefix(const CellList &l, size_t ncell, int *s_pref)
+__device__ __forceinline__ int list_prefix(const CellList &l, int *s_pref)
 {
     int t = threadIdx.y * blockDim.x + threadIdx.x;
-    size_t lo = (size_t)t * l.seg_len;
-    int cap = lo >= ncell ? 0 : (int)std::min((size_t)l.seg_len, ncell - lo);
-    int v = imin(l.count(t), cap);
+    int v = imin(l.count(t), (int)l.cap(t));
     s_pref[t + 1] = v;
     if (t == 0) s_pref[0] = 0;
     __syncthreads();
@@ -217,7 +239,7 @@ __device__ __forceinline__ size_t list_at(const CellList &l, const int *s_pref, 
         if (s_pref[mid] <= t) lo = mid;
         else hi = mid;
     }
-    return (size_t)l.cells[(size_t)lo * l.seg_len + (t - s_pref[lo])];
+    return (size_t)l.cells[l.start(lo) + (size_t)(t - s_pref[lo])];
 }
 
 /* (i, j, k) of cell c; the fused step limits grids to INT_MAX cells, so 32-bit division */
@@ -483,8 +505,8 @@ __global__ void __launch_bounds__(kBlock) k_step_begin(waam_step_fields f, StepC
     const waam_step_params &p = s.p;
     const ArcStep at = arc_step(s);
     if (blockIdx.x == 0) {
-        if (!p.thermal_list_ready) cell_list(f, s.seg_len, 0).count(threadIdx.x) = 0;
-        cell_list(f, s.seg_len, 1).count(threadIdx.x) = 0;
+        if (!p.thermal_list_ready) cell_list(f, s.d.n(), 0).count(threadIdx.x) = 0;
+        cell_list(f, s.d.n(), 1).count(threadIdx.x) = 0;
         if (p.quiet_skip && threadIdx.x == 0) *quiet_rec(f, d).count = 0;
     }
     int best = -1;
@@ -891,7 +913,7 @@ __global__ void __launch_bounds__(256) k_step_thermal(waam_step_fields f, StepCo
     uint8_t flag = in ? f.flags_in[c] : (uint8_t)GAS;
     float h_in = in ? f.H_in[c] : 0.0f;
     float phi_new = (in && s.p.enable_vof) ? vof_advect_cell(f, d, i, j, k, flag) : 0.0f;
-    if (LIST) list_append(in && is_surface_class(flag), c, cell_list(f, s.seg_len, 0), d.n());
+    if (LIST) list_append(in && is_surface_class(flag), c, cell_list(f, s.d.n(), 0));
     if (!in || is_surface_class(flag)) return;
     thermal_cell<TAB>(f, s, i, j, k, c, flag, h_in);
     if (s.p.enable_vof) f.phi_tmp[c] = phi_new;
@@ -940,8 +962,8 @@ __device__ __forceinline__ void thermal_surface_list(const waam_step_fields &f,
 {
     __shared__ int s_pref[kListSegments + 1];
     const Dims &d = s.d;
-    CellList l = cell_list(f, s.seg_len, 0);
-    int n = list_prefix(l, d.n(), s_pref);
+    CellList l = cell_list(f, s.d.n(), 0);
+    int n = list_prefix(l, s_pref);
     int tid = threadIdx.y * blockDim.x + threadIdx.x;
     for (int t = b * kBlock + tid; t < n; t += nb * kBlock) {
         size_t c = list_at(l, s_pref, t);
@@ -970,10 +992,10 @@ __global__ void __launch_bounds__(256) k_step_thermal_all(waam_step_fields f, St
     __shared__ int s_pref[kListSegments + 1];
     const Dims &d = s.d;
     const bool list = (int)blockIdx.z < lz;
-    CellList l = cell_list(f, s.seg_len, 0);
+    CellList l = cell_list(f, s.d.n(), 0);
     int n = 0, t = 0, stride = 0;
     if (list) {
-        n = list_prefix(l, d.n(), s_pref);
+        n = list_prefix(l, s_pref);
         t = (((int)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x) * kBlock +
             threadIdx.y * blockDim.x + threadIdx.x;
         stride = lz * gridDim.y * gridDim.x * kBlock;
@@ -1020,43 +1042,65 @@ __device__ __forceinline__ void time_above_store(const waam_step_fields &f,
     if (t.T >= p.T_s) f.tsol[c] = t.tsol + p.dt;
 }
 
-/* the body of kernel 4 for one cell; returns its new flag. SOLID_FAST: a SOLID cell (whose
- * contact-ruled phi is its own) skips the contact rule's neighbour loads, at the price of a
- * second memory round trip for the other cells */
+/* kernel 4's inputs of one cell: its flag, contact-ruled phi, f_l and (for a cell that was not
+ * GAS) time-above fields */
+struct VofIn {
+    uint8_t f_in;
+    float p, flo;
+    TimeAbove ta;
+};
+
+/* the loads of kernel 4 for one cell (no stores: a caller may issue several cells' loads before
+ * any cell's stores). SOLID_FAST: a SOLID cell (whose contact-ruled phi is its own) skips the
+ * contact rule's neighbour loads, at the price of a second memory round trip for the other
+ * cells */
 template <bool SOLID_FAST>
-__device__ __forceinline__ uint8_t vof_flags_cell(const waam_step_fields &f, const StepConsts &s,
-                                                  int i, int j, int k, size_t c)
+__device__ __forceinline__ VofIn vof_flags_load(const waam_step_fields &f, const StepConsts &s,
+                                                int i, int j, int k, size_t c)
 {
     const Dims &d = s.d;
-    uint8_t nf = GAS;
     /* the flag, the contact rule's operands and f_l are requested together (none depends on
      * another), so a GAS cell waits for one memory round trip; the time-above fields of
      * the cells that were not GAS (nearly all that stay so) follow the flag */
-    uint8_t f_in = f.flags_in[c];
-    float p = 0.0f, flo = 0.0f;
+    VofIn v{};
+    v.f_in = f.flags_in[c];
     if (s.p.enable_vof) {
-        p = (s.p.enable_wetting && !(SOLID_FAST && f_in == SOLID))
-                ? contact_bc_phi(f.phi_tmp, f.flags_in, i, j, k, s.p.cos_t, s.p.sin_t, d)
-                : f.phi_tmp[c];
-        flo = f.fl_out[c];
+        v.p = (s.p.enable_wetting && !(SOLID_FAST && v.f_in == SOLID))
+                  ? contact_bc_phi(f.phi_tmp, f.flags_in, i, j, k, s.p.cos_t, s.p.sin_t, d)
+                  : f.phi_tmp[c];
+        v.flo = f.fl_out[c];
     }
-    bool pre = f_in != GAS;
-    TimeAbove ta{};
-    if (pre) ta = time_above_load(f, c);
+    if (v.f_in != GAS) v.ta = time_above_load(f, c);
+    return v;
+}
+
+/* the rest of kernel 4 for one cell on its loaded inputs; returns its new flag */
+__device__ __forceinline__ uint8_t vof_flags_apply(const waam_step_fields &f, const StepConsts &s,
+                                                   int k, size_t c, const VofIn &v)
+{
+    uint8_t nf = GAS;
     if (!s.p.enable_vof) {
-        nf = f_in;
+        nf = v.f_in;
     } else {
-        nf = classify_cell(k, s.p.nz_solid, p, flo);
-        f.phi[c] = p;
-        if (nf == GAS || p < 0.05f) f.fl_out[c] = 0.0f;
+        nf = classify_cell(k, s.p.nz_solid, v.p, v.flo);
+        f.phi[c] = v.p;
+        if (nf == GAS || v.p < 0.05f) f.fl_out[c] = 0.0f;
         else if (nf == SOLID) f.fl_out[c] = 0.0f;
     }
     f.flags_out[c] = nf;
     if (nf != GAS) {
-        if (!pre) ta = time_above_load(f, c);
+        const TimeAbove ta = v.f_in != GAS ? v.ta : time_above_load(f, c);
         time_above_store(f, s.p, c, ta);
     }
     return nf;
+}
+
+/* the body of kernel 4 for one cell; returns its new flag */
+template <bool SOLID_FAST>
+__device__ __forceinline__ uint8_t vof_flags_cell(const waam_step_fields &f, const StepConsts &s,
+                                                  int i, int j, int k, size_t c)
+{
+    return vof_flags_apply(f, s, k, c, vof_flags_load<SOLID_FAST>(f, s, i, j, k, c));
 }
 
 /* 4 — apply_contact_angle_phi_bc (Jacobi), update_flags_from_phi, sync_phi_liquid_fraction,
@@ -1066,12 +1110,12 @@ __global__ void __launch_bounds__(256) k_step_vof_flags(waam_step_fields f, Step
 {
     const Dims &d = s.d;
     if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0)  // kernel 2 has consumed the
-        cell_list(f, s.seg_len, 0).count(threadIdx.y * blockDim.x + threadIdx.x) = 0;  // list
+        cell_list(f, s.d.n(), 0).count(threadIdx.y * blockDim.x + threadIdx.x) = 0;  // list
     int i, j, k;
     bool in = d.cell(i, j, k);
     size_t c = in ? d.idx(i, j, k) : 0;
     uint8_t nf = in ? vof_flags_cell<false>(f, s, i, j, k, c) : (uint8_t)GAS;
-    list_append(in && is_surface_class(nf), c, cell_list(f, s.seg_len, 1), d.n());
+    list_append(in && is_surface_class(nf), c, cell_list(f, s.d.n(), 1));
 }
 
 /* 4 over the segments kernel 2 left in */
@@ -1079,14 +1123,14 @@ __global__ void __launch_bounds__(256) k_step_vof_flags_seg(waam_step_fields f, 
 {
     const Dims &d = s.d;
     if (blockIdx.x == 0)  // kernel 2 has consumed the list
-        cell_list(f, s.seg_len, 0).count(threadIdx.y * blockDim.x + threadIdx.x) = 0;
+        cell_list(f, d.n(), 0).count(threadIdx.y * blockDim.x + threadIdx.x) = 0;
     const QuietRec q = quiet_rec(f, d);
     for_needed_segments(q, *q.count, [&](size_t seg) {
         int i, j, k;
         const bool in = seg_cell(q, seg, d, i, j, k);
         const size_t c = in ? d.idx(i, j, k) : 0;
         const uint8_t nf = in ? vof_flags_cell<true>(f, s, i, j, k, c) : (uint8_t)GAS;
-        list_append(in && is_surface_class(nf), c, cell_list(f, s.seg_len, 1), d.n());
+        list_append(in && is_surface_class(nf), c, cell_list(f, d.n(), 1));
     });
 }
 
@@ -1254,8 +1298,8 @@ __global__ void __launch_bounds__(256) k_step_collide(waam_step_fields f, StepCo
 {
     __shared__ int s_pref[kListSegments + 1];
     const Dims &d = s.d;
-    CellList l = cell_list(f, s.seg_len, 1);
-    int n = list_prefix(l, d.n(), s_pref);
+    CellList l = cell_list(f, s.d.n(), 1);
+    int n = list_prefix(l, s_pref);
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
         size_t c = list_at(l, s_pref, t);
         int i, j, k;
@@ -1363,7 +1407,7 @@ __device__ __forceinline__ bool stream_cell(const waam_step_fields &f, const Ste
         const float *tot = reinterpret_cast<const float *>((char *)f.scratch + kScratchTelemetry);
         fin = freeze_cell(f, s, i, j, k, c, fc, T, phi, fl, step_arc_k(p, tot));
     }
-    list_append(is_surface_class(fin), c, cell_list(f, s.seg_len, 0), d.n());
+    list_append(is_surface_class(fin), c, cell_list(f, s.d.n(), 0));
     return fc == GAS && phi == 0.0f;  // the freeze rules never leave a cell GAS
 }
 
@@ -1391,7 +1435,7 @@ __device__ __forceinline__ bool stream_rest_cell(const waam_step_fields &f, cons
         const float *tot = reinterpret_cast<const float *>((char *)f.scratch + kScratchTelemetry);
         fin = freeze_cell(f, s, i, j, k, c, fc, T, phi, fl, step_arc_k(p, tot));
     }
-    list_append(is_surface_class(fin), c, cell_list(f, s.seg_len, 0), s.d.n());
+    list_append(is_surface_class(fin), c, cell_list(f, s.d.n(), 0));
     return fc == GAS && phi == 0.0f;  // the freeze rules never leave a cell GAS
 }
 
@@ -1923,8 +1967,6 @@ static StepConsts make_step_consts(const waam_step_params &p, const Dims &d)
     arc_radii(s, ri, rj);
     arc_box(ri, rj, p.arc_i, p.arc_j, s.at);
     s.track = nullptr;
-    size_t seg = (d.n() + kListSegments - 1) / kListSegments;
-    s.seg_len = (int)((seg + 255) / 256 * 256);
     return s;
 }
 
