@@ -360,11 +360,10 @@ __device__ __forceinline__ bool segment_active(const QuietRec &q, int bx, int j,
     return !((own == 3u) && (all & 1u));
 }
 
-/* 1b — the active segments of a steady step, one thread per segment, appended to the list
- * (wavefront-aggregated; the order is free: each segment's results depend only on the
- * sweeps' inputs). Appending a block's segments in order with one atomic per block built the
- * list faster (23 against 35 us at 768x384x192) but slowed the sweeps over it by 50 us: the
- * wavefront-sized runs in arrival order spread the sweeps' concurrent accesses wider. */
+/* 1b — the active segments of a steady step, one thread per segment, appended to the list a
+ * block at a time (one atomic per block; the block's segments in index order, blocks in
+ * arrival order: each segment's results depend only on the sweeps' inputs, so the order is
+ * free). One atomic per wavefront, on the one counter, took 36 against 20 us at 768x384x192. */
 __global__ void __launch_bounds__(kBlock) k_step_quiet_list(waam_step_fields f, StepConsts s)
 {
     const Dims &d = s.d;
@@ -377,13 +376,21 @@ __global__ void __launch_bounds__(kBlock) k_step_quiet_list(waam_step_fields f, 
         const size_t r = sg / (size_t)q.nbx;
         act = segment_active(q, bx, (int)(r % (size_t)d.ny), (int)(r / (size_t)d.ny), d);
     }
+    __shared__ int s_cnt[kBlock / 64];
+    __shared__ int s_base;
     const unsigned long long m = __ballot(act);
-    if (m == 0ull) return;
     const int lane = (int)__lane_id();
-    const int leader = __builtin_ctzll(m);
-    int base = 0;
-    if (lane == leader) base = atomicAdd(q.count, __popcll(m));
-    base = __shfl(base, leader);
+    const int wave = (int)threadIdx.x >> 6;
+    if (lane == 0) s_cnt[wave] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int w = 0; w < kBlock / 64; ++w) tot += s_cnt[w];
+        s_base = tot ? atomicAdd(q.count, tot) : 0;
+    }
+    __syncthreads();
+    int base = s_base;
+    for (int w = 0; w < wave; ++w) base += s_cnt[w];
     if (act) q.list[base + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)sg;
 }
 
@@ -715,9 +722,15 @@ struct HView {
     int box_i0, box_i1, box_j0, box_j1;
     bool arc_on, cap_on, tab;
 
+    /* whether the arc heats cell column (i, j) this step (its weights are exactly 0 outside
+     * the box) */
+    __device__ __forceinline__ bool heated(int i, int j) const
+    {
+        return arc_on && i >= box_i0 && i <= box_i1 && j >= box_j0 && j <= box_j1;
+    }
     __device__ __forceinline__ float value(float h, float fl, float cp, int i, int j, int k) const
     {
-        if (arc_on && i >= box_i0 && i <= box_i1 && j >= box_j0 && j <= box_j1) {
+        if (heated(i, j)) {
             float wg, wdep;
             if (arc_cell_weights(ap, i, j, k, fl, wg, wdep)) h += epw * wg * wdep;
         }
@@ -727,9 +740,10 @@ struct HView {
         }
         return h;
     }
+    /* f_l is read only where the arc heats */
     __device__ __forceinline__ float at(size_t c, int i, int j, int k) const
     {
-        return value(H_in[c], fl_in[c], tab ? cp_prev[c] : 0.0f, i, j, k);
+        return value(H_in[c], heated(i, j) ? fl_in[c] : 0.0f, tab ? cp_prev[c] : 0.0f, i, j, k);
     }
     /* _masked_at on the enthalpy view: clamped index, GAS neighbours give the fallback (loads
      * first, select last, so the loads are not sunk behind the flag test) */
@@ -740,7 +754,8 @@ struct HView {
         int kk = imax(0, imin(k, d.nz - 1));
         size_t c = d.idx(ii, jj, kk);
         uint8_t fg = flags_in[c];
-        float v = value(H_in[c], fl_in[c], tab ? cp_prev[c] : 0.0f, ii, jj, kk);
+        float v = value(H_in[c], heated(ii, jj) ? fl_in[c] : 0.0f, tab ? cp_prev[c] : 0.0f, ii,
+                        jj, kk);
         return fg == GAS ? fallback : v;
     }
 };
@@ -794,11 +809,13 @@ __device__ __forceinline__ void store_props(const waam_step_fields &f, size_t c,
  * place of the constants, the _variable forms of conduction, floor, ceiling and phase map
  * (kernels.py:1775-1863, 1937-1950, 1987-2010), and the properties stored for the rest of the
  * step and the next step's first ceiling.
- * ADV false: the caller passes SOLID and GAS cells only (no enthalpy advection is compiled) */
+ * ADV false: the caller passes SOLID and GAS cells only (no enthalpy advection is compiled).
+ * fl_zero_held: the cell's f_l output buffer holds 0 already (a SOLID cell of a segment that
+ * ended the last two steps cold), so a new f_l of 0 is not stored again */
 template <bool TAB, bool ADV = true>
 __device__ __forceinline__ float2 thermal_cell(const waam_step_fields &f, const StepConsts &s,
                                              int i, int j, int k, size_t c, uint8_t flag,
-                                             float h_gas)
+                                             float h_gas, bool fl_zero_held = false)
 {
     const Dims &d = s.d;
     const waam_step_params &p = s.p;
@@ -872,7 +889,7 @@ __device__ __forceinline__ float2 thermal_cell(const waam_step_fields &f, const 
     float T_max = f.T_max[c], T_prev = f.T_prev[c];
     f.H_out[c] = h;
     f.T_out[c] = T;
-    f.fl_out[c] = fl;
+    if (!(fl_zero_held && fl == 0.0f)) f.fl_out[c] = fl;
     if (T > T_max) f.T_max[c] = T;
     f.dT_dt[c] = (T - T_prev) * s.inv_dt;
     f.T_prev[c] = T;
@@ -938,15 +955,19 @@ __global__ void __launch_bounds__(256) k_step_thermal_seg(waam_step_fields f, St
          * the surface cells: only GAS cells advect phi here */
         float phi_new = 1.0f;
         if (in && s.p.enable_vof && flag == GAS) phi_new = vof_advect_cell(f, d, i, j, k, flag);
+        /* cold at the end of the last two steps: every cell is SOLID, its phi_tmp holds 1 (this
+         * sweep stored it at the step before) and both f_l buffers hold 0 */
+        const bool hist = (bits & 12u) == 12u;
         bool cold = !in;
         if (in && !is_surface_class(flag)) {
             /* flag is SOLID or GAS: no enthalpy advection */
-            const float2 tf = thermal_cell<false, false>(f, s, i, j, k, c, flag, h_in);
-            if (s.p.enable_vof) f.phi_tmp[c] = phi_new;
+            const bool held = hist && flag == SOLID;
+            const float2 tf = thermal_cell<false, false>(f, s, i, j, k, c, flag, h_in, held);
+            if (s.p.enable_vof && !held) f.phi_tmp[c] = phi_new;
             cold = flag == SOLID && tf.x < T_cold && tf.y == 0.0f;
         }
-        /* cold at the end of the last two steps and still cold on the new T and f_l */
-        const bool skip = (bits & 12u) == 12u && __ballot(!cold) == 0ull;
+        /* still cold on the new T and f_l */
+        const bool skip = hist && __ballot(!cold) == 0ull;
         if (threadIdx.x == 0) {
             q.need[pos] = skip ? 0 : 1;
             if (skip) q.bits[seg] = 12u;
