@@ -410,8 +410,9 @@ __device__ __forceinline__ bool seg_cell(const QuietRec &q, size_t seg, const Di
 }
 
 /* a wavefront's walk over the active-segment list (entries w, w + nw, ... of a grid of nw
- * wavefronts): the entries are loaded 64 at a time, one per lane, and handed out by
- * shuffles, so the list load drops out of each segment's chain of memory round trips */
+ * wavefronts): the entries and their segments' record bytes are loaded 64 at a time, one per
+ * lane, and handed out by shuffles, so those loads drop out of each segment's chain of memory
+ * round trips. body(segment, list position, record byte). */
 template <class Body>
 __device__ __forceinline__ void for_listed_segments(const QuietRec &q, int n, Body &&body)
 {
@@ -421,9 +422,11 @@ __device__ __forceinline__ void for_listed_segments(const QuietRec &q, int n, Bo
     for (long long t0 = w; t0 < n; t0 += 64LL * nw) {
         const long long tl = t0 + (long long)lane * nw;
         const int mine = tl < n ? q.list[tl] : 0;
+        const int mbits = tl < n ? (int)q.bits[mine] : 0;
         const int cnt = (int)std::min<long long>(64, (n - t0 + nw - 1) / nw);
         for (int it = 0; it < cnt; ++it)
-            body((size_t)(unsigned)__shfl(mine, it), t0 + (long long)it * nw);
+            body((size_t)(unsigned)__shfl(mine, it), t0 + (long long)it * nw,
+                 (unsigned)__shfl(mbits, it));
     }
 }
 
@@ -943,28 +946,35 @@ __global__ void __launch_bounds__(256) k_step_thermal_seg(waam_step_fields f, St
     const Dims &d = s.d;
     const QuietRec q = quiet_rec(f, d);
     const float T_cold = cold_limit(s.p);
-    for_listed_segments(q, *q.count, [&](size_t seg, long long pos) {
+    for_listed_segments(q, *q.count, [&](size_t seg, long long pos, unsigned bits) {
         int i, j, k;
         const bool in = seg_cell(q, seg, d, i, j, k);
         const size_t c = in ? d.idx(i, j, k) : 0;
-        const uint8_t bits = q.bits[seg];
-        const uint8_t flag = in ? f.flags_in[c] : (uint8_t)GAS;
-        const float h_in = in ? f.H_in[c] : 0.0f;
-        /* a SOLID cell's advected phi is re-initialised to 1 whatever its neighbours hold (the
-         * plate's segments, most of a steady step's list, skip those loads), and kernel 3 owns
-         * the surface cells: only GAS cells advect phi here */
-        float phi_new = 1.0f;
-        if (in && s.p.enable_vof && flag == GAS) phi_new = vof_advect_cell(f, d, i, j, k, flag);
-        /* cold at the end of the last two steps: every cell is SOLID, its phi_tmp holds 1 (this
-         * sweep stored it at the step before) and both f_l buffers hold 0 */
+        /* cold at the end of the last two steps: every cell is SOLID (so no flag load and no
+         * phi advection), its phi_tmp holds 1 (this sweep stored it at the step before) and
+         * both f_l buffers hold 0 */
         const bool hist = (bits & 12u) == 12u;
         bool cold = !in;
-        if (in && !is_surface_class(flag)) {
-            /* flag is SOLID or GAS: no enthalpy advection */
-            const bool held = hist && flag == SOLID;
-            const float2 tf = thermal_cell<false, false>(f, s, i, j, k, c, flag, h_in, held);
-            if (s.p.enable_vof && !held) f.phi_tmp[c] = phi_new;
-            cold = flag == SOLID && tf.x < T_cold && tf.y == 0.0f;
+        if (hist) {
+            if (in) {
+                const float2 tf = thermal_cell<false, false>(f, s, i, j, k, c, SOLID, 0.0f, true);
+                cold = tf.x < T_cold && tf.y == 0.0f;
+            }
+        } else {
+            const uint8_t flag = in ? f.flags_in[c] : (uint8_t)GAS;
+            const float h_in = in ? f.H_in[c] : 0.0f;
+            /* a SOLID cell's advected phi is re-initialised to 1 whatever its neighbours hold
+             * (the plate's segments skip those loads), and kernel 3 owns the surface cells:
+             * only GAS cells advect phi here */
+            float phi_new = 1.0f;
+            if (in && s.p.enable_vof && flag == GAS)
+                phi_new = vof_advect_cell(f, d, i, j, k, flag);
+            if (in && !is_surface_class(flag)) {
+                /* flag is SOLID or GAS: no enthalpy advection */
+                const float2 tf = thermal_cell<false, false>(f, s, i, j, k, c, flag, h_in);
+                if (s.p.enable_vof) f.phi_tmp[c] = phi_new;
+                cold = flag == SOLID && tf.x < T_cold && tf.y == 0.0f;
+            }
         }
         /* still cold on the new T and f_l */
         const bool skip = hist && __ballot(!cold) == 0ull;
