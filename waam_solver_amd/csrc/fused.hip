@@ -1675,8 +1675,11 @@ __global__ void __launch_bounds__(256) k_step_pull_pair16(waam_step_fields f, St
  * load that one element directly. Values are those of the cell-by-cell pull. Needs a contiguous
  * extent divisible by 64 / EB (the host launches the narrower pulls otherwise). A block's
  * wavefronts lie side by side along the row (pull_block), so the cache lines holding the edge
- * elements a wavefront takes from beyond its own words are fetched by the same CU. */
-template <int EB>
+ * elements a wavefront takes from beyond its own words are fetched by the same CU. Directions
+ * QLO..QHI-1 only: the pull runs as two launches of 10 and 9 directions, each with half the
+ * concurrent row streams (768x384x192: fp16 719 -> 710 us, fp32 1418 -> 1383 us; three or
+ * four launches were slower). */
+template <int EB, int QLO, int QHI>
 __global__ void __launch_bounds__(256) k_step_pull_x2(waam_step_fields f, StepConsts s)
 {
     constexpr int CPL = 64 / EB;  // cells per lane
@@ -1723,10 +1726,10 @@ __global__ void __launch_bounds__(256) k_step_pull_x2(waam_step_fields f, StepCo
     /* every load first, then the shuffles: all 19 words in flight at once */
     uint64_t v[Q];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) v[q] = reinterpret_cast<const uint64_t *>(row_of(q))[w];
+    for (int q = QLO; q < QHI; ++q) v[q] = reinterpret_cast<const uint64_t *>(row_of(q))[w];
     const bool first = lane == 0 || w == 0, last = lane == 63 || w == nw - 1;
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
+    for (int q = QLO; q < QHI; ++q) {
         const int sh = zyx ? ex(q) : ez(q);
         if (sh == 0) continue;
         const uint64_t aw = v[q];
@@ -1752,7 +1755,7 @@ __global__ void __launch_bounds__(256) k_step_pull_x2(waam_step_fields f, StepCo
         const int a = a0 + h;
         const int i = zyx ? a : b, k = zyx ? b : a;
         const size_t c = c0 + (size_t)h;
-        for (int q = 0; q < Q; ++q) {
+        for (int q = QLO; q < QHI; ++q) {
             int ni = i - ex(q), nj = j - ey(q), nk = k - ez(q);
             ni = ni < 0 ? ni + d.nx : (ni >= d.nx ? ni - d.nx : ni);
             nj = imax(0, imin(nj, d.ny - 1));
@@ -1764,7 +1767,7 @@ __global__ void __launch_bounds__(256) k_step_pull_x2(waam_step_fields f, StepCo
         }
     }
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
+    for (int q = QLO; q < QHI; ++q) {
         Elem *o = dst + (size_t)q * N + c0;
         if (all) {
             /* a non-temporal store: the next step reads these words, nothing in this one
@@ -2118,8 +2121,13 @@ static void launch_step(const StepConsts &s, const waam_step_fields &f, hipStrea
             if (wide) {
                 dim3 gp, bp;
                 pull_block(d, f32 ? 2 : 4, gp, bp);
-                if (f32) launch_k(capture, k_step_pull_x2<32>, gp, bp, st, a, b, f, s);
-                else launch_k(capture, k_step_pull_x2<16>, gp, bp, st, a, b, f, s);
+                if (f32) {
+                    launch_k(capture, k_step_pull_x2<32, 0, 10>, gp, bp, st, a, nullptr, f, s);
+                    launch_k(capture, k_step_pull_x2<32, 10, Q>, gp, bp, st, nullptr, b, f, s);
+                } else {
+                    launch_k(capture, k_step_pull_x2<16, 0, 10>, gp, bp, st, a, nullptr, f, s);
+                    launch_k(capture, k_step_pull_x2<16, 10, Q>, gp, bp, st, nullptr, b, f, s);
+                }
             } else if (pair16) {
                 dim3 gp((unsigned)((d.fast() / 2 + kTileFast - 1) / kTileFast), g3.y, g3.z);
                 launch_k(capture, k_step_pull_pair16, gp, block3(), st, a, b, f, s);
