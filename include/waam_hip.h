@@ -530,8 +530,9 @@ typedef struct waam_step_fields {
     /* waam_quiet_bytes(nx, ny, nz, layout) bytes: per row segment (the 64 cells of one
      * wavefront along the contiguous axis) whether it ended the last two steps quiet (all GAS,
      * phi 0) or cold (SOLID below the solidus and the HAZ thresholds), the list of the segments
-     * a steady step sweeps and which of them the flag-update and stream sweeps visit; written
-     * by every step, contents need no initial value (the size is queried, not a layout) */
+     * a steady step sweeps and which of them the flag-update and stream sweeps visit, and the
+     * pull's per-lane cell masks; written by every step, contents need no initial value (the
+     * size is queried, not a layout) */
     uint8_t *quiet;
     /* temperature-dependent material (params.use_tables; NULL otherwise): cp_rho_prev holds
      * the previous step's refresh_thermal_properties cp*rho (kernels.py:1710-1772), read by the

@@ -137,8 +137,8 @@ FIELDS = ("H", "T", "f_l", "phi", "flags", "rho", "ux", "uy", "uz", "T_max", "T_
           "Fx", "Fy", "Fz", "time_above_800_s", "time_above_1100_s", "time_above_solidus_s")
 
 
-def cold_skipped_segments(tw) -> tuple[int, int]:
-    """(listed, left out as cold plate) row segments of the fused engine's last steady step.
+def segment_record(tw) -> tuple[int, int, int]:
+    """(row segments, listed, left out as cold plate) of the fused engine's last steady step.
 
     Reads the step's segment record (``waam_step_fields.quiet``): per row segment of 64 cells
     along the contiguous axis a byte of history, then the listed-segment count on a 256-byte
@@ -151,7 +151,13 @@ def cold_skipped_segments(tw) -> tuple[int, int]:
     off = -(-nseg // 256) * 256
     count = int(rec[off:off + 4].view(np.int32)[0])
     need = rec[off + 256 + 4 * nseg: off + 256 + 4 * nseg + count]
-    return count, int((need == 0).sum())
+    return nseg, count, int((need == 0).sum())
+
+
+def cold_skipped_segments(tw) -> tuple[int, int]:
+    """(listed, left out as cold plate) row segments of the fused engine's last steady step."""
+    _, listed, cold = segment_record(tw)
+    return listed, cold
 
 
 def compare(tw, ot, fields=FIELDS) -> dict:

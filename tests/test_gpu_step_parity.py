@@ -11,7 +11,8 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from tests.step_cases import FIELDS, build_pair, cold_skipped_segments, compare, run_pair
+from tests.step_cases import (FIELDS, build_pair, cold_skipped_segments, compare, run_pair,
+                              segment_record)
 
 TABLE_FIELDS = ("cp_rho_field", "alpha_lu_field", "dgamma_lu_field", "tau_field")
 
@@ -173,9 +174,10 @@ def test_moving_window_parity(hip_device, engine, layout):
 @pytest.mark.parametrize("case", ["bench_small", "droplets_srt"])
 def test_quiet_segments_skip_with_identical_results(hip_device, case):
     """Two fused twins from the same state, one skipping the sweeps on quiet row segments (all
-    GAS with phi 0 around them for two steps) and one sweeping every cell: every field and both
-    population buffers stay identical over 30 steps (with droplets, whose deposition writes
-    cells between steps), and the skipping twin did skip segments."""
+    GAS with phi 0 around them for two steps) and the flag-update and stream sweeps on cold
+    plate segments, one sweeping every cell: every field and both population buffers stay
+    identical over 30 steps (with droplets, whose deposition writes cells between steps), and
+    the skipping twin did skip segments of both kinds."""
     import torch
 
     from waam_solver_amd.field import tensor_of
@@ -186,23 +188,26 @@ def test_quiet_segments_skip_with_identical_results(hip_device, case):
     ref._fused = FusedStepper(ref)
     ref._fused.quiet_skip = False
     g = tw.grid
-    skipped = []
+    skipped, cold = [], []
     for s in range(30):
         x = (g.nx // 3) * g.dx + 8e-3 * s * g.dt
         y = (g.ny // 2) * g.dx
         tw.step(x, y, True)
         ref.step(x, y, True)
-        q = tensor_of(g.quiet)
-        nseg = q.numel() // 2
-        skipped.append(int(q[nseg:].sum().item()) if tw._fused.steady() else 0)
+        steady = tw._fused.steady() and s > 0
+        nseg, listed, nc = segment_record(tw)
+        skipped.append(nseg - listed if steady else 0)
+        cold.append(nc if steady else 0)
     torch.cuda.synchronize()
     tw.sync()
     ref.sync()
     diff = [n for n in FIELDS + ("f_src", "f_dst")
             if getattr(tw.grid, n).to_numpy().tobytes() != getattr(ref.grid, n).to_numpy().tobytes()]
-    print(f"[quiet] {case}: segments skipped per step {skipped}")
+    print(f"[quiet] {case}: quiet segments skipped per step {skipped}, cold {cold}")
     assert not diff, f"{case}: quiet skipping changed {diff}"
     assert max(skipped) > 0
+    if g.layout == 1:
+        assert max(cold) > 0
 
 
 @pytest.mark.parametrize("pop,nx", [("fp32", 64), ("fp32", 63), ("fp16", 64), ("fp16", 62),

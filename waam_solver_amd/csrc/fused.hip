@@ -302,9 +302,15 @@ __host__ __device__ inline size_t quiet_need_offset(const Dims &d)
 {
     return quiet_count_offset(d) + 256 + 4 * quiet_segments(d);
 }
+/* then, 256-aligned, the pull's per-lane masks: one byte per two cells (the second launch of
+ * the pull reads its lanes' need and GAS bits there instead of two flag bytes per cell) */
+__host__ __device__ inline size_t quiet_pull_offset(const Dims &d)
+{
+    return (quiet_need_offset(d) + quiet_segments(d) + 255) / 256 * 256;
+}
 __host__ __device__ inline size_t quiet_bytes(const Dims &d)
 {
-    return quiet_need_offset(d) + quiet_segments(d);
+    return quiet_pull_offset(d) + (d.n() + 1) / 2;
 }
 
 __device__ __forceinline__ QuietRec quiet_rec(const waam_step_fields &f, const Dims &d)
@@ -1696,19 +1702,38 @@ __global__ void __launch_bounds__(256) k_step_pull_x2(waam_step_fields f, StepCo
     const int a0 = CPL * w;
     const size_t N = d.n();
     const size_t c0 = zyx ? d.idx(a0, j, b) : d.idx(b, j, a0);
-    /* the CPL cells' flags in one load each */
-    const uint32_t fcw = CPL == 4 ? *reinterpret_cast<const uint32_t *>(f.flags_out + c0)
-                                  : *reinterpret_cast<const uint16_t *>(f.flags_out + c0);
-    const uint32_t fpw = CPL == 4 ? *reinterpret_cast<const uint32_t *>(f.sflags_prev + c0)
-                                  : *reinterpret_cast<const uint16_t *>(f.sflags_prev + c0);
-    uint8_t fph[CPL];
-    bool need[CPL];
+    /* per cell: whether it pulls, and whether it was GAS at the step being completed (no
+     * bounce-back then). The first launch reads the CPL cells' flags (one load each) and leaves
+     * these bits in the lane's mask byte, which the second launch reads instead */
+    uint8_t *mask = f.quiet + quiet_pull_offset(d) + c0 / CPL;
+    bool need[CPL], gas[CPL];
+    if (QLO == 0) {
+        const uint32_t fcw = CPL == 4 ? *reinterpret_cast<const uint32_t *>(f.flags_out + c0)
+                                      : *reinterpret_cast<const uint16_t *>(f.flags_out + c0);
+        const uint32_t fpw = CPL == 4 ? *reinterpret_cast<const uint32_t *>(f.sflags_prev + c0)
+                                      : *reinterpret_cast<const uint16_t *>(f.sflags_prev + c0);
+        unsigned mb = 0;
+#pragma unroll
+        for (int h = 0; h < CPL; ++h) {
+            const uint8_t fch = (uint8_t)((fcw >> (8 * h)) & 0xffu);
+            const uint8_t fph = (uint8_t)((fpw >> (8 * h)) & 0xffu);
+            need[h] = !is_surface_class(fch) && fph != SOLID;
+            gas[h] = fph == GAS;
+            mb |= (need[h] ? 1u : 0u) << h;
+            mb |= (gas[h] ? 1u : 0u) << (4 + h);
+        }
+        *mask = (uint8_t)mb;
+    } else {
+        const unsigned mb = *mask;
+#pragma unroll
+        for (int h = 0; h < CPL; ++h) {
+            need[h] = ((mb >> h) & 1u) != 0u;
+            gas[h] = ((mb >> (4 + h)) & 1u) != 0u;
+        }
+    }
     bool any = false, all = true;
 #pragma unroll
     for (int h = 0; h < CPL; ++h) {
-        const uint8_t fch = (uint8_t)((fcw >> (8 * h)) & 0xffu);
-        fph[h] = (uint8_t)((fpw >> (8 * h)) & 0xffu);
-        need[h] = !is_surface_class(fch) && fph[h] != SOLID;
         any = any || need[h];
         all = all && need[h];
     }
@@ -1751,7 +1776,7 @@ __global__ void __launch_bounds__(256) k_step_pull_x2(waam_step_fields f, StepCo
      * stream this is (its flag then, as every other flag of that stream) */
 #pragma unroll
     for (int h = 0; h < CPL; ++h) {
-        if (!need[h] || fph[h] == GAS) continue;
+        if (!need[h] || gas[h]) continue;
         const int a = a0 + h;
         const int i = zyx ? a : b, k = zyx ? b : a;
         const size_t c = c0 + (size_t)h;
