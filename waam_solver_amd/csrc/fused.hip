@@ -810,6 +810,55 @@ __device__ __forceinline__ void store_props(const waam_step_fields &f, size_t c,
     f.tau[c] = pr.tau;
 }
 
+/* the conduction increment of advect_diffuse_temperature (Jacobi: every neighbour value is
+ * pre-kernel, GAS neighbours already replaced by the cell's own T) */
+__device__ __forceinline__ float conduction_dH(const CellProps &pr, float T_c, float T_im,
+                                               float T_ip, float T_jm, float T_jp, float T_km,
+                                               float T_kp)
+{
+    float lap = T_ip + T_im + T_jp + T_jm + T_kp + T_km - 6.0f * T_c;
+    return pr.cp_rho * pr.alpha_lu * lap * 1.0f;
+}
+
+/* the rest of a non-GAS cell's chain on its loaded operands: losses through its n_exp exposed
+ * faces, floor, ceiling, phase map, HAZ diagnostics (reference kernels.py:3027-3032, 2023-2027;
+ * they run before the VOF flag update, so they see this step's incoming flags) and the stores;
+ * returns the new T and f_l. fl_zero_held: the cell's f_l output buffer holds 0 already, so a
+ * new f_l of 0 is not stored again */
+template <bool TAB>
+__device__ __forceinline__ float2 thermal_finish(const waam_step_fields &f, const StepConsts &s,
+                                                 size_t c, float T_c, float H_c, float dH,
+                                                 int n_exp, const CellProps &pr, float T_max,
+                                                 float T_prev, bool fl_zero_held)
+{
+    const waam_step_params &p = s.p;
+    float h = H_c + dH;
+    if (p.enable_heat_loss) {
+        if (n_exp != 0) h -= loss_flux(T_c, s.lc) * (float)n_exp * s.lc.dt / s.lc.dx;
+        const float h_min = TAB ? pr.cp_rho * p.T_amb : s.h_min;
+        if (h < h_min) h = h_min;
+    }
+    if (p.enable_cap) {
+        if (TAB)
+            h = ceiling_clamp(h, s.pc.H_liq, s.pc.H_liq + pr.cp_rho * (p.T_cap - p.T_l),
+                              pr.cp_rho * p.T_cap);
+        else
+            h = ceiling_clamp(h, s.pc.H_liq, s.cap_l, s.cap_s);
+    }
+    float T, fl;
+    PhaseConsts pc = s.pc;
+    if (TAB) pc.cp_rho = pr.cp_rho + 1e-9f;
+    phase_of(h, pc, T, fl);
+    f.H_out[c] = h;
+    f.T_out[c] = T;
+    if (!(fl_zero_held && fl == 0.0f)) f.fl_out[c] = fl;
+    if (T > T_max) f.T_max[c] = T;
+    f.dT_dt[c] = (T - T_prev) * s.inv_dt;
+    f.T_prev[c] = T;
+    if (TAB) store_props(f, c, pr);
+    return make_float2(T, fl);
+}
+
 /* thermal chain of one cell (reference kernels.py:497-510, 1953-1984, 1611-1682, 2030-2082,
  * 1937-1950, 1519-1556, 3016-3032, 2013-2027 in coupled_step order); returns the cell's new T
  * and f_l.
@@ -850,10 +899,9 @@ __device__ __forceinline__ float2 thermal_cell(const waam_step_fields &f, const 
     float T_jp = masked_at(f.T_in, f.flags_in, i, j + 1, k, T_c, d);
     float T_km = masked_at(f.T_in, f.flags_in, i, j, k - 1, T_c, d);
     float T_kp = masked_at(f.T_in, f.flags_in, i, j, k + 1, T_c, d);
-    float lap = T_ip + T_im + T_jp + T_jm + T_kp + T_km - 6.0f * T_c;
     CellProps pr{p.cp_rho, p.alpha_lu, p.dgamma, 0.0f};
     if (TAB) pr = cell_props(p.tables, p.refs, 1, flag, T_c);
-    float dH = pr.cp_rho * pr.alpha_lu * lap * 1.0f;
+    float dH = conduction_dH(pr, T_c, T_im, T_ip, T_jm, T_jp, T_km, T_kp);
     float H_c = hv.at(c, i, j, k);
     if (ADV && flag != SOLID) {
         float u = f.ux[c], v = f.uy[c], w = f.uz[c];
@@ -874,36 +922,11 @@ __device__ __forceinline__ float2 thermal_cell(const waam_step_fields &f, const 
         float gz = upwind_grad(w, H_kmm, H_km, H_c, H_kp, H_kpp);
         dH -= (u * gx + v * gy + w * gz) * 1.0f;
     }
-    float h = H_c + dH;
-    if (p.enable_heat_loss) {
-        int n_exp = exposed_faces(f.flags_in, i, j, k, d);
-        if (n_exp != 0) h -= loss_flux(T_c, s.lc) * (float)n_exp * s.lc.dt / s.lc.dx;
-        const float h_min = TAB ? pr.cp_rho * p.T_amb : s.h_min;
-        if (h < h_min) h = h_min;
-    }
-    if (p.enable_cap) {
-        if (TAB)
-            h = ceiling_clamp(h, s.pc.H_liq, s.pc.H_liq + pr.cp_rho * (p.T_cap - p.T_l),
-                              pr.cp_rho * p.T_cap);
-        else
-            h = ceiling_clamp(h, s.pc.H_liq, s.cap_l, s.cap_s);
-    }
-    float T, fl;
-    PhaseConsts pc = s.pc;
-    if (TAB) pc.cp_rho = pr.cp_rho + 1e-9f;
-    phase_of(h, pc, T, fl);
-    /* HAZ diagnostics (reference kernels.py:3027-3032, 2023-2027); they run before the VOF
-     * flag update, so they see this step's incoming flags. Every load precedes the first store
-     * (the field pointers carry no alias information). */
-    float T_max = f.T_max[c], T_prev = f.T_prev[c];
-    f.H_out[c] = h;
-    f.T_out[c] = T;
-    if (!(fl_zero_held && fl == 0.0f)) f.fl_out[c] = fl;
-    if (T > T_max) f.T_max[c] = T;
-    f.dT_dt[c] = (T - T_prev) * s.inv_dt;
-    f.T_prev[c] = T;
-    if (TAB) store_props(f, c, pr);
-    return make_float2(T, fl);
+    const int n_exp = p.enable_heat_loss ? exposed_faces(f.flags_in, i, j, k, d) : 0;
+    /* HAZ diagnostics' inputs; every load precedes the first store (the field pointers carry
+     * no alias information) */
+    const float T_max = f.T_max[c], T_prev = f.T_prev[c];
+    return thermal_finish<TAB>(f, s, c, T_c, H_c, dH, n_exp, pr, T_max, T_prev, fl_zero_held);
 }
 
 /* advect_phi + reinitialize_phi of one cell (reference kernels.py:2320-2391) */
@@ -945,6 +968,175 @@ __global__ void __launch_bounds__(256) k_step_thermal(waam_step_fields f, StepCo
     if (s.p.enable_vof) f.phi_tmp[c] = phi_new;
 }
 
+/* slices of the plate that kernel 2p marches (ZYX layout; 0 in the XYZ layout, whose row
+ * segments run along z) */
+__host__ __device__ inline int plate_slices(const waam_step_params &p, const Dims &d)
+{
+    int n = p.nz_solid < d.nz ? p.nz_solid : d.nz;
+    n = n < 64 ? n : 64;
+    return d.layout == kLayoutZYX && n > 0 ? n : 0;
+}
+
+/* 2p — on a steady step in the ZYX layout, the listed segments of the plate's slices
+ * (k < plate_slices) that ended the last two steps cold: every cell SOLID, so the chain of
+ * kernel 2 without flag-dependent work. A block takes 64 cells (i) of kPlateRows adjacent rows
+ * (j) and marches up the slices: each lane keeps its cell's T and flag at k-1, k and k+1 in
+ * registers, the block stages the rows' T and flags of slice k in LDS for the j neighbours
+ * (the rows either side of the block, and the i neighbours beyond the wavefront's 64 cells,
+ * are loaded), the i neighbours come from the adjacent lanes, and every load of slice k+1
+ * (k+2 for T and flags) is issued before slice k's arithmetic. The arithmetic is kernel 2's
+ * (conduction_dH, thermal_finish). Records each segment cold again (12) or visited and no
+ * longer cold (28: kernel 2's walk then lists it for kernels 4 and 7, and kernel 7's record
+ * drops bit 4). */
+constexpr int kPlateRows = 4;
+
+__global__ void __launch_bounds__(kTileFast * kPlateRows) k_step_thermal_plate(waam_step_fields f,
+                                                                              StepConsts s)
+{
+    __shared__ float sT[kPlateRows][kTileFast];
+    __shared__ uint8_t sF[kPlateRows][kTileFast];
+    const Dims &d = s.d;
+    const waam_step_params &p = s.p;
+    const QuietRec q = quiet_rec(f, d);
+    const HView hv = make_hview(f, s);
+    const float T_cold = cold_limit(p);
+    const int KP = plate_slices(p, d);
+    const int lane = (int)threadIdx.x, r = (int)threadIdx.y;
+    const int bx = (int)blockIdx.x;
+    const int j = (int)blockIdx.y * kPlateRows + r;
+    const int i = bx * kTileFast + lane;
+    const bool row_in = j < d.ny;
+    const bool in = row_in && i < d.nx;
+    /* every lane loads at clamped indices, so a lane past the grid's edge holds the edge
+     * cell's values: exactly what the clamped neighbour of the edge cell reads */
+    const int ic = imin(i, d.nx - 1), jc = imin(j, d.ny - 1);
+    const bool first = lane == 0, last = lane == kTileFast - 1;
+    const int i_h = first ? imax(i - 1, 0) : imin(i + 1, d.nx - 1);  // the lane's halo column
+    const bool edge_row = r == 0 || r == kPlateRows - 1;
+    const int j_h = r == 0 ? imax(j - 1, 0) : imin(j + 1, d.ny - 1);  // the wave's halo row
+    const CellProps pr{p.cp_rho, p.alpha_lu, p.dgamma, 0.0f};
+    /* the record bytes of the wave's segments, one slice per lane */
+    const unsigned mybits = (row_in && lane < KP) ? q.bits[seg_index(q, bx, jc, lane, d)] : 0u;
+    auto hist_at = [&](int k) {
+        return ((unsigned)__shfl((int)mybits, k) & 12u) == 12u;
+    };
+    /* slice state: own T and flag at k-1, k, k+1 (and k+2 in flight), the halo column's and
+     * halo row's T and flag at k (and k+1 in flight), the cell's H, f_l, T_max and T_prev at k
+     * (and k+1 in flight) */
+    auto idx = [&](int ii, int jj, int kk) { return d.idx(ii, jj, imax(0, imin(kk, d.nz - 1))); };
+    float Tm, Tc, Tp, Tn;
+    uint8_t Fm, Fc, Fp, Fn;
+    Tc = f.T_in[idx(ic, jc, 0)];
+    Fc = f.flags_in[idx(ic, jc, 0)];
+    Tm = Tc;
+    Fm = Fc;
+    Tp = f.T_in[idx(ic, jc, 1)];
+    Fp = f.flags_in[idx(ic, jc, 1)];
+    float Th = 0.0f, Tr = 0.0f, H = 0.0f, FL = 0.0f, TX = 0.0f, TP = 0.0f;
+    uint8_t Fh = GAS, Fr = GAS;
+    auto load_k = [&](int k, bool hist, float &th, uint8_t &fh, float &tr, uint8_t &fr,
+                      float &h, float &fl, float &tx, float &tp) {
+        if (first || last) {
+            th = f.T_in[idx(i_h, jc, k)];
+            fh = f.flags_in[idx(i_h, jc, k)];
+        }
+        if (edge_row) {
+            tr = f.T_in[idx(ic, j_h, k)];
+            fr = f.flags_in[idx(ic, j_h, k)];
+        }
+        if (hist && in) {
+            const size_t c = d.idx(i, j, k);
+            h = f.H_in[c];
+            fl = hv.heated(i, j) ? f.fl_in[c] : 0.0f;
+            tx = f.T_max[c];
+            tp = f.T_prev[c];
+        }
+    };
+    if (KP > 0) load_k(0, hist_at(0), Th, Fh, Tr, Fr, H, FL, TX, TP);
+    for (int k = 0; k < KP; ++k) {
+        const bool hist = hist_at(k);
+        /* in flight during slice k's arithmetic: T and flags of k+2, the rest of k+1 */
+        Tn = f.T_in[idx(ic, jc, k + 2)];
+        Fn = f.flags_in[idx(ic, jc, k + 2)];
+        float Th1 = 0.0f, Tr1 = 0.0f, H1 = 0.0f, FL1 = 0.0f, TX1 = 0.0f, TP1 = 0.0f;
+        uint8_t Fh1 = GAS, Fr1 = GAS;
+        if (k + 1 < KP) load_k(k + 1, hist_at(k + 1), Th1, Fh1, Tr1, Fr1, H1, FL1, TX1, TP1);
+        sT[r][lane] = Tc;
+        sF[r][lane] = Fc;
+        __syncthreads();
+        if (hist) {
+            /* i neighbours from the adjacent lanes, the wavefront's ends from the halo column
+             * (lane i+1 past the grid holds the edge cell: the clamp) */
+            float Tim = __shfl(Tc, lane - 1), Tip = __shfl(Tc, lane + 1);
+            uint8_t Fim = (uint8_t)__shfl((int)Fc, lane - 1), Fip = (uint8_t)__shfl((int)Fc, lane + 1);
+            if (first) {
+                Tim = Th;
+                Fim = Fh;
+            }
+            if (last) {
+                Tip = Th;
+                Fip = Fh;
+            }
+            /* j neighbours from LDS, the block's edge rows from the halo row (a row past the
+             * grid loaded the edge row: the clamp) */
+            float Tjm, Tjp;
+            uint8_t Fjm, Fjp;
+            if (r > 0) {
+                Tjm = sT[r - 1][lane];
+                Fjm = sF[r - 1][lane];
+            } else {
+                Tjm = Tr;
+                Fjm = Fr;
+            }
+            if (r < kPlateRows - 1) {
+                Tjp = sT[r + 1][lane];
+                Fjp = sF[r + 1][lane];
+            } else {
+                Tjp = Tr;
+                Fjp = Fr;
+            }
+            bool cold = !in;
+            if (in) {
+                /* masked_at: a GAS neighbour gives the cell's own T */
+                const float m_im = Fim == GAS ? Tc : Tim, m_ip = Fip == GAS ? Tc : Tip;
+                const float m_jm = Fjm == GAS ? Tc : Tjm, m_jp = Fjp == GAS ? Tc : Tjp;
+                const float m_km = Fm == GAS ? Tc : Tm, m_kp = Fp == GAS ? Tc : Tp;
+                /* exposed_faces: the in-grid GAS neighbours at k+1, j+-1, i+-1 */
+                const int n_exp = p.enable_heat_loss
+                                      ? (int)(k + 1 < d.nz && Fp == GAS) +
+                                            (int)(j + 1 < d.ny && Fjp == GAS) +
+                                            (int)(j - 1 >= 0 && Fjm == GAS) +
+                                            (int)(i + 1 < d.nx && Fip == GAS) +
+                                            (int)(i - 1 >= 0 && Fim == GAS)
+                                      : 0;
+                const size_t c = d.idx(i, j, k);
+                const float H_c = hv.value(H, FL, 0.0f, i, j, k);
+                const float dH = conduction_dH(pr, Tc, m_im, m_ip, m_jm, m_jp, m_km, m_kp);
+                const float2 tf = thermal_finish<false>(f, s, c, Tc, H_c, dH, n_exp, pr, TX, TP,
+                                                        true);
+                cold = tf.x < T_cold && tf.y == 0.0f;
+            }
+            const bool all_cold = __ballot(!cold) == 0ull;
+            if (first && row_in) q.bits[seg_index(q, bx, j, k, d)] = all_cold ? 12u : 28u;
+        }
+        __syncthreads();
+        Tm = Tc;
+        Fm = Fc;
+        Tc = Tp;
+        Fc = Fp;
+        Tp = Tn;
+        Fp = Fn;
+        Th = Th1;
+        Fh = Fh1;
+        Tr = Tr1;
+        Fr = Fr1;
+        H = H1;
+        FL = FL1;
+        TX = TX1;
+        TP = TP1;
+    }
+}
+
 /* 2 over the active segments of a steady step (the surface cells go to kernel 3); marks which
  * of them kernels 4 and 7 visit: all but the segments that stay cold plate */
 __global__ void __launch_bounds__(256) k_step_thermal_seg(waam_step_fields f, StepConsts s)
@@ -952,6 +1144,8 @@ __global__ void __launch_bounds__(256) k_step_thermal_seg(waam_step_fields f, St
     const Dims &d = s.d;
     const QuietRec q = quiet_rec(f, d);
     const float T_cold = cold_limit(s.p);
+    const int KP = plate_slices(s.p, d);
+    const size_t plate_row = (size_t)q.nbx * (size_t)d.ny;  // segments per slice
     for_listed_segments(q, *q.count, [&](size_t seg, long long pos, unsigned bits) {
         int i, j, k;
         const bool in = seg_cell(q, seg, d, i, j, k);
@@ -961,6 +1155,12 @@ __global__ void __launch_bounds__(256) k_step_thermal_seg(waam_step_fields f, St
          * both f_l buffers hold 0 */
         const bool hist = (bits & 12u) == 12u;
         bool cold = !in;
+        /* a plate slice's cold segment: kernel 2p ran it and recorded whether it is still
+         * cold (12) or not (28) */
+        if (hist && (int)(seg / plate_row) < KP) {
+            if (threadIdx.x == 0) q.need[pos] = (bits & 16u) ? 1 : 0;
+            return;
+        }
         if (hist) {
             if (in) {
                 const float2 tf = thermal_cell<false, false>(f, s, i, j, k, c, SOLID, 0.0f, true);
@@ -2082,8 +2282,13 @@ static void launch_step(const StepConsts &s, const waam_step_fields &f, hipStrea
     };
     if (seg) {
         timed_launch(WAAM_K_THERMAL, [&](hipEvent_t a, hipEvent_t b) {
+            const bool plate = plate_slices(p, d) > 0;
+            if (plate)
+                launch_k(capture, k_step_thermal_plate,
+                         dim3((unsigned)quiet_nbx(d), (unsigned)((d.ny + kPlateRows - 1) / kPlateRows)),
+                         dim3(kTileFast, kPlateRows), st, a, nullptr, f, s);
             launch_k(capture, k_step_thermal_seg, dim3(resident_blocks(k_step_thermal_seg, kBlock)),
-                     block3(), st, a, b, f, s);
+                     block3(), st, plate ? nullptr : a, b, f, s);
         });
         timed_launch(WAAM_K_THERMAL_SURFACE, launch_surface);
     } else if (p.thermal_list_ready == 1) {
