@@ -984,11 +984,13 @@ __host__ __device__ inline int plate_slices(const waam_step_params &p, const Dim
  * registers, the block stages the rows' T and flags of slice k in LDS for the j neighbours
  * (the rows either side of the block, and the i neighbours beyond the wavefront's 64 cells,
  * are loaded), the i neighbours come from the adjacent lanes, and every load of slice k+1
- * (k+2 for T and flags) is issued before slice k's arithmetic. The arithmetic is kernel 2's
+ * (k+2 for T and flags) is issued before slice k's arithmetic; the slices are split into
+ * kPlateChunks ranges, one per block of a column. The arithmetic is kernel 2's
  * (conduction_dH, thermal_finish). Records each segment cold again (12) or visited and no
  * longer cold (28: kernel 2's walk then lists it for kernels 4 and 7, and kernel 7's record
  * drops bit 4). */
 constexpr int kPlateRows = 4;
+constexpr int kPlateChunks = 3;  // slice ranges per block column (1: 77 us, 3: 72 us)
 
 __global__ void __launch_bounds__(kTileFast * kPlateRows) k_step_thermal_plate(waam_step_fields f,
                                                                               StepConsts s)
@@ -1024,14 +1026,17 @@ __global__ void __launch_bounds__(kTileFast * kPlateRows) k_step_thermal_plate(w
      * halo row's T and flag at k (and k+1 in flight), the cell's H, f_l, T_max and T_prev at k
      * (and k+1 in flight) */
     auto idx = [&](int ii, int jj, int kk) { return d.idx(ii, jj, imax(0, imin(kk, d.nz - 1))); };
+    /* the block's slices: chunk blockIdx.z of kPlateChunks */
+    const int kc = (KP + kPlateChunks - 1) / kPlateChunks;
+    const int k0 = (int)blockIdx.z * kc, k1 = imin(KP, k0 + kc);
     float Tm, Tc, Tp, Tn;
     uint8_t Fm, Fc, Fp, Fn;
-    Tc = f.T_in[idx(ic, jc, 0)];
-    Fc = f.flags_in[idx(ic, jc, 0)];
-    Tm = Tc;
-    Fm = Fc;
-    Tp = f.T_in[idx(ic, jc, 1)];
-    Fp = f.flags_in[idx(ic, jc, 1)];
+    Tm = f.T_in[idx(ic, jc, k0 - 1)];
+    Fm = f.flags_in[idx(ic, jc, k0 - 1)];
+    Tc = f.T_in[idx(ic, jc, k0)];
+    Fc = f.flags_in[idx(ic, jc, k0)];
+    Tp = f.T_in[idx(ic, jc, k0 + 1)];
+    Fp = f.flags_in[idx(ic, jc, k0 + 1)];
     float Th = 0.0f, Tr = 0.0f, H = 0.0f, FL = 0.0f, TX = 0.0f, TP = 0.0f;
     uint8_t Fh = GAS, Fr = GAS;
     auto load_k = [&](int k, bool hist, float &th, uint8_t &fh, float &tr, uint8_t &fr,
@@ -1052,15 +1057,15 @@ __global__ void __launch_bounds__(kTileFast * kPlateRows) k_step_thermal_plate(w
             tp = f.T_prev[c];
         }
     };
-    if (KP > 0) load_k(0, hist_at(0), Th, Fh, Tr, Fr, H, FL, TX, TP);
-    for (int k = 0; k < KP; ++k) {
+    if (k0 < k1) load_k(k0, hist_at(k0), Th, Fh, Tr, Fr, H, FL, TX, TP);
+    for (int k = k0; k < k1; ++k) {
         const bool hist = hist_at(k);
         /* in flight during slice k's arithmetic: T and flags of k+2, the rest of k+1 */
         Tn = f.T_in[idx(ic, jc, k + 2)];
         Fn = f.flags_in[idx(ic, jc, k + 2)];
         float Th1 = 0.0f, Tr1 = 0.0f, H1 = 0.0f, FL1 = 0.0f, TX1 = 0.0f, TP1 = 0.0f;
         uint8_t Fh1 = GAS, Fr1 = GAS;
-        if (k + 1 < KP) load_k(k + 1, hist_at(k + 1), Th1, Fh1, Tr1, Fr1, H1, FL1, TX1, TP1);
+        if (k + 1 < k1) load_k(k + 1, hist_at(k + 1), Th1, Fh1, Tr1, Fr1, H1, FL1, TX1, TP1);
         sT[r][lane] = Tc;
         sF[r][lane] = Fc;
         __syncthreads();
@@ -2285,7 +2290,8 @@ static void launch_step(const StepConsts &s, const waam_step_fields &f, hipStrea
             const bool plate = plate_slices(p, d) > 0;
             if (plate)
                 launch_k(capture, k_step_thermal_plate,
-                         dim3((unsigned)quiet_nbx(d), (unsigned)((d.ny + kPlateRows - 1) / kPlateRows)),
+                         dim3((unsigned)quiet_nbx(d), (unsigned)((d.ny + kPlateRows - 1) / kPlateRows),
+                              (unsigned)kPlateChunks),
                          dim3(kTileFast, kPlateRows), st, a, nullptr, f, s);
             launch_k(capture, k_step_thermal_seg, dim3(resident_blocks(k_step_thermal_seg, kBlock)),
                      block3(), st, plate ? nullptr : a, b, f, s);
