@@ -141,17 +141,23 @@ def segment_record(tw) -> tuple[int, int, int]:
     """(row segments, listed, left out as cold plate) of the fused engine's last steady step.
 
     Reads the step's segment record (``waam_step_fields.quiet``): per row segment of 64 cells
-    along the contiguous axis a byte of history, then the listed-segment count on a 256-byte
-    line, the list (int32) and one byte per list position that is 0 where the thermal sweep
-    left the segment out of the flag-update and stream sweeps."""
+    along the contiguous axis a byte of history (12: cold again this step, left out of the
+    flag-update and stream sweeps), then the listed-segment count on a 256-byte line, the list
+    (int32) and one byte per list position that is 0 where the thermal sweep left the segment
+    out of those sweeps. The plate slices' cold segments are not listed at all (their own
+    kernel runs them)."""
     g = tw.grid
     fast, slow = (g.nx, g.nz) if g.layout == 1 else (g.nz, g.nx)
     nseg = -(-fast // 64) * g.ny * slow
     rec = g.quiet.to_numpy().view(np.uint8)
     off = -(-nseg // 256) * 256
     count = int(rec[off:off + 4].view(np.int32)[0])
+    lst = rec[off + 256: off + 256 + 4 * count].view(np.int32)
     need = rec[off + 256 + 4 * nseg: off + 256 + 4 * nseg + count]
-    return nseg, count, int((need == 0).sum())
+    bits = rec[:nseg]
+    unlisted = np.ones(nseg, dtype=bool)
+    unlisted[lst] = False
+    return nseg, count, int((need == 0).sum()) + int(((bits == 12) & unlisted).sum())
 
 
 def cold_skipped_segments(tw) -> tuple[int, int]:

@@ -62,7 +62,7 @@ def test_coupled_step_parity(hip_device, case, engine, layout):
             # steady steps left cold plate segments (rows along x inside the plate) out of the
             # flag-update and stream sweeps; in layout 0 a segment runs along z, through the
             # plate's top, and is never all SOLID on these grids
-            assert 0 < skipped < listed
+            assert skipped > 0
     if case == "bead_box_mrt" and engine == "fused":
         # the arc's zero-heat box (half-width sqrt(220) sigma + 2 cells) is narrower than the
         # grid in x, so the box branch of the arc sums and the box gate ran

@@ -313,6 +313,15 @@ __host__ __device__ inline size_t quiet_bytes(const Dims &d)
     return quiet_pull_offset(d) + (d.n() + 1) / 2;
 }
 
+/* slices of the plate that kernel 2p marches (ZYX layout; 0 in the XYZ layout, whose row
+ * segments run along z) */
+__host__ __device__ inline int plate_slices(const waam_step_params &p, const Dims &d)
+{
+    int n = p.nz_solid < d.nz ? p.nz_solid : d.nz;
+    n = n < 64 ? n : 64;
+    return d.layout == kLayoutZYX && n > 0 ? n : 0;
+}
+
 __device__ __forceinline__ QuietRec quiet_rec(const waam_step_fields &f, const Dims &d)
 {
     QuietRec q;
@@ -376,11 +385,16 @@ __global__ void __launch_bounds__(kBlock) k_step_quiet_list(waam_step_fields f, 
     const QuietRec q = quiet_rec(f, d);
     const size_t nseg = quiet_segments(d);
     const size_t sg = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int KP = plate_slices(s.p, d);
     bool act = false;
     if (sg < nseg) {
         const int bx = (int)(sg % (size_t)q.nbx);
         const size_t r = sg / (size_t)q.nbx;
-        act = segment_active(q, bx, (int)(r % (size_t)d.ny), (int)(r / (size_t)d.ny), d);
+        const int b = (int)(r / (size_t)d.ny);
+        act = segment_active(q, bx, (int)(r % (size_t)d.ny), b, d);
+        /* a plate slice's segment cold at the end of the last two steps goes to kernel 2p,
+         * which lists it only if it is no longer cold */
+        if (b < KP && (q.bits[sg] & 12u) == 12u) act = false;
     }
     __shared__ int s_cnt[kBlock / 64];
     __shared__ int s_base;
@@ -968,14 +982,6 @@ __global__ void __launch_bounds__(256) k_step_thermal(waam_step_fields f, StepCo
     if (s.p.enable_vof) f.phi_tmp[c] = phi_new;
 }
 
-/* slices of the plate that kernel 2p marches (ZYX layout; 0 in the XYZ layout, whose row
- * segments run along z) */
-__host__ __device__ inline int plate_slices(const waam_step_params &p, const Dims &d)
-{
-    int n = p.nz_solid < d.nz ? p.nz_solid : d.nz;
-    n = n < 64 ? n : 64;
-    return d.layout == kLayoutZYX && n > 0 ? n : 0;
-}
 
 /* 2p — on a steady step in the ZYX layout, the listed segments of the plate's slices
  * (k < plate_slices) that ended the last two steps cold: every cell SOLID, so the chain of
@@ -986,9 +992,10 @@ __host__ __device__ inline int plate_slices(const waam_step_params &p, const Dim
  * are loaded), the i neighbours come from the adjacent lanes, and every load of slice k+1
  * (k+2 for T and flags) is issued before slice k's arithmetic; the slices are split into
  * kPlateChunks ranges, one per block of a column. The arithmetic is kernel 2's
- * (conduction_dH, thermal_finish). Records each segment cold again (12) or visited and no
- * longer cold (28: kernel 2's walk then lists it for kernels 4 and 7, and kernel 7's record
- * drops bit 4). */
+ * (conduction_dH, thermal_finish). Kernel 1b leaves these segments out of the list; this
+ * kernel records each cold again (12) or, visited and no longer cold, records 28 and appends it
+ * to the list (kernel 2's walk then marks it for kernels 4 and 7, and kernel 7's record drops
+ * bit 4). */
 constexpr int kPlateRows = 4;
 constexpr int kPlateChunks = 3;  // slice ranges per block column (1: 77 us, 3: 72 us)
 
@@ -1122,7 +1129,11 @@ __global__ void __launch_bounds__(kTileFast * kPlateRows) k_step_thermal_plate(w
                 cold = tf.x < T_cold && tf.y == 0.0f;
             }
             const bool all_cold = __ballot(!cold) == 0ull;
-            if (first && row_in) q.bits[seg_index(q, bx, j, k, d)] = all_cold ? 12u : 28u;
+            if (first && row_in) {
+                const size_t seg = seg_index(q, bx, j, k, d);
+                q.bits[seg] = all_cold ? 12u : 28u;
+                if (!all_cold) q.list[atomicAdd(q.count, 1)] = (int32_t)seg;
+            }
         }
         __syncthreads();
         Tm = Tc;
