@@ -184,6 +184,7 @@ STEP_B_EVERY = 62
 STEP_B_POP = {"fp32": 152, "fp16": 76, "bf16": 76}
 STEP_B_HAZ = 44
 DOMINANT = "k_step_pull"
+TIMING_EVERY = 4
 
 
 def traffic_file(w: dict, pop: str) -> str:
@@ -241,7 +242,10 @@ def timed_run(w: dict, args, pop: str):
     if fused:
         from waam_solver_amd._lib import STEP_KERNELS, lib
 
-        lib().call("waam_timing_begin", STEP_KERNELS.index(DOMINANT), args.steps)
+        # every fourth step's pull carries the timing events (each recorded dispatch costs a
+        # few microseconds of the step)
+        lib().call("waam_timing_begin_every", STEP_KERNELS.index(DOMINANT),
+                   (args.steps + TIMING_EVERY - 1) // TIMING_EVERY, TIMING_EVERY)
     t0 = time.perf_counter()
     for s in range(s0 + args.warmup, s0 + args.warmup + args.steps):
         tw.step(*torch_xy(w, s, g.dt), True)
@@ -533,6 +537,7 @@ def main() -> int:
                     "traffic": meas.get("traffic_bytes_per_launch") if meas else None,
                     "algorithmic_bytes_per_launch": round(per_launch),
                     "avg_launch_us": round(avg_s * 1e6, 2), "timed_launches": n_launch,
+                    "timed_every": TIMING_EVERY,
                     "cells_collided": round(sc[0]), "cells_pulled": round(sc[1]),
                     "step": {"compulsory_bytes_per_cell": round(sb / ncell, 1),
                              "cells_solid": round(classes[0]), "cells_gas": round(classes[1]),

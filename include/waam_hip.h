@@ -709,9 +709,10 @@ int waam_graph_destroy(waam_step_graph *g);
 /* ---- instrumentation (no counterpart in the reference) -------------------------------------
  * HIP-event timing of one kernel of waam_step_fused, recorded on the stream that kernel is
  * launched on: waam_timing_begin arms it for up to max_launches launches of kernel `which`
- * (one of WAAM_K_*); waam_timing_end waits for the recorded events, returns the summed
- * duration and the number of timed launches, and disarms. Process-wide, one kernel at a time;
- * not for concurrent use from several threads. */
+ * (one of WAAM_K_*); waam_timing_begin_every records only every `every`-th launch of it (the
+ * events add a few microseconds around each recorded dispatch); waam_timing_end waits for the
+ * recorded events, returns the summed duration and the number of timed launches, and disarms.
+ * Process-wide, one kernel at a time; not for concurrent use from several threads. */
 enum {
     WAAM_K_BEGIN = 0,
     WAAM_K_THERMAL = 1,
@@ -724,6 +725,7 @@ enum {
     WAAM_K_COUNT = 8
 };
 int waam_timing_begin(int which, int max_launches);
+int waam_timing_begin_every(int which, int max_launches, int every);
 int waam_timing_end(double *total_ms, int64_t *launches);
 
 #ifdef __cplusplus

@@ -95,6 +95,7 @@ SIGNATURES: dict[str, str] = {
     "waam_graph_steps": "p",
     "waam_graph_destroy": "p",
     "waam_timing_begin": "ii",
+    "waam_timing_begin_every": "iii",
     "waam_timing_end": "pp",
     "waam_refresh_thermal_properties": "pppppppiplp",
     "waam_advect_diffuse_temperature_variable": "pppppppppfiiiip",

@@ -2114,6 +2114,8 @@ static void arc_box(double ri, double rj, float arc_i, float arc_j, ArcStep &a)
 struct KernelTimer {
     bool armed = false;
     int which = -1, used = 0;
+    int every = 1;
+    int64_t calls = 0;           // launches of `which` since arming
     std::vector<hipEvent_t> ev;  // start/stop pairs
 };
 static KernelTimer g_timer;
@@ -2124,7 +2126,9 @@ template <class Launch>
 static void timed_launch(int which, Launch &&launch)
 {
     KernelTimer &t = g_timer;
-    bool rec = t.armed && t.which == which && 2 * t.used < (int)t.ev.size();
+    bool rec = false;
+    if (t.armed && t.which == which)
+        rec = (t.calls++ % t.every) == 0 && 2 * t.used < (int)t.ev.size();
     launch(rec ? t.ev[2 * t.used] : nullptr, rec ? t.ev[2 * t.used + 1] : nullptr);
     if (rec) ++t.used;
 }
@@ -3005,7 +3009,14 @@ int waam_step_arc_readback(const void *scratch, float *host_out3)
 
 int waam_timing_begin(int which, int max_launches)
 {
+    return waam_timing_begin_every(which, max_launches, 1);
+}
+
+int waam_timing_begin_every(int which, int max_launches, int every)
+{
     KernelTimer &t = g_timer;
+    if (every <= 0)
+        return set_error(WAAM_ERR_INVALID, "waam_timing_begin_every: every %d out of range", every);
     if (t.armed) return set_error(WAAM_ERR_INVALID, "waam_timing_begin: timing already armed");
     if (which < 0 || which >= WAAM_K_COUNT)
         return set_error(WAAM_ERR_INVALID, "waam_timing_begin: kernel id %d out of range", which);
@@ -3024,6 +3035,8 @@ int waam_timing_begin(int which, int max_launches)
     }
     t.which = which;
     t.used = 0;
+    t.every = every;
+    t.calls = 0;
     t.armed = true;
     return WAAM_OK;
 }
