@@ -269,10 +269,29 @@ def timed_run(w: dict, args, pop: str):
     return tw, el, t_host, classes, sc, timing
 
 
+def form_error(tw, ref) -> dict:
+    """Relative L2 of T, phi, f_l and |u| of twin `tw` against twin `ref` (the same steps from
+    the same state in another storage form), on the device."""
+    import torch
+
+    from waam_solver_amd.field import tensor_of
+
+    out = {}
+    for name in ("T", "phi", "f_l"):
+        a = tensor_of(getattr(tw.grid, name)).double()
+        b = tensor_of(getattr(ref.grid, name)).double()
+        out[name] = float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+    ua = torch.sqrt(sum(tensor_of(getattr(tw.grid, c)).double() ** 2 for c in ("ux", "uy", "uz")))
+    ub = torch.sqrt(sum(tensor_of(getattr(ref.grid, c)).double() ** 2 for c in ("ux", "uy", "uz")))
+    out["u"] = float(torch.linalg.vector_norm(ua - ub) / torch.linalg.vector_norm(ub))
+    return {k: float(f"{v:.3g}") for k, v in out.items()}
+
+
 def compare_form(w: dict, args, pop: str):
     """The same workload, pre-roll, warm-up and timed steps with population storage `pop`:
-    (twin, MLUPS, summary). Speed only: the forms' errors against a binary64 build are measured
-    over 2000 steps with a molten pool by tests/precision_c3.py."""
+    (twin, MLUPS, summary). The forms' errors against a binary64 build are measured over 2000
+    steps with a molten pool by tests/precision_c3.py; main() adds each 16-bit form's error
+    against this fp32 run over the same steps."""
     tw, el, _, classes, _, _ = timed_run(w, args, pop)
     g = tw.grid
     mlups = g.n_cells * args.steps / el / 1e6
@@ -551,18 +570,30 @@ def main() -> int:
         # plain method) first, then the remaining 16-bit form
         forms = {}
         m32 = mlups
+        ref32 = tw if args.pop == "fp32" else None
         for pop in ("fp32", "fp16", "bf16"):
             if pop == args.pop:
                 continue
             other, m, out = compare_form(w, args, pop)
             if pop == "fp32":
                 m32 = m
+                ref32 = other
             else:
                 out["speedup_vs_fp32"] = round(m / m32, 3)
+                out["error_vs_fp32"] = form_error(other, ref32)
             forms[pop] = out
-            del other
+            if other is not ref32:
+                del other
             torch.cuda.empty_cache()
         forms["value_speedup_vs_fp32"] = round(mlups / m32, 3)
+        if ref32 is not tw:
+            # the value's form against the fp32 run: the same pre-roll, warm-up and timed
+            # steps from the same seeded state
+            forms["value_error_vs_fp32"] = form_error(tw, ref32)
+            forms["error_note"] = ("relative L2 against the fp32 run after the same steps; "
+                                   "against a binary64 build see tests/precision_c3.py")
+        del ref32
+        torch.cuda.empty_cache()
     graph = None
     if fused:
         after = args.preroll + args.warmup + args.steps
