@@ -171,7 +171,7 @@ def test_moving_window_parity(hip_device, engine, layout):
     assert tw._window_offset_x_m == ot._window_offset_x_m == 3 * 8 * g.dx
 
 
-@pytest.mark.parametrize("case", ["bench_small", "droplets_srt"])
+@pytest.mark.parametrize("case", ["bench_small", "droplets_srt", "tables_mrt"])
 def test_quiet_segments_skip_with_identical_results(hip_device, case):
     """Two fused twins from the same state, one skipping the sweeps on quiet row segments (all
     GAS with phi 0 around them for two steps) and the flag-update and stream sweeps on cold

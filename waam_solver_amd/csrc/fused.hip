@@ -313,13 +313,13 @@ __host__ __device__ inline size_t quiet_bytes(const Dims &d)
     return quiet_pull_offset(d) + (d.n() + 1) / 2;
 }
 
-/* slices of the plate that kernel 2p marches (ZYX layout; 0 in the XYZ layout, whose row
- * segments run along z) */
+/* slices of the plate that kernel 2p marches (ZYX layout with constant material; 0 in the XYZ
+ * layout, whose row segments run along z, and with table materials) */
 __host__ __device__ inline int plate_slices(const waam_step_params &p, const Dims &d)
 {
     int n = p.nz_solid < d.nz ? p.nz_solid : d.nz;
     n = n < 64 ? n : 64;
-    return d.layout == kLayoutZYX && n > 0 ? n : 0;
+    return d.layout == kLayoutZYX && !p.use_tables && n > 0 ? n : 0;
 }
 
 __device__ __forceinline__ QuietRec quiet_rec(const waam_step_fields &f, const Dims &d)
@@ -1155,6 +1155,7 @@ __global__ void __launch_bounds__(kTileFast * kPlateRows) k_step_thermal_plate(w
 
 /* 2 over the active segments of a steady step (the surface cells go to kernel 3); marks which
  * of them kernels 4 and 7 visit: all but the segments that stay cold plate */
+template <bool TAB>
 __global__ void __launch_bounds__(256) k_step_thermal_seg(waam_step_fields f, StepConsts s)
 {
     const Dims &d = s.d;
@@ -1179,7 +1180,7 @@ __global__ void __launch_bounds__(256) k_step_thermal_seg(waam_step_fields f, St
         }
         if (hist) {
             if (in) {
-                const float2 tf = thermal_cell<false, false>(f, s, i, j, k, c, SOLID, 0.0f, true);
+                const float2 tf = thermal_cell<TAB, false>(f, s, i, j, k, c, SOLID, 0.0f, true);
                 cold = tf.x < T_cold && tf.y == 0.0f;
             }
         } else {
@@ -1193,7 +1194,7 @@ __global__ void __launch_bounds__(256) k_step_thermal_seg(waam_step_fields f, St
                 phi_new = vof_advect_cell(f, d, i, j, k, flag);
             if (in && !is_surface_class(flag)) {
                 /* flag is SOLID or GAS: no enthalpy advection */
-                const float2 tf = thermal_cell<false, false>(f, s, i, j, k, c, flag, h_in);
+                const float2 tf = thermal_cell<TAB, false>(f, s, i, j, k, c, flag, h_in);
                 if (s.p.enable_vof) f.phi_tmp[c] = phi_new;
                 cold = flag == SOLID && tf.x < T_cold && tf.y == 0.0f;
             }
@@ -2181,8 +2182,7 @@ static StepConsts make_step_consts(const waam_step_params &p, const Dims &d)
     s.d = d;
     /* the quiet record is continued only on steady steps (see waam_step_params.quiet_skip) */
     s.p.quiet_skip = (p.quiet_skip && p.pending_stream == 1 && p.thermal_list_ready != 0 &&
-                      !p.arc_applied && !p.forces_from_fields && !p.force_reset_all &&
-                      !p.use_tables)
+                      !p.arc_applied && !p.forces_from_fields && !p.force_reset_all)
                          ? 1
                          : 0;
     s.arc.kind = p.arc_kind;
@@ -2308,8 +2308,14 @@ static void launch_step(const StepConsts &s, const waam_step_fields &f, hipStrea
                          dim3((unsigned)quiet_nbx(d), (unsigned)((d.ny + kPlateRows - 1) / kPlateRows),
                               (unsigned)kPlateChunks),
                          dim3(kTileFast, kPlateRows), st, a, nullptr, f, s);
-            launch_k(capture, k_step_thermal_seg, dim3(resident_blocks(k_step_thermal_seg, kBlock)),
-                     block3(), st, plate ? nullptr : a, b, f, s);
+            if (tab)
+                launch_k(capture, k_step_thermal_seg<true>,
+                         dim3(resident_blocks(k_step_thermal_seg<true>, kBlock)), block3(), st,
+                         plate ? nullptr : a, b, f, s);
+            else
+                launch_k(capture, k_step_thermal_seg<false>,
+                         dim3(resident_blocks(k_step_thermal_seg<false>, kBlock)), block3(), st,
+                         plate ? nullptr : a, b, f, s);
         });
         timed_launch(WAAM_K_THERMAL_SURFACE, launch_surface);
     } else if (p.thermal_list_ready == 1) {
