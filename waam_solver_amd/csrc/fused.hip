@@ -17,6 +17,10 @@
 //   6 k_advect_tracers       porosity tracers
 //   7 k_step_stream          every other cell: the previous step's pull stream, force reset;
 //                            every cell: remelt, trailing-pool and cooled-metal solidification
+// On steady steps (the row-segment record below) 2, 4 and 7 walk a list of the row segments
+// whose results can change (k_step_quiet_list builds it after 1), the pull runs ahead of 7 in
+// two launches of its own (k_step_pull_x2), and in the ZYX layout the plate's cold segments are
+// marched along z by k_step_thermal_plate instead of being listed.
 // Every cell value is produced by the same fp32 expression sequence as the one-kernel-per-
 // reference-kernel path, so both paths give identical bits.
 //
