@@ -10,6 +10,8 @@
   64 direct steps from the same state, every field and both population buffers equal; fields
   finite; the binary64 sum of the enthalpy (J) gains the arc's energy over those steps, and every
   metal cell holds rho = 1 up to the collision's mass defect.
+- The 256x128x64 workload with droplets, 400 steps with and without row-segment skipping (quiet
+  gas, cold plate, the plate march): every field bit-equal.
 """
 
 from __future__ import annotations
@@ -121,3 +123,53 @@ def test_c5_captured_steps_equal_direct_steps_at_full_size(hip_device):
     assert abs((H2 - H1) - e_arc) <= 1e-2 * e_arc
     # every metal cell holds rho = 1 up to the collision's mass defect
     assert abs(r2 - m2) <= 1e-4 * m2 and abs(r1 - m1) <= 1e-4 * m1
+
+
+def test_segment_skipping_is_bit_equal_over_400_steps(hip_device):
+    """The 256x128x64 bead-on-plate workload (seeded molten dome, droplets every 100 steps,
+    binary16 populations) stepped 400 times by two twins, one skipping quiet gas segments and
+    running the plate's cold segments by the plate march (the flag-update and stream sweeps
+    left out), one sweeping every cell: every field and both population buffers bit-equal at
+    the end, with droplets placed and cold segments left out along the way."""
+    import bench
+    from tests.golden.make_c2_telemetry import seed_state
+    from tests.step_cases import FIELDS, segment_record
+    from waam_solver_amd.solvers.fused import FusedStepper
+    from waam_solver_amd.twin import WAAMTwin
+
+    w = bench.WORKLOAD
+    n = 400
+    print(f"[skip] {w['nx']}x{w['ny']}x{w['nz']} fp16, {n} steps skipping vs sweeping", flush=True)
+    kw = dict(bench.twin_kwargs(w), droplet_freq_hz=500.0)
+    twins = []
+    for _ in range(2):
+        tw = WAAMTwin(material="LCS-bead-on-plate", population_storage="fp16", **kw)
+        bench.configure(tw, w)
+        tw.reset()
+        g = tw.grid
+        arrs = {k: getattr(g, k).to_numpy() for k in ("T", "f_l", "phi", "flags", "H")}
+        seed_state(arrs["T"], arrs["f_l"], arrs["phi"], arrs["flags"], arrs["H"], tw.nz_solid,
+                   tw.mat.T_liquidus, tw.cp_rho, tw.L_rho, w)
+        for k, a in arrs.items():
+            getattr(g, k).from_numpy(a)
+        twins.append(tw)
+    skip, ref = twins
+    ref._fused = FusedStepper(ref)
+    ref._fused.quiet_skip = False
+    g = skip.grid
+    cold = 0
+    for s in range(n):
+        xy = bench.torch_xy(w, s, g.dt)
+        skip.step(*xy, True)
+        ref.step(*xy, True)
+        if s % 50 == 49:
+            cold = max(cold, segment_record(skip)[2])
+    skip.sync()
+    ref.sync()
+    diff = [f for f in FIELDS + ("f_src", "f_dst")
+            if getattr(skip.grid, f).to_numpy().tobytes() != getattr(ref.grid, f).to_numpy().tobytes()]
+    print(f"[skip] droplets {skip._n_droplets_fired}, cold segments left out (max) {cold}, "
+          f"differing {diff}", flush=True)
+    assert not diff, f"segment skipping changed {diff}"
+    assert skip._n_droplets_fired == ref._n_droplets_fired >= 2
+    assert cold > 0
